@@ -1,0 +1,166 @@
+// C entry points over the reference lettuce CPU backend and scenario generator, compiled from the
+// reference sources where they lie (see oracle/Makefile). TEST INFRASTRUCTURE ONLY: used to pin
+// the oracle restatement (oracle/lettuce_oracle.cpp), to write golden fixtures, and as the
+// "reference" CPU baseline in bench.py. Never linked into the HIP product library.
+//
+// Interfaces used (reference files): Backend, ForceParams, Diag, PhaseTimes, LetTree
+// (src/backend.hpp:12-148), make_cpu_backend (src/cpu_backend.cpp:911), Config and
+// build_scenario (src/config.hpp:14-70, src/scenarios.cpp:411-420).
+#include <cstdint>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "backend.hpp"
+#include "config.hpp"
+#include "scenarios.hpp"
+
+namespace {
+thread_local std::string g_err;
+
+struct RefCtx {
+    std::unique_ptr<Backend> be;
+};
+
+ForceParams make_params(double G, double eps, double theta, int method) {
+    ForceParams p;
+    p.G = G;
+    p.eps = eps;
+    p.theta = theta;
+    p.method = method == 1 ? ForceMethod::tree : ForceMethod::direct;
+    return p;
+}
+
+void copy_out(const Bodies& b, float* x, float* y, float* z, float* vx, float* vy, float* vz, float* m,
+              uint32_t* id) {
+    const std::size_t n = b.size();
+    std::memcpy(x, b.x.data(), n * 4); std::memcpy(y, b.y.data(), n * 4); std::memcpy(z, b.z.data(), n * 4);
+    std::memcpy(vx, b.vx.data(), n * 4); std::memcpy(vy, b.vy.data(), n * 4); std::memcpy(vz, b.vz.data(), n * 4);
+    std::memcpy(m, b.m.data(), n * 4); std::memcpy(id, b.id.data(), n * 4);
+}
+}  // namespace
+
+extern "C" {
+
+const char* ref_last_error() { return g_err.c_str(); }
+
+// Scenario `name` ("plummer", "disk", ...) with n bodies, the seed, G and "key=value" scenario
+// parameters separated by ';'. Returns the body count (build_scenario may change n), or -1.
+// Call with x == nullptr to get the count only.
+int64_t ref_scenario(const char* name, uint64_t n, uint64_t seed, double G, const char* params, float* x, float* y,
+                     float* z, float* vx, float* vy, float* vz, float* m, uint32_t* id) {
+    try {
+        Config c;
+        c.scenario = name;
+        c.n = n;
+        c.seed = seed;
+        c.G = G;
+        std::string ps = params ? params : "";
+        std::size_t pos = 0;
+        while (pos < ps.size()) {
+            std::size_t end = ps.find(';', pos);
+            if (end == std::string::npos) end = ps.size();
+            std::string kv = ps.substr(pos, end - pos);
+            std::size_t eq = kv.find('=');
+            if (eq != std::string::npos) c.params[kv.substr(0, eq)] = kv.substr(eq + 1);
+            pos = end + 1;
+        }
+        if (!x) return int64_t(scenario_body_count(c));
+        Bodies b = build_scenario(c);
+        copy_out(b, x, y, z, vx, vy, vz, m, id);
+        return int64_t(b.size());
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+void* ref_cpu_create(int threads) { return new RefCtx{make_cpu_backend(threads)}; }
+void ref_cpu_destroy(void* h) { delete static_cast<RefCtx*>(h); }
+
+void ref_upload(void* h, uint64_t n, const float* x, const float* y, const float* z, const float* vx, const float* vy,
+                const float* vz, const float* m, const uint32_t* id) {
+    Bodies b;
+    b.x.assign(x, x + n); b.y.assign(y, y + n); b.z.assign(z, z + n);
+    b.vx.assign(vx, vx + n); b.vy.assign(vy, vy + n); b.vz.assign(vz, vz + n);
+    b.m.assign(m, m + n); b.id.assign(id, id + n);
+    static_cast<RefCtx*>(h)->be->upload(b);
+}
+
+void ref_download(void* h, float* x, float* y, float* z, float* vx, float* vy, float* vz, float* m, uint32_t* id) {
+    Bodies b;
+    static_cast<RefCtx*>(h)->be->download(b);
+    copy_out(b, x, y, z, vx, vy, vz, m, id);
+}
+
+int ref_compute_forces(void* h, double G, double eps, double theta, int method, int potential) {
+    try {
+        static_cast<RefCtx*>(h)->be->compute_forces(make_params(G, eps, theta, method), potential != 0);
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+void ref_kick(void* h, double dt) { static_cast<RefCtx*>(h)->be->kick(dt); }
+void ref_drift(void* h, double dt) { static_cast<RefCtx*>(h)->be->drift(dt); }
+
+// out: mass, ke, pe, has_pe, px, py, pz, lx, ly, lz, cx, cy, cz
+void ref_diagnostics(void* h, double* out) {
+    Diag d = static_cast<RefCtx*>(h)->be->diagnostics();
+    double v[13] = {d.mass, d.ke, d.pe, d.has_pe ? 1.0 : 0.0, d.px, d.py, d.pz, d.lx, d.ly, d.lz, d.cx, d.cy, d.cz};
+    std::memcpy(out, v, sizeof v);
+}
+
+double ref_exact_pe(void* h, double G, double eps) { return static_cast<RefCtx*>(h)->be->exact_potential_energy(G, eps); }
+
+// out: force_ms, sort_ms, build_ms, walk_ms, step_ms; *inter = interactions
+void ref_times(void* h, double* out, uint64_t* inter) {
+    PhaseTimes t = static_cast<RefCtx*>(h)->be->times();
+    out[0] = t.force_ms; out[1] = t.sort_ms; out[2] = t.build_ms; out[3] = t.walk_ms; out[4] = t.step_ms;
+    *inter = t.interactions;
+}
+
+int ref_body_costs(void* h, float* cost) {
+    std::vector<float> c;
+    if (!static_cast<RefCtx*>(h)->be->body_costs(c)) return 0;
+    std::memcpy(cost, c.data(), c.size() * 4);
+    return 1;
+}
+
+// The whole local tree, breadth first: build_tree, then export_let for a target box that holds every
+// body (every node with mass is then opened, src/cpu_backend.cpp:222-271). Two-phase: call with
+// nodes == nullptr to get the node and body counts, then again with buffers (60-byte LetNode records
+// as 15 floats/ints each, and nb leaf-body floats per array). Keeps the LET between the calls.
+struct RefTree { LetTree t; };
+static thread_local RefTree g_tree;
+
+int ref_export_tree(void* h, double G, double eps, double theta, uint64_t* nnodes, uint64_t* nbodies, void* nodes,
+                    float* bx, float* by, float* bz, float* bm) {
+    try {
+        auto& be = *static_cast<RefCtx*>(h)->be;
+        if (!nodes) {
+            ForceParams p = make_params(G, eps, theta, 1);
+            be.build_tree(p);
+            Box3 all = be.local_bounds();
+            for (int c = 0; c < 3; ++c) { all.lo[c] -= 1.0f; all.hi[c] += 1.0f; }
+            g_tree.t = be.export_let(all, p);
+            *nnodes = g_tree.t.nodes.size();
+            *nbodies = g_tree.t.bx.size();
+            return 0;
+        }
+        std::memcpy(nodes, g_tree.t.nodes.data(), g_tree.t.nodes.size() * sizeof(LetNode));
+        const std::size_t nb = g_tree.t.bx.size();
+        std::memcpy(bx, g_tree.t.bx.data(), nb * 4); std::memcpy(by, g_tree.t.by.data(), nb * 4);
+        std::memcpy(bz, g_tree.t.bz.data(), nb * 4); std::memcpy(bm, g_tree.t.bm.data(), nb * 4);
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+}  // extern "C"
