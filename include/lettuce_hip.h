@@ -1,0 +1,142 @@
+/* lettuce HIP backend: C ABI of the MI355X (gfx950) force backend library liblettuce_hip.so.
+ *
+ * Every entry point mirrors one member of the reference's Backend interface (src/backend.hpp) or
+ * its factory functions, with plain pointers and sizes. C++ callers use the HipBackend class
+ * (lettuce_simulate_physics_amd/csrc/hip_backend.hpp), which implements Backend on top of this ABI
+ * and rethrows errors as the reference's exception types.
+ *
+ * Status codes: LHIP_OK on success; LHIP_RUNTIME for device/runtime failures (reference:
+ * std::runtime_error), LHIP_LOGIC for protocol misuse (reference: std::logic_error), etc. The
+ * message of the last failure on a context is lhip_last_error(ctx).
+ *
+ * Threading: one caller thread per context (as the reference backends). Bodies live on the device
+ * between calls; kick and drift are asynchronous; compute_forces(_with), diagnostics, download,
+ * body_costs and times synchronise.
+ */
+#ifndef LETTUCE_HIP_H
+#define LETTUCE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    LHIP_OK = 0,
+    LHIP_RUNTIME = 1,
+    LHIP_LOGIC = 2,
+    LHIP_UNAVAILABLE = 3,
+    LHIP_OUT_OF_MEMORY = 4,
+    LHIP_INVALID = 5,
+    LHIP_NOT_PROVIDED = 6
+};
+
+enum { LHIP_METHOD_DIRECT = 0, LHIP_METHOD_TREE = 1 };
+
+typedef struct lhip_ctx lhip_ctx;
+
+/* ForceParams (src/backend.hpp:14-19). method: LHIP_METHOD_DIRECT or LHIP_METHOD_TREE. */
+typedef struct lhip_force_params {
+    double G;
+    double eps;
+    double theta;
+    int32_t method;
+    int32_t reserved;
+} lhip_force_params;
+
+/* Diag (src/backend.hpp:30-38). */
+typedef struct lhip_diag {
+    double mass, ke, pe;
+    int32_t has_pe, pad;
+    double px, py, pz;
+    double lx, ly, lz;
+    double cx, cy, cz;
+} lhip_diag;
+
+/* PhaseTimes (src/backend.hpp:40-49) plus split counters: p2p (body-body) and p2c (body-node)
+ * interactions, and `listed`, the reference CPU backend's count (list entries per body, the body's
+ * own entry included). interactions = p2p + p2c. */
+typedef struct lhip_phase_times {
+    double force_ms, sort_ms, build_ms, walk_ms, step_ms;
+    uint64_t interactions;
+    double comm_ms, balance_ms;
+    uint64_t p2p, p2c, listed;
+} lhip_phase_times;
+
+/* ---- factory (src/backend.hpp:143-148; make_cuda_backend src/cuda_backend.cu:2026-2048) ---- */
+
+/* Devices visible to this process (cuda_device_count, src/backend.hpp:148); 0 without a driver. */
+int lhip_device_count(void);
+/* 1: this library was built with its device code (cuda_compiled, src/backend.hpp:146). */
+int lhip_compiled(void);
+/* make_cuda_backend(device, why): on failure returns LHIP_UNAVAILABLE (or another code), *out =
+ * NULL and the reason in why[0..why_len). Never throws. */
+int lhip_create(int device, lhip_ctx** out, char* why, size_t why_len);
+void lhip_destroy(lhip_ctx* ctx);
+/* Backend::name (src/backend.hpp:93), e.g. "hip: AMD Instinct MI355X (gfx950, 256 CUs, 288 GB)". */
+int lhip_name(const lhip_ctx* ctx, char* buf, size_t len);
+const char* lhip_last_error(const lhip_ctx* ctx);
+
+/* ---- state (src/backend.hpp:95-104) ---- */
+
+/* Backend::upload: copies n bodies (SoA, reference Bodies src/bodies.hpp:8-13), zeroes a and phi,
+ * synchronises. Refuses n >= 2^31 - 1 (src/cuda_backend.cu:1292-1293). */
+int lhip_upload(lhip_ctx* ctx, uint64_t n, const float* x, const float* y, const float* z, const float* vx,
+                const float* vy, const float* vz, const float* m, const uint32_t* id);
+uint64_t lhip_size(const lhip_ctx* ctx);
+/* Backend::download: fills n = lhip_size() entries per array, in the backend's current order. */
+int lhip_download(lhip_ctx* ctx, float* x, float* y, float* z, float* vx, float* vy, float* vz, float* m,
+                  uint32_t* id);
+/* Backend::compute_forces = build_tree + compute_forces_with(no remote trees). */
+int lhip_compute_forces(lhip_ctx* ctx, const lhip_force_params* p, int potential);
+/* Backend::kick: v += a dt (asynchronous). LHIP_LOGIC if the accelerations are stale (a
+ * build_tree since the last force call, src/cuda_backend.cu:1567-1571). */
+int lhip_kick(lhip_ctx* ctx, double dt);
+/* Backend::drift: x += v dt (asynchronous); invalidates the tree and the potential. */
+int lhip_drift(lhip_ctx* ctx, double dt);
+
+/* ---- reductions (src/backend.hpp:106-110) ---- */
+
+int lhip_diagnostics(lhip_ctx* ctx, lhip_diag* out);
+/* Backend::exact_potential_energy: O(N^2) pair sum in double. */
+int lhip_exact_potential_energy(lhip_ctx* ctx, double G, double eps, double* out);
+int lhip_times(lhip_ctx* ctx, lhip_phase_times* out);
+
+/* ---- distributed hooks used on one rank (src/backend.hpp:114-140) ---- */
+
+/* Backend::local_bounds: float bounding box of the current positions. */
+int lhip_local_bounds(lhip_ctx* ctx, float lo[3], float hi[3]);
+/* Backend::build_tree: sort + tree for the current positions (tree method; no-op for direct). */
+int lhip_build_tree(lhip_ctx* ctx, const lhip_force_params* p);
+/* Backend::compute_forces_with with an empty remote list (the single-rank case). LHIP_LOGIC for
+ * the tree method without a current tree (src/cuda_backend.cu:1399-1400). */
+int lhip_compute_forces_with(lhip_ctx* ctx, const lhip_force_params* p, int potential);
+/* Backend::memory_need (device and host bytes for n bodies). */
+int lhip_memory_need(uint64_t n, int32_t method, uint64_t* device_bytes, uint64_t* host_bytes);
+/* Backend::device_memory: free (including what this context holds) and total device bytes. */
+int lhip_device_memory(lhip_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
+/* Backend::body_costs: interactions per body of the last force call (self excluded), in download
+ * order; LHIP_NOT_PROVIDED when there is none. */
+int lhip_body_costs(lhip_ctx* ctx, float* cost);
+
+/* ---- inspection (no reference counterpart; for parity tests and tools) ---- */
+
+/* Accelerations and potential of the last force call, in the backend's current order. */
+int lhip_accelerations(lhip_ctx* ctx, float* ax, float* ay, float* az, float* pot);
+/* Sorted Morton keys of the current tree (n entries). */
+int lhip_tree_keys(lhip_ctx* ctx, uint64_t* keys);
+/* Node count, walk-group count and the number of bodies in over-full finest cells. */
+int lhip_tree_counts(lhip_ctx* ctx, uint64_t* nodes, uint64_t* groups, uint64_t* overfull);
+/* Pre-order node records: link = (skip, lo, hi, info) per node with info = level | leaf << 5 |
+ * children << 8; prefix = key prefix (3*level bits); mom = 16 floats per node (cx, cy, cz, m, qxx,
+ * qxy, qxz, qyy, qyz, qzz, rcrit2, 0, gx, gy, gz, half). Any pointer may be NULL. */
+int lhip_tree_nodes(lhip_ctx* ctx, int32_t* link, uint64_t* prefix, float* mom);
+/* Walk-group starts (groups + 1 entries; the last is n). */
+int lhip_tree_groups(lhip_ctx* ctx, uint32_t* gstart);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LETTUCE_HIP_H */

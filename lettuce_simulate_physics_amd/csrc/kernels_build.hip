@@ -1,0 +1,369 @@
+// Octree build on sorted Morton keys: leaf and group levels by windows, pre-order node numbering by
+// a scan, node records (range, skip pointer, geometric cell), and bottom-up multipoles by level.
+//
+// Reference: build_node / build_octree (src/cpu_backend.cpp:534-664). The octree is the same one
+// (same leaf rule, same cells, single-child cells kept, same walk groups); only its numbering
+// (depth-first pre-order instead of per-parent child blocks) and construction order differ.
+#include <cfloat>
+
+#include "kernels.hpp"
+#include "tree.hpp"
+
+namespace lhip {
+namespace {
+
+// Number of leading octal digits two 63-bit keys share (21 when equal).
+__device__ __forceinline__ int common_octs(uint64_t a, uint64_t b) {
+    return a == b ? 21 : (__clzll(static_cast<long long>(a ^ b)) - 1) / 3;
+}
+
+// ---------------------------------------------------------------- state reset
+
+__global__ void reset_build(uint32_t* __restrict__ levcnt, WalkCounters* __restrict__ c) {
+    const int t = threadIdx.x;
+    if (t < kLevels) levcnt[t] = 0;
+    if (t == 0) *c = WalkCounters{};
+}
+
+// ---------------------------------------------------------------- leaf and group levels
+
+constexpr int kLvTile = 256;
+constexpr int kHalo = kGroupMax;  // 128
+
+// Per sorted body i: L = leaf level (1 + max common prefix over 17-body windows holding i, capped at
+// 22 = "more than 16 identical keys"), G = group level (the same with 129-body windows, never deeper
+// than the leaf), c = common prefix with body i - 1 (-1 for i = 0).
+// cnt[i] = number of octree cells that start at body i = levels (c, min(L, 21)].
+// gflag[i] = 1 iff the group cell of body i starts at i.
+__global__ void __launch_bounds__(kLvTile) levels_kernel(const uint64_t* __restrict__ key, int n,
+                                                         uint32_t* __restrict__ bl, uint32_t* __restrict__ cnt,
+                                                         uint32_t* __restrict__ gflag, WalkCounters* __restrict__ ctr) {
+    __shared__ uint64_t sk[kHalo + kLvTile + kHalo];
+    __shared__ uint8_t d16[kHalo + kLvTile];
+    __shared__ uint8_t d128[kHalo + kLvTile];
+    const int tid = threadIdx.x;
+    const int base = blockIdx.x * kLvTile;
+    for (int t = tid; t < kHalo + kLvTile + kHalo; t += kLvTile) {
+        const long long j = (long long)base - kHalo + t;
+        sk[t] = (j >= 0 && j < n) ? key[j] : 0ull;
+    }
+    __syncthreads();
+    for (int t = tid; t < kHalo + kLvTile; t += kLvTile) {
+        const long long j = (long long)base - kHalo + t;
+        d16[t] = (j >= 0 && j + kLeafMax < n) ? uint8_t(common_octs(sk[t], sk[t + kLeafMax])) : uint8_t(0xff);
+        d128[t] = (j >= 0 && j + kGroupMax < n) ? uint8_t(common_octs(sk[t], sk[t + kGroupMax])) : uint8_t(0xff);
+    }
+    __syncthreads();
+    const int i = base + tid;
+    bool over = false;
+    if (i < n) {
+        int best = -1;
+        for (int u = 0; u <= kLeafMax; ++u) {
+            const int v = d16[tid + kHalo - kLeafMax + u];
+            if (v != 0xff) best = max(best, v);
+        }
+        const int L = best + 1;
+        best = -1;
+        for (int u = 0; u <= kGroupMax; ++u) {
+            const int v = d128[tid + u];
+            if (v != 0xff) best = max(best, v);
+        }
+        const int Lc = min(L, kMaxLevel);
+        const int Gc = min(best + 1, Lc);
+        const int c = i == 0 ? -1 : common_octs(sk[tid + kHalo - 1], sk[tid + kHalo]);
+        bl[i] = uint32_t(L) | uint32_t(Gc) << 8 | uint32_t(c + 1) << 16;
+        cnt[i] = uint32_t(max(0, Lc - c));
+        gflag[i] = c < Gc ? 1u : 0u;
+        over = L > kMaxLevel;
+    }
+    const unsigned long long ov = __ballot(over);
+    if ((tid & 63) == 0 && ov) atomicAdd(&ctr->overfull, __popcll(ov));
+}
+
+// ---------------------------------------------------------------- exclusive scan (u32, in place)
+// out[0..n) = exclusive prefix sums, out[n] = total. Reduce, scan the block sums, downsweep.
+
+constexpr int kScanThreads = 256;
+constexpr int kScanPT = 8;
+constexpr int kScanTile = kScanThreads * kScanPT;
+
+__device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t& total) {
+    const int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int o = 1; o < kScanThreads; o <<= 1) {
+        const uint32_t a = tid >= o ? sh[tid - o] : 0u;
+        __syncthreads();
+        sh[tid] += a;
+        __syncthreads();
+    }
+    total = sh[kScanThreads - 1];
+    const uint32_t r = sh[tid] - v;
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(kScanThreads) scan_reduce(const uint32_t* __restrict__ a, int n,
+                                                            uint32_t* __restrict__ bsum) {
+    __shared__ uint32_t sh[kScanThreads];
+    const long long b0 = (long long)blockIdx.x * kScanTile + threadIdx.x * kScanPT;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPT; ++k)
+        if (b0 + k < n) s += a[b0 + k];
+    uint32_t total;
+    block_exclusive_scan(s, sh, total);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(kScanThreads) scan_top(uint32_t* __restrict__ bsum, int nb,
+                                                         uint32_t* __restrict__ out, int n) {
+    __shared__ uint32_t sh[kScanThreads];
+    uint32_t carry = 0;
+    for (int b = 0; b < nb; b += kScanThreads) {
+        const int t = b + threadIdx.x;
+        const uint32_t v = t < nb ? bsum[t] : 0u;
+        uint32_t total;
+        const uint32_t e = block_exclusive_scan(v, sh, total);
+        if (t < nb) bsum[t] = carry + e;
+        carry += total;
+    }
+    if (threadIdx.x == 0) out[n] = carry;
+}
+
+__global__ void __launch_bounds__(kScanThreads) scan_down(uint32_t* __restrict__ a, int n,
+                                                          const uint32_t* __restrict__ bsum) {
+    __shared__ uint32_t sh[kScanThreads];
+    const long long b0 = (long long)blockIdx.x * kScanTile + threadIdx.x * kScanPT;
+    uint32_t v[kScanPT];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPT; ++k) {
+        v[k] = b0 + k < n ? a[b0 + k] : 0u;
+        s += v[k];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(s, sh, total) + bsum[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanPT; ++k) {
+        if (b0 + k < n) a[b0 + k] = run;
+        run += v[k];
+    }
+}
+
+void scan_exclusive(hipStream_t s, uint32_t* a, int n, uint32_t* tmp) {
+    const int nb = int((static_cast<long long>(n) + kScanTile - 1) / kScanTile);
+    scan_reduce<<<nb, kScanThreads, 0, s>>>(a, n, tmp);
+    LHIP_LAUNCH_CHECK();
+    scan_top<<<1, kScanThreads, 0, s>>>(tmp, nb, a, n);
+    LHIP_LAUNCH_CHECK();
+    scan_down<<<nb, kScanThreads, 0, s>>>(a, n, tmp);
+    LHIP_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- nodes
+
+// Emits every cell that starts at body i (levels c+1 .. min(L, 21)) at pre-order index
+// scan[i] + (level - c - 1). The geometric centre follows the reference's halving in double
+// (src/cpu_backend.cpp:576-581): child centre = parent centre +- half/2 per axis bit.
+__global__ void __launch_bounds__(kBS) nodes_kernel(const uint64_t* __restrict__ key, int n,
+                                                    const uint32_t* __restrict__ bl, const uint32_t* __restrict__ scan,
+                                                    const Cube* __restrict__ cube, NodeLink* __restrict__ link,
+                                                    NodeMom* __restrict__ mom, uint64_t* __restrict__ prefix,
+                                                    uint32_t* __restrict__ levlist, uint32_t* __restrict__ levcnt) {
+    const int i = blockIdx.x * kBS + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t b = bl[i];
+    const int L = int(b & 0xff), c = int((b >> 16) & 0xff) - 1;
+    const int Lc = min(L, kMaxLevel);
+    if (Lc <= c) return;
+    const uint64_t ki = key[i];
+    const uint32_t first = scan[i];
+    // Cell ends, deepest level first: each shallower cell ends at or after the deeper one.
+    int hi[kLevels];
+    int lo_bound = i + 1;
+    for (int l = Lc; l > c; --l) {
+        int end;
+        if (l == 0) {
+            end = n;
+        } else {
+            const int sh = 63 - 3 * l;
+            const uint64_t p = ki >> sh;
+            // Gallop, then bisect, for the first j >= lo_bound whose key leaves the cell.
+            int a = lo_bound - 1, step = 1, bnd = lo_bound;
+            while (bnd < n && (key[bnd] >> sh) == p) {
+                a = bnd;
+                bnd = min(n, bnd + step);
+                step <<= 1;
+            }
+            // key[a] is in the cell (a >= i), key[bnd] is not or bnd == n.
+            while (bnd - a > 1) {
+                const int mid = a + (bnd - a) / 2;
+                if ((key[mid] >> sh) == p) a = mid; else bnd = mid;
+            }
+            end = bnd;
+        }
+        hi[l] = end;
+        lo_bound = end;
+    }
+    const Cube cb = *cube;
+    double half = 0.5 * cb.side;
+    double gx = cb.ox + half, gy = cb.oy + half, gz = cb.oz + half;
+    for (int l = 0; l <= Lc; ++l) {
+        if (l > c) {
+            const int k = int(first) + (l - c - 1);
+            const int end = hi[l];
+            const int skip = int(scan[end]);
+            NodeLink nl;
+            nl.skip = skip;
+            nl.lo = i;
+            nl.hi = end;
+            nl.info = l | (l == Lc ? kInfoLeaf : 0);
+            link[k] = nl;
+            prefix[k] = l == 0 ? 0ull : (ki >> (63 - 3 * l));
+            NodeMom& md = mom[k];
+            md.gx = float(gx);
+            md.gy = float(gy);
+            md.gz = float(gz);
+            md.half = float(half);
+            const uint32_t slot = atomicAdd(&levcnt[l], 1u);
+            levlist[(long long)l * n + slot] = uint32_t(k);
+        }
+        if (l < Lc) {
+            const int oct = int((ki >> (3 * (kMaxLevel - 1 - l))) & 7);
+            const double q = 0.5 * half;
+            gx = gx + ((oct & 4) ? q : -q);
+            gy = gy + ((oct & 2) ? q : -q);
+            gz = gz + ((oct & 1) ? q : -q);
+            half = q;
+        }
+    }
+}
+
+// Group starts in body order; gstart[count] = n.
+__global__ void __launch_bounds__(kBS) groups_kernel(const uint32_t* __restrict__ gflag_scan, int n,
+                                                     const uint32_t* __restrict__ bl, uint32_t* __restrict__ gstart) {
+    const int i = blockIdx.x * kBS + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t b = bl[i];
+    const int c = int((b >> 16) & 0xff) - 1, G = int((b >> 8) & 0xff);
+    if (c < G) gstart[gflag_scan[i]] = uint32_t(i);
+    if (i == n - 1) gstart[gflag_scan[n]] = uint32_t(n);
+}
+
+// ---------------------------------------------------------------- multipoles, one level per launch
+// Reference: leaf moments and the internal reverse sweep with the parallel-axis theorem for the
+// traceless quadrupole, accumulated in double and stored as float (src/cpu_backend.cpp:615-663);
+// rcrit = 2 half / theta + |com - cell centre| (src/cpu_backend.cpp:596-600).
+
+__device__ __forceinline__ float rcrit2_of(float cx, float cy, float cz, float gx, float gy, float gz, float half,
+                                          float inv_theta) {
+    const float dx = cx - gx, dy = cy - gy, dz = cz - gz;
+    // Rounded as the reference build evaluates it: fma(dz, dz, fma(dx, dx, dy * dy)), then the
+    // product and the square root added in double.
+    const float t = __fmaf_rn(dz, dz, __fmaf_rn(dx, dx, __fmul_rn(dy, dy)));
+    const double r = __dadd_rn(__dmul_rn(double(2.f * half), double(inv_theta)), __dsqrt_rn(double(t)));
+    return float(fmin(__dmul_rn(r, r), double(FLT_MAX)));
+}
+
+__global__ void __launch_bounds__(kBS) moments_level(int level, int n, const uint32_t* __restrict__ levlist,
+                                                     const uint32_t* __restrict__ levcnt, const float4* __restrict__ posm,
+                                                     NodeLink* __restrict__ link, NodeMom* __restrict__ mom,
+                                                     float4* __restrict__ trav, float inv_theta) {
+    const uint32_t count = levcnt[level];
+    const uint32_t* list = levlist + (long long)level * n;
+    for (uint32_t t = blockIdx.x * kBS + threadIdx.x; t < count; t += gridDim.x * kBS) {
+        const int k = int(list[t]);
+        NodeLink nl = link[k];
+        NodeMom md = mom[k];
+        double M = 0, sx = 0, sy = 0, sz = 0;
+        double q[6] = {0, 0, 0, 0, 0, 0};
+        double cx, cy, cz;
+        int nchild = 0;
+        if (info_leaf(nl.info)) {
+            for (int j = nl.lo; j < nl.hi; ++j) {
+                const float4 p = posm[j];
+                const double m = p.w;
+                M += m;
+                sx = fma(m, double(p.x), sx);
+                sy = fma(m, double(p.y), sy);
+                sz = fma(m, double(p.z), sz);
+            }
+            cx = M > 0 ? sx / M : double(md.gx);
+            cy = M > 0 ? sy / M : double(md.gy);
+            cz = M > 0 ? sz / M : double(md.gz);
+            for (int j = nl.lo; j < nl.hi; ++j) {
+                const float4 p = posm[j];
+                const double m = p.w, dx = p.x - cx, dy = p.y - cy, dz = p.z - cz;
+                const double r2 = fma(dz, dz, fma(dx, dx, dy * dy));
+                q[0] = fma(m, fma(3 * dx, dx, -r2), q[0]);
+                q[3] = fma(m, fma(3 * dy, dy, -r2), q[3]);
+                q[5] += m * fma(3 * dz, dz, -r2);
+                const double m3 = m * 3;
+                q[1] = fma(m3 * dx, dy, q[1]);
+                q[2] = fma(m3 * dx, dz, q[2]);
+                q[4] += m3 * dy * dz;
+            }
+        } else {
+            for (int ch = k + 1; ch < nl.skip; ch = link[ch].skip) {
+                const NodeMom& c = mom[ch];
+                const double m = c.m;
+                M += m;
+                sx = fma(m, double(c.cx), sx);
+                sy = fma(m, double(c.cy), sy);
+                sz = fma(m, double(c.cz), sz);
+                ++nchild;
+            }
+            cx = M > 0 ? sx / M : double(md.gx);
+            cy = M > 0 ? sy / M : double(md.gy);
+            cz = M > 0 ? sz / M : double(md.gz);
+            for (int ch = k + 1; ch < nl.skip; ch = link[ch].skip) {
+                const NodeMom& c = mom[ch];
+                const double m = c.m, dx = c.cx - cx, dy = c.cy - cy, dz = c.cz - cz;
+                const double r2 = fma(dz, dz, fma(dx, dx, dy * dy));
+                q[0] += c.qxx + m * (3 * dx * dx - r2);
+                q[3] += c.qyy + m * (3 * dy * dy - r2);
+                q[5] += c.qzz + m * (3 * dz * dz - r2);
+                q[1] += c.qxy + m * 3 * dx * dy;
+                q[2] += c.qxz + m * 3 * dx * dz;
+                q[4] += c.qyz + m * 3 * dy * dz;
+            }
+        }
+        md.cx = float(cx); md.cy = float(cy); md.cz = float(cz); md.m = float(M);
+        md.qxx = float(q[0]); md.qxy = float(q[1]); md.qxz = float(q[2]);
+        md.qyy = float(q[3]); md.qyz = float(q[4]); md.qzz = float(q[5]);
+        md.rcrit2 = rcrit2_of(md.cx, md.cy, md.cz, md.gx, md.gy, md.gz, md.half, inv_theta);
+        md.pad0 = 0.f;
+        mom[k] = md;
+        trav[k] = make_float4(md.cx, md.cy, md.cz, md.rcrit2);
+        if (nchild) link[k].info = nl.info | (nchild << 8);
+    }
+}
+
+}  // namespace
+
+std::size_t scan_tmp_entries(int n) {
+    return std::size_t((static_cast<long long>(n) + kScanTile) / kScanTile) + 1;
+}
+
+void tree_build(hipStream_t s, const BuildBufs& b, float inv_theta) {
+    const int n = b.n;
+    reset_build<<<1, 64, 0, s>>>(b.levcnt, b.counters);
+    LHIP_LAUNCH_CHECK();
+    levels_kernel<<<grid_for(n, kLvTile), kLvTile, 0, s>>>(b.key, n, b.bl, b.cnt, b.gflag, b.counters);
+    LHIP_LAUNCH_CHECK();
+    scan_exclusive(s, b.cnt, n, b.scan_tmp);
+    scan_exclusive(s, b.gflag, n, b.scan_tmp);
+    nodes_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(b.key, n, b.bl, b.cnt, b.cube, b.link, b.mom, b.prefix, b.levlist,
+                                                  b.levcnt);
+    LHIP_LAUNCH_CHECK();
+    groups_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(b.gflag, n, b.bl, b.gstart);
+    LHIP_LAUNCH_CHECK();
+    const unsigned g = std::min<unsigned>(grid_for(n, kBS), 1024u);
+    for (int l = kMaxLevel; l >= 0; --l) {
+        moments_level<<<g, kBS, 0, s>>>(l, n, b.levlist, b.levcnt, b.posm, b.link, b.mom, b.trav, inv_theta);
+        LHIP_LAUNCH_CHECK();
+    }
+}
+
+}  // namespace lhip

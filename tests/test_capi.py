@@ -1,0 +1,73 @@
+"""CPU tests of the product's boundary: the C-ABI library loads, exports every entry point the
+header declares, and behaves as the reference factory does without a device."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import pytest
+
+import lettuce_simulate_physics_amd as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h")]
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(lhip_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for required in ("lhip_create", "lhip_upload", "lhip_download", "lhip_compute_forces", "lhip_kick",
+                     "lhip_drift", "lhip_diagnostics", "lhip_exact_potential_energy", "lhip_times",
+                     "lhip_local_bounds", "lhip_build_tree", "lhip_compute_forces_with", "lhip_memory_need",
+                     "lhip_device_memory", "lhip_body_costs"):
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.library()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, f"not exported: {missing}"
+
+
+def test_library_is_built_for_gfx950():
+    data = open(L.LIB_PATH, "rb").read()
+    assert b"gfx950" in data
+
+
+def test_memory_need_is_host_only_and_monotone():
+    d1, h1 = L.HipBackend.memory_need(1 << 20, "tree")
+    d2, _ = L.HipBackend.memory_need(1 << 22, "tree")
+    dd, _ = L.HipBackend.memory_need(1 << 20, "direct")
+    assert d2 > d1 > 0 and dd > 0 and h1 == 0
+
+
+def test_factory_without_device_returns_reason(monkeypatch):
+    if L.hip_device_count() > 0:
+        pytest.skip("a device is present")
+    be, why = L.make_hip_backend(0)
+    assert be is None and why
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(L, "_lib", None)
+    monkeypatch.setattr(L, "LIB_PATH", str(tmp_path / "absent.so"))
+    with pytest.raises(RuntimeError, match="missing"):
+        L.library()
+
+
+def test_force_params_validation():
+    with pytest.raises(ValueError):
+        L.ForceParams(method="auto").c()
+    c = L.ForceParams(G=2.0, eps=0.1, theta=0.5, method="tree").c()
+    assert c.method == L.METHOD_TREE and c.G == 2.0
+    assert ctypes.sizeof(c) == 32
