@@ -48,7 +48,8 @@ struct Context {
         LHIP_CHECK(hipGetDeviceProperties(&prop, dev));
         cus = prop.multiProcessorCount;
         char buf[512];
-        std::snprintf(buf, sizeof buf, "hip: %s (%s, %d CUs, %.0f GB)", prop.name, prop.gcnArchName, cus,
+        std::snprintf(buf, sizeof buf, "hip: %s (%s, %d CUs, %.0f GB)", prop.name[0] ? prop.name : "AMD GPU",
+                      prop.gcnArchName, cus,
                       double(prop.totalGlobalMem) / 1073741824.0);
         name = buf;
         // Default priority on purpose: streams of different priorities are not mixed here.

@@ -1,0 +1,150 @@
+// HipBackend: the reference Backend interface (src/backend.hpp:89-141) implemented over the C ABI
+// of liblettuce_hip.so (include/lettuce_hip.h). C++ side only: no HIP headers, no device code.
+// Errors keep the reference's exception types: device/runtime failures std::runtime_error,
+// protocol misuse std::logic_error (src/cuda_backend.cu:45-51, 1399-1400, 1570-1571); the factory
+// never throws and reports `why` (src/cuda_backend.cu:2026-2048).
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "backend.hpp"
+#include "lettuce_hip.h"
+
+namespace {
+
+lhip_force_params to_c(const ForceParams& p) {
+    lhip_force_params c{};
+    c.G = p.G;
+    c.eps = p.eps;
+    c.theta = p.theta;
+    c.method = p.method == ForceMethod::tree ? LHIP_METHOD_TREE : LHIP_METHOD_DIRECT;
+    return c;
+}
+
+class HipBackend final : public Backend {
+public:
+    explicit HipBackend(lhip_ctx* c) : c_(c) {}
+    ~HipBackend() override { lhip_destroy(c_); }
+    HipBackend(const HipBackend&) = delete;
+    HipBackend& operator=(const HipBackend&) = delete;
+
+    std::string name() const override {
+        char buf[512];
+        lhip_name(c_, buf, sizeof buf);
+        return buf;
+    }
+
+    void upload(const Bodies& b) override {
+        ck(lhip_upload(c_, b.size(), b.x.data(), b.y.data(), b.z.data(), b.vx.data(), b.vy.data(), b.vz.data(),
+                       b.m.data(), b.id.data()));
+    }
+
+    void download(Bodies& b) override {
+        b.resize(lhip_size(c_));
+        ck(lhip_download(c_, b.x.data(), b.y.data(), b.z.data(), b.vx.data(), b.vy.data(), b.vz.data(), b.m.data(),
+                         b.id.data()));
+    }
+
+    void compute_forces(const ForceParams& p, bool potential) override {
+        const lhip_force_params c = to_c(p);
+        ck(lhip_compute_forces(c_, &c, potential ? 1 : 0));
+    }
+
+    void kick(double dt) override { ck(lhip_kick(c_, dt)); }
+    void drift(double dt) override { ck(lhip_drift(c_, dt)); }
+
+    Diag diagnostics() override {
+        lhip_diag d{};
+        ck(lhip_diagnostics(c_, &d));
+        Diag o;
+        o.mass = d.mass; o.ke = d.ke; o.pe = d.pe; o.has_pe = d.has_pe != 0;
+        o.px = d.px; o.py = d.py; o.pz = d.pz;
+        o.lx = d.lx; o.ly = d.ly; o.lz = d.lz;
+        o.cx = d.cx; o.cy = d.cy; o.cz = d.cz;
+        return o;
+    }
+
+    double exact_potential_energy(double G, double eps) override {
+        double v = 0;
+        ck(lhip_exact_potential_energy(c_, G, eps, &v));
+        return v;
+    }
+
+    PhaseTimes times() const override {
+        lhip_phase_times t{};
+        ck(lhip_times(c_, &t));
+        PhaseTimes o;
+        o.force_ms = t.force_ms; o.sort_ms = t.sort_ms; o.build_ms = t.build_ms; o.walk_ms = t.walk_ms;
+        o.step_ms = t.step_ms; o.interactions = t.interactions; o.comm_ms = t.comm_ms; o.balance_ms = t.balance_ms;
+        return o;
+    }
+
+    Box3 local_bounds() override {
+        Box3 b;
+        ck(lhip_local_bounds(c_, b.lo, b.hi));
+        return b;
+    }
+
+    void build_tree(const ForceParams& p) override {
+        const lhip_force_params c = to_c(p);
+        ck(lhip_build_tree(c_, &c));
+    }
+
+    // The locally essential tree exchange is a multi-rank feature; one GPU never calls it.
+    LetTree export_let(const Box3&, const ForceParams&) override {
+        throw std::logic_error("hip export_let: remote trees (multi-rank) are not supported by this backend");
+    }
+
+    void compute_forces_with(const ForceParams& p, bool potential, const std::vector<const LetTree*>& remote) override {
+        if (!remote.empty())
+            throw std::logic_error("hip compute_forces_with: remote trees (multi-rank) are not supported");
+        const lhip_force_params c = to_c(p);
+        ck(lhip_compute_forces_with(c_, &c, potential ? 1 : 0));
+    }
+
+    MemoryNeed memory_need(uint64_t n, ForceMethod m) const override {
+        MemoryNeed r;
+        lhip_memory_need(n, m == ForceMethod::tree ? LHIP_METHOD_TREE : LHIP_METHOD_DIRECT, &r.device, &r.host);
+        return r;
+    }
+
+    bool device_memory(uint64_t& free_bytes, uint64_t& total_bytes) override {
+        ck(lhip_device_memory(c_, &free_bytes, &total_bytes));
+        return true;
+    }
+
+    bool body_costs(std::vector<float>& cost) override {
+        std::vector<float> c(lhip_size(c_));
+        if (c.empty()) return false;
+        const int rc = lhip_body_costs(c_, c.data());
+        if (rc == LHIP_NOT_PROVIDED) return false;
+        ck(rc);
+        cost.swap(c);
+        return true;
+    }
+
+private:
+    void ck(int rc) const {
+        if (rc == LHIP_OK) return;
+        const std::string msg = lhip_last_error(c_);
+        if (rc == LHIP_LOGIC) throw std::logic_error(msg);
+        throw std::runtime_error(msg.empty() ? "hip backend error " + std::to_string(rc) : msg);
+    }
+    lhip_ctx* c_;
+};
+
+}  // namespace
+
+std::unique_ptr<Backend> make_hip_backend(int device, std::string& why) {
+    lhip_ctx* c = nullptr;
+    char buf[512] = {0};
+    const int rc = lhip_create(device, &c, buf, sizeof buf);
+    if (rc != LHIP_OK || !c) {
+        why = buf[0] ? buf : "hip backend unavailable";
+        return nullptr;
+    }
+    return std::make_unique<HipBackend>(c);
+}
+
+bool hip_compiled() { return lhip_compiled() != 0; }
+int hip_device_count() { return lhip_device_count(); }
