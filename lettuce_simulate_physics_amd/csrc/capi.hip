@@ -34,7 +34,7 @@ struct Context {
     // small
     DevBuf cube, bbox_part, red_part, red_out, direct_part;
     // tree
-    DevBuf bl, cnt, gflag, scan_tmp, link, mom, trav, prefix, levlist, levcnt, gstart, counters;
+    DevBuf bl, cnt, gflag, scan_tmp, link, mom, trav, prefix, gstart, counters;
     long long cap_nodes = 0;
 
     bool has_pe = false, acc_ok = false, tree_ok = false, costs_ok = false;
@@ -60,7 +60,6 @@ struct Context {
         red_part.reserve(sizeof(double) * 12 * std::max<std::size_t>(kReduceBlocks, 1));
         red_out.reserve(sizeof(double) * 16);
         counters.reserve(sizeof(WalkCounters));
-        levcnt.reserve(sizeof(uint32_t) * kLevels);
         walk_blocks = cus * walk_blocks_per_cu();
     }
     ~Context() {
@@ -95,7 +94,6 @@ struct Context {
         mom.reserve(std::size_t(cap_nodes) * sizeof(NodeMom));
         trav.reserve(std::size_t(cap_nodes) * sizeof(float4));
         prefix.reserve(std::size_t(cap_nodes) * 8);
-        levlist.reserve(N * kLevels * 4);
     }
 
     float ms(int a, int b) const {
@@ -208,8 +206,6 @@ struct Context {
         bb.mom = mom.as<NodeMom>();
         bb.trav = trav.as<float4>();
         bb.prefix = prefix.as<uint64_t>();
-        bb.levlist = levlist.as<uint32_t>();
-        bb.levcnt = levcnt.as<uint32_t>();
         bb.gstart = gstart.as<uint32_t>();
         bb.counters = counters.as<WalkCounters>();
         // inv_theta as the reference CPU forms it (src/cpu_backend.cpp:595).
@@ -367,7 +363,7 @@ struct Context {
         std::size_t held = 0;
         for (const DevBuf* b : {&x, &y, &z, &vx, &vy, &vz, &m, &id, &cost, &x2, &y2, &z2, &vx2, &vy2, &vz2, &m2, &id2,
                                 &cost2, &ax, &ay, &az, &pot, &posm, &key, &key2, &idx, &idx2, &tile_hist, &direct_part,
-                                &bl, &cnt, &gflag, &scan_tmp, &link, &mom, &trav, &prefix, &levlist, &gstart})
+                                &bl, &cnt, &gflag, &scan_tmp, &link, &mom, &trav, &prefix, &gstart})
             held += b->capacity();
         *fr = f + held;
         *tot = tt;
@@ -375,12 +371,12 @@ struct Context {
 };
 
 // Device bytes per body: 22 four-byte state/alternate arrays + posm; the tree adds keys and indices
-// (double-buffered), per-body build arrays and the level lists, plus node records for the bound
+// (double-buffered) and per-body build arrays, plus node records for the bound
 // node_capacity(n) (NodeLink 16 + NodeMom 64 + trav 16 + prefix 8 = 104 B per node).
 void memory_need(uint64_t n, int method, uint64_t* dev, uint64_t* host) {
     uint64_t d = n * (22 * 4 + 16) + (uint64_t(64) << 20);
     if (method == LHIP_METHOD_TREE) {
-        d += n * (16 + 8 + 4 + 4 * 4 + kLevels * 4) + radix_tile_hist_entries(int(std::min<uint64_t>(n, INT32_MAX))) * 4;
+        d += n * (16 + 8 + 4 + 4 * 4) + radix_tile_hist_entries(int(std::min<uint64_t>(n, INT32_MAX))) * 4;
         d += uint64_t(node_capacity((long long)n)) * 104;
     } else {
         d += n * 16 * 64;  // worst-case split partial sums

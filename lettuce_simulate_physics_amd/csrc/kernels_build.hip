@@ -19,10 +19,8 @@ __device__ __forceinline__ int common_octs(uint64_t a, uint64_t b) {
 
 // ---------------------------------------------------------------- state reset
 
-__global__ void reset_build(uint32_t* __restrict__ levcnt, WalkCounters* __restrict__ c) {
-    const int t = threadIdx.x;
-    if (t < kLevels) levcnt[t] = 0;
-    if (t == 0) *c = WalkCounters{};
+__global__ void reset_build(WalkCounters* __restrict__ c) {
+    if (threadIdx.x == 0) *c = WalkCounters{};
 }
 
 // ---------------------------------------------------------------- leaf and group levels
@@ -164,13 +162,14 @@ void scan_exclusive(hipStream_t s, uint32_t* a, int n, uint32_t* tmp) {
 // ---------------------------------------------------------------- nodes
 
 // Emits every cell that starts at body i (levels c+1 .. min(L, 21)) at pre-order index
-// scan[i] + (level - c - 1). The geometric centre follows the reference's halving in double
+// scan[i] + (level - c - 1): body range start, level, leaf flag, key prefix and geometric cell. The
+// cell's end and skip pointer are filled by the bottom-up pass (from its bodies for a leaf, from
+// its last child otherwise). The geometric centre follows the reference's halving in double
 // (src/cpu_backend.cpp:576-581): child centre = parent centre +- half/2 per axis bit.
 __global__ void __launch_bounds__(kBS) nodes_kernel(const uint64_t* __restrict__ key, int n,
                                                     const uint32_t* __restrict__ bl, const uint32_t* __restrict__ scan,
                                                     const Cube* __restrict__ cube, NodeLink* __restrict__ link,
-                                                    NodeMom* __restrict__ mom, uint64_t* __restrict__ prefix,
-                                                    uint32_t* __restrict__ levlist, uint32_t* __restrict__ levcnt) {
+                                                    NodeMom* __restrict__ mom, uint64_t* __restrict__ prefix) {
     const int i = blockIdx.x * kBS + threadIdx.x;
     if (i >= n) return;
     const uint32_t b = bl[i];
@@ -179,45 +178,16 @@ __global__ void __launch_bounds__(kBS) nodes_kernel(const uint64_t* __restrict__
     if (Lc <= c) return;
     const uint64_t ki = key[i];
     const uint32_t first = scan[i];
-    // Cell ends, deepest level first: each shallower cell ends at or after the deeper one.
-    int hi[kLevels];
-    int lo_bound = i + 1;
-    for (int l = Lc; l > c; --l) {
-        int end;
-        if (l == 0) {
-            end = n;
-        } else {
-            const int sh = 63 - 3 * l;
-            const uint64_t p = ki >> sh;
-            // Gallop, then bisect, for the first j >= lo_bound whose key leaves the cell.
-            int a = lo_bound - 1, step = 1, bnd = lo_bound;
-            while (bnd < n && (key[bnd] >> sh) == p) {
-                a = bnd;
-                bnd = min(n, bnd + step);
-                step <<= 1;
-            }
-            // key[a] is in the cell (a >= i), key[bnd] is not or bnd == n.
-            while (bnd - a > 1) {
-                const int mid = a + (bnd - a) / 2;
-                if ((key[mid] >> sh) == p) a = mid; else bnd = mid;
-            }
-            end = bnd;
-        }
-        hi[l] = end;
-        lo_bound = end;
-    }
     const Cube cb = *cube;
     double half = 0.5 * cb.side;
     double gx = cb.ox + half, gy = cb.oy + half, gz = cb.oz + half;
     for (int l = 0; l <= Lc; ++l) {
         if (l > c) {
             const int k = int(first) + (l - c - 1);
-            const int end = hi[l];
-            const int skip = int(scan[end]);
             NodeLink nl;
-            nl.skip = skip;
+            nl.skip = -1;
             nl.lo = i;
-            nl.hi = end;
+            nl.hi = -1;
             nl.info = l | (l == Lc ? kInfoLeaf : 0);
             link[k] = nl;
             prefix[k] = l == 0 ? 0ull : (ki >> (63 - 3 * l));
@@ -226,8 +196,6 @@ __global__ void __launch_bounds__(kBS) nodes_kernel(const uint64_t* __restrict__
             md.gy = float(gy);
             md.gz = float(gz);
             md.half = float(half);
-            const uint32_t slot = atomicAdd(&levcnt[l], 1u);
-            levlist[(long long)l * n + slot] = uint32_t(k);
         }
         if (l < Lc) {
             const int oct = int((ki >> (3 * (kMaxLevel - 1 - l))) & 7);
@@ -266,16 +234,38 @@ __device__ __forceinline__ float rcrit2_of(float cx, float cy, float cz, float g
     return float(fmin(__dmul_rn(r, r), double(FLT_MAX)));
 }
 
-__global__ void __launch_bounds__(kBS) moments_level(int level, int n, const uint32_t* __restrict__ levlist,
-                                                     const uint32_t* __restrict__ levcnt, const float4* __restrict__ posm,
+// Also completes the node's link: a leaf ends at the first body whose boundary level c is below the
+// leaf's level (at most 16 steps except in an over-full finest cell); an internal node ends where
+// its last child ends and skips to where that child skips. Children are k + 1 and then each
+// child's skip while that node is one level deeper.
+// One launch per level, deepest first; each scans the node array for the nodes of its level (no
+// per-level lists: appending to them would serialise on a handful of device-scope counters).
+__global__ void __launch_bounds__(kBS) moments_level(int level, int n, const float4* __restrict__ posm,
+                                                     const uint32_t* __restrict__ bl, const uint32_t* __restrict__ scan,
                                                      NodeLink* __restrict__ link, NodeMom* __restrict__ mom,
                                                      float4* __restrict__ trav, float inv_theta) {
-    const uint32_t count = levcnt[level];
-    const uint32_t* list = levlist + (long long)level * n;
-    for (uint32_t t = blockIdx.x * kBS + threadIdx.x; t < count; t += gridDim.x * kBS) {
-        const int k = int(list[t]);
+    const int M = int(scan[n]);
+    for (int k = blockIdx.x * kBS + threadIdx.x; k < M; k += gridDim.x * kBS) {
         NodeLink nl = link[k];
+        if (info_level(nl.info) != level) continue;
         NodeMom md = mom[k];
+        if (info_leaf(nl.info)) {
+            int hi = nl.lo + 1;
+            while (hi < n && int((bl[hi] >> 16) & 0xff) - 1 >= level) ++hi;
+            nl.hi = hi;
+            nl.skip = int(scan[hi]);
+        } else {
+            int ch = k + 1;
+            while (true) {
+                const int nxt = link[ch].skip;
+                if (nxt >= M || info_level(link[nxt].info) <= level) {
+                    nl.hi = link[ch].hi;
+                    nl.skip = nxt;
+                    break;
+                }
+                ch = nxt;
+            }
+        }
         double M = 0, sx = 0, sy = 0, sz = 0;
         double q[6] = {0, 0, 0, 0, 0, 0};
         double cx, cy, cz;
@@ -336,7 +326,8 @@ __global__ void __launch_bounds__(kBS) moments_level(int level, int n, const uin
         md.pad0 = 0.f;
         mom[k] = md;
         trav[k] = make_float4(md.cx, md.cy, md.cz, md.rcrit2);
-        if (nchild) link[k].info = nl.info | (nchild << 8);
+        nl.info |= nchild << 8;
+        link[k] = nl;
     }
 }
 
@@ -348,20 +339,19 @@ std::size_t scan_tmp_entries(int n) {
 
 void tree_build(hipStream_t s, const BuildBufs& b, float inv_theta) {
     const int n = b.n;
-    reset_build<<<1, 64, 0, s>>>(b.levcnt, b.counters);
+    reset_build<<<1, 64, 0, s>>>(b.counters);
     LHIP_LAUNCH_CHECK();
     levels_kernel<<<grid_for(n, kLvTile), kLvTile, 0, s>>>(b.key, n, b.bl, b.cnt, b.gflag, b.counters);
     LHIP_LAUNCH_CHECK();
     scan_exclusive(s, b.cnt, n, b.scan_tmp);
     scan_exclusive(s, b.gflag, n, b.scan_tmp);
-    nodes_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(b.key, n, b.bl, b.cnt, b.cube, b.link, b.mom, b.prefix, b.levlist,
-                                                  b.levcnt);
+    nodes_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(b.key, n, b.bl, b.cnt, b.cube, b.link, b.mom, b.prefix);
     LHIP_LAUNCH_CHECK();
     groups_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(b.gflag, n, b.bl, b.gstart);
     LHIP_LAUNCH_CHECK();
-    const unsigned g = std::min<unsigned>(grid_for(n, kBS), 1024u);
+    const unsigned g = std::min<unsigned>(grid_for(n, kBS), 512u);
     for (int l = kMaxLevel; l >= 0; --l) {
-        moments_level<<<g, kBS, 0, s>>>(l, n, b.levlist, b.levcnt, b.posm, b.link, b.mom, b.trav, inv_theta);
+        moments_level<<<g, kBS, 0, s>>>(l, n, b.posm, b.bl, b.cnt, b.link, b.mom, b.trav, inv_theta);
         LHIP_LAUNCH_CHECK();
     }
 }

@@ -1,5 +1,5 @@
 // Tree walk: one wavefront per walk group, stackless traversal of the pre-order octree, interaction
-// lists buffered in LDS and evaluated by every lane for its one or two bodies.
+// lists buffered in LDS and evaluated with every lane busy.
 //
 // Reference: walk / build_list / eval_group (src/cpu_backend.cpp:667-689, 816-882). Same
 // acceptance test against the group's bounding box (a node is accepted when it does not contain
@@ -13,9 +13,12 @@
 // s_j = skip_j for such nodes and 0 otherwise, node k is visited iff max_{j<k} s_j <= k, an
 // exclusive prefix-max across the wave. The next window starts at max(window end, max_j s_j).
 //
-// Evaluation. Accepted cells are staged in LDS with their moments (loaded by the accepting lane);
-// opened leaves as body ranges, whose bodies are then read as wave-uniform scalar loads. Lists are
-// evaluated whenever the next append would overflow them and once at the end.
+// Evaluation. Accepted cells are staged in LDS with their moments, bodies of opened leaves with
+// their positions. Walk groups hold 1-128 bodies (39 on average for a Plummer sphere), so one body
+// per lane would idle most lanes. Instead the wave has V = 64 * NS virtual slots (NS = 1 for groups
+// of <= 32 bodies, else 2 per lane); with S = floor(V / nb) list shares, slot v evaluates body
+// v mod nb against list entries e = v div nb (mod S). Partial sums of the S shares are added in
+// LDS at the end. Lists are evaluated whenever the next append would overflow them.
 //
 // Scheduling. A persistent grid; each wave takes groups from an atomic counter until none are left.
 #include <cfloat>
@@ -28,54 +31,52 @@ namespace {
 
 constexpr int kWalkWaves = 4;
 constexpr int kWalkBS = 64 * kWalkWaves;
-constexpr int kCellCap = 128;
-constexpr int kRangeCap = 64;
+constexpr int kCellCap = 128;   // cells staged per wave
+constexpr int kBodyCap = 256;   // bodies staged per wave
+constexpr int kMaxShare = 8;
 
-struct Body {
+struct Slot {
     float x, y, z;
-    int i;
+    int i;       // body index
+    int bi;      // body within the chunk
+    int sp;      // list share
     bool on;
-    float sx, sy, sz, sp;
+    float sx, sy, sz, sp_pot;
     uint32_t self;
 };
 
-template <bool POT, bool GUARD>
-__device__ __forceinline__ void p2p(Body& b, float4 q, float e2) {
-    const float dx = q.x - b.x, dy = q.y - b.y, dz = q.z - b.z;
-    const float r2 = __fmaf_rn(dz, dz, __fmaf_rn(dy, dy, __fmaf_rn(dx, dx, e2)));
-    const float inv = (!GUARD || r2 > 0.f) ? __builtin_amdgcn_rsqf(r2) : 0.f;
-    const float mi = q.w * inv;
-    const float f = mi * (inv * inv);
-    b.sx = __fmaf_rn(f, dx, b.sx);
-    b.sy = __fmaf_rn(f, dy, b.sy);
-    b.sz = __fmaf_rn(f, dz, b.sz);
-    if (POT) b.sp += mi;
-}
+struct WaveLds {
+    float4 cA[kCellCap];  // com, mass
+    float4 cB[kCellCap];  // qxx, qxy, qxz, qyy
+    float2 cC[kCellCap];  // qyz, qzz
+    float4 bP[kBodyCap];  // position, mass
+    int bI[kBodyCap];     // body index
+};
 
 template <bool POT, bool GUARD>
-__device__ __forceinline__ void p2p_self(Body& b, float4 q, int j, float e2) {
+__device__ __forceinline__ void p2p(Slot& b, float4 q, int j, float e2) {
     const float dx = q.x - b.x, dy = q.y - b.y, dz = q.z - b.z;
     const float r2 = __fmaf_rn(dz, dz, __fmaf_rn(dy, dy, __fmaf_rn(dx, dx, e2)));
-    const bool ok = j != b.i && (!GUARD || r2 > 0.f);
+    const bool self = j == b.i;
+    const bool ok = !self && (!GUARD || r2 > 0.f);
     const float inv = ok ? __builtin_amdgcn_rsqf(r2) : 0.f;
     const float mi = q.w * inv;
     const float f = mi * (inv * inv);
     b.sx = __fmaf_rn(f, dx, b.sx);
     b.sy = __fmaf_rn(f, dy, b.sy);
     b.sz = __fmaf_rn(f, dz, b.sz);
-    if (POT) b.sp += mi;
-    b.self += j == b.i ? 1u : 0u;
+    if (POT) b.sp_pot += mi;
+    b.self += self ? 1u : 0u;
 }
 
 // Monopole + quadrupole about the com, d = com - x (reference src/cpu_backend.cpp:855-871):
 //   a += M d/r^3 - Q d/r^5 + 5/2 (d.Q.d) d/r^7,  phi_sum += M/r + 1/2 d.Q.d/r^5.
 template <bool POT, bool GUARD>
-__device__ __forceinline__ void p2c(Body& b, float4 A, float4 B, float4 C, float e2) {
+__device__ __forceinline__ void p2c(Slot& b, float4 A, float4 B, float2 C, float e2) {
     const float dx = A.x - b.x, dy = A.y - b.y, dz = A.z - b.z;
     const float r2 = __fmaf_rn(dz, dz, __fmaf_rn(dy, dy, __fmaf_rn(dx, dx, e2)));
     const float inv = (!GUARD || r2 > 0.f) ? __builtin_amdgcn_rsqf(r2) : 0.f;
     const float inv2 = inv * inv, inv3 = inv * inv2, inv5 = inv3 * inv2;
-    // B = (qxx, qxy, qxz, qyy), C = (qyz, qzz, -, -)
     const float qx = B.x * dx + B.y * dy + B.z * dz;
     const float qy = B.y * dx + B.w * dy + C.x * dz;
     const float qz = B.z * dx + C.x * dy + C.y * dz;
@@ -84,7 +85,31 @@ __device__ __forceinline__ void p2c(Body& b, float4 A, float4 B, float4 C, float
     b.sx += f * dx - qx * inv5;
     b.sy += f * dy - qy * inv5;
     b.sz += f * dz - qz * inv5;
-    if (POT) b.sp += A.w * inv + 0.5f * dqd * inv5;
+    if (POT) b.sp_pot += A.w * inv + 0.5f * dqd * inv5;
+}
+
+template <int NS, bool POT, bool GUARD>
+__device__ __forceinline__ void eval_cells(Slot (&sl)[2], int S, int nc, const WaveLds& L, float e2) {
+    const int steps = (nc + S - 1) / S;
+    for (int t = 0; t < steps; ++t) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int e = t * S + sl[s].sp;
+            if (sl[s].on && e < nc) p2c<POT, GUARD>(sl[s], L.cA[e], L.cB[e], L.cC[e], e2);
+        }
+    }
+}
+
+template <int NS, bool POT, bool GUARD>
+__device__ __forceinline__ void eval_bodies(Slot (&sl)[2], int S, int nb, const WaveLds& L, float e2) {
+    const int steps = (nb + S - 1) / S;
+    for (int t = 0; t < steps; ++t) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int e = t * S + sl[s].sp;
+            if (sl[s].on && e < nb) p2p<POT, GUARD>(sl[s], L.bP[e], L.bI[e], e2);
+        }
+    }
 }
 
 __device__ __forceinline__ int prefix_max_incl(int v, int lane) {
@@ -92,6 +117,15 @@ __device__ __forceinline__ int prefix_max_incl(int v, int lane) {
     for (int d = 1; d < 64; d <<= 1) {
         const int u = __shfl_up(v, d, 64);
         if (lane >= d) v = max(v, u);
+    }
+    return v;
+}
+
+__device__ __forceinline__ int prefix_sum_incl(int v, int lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int u = __shfl_up(v, d, 64);
+        if (lane >= d) v += u;
     }
     return v;
 }
@@ -118,11 +152,9 @@ __device__ __forceinline__ uint32_t lanes_below(unsigned long long mask) {
 
 template <bool POT, bool GUARD>
 __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
-    __shared__ float4 sA[kWalkWaves][kCellCap];
-    __shared__ float4 sB[kWalkWaves][kCellCap];
-    __shared__ float4 sC[kWalkWaves][kCellCap];
-    __shared__ int2 sR[kWalkWaves][kRangeCap];
+    __shared__ WaveLds lds[kWalkWaves];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    WaveLds& L = lds[w];
     const int M = int(a.cnt_scan[a.n]);
     const int ngroups = int(a.gcount[a.n]);
     const float e2 = a.eps2;
@@ -137,75 +169,49 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
         const bool chunked = ghi - glo > kGroupMax;
         for (int clo = glo; clo < ghi; clo += kGroupMax) {
             const int chi = min(ghi, clo + kGroupMax);
-            const bool two = chi - clo > 64;
-            Body b0, b1;
-            b0.i = clo + lane;
-            b0.on = b0.i < chi;
-            b1.i = clo + 64 + lane;
-            b1.on = two && b1.i < chi;
-            {
-                const float4 p = a.posm[b0.on ? b0.i : clo];
-                b0.x = p.x; b0.y = p.y; b0.z = p.z;
-                const float4 q = a.posm[b1.on ? b1.i : clo];
-                b1.x = q.x; b1.y = q.y; b1.z = q.z;
+            const int nbody = chi - clo;                      // 1..128, wave-uniform
+            const int NS = nbody > 32 ? 2 : 1;
+            const int S = min(kMaxShare, (64 * NS) / nbody);  // list shares
+            Slot sl[2];
+            float mnx = FLT_MAX, mny = FLT_MAX, mnz = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX, mxz = -FLT_MAX;
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int v = s * 64 + lane;
+                sl[s].bi = v % nbody;
+                sl[s].sp = v / nbody;
+                sl[s].on = s < NS && sl[s].sp < S;
+                sl[s].i = clo + sl[s].bi;
+                const float4 p = a.posm[sl[s].i];
+                sl[s].x = p.x; sl[s].y = p.y; sl[s].z = p.z;
+                sl[s].sx = sl[s].sy = sl[s].sz = sl[s].sp_pot = 0.f;
+                sl[s].self = 0;
+                mnx = fminf(mnx, p.x); mny = fminf(mny, p.y); mnz = fminf(mnz, p.z);
+                mxx = fmaxf(mxx, p.x); mxy = fmaxf(mxy, p.y); mxz = fmaxf(mxz, p.z);
             }
-            b0.sx = b0.sy = b0.sz = b0.sp = 0.f;
-            b1.sx = b1.sy = b1.sz = b1.sp = 0.f;
-            b0.self = b1.self = 0;
             // Group box as the reference forms it (src/cpu_backend.cpp:606-612): centre and half
-            // extent from the float min/max.
-            const float mnx = wave_min(fminf(b0.x, b1.x)), mxx = wave_max(fmaxf(b0.x, b1.x));
-            const float mny = wave_min(fminf(b0.y, b1.y)), mxy = wave_max(fmaxf(b0.y, b1.y));
-            const float mnz = wave_min(fminf(b0.z, b1.z)), mxz = wave_max(fmaxf(b0.z, b1.z));
+            // extent from the float min/max (every slot holds a body of the chunk).
+            mnx = wave_min(mnx); mny = wave_min(mny); mnz = wave_min(mnz);
+            mxx = wave_max(mxx); mxy = wave_max(mxy); mxz = wave_max(mxz);
             const float bx = 0.5f * (mnx + mxx), by = 0.5f * (mny + mxy), bz = 0.5f * (mnz + mxz);
             const float hx = 0.5f * (mxx - mnx), hy = 0.5f * (mxy - mny), hz = 0.5f * (mxz - mnz);
 
-            int nc = 0, nr = 0;
+            int nc = 0, nb = 0;
             uint32_t ncell_total = 0, nbody_total = 0;
-
             auto flush_cells = [&]() {
-                for (int c = 0; c < nc; ++c) {
-                    const float4 A = sA[w][c], B = sB[w][c], C = sC[w][c];
-                    p2c<POT, GUARD>(b0, A, B, C, e2);
-                    if (two) p2c<POT, GUARD>(b1, A, B, C, e2);
-                }
+                __builtin_amdgcn_wave_barrier();
+                if (NS == 2) eval_cells<2, POT, GUARD>(sl, S, nc, L, e2);
+                else eval_cells<1, POT, GUARD>(sl, S, nc, L, e2);
+                __builtin_amdgcn_wave_barrier();
                 ncell_total += uint32_t(nc);
                 nc = 0;
             };
-            auto flush_ranges = [&]() {
-                for (int r = 0; r < nr; ++r) {
-                    const int2 R = sR[w][r];
-                    const int lo = __builtin_amdgcn_readfirstlane(R.x), hi = __builtin_amdgcn_readfirstlane(R.y);
-                    nbody_total += uint32_t(hi - lo);
-                    if (lo < chi && clo < hi) {
-                        for (int j = lo; j < hi; ++j) {
-                            const float4 q = a.posm[j];
-                            p2p_self<POT, GUARD>(b0, q, j, e2);
-                            if (two) p2p_self<POT, GUARD>(b1, q, j, e2);
-                        }
-                    } else {
-                        int j = lo;
-                        for (; j + 4 <= hi; j += 4) {
-                            const float4 q0 = a.posm[j], q1 = a.posm[j + 1], q2 = a.posm[j + 2], q3 = a.posm[j + 3];
-                            p2p<POT, GUARD>(b0, q0, e2);
-                            p2p<POT, GUARD>(b0, q1, e2);
-                            p2p<POT, GUARD>(b0, q2, e2);
-                            p2p<POT, GUARD>(b0, q3, e2);
-                            if (two) {
-                                p2p<POT, GUARD>(b1, q0, e2);
-                                p2p<POT, GUARD>(b1, q1, e2);
-                                p2p<POT, GUARD>(b1, q2, e2);
-                                p2p<POT, GUARD>(b1, q3, e2);
-                            }
-                        }
-                        for (; j < hi; ++j) {
-                            const float4 q = a.posm[j];
-                            p2p<POT, GUARD>(b0, q, e2);
-                            if (two) p2p<POT, GUARD>(b1, q, e2);
-                        }
-                    }
-                }
-                nr = 0;
+            auto flush_bodies = [&]() {
+                __builtin_amdgcn_wave_barrier();
+                if (NS == 2) eval_bodies<2, POT, GUARD>(sl, S, nb, L, e2);
+                else eval_bodies<1, POT, GUARD>(sl, S, nb, L, e2);
+                __builtin_amdgcn_wave_barrier();
+                nbody_total += uint32_t(nb);
+                nb = 0;
             };
 
             int cur = 0;
@@ -213,27 +219,27 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                 const int k = cur + lane;
                 const bool act = k < M;
                 float4 T = make_float4(0.f, 0.f, 0.f, 0.f);
-                NodeLink L{0, 0, 0, 0};
+                NodeLink nl{0, 0, 0, 0};
                 if (act) {
                     T = a.trav[k];
-                    L = a.link[k];
+                    nl = a.link[k];
                 }
                 // A node holding bodies of this (sub)group is always opened: the reference's
                 // "contains" test for whole groups; any overlap for chunks of an over-full group.
-                const bool keep_open = chunked ? (L.lo < chi && clo < L.hi) : (L.lo <= clo && chi <= L.hi);
+                const bool keep_open = chunked ? (nl.lo < chi && clo < nl.hi) : (nl.lo <= clo && chi <= nl.hi);
                 const float dx = fmaxf(fabsf(T.x - bx) - hx, 0.f);
                 const float dy = fmaxf(fabsf(T.y - by) - hy, 0.f);
                 const float dz = fmaxf(fabsf(T.z - bz) - hz, 0.f);
                 // Rounded as the reference build evaluates it: three products, then two sums.
                 const float d2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
                 const bool acc = act && !keep_open && d2 >= T.w;
-                const bool leaf = act && info_leaf(L.info);
+                const bool leaf = act && info_leaf(nl.info);
                 const bool term = acc || leaf;
-                const int incl = prefix_max_incl(term ? L.skip : 0, lane);
+                const int incl = prefix_max_incl(term ? nl.skip : 0, lane);
                 int excl = __shfl_up(incl, 1, 64);
                 if (lane == 0) excl = 0;
                 const bool visited = act && excl <= k;
-                const bool small = leaf && (L.hi - L.lo) <= kSmallLeaf;
+                const bool small = leaf && (nl.hi - nl.lo) <= kSmallLeaf;
                 const bool is_cell = visited && acc && !small;
                 const bool is_range = visited && leaf && (!acc || small);
 
@@ -244,47 +250,81 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                     if (is_cell) {
                         const int slot = nc + int(lanes_below(mc));
                         const NodeMom& md = a.mom[k];
-                        sA[w][slot] = make_float4(md.cx, md.cy, md.cz, md.m);
-                        sB[w][slot] = make_float4(md.qxx, md.qxy, md.qxz, md.qyy);
-                        sC[w][slot] = make_float4(md.qyz, md.qzz, 0.f, 0.f);
+                        L.cA[slot] = make_float4(md.cx, md.cy, md.cz, md.m);
+                        L.cB[slot] = make_float4(md.qxx, md.qxy, md.qxz, md.qyy);
+                        L.cC[slot] = make_float2(md.qyz, md.qzz);
                     }
                     nc += ncnew;
                 }
+                // Bodies of opened leaves: staged in chunks of at most the free buffer space.
                 const unsigned long long mr = __ballot(is_range);
-                const int nrnew = __popcll(mr);
-                if (nrnew) {
-                    if (nr + nrnew > kRangeCap) flush_ranges();
-                    if (is_range) sR[w][nr + int(lanes_below(mr))] = make_int2(L.lo, L.hi);
-                    nr += nrnew;
+                if (mr) {
+                    const int len = is_range ? nl.hi - nl.lo : 0;
+                    const int end = prefix_sum_incl(len, lane);
+                    const int off = end - len;
+                    const int total = __builtin_amdgcn_readlane(end, 63);
+                    int done = 0;
+                    while (done < total) {
+                        if (nb == kBodyCap) flush_bodies();
+                        const int take = min(kBodyCap - nb, total - done);
+                        const int u0 = max(off, done), u1 = min(off + len, done + take);
+                        for (int u = u0; u < u1; ++u) {
+                            const int j = nl.lo + (u - off);
+                            L.bP[nb + (u - done)] = a.posm[j];
+                            L.bI[nb + (u - done)] = j;
+                        }
+                        nb += take;
+                        done += take;
+                    }
                 }
                 const int wmax = __builtin_amdgcn_readlane(incl, 63);
                 cur = max(cur + 64, wmax);
             }
             flush_cells();
-            flush_ranges();
+            flush_bodies();
 
+            // Add the list shares of each body (slot v = share * nbody + body) through LDS.
+            float4* red = L.bP;                      // kBodyCap >= 128 entries
+            uint32_t* rself = reinterpret_cast<uint32_t*>(L.bI);
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+                if (sl[s].on) {
+                    const int v = sl[s].sp * nbody + sl[s].bi;
+                    red[v] = make_float4(sl[s].sx, sl[s].sy, sl[s].sz, sl[s].sp_pot);
+                    rself[v] = sl[s].self;
+                }
+            __builtin_amdgcn_wave_barrier();
             const float G = a.G;
             const uint32_t listed = ncell_total + nbody_total;
-            if (b0.on) {
-                a.ax[b0.i] = G * b0.sx;
-                a.ay[b0.i] = G * b0.sy;
-                a.az[b0.i] = G * b0.sz;
-                if (POT) a.pot[b0.i] = -G * b0.sp;
-                if (a.cost) a.cost[b0.i] = float(listed - b0.self);
+            uint32_t self_sum = 0;
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int v = s * 64 + lane;
+                if (v >= nbody) continue;
+                float4 acc4 = red[v];
+                uint32_t self = rself[v];
+                for (int sp = 1; sp < S; ++sp) {
+                    const float4 q = red[sp * nbody + v];
+                    acc4.x += q.x; acc4.y += q.y; acc4.z += q.z; acc4.w += q.w;
+                    self += rself[sp * nbody + v];
+                }
+                const int i = clo + v;
+                a.ax[i] = G * acc4.x;
+                a.ay[i] = G * acc4.y;
+                a.az[i] = G * acc4.z;
+                if (POT) a.pot[i] = -G * acc4.w;
+                if (a.cost) a.cost[i] = float(listed - self);
+                self_sum += self;
             }
-            if (b1.on) {
-                a.ax[b1.i] = G * b1.sx;
-                a.ay[b1.i] = G * b1.sy;
-                a.az[b1.i] = G * b1.sz;
-                if (POT) a.pot[b1.i] = -G * b1.sp;
-                if (a.cost) a.cost[b1.i] = float(listed - b1.self);
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0) {
+                t_listed += (unsigned long long)listed * nbody;
+                t_p2c += (unsigned long long)ncell_total * nbody;
             }
-            const uint32_t nb = uint32_t(b0.on) + uint32_t(b1.on);
-            const uint32_t self = (b0.on ? b0.self : 0u) + (b1.on ? b1.self : 0u);
-            t_listed += (unsigned long long)listed * nb;
-            t_p2c += (unsigned long long)ncell_total * nb;
-            t_p2p += (unsigned long long)nbody_total * nb - self;
-            t_inter += (unsigned long long)listed * nb - self;
+            // self_sum is per lane; the totals subtract it once per body.
+            t_p2p += (lane == 0 ? (unsigned long long)nbody_total * nbody : 0ull) - self_sum;
+            t_inter += (lane == 0 ? (unsigned long long)listed * nbody : 0ull) - self_sum;
         }
     }
     t_inter = wave_sum_u64(t_inter);

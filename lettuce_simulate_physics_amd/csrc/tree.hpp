@@ -30,8 +30,6 @@ struct BuildBufs {
     NodeMom* mom;              // cap_nodes
     float4* trav;              // cap_nodes: com xyz + rcrit2 (the walk's test data)
     uint64_t* prefix;          // cap_nodes
-    uint32_t* levlist;         // kLevels * n
-    uint32_t* levcnt;          // kLevels
     uint32_t* gstart;          // n + 1
     WalkCounters* counters;
 };
