@@ -34,7 +34,7 @@ struct Context {
     // small
     DevBuf cube, bbox_part, red_part, red_out, direct_part;
     // tree
-    DevBuf bl, cnt, gflag, scan_tmp, link, mom, trav, prefix, gstart, counters;
+    DevBuf bl, cnt, gflag, scan_tmp, pkey, pkey2, pval, pval2, inv, levstart, node, mom, prefix, gstart, counters;
     long long cap_nodes = 0;
 
     bool has_pe = false, acc_ok = false, tree_ok = false, costs_ok = false;
@@ -83,17 +83,20 @@ struct Context {
         const std::size_t N = std::size_t(nn);
         key.reserve(N * 8); key2.reserve(N * 8);
         idx.reserve(N * 4); idx2.reserve(N * 4);
-        tile_hist.reserve(radix_tile_hist_entries(nn) * 4);
+        cap_nodes = node_capacity(nn);
+        tile_hist.reserve(radix_tile_hist_entries(int(std::max<long long>(nn, cap_nodes))) * 4);
         bl.reserve(N * 4);
         cnt.reserve((N + 1) * 4);
         gflag.reserve((N + 1) * 4);
         gstart.reserve((N + 1) * 4);
         scan_tmp.reserve(scan_tmp_entries(nn) * 4);
-        cap_nodes = node_capacity(nn);
-        link.reserve(std::size_t(cap_nodes) * sizeof(NodeLink));
-        mom.reserve(std::size_t(cap_nodes) * sizeof(NodeMom));
-        trav.reserve(std::size_t(cap_nodes) * sizeof(float4));
-        prefix.reserve(std::size_t(cap_nodes) * 8);
+        const std::size_t C = std::size_t(cap_nodes);
+        pkey.reserve(C * 8); pkey2.reserve(C * 8);
+        pval.reserve(C * 4); pval2.reserve(C * 4); inv.reserve(C * 4);
+        levstart.reserve((kLevels + 2) * 4);
+        node.reserve(C * sizeof(WalkNode));
+        mom.reserve(C * sizeof(NodeMom));
+        prefix.reserve(C * 8);
     }
 
     float ms(int a, int b) const {
@@ -202,9 +205,13 @@ struct Context {
         bb.cnt = cnt.as<uint32_t>();
         bb.gflag = gflag.as<uint32_t>();
         bb.scan_tmp = scan_tmp.as<uint32_t>();
-        bb.link = link.as<NodeLink>();
+        bb.pkey = pkey.as<uint64_t>(); bb.pkey2 = pkey2.as<uint64_t>();
+        bb.pval = pval.as<uint32_t>(); bb.pval2 = pval2.as<uint32_t>();
+        bb.inv = inv.as<uint32_t>();
+        bb.tile_hist = tile_hist.as<uint32_t>();
+        bb.levstart = levstart.as<uint32_t>();
+        bb.node = node.as<WalkNode>();
         bb.mom = mom.as<NodeMom>();
-        bb.trav = trav.as<float4>();
         bb.prefix = prefix.as<uint64_t>();
         bb.gstart = gstart.as<uint32_t>();
         bb.counters = counters.as<WalkCounters>();
@@ -248,10 +255,8 @@ struct Context {
         } else {
             WalkArgs a{};
             a.posm = posm.as<float4>();
-            a.trav = trav.as<float4>();
-            a.link = link.as<NodeLink>();
+            a.node = node.as<WalkNode>();
             a.mom = mom.as<NodeMom>();
-            a.cnt_scan = cnt.as<uint32_t>();
             a.gstart = gstart.as<uint32_t>();
             a.gcount = gflag.as<uint32_t>();
             a.n = n;
@@ -265,7 +270,8 @@ struct Context {
             LHIP_CHECK(hipEventRecord(ev[4], stream));
             LHIP_CHECK(hipMemcpyAsync(&hc, counters.as<void>(), sizeof hc, hipMemcpyDeviceToHost, stream));
             sync();
-            if (hc.error) throw Error(kRuntime, "hip tree walk failed; accelerations are invalid");
+            if (hc.error)
+                throw Error(kRuntime, "hip tree walk: traversal stack overflow; accelerations are invalid");
             direct_costs_pending = false;
         }
         t.sort_ms = tree ? ms(0, 1) : 0.0;
@@ -363,7 +369,7 @@ struct Context {
         std::size_t held = 0;
         for (const DevBuf* b : {&x, &y, &z, &vx, &vy, &vz, &m, &id, &cost, &x2, &y2, &z2, &vx2, &vy2, &vz2, &m2, &id2,
                                 &cost2, &ax, &ay, &az, &pot, &posm, &key, &key2, &idx, &idx2, &tile_hist, &direct_part,
-                                &bl, &cnt, &gflag, &scan_tmp, &link, &mom, &trav, &prefix, &gstart})
+                                &bl, &cnt, &gflag, &scan_tmp, &pkey, &pkey2, &pval, &pval2, &inv, &node, &mom, &prefix, &gstart})
             held += b->capacity();
         *fr = f + held;
         *tot = tt;
@@ -371,13 +377,13 @@ struct Context {
 };
 
 // Device bytes per body: 22 four-byte state/alternate arrays + posm; the tree adds keys and indices
-// (double-buffered) and per-body build arrays, plus node records for the bound
-// node_capacity(n) (NodeLink 16 + NodeMom 64 + trav 16 + prefix 8 = 104 B per node).
+// (double-buffered) and per-body build arrays, plus node records for the bound node_capacity(n)
+// (WalkNode 32 + NodeMom 64 + prefix 8 + level-sort scratch 28 = 132 B per node).
 void memory_need(uint64_t n, int method, uint64_t* dev, uint64_t* host) {
     uint64_t d = n * (22 * 4 + 16) + (uint64_t(64) << 20);
     if (method == LHIP_METHOD_TREE) {
         d += n * (16 + 8 + 4 + 4 * 4) + radix_tile_hist_entries(int(std::min<uint64_t>(n, INT32_MAX))) * 4;
-        d += uint64_t(node_capacity((long long)n)) * 104;
+        d += uint64_t(node_capacity((long long)n)) * 132;
     } else {
         d += n * 16 * 64;  // worst-case split partial sums
     }
@@ -591,7 +597,8 @@ int lhip_tree_nodes(lhip_ctx* ctx, int32_t* link, uint64_t* prefix, float* mom) 
         uint32_t M = 0;
         LHIP_CHECK(hipMemcpyAsync(&M, c.cnt.as<uint32_t>() + c.n, 4, hipMemcpyDeviceToHost, c.stream));
         c.sync();
-        if (link) LHIP_CHECK(hipMemcpyAsync(link, c.link.as<void>(), std::size_t(M) * 16, hipMemcpyDeviceToHost, c.stream));
+        if (link) LHIP_CHECK(hipMemcpy2DAsync(link, 16, c.node.as<char>() + 16, sizeof(lhip::WalkNode), 16, M,
+                                              hipMemcpyDeviceToHost, c.stream));
         if (prefix) LHIP_CHECK(hipMemcpyAsync(prefix, c.prefix.as<void>(), std::size_t(M) * 8, hipMemcpyDeviceToHost, c.stream));
         if (mom) LHIP_CHECK(hipMemcpyAsync(mom, c.mom.as<void>(), std::size_t(M) * 64, hipMemcpyDeviceToHost, c.stream));
         c.sync();

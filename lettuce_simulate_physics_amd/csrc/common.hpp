@@ -90,13 +90,7 @@ struct Cube {
     float lo[3], hi[3];
 };
 
-// Node records of the pre-order octree (a node's subtree is [i, skip); its first child is i + 1).
-struct NodeLink {
-    int32_t skip;   // first node after this subtree
-    int32_t lo;     // body range [lo, hi) in sorted order
-    int32_t hi;
-    int32_t info;   // bits 0-4: level, bit 5: leaf, bits 8-11: children (0 for a leaf)
-};
+// Node info word: level in bits 0-4, leaf flag in bit 5, child count in bits 8-11.
 constexpr int32_t kInfoLeaf = 1 << 5;
 __host__ __device__ inline int info_level(int32_t info) { return info & 31; }
 __host__ __device__ inline bool info_leaf(int32_t info) { return (info & kInfoLeaf) != 0; }

@@ -53,5 +53,11 @@ struct TreeBufs {
 // pointers: after the call b.x etc. hold the sorted arrays). Keys stay in b.key.
 void tree_sort(hipStream_t s, TreeBufs& b, int n, bool permute_cost);
 std::size_t radix_tile_hist_entries(int n);
+// One stable 8-bit LSD pass over bits [shift, shift + 8) of n (key, value) pairs; with n_dev the
+// count is read on the device and n is the capacity. The digit totals follow the tile histograms:
+// tile_hist[ntiles * 256 + d], ntiles = ceil(n / 2048).
+void radix_pass(hipStream_t s, const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, int n,
+                const uint32_t* n_dev, int shift, uint32_t* tile_hist);
+constexpr int kRadixTile = 2048;
 
 }  // namespace lhip

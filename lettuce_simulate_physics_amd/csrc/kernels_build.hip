@@ -161,15 +161,51 @@ void scan_exclusive(hipStream_t s, uint32_t* a, int n, uint32_t* tmp) {
 
 // ---------------------------------------------------------------- nodes
 
-// Emits every cell that starts at body i (levels c+1 .. min(L, 21)) at pre-order index
-// scan[i] + (level - c - 1): body range start, level, leaf flag, key prefix and geometric cell. The
-// cell's end and skip pointer are filled by the bottom-up pass (from its bodies for a leaf, from
-// its last child otherwise). The geometric centre follows the reference's halving in double
-// (src/cpu_backend.cpp:576-581): child centre = parent centre +- half/2 per axis bit.
+// Pre-order enumeration: every cell that starts at body i (levels c+1 .. min(L, 21)) is node
+// k = scan[i] + (level - c - 1). Writes the sort key (its level) and the value k, so a stable sort by
+// level yields the level order (within a level, pre-order = first-body order).
+__global__ void __launch_bounds__(kBS) enumerate_kernel(int n, const uint32_t* __restrict__ bl,
+                                                        const uint32_t* __restrict__ scan, uint64_t* __restrict__ pkey,
+                                                        uint32_t* __restrict__ pval) {
+    const int i = blockIdx.x * kBS + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t b = bl[i];
+    const int L = int(b & 0xff), c = int((b >> 16) & 0xff) - 1;
+    const int Lc = min(L, kMaxLevel);
+    const uint32_t first = scan[i];
+    for (int l = c + 1; l <= Lc; ++l) {
+        const uint32_t k = first + uint32_t(l - c - 1);
+        pkey[k] = uint64_t(l);
+        pval[k] = k;
+    }
+}
+
+// inv[perm[p]] = p; level starts from the level totals (the radix digit counts).
+__global__ void __launch_bounds__(kBS) invert_kernel(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ M_dev,
+                                                     uint32_t* __restrict__ inv, const uint32_t* __restrict__ level_tot,
+                                                     uint32_t* __restrict__ levstart) {
+    const int M = int(*M_dev);
+    const int p = blockIdx.x * kBS + threadIdx.x;
+    if (p < M) inv[perm[p]] = uint32_t(p);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int l = 0; l < kLevels; ++l) {
+            levstart[l] = s;
+            s += level_tot[l];
+        }
+        levstart[kLevels] = s;
+        levstart[kLevels + 1] = s;
+    }
+}
+
+// Level-order node records: body range start, level, leaf flag, first child, key prefix and the
+// geometric cell. The centre follows the reference's halving in double (src/cpu_backend.cpp:576-581):
+// child centre = parent centre +- half/2 per axis bit, from the root centre at origin + side/2.
 __global__ void __launch_bounds__(kBS) nodes_kernel(const uint64_t* __restrict__ key, int n,
                                                     const uint32_t* __restrict__ bl, const uint32_t* __restrict__ scan,
-                                                    const Cube* __restrict__ cube, NodeLink* __restrict__ link,
-                                                    NodeMom* __restrict__ mom, uint64_t* __restrict__ prefix) {
+                                                    const uint32_t* __restrict__ inv, const Cube* __restrict__ cube,
+                                                    WalkNode* __restrict__ node, NodeMom* __restrict__ mom,
+                                                    uint64_t* __restrict__ prefix) {
     const int i = blockIdx.x * kBS + threadIdx.x;
     if (i >= n) return;
     const uint32_t b = bl[i];
@@ -183,15 +219,19 @@ __global__ void __launch_bounds__(kBS) nodes_kernel(const uint64_t* __restrict__
     double gx = cb.ox + half, gy = cb.oy + half, gz = cb.oz + half;
     for (int l = 0; l <= Lc; ++l) {
         if (l > c) {
-            const int k = int(first) + (l - c - 1);
-            NodeLink nl;
-            nl.skip = -1;
-            nl.lo = i;
-            nl.hi = -1;
-            nl.info = l | (l == Lc ? kInfoLeaf : 0);
-            link[k] = nl;
-            prefix[k] = l == 0 ? 0ull : (ki >> (63 - 3 * l));
-            NodeMom& md = mom[k];
+            const uint32_t k = first + uint32_t(l - c - 1);
+            const int p = int(inv[k]);
+            WalkNode nd;
+            nd.cx = nd.cy = nd.cz = 0.f;
+            nd.rcrit2 = 0.f;
+            nd.lo = i;
+            nd.hi = -1;
+            nd.info = l | (l == Lc ? kInfoLeaf : 0);
+            // The first child is the next pre-order node (same first body, one level deeper).
+            nd.first = l == Lc ? -1 : int(inv[k + 1]);
+            node[p] = nd;
+            prefix[p] = l == 0 ? 0ull : (ki >> (63 - 3 * l));
+            NodeMom& md = mom[p];
             md.gx = float(gx);
             md.gy = float(gy);
             md.gz = float(gz);
@@ -234,57 +274,42 @@ __device__ __forceinline__ float rcrit2_of(float cx, float cy, float cz, float g
     return float(fmin(__dmul_rn(r, r), double(FLT_MAX)));
 }
 
-// Also completes the node's link: a leaf ends at the first body whose boundary level c is below the
-// leaf's level (at most 16 steps except in an over-full finest cell); an internal node ends where
-// its last child ends and skips to where that child skips. Children are k + 1 and then each
-// child's skip while that node is one level deeper.
-// One launch per level, deepest first; each scans the node array for the nodes of its level (no
-// per-level lists: appending to them would serialise on a handful of device-scope counters).
+// Nodes of one level are contiguous in level order: [levstart[level], levstart[level + 1]). A leaf
+// ends at the first body whose boundary level c is below the leaf's level (at most 16 steps except in
+// an over-full finest cell); an internal node's children are the contiguous run of next-level nodes
+// from `first` whose key prefix extends the node's, and the node ends where its last child ends.
 __global__ void __launch_bounds__(kBS) moments_level(int level, int n, const float4* __restrict__ posm,
-                                                     const uint32_t* __restrict__ bl, const uint32_t* __restrict__ scan,
-                                                     NodeLink* __restrict__ link, NodeMom* __restrict__ mom,
-                                                     float4* __restrict__ trav, float inv_theta) {
-    const int M = int(scan[n]);
-    for (int k = blockIdx.x * kBS + threadIdx.x; k < M; k += gridDim.x * kBS) {
-        NodeLink nl = link[k];
-        if (info_level(nl.info) != level) continue;
-        NodeMom md = mom[k];
-        if (info_leaf(nl.info)) {
-            int hi = nl.lo + 1;
-            while (hi < n && int((bl[hi] >> 16) & 0xff) - 1 >= level) ++hi;
-            nl.hi = hi;
-            nl.skip = int(scan[hi]);
-        } else {
-            int ch = k + 1;
-            while (true) {
-                const int nxt = link[ch].skip;
-                if (nxt >= M || info_level(link[nxt].info) <= level) {
-                    nl.hi = link[ch].hi;
-                    nl.skip = nxt;
-                    break;
-                }
-                ch = nxt;
-            }
-        }
+                                                     const uint32_t* __restrict__ bl,
+                                                     const uint32_t* __restrict__ levstart,
+                                                     const uint64_t* __restrict__ prefix, WalkNode* __restrict__ node,
+                                                     NodeMom* __restrict__ mom, float inv_theta) {
+    const int p0 = int(levstart[level]), p1 = int(levstart[level + 1]);
+    const int cend = int(levstart[level + 2]);  // end of the next level
+    for (int p = p0 + blockIdx.x * kBS + threadIdx.x; p < p1; p += gridDim.x * kBS) {
+        WalkNode nd = node[p];
+        NodeMom md = mom[p];
         double M = 0, sx = 0, sy = 0, sz = 0;
         double q[6] = {0, 0, 0, 0, 0, 0};
         double cx, cy, cz;
         int nchild = 0;
-        if (info_leaf(nl.info)) {
-            for (int j = nl.lo; j < nl.hi; ++j) {
-                const float4 p = posm[j];
-                const double m = p.w;
+        if (info_leaf(nd.info)) {
+            int hi = nd.lo + 1;
+            while (hi < n && int((bl[hi] >> 16) & 0xff) - 1 >= level) ++hi;
+            nd.hi = hi;
+            for (int j = nd.lo; j < nd.hi; ++j) {
+                const float4 b = posm[j];
+                const double m = b.w;
                 M += m;
-                sx = fma(m, double(p.x), sx);
-                sy = fma(m, double(p.y), sy);
-                sz = fma(m, double(p.z), sz);
+                sx = fma(m, double(b.x), sx);
+                sy = fma(m, double(b.y), sy);
+                sz = fma(m, double(b.z), sz);
             }
             cx = M > 0 ? sx / M : double(md.gx);
             cy = M > 0 ? sy / M : double(md.gy);
             cz = M > 0 ? sz / M : double(md.gz);
-            for (int j = nl.lo; j < nl.hi; ++j) {
-                const float4 p = posm[j];
-                const double m = p.w, dx = p.x - cx, dy = p.y - cy, dz = p.z - cz;
+            for (int j = nd.lo; j < nd.hi; ++j) {
+                const float4 b = posm[j];
+                const double m = b.w, dx = b.x - cx, dy = b.y - cy, dz = b.z - cz;
                 const double r2 = fma(dz, dz, fma(dx, dx, dy * dy));
                 q[0] = fma(m, fma(3 * dx, dx, -r2), q[0]);
                 q[3] = fma(m, fma(3 * dy, dy, -r2), q[3]);
@@ -295,20 +320,24 @@ __global__ void __launch_bounds__(kBS) moments_level(int level, int n, const flo
                 q[4] += m3 * dy * dz;
             }
         } else {
-            for (int ch = k + 1; ch < nl.skip; ch = link[ch].skip) {
+            const uint64_t pre = prefix[p];
+            int ch = nd.first;
+            while (ch < cend && (prefix[ch] >> 3) == pre) {
                 const NodeMom& c = mom[ch];
                 const double m = c.m;
                 M += m;
                 sx = fma(m, double(c.cx), sx);
                 sy = fma(m, double(c.cy), sy);
                 sz = fma(m, double(c.cz), sz);
-                ++nchild;
+                ++ch;
             }
+            nchild = ch - nd.first;
+            nd.hi = node[ch - 1].hi;
             cx = M > 0 ? sx / M : double(md.gx);
             cy = M > 0 ? sy / M : double(md.gy);
             cz = M > 0 ? sz / M : double(md.gz);
-            for (int ch = k + 1; ch < nl.skip; ch = link[ch].skip) {
-                const NodeMom& c = mom[ch];
+            for (int k = nd.first; k < ch; ++k) {
+                const NodeMom& c = mom[k];
                 const double m = c.m, dx = c.cx - cx, dy = c.cy - cy, dz = c.cz - cz;
                 const double r2 = fma(dz, dz, fma(dx, dx, dy * dy));
                 q[0] += c.qxx + m * (3 * dx * dx - r2);
@@ -324,10 +353,10 @@ __global__ void __launch_bounds__(kBS) moments_level(int level, int n, const flo
         md.qyy = float(q[3]); md.qyz = float(q[4]); md.qzz = float(q[5]);
         md.rcrit2 = rcrit2_of(md.cx, md.cy, md.cz, md.gx, md.gy, md.gz, md.half, inv_theta);
         md.pad0 = 0.f;
-        mom[k] = md;
-        trav[k] = make_float4(md.cx, md.cy, md.cz, md.rcrit2);
-        nl.info |= nchild << 8;
-        link[k] = nl;
+        mom[p] = md;
+        nd.cx = md.cx; nd.cy = md.cy; nd.cz = md.cz; nd.rcrit2 = md.rcrit2;
+        nd.info |= nchild << 8;
+        node[p] = nd;
     }
 }
 
@@ -345,13 +374,22 @@ void tree_build(hipStream_t s, const BuildBufs& b, float inv_theta) {
     LHIP_LAUNCH_CHECK();
     scan_exclusive(s, b.cnt, n, b.scan_tmp);
     scan_exclusive(s, b.gflag, n, b.scan_tmp);
-    nodes_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(b.key, n, b.bl, b.cnt, b.cube, b.link, b.mom, b.prefix);
+    const uint32_t* M_dev = b.cnt + n;  // node count
+    enumerate_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(n, b.bl, b.cnt, b.pkey, b.pval);
+    LHIP_LAUNCH_CHECK();
+    // Stable sort of the pre-order nodes by level (one 8-bit pass: levels < 32).
+    radix_pass(s, b.pkey, b.pval, b.pkey2, b.pval2, b.cap_nodes, M_dev, 0, b.tile_hist);
+    const std::size_t ntiles = (static_cast<std::size_t>(b.cap_nodes) + kRadixTile - 1) / kRadixTile;
+    const uint32_t* level_tot = b.tile_hist + ntiles * 256;
+    invert_kernel<<<grid_for(b.cap_nodes, kBS), kBS, 0, s>>>(b.pval2, M_dev, b.inv, level_tot, b.levstart);
+    LHIP_LAUNCH_CHECK();
+    nodes_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(b.key, n, b.bl, b.cnt, b.inv, b.cube, b.node, b.mom, b.prefix);
     LHIP_LAUNCH_CHECK();
     groups_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(b.gflag, n, b.bl, b.gstart);
     LHIP_LAUNCH_CHECK();
     const unsigned g = std::min<unsigned>(grid_for(n, kBS), 512u);
     for (int l = kMaxLevel; l >= 0; --l) {
-        moments_level<<<g, kBS, 0, s>>>(l, n, b.posm, b.bl, b.cnt, b.link, b.mom, b.trav, inv_theta);
+        moments_level<<<g, kBS, 0, s>>>(l, n, b.posm, b.bl, b.levstart, b.prefix, b.node, b.mom, inv_theta);
         LHIP_LAUNCH_CHECK();
     }
 }

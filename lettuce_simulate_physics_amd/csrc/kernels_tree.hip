@@ -73,9 +73,13 @@ constexpr int kSortKPT = 8;                           // keys per thread per til
 constexpr int kSortTile = kSortThreads * kSortKPT;    // 2048 keys per tile
 constexpr int kRadix = 256;
 
-__global__ void __launch_bounds__(kSortThreads) radix_count(const uint64_t* __restrict__ key, int n, int shift,
-                                                            int ntiles, uint32_t* __restrict__ hist) {
+// n_dev (optional) holds the element count on the device; n is then only the capacity the grid
+// was sized for.
+__global__ void __launch_bounds__(kSortThreads) radix_count(const uint64_t* __restrict__ key, int n,
+                                                            const uint32_t* __restrict__ n_dev, int shift, int ntiles,
+                                                            uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[kRadix];
+    if (n_dev) n = min(n, int(*n_dev));
     h[threadIdx.x] = 0;
     __syncthreads();
     const int base = blockIdx.x * kSortTile;
@@ -116,8 +120,8 @@ __global__ void __launch_bounds__(kSortThreads) radix_row_scan(uint32_t* __restr
 __global__ void __launch_bounds__(kSortThreads) radix_scatter(const uint64_t* __restrict__ kin,
                                                               const uint32_t* __restrict__ vin,
                                                               uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                              int n, int shift, int ntiles,
-                                                              const uint32_t* __restrict__ hist,
+                                                              int n, const uint32_t* __restrict__ n_dev, int shift,
+                                                              int ntiles, const uint32_t* __restrict__ hist,
                                                               const uint32_t* __restrict__ tot) {
     __shared__ uint64_t sk[kSortTile];
     __shared__ uint32_t sv[kSortTile];
@@ -128,6 +132,7 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter(const uint64_t* __
     __shared__ uint32_t gbase[kRadix];    // global start of each digit for this tile
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int base = blockIdx.x * kSortTile;
+    if (n_dev) n = min(n, int(*n_dev));
     run[tid] = 0;
     for (int w = 0; w < kSortThreads / 64; ++w) cnt[w][tid] = 0;
     // Digit base = exclusive scan of the digit totals (block scan of 256 values).
@@ -202,7 +207,7 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter(const uint64_t* __
         sv[p] = v[r];
     }
     __syncthreads();
-    const int valid = min(kSortTile, n - base);
+    const int valid = max(0, min(kSortTile, n - base));
 #pragma unroll
     for (int r = 0; r < kSortKPT; ++r) {
         const int p = r * kSortThreads + tid;
@@ -242,6 +247,18 @@ __global__ void __launch_bounds__(kBS) permute_kernel(const uint32_t* __restrict
 
 }  // namespace
 
+void radix_pass(hipStream_t s, const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, int n,
+                const uint32_t* n_dev, int shift, uint32_t* tile_hist) {
+    const int ntiles = int((static_cast<long long>(n) + kSortTile - 1) / kSortTile);
+    uint32_t* tot = tile_hist + static_cast<std::size_t>(ntiles) * kRadix;
+    radix_count<<<ntiles, kSortThreads, 0, s>>>(kin, n, n_dev, shift, ntiles, tile_hist);
+    LHIP_LAUNCH_CHECK();
+    radix_row_scan<<<kRadix, kSortThreads, 0, s>>>(tile_hist, ntiles, tot);
+    LHIP_LAUNCH_CHECK();
+    radix_scatter<<<ntiles, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, n_dev, shift, ntiles, tile_hist, tot);
+    LHIP_LAUNCH_CHECK();
+}
+
 std::size_t radix_tile_hist_entries(int n) {
     const std::size_t ntiles = (static_cast<std::size_t>(n) + kSortTile - 1) / kSortTile;
     return ntiles * kRadix + kRadix;  // + digit totals
@@ -252,17 +269,10 @@ void tree_sort(hipStream_t s, TreeBufs& b, int n, bool permute_cost) {
     launch_bbox(s, b.x, b.y, b.z, n, b.bbox_part, b.cube);
     keys_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(b.x, b.y, b.z, b.cube, b.key, b.idx, n);
     LHIP_LAUNCH_CHECK();
-    const int ntiles = int((static_cast<long long>(n) + kSortTile - 1) / kSortTile);
-    uint32_t* tot = b.tile_hist + static_cast<std::size_t>(ntiles) * kRadix;
     uint64_t *ka = b.key, *kb = b.key2;
     uint32_t *va = b.idx, *vb = b.idx2;
     for (int shift = 0; shift < 63; shift += 8) {
-        radix_count<<<ntiles, kSortThreads, 0, s>>>(ka, n, shift, ntiles, b.tile_hist);
-        LHIP_LAUNCH_CHECK();
-        radix_row_scan<<<kRadix, kSortThreads, 0, s>>>(b.tile_hist, ntiles, tot);
-        LHIP_LAUNCH_CHECK();
-        radix_scatter<<<ntiles, kSortThreads, 0, s>>>(ka, va, kb, vb, n, shift, ntiles, b.tile_hist, tot);
-        LHIP_LAUNCH_CHECK();
+        radix_pass(s, ka, va, kb, vb, n, nullptr, shift, b.tile_hist);
         std::swap(ka, kb);
         std::swap(va, vb);
     }

@@ -1,5 +1,5 @@
-// Tree walk: one wavefront per walk group, stackless traversal of the pre-order octree, interaction
-// lists buffered in LDS and evaluated with every lane busy.
+// Tree walk: one wavefront per walk group, a parallel depth-first traversal of the level-order
+// octree, interaction lists buffered in LDS and evaluated with every lane busy.
 //
 // Reference: walk / build_list / eval_group (src/cpu_backend.cpp:667-689, 816-882). Same
 // acceptance test against the group's bounding box (a node is accepted when it does not contain
@@ -7,11 +7,14 @@
 // leaves of <= 4 bodies contribute bodies, other accepted nodes their monopole + quadrupole), same
 // self exclusion by index. The list a group gets is therefore the reference's list.
 //
-// Traversal. Nodes are in depth-first pre-order: the subtree of node k is [k, skip_k). A window of
-// 64 consecutive nodes is tested at once, one per lane (coalesced loads). A node is visited iff no
-// earlier node of the window that ends the descent (accepted, or a leaf) skips past it: with
-// s_j = skip_j for such nodes and 0 otherwise, node k is visited iff max_{j<k} s_j <= k, an
-// exclusive prefix-max across the wave. The next window starts at max(window end, max_j s_j).
+// Traversal. Nodes are in level order, so the children of a node are one contiguous block. The wave
+// keeps a stack of pending nodes in LDS; each step pops up to 64 of them (one per lane), tests them
+// in parallel, compacts accepted cells and opened leaves into the lists (ballot + lane prefix), and
+// pushes the child blocks of opened internal nodes (lane prefix sum of child counts). The number
+// popped is limited so that the stack cannot outgrow its LDS array: a step popping P nodes grows it
+// by at most 7P, so P is chosen to keep it below a threshold; above it the wave pops one node at a
+// time (a plain depth-first walk), which adds at most 7 per level of the tree. The stack holds the
+// same nodes the reference's depth-first build_list visits, so the lists are the same sets.
 //
 // Evaluation. Accepted cells are staged in LDS with their moments, bodies of opened leaves with
 // their positions. Walk groups hold 1-128 bodies (39 on average for a Plummer sphere), so one body
@@ -31,8 +34,8 @@ namespace {
 
 constexpr int kWalkWaves = 4;
 constexpr int kWalkBS = 64 * kWalkWaves;
-constexpr int kCellCap = 128;   // cells staged per wave
-constexpr int kBodyCap = 256;   // bodies staged per wave
+constexpr int kCellCap = 64;    // cells staged per wave
+constexpr int kBodyCap = 128;   // bodies staged per wave (>= 128: also the share-reduction buffer)
 constexpr int kMaxShare = 8;
 
 struct Slot {
@@ -112,15 +115,6 @@ __device__ __forceinline__ void eval_bodies(Slot (&sl)[2], int S, int nb, const 
     }
 }
 
-__device__ __forceinline__ int prefix_max_incl(int v, int lane) {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int u = __shfl_up(v, d, 64);
-        if (lane >= d) v = max(v, u);
-    }
-    return v;
-}
-
 __device__ __forceinline__ int prefix_sum_incl(int v, int lane) {
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -150,15 +144,21 @@ __device__ __forceinline__ uint32_t lanes_below(unsigned long long mask) {
     return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
 }
 
+constexpr int kStackCap = 1024;
+// Deepest possible tree: levels 0..21 (no key restarts).
+constexpr int kStackThr = kStackCap - 7 * kLevels - 8;
+
 template <bool POT, bool GUARD>
 __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
     __shared__ WaveLds lds[kWalkWaves];
+    __shared__ int stk[kWalkWaves][kStackCap];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     WaveLds& L = lds[w];
-    const int M = int(a.cnt_scan[a.n]);
+    int* st = stk[w];
     const int ngroups = int(a.gcount[a.n]);
     const float e2 = a.eps2;
     unsigned long long t_inter = 0, t_p2p = 0, t_p2c = 0, t_listed = 0;
+    bool overflow = false;
 
     while (true) {
         int g = 0;
@@ -214,34 +214,43 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                 nb = 0;
             };
 
-            int cur = 0;
-            while (cur < M) {
-                const int k = cur + lane;
-                const bool act = k < M;
-                float4 T = make_float4(0.f, 0.f, 0.f, 0.f);
-                NodeLink nl{0, 0, 0, 0};
-                if (act) {
-                    T = a.trav[k];
-                    nl = a.link[k];
-                }
+            if (lane == 0) st[0] = 0;  // the root
+            int top = 1;
+            while (top > 0) {
+                const int room = (kStackThr - top) / 7;
+                const int P = min(64, min(top, max(1, room)));
+                __builtin_amdgcn_wave_barrier();
+                const bool act = lane < P;
+                const int k = act ? st[top - P + lane] : 0;
+                __builtin_amdgcn_wave_barrier();  // the popped entries are read before pushes overwrite them
+                top -= P;
+                WalkNode nd{0.f, 0.f, 0.f, 0.f, 0, 0, 0, -1};
+                if (act) nd = a.node[k];
                 // A node holding bodies of this (sub)group is always opened: the reference's
                 // "contains" test for whole groups; any overlap for chunks of an over-full group.
-                const bool keep_open = chunked ? (nl.lo < chi && clo < nl.hi) : (nl.lo <= clo && chi <= nl.hi);
-                const float dx = fmaxf(fabsf(T.x - bx) - hx, 0.f);
-                const float dy = fmaxf(fabsf(T.y - by) - hy, 0.f);
-                const float dz = fmaxf(fabsf(T.z - bz) - hz, 0.f);
+                const bool keep_open = chunked ? (nd.lo < chi && clo < nd.hi) : (nd.lo <= clo && chi <= nd.hi);
+                const float dx = fmaxf(fabsf(nd.cx - bx) - hx, 0.f);
+                const float dy = fmaxf(fabsf(nd.cy - by) - hy, 0.f);
+                const float dz = fmaxf(fabsf(nd.cz - bz) - hz, 0.f);
                 // Rounded as the reference build evaluates it: three products, then two sums.
                 const float d2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
-                const bool acc = act && !keep_open && d2 >= T.w;
-                const bool leaf = act && info_leaf(nl.info);
-                const bool term = acc || leaf;
-                const int incl = prefix_max_incl(term ? nl.skip : 0, lane);
-                int excl = __shfl_up(incl, 1, 64);
-                if (lane == 0) excl = 0;
-                const bool visited = act && excl <= k;
-                const bool small = leaf && (nl.hi - nl.lo) <= kSmallLeaf;
-                const bool is_cell = visited && acc && !small;
-                const bool is_range = visited && leaf && (!acc || small);
+                const bool acc = act && !keep_open && d2 >= nd.rcrit2;
+                const bool leaf = act && info_leaf(nd.info);
+                const bool small = leaf && (nd.hi - nd.lo) <= kSmallLeaf;
+                const bool is_cell = acc && !small;
+                const bool is_range = leaf && (!acc || small);
+                const bool open_int = act && !acc && !leaf;
+
+                // push the child blocks of opened internal nodes
+                const int nch = open_int ? info_nchild(nd.info) : 0;
+                const int cend = prefix_sum_incl(nch, lane);
+                const int pushed = __builtin_amdgcn_readlane(cend, 63);
+                if (top + pushed > kStackCap) {
+                    overflow = true;
+                    break;
+                }
+                for (int c = 0; c < nch; ++c) st[top + cend - nch + c] = nd.first + c;
+                top += pushed;
 
                 const unsigned long long mc = __ballot(is_cell);
                 const int ncnew = __popcll(mc);
@@ -259,7 +268,7 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                 // Bodies of opened leaves: staged in chunks of at most the free buffer space.
                 const unsigned long long mr = __ballot(is_range);
                 if (mr) {
-                    const int len = is_range ? nl.hi - nl.lo : 0;
+                    const int len = is_range ? nd.hi - nd.lo : 0;
                     const int end = prefix_sum_incl(len, lane);
                     const int off = end - len;
                     const int total = __builtin_amdgcn_readlane(end, 63);
@@ -269,7 +278,7 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                         const int take = min(kBodyCap - nb, total - done);
                         const int u0 = max(off, done), u1 = min(off + len, done + take);
                         for (int u = u0; u < u1; ++u) {
-                            const int j = nl.lo + (u - off);
+                            const int j = nd.lo + (u - off);
                             L.bP[nb + (u - done)] = a.posm[j];
                             L.bI[nb + (u - done)] = j;
                         }
@@ -277,14 +286,12 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                         done += take;
                     }
                 }
-                const int wmax = __builtin_amdgcn_readlane(incl, 63);
-                cur = max(cur + 64, wmax);
             }
             flush_cells();
             flush_bodies();
 
             // Add the list shares of each body (slot v = share * nbody + body) through LDS.
-            float4* red = L.bP;                      // kBodyCap >= 128 entries
+            float4* red = L.bP;                      // kBodyCap >= 128 = 64 * NS entries
             uint32_t* rself = reinterpret_cast<uint32_t*>(L.bI);
             __builtin_amdgcn_wave_barrier();
 #pragma unroll
@@ -327,6 +334,7 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
             t_inter += (lane == 0 ? (unsigned long long)listed * nbody : 0ull) - self_sum;
         }
     }
+    if (overflow && lane == 0) atomicOr(&a.counters->error, 1);
     t_inter = wave_sum_u64(t_inter);
     t_p2p = wave_sum_u64(t_p2p);
     t_p2c = wave_sum_u64(t_p2c);

@@ -10,11 +10,21 @@ struct alignas(16) WalkCounters {
     unsigned long long p2p;           // body-body (self excluded)
     unsigned long long p2c;           // body-node
     unsigned long long listed;        // list entries per body summed (the reference CPU count)
-    int error;                        // nonzero: walk failure
+    int error;                        // nonzero: walk failure (traversal stack overflow)
     int overfull;                     // bodies in finest-level cells with > 16 bodies
     unsigned int next_group;          // walk work counter
     int pad;
 };
+
+// The walk's node record, in level order (all cells of level 0, then level 1, ...; within a level
+// by first body), so that the children of a node are contiguous: [first, first + nchild).
+struct alignas(32) WalkNode {
+    float cx, cy, cz, rcrit2;  // centre of mass; open the node if dist^2(group box, com) < rcrit2
+    int32_t lo, hi;            // body range [lo, hi) in sorted order
+    int32_t info;              // level | leaf << 5 | children << 8
+    int32_t first;             // first child (level order), -1 for a leaf
+};
+static_assert(sizeof(WalkNode) == 32, "WalkNode is one 32-byte record");
 
 struct BuildBufs {
     int n;
@@ -26,10 +36,18 @@ struct BuildBufs {
     uint32_t* cnt;             // n + 1: nodes starting at each body, then their exclusive scan
     uint32_t* gflag;           // n + 1: group starts, then their exclusive scan
     uint32_t* scan_tmp;        // scan block sums
-    NodeLink* link;            // cap_nodes
-    NodeMom* mom;              // cap_nodes
-    float4* trav;              // cap_nodes: com xyz + rcrit2 (the walk's test data)
-    uint64_t* prefix;          // cap_nodes
+    // pre-order scratch (node k = the k-th cell by (first body, level))
+    uint64_t* pkey;            // cap: level of node k (sort key)
+    uint64_t* pkey2;
+    uint32_t* pval;            // cap: k, then the level-order permutation
+    uint32_t* pval2;
+    uint32_t* inv;             // cap: pre-order -> level order
+    uint32_t* tile_hist;       // radix histograms for cap_nodes elements
+    uint32_t* levstart;        // kLevels + 2 level starts in level order
+    // level-order node data
+    WalkNode* node;            // cap
+    NodeMom* mom;              // cap
+    uint64_t* prefix;          // cap
     uint32_t* gstart;          // n + 1
     WalkCounters* counters;
 };
@@ -39,15 +57,14 @@ std::size_t scan_tmp_entries(int n);
 // level (cells of one level are disjoint and hold >= 17 bodies each).
 inline long long node_capacity(long long n) { return n + (n / 17 + 1) * kLevels + 1; }
 
-// Leaf levels, node enumeration, groups and multipoles. inv_theta as the reference CPU computes it.
+// Leaf levels, node enumeration, level-order layout, groups and multipoles. inv_theta as the
+// reference CPU computes it.
 void tree_build(hipStream_t s, const BuildBufs& b, float inv_theta);
 
 struct WalkArgs {
     const float4* posm;
-    const float4* trav;
-    const NodeLink* link;
+    const WalkNode* node;
     const NodeMom* mom;
-    const uint32_t* cnt_scan;  // node count = cnt_scan[n]
     const uint32_t* gstart;
     const uint32_t* gcount;    // group count = gcount[n]
     int n;
