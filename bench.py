@@ -66,6 +66,34 @@ def cpu_baseline(b, steps: int = 2):
                 sample=f"N=2^20 Plummer tree theta=0.6 step: 1 warm-up force call + {steps} KDK steps")
 
 
+def adjoint_leg(be):
+    """Adjoint through direct summation: Plummer N = 65,536 (a = 1, cut 10), eps 0.05, 200 KDK steps
+    of dt 1e-3, checkpoints every 20 steps with recompute. Target snapshot: the forward run from
+    positions perturbed by N(0, 0.01^2) and masses scaled by (1 + 0.01 N(0, 1)); loss = sum of squared
+    final-position differences. Times one forward and one backward (device events)."""
+    from lettuce_simulate_physics_amd.autograd import SimParams, Simulation
+
+    n, steps = 65536, 200
+    b = plummer(n)
+    x0 = np.stack([b.x, b.y, b.z], 1).astype(np.float64)
+    v0 = np.stack([b.vx, b.vy, b.vz], 1).astype(np.float64)
+    m = b.m.astype(np.float64)
+    rng = np.random.default_rng(2026)
+    sim = Simulation(be, SimParams(G=1.0, eps=0.05, dt=1e-3, steps=steps, checkpoint_every=20))
+    xt, _ = sim.forward(x0 + 0.01 * rng.standard_normal(x0.shape), v0, m * (1 + 0.01 * rng.standard_normal(n)))
+    xT, vT = sim.forward(x0, v0, m)
+    fwd_ms, _ = sim.times()
+    gx, gv, gm = sim.backward(2 * (xT - xt), np.zeros_like(vT))
+    _, bwd_ms = sim.times()
+    assert np.all(np.isfinite(gx)) and np.all(np.isfinite(gm))
+    pairs = float(n) * (n - 1)
+    return {"workload": "plummer_direct_adjoint_65536", "steps": steps, "checkpoint_every": 20,
+            "forward_ms_per_step": round(fwd_ms / steps, 4), "adjoint_ms_per_step": round(bwd_ms / steps, 4),
+            "adjoint_forward_ratio": round(bwd_ms / fwd_ms, 3),
+            "forward_gpairs_per_s": round(pairs * (steps + 1) / (fwd_ms * 1e-3) / 1e9, 1),
+            "loss": float(((xT - xt) ** 2).sum())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +102,7 @@ def main():
     ap.add_argument("--n", type=int, default=N_BODIES)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -126,7 +155,7 @@ def main():
         traffic = json.load(open(tpath)).get("bytes_per_launch")
 
     out = {
-        "metric": "tree ms/step & Ginteractions/s at N=1M theta=0.6",
+        "metric": "tree ms/step & Ginteractions/s at N=1M theta=0.6; adjoint/forward step-time ratio",
         "value": round(ginter, 3),
         "unit": "Ginteractions/s",
         "n_gpus": 1,
@@ -149,7 +178,11 @@ def main():
                      "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4),
                      "traffic": traffic},
         "cpu_baseline": None,
+        "adjoint": None,
     }
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg(be)
+        out["adjoint_forward_ratio"] = out["adjoint"]["adjoint_forward_ratio"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(b, args.cpu_steps)
         out["cpu_baseline"]["value"] = round(out["cpu_baseline"]["value"], 4)

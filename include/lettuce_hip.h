@@ -137,6 +137,32 @@ int lhip_tree_nodes(lhip_ctx* ctx, int32_t* link, uint64_t* prefix, float* mom);
 /* Walk-group starts (groups + 1 entries; the last is n). */
 int lhip_tree_groups(lhip_ctx* ctx, uint32_t* gstart);
 
+/* ---- differentiable simulation (no reference counterpart: the reference has no differentiation) ----
+ *
+ * FP64 state (positions, velocities, masses) integrated with the reference's KDK leapfrog
+ * (src/simulation.cpp:406-438) for `steps` steps of dt, direct-sum forces (mixed precision: FP32 pair
+ * arithmetic on FP64 separations, FP64 accumulation; or all-FP64). The forward keeps (x, v)
+ * checkpoints every `checkpoint_every` steps on the device; the backward recomputes each segment from
+ * its checkpoint and runs the adjoint sweep. Arrays are n x 3 (row-major) or n, host memory. */
+enum { LHIP_PRECISION_MIXED = 0, LHIP_PRECISION_F64 = 1 };
+
+typedef struct lhip_sim_params {
+    double G, eps, dt, theta;
+    int32_t steps;
+    int32_t checkpoint_every;
+    int32_t method;     /* LHIP_METHOD_DIRECT */
+    int32_t precision;  /* LHIP_PRECISION_MIXED or LHIP_PRECISION_F64 */
+} lhip_sim_params;
+
+/* Runs the forward from (x0, v0, m); writes the final state (xT, vT may be NULL). */
+int lhip_sim_forward(lhip_ctx* ctx, const lhip_sim_params* p, uint64_t n, const double* x0, const double* v0,
+                     const double* m, double* xT, double* vT);
+/* Given dL/dxT and dL/dvT, writes dL/dx0, dL/dv0, dL/dm (any output may be NULL). LHIP_LOGIC without
+ * a forward. */
+int lhip_sim_backward(lhip_ctx* ctx, const double* gxT, const double* gvT, double* gx0, double* gv0, double* gm);
+/* Device time of the last forward and backward (HIP events on the context's stream). */
+int lhip_sim_times(lhip_ctx* ctx, double* forward_ms, double* backward_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,426 @@
+// Context: one backend instance (device buffers, one stream, phase events). Semantics follow the
+// reference Backend contract (src/backend.hpp:89-141) and the CUDA backend's state machine
+// (src/cuda_backend.cu:1289-1650): uploads synchronise, kick/drift are asynchronous, drift
+// invalidates the tree, build_tree invalidates accelerations, compute_forces_with synchronises once
+// at its end.
+#pragma once
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lettuce_hip.h"
+#include "kernels.hpp"
+#include "tree.hpp"
+
+namespace lhip {
+
+struct AdjSim;  // differentiable FP64-state simulation (adjoint.hip)
+void adj_destroy(AdjSim* a);
+
+struct Context {
+    AdjSim* adj = nullptr;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    std::string name, err;
+    int n = 0;
+    int cus = 1;
+    int walk_blocks = 1;
+
+    // bodies (backend order) and their permute alternates
+    DevBuf x, y, z, vx, vy, vz, m, id, cost;
+    DevBuf x2, y2, z2, vx2, vy2, vz2, m2, id2, cost2;
+    DevBuf ax, ay, az, pot, posm;
+    // sort
+    DevBuf key, key2, idx, idx2, tile_hist;
+    // small
+    DevBuf cube, bbox_part, red_part, red_out, direct_part;
+    // tree
+    DevBuf bl, cnt, gflag, scan_tmp, pkey, pkey2, pval, pval2, inv, levstart, node, mom, prefix, gstart, counters;
+    long long cap_nodes = 0;
+
+    bool has_pe = false, acc_ok = false, tree_ok = false, costs_ok = false;
+    bool kick_rec = false, drift_rec = false;
+    lhip_phase_times t{};
+    hipEvent_t ev[10] = {};  // 0 build start, 1 sorted, 2 built, 3 walk start, 4 walk end, 5-6 kick, 7-8 drift
+
+    explicit Context(int device) : dev(device) {
+        LHIP_CHECK(hipSetDevice(dev));
+        hipDeviceProp_t prop{};
+        LHIP_CHECK(hipGetDeviceProperties(&prop, dev));
+        cus = prop.multiProcessorCount;
+        char buf[512];
+        std::snprintf(buf, sizeof buf, "hip: %s (%s, %d CUs, %.0f GB)", prop.name[0] ? prop.name : "AMD GPU",
+                      prop.gcnArchName, cus,
+                      double(prop.totalGlobalMem) / 1073741824.0);
+        name = buf;
+        // Default priority on purpose: streams of different priorities are not mixed here.
+        LHIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        for (auto& e : ev) LHIP_CHECK(hipEventCreate(&e));
+        cube.reserve(sizeof(Cube));
+        bbox_part.reserve(sizeof(float) * 6 * kReduceBlocks);
+        red_part.reserve(sizeof(double) * 12 * std::max<std::size_t>(kReduceBlocks, 1));
+        red_out.reserve(sizeof(double) * 16);
+        counters.reserve(sizeof(WalkCounters));
+        walk_blocks = cus * walk_blocks_per_cu();
+    }
+    ~Context() {
+        adj_destroy(adj);
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+
+    void reserve_bodies(int nn) {
+        const std::size_t f = std::size_t(nn) * 4;
+        for (DevBuf* b : {&x, &y, &z, &vx, &vy, &vz, &m, &id, &cost, &x2, &y2, &z2, &vx2, &vy2, &vz2, &m2, &id2, &cost2,
+                          &ax, &ay, &az, &pot})
+            b->reserve(f);
+        posm.reserve(std::size_t(nn) * sizeof(float4));
+    }
+
+    void reserve_tree(int nn) {
+        const std::size_t N = std::size_t(nn);
+        key.reserve(N * 8); key2.reserve(N * 8);
+        idx.reserve(N * 4); idx2.reserve(N * 4);
+        cap_nodes = node_capacity(nn);
+        tile_hist.reserve(radix_tile_hist_entries(int(std::max<long long>(nn, cap_nodes))) * 4);
+        bl.reserve(N * 4);
+        cnt.reserve((N + 1) * 4);
+        gflag.reserve((N + 1) * 4);
+        gstart.reserve((N + 1) * 4);
+        scan_tmp.reserve(scan_tmp_entries(nn) * 4);
+        const std::size_t C = std::size_t(cap_nodes);
+        pkey.reserve(C * 8); pkey2.reserve(C * 8);
+        pval.reserve(C * 4); pval2.reserve(C * 4); inv.reserve(C * 4);
+        levstart.reserve((kLevels + 2) * 4);
+        node.reserve(C * sizeof(WalkNode));
+        mom.reserve(C * sizeof(NodeMom));
+        prefix.reserve(C * 8);
+    }
+
+    float ms(int a, int b) const {
+        float v = 0;
+        LHIP_CHECK(hipEventElapsedTime(&v, ev[a], ev[b]));
+        return v;
+    }
+
+    void sync() { LHIP_CHECK(hipStreamSynchronize(stream)); }
+
+    // ---- Backend members ----
+
+    void upload(uint64_t nn, const float* hx, const float* hy, const float* hz, const float* hvx, const float* hvy,
+                const float* hvz, const float* hm, const uint32_t* hid) {
+        if (nn >= uint64_t(INT32_MAX)) throw Error(kInvalid, "hip backend: n = " + std::to_string(nn) +
+                                                                " exceeds the 2^31-1 body limit");
+        LHIP_CHECK(hipSetDevice(dev));
+        reserve_bodies(int(nn));
+        n = int(nn);
+        has_pe = false;
+        acc_ok = true;  // zeroed below
+        tree_ok = costs_ok = false;
+        kick_rec = drift_rec = false;
+        t = lhip_phase_times{};
+        if (n == 0) return;
+        const std::size_t fb = std::size_t(n) * 4;
+        const std::pair<DevBuf*, const void*> src[] = {{&x, hx}, {&y, hy}, {&z, hz}, {&vx, hvx},
+                                                       {&vy, hvy}, {&vz, hvz}, {&m, hm}, {&id, hid}};
+        for (auto& s : src) LHIP_CHECK(hipMemcpyAsync(s.first->as<void>(), s.second, fb, hipMemcpyHostToDevice, stream));
+        for (DevBuf* b : {&ax, &ay, &az, &pot}) launch_zero_f32(stream, b->as<float>(), n);
+        sync();
+    }
+
+    void download(float* hx, float* hy, float* hz, float* hvx, float* hvy, float* hvz, float* hm, uint32_t* hid) {
+        LHIP_CHECK(hipSetDevice(dev));
+        if (n == 0) return;
+        const std::size_t fb = std::size_t(n) * 4;
+        const std::pair<void*, DevBuf*> dst[] = {{hx, &x}, {hy, &y}, {hz, &z}, {hvx, &vx},
+                                                 {hvy, &vy}, {hvz, &vz}, {hm, &m}, {hid, &id}};
+        for (auto& d : dst)
+            if (d.first) LHIP_CHECK(hipMemcpyAsync(d.first, d.second->as<void>(), fb, hipMemcpyDeviceToHost, stream));
+        sync();
+    }
+
+    void kick(double dt) {
+        LHIP_CHECK(hipSetDevice(dev));
+        if (n == 0) return;
+        if (!acc_ok) throw Error(kLogic, "hip kick: accelerations are stale (build_tree since the last force call)");
+        LHIP_CHECK(hipEventRecord(ev[5], stream));
+        launch_kick(stream, vx.as<float>(), vy.as<float>(), vz.as<float>(), ax.as<float>(), ay.as<float>(),
+                    az.as<float>(), float(dt), n);
+        LHIP_CHECK(hipEventRecord(ev[6], stream));
+        kick_rec = true;
+    }
+
+    void drift(double dt) {
+        LHIP_CHECK(hipSetDevice(dev));
+        if (n == 0) return;
+        LHIP_CHECK(hipEventRecord(ev[7], stream));
+        launch_drift(stream, x.as<float>(), y.as<float>(), z.as<float>(), vx.as<float>(), vy.as<float>(),
+                     vz.as<float>(), float(dt), n);
+        LHIP_CHECK(hipEventRecord(ev[8], stream));
+        drift_rec = true;
+        tree_ok = false;
+        has_pe = false;
+    }
+
+    void build_tree(const lhip_force_params& p) {
+        LHIP_CHECK(hipSetDevice(dev));
+        tree_ok = false;
+        t.sort_ms = t.build_ms = 0;
+        if (p.method != LHIP_METHOD_TREE || n == 0) return;
+        // The sort reorders the bodies but not a, phi or costs: stale until the next force call.
+        reserve_tree(n);
+        const bool keep_cost = false;
+        costs_ok = false;
+        has_pe = false;
+        acc_ok = false;
+        LHIP_CHECK(hipEventRecord(ev[0], stream));
+        TreeBufs tb{};
+        tb.x = x.as<float>(); tb.y = y.as<float>(); tb.z = z.as<float>();
+        tb.vx = vx.as<float>(); tb.vy = vy.as<float>(); tb.vz = vz.as<float>();
+        tb.m = m.as<float>(); tb.id = id.as<uint32_t>(); tb.cost = cost.as<float>();
+        tb.x2 = x2.as<float>(); tb.y2 = y2.as<float>(); tb.z2 = z2.as<float>();
+        tb.vx2 = vx2.as<float>(); tb.vy2 = vy2.as<float>(); tb.vz2 = vz2.as<float>();
+        tb.m2 = m2.as<float>(); tb.id2 = id2.as<uint32_t>(); tb.cost2 = cost2.as<float>();
+        tb.posm = posm.as<float4>();
+        tb.key = key.as<uint64_t>(); tb.key2 = key2.as<uint64_t>();
+        tb.idx = idx.as<uint32_t>(); tb.idx2 = idx2.as<uint32_t>();
+        tb.tile_hist = tile_hist.as<uint32_t>();
+        tb.cube = cube.as<Cube>();
+        tb.bbox_part = bbox_part.as<float>();
+        tree_sort(stream, tb, n, keep_cost);
+        // The permute wrote every body array into its alternate: swap the owners.
+        swap(x, x2); swap(y, y2); swap(z, z2); swap(vx, vx2); swap(vy, vy2); swap(vz, vz2);
+        swap(m, m2); swap(id, id2);
+        if (keep_cost) swap(cost, cost2);
+        LHIP_CHECK(hipEventRecord(ev[1], stream));
+        BuildBufs bb{};
+        bb.n = n;
+        bb.cap_nodes = int(cap_nodes);
+        bb.key = key.as<uint64_t>();
+        bb.posm = posm.as<float4>();
+        bb.cube = cube.as<Cube>();
+        bb.bl = bl.as<uint32_t>();
+        bb.cnt = cnt.as<uint32_t>();
+        bb.gflag = gflag.as<uint32_t>();
+        bb.scan_tmp = scan_tmp.as<uint32_t>();
+        bb.pkey = pkey.as<uint64_t>(); bb.pkey2 = pkey2.as<uint64_t>();
+        bb.pval = pval.as<uint32_t>(); bb.pval2 = pval2.as<uint32_t>();
+        bb.inv = inv.as<uint32_t>();
+        bb.tile_hist = tile_hist.as<uint32_t>();
+        bb.levstart = levstart.as<uint32_t>();
+        bb.node = node.as<WalkNode>();
+        bb.mom = mom.as<NodeMom>();
+        bb.prefix = prefix.as<uint64_t>();
+        bb.gstart = gstart.as<uint32_t>();
+        bb.counters = counters.as<WalkCounters>();
+        // inv_theta as the reference CPU forms it (src/cpu_backend.cpp:595).
+        const float inv_theta = p.theta > 0 ? float(1.0 / p.theta) : FLT_MAX;
+        tree_build(stream, bb, inv_theta);
+        LHIP_CHECK(hipEventRecord(ev[2], stream));
+        tree_ok = true;
+    }
+
+    void compute_forces_with(const lhip_force_params& p, bool potential) {
+        LHIP_CHECK(hipSetDevice(dev));
+        t.walk_ms = t.force_ms = 0;
+        t.interactions = t.p2p = t.p2c = t.listed = 0;
+        costs_ok = false;
+        if (n == 0) {
+            t.sort_ms = t.build_ms = 0;
+            has_pe = potential;
+            acc_ok = true;
+            return;
+        }
+        const bool tree = p.method == LHIP_METHOD_TREE;
+        if (tree && !tree_ok)
+            throw Error(kLogic, "hip compute_forces_with: call build_tree after the last upload/drift first");
+        const float G = float(p.G), eps2 = float(p.eps * p.eps);
+        LHIP_CHECK(hipEventRecord(ev[3], stream));
+        WalkCounters hc{};
+        if (!tree) {
+            launch_pack_posm(stream, x.as<float>(), y.as<float>(), z.as<float>(), m.as<float>(), posm.as<float4>(), n);
+            const int S = direct_split(n);
+            if (S > 1) direct_part.reserve(std::size_t(S) * std::size_t(n) * sizeof(float4));
+            launch_direct(stream, posm.as<float4>(), n, eps2, G, potential, ax.as<float>(), ay.as<float>(),
+                          az.as<float>(), pot.as<float>(), direct_part.as<float4>());
+            LHIP_CHECK(hipEventRecord(ev[4], stream));
+            sync();
+            const uint64_t pairs = uint64_t(n) * uint64_t(n - 1);
+            hc.interactions = pairs;
+            hc.p2p = pairs;
+            hc.listed = uint64_t(n) * uint64_t(n);
+            direct_costs_pending = true;
+        } else {
+            WalkArgs a{};
+            a.posm = posm.as<float4>();
+            a.node = node.as<WalkNode>();
+            a.mom = mom.as<NodeMom>();
+            a.gstart = gstart.as<uint32_t>();
+            a.gcount = gflag.as<uint32_t>();
+            a.n = n;
+            a.eps2 = eps2;
+            a.G = G;
+            a.ax = ax.as<float>(); a.ay = ay.as<float>(); a.az = az.as<float>(); a.pot = pot.as<float>();
+            a.cost = cost.as<float>();
+            a.counters = counters.as<WalkCounters>();
+            a.blocks = walk_blocks;
+            tree_walk(stream, a, potential);
+            LHIP_CHECK(hipEventRecord(ev[4], stream));
+            LHIP_CHECK(hipMemcpyAsync(&hc, counters.as<void>(), sizeof hc, hipMemcpyDeviceToHost, stream));
+            sync();
+            if (hc.error)
+                throw Error(kRuntime, "hip tree walk: traversal stack overflow; accelerations are invalid");
+            direct_costs_pending = false;
+        }
+        t.sort_ms = tree ? ms(0, 1) : 0.0;
+        t.build_ms = tree ? ms(1, 2) : 0.0;
+        t.walk_ms = ms(3, 4);
+        t.force_ms = t.sort_ms + t.build_ms + t.walk_ms;
+        t.interactions = hc.interactions;
+        t.p2p = hc.p2p;
+        t.p2c = hc.p2c;
+        t.listed = hc.listed;
+        overfull = hc.overfull;
+        has_pe = potential;
+        acc_ok = true;
+        costs_ok = true;
+    }
+
+    bool direct_costs_pending = false;
+    int overfull = 0;
+
+    void compute_forces(const lhip_force_params& p, bool potential) {
+        build_tree(p);
+        compute_forces_with(p, potential);
+    }
+
+    lhip_diag diagnostics() {
+        LHIP_CHECK(hipSetDevice(dev));
+        lhip_diag d{};
+        d.has_pe = has_pe ? 1 : 0;
+        if (n == 0) return d;
+        launch_diag(stream, x.as<float>(), y.as<float>(), z.as<float>(), vx.as<float>(), vy.as<float>(),
+                    vz.as<float>(), m.as<float>(), pot.as<float>(), has_pe ? 1 : 0, n, red_part.as<double>(),
+                    red_out.as<double>());
+        double v[12];
+        LHIP_CHECK(hipMemcpyAsync(v, red_out.as<void>(), sizeof v, hipMemcpyDeviceToHost, stream));
+        sync();
+        d.mass = v[0];
+        d.ke = v[1];
+        d.pe = has_pe ? 0.5 * v[2] : 0.0;
+        d.px = v[3]; d.py = v[4]; d.pz = v[5];
+        d.lx = v[6]; d.ly = v[7]; d.lz = v[8];
+        if (v[0] > 0) { d.cx = v[9] / v[0]; d.cy = v[10] / v[0]; d.cz = v[11] / v[0]; }
+        return d;
+    }
+
+    double exact_pe(double G, double eps) {
+        LHIP_CHECK(hipSetDevice(dev));
+        if (n < 2) return 0.0;
+        launch_pack_posm(stream, x.as<float>(), y.as<float>(), z.as<float>(), m.as<float>(), posm.as<float4>(), n);
+        const std::size_t blocks = grid_for(n, kBS);
+        red_part.reserve(std::max<std::size_t>(blocks, 12 * kReduceBlocks) * sizeof(double));
+        launch_exact_pe(stream, posm.as<float4>(), n, eps * eps, red_part.as<double>(), red_out.as<double>());
+        double h = 0;
+        LHIP_CHECK(hipMemcpyAsync(&h, red_out.as<void>(), sizeof h, hipMemcpyDeviceToHost, stream));
+        sync();
+        // The tree's posm (sorted order) is rebuilt by every sort; nothing else reads it in between.
+        return -0.5 * G * h;
+    }
+
+    lhip_phase_times times() {
+        lhip_phase_times r = t;
+        double step = 0;
+        if (kick_rec) { LHIP_CHECK(hipEventSynchronize(ev[6])); step += ms(5, 6); }
+        if (drift_rec) { LHIP_CHECK(hipEventSynchronize(ev[8])); step += ms(7, 8); }
+        r.step_ms = step;
+        return r;
+    }
+
+    void local_bounds(float lo[3], float hi[3]) {
+        LHIP_CHECK(hipSetDevice(dev));
+        for (int c = 0; c < 3; ++c) lo[c] = hi[c] = 0.f;
+        if (n == 0) return;
+        launch_bbox(stream, x.as<float>(), y.as<float>(), z.as<float>(), n, bbox_part.as<float>(), cube.as<Cube>());
+        Cube c{};
+        LHIP_CHECK(hipMemcpyAsync(&c, cube.as<void>(), sizeof c, hipMemcpyDeviceToHost, stream));
+        sync();
+        for (int k = 0; k < 3; ++k) { lo[k] = c.lo[k]; hi[k] = c.hi[k]; }
+    }
+
+    bool body_costs(float* out) {
+        LHIP_CHECK(hipSetDevice(dev));
+        if (!costs_ok || n == 0) return false;
+        if (direct_costs_pending) {
+            for (int i = 0; i < n; ++i) out[i] = float(n - 1);
+            return true;
+        }
+        LHIP_CHECK(hipMemcpyAsync(out, cost.as<void>(), std::size_t(n) * 4, hipMemcpyDeviceToHost, stream));
+        sync();
+        return true;
+    }
+
+    void device_memory(uint64_t* fr, uint64_t* tot) {
+        LHIP_CHECK(hipSetDevice(dev));
+        std::size_t f = 0, tt = 0;
+        LHIP_CHECK(hipMemGetInfo(&f, &tt));
+        std::size_t held = 0;
+        for (const DevBuf* b : {&x, &y, &z, &vx, &vy, &vz, &m, &id, &cost, &x2, &y2, &z2, &vx2, &vy2, &vz2, &m2, &id2,
+                                &cost2, &ax, &ay, &az, &pot, &posm, &key, &key2, &idx, &idx2, &tile_hist, &direct_part,
+                                &bl, &cnt, &gflag, &scan_tmp, &pkey, &pkey2, &pval, &pval2, &inv, &node, &mom, &prefix, &gstart})
+            held += b->capacity();
+        *fr = f + held;
+        *tot = tt;
+    }
+};
+
+// Device bytes per body: 22 four-byte state/alternate arrays + posm; the tree adds keys and indices
+// (double-buffered) and per-body build arrays, plus node records for the bound node_capacity(n)
+// (WalkNode 32 + NodeMom 64 + prefix 8 + level-sort scratch 28 = 132 B per node).
+inline void memory_need(uint64_t n, int method, uint64_t* dev, uint64_t* host) {
+    uint64_t d = n * (22 * 4 + 16) + (uint64_t(64) << 20);
+    if (method == LHIP_METHOD_TREE) {
+        d += n * (16 + 8 + 4 + 4 * 4) + radix_tile_hist_entries(int(std::min<uint64_t>(n, INT32_MAX))) * 4;
+        d += uint64_t(node_capacity((long long)n)) * 132;
+    } else {
+        d += n * 16 * 64;  // worst-case split partial sums
+    }
+    *dev = d;
+    *host = 0;
+}
+
+
+}  // namespace lhip
+
+struct lhip_ctx {
+    lhip::Context* c;
+};
+
+namespace lhip {
+
+// Runs f(Context&) for a C entry point: exceptions become status codes and the context's message.
+template <class F>
+int guarded(lhip_ctx* ctx, F&& f) {
+    if (!ctx || !ctx->c) return LHIP_INVALID;
+    try {
+        f(*ctx->c);
+        ctx->c->err.clear();
+        return LHIP_OK;
+    } catch (const Error& e) {
+        ctx->c->err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        ctx->c->err = e.what();
+        return LHIP_RUNTIME;
+    }
+}
+
+}  // namespace lhip

@@ -144,6 +144,27 @@ def direct_f64(b: Bodies, G: float, eps: float, targets=None):
     return np.stack(out[:3], axis=1), out[3]
 
 
+def sim_forward(x0, v0, m, G, eps, dt, steps):
+    """FP64 KDK with FP64 direct forces (the oracle's differentiable simulation); (xT, vT)."""
+    n = len(m)
+    x0 = np.ascontiguousarray(x0, np.float64).reshape(n, 3)
+    v0 = np.ascontiguousarray(v0, np.float64).reshape(n, 3)
+    m = np.ascontiguousarray(m, np.float64)
+    xT, vT = np.zeros((n, 3)), np.zeros((n, 3))
+    lib().orc_sim_forward(_U64(n), _ptr(x0), _ptr(v0), _ptr(m), _D(G), _D(eps), _D(dt), int(steps), _ptr(xT), _ptr(vT))
+    return xT, vT
+
+
+def sim_backward(gxT, gvT):
+    """Adjoint of the last sim_forward: (dL/dx0, dL/dv0, dL/dm)."""
+    gxT = np.ascontiguousarray(gxT, np.float64)
+    gvT = np.ascontiguousarray(gvT, np.float64)
+    n = gxT.shape[0]
+    gx0, gv0, gm = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+    lib().orc_sim_backward(_ptr(gxT), _ptr(gvT), _ptr(gx0), _ptr(gv0), _ptr(gm))
+    return gx0, gv0, gm
+
+
 class _Common:
     """Accelerations are read as the reference tests do: upload with v = 0, compute, kick(1), read v
     (tests/test_cpu_backend.cpp:45-60)."""

@@ -19,6 +19,8 @@
 //   scenario_disk_sech2  NOT a reference function (the reference disk is Gaussian in z): pinned by
 //                        closed-form moments of the exponential / sech^2 profiles only
 //   direct_f64           textbook O(N^2) sum; two-body closed form a = m/(d^2+eps^2)^1.5
+//   sim_forward/backward FP64 KDK and its adjoint (no reference counterpart): pinned by central finite
+//                        differences of sim_forward on small systems (tests/test_oracle.py)
 #include <omp.h>
 
 #include <algorithm>
@@ -582,6 +584,99 @@ void orc_direct_f64(uint64_t n, const double* x, const double* y, const double* 
         }
         ax[t] = G * sx; ay[t] = G * sy; az[t] = G * sz; phi[t] = -G * sp;
     }
+}
+
+// ---- differentiable FP64 simulation (no reference counterpart) ----
+// Forward: the reference's KDK (src/simulation.cpp:406-438) in FP64 with FP64 direct forces; every
+// position is kept. Backward: the textbook reverse of each step, two half-kick VJPs per step (not
+// merged), each from the explicit Jacobian blocks da_i/dx_k and da_i/dm_k.
+namespace {
+struct Sim {
+    std::size_t n = 0;
+    int steps = 0;
+    double G = 1, eps = 0, dt = 0;
+    std::vector<double> m;
+    std::vector<std::vector<double>> pos;  // x_0 .. x_T, 3n each (AoS)
+};
+Sim g_sim;
+
+void accel64(const Sim& S, const std::vector<double>& x, std::vector<double>& a) {
+    const std::size_t n = S.n;
+    a.assign(3 * n, 0.0);
+    const double e2 = S.eps * S.eps;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < long(n); ++i) {
+        double s[3] = {0, 0, 0};
+        for (std::size_t j = 0; j < n; ++j) {
+            if (long(j) == i) continue;
+            double d[3], r2 = e2;
+            for (int c = 0; c < 3; ++c) { d[c] = x[3 * j + c] - x[3 * i + c]; r2 += d[c] * d[c]; }
+            const double inv = 1.0 / std::sqrt(r2);
+            for (int c = 0; c < 3; ++c) s[c] += S.m[j] * d[c] * inv * inv * inv;
+        }
+        for (int c = 0; c < 3; ++c) a[3 * i + c] = S.G * s[c];
+    }
+}
+
+// lx += c * sum_i (da_i/dx_k)^T u_i, lm += c * sum_i (da_i/dm_k) . u_i, with
+// da_i/dx_j = G m_j T_ij (j != i), da_i/dx_i = -sum_j G m_j T_ij, da_i/dm_j = G d_ij / s^3.
+void vjp64(const Sim& S, const std::vector<double>& x, const std::vector<double>& u, double c,
+           std::vector<double>& lx, std::vector<double>& lm) {
+    const std::size_t n = S.n;
+    const double e2 = S.eps * S.eps;
+#pragma omp parallel for schedule(static)
+    for (long k = 0; k < long(n); ++k) {
+        double w[3] = {0, 0, 0}, gm = 0;
+        for (std::size_t i = 0; i < n; ++i) {
+            if (long(i) == k) continue;
+            double d[3], r2 = e2;  // d = x_k - x_i (from target i to source k)
+            for (int q = 0; q < 3; ++q) { d[q] = x[3 * k + q] - x[3 * i + q]; r2 += d[q] * d[q]; }
+            const double inv = 1.0 / std::sqrt(r2), s3 = inv * inv * inv, s5 = s3 * inv * inv;
+            double T[3][3];
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) T[a][b] = (a == b ? s3 : 0.0) - 3 * s5 * d[a] * d[b];
+            // i != k: da_i/dx_k = G m_k T_ik applied to u_i; self block of a_k: -G m_i T_ki applied to u_k.
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) w[a] += S.G * (S.m[k] * T[a][b] * u[3 * i + b] - S.m[i] * T[a][b] * u[3 * k + b]);
+            for (int q = 0; q < 3; ++q) gm += S.G * d[q] * s3 * u[3 * i + q];
+        }
+        for (int q = 0; q < 3; ++q) lx[3 * k + q] += c * w[q];
+        lm[k] += c * gm;
+    }
+}
+}  // namespace
+
+void orc_sim_forward(uint64_t n, const double* x0, const double* v0, const double* m, double G, double eps, double dt,
+                     int steps, double* xT, double* vT) {
+    Sim& S = g_sim;
+    S.n = n; S.steps = steps; S.G = G; S.eps = eps; S.dt = dt;
+    S.m.assign(m, m + n);
+    std::vector<double> x(x0, x0 + 3 * n), v(v0, v0 + 3 * n), a;
+    S.pos.assign(1, x);
+    accel64(S, x, a);
+    for (int s = 0; s < steps; ++s) {
+        for (std::size_t k = 0; k < 3 * n; ++k) v[k] += 0.5 * dt * a[k];
+        for (std::size_t k = 0; k < 3 * n; ++k) x[k] += dt * v[k];
+        accel64(S, x, a);
+        for (std::size_t k = 0; k < 3 * n; ++k) v[k] += 0.5 * dt * a[k];
+        S.pos.push_back(x);
+    }
+    std::memcpy(xT, x.data(), 3 * n * 8);
+    std::memcpy(vT, v.data(), 3 * n * 8);
+}
+
+void orc_sim_backward(const double* gxT, const double* gvT, double* gx0, double* gv0, double* gm) {
+    Sim& S = g_sim;
+    const std::size_t n = S.n;
+    std::vector<double> lx(gxT, gxT + 3 * n), lv(gvT, gvT + 3 * n), lm(n, 0.0);
+    for (int s = S.steps - 1; s >= 0; --s) {
+        vjp64(S, S.pos[s + 1], lv, 0.5 * S.dt, lx, lm);                 // second half kick
+        for (std::size_t k = 0; k < 3 * n; ++k) lv[k] += S.dt * lx[k];  // drift
+        vjp64(S, S.pos[s], lv, 0.5 * S.dt, lx, lm);                     // first half kick
+    }
+    std::memcpy(gx0, lx.data(), 3 * n * 8);
+    std::memcpy(gv0, lv.data(), 3 * n * 8);
+    std::memcpy(gm, lm.data(), n * 8);
 }
 
 void* orc_create(int threads) {
