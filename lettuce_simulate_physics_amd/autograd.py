@@ -62,7 +62,7 @@ class Simulation:
         return xT, vT
 
     def backward(self, gxT, gvT):
-        n = self.n
+        n = getattr(self, "n", None) or int(np.asarray(gxT).reshape(-1, 3).shape[0])
         gxT, gvT = _f64(gxT, (n, 3)), _f64(gvT, (n, 3))
         gx0, gv0, gm = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
         self.be._check(library().lhip_sim_backward(self.be._h, gxT.ctypes.data_as(_P), gvT.ctypes.data_as(_P),

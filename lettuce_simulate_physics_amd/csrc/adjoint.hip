@@ -8,8 +8,10 @@
 // term); a drift x' = x + dt v adds dt lambda_x to lambda_v. The two half-kicks at an interior
 // position x_n therefore see the same lambda_v and merge into one VJP of weight dt:
 //   VJP(x_T, dt/2); for n = T-1 .. 0 { lambda_v += dt lambda_x; VJP(x_n, n > 0 ? dt : dt/2) }.
-// Positions x_n come back segment by segment: checkpoints (x, v) every K steps, and within a segment
-// a forward recompute that stores the K + 1 positions. Cost: one force + one VJP per step.
+// Positions x_n come back segment by segment: checkpoints (x, v, a) every K steps, and within a
+// segment [s, e) a forward recompute from the checkpoint that stores x_s .. x_e; it needs the forces
+// at x_{s+1} .. x_{e-1} only (a(x_s) is in the checkpoint, a(x_e) is never used), so the sweep costs
+// (K - 1) / K forces and one VJP per step.
 #include "adjoint.hpp"
 #include "context.hpp"
 
@@ -19,9 +21,9 @@ struct AdjSim {
     int n = 0;
     lhip_sim_params p{};
     int nck = 0;
-    DevBuf x, y, z, vx, vy, vz, m, ax, ay, az;
+    DevBuf x, y, z, vx, vy, vz, m, ax, ay, az, rec, vrec;
     DevBuf ckpt, seg;
-    DevBuf lx, ly, lz, lvx, lvy, lvz, lm, ux, uy, uz, wx, wy, wz, gmv, part;
+    DevBuf lx, ly, lz, lvx, lvy, lvz, lm, part;
     hipEvent_t ev[4] = {};
     bool have_forward = false;
     double fwd_ms = 0, bwd_ms = 0;
@@ -35,55 +37,45 @@ struct AdjSim {
     }
 
     double* d(DevBuf& b) { return b.as<double>(); }
+    SimRec* r(DevBuf& b) { return b.as<SimRec>(); }
 
     void reserve(int nn) {
         const std::size_t f = std::size_t(nn) * 8;
-        for (DevBuf* b : {&x, &y, &z, &vx, &vy, &vz, &m, &ax, &ay, &az, &lx, &ly, &lz, &lvx, &lvy, &lvz, &lm, &ux, &uy,
-                          &uz, &wx, &wy, &wz, &gmv})
+        for (DevBuf* b : {&x, &y, &z, &vx, &vy, &vz, &m, &ax, &ay, &az, &lx, &ly, &lz, &lvx, &lvy, &lvz, &lm})
             b->reserve(f);
+        rec.reserve(std::size_t(nn) * sizeof(SimRec));
+        vrec.reserve(std::size_t(nn) * sizeof(SimRec));
         part.reserve(std::size_t(adj_split(nn)) * 4 * f);
     }
 
-    void force(hipStream_t s, const double* px, const double* py, const double* pz) {
-        launch_force64(s, px, py, pz, d(m), n, p.eps, p.G, p.precision == LHIP_PRECISION_F64, d(ax), d(ay), d(az),
-                       d(part));
+    bool f64() const { return p.precision == LHIP_PRECISION_F64; }
+
+    // a = force at the positions in rec; v += h a.
+    void force_kick(hipStream_t s, double h) {
+        launch_force_finish(s, r(rec), d(m), n, p.eps, p.G, f64(), h, d(ax), d(ay), d(az), d(vx), d(vy), d(vz),
+                            d(part));
     }
 
-    void step(hipStream_t s) {
+    // One KDK step from (x, v, a(x)); copy (may be null) receives the new positions. With
+    // need_force false the closing kick is skipped (recompute of a segment's last position).
+    void step(hipStream_t s, double* copy, bool need_force = true) {
         const double h = 0.5 * p.dt;
-        launch_kick64(s, d(vx), d(vy), d(vz), d(ax), d(ay), d(az), h, n);
-        launch_axpy3(s, d(x), d(y), d(z), d(vx), d(vy), d(vz), p.dt, n);
-        force(s, d(x), d(y), d(z));
-        launch_kick64(s, d(vx), d(vy), d(vz), d(ax), d(ay), d(az), h, n);
+        launch_kick_drift(s, d(x), d(y), d(z), d(vx), d(vy), d(vz), d(ax), d(ay), d(az), d(m), h, p.dt, r(rec), copy,
+                          n);
+        if (need_force) force_kick(s, h);
     }
 
-    void save_state(hipStream_t s, double* dst) {
+    static constexpr int kCkArrays = 9;  // x, y, z, vx, vy, vz, ax, ay, az
+    void copy_state(hipStream_t s, double* ck, bool save) {
         const std::size_t f = std::size_t(n) * 8;
-        DevBuf* src[6] = {&x, &y, &z, &vx, &vy, &vz};
-        for (int c = 0; c < 6; ++c)
-            LHIP_CHECK(hipMemcpyAsync(dst + std::size_t(c) * n, src[c]->as<void>(), f, hipMemcpyDeviceToDevice, s));
+        DevBuf* a[kCkArrays] = {&x, &y, &z, &vx, &vy, &vz, &ax, &ay, &az};
+        for (int c = 0; c < kCkArrays; ++c) {
+            void* dev = a[c]->as<void>();
+            void* slot = ck + std::size_t(c) * n;
+            LHIP_CHECK(hipMemcpyAsync(save ? slot : dev, save ? dev : slot, f, hipMemcpyDeviceToDevice, s));
+        }
     }
-    void load_state(hipStream_t s, const double* srcp) {
-        const std::size_t f = std::size_t(n) * 8;
-        DevBuf* dst[6] = {&x, &y, &z, &vx, &vy, &vz};
-        for (int c = 0; c < 6; ++c)
-            LHIP_CHECK(hipMemcpyAsync(dst[c]->as<void>(), srcp + std::size_t(c) * n, f, hipMemcpyDeviceToDevice, s));
-    }
-    void save_pos(hipStream_t s, double* dst) {
-        const std::size_t f = std::size_t(n) * 8;
-        DevBuf* src[3] = {&x, &y, &z};
-        for (int c = 0; c < 3; ++c)
-            LHIP_CHECK(hipMemcpyAsync(dst + std::size_t(c) * n, src[c]->as<void>(), f, hipMemcpyDeviceToDevice, s));
-    }
-
-    // One VJP at positions pos (3 SoA arrays) with cotangent weight c on lambda_v.
-    void vjp_at(hipStream_t s, const double* pos, double c) {
-        launch_scale3(s, d(ux), d(uy), d(uz), d(lvx), d(lvy), d(lvz), c, n);
-        launch_vjp_direct(s, pos, pos + n, pos + 2 * std::size_t(n), d(m), d(ux), d(uy), d(uz), n, p.eps, p.G, d(wx),
-                          d(wy), d(wz), d(gmv), d(part));
-        launch_axpy3(s, d(lx), d(ly), d(lz), d(wx), d(wy), d(wz), 1.0, n);
-        launch_axpy1(s, d(lm), d(gmv), 1.0, n);
-    }
+    double* ck_at(int k) { return ckpt.as<double>() + std::size_t(k) * kCkArrays * n; }
 };
 
 void adj_destroy(AdjSim* a) { delete a; }
@@ -139,7 +131,7 @@ int lhip_sim_forward(lhip_ctx* ctx, const lhip_sim_params* p, uint64_t n, const 
         A.reserve(A.n);
         const int T = p->steps, K = p->checkpoint_every;
         A.nck = T > 0 ? (T + K - 1) / K : 1;
-        A.ckpt.reserve(std::size_t(A.nck) * 6 * n * 8);
+        A.ckpt.reserve(std::size_t(A.nck) * AdjSim::kCkArrays * n * 8);
         A.seg.reserve(std::size_t(K + 1) * 3 * n * 8);
         hipStream_t s = c.stream;
         std::vector<double> soa;
@@ -151,11 +143,12 @@ int lhip_sim_forward(lhip_ctx* ctx, const lhip_sim_params* p, uint64_t n, const 
         LHIP_CHECK(hipMemcpyAsync(A.m.as<void>(), m, n * 8, hipMemcpyHostToDevice, s));
         c.sync();
         LHIP_CHECK(hipEventRecord(A.ev[0], s));
-        A.save_state(s, A.ckpt.as<double>());
-        A.force(s, A.d(A.x), A.d(A.y), A.d(A.z));
+        launch_pack(s, A.d(A.x), A.d(A.y), A.d(A.z), A.d(A.m), A.r(A.rec), A.n);
+        A.force_kick(s, 0.0);
+        A.copy_state(s, A.ck_at(0), true);
         for (int st = 0; st < T; ++st) {
-            A.step(s);
-            if ((st + 1) % K == 0 && st + 1 < T) A.save_state(s, A.ckpt.as<double>() + std::size_t((st + 1) / K) * 6 * n);
+            A.step(s, nullptr);
+            if ((st + 1) % K == 0 && st + 1 < T) A.copy_state(s, A.ck_at((st + 1) / K), true);
         }
         LHIP_CHECK(hipEventRecord(A.ev[1], s));
         if (xT) {
@@ -197,20 +190,24 @@ int lhip_sim_backward(lhip_ctx* ctx, const double* gxT, const double* gvT, doubl
         LHIP_CHECK(hipEventRecord(A.ev[2], s));
         double* seg = A.seg.as<double>();
         const std::size_t stride = 3 * std::size_t(n);
+        auto weight = [&](int q) { return (q == T || q == 0) ? 0.5 * dt : dt; };
         for (int sg = A.nck - 1; sg >= 0 && T > 0; --sg) {
             const int start = sg * K, end = std::min(T, start + K);
-            // Recompute the segment's positions from its checkpoint.
-            A.load_state(s, A.ckpt.as<double>() + std::size_t(sg) * 6 * n);
-            A.save_pos(s, seg);
-            A.force(s, A.d(A.x), A.d(A.y), A.d(A.z));
-            for (int t = start; t < end; ++t) {
-                A.step(s);
-                A.save_pos(s, seg + std::size_t(t + 1 - start) * stride);
-            }
-            for (int st = end - 1; st >= start; --st) {
-                if (st == T - 1) A.vjp_at(s, seg + std::size_t(end - start) * stride, 0.5 * dt);
-                launch_axpy3(s, A.d(A.lvx), A.d(A.lvy), A.d(A.lvz), A.d(A.lx), A.d(A.ly), A.d(A.lz), dt, n);
-                A.vjp_at(s, seg + std::size_t(st - start) * stride, st > 0 ? dt : 0.5 * dt);
+            // Recompute x_start .. x_end into seg from the checkpoint (x, v, a at x_start).
+            A.copy_state(s, A.ck_at(sg), false);
+            for (int cc = 0; cc < 3; ++cc)
+                LHIP_CHECK(hipMemcpyAsync(seg + std::size_t(cc) * n, A.ck_at(sg) + std::size_t(cc) * n,
+                                          std::size_t(n) * 8, hipMemcpyDeviceToDevice, s));
+            for (int t = start; t < end; ++t) A.step(s, seg + std::size_t(t + 1 - start) * stride, t + 1 < end);
+            // VJPs at x_q for q = first .. start (x_end belongs to this segment only when end == T).
+            const int first = end == T ? end : end - 1;
+            launch_pack_vjp(s, seg + std::size_t(first - start) * stride, A.d(A.m), A.d(A.lvx), A.d(A.lvy),
+                            A.d(A.lvz), weight(first), A.r(A.vrec), n);
+            for (int q = first; q >= start; --q) {
+                const double* npos = q > start ? seg + std::size_t(q - 1 - start) * stride : nullptr;
+                launch_vjp_update(s, A.r(A.vrec), n, A.p.eps, A.p.G, A.d(A.part), A.d(A.lx), A.d(A.ly), A.d(A.lz),
+                                  A.d(A.lm), A.d(A.lvx), A.d(A.lvy), A.d(A.lvz), q > 0 ? dt : 0.0, npos, A.d(A.m),
+                                  q > 0 ? weight(q - 1) : 0.0, A.r(A.vrec));
             }
         }
         LHIP_CHECK(hipEventRecord(A.ev[3], s));

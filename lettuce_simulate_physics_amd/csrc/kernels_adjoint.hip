@@ -1,53 +1,151 @@
-// Kernels of the differentiable FP64-state simulation: FP64 kick/drift, the mixed-precision direct
-// force, an all-FP64 direct force (the finite-difference reference), and the direct-sum
-// vector-Jacobian product.
+// Kernels of the differentiable FP64-state simulation: fused KDK pieces, the mixed-precision direct
+// force, an all-FP64 direct force (the finite-difference reference), the direct-sum vector-Jacobian
+// product and the fused adjoint update.
 //
-// Mixed precision. Positions are FP64. A pair's separation d = x_j - x_i is formed in FP64 and
-// rounded to FP32 (relative coordinates: exact to FP32 precision however far the pair is from the
-// origin); the pair arithmetic is FP32; each lane's FP32 partial sums are folded into FP64 totals
-// every 64 sources, which bounds the FP32 accumulation error by 64 terms.
+// Sources are read from SimRec records (FP64 position, FP32 mass, FP32 cotangent), one 48-byte
+// record per body, rewritten by the element-wise kernels each step. A wave loads the record of
+// source j with scalar loads (j is wave-uniform), so the pair loops spend no vector instructions
+// on source data.
+//
+// Mixed precision. A pair's separation d = x_j - x_i is formed in FP64 and rounded to FP32 (relative
+// coordinates: exact to FP32 precision however far the pair is from the origin); the pair
+// arithmetic is FP32; each lane's FP32 partial sums are folded into FP64 totals every 64 sources,
+// which bounds the FP32 accumulation error by 64 terms.
+//
+// Self pairs. With eps > 0 the self pair has d = 0 exactly and adds exactly zero to the force, to
+// Q and to the mass cotangent below, so the force loop has no mask at all; only the VJP's P term
+// (weight s^-3 on m_k u_j - m_j u_k, a rounding residue when j = k) is masked, in the one block of
+// 64 sources that holds the wave's own bodies. eps = 0 takes a guarded variant that skips j = i and
+// coincident pairs everywhere.
 //
 // VJP. For a_i = G sum_{j != i} m_j d_ij / s_ij^3 (d_ij = x_j - x_i, s^2 = |d|^2 + eps^2) and a
 // cotangent u_i on each a_i, the cotangent on positions is
 //   w_k = G sum_{j != k} T_kj (m_k u_j - m_j u_k),   T = s^-3 I - 3 s^-5 d d^T  (tidal tensor),
-// and on masses g_k = -G sum_{j != k} s_kj^-3 (u_j . d_kj). One pass over pairs, no transposes.
+// accumulated as P = sum s^-3 v and Q = sum s^-5 (d.v) d with v = m_k u_j - m_j u_k, w = G (P - 3Q);
+// on masses g_k = -G sum_{j != k} s_kj^-3 (u_j . d_kj). One pass over pairs, no transposes.
 #include "adjoint.hpp"
 
 namespace lhip {
 namespace {
 
 constexpr int kAdjBS = 256;
+constexpr int kWaves = kAdjBS / 64;
 constexpr int kFold = 64;  // FP32 partial sums are folded into FP64 every kFold sources
+constexpr int kT = 2;      // targets per lane
 
-__global__ void __launch_bounds__(kBS) kick64_kernel(double* __restrict__ vx, double* __restrict__ vy,
-                                                     double* __restrict__ vz, const double* __restrict__ ax,
-                                                     const double* __restrict__ ay, const double* __restrict__ az,
-                                                     double dt, int n) {
+__device__ __forceinline__ int lane_id() { return int(threadIdx.x) & 63; }
+
+// ------------------------------------------------------------------ element-wise kernels
+
+// rec <- (x, m) for the force at the current positions.
+__global__ void __launch_bounds__(kBS) pack_kernel(const double* __restrict__ x, const double* __restrict__ y,
+                                                   const double* __restrict__ z, const double* __restrict__ m,
+                                                   SimRec* __restrict__ rec, int n) {
     const int i = blockIdx.x * kBS + threadIdx.x;
     if (i < n) {
-        vx[i] = fma(ax[i], dt, vx[i]);
-        vy[i] = fma(ay[i], dt, vy[i]);
-        vz[i] = fma(az[i], dt, vz[i]);
+        SimRec r;
+        r.x = x[i]; r.y = y[i]; r.z = z[i];
+        r.m = float(m[i]); r.ux = r.uy = r.uz = 0.f; r.pad[0] = r.pad[1] = 0.f;
+        rec[i] = r;
     }
 }
 
-// y += c * x for three arrays (drift: x += dt v; cotangent updates).
-__global__ void __launch_bounds__(kBS) axpy3_kernel(double* __restrict__ y0, double* __restrict__ y1,
-                                                    double* __restrict__ y2, const double* __restrict__ x0,
-                                                    const double* __restrict__ x1, const double* __restrict__ x2,
-                                                    double c, int n) {
+// Opening half of a KDK step: v += h a; x += dt v; rec <- (x, m); optionally a copy of x.
+__global__ void __launch_bounds__(kBS) kick_drift_kernel(double* __restrict__ x, double* __restrict__ y,
+                                                         double* __restrict__ z, double* __restrict__ vx,
+                                                         double* __restrict__ vy, double* __restrict__ vz,
+                                                         const double* __restrict__ ax, const double* __restrict__ ay,
+                                                         const double* __restrict__ az, const double* __restrict__ m,
+                                                         double h, double dt, SimRec* __restrict__ rec,
+                                                         double* __restrict__ copy, int n) {
     const int i = blockIdx.x * kBS + threadIdx.x;
-    if (i < n) {
-        y0[i] = fma(x0[i], c, y0[i]);
-        y1[i] = fma(x1[i], c, y1[i]);
-        y2[i] = fma(x2[i], c, y2[i]);
+    if (i >= n) return;
+    const double wx = fma(ax[i], h, vx[i]), wy = fma(ay[i], h, vy[i]), wz = fma(az[i], h, vz[i]);
+    vx[i] = wx; vy[i] = wy; vz[i] = wz;
+    const double px = fma(wx, dt, x[i]), py = fma(wy, dt, y[i]), pz = fma(wz, dt, z[i]);
+    x[i] = px; y[i] = py; z[i] = pz;
+    SimRec r;
+    r.x = px; r.y = py; r.z = pz;
+    r.m = float(m[i]); r.ux = r.uy = r.uz = 0.f; r.pad[0] = r.pad[1] = 0.f;
+    rec[i] = r;
+    if (copy) {
+        copy[i] = px;
+        copy[std::size_t(n) + i] = py;
+        copy[2 * std::size_t(n) + i] = pz;
     }
 }
 
-__global__ void __launch_bounds__(kBS) axpy1_kernel(double* __restrict__ y, const double* __restrict__ x, double c,
-                                                    int n) {
+// Closing half of a KDK step: a = G * (sum of the S split partials, in split order); v += h a.
+__global__ void __launch_bounds__(kBS) finish_kernel(const double* __restrict__ part, int n, int S, double G, double h,
+                                                     double* __restrict__ ax, double* __restrict__ ay,
+                                                     double* __restrict__ az, double* __restrict__ vx,
+                                                     double* __restrict__ vy, double* __restrict__ vz) {
     const int i = blockIdx.x * kBS + threadIdx.x;
-    if (i < n) y[i] = fma(x[i], c, y[i]);
+    if (i >= n) return;
+    double sx = 0, sy = 0, sz = 0;
+    for (int s = 0; s < S; ++s) {
+        sx += part[std::size_t(3 * s + 0) * n + i];
+        sy += part[std::size_t(3 * s + 1) * n + i];
+        sz += part[std::size_t(3 * s + 2) * n + i];
+    }
+    const double a0 = G * sx, a1 = G * sy, a2 = G * sz;
+    ax[i] = a0; ay[i] = a1; az[i] = a2;
+    if (h != 0.0) {
+        vx[i] = fma(a0, h, vx[i]);
+        vy[i] = fma(a1, h, vy[i]);
+        vz[i] = fma(a2, h, vz[i]);
+    }
+}
+
+// rec <- (pos, m, c * lambda_v): the sources of a VJP.
+__global__ void __launch_bounds__(kBS) pack_vjp_kernel(const double* __restrict__ pos, const double* __restrict__ m,
+                                                       const double* __restrict__ lvx, const double* __restrict__ lvy,
+                                                       const double* __restrict__ lvz, double c,
+                                                       SimRec* __restrict__ rec, int n) {
+    const int i = blockIdx.x * kBS + threadIdx.x;
+    if (i >= n) return;
+    SimRec r;
+    r.x = pos[i]; r.y = pos[std::size_t(n) + i]; r.z = pos[2 * std::size_t(n) + i];
+    r.m = float(m[i]);
+    r.ux = float(c * lvx[i]); r.uy = float(c * lvy[i]); r.uz = float(c * lvz[i]);
+    r.pad[0] = r.pad[1] = 0.f;
+    rec[i] = r;
+}
+
+// After a VJP: lambda_x += G w, lambda_m += G g (sums of the S split partials, in split order); then,
+// if drift != 0, the drift's adjoint lambda_v += drift * lambda_x; then, if npos, the sources of the
+// next VJP: rec <- (npos, m, c_next * lambda_v).
+__global__ void __launch_bounds__(kBS) adj_update_kernel(const double* __restrict__ part, int n, int S, double G,
+                                                         double* __restrict__ lx, double* __restrict__ ly,
+                                                         double* __restrict__ lz, double* __restrict__ lm,
+                                                         double* __restrict__ lvx, double* __restrict__ lvy,
+                                                         double* __restrict__ lvz, double drift,
+                                                         const double* __restrict__ npos,
+                                                         const double* __restrict__ m, double c_next,
+                                                         SimRec* __restrict__ rec) {
+    const int i = blockIdx.x * kBS + threadIdx.x;
+    if (i >= n) return;
+    double s[4] = {0, 0, 0, 0};
+    for (int sp = 0; sp < S; ++sp)
+        for (int c = 0; c < 4; ++c) s[c] += part[std::size_t(4 * sp + c) * n + i];
+    const double gx = lx[i] + G * s[0], gy = ly[i] + G * s[1], gz = lz[i] + G * s[2];
+    lx[i] = gx; ly[i] = gy; lz[i] = gz;
+    lm[i] += G * s[3];
+    double vx = lvx[i], vy = lvy[i], vz = lvz[i];
+    if (drift != 0.0) {
+        vx = fma(gx, drift, vx);
+        vy = fma(gy, drift, vy);
+        vz = fma(gz, drift, vz);
+        lvx[i] = vx; lvy[i] = vy; lvz[i] = vz;
+    }
+    if (npos) {
+        SimRec r;
+        r.x = npos[i]; r.y = npos[std::size_t(n) + i]; r.z = npos[2 * std::size_t(n) + i];
+        r.m = float(m[i]);
+        r.ux = float(c_next * vx); r.uy = float(c_next * vy); r.uz = float(c_next * vz);
+        r.pad[0] = r.pad[1] = 0.f;
+        rec[i] = r;
+    }
 }
 
 __global__ void __launch_bounds__(kBS) fill64_kernel(double* __restrict__ y, double v, int n) {
@@ -55,56 +153,56 @@ __global__ void __launch_bounds__(kBS) fill64_kernel(double* __restrict__ y, dou
     if (i < n) y[i] = v;
 }
 
-// y = c * x for three arrays.
-__global__ void __launch_bounds__(kBS) scale3_kernel(double* __restrict__ y0, double* __restrict__ y1,
-                                                     double* __restrict__ y2, const double* __restrict__ x0,
-                                                     const double* __restrict__ x1, const double* __restrict__ x2,
-                                                     double c, int n) {
-    const int i = blockIdx.x * kBS + threadIdx.x;
-    if (i < n) {
-        y0[i] = c * x0[i];
-        y1[i] = c * x1[i];
-        y2[i] = c * x2[i];
+// ------------------------------------------------------------------ pair kernels
+//
+// Wave w of the grid owns targets [w * 64 T, (w + 1) * 64 T): target k of lane l is w * 64 T + 64 k + l.
+// blockIdx.y selects the source range [split * jchunk, ...), jchunk a multiple of 64, so every block
+// of 64 sources either holds all of the wave's own bodies for some k or none of them.
+
+template <int T, bool GUARD>
+__device__ __forceinline__ void force_block(const SimRec* __restrict__ rec, int jb, int je, float e2,
+                                            const double (&px)[T], const double (&py)[T], const double (&pz)[T],
+                                            const int (&ii)[T], float (&fx)[T], float (&fy)[T], float (&fz)[T]) {
+#pragma unroll 4
+    for (int j = jb; j < je; ++j) {
+        const double qx = rec[j].x, qy = rec[j].y, qz = rec[j].z;
+        const float qm = rec[j].m;
+#pragma unroll
+        for (int k = 0; k < T; ++k) {
+            const float dx = float(qx - px[k]), dy = float(qy - py[k]), dz = float(qz - pz[k]);
+            const float r2 = __fmaf_rn(dz, dz, __fmaf_rn(dy, dy, __fmaf_rn(dx, dx, e2)));
+            float inv = __builtin_amdgcn_rsqf(r2);
+            if (GUARD) inv = (j != ii[k] && r2 > 0.f) ? inv : 0.f;
+            const float f = (qm * inv) * (inv * inv);
+            fx[k] = __fmaf_rn(f, dx, fx[k]);
+            fy[k] = __fmaf_rn(f, dy, fy[k]);
+            fz[k] = __fmaf_rn(f, dz, fz[k]);
+        }
     }
 }
 
-// Mixed-precision direct force for targets [blockIdx.x * 256 * T ...), sources split S ways.
-template <int T, bool SPLIT>
-__global__ void __launch_bounds__(kAdjBS) force_mixed_kernel(const double* __restrict__ x, const double* __restrict__ y,
-                                                             const double* __restrict__ z, const double* __restrict__ m,
-                                                             int n, float e2, double G, int jchunk,
-                                                             double* __restrict__ ax, double* __restrict__ ay,
-                                                             double* __restrict__ az, double* __restrict__ part) {
-    const int base = blockIdx.x * (kAdjBS * T);
+// Mixed-precision direct force; writes the split's partial sums (without G).
+template <int T, bool GUARD>
+__global__ void __launch_bounds__(kAdjBS) force_mixed_kernel(const SimRec* __restrict__ rec, int n, float e2,
+                                                             int jchunk, double* __restrict__ part) {
+    const int lane = lane_id();
+    const int t0 = (blockIdx.x * kWaves + int(threadIdx.x >> 6)) * 64 * T;
+    if (t0 >= n) return;  // wave-uniform
     const int split = blockIdx.y;
     double px[T], py[T], pz[T], sx[T], sy[T], sz[T];
     float fx[T], fy[T], fz[T];
     int ii[T];
 #pragma unroll
     for (int k = 0; k < T; ++k) {
-        ii[k] = base + k * kAdjBS + int(threadIdx.x);
+        ii[k] = t0 + 64 * k + lane;
         const int c = ii[k] < n ? ii[k] : n - 1;
-        px[k] = x[c]; py[k] = y[c]; pz[k] = z[c];
+        px[k] = rec[c].x; py[k] = rec[c].y; pz[k] = rec[c].z;
         sx[k] = sy[k] = sz[k] = 0.0;
         fx[k] = fy[k] = fz[k] = 0.f;
     }
     const int j0 = split * jchunk, j1 = min(n, j0 + jchunk);
     for (int jb = j0; jb < j1; jb += kFold) {
-        const int je = min(j1, jb + kFold);
-        for (int j = jb; j < je; ++j) {
-            const double qx = x[j], qy = y[j], qz = z[j];
-            const float qm = float(m[j]);
-#pragma unroll
-            for (int k = 0; k < T; ++k) {
-                const float dx = float(qx - px[k]), dy = float(qy - py[k]), dz = float(qz - pz[k]);
-                const float r2 = __fmaf_rn(dz, dz, __fmaf_rn(dy, dy, __fmaf_rn(dx, dx, e2)));
-                const float inv = (j != ii[k] && r2 > 0.f) ? __builtin_amdgcn_rsqf(r2) : 0.f;
-                const float f = qm * inv * (inv * inv);
-                fx[k] = __fmaf_rn(f, dx, fx[k]);
-                fy[k] = __fmaf_rn(f, dy, fy[k]);
-                fz[k] = __fmaf_rn(f, dz, fz[k]);
-            }
-        }
+        force_block<T, GUARD>(rec, jb, min(j1, jb + kFold), e2, px, py, pz, ii, fx, fy, fz);
 #pragma unroll
         for (int k = 0; k < T; ++k) {
             sx[k] += fx[k]; sy[k] += fy[k]; sz[k] += fz[k];
@@ -115,33 +213,24 @@ __global__ void __launch_bounds__(kAdjBS) force_mixed_kernel(const double* __res
     for (int k = 0; k < T; ++k) {
         const int i = ii[k];
         if (i >= n) continue;
-        if (SPLIT) {
-            part[(long long)(3 * split + 0) * n + i] = sx[k];
-            part[(long long)(3 * split + 1) * n + i] = sy[k];
-            part[(long long)(3 * split + 2) * n + i] = sz[k];
-        } else {
-            ax[i] = G * sx[k];
-            ay[i] = G * sy[k];
-            az[i] = G * sz[k];
-        }
+        part[std::size_t(3 * split + 0) * n + i] = sx[k];
+        part[std::size_t(3 * split + 1) * n + i] = sy[k];
+        part[std::size_t(3 * split + 2) * n + i] = sz[k];
     }
 }
 
-// All-FP64 direct force (pair arithmetic in double, correctly rounded sqrt and division).
-template <bool SPLIT>
-__global__ void __launch_bounds__(kAdjBS) force_f64_kernel(const double* __restrict__ x, const double* __restrict__ y,
-                                                           const double* __restrict__ z, const double* __restrict__ m,
-                                                           int n, double e2, double G, int jchunk,
-                                                           double* __restrict__ ax, double* __restrict__ ay,
-                                                           double* __restrict__ az, double* __restrict__ part) {
+// All-FP64 direct force (pair arithmetic in double, correctly rounded sqrt and division; FP64 masses).
+__global__ void __launch_bounds__(kAdjBS) force_f64_kernel(const SimRec* __restrict__ rec,
+                                                           const double* __restrict__ m, int n, double e2, int jchunk,
+                                                           double* __restrict__ part) {
     const int i = blockIdx.x * kAdjBS + threadIdx.x;
     const int split = blockIdx.y;
     const int c = i < n ? i : n - 1;
-    const double px = x[c], py = y[c], pz = z[c];
+    const double px = rec[c].x, py = rec[c].y, pz = rec[c].z;
     double sx = 0, sy = 0, sz = 0;
     const int j0 = split * jchunk, j1 = min(n, j0 + jchunk);
     for (int j = j0; j < j1; ++j) {
-        const double dx = x[j] - px, dy = y[j] - py, dz = z[j] - pz;
+        const double dx = rec[j].x - px, dy = rec[j].y - py, dz = rec[j].z - pz;
         const double r2 = dx * dx + dy * dy + dz * dz + e2;
         if (j == i || !(r2 > 0)) continue;
         const double inv = 1.0 / sqrt(r2);
@@ -149,129 +238,176 @@ __global__ void __launch_bounds__(kAdjBS) force_f64_kernel(const double* __restr
         sx += f * dx; sy += f * dy; sz += f * dz;
     }
     if (i >= n) return;
-    if (SPLIT) {
-        part[(long long)(3 * split + 0) * n + i] = sx;
-        part[(long long)(3 * split + 1) * n + i] = sy;
-        part[(long long)(3 * split + 2) * n + i] = sz;
-    } else {
-        ax[i] = G * sx; ay[i] = G * sy; az[i] = G * sz;
+    part[std::size_t(3 * split + 0) * n + i] = sx;
+    part[std::size_t(3 * split + 1) * n + i] = sy;
+    part[std::size_t(3 * split + 2) * n + i] = sz;
+}
+
+template <int T, bool MASK, bool GUARD>
+__device__ __forceinline__ void vjp_block(const SimRec* __restrict__ rec, int jb, int je, float e2,
+                                          const double (&px)[T], const double (&py)[T], const double (&pz)[T],
+                                          const float (&mk)[T], const float (&ukx)[T], const float (&uky)[T],
+                                          const float (&ukz)[T], const int (&ii)[T], float (&Px)[T], float (&Py)[T],
+                                          float (&Pz)[T], float (&Qx)[T], float (&Qy)[T], float (&Qz)[T],
+                                          float (&Gm)[T]) {
+#pragma unroll 2
+    for (int j = jb; j < je; ++j) {
+        const double qx = rec[j].x, qy = rec[j].y, qz = rec[j].z;
+        const float qm = rec[j].m, ujx = rec[j].ux, ujy = rec[j].uy, ujz = rec[j].uz;
+#pragma unroll
+        for (int k = 0; k < T; ++k) {
+            const float dx = float(qx - px[k]), dy = float(qy - py[k]), dz = float(qz - pz[k]);
+            const float r2 = __fmaf_rn(dz, dz, __fmaf_rn(dy, dy, __fmaf_rn(dx, dx, e2)));
+            float inv = __builtin_amdgcn_rsqf(r2);
+            if (MASK) inv = (j != ii[k] && (!GUARD || r2 > 0.f)) ? inv : 0.f;
+            const float is2 = inv * inv, is3 = inv * is2, is5 = is3 * is2;
+            const float vx = __fmaf_rn(mk[k], ujx, -(qm * ukx[k]));
+            const float vy = __fmaf_rn(mk[k], ujy, -(qm * uky[k]));
+            const float vz = __fmaf_rn(mk[k], ujz, -(qm * ukz[k]));
+            const float c = is5 * __fmaf_rn(dz, vz, __fmaf_rn(dy, vy, dx * vx));
+            Px[k] = __fmaf_rn(is3, vx, Px[k]);
+            Py[k] = __fmaf_rn(is3, vy, Py[k]);
+            Pz[k] = __fmaf_rn(is3, vz, Pz[k]);
+            Qx[k] = __fmaf_rn(c, dx, Qx[k]);
+            Qy[k] = __fmaf_rn(c, dy, Qy[k]);
+            Qz[k] = __fmaf_rn(c, dz, Qz[k]);
+            Gm[k] = __fmaf_rn(is3, __fmaf_rn(dz, ujz, __fmaf_rn(dy, ujy, dx * ujx)), Gm[k]);
+        }
     }
 }
 
-// VJP of the direct force (see the file comment). u = cotangent on accelerations.
-template <int T, bool SPLIT>
-__global__ void __launch_bounds__(kAdjBS) vjp_direct_kernel(const double* __restrict__ x, const double* __restrict__ y,
-                                                            const double* __restrict__ z, const double* __restrict__ m,
-                                                            const double* __restrict__ ux, const double* __restrict__ uy,
-                                                            const double* __restrict__ uz, int n, float e2, double G,
-                                                            int jchunk, double* __restrict__ wx, double* __restrict__ wy,
-                                                            double* __restrict__ wz, double* __restrict__ gm,
-                                                            double* __restrict__ part) {
-    const int base = blockIdx.x * (kAdjBS * T);
+// VJP of the direct force (see the file comment); writes the split's partial sums (without G):
+// components w_x, w_y, w_z, g.
+template <int T, bool GUARD>
+__global__ void __launch_bounds__(kAdjBS) vjp_direct_kernel(const SimRec* __restrict__ rec, int n, float e2,
+                                                            int jchunk, double* __restrict__ part) {
+    const int lane = lane_id();
+    const int t0 = (blockIdx.x * kWaves + int(threadIdx.x >> 6)) * 64 * T;
+    if (t0 >= n) return;  // wave-uniform
     const int split = blockIdx.y;
     double px[T], py[T], pz[T], sx[T], sy[T], sz[T], sm[T];
-    float mk[T], ukx[T], uky[T], ukz[T], fx[T], fy[T], fz[T], fm[T];
+    float mk[T], ukx[T], uky[T], ukz[T], Px[T], Py[T], Pz[T], Qx[T], Qy[T], Qz[T], Gm[T];
     int ii[T];
 #pragma unroll
     for (int k = 0; k < T; ++k) {
-        ii[k] = base + k * kAdjBS + int(threadIdx.x);
+        ii[k] = t0 + 64 * k + lane;
         const int c = ii[k] < n ? ii[k] : n - 1;
-        px[k] = x[c]; py[k] = y[c]; pz[k] = z[c];
-        mk[k] = float(m[c]);
-        ukx[k] = float(ux[c]); uky[k] = float(uy[c]); ukz[k] = float(uz[c]);
+        px[k] = rec[c].x; py[k] = rec[c].y; pz[k] = rec[c].z;
+        mk[k] = rec[c].m; ukx[k] = rec[c].ux; uky[k] = rec[c].uy; ukz[k] = rec[c].uz;
         sx[k] = sy[k] = sz[k] = sm[k] = 0.0;
-        fx[k] = fy[k] = fz[k] = fm[k] = 0.f;
+        Px[k] = Py[k] = Pz[k] = Qx[k] = Qy[k] = Qz[k] = Gm[k] = 0.f;
     }
     const int j0 = split * jchunk, j1 = min(n, j0 + jchunk);
+    const int t1 = t0 + 64 * T;
     for (int jb = j0; jb < j1; jb += kFold) {
         const int je = min(j1, jb + kFold);
-        for (int j = jb; j < je; ++j) {
-            const double qx = x[j], qy = y[j], qz = z[j];
-            const float qm = float(m[j]);
-            const float ujx = float(ux[j]), ujy = float(uy[j]), ujz = float(uz[j]);
-#pragma unroll
-            for (int k = 0; k < T; ++k) {
-                const float dx = float(qx - px[k]), dy = float(qy - py[k]), dz = float(qz - pz[k]);
-                const float r2 = __fmaf_rn(dz, dz, __fmaf_rn(dy, dy, __fmaf_rn(dx, dx, e2)));
-                const float inv = (j != ii[k] && r2 > 0.f) ? __builtin_amdgcn_rsqf(r2) : 0.f;
-                const float is2 = inv * inv, is3 = inv * is2, is5 = is3 * is2;
-                const float vx = mk[k] * ujx - qm * ukx[k];
-                const float vy = mk[k] * ujy - qm * uky[k];
-                const float vz = mk[k] * ujz - qm * ukz[k];
-                const float dv = dx * vx + dy * vy + dz * vz;
-                const float c3 = 3.f * is5 * dv;
-                fx[k] += is3 * vx - c3 * dx;
-                fy[k] += is3 * vy - c3 * dy;
-                fz[k] += is3 * vz - c3 * dz;
-                fm[k] -= is3 * (ujx * dx + ujy * dy + ujz * dz);
-            }
-        }
+        if (GUARD || (jb < t1 && je > t0))
+            vjp_block<T, true, GUARD>(rec, jb, je, e2, px, py, pz, mk, ukx, uky, ukz, ii, Px, Py, Pz, Qx, Qy, Qz, Gm);
+        else
+            vjp_block<T, false, false>(rec, jb, je, e2, px, py, pz, mk, ukx, uky, ukz, ii, Px, Py, Pz, Qx, Qy, Qz,
+                                       Gm);
 #pragma unroll
         for (int k = 0; k < T; ++k) {
-            sx[k] += fx[k]; sy[k] += fy[k]; sz[k] += fz[k]; sm[k] += fm[k];
-            fx[k] = fy[k] = fz[k] = fm[k] = 0.f;
+            sx[k] = fma(-3.0, double(Qx[k]), sx[k] + double(Px[k]));
+            sy[k] = fma(-3.0, double(Qy[k]), sy[k] + double(Py[k]));
+            sz[k] = fma(-3.0, double(Qz[k]), sz[k] + double(Pz[k]));
+            sm[k] -= double(Gm[k]);
+            Px[k] = Py[k] = Pz[k] = Qx[k] = Qy[k] = Qz[k] = Gm[k] = 0.f;
         }
     }
 #pragma unroll
     for (int k = 0; k < T; ++k) {
         const int i = ii[k];
         if (i >= n) continue;
-        if (SPLIT) {
-            part[(long long)(4 * split + 0) * n + i] = sx[k];
-            part[(long long)(4 * split + 1) * n + i] = sy[k];
-            part[(long long)(4 * split + 2) * n + i] = sz[k];
-            part[(long long)(4 * split + 3) * n + i] = sm[k];
-        } else {
-            wx[i] = G * sx[k]; wy[i] = G * sy[k]; wz[i] = G * sz[k]; gm[i] = G * sm[k];
-        }
+        part[std::size_t(4 * split + 0) * n + i] = sx[k];
+        part[std::size_t(4 * split + 1) * n + i] = sy[k];
+        part[std::size_t(4 * split + 2) * n + i] = sz[k];
+        part[std::size_t(4 * split + 3) * n + i] = sm[k];
     }
 }
 
-// Adds the S split partials of `comps` components in split order and applies G.
-__global__ void __launch_bounds__(kBS) reduce_split_kernel(const double* __restrict__ part, int n, int S, int comps,
-                                                           double G, double* __restrict__ o0, double* __restrict__ o1,
-                                                           double* __restrict__ o2, double* __restrict__ o3) {
-    const int i = blockIdx.x * kBS + threadIdx.x;
-    if (i >= n) return;
-    double acc[4] = {0, 0, 0, 0};
-    for (int s = 0; s < S; ++s)
-        for (int c = 0; c < comps; ++c) acc[c] += part[(long long)(comps * s + c) * n + i];
-    o0[i] = G * acc[0];
-    o1[i] = G * acc[1];
-    o2[i] = G * acc[2];
-    if (comps == 4) o3[i] = G * acc[3];
+// Source split: enough waves to give every SIMD of the chip about eight, at least 256 sources per
+// split, at most 64 splits; the chunk is a multiple of 64 sources.
+struct Split {
+    int S, jchunk;
+};
+
+Split split_for(int n, int per_lane) {
+    const long long waves = (static_cast<long long>(n) + 64 * per_lane - 1) / (64 * per_lane);
+    long long S = (8192 + waves - 1) / waves;
+    S = std::min<long long>(S, std::max(1, n / 256));
+    S = std::max<long long>(1, std::min<long long>(S, 64));
+    long long chunk = (n + S - 1) / S;
+    chunk = (chunk + 63) / 64 * 64;
+    S = (n + chunk - 1) / chunk;
+    return {int(S), int(chunk)};
 }
 
-constexpr int kT = 2;
-
-int split_for(int n, int per_lane) {
+int wave_blocks(int n, int per_lane) {
     const long long waves = (static_cast<long long>(n) + 64 * per_lane - 1) / (64 * per_lane);
-    long long S = (4096 + waves - 1) / waves;
-    S = std::min<long long>(S, std::max(1, n / 256));
-    return int(std::max<long long>(1, std::min<long long>(S, 64)));
+    return int((waves + kWaves - 1) / kWaves);
 }
 
 }  // namespace
 
-int adj_split(int n) { return std::max(split_for(n, kT), split_for(n, 1)); }
+int adj_split(int n) { return std::max(split_for(n, kT).S, split_for(n, 1).S); }
 
-void launch_kick64(hipStream_t s, double* vx, double* vy, double* vz, const double* ax, const double* ay,
-                   const double* az, double dt, int n) {
+void launch_pack(hipStream_t s, const double* x, const double* y, const double* z, const double* m, SimRec* rec,
+                 int n) {
     if (n <= 0) return;
-    kick64_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(vx, vy, vz, ax, ay, az, dt, n);
+    pack_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(x, y, z, m, rec, n);
     LHIP_LAUNCH_CHECK();
 }
 
-void launch_axpy3(hipStream_t s, double* y0, double* y1, double* y2, const double* x0, const double* x1,
-                  const double* x2, double c, int n) {
+void launch_kick_drift(hipStream_t s, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                       const double* ax, const double* ay, const double* az, const double* m, double h, double dt,
+                       SimRec* rec, double* copy, int n) {
     if (n <= 0) return;
-    axpy3_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(y0, y1, y2, x0, x1, x2, c, n);
+    kick_drift_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(x, y, z, vx, vy, vz, ax, ay, az, m, h, dt, rec, copy, n);
     LHIP_LAUNCH_CHECK();
 }
 
-void launch_axpy1(hipStream_t s, double* y, const double* x, double c, int n) {
+void launch_force_finish(hipStream_t s, const SimRec* rec, const double* m, int n, double eps, double G, bool pure_f64,
+                         double h, double* ax, double* ay, double* az, double* vx, double* vy, double* vz,
+                         double* part) {
     if (n <= 0) return;
-    axpy1_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(y, x, c, n);
+    int S;
+    if (pure_f64) {
+        const Split sp = split_for(n, 1);
+        S = sp.S;
+        force_f64_kernel<<<dim3(grid_for(n, kAdjBS), S), kAdjBS, 0, s>>>(rec, m, n, eps * eps, sp.jchunk, part);
+    } else {
+        const Split sp = split_for(n, kT);
+        S = sp.S;
+        const float e2 = float(eps * eps);
+        const dim3 grid(wave_blocks(n, kT), S);
+        if (e2 > 0.f) force_mixed_kernel<kT, false><<<grid, kAdjBS, 0, s>>>(rec, n, e2, sp.jchunk, part);
+        else force_mixed_kernel<kT, true><<<grid, kAdjBS, 0, s>>>(rec, n, e2, sp.jchunk, part);
+    }
+    LHIP_LAUNCH_CHECK();
+    finish_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(part, n, S, G, h, ax, ay, az, vx, vy, vz);
+    LHIP_LAUNCH_CHECK();
+}
+
+void launch_pack_vjp(hipStream_t s, const double* pos, const double* m, const double* lvx, const double* lvy,
+                     const double* lvz, double c, SimRec* rec, int n) {
+    if (n <= 0) return;
+    pack_vjp_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(pos, m, lvx, lvy, lvz, c, rec, n);
+    LHIP_LAUNCH_CHECK();
+}
+
+void launch_vjp_update(hipStream_t s, const SimRec* rec, int n, double eps, double G, double* part, double* lx,
+                       double* ly, double* lz, double* lm, double* lvx, double* lvy, double* lvz, double drift,
+                       const double* npos, const double* m, double c_next, SimRec* next_rec) {
+    if (n <= 0) return;
+    const Split sp = split_for(n, kT);
+    const float e2 = float(eps * eps);
+    const dim3 grid(wave_blocks(n, kT), sp.S);
+    if (e2 > 0.f) vjp_direct_kernel<kT, false><<<grid, kAdjBS, 0, s>>>(rec, n, e2, sp.jchunk, part);
+    else vjp_direct_kernel<kT, true><<<grid, kAdjBS, 0, s>>>(rec, n, e2, sp.jchunk, part);
+    LHIP_LAUNCH_CHECK();
+    adj_update_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(part, n, sp.S, G, lx, ly, lz, lm, lvx, lvy, lvz, drift, npos, m,
+                                                       c_next, next_rec);
     LHIP_LAUNCH_CHECK();
 }
 
@@ -279,62 +415,6 @@ void launch_fill64(hipStream_t s, double* y, double v, int n) {
     if (n <= 0) return;
     fill64_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(y, v, n);
     LHIP_LAUNCH_CHECK();
-}
-
-void launch_scale3(hipStream_t s, double* y0, double* y1, double* y2, const double* x0, const double* x1,
-                   const double* x2, double c, int n) {
-    if (n <= 0) return;
-    scale3_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(y0, y1, y2, x0, x1, x2, c, n);
-    LHIP_LAUNCH_CHECK();
-}
-
-void launch_force64(hipStream_t s, const double* x, const double* y, const double* z, const double* m, int n,
-                    double eps, double G, bool pure_f64, double* ax, double* ay, double* az, double* part) {
-    if (n <= 0) return;
-    if (pure_f64) {
-        const int S = split_for(n, 1);
-        const int jchunk = (n + S - 1) / S;
-        dim3 grid(grid_for(n, kAdjBS), S);
-        if (S > 1) force_f64_kernel<true><<<grid, kAdjBS, 0, s>>>(x, y, z, m, n, eps * eps, G, jchunk, ax, ay, az, part);
-        else force_f64_kernel<false><<<grid, kAdjBS, 0, s>>>(x, y, z, m, n, eps * eps, G, jchunk, ax, ay, az, part);
-        LHIP_LAUNCH_CHECK();
-        if (S > 1) {
-            reduce_split_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(part, n, S, 3, G, ax, ay, az, nullptr);
-            LHIP_LAUNCH_CHECK();
-        }
-        return;
-    }
-    const int S = split_for(n, kT);
-    const int jchunk = (n + S - 1) / S;
-    dim3 grid(grid_for(n, kAdjBS * kT), S);
-    const float e2 = float(eps * eps);
-    if (S > 1) force_mixed_kernel<kT, true><<<grid, kAdjBS, 0, s>>>(x, y, z, m, n, e2, G, jchunk, ax, ay, az, part);
-    else force_mixed_kernel<kT, false><<<grid, kAdjBS, 0, s>>>(x, y, z, m, n, e2, G, jchunk, ax, ay, az, part);
-    LHIP_LAUNCH_CHECK();
-    if (S > 1) {
-        reduce_split_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(part, n, S, 3, G, ax, ay, az, nullptr);
-        LHIP_LAUNCH_CHECK();
-    }
-}
-
-void launch_vjp_direct(hipStream_t s, const double* x, const double* y, const double* z, const double* m,
-                       const double* ux, const double* uy, const double* uz, int n, double eps, double G, double* wx,
-                       double* wy, double* wz, double* gm, double* part) {
-    if (n <= 0) return;
-    const int S = split_for(n, kT);
-    const int jchunk = (n + S - 1) / S;
-    dim3 grid(grid_for(n, kAdjBS * kT), S);
-    const float e2 = float(eps * eps);
-    if (S > 1)
-        vjp_direct_kernel<kT, true><<<grid, kAdjBS, 0, s>>>(x, y, z, m, ux, uy, uz, n, e2, G, jchunk, wx, wy, wz, gm, part);
-    else
-        vjp_direct_kernel<kT, false><<<grid, kAdjBS, 0, s>>>(x, y, z, m, ux, uy, uz, n, e2, G, jchunk, wx, wy, wz, gm,
-                                                             part);
-    LHIP_LAUNCH_CHECK();
-    if (S > 1) {
-        reduce_split_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(part, n, S, 4, G, wx, wy, wz, gm);
-        LHIP_LAUNCH_CHECK();
-    }
 }
 
 }  // namespace lhip
