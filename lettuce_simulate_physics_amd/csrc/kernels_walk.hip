@@ -21,7 +21,12 @@
 // per lane would idle most lanes. Instead the wave has V = 64 * NS virtual slots (NS = 1 for groups
 // of <= 32 bodies, else 2 per lane); with S = floor(V / nb) list shares, slot v evaluates body
 // v mod nb against list entries e = v div nb (mod S). Partial sums of the S shares are added in
-// LDS at the end. Lists are evaluated whenever the next append would overflow them.
+// LDS at the end. Lists are evaluated whenever the next append would overflow them. Before an
+// evaluation the list is padded with zero-mass entries to a multiple of S, so the loops carry no
+// bounds test; a zero-mass entry adds exactly zero. With eps > 0 a body's own entry adds exactly
+// zero too (d = 0), so only the potential and eps = 0 need the per-pair self test; every body of a
+// group meets itself exactly once (the leaves holding the group are always opened), which is how
+// the self pairs are taken out of the interaction counts.
 //
 // Scheduling. A persistent grid; each wave takes groups from an atomic counter until none are left.
 #include <cfloat>
@@ -42,34 +47,39 @@ struct Slot {
     float x, y, z;
     int i;       // body index
     int bi;      // body within the chunk
-    int sp;      // list share
+    int sp;      // list share (slots without a share read share 0 and are discarded)
     bool on;
     float sx, sy, sz, sp_pot;
-    uint32_t self;
+};
+
+constexpr int kPad = kMaxShare - 1;  // zero entries appended to a list before it is evaluated
+
+struct CellL {
+    float4 A;  // com, mass
+    float4 B;  // qxx, qxy, qxz, qyy
+    float2 C;  // qyz, qzz
+    float2 pad;
 };
 
 struct WaveLds {
-    float4 cA[kCellCap];  // com, mass
-    float4 cB[kCellCap];  // qxx, qxy, qxz, qyy
-    float2 cC[kCellCap];  // qyz, qzz
-    float4 bP[kBodyCap];  // position, mass
-    int bI[kBodyCap];     // body index
+    CellL c[kCellCap + kPad];
+    float4 bP[kBodyCap + kPad];  // position, mass
+    int bI[kBodyCap + kPad];     // body index (-1 for padding)
 };
 
-template <bool POT, bool GUARD>
+// MASK: exclude j == i (and, with GUARD, coincident pairs); needed for the potential and eps = 0.
+template <bool POT, bool MASK, bool GUARD>
 __device__ __forceinline__ void p2p(Slot& b, float4 q, int j, float e2) {
     const float dx = q.x - b.x, dy = q.y - b.y, dz = q.z - b.z;
     const float r2 = __fmaf_rn(dz, dz, __fmaf_rn(dy, dy, __fmaf_rn(dx, dx, e2)));
-    const bool self = j == b.i;
-    const bool ok = !self && (!GUARD || r2 > 0.f);
-    const float inv = ok ? __builtin_amdgcn_rsqf(r2) : 0.f;
+    float inv = __builtin_amdgcn_rsqf(r2);
+    if (MASK) inv = (j != b.i && (!GUARD || r2 > 0.f)) ? inv : 0.f;
     const float mi = q.w * inv;
     const float f = mi * (inv * inv);
     b.sx = __fmaf_rn(f, dx, b.sx);
     b.sy = __fmaf_rn(f, dy, b.sy);
     b.sz = __fmaf_rn(f, dz, b.sz);
     if (POT) b.sp_pot += mi;
-    b.self += self ? 1u : 0u;
 }
 
 // Monopole + quadrupole about the com, d = com - x (reference src/cpu_backend.cpp:855-871):
@@ -78,19 +88,22 @@ template <bool POT, bool GUARD>
 __device__ __forceinline__ void p2c(Slot& b, float4 A, float4 B, float2 C, float e2) {
     const float dx = A.x - b.x, dy = A.y - b.y, dz = A.z - b.z;
     const float r2 = __fmaf_rn(dz, dz, __fmaf_rn(dy, dy, __fmaf_rn(dx, dx, e2)));
-    const float inv = (!GUARD || r2 > 0.f) ? __builtin_amdgcn_rsqf(r2) : 0.f;
+    float inv = __builtin_amdgcn_rsqf(r2);
+    if (GUARD) inv = r2 > 0.f ? inv : 0.f;
     const float inv2 = inv * inv, inv3 = inv * inv2, inv5 = inv3 * inv2;
     const float qx = B.x * dx + B.y * dy + B.z * dz;
     const float qy = B.y * dx + B.w * dy + C.x * dz;
     const float qz = B.z * dx + C.x * dy + C.y * dz;
     const float dqd = dx * qx + dy * qy + dz * qz;
-    const float f = A.w * inv3 + 2.5f * dqd * inv5 * inv2;
-    b.sx += f * dx - qx * inv5;
-    b.sy += f * dy - qy * inv5;
-    b.sz += f * dz - qz * inv5;
-    if (POT) b.sp_pot += A.w * inv + 0.5f * dqd * inv5;
+    const float t = dqd * inv5;
+    const float f = __fmaf_rn(2.5f * t, inv2, A.w * inv3);
+    b.sx = __fmaf_rn(-inv5, qx, __fmaf_rn(f, dx, b.sx));
+    b.sy = __fmaf_rn(-inv5, qy, __fmaf_rn(f, dy, b.sy));
+    b.sz = __fmaf_rn(-inv5, qz, __fmaf_rn(f, dz, b.sz));
+    if (POT) b.sp_pot += __fmaf_rn(0.5f, t, A.w * inv);
 }
 
+// Every slot evaluates entries sp, sp + S, ... < steps * S (the list is padded to that length).
 template <int NS, bool POT, bool GUARD>
 __device__ __forceinline__ void eval_cells(Slot (&sl)[2], int S, int nc, const WaveLds& L, float e2) {
     const int steps = (nc + S - 1) / S;
@@ -98,19 +111,19 @@ __device__ __forceinline__ void eval_cells(Slot (&sl)[2], int S, int nc, const W
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             const int e = t * S + sl[s].sp;
-            if (sl[s].on && e < nc) p2c<POT, GUARD>(sl[s], L.cA[e], L.cB[e], L.cC[e], e2);
+            p2c<POT, GUARD>(sl[s], L.c[e].A, L.c[e].B, L.c[e].C, e2);
         }
     }
 }
 
-template <int NS, bool POT, bool GUARD>
+template <int NS, bool POT, bool MASK, bool GUARD>
 __device__ __forceinline__ void eval_bodies(Slot (&sl)[2], int S, int nb, const WaveLds& L, float e2) {
     const int steps = (nb + S - 1) / S;
     for (int t = 0; t < steps; ++t) {
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             const int e = t * S + sl[s].sp;
-            if (sl[s].on && e < nb) p2p<POT, GUARD>(sl[s], L.bP[e], L.bI[e], e2);
+            p2p<POT, MASK, GUARD>(sl[s], L.bP[e], MASK ? L.bI[e] : 0, e2);
         }
     }
 }
@@ -150,6 +163,7 @@ constexpr int kStackThr = kStackCap - 7 * kLevels - 8;
 
 template <bool POT, bool GUARD>
 __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
+    constexpr bool MASK = POT || GUARD;
     __shared__ WaveLds lds[kWalkWaves];
     __shared__ int stk[kWalkWaves][kStackCap];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -180,11 +194,11 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                 sl[s].bi = v % nbody;
                 sl[s].sp = v / nbody;
                 sl[s].on = s < NS && sl[s].sp < S;
+                if (!sl[s].on) sl[s].sp = 0;
                 sl[s].i = clo + sl[s].bi;
                 const float4 p = a.posm[sl[s].i];
                 sl[s].x = p.x; sl[s].y = p.y; sl[s].z = p.z;
                 sl[s].sx = sl[s].sy = sl[s].sz = sl[s].sp_pot = 0.f;
-                sl[s].self = 0;
                 mnx = fminf(mnx, p.x); mny = fminf(mny, p.y); mnz = fminf(mnz, p.z);
                 mxx = fmaxf(mxx, p.x); mxy = fmaxf(mxy, p.y); mxz = fmaxf(mxz, p.z);
             }
@@ -198,6 +212,11 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
             int nc = 0, nb = 0;
             uint32_t ncell_total = 0, nbody_total = 0;
             auto flush_cells = [&]() {
+                if (lane < kPad) {
+                    L.c[nc + lane].A = make_float4(0.f, 0.f, 0.f, 0.f);
+                    L.c[nc + lane].B = make_float4(0.f, 0.f, 0.f, 0.f);
+                    L.c[nc + lane].C = make_float2(0.f, 0.f);
+                }
                 __builtin_amdgcn_wave_barrier();
                 if (NS == 2) eval_cells<2, POT, GUARD>(sl, S, nc, L, e2);
                 else eval_cells<1, POT, GUARD>(sl, S, nc, L, e2);
@@ -206,9 +225,13 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                 nc = 0;
             };
             auto flush_bodies = [&]() {
+                if (lane < kPad) {
+                    L.bP[nb + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    L.bI[nb + lane] = -1;
+                }
                 __builtin_amdgcn_wave_barrier();
-                if (NS == 2) eval_bodies<2, POT, GUARD>(sl, S, nb, L, e2);
-                else eval_bodies<1, POT, GUARD>(sl, S, nb, L, e2);
+                if (NS == 2) eval_bodies<2, POT, MASK, GUARD>(sl, S, nb, L, e2);
+                else eval_bodies<1, POT, MASK, GUARD>(sl, S, nb, L, e2);
                 __builtin_amdgcn_wave_barrier();
                 nbody_total += uint32_t(nb);
                 nb = 0;
@@ -259,9 +282,9 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                     if (is_cell) {
                         const int slot = nc + int(lanes_below(mc));
                         const NodeMom& md = a.mom[k];
-                        L.cA[slot] = make_float4(md.cx, md.cy, md.cz, md.m);
-                        L.cB[slot] = make_float4(md.qxx, md.qxy, md.qxz, md.qyy);
-                        L.cC[slot] = make_float2(md.qyz, md.qzz);
+                        L.c[slot].A = make_float4(md.cx, md.cy, md.cz, md.m);
+                        L.c[slot].B = make_float4(md.qxx, md.qxy, md.qxz, md.qyy);
+                        L.c[slot].C = make_float2(md.qyz, md.qzz);
                     }
                     nc += ncnew;
                 }
@@ -291,47 +314,38 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
             flush_bodies();
 
             // Add the list shares of each body (slot v = share * nbody + body) through LDS.
-            float4* red = L.bP;                      // kBodyCap >= 128 = 64 * NS entries
-            uint32_t* rself = reinterpret_cast<uint32_t*>(L.bI);
+            float4* red = L.bP;  // kBodyCap >= 128 = 64 * NS entries
             __builtin_amdgcn_wave_barrier();
 #pragma unroll
             for (int s = 0; s < 2; ++s)
-                if (sl[s].on) {
-                    const int v = sl[s].sp * nbody + sl[s].bi;
-                    red[v] = make_float4(sl[s].sx, sl[s].sy, sl[s].sz, sl[s].sp_pot);
-                    rself[v] = sl[s].self;
-                }
+                if (sl[s].on) red[sl[s].sp * nbody + sl[s].bi] = make_float4(sl[s].sx, sl[s].sy, sl[s].sz, sl[s].sp_pot);
             __builtin_amdgcn_wave_barrier();
             const float G = a.G;
             const uint32_t listed = ncell_total + nbody_total;
-            uint32_t self_sum = 0;
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
                 const int v = s * 64 + lane;
                 if (v >= nbody) continue;
                 float4 acc4 = red[v];
-                uint32_t self = rself[v];
                 for (int sp = 1; sp < S; ++sp) {
                     const float4 q = red[sp * nbody + v];
                     acc4.x += q.x; acc4.y += q.y; acc4.z += q.z; acc4.w += q.w;
-                    self += rself[sp * nbody + v];
                 }
                 const int i = clo + v;
                 a.ax[i] = G * acc4.x;
                 a.ay[i] = G * acc4.y;
                 a.az[i] = G * acc4.z;
                 if (POT) a.pot[i] = -G * acc4.w;
-                if (a.cost) a.cost[i] = float(listed - self);
-                self_sum += self;
+                if (a.cost) a.cost[i] = float(listed - 1u);
             }
             __builtin_amdgcn_wave_barrier();
             if (lane == 0) {
+                // Each body of the chunk meets itself once in the body list.
                 t_listed += (unsigned long long)listed * nbody;
                 t_p2c += (unsigned long long)ncell_total * nbody;
+                t_p2p += (unsigned long long)nbody_total * nbody - nbody;
+                t_inter += (unsigned long long)listed * nbody - nbody;
             }
-            // self_sum is per lane; the totals subtract it once per body.
-            t_p2p += (lane == 0 ? (unsigned long long)nbody_total * nbody : 0ull) - self_sum;
-            t_inter += (lane == 0 ? (unsigned long long)listed * nbody : 0ull) - self_sum;
         }
     }
     if (overflow && lane == 0) atomicOr(&a.counters->error, 1);
