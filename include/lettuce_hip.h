@@ -121,9 +121,34 @@ int lhip_device_memory(lhip_ctx* ctx, uint64_t* free_bytes, uint64_t* total_byte
  * order; LHIP_NOT_PROVIDED when there is none. */
 int lhip_body_costs(lhip_ctx* ctx, float* cost);
 
+/* ---- mixed-precision form (no reference counterpart: the reference's state is FP32,
+ *      src/bodies.hpp:8-13) ----
+ *
+ * LHIP_MODE_MIXED keeps positions and velocities in FP64 on the device: FP64 kick and drift, Morton
+ * keys quantised from the FP64 positions (the reference recipe, src/cpu_backend.cpp:396-426), FP64
+ * multipole accumulation at every tree level, pair arithmetic in FP32 on coordinates relative to the
+ * walk group's centre (tree) or on FP64 separations (direct), FP32 partial sums carried into FP64,
+ * and FP64 diagnostics. LHIP_MODE_PLAIN is the reference's FP32 state (the default). Switching
+ * converts the resident bodies, accelerations and potential and invalidates the tree.
+ * group_max: walk group size for later trees, in [16, 128] (128 = the reference's kGroupMax,
+ * src/cpu_backend.cpp:27). */
+enum { LHIP_MODE_PLAIN = 0, LHIP_MODE_MIXED = 1 };
+int lhip_set_precision(lhip_ctx* ctx, int32_t mode, int32_t group_max);
+int lhip_get_precision(const lhip_ctx* ctx, int32_t* mode, int32_t* group_max);
+/* FP64 initial conditions and state (the extra entry Bodies' FP32 fields cannot carry). Mixed
+ * mode only (LHIP_LOGIC otherwise). Masses and ids as in lhip_upload. */
+int lhip_upload_f64(lhip_ctx* ctx, uint64_t n, const double* x, const double* y, const double* z, const double* vx,
+                    const double* vy, const double* vz, const float* m, const uint32_t* id);
+int lhip_download_f64(lhip_ctx* ctx, double* x, double* y, double* z, double* vx, double* vy, double* vz, float* m,
+                      uint32_t* id);
+/* FP64 accelerations and potential of the last force call (mixed mode only). Any pointer may be
+ * NULL. */
+int lhip_accelerations_f64(lhip_ctx* ctx, double* ax, double* ay, double* az, double* pot);
+
 /* ---- inspection (no reference counterpart; for parity tests and tools) ---- */
 
-/* Accelerations and potential of the last force call, in the backend's current order. */
+/* Accelerations and potential of the last force call, in the backend's current order (rounded to
+ * float in the mixed mode). */
 int lhip_accelerations(lhip_ctx* ctx, float* ax, float* ay, float* az, float* pot);
 /* Sorted Morton keys of the current tree (n entries). */
 int lhip_tree_keys(lhip_ctx* ctx, uint64_t* keys);

@@ -25,6 +25,8 @@ _U64 = ctypes.c_uint64
 
 OK, RUNTIME, LOGIC, UNAVAILABLE, OUT_OF_MEMORY, INVALID, NOT_PROVIDED = range(7)
 METHOD_DIRECT, METHOD_TREE = 0, 1
+MODE_PLAIN, MODE_MIXED = 0, 1
+REFERENCE_GROUP_MAX = 128  # src/cpu_backend.cpp:27
 
 
 class LogicError(RuntimeError):
@@ -253,6 +255,45 @@ class HipBackend:
             return None
         self._check(rc)
         return c
+
+    # ---- mixed-precision form (no reference counterpart) ----
+    def set_precision(self, mode: str = "plain", group_max: int = REFERENCE_GROUP_MAX) -> None:
+        """'plain': the reference's FP32 state; 'mixed': FP64 state with FP32 relative-coordinate
+        pair arithmetic. group_max: walk group size of later trees (16..128)."""
+        if mode not in ("plain", "mixed"):
+            raise ValueError("mode must be 'plain' or 'mixed'")
+        self._check(self._L.lhip_set_precision(self._h, MODE_MIXED if mode == "mixed" else MODE_PLAIN,
+                                               int(group_max)))
+
+    def precision(self):
+        m, g = ctypes.c_int32(), ctypes.c_int32()
+        self._check(self._L.lhip_get_precision(self._h, ctypes.byref(m), ctypes.byref(g)))
+        return ("mixed" if m.value == MODE_MIXED else "plain"), int(g.value)
+
+    def upload_f64(self, x, v, m, ids=None) -> None:
+        """FP64 positions and velocities (n x 3), float masses, uint32 ids (default 0..n-1)."""
+        n = len(m)
+        x = np.asarray(x, np.float64).reshape(n, 3)
+        v = np.asarray(v, np.float64).reshape(n, 3)
+        cols = [np.ascontiguousarray(a[:, c]) for a in (x, v) for c in range(3)]
+        mm = np.ascontiguousarray(m, np.float32)
+        ids = np.arange(n, dtype=np.uint32) if ids is None else np.ascontiguousarray(ids, np.uint32)
+        self._check(self._L.lhip_upload_f64(self._h, _U64(n), *[_ptr(c) for c in cols], _ptr(mm), _ptr(ids)))
+
+    def download_f64(self):
+        """(x (n x 3), v (n x 3), m, id) in the backend's order, FP64 positions and velocities."""
+        n = self.size()
+        cols = [np.zeros(n) for _ in range(6)]
+        mm = np.zeros(n, np.float32)
+        ids = np.zeros(n, np.uint32)
+        self._check(self._L.lhip_download_f64(self._h, *[_ptr(c) for c in cols], _ptr(mm), _ptr(ids)))
+        return np.stack(cols[:3], 1), np.stack(cols[3:], 1), mm, ids
+
+    def accelerations_f64(self):
+        n = self.size()
+        out = [np.zeros(n) for _ in range(4)]
+        self._check(self._L.lhip_accelerations_f64(self._h, *[_ptr(o) for o in out]))
+        return out
 
     # ---- inspection ----
     def accelerations(self):

@@ -39,4 +39,12 @@ void launch_vjp_update(hipStream_t s, const SimRec* rec, int n, double eps, doub
                        const double* npos, const double* m, double c_next, SimRec* next_rec);
 void launch_fill64(hipStream_t s, double* y, double v, int n);
 
+// Direct summation on FP64 positions with float masses (the mixed-precision backend form): FP32
+// pair arithmetic on FP64 separations, FP64 carry every 64 sources; a = G sum, phi = -G sum m/s.
+// rec: n records of scratch; part: direct64_split(n) * 4 * n doubles.
+int direct64_split(int n);
+void launch_direct64(hipStream_t s, const double* x, const double* y, const double* z, const float* m, int n,
+                     double eps, double G, bool potential, double* ax, double* ay, double* az, double* pot,
+                     SimRec* rec, double* part);
+
 }  // namespace lhip

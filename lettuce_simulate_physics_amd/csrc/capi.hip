@@ -148,16 +148,29 @@ int lhip_body_costs(lhip_ctx* ctx, float* cost) {
 }
 
 int lhip_accelerations(lhip_ctx* ctx, float* ax, float* ay, float* az, float* pot) {
-    return guarded(ctx, [&](lhip::Context& c) {
-        if (c.n == 0) return;
-        const std::size_t fb = std::size_t(c.n) * 4;
-        const std::pair<float*, lhip::DevBuf*> d[] = {{ax, &c.ax}, {ay, &c.ay}, {az, &c.az}, {pot, &c.pot}};
-        for (auto& e : d)
-            if (e.first) LHIP_CHECK(hipMemcpyAsync(e.first, e.second->as<void>(), fb, hipMemcpyDeviceToHost, c.stream));
-        c.sync();
-    });
+    return guarded(ctx, [&](lhip::Context& c) { c.accelerations(ax, ay, az, pot); });
 }
-
+int lhip_accelerations_f64(lhip_ctx* ctx, double* ax, double* ay, double* az, double* pot) {
+    return guarded(ctx, [&](lhip::Context& c) { c.accelerations_f64(ax, ay, az, pot); });
+}
+int lhip_set_precision(lhip_ctx* ctx, int32_t mode, int32_t group_max) {
+    return guarded(ctx, [&](lhip::Context& c) { c.set_precision(mode, group_max); });
+}
+int lhip_get_precision(const lhip_ctx* ctx, int32_t* mode, int32_t* group_max) {
+    if (!ctx || !ctx->c) return LHIP_INVALID;
+    if (mode) *mode = ctx->c->mixed ? LHIP_MODE_MIXED : LHIP_MODE_PLAIN;
+    if (group_max) *group_max = ctx->c->gmax;
+    return LHIP_OK;
+}
+int lhip_upload_f64(lhip_ctx* ctx, uint64_t n, const double* x, const double* y, const double* z, const double* vx,
+                    const double* vy, const double* vz, const float* m, const uint32_t* id) {
+    if (n > 0 && (!x || !y || !z || !vx || !vy || !vz || !m || !id)) return LHIP_INVALID;
+    return guarded(ctx, [&](lhip::Context& c) { c.upload_f64(n, x, y, z, vx, vy, vz, m, id); });
+}
+int lhip_download_f64(lhip_ctx* ctx, double* x, double* y, double* z, double* vx, double* vy, double* vz, float* m,
+                      uint32_t* id) {
+    return guarded(ctx, [&](lhip::Context& c) { c.download_f64(x, y, z, vx, vy, vz, m, id); });
+}
 int lhip_tree_keys(lhip_ctx* ctx, uint64_t* keys) {
     return guarded(ctx, [&](lhip::Context& c) {
         if (!c.tree_ok) throw lhip::Error(LHIP_LOGIC, "no current tree");

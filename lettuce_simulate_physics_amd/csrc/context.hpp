@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/lettuce_hip.h"
+#include "adjoint.hpp"
 #include "kernels.hpp"
 #include "tree.hpp"
 
@@ -42,6 +43,13 @@ struct Context {
     DevBuf bl, cnt, gflag, scan_tmp, pkey, pkey2, pval, pval2, inv, levstart, node, mom, prefix, gstart, counters;
     long long cap_nodes = 0;
 
+    // Mixed-precision form: FP64 positions and velocities (backend order) with permute alternates,
+    // FP64 accelerations and potential, FP64 node moments and direct-sum records. Masses and ids
+    // stay in m and id; the float position arrays are then only download staging.
+    bool mixed = false;
+    int gmax = kGroupMax;  // walk group size (128: the reference's groups)
+    DevBuf xd, yd, zd, vxd, vyd, vzd, xd2, yd2, zd2, vxd2, vyd2, vzd2, axd, ayd, azd, potd, momd, rec64;
+
     bool has_pe = false, acc_ok = false, tree_ok = false, costs_ok = false;
     bool kick_rec = false, drift_rec = false;
     lhip_phase_times t{};
@@ -61,7 +69,7 @@ struct Context {
         LHIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         for (auto& e : ev) LHIP_CHECK(hipEventCreate(&e));
         cube.reserve(sizeof(Cube));
-        bbox_part.reserve(sizeof(float) * 6 * kReduceBlocks);
+        bbox_part.reserve(sizeof(double) * 6 * kReduceBlocks);
         red_part.reserve(sizeof(double) * 12 * std::max<std::size_t>(kReduceBlocks, 1));
         red_out.reserve(sizeof(double) * 16);
         counters.reserve(sizeof(WalkCounters));
@@ -83,6 +91,114 @@ struct Context {
                           &ax, &ay, &az, &pot})
             b->reserve(f);
         posm.reserve(std::size_t(nn) * sizeof(float4));
+        if (mixed) reserve_f64(nn);
+    }
+
+    void reserve_f64(int nn) {
+        const std::size_t d = std::size_t(nn) * 8;
+        for (DevBuf* b : {&xd, &yd, &zd, &vxd, &vyd, &vzd, &xd2, &yd2, &zd2, &vxd2, &vyd2, &vzd2, &axd, &ayd, &azd, &potd})
+            b->reserve(d);
+    }
+
+    double* D(DevBuf& b) { return b.as<double>(); }
+    float* F(DevBuf& b) { return b.as<float>(); }
+
+    // Switches between the plain (FP32 state) and mixed-precision (FP64 state) forms, converting the
+    // resident bodies, accelerations and potential; group is the walk group size for later trees.
+    void set_precision(int mode, int group) {
+        LHIP_CHECK(hipSetDevice(dev));
+        if (mode != LHIP_MODE_PLAIN && mode != LHIP_MODE_MIXED) throw Error(kInvalid, "unknown precision mode");
+        if (group < kLeafMax || group > kGroupMax)
+            throw Error(kInvalid, "walk group size must be in [16, 128], got " + std::to_string(group));
+        const bool mix = mode == LHIP_MODE_MIXED;
+        if (mix != mixed && n > 0) {
+            if (mix) {
+                reserve_f64(n);
+                const std::pair<DevBuf*, DevBuf*> cv[] = {{&x, &xd}, {&y, &yd}, {&z, &zd}, {&vx, &vxd}, {&vy, &vyd},
+                                                          {&vz, &vzd}, {&ax, &axd}, {&ay, &ayd}, {&az, &azd},
+                                                          {&pot, &potd}};
+                for (auto& e : cv) launch_f32_to_f64(stream, F(*e.first), D(*e.second), n);
+            } else {
+                const std::pair<DevBuf*, DevBuf*> cv[] = {{&xd, &x}, {&yd, &y}, {&zd, &z}, {&vxd, &vx}, {&vyd, &vy},
+                                                          {&vzd, &vz}, {&axd, &ax}, {&ayd, &ay}, {&azd, &az},
+                                                          {&potd, &pot}};
+                for (auto& e : cv) launch_f64_to_f32(stream, D(*e.first), F(*e.second), n);
+            }
+            sync();
+        }
+        mixed = mix;
+        gmax = group;
+        tree_ok = false;  // the next tree uses the new form and group size
+    }
+
+    // FP64 state in and out (mixed form only).
+    void upload_f64(uint64_t nn, const double* hx, const double* hy, const double* hz, const double* hvx,
+                    const double* hvy, const double* hvz, const float* hm, const uint32_t* hid) {
+        if (!mixed) throw Error(kLogic, "hip upload_f64: FP64 state needs the mixed-precision mode");
+        if (nn >= uint64_t(INT32_MAX)) throw Error(kInvalid, "hip backend: n = " + std::to_string(nn) +
+                                                                " exceeds the 2^31-1 body limit");
+        LHIP_CHECK(hipSetDevice(dev));
+        reserve_bodies(int(nn));
+        n = int(nn);
+        reset_state();
+        if (n == 0) return;
+        const std::size_t db = std::size_t(n) * 8, fb = std::size_t(n) * 4;
+        const std::pair<DevBuf*, const double*> src[] = {{&xd, hx}, {&yd, hy}, {&zd, hz},
+                                                         {&vxd, hvx}, {&vyd, hvy}, {&vzd, hvz}};
+        for (auto& e : src) LHIP_CHECK(hipMemcpyAsync(e.first->as<void>(), e.second, db, hipMemcpyHostToDevice, stream));
+        LHIP_CHECK(hipMemcpyAsync(m.as<void>(), hm, fb, hipMemcpyHostToDevice, stream));
+        LHIP_CHECK(hipMemcpyAsync(id.as<void>(), hid, fb, hipMemcpyHostToDevice, stream));
+        for (DevBuf* b : {&axd, &ayd, &azd, &potd}) launch_fill64(stream, D(*b), 0.0, n);
+        sync();
+    }
+
+    void download_f64(double* hx, double* hy, double* hz, double* hvx, double* hvy, double* hvz, float* hm,
+                      uint32_t* hid) {
+        if (!mixed) throw Error(kLogic, "hip download_f64: FP64 state needs the mixed-precision mode");
+        LHIP_CHECK(hipSetDevice(dev));
+        if (n == 0) return;
+        const std::size_t db = std::size_t(n) * 8, fb = std::size_t(n) * 4;
+        const std::pair<void*, DevBuf*> dst[] = {{hx, &xd}, {hy, &yd}, {hz, &zd}, {hvx, &vxd}, {hvy, &vyd}, {hvz, &vzd}};
+        for (auto& e : dst)
+            if (e.first) LHIP_CHECK(hipMemcpyAsync(e.first, e.second->as<void>(), db, hipMemcpyDeviceToHost, stream));
+        if (hm) LHIP_CHECK(hipMemcpyAsync(hm, m.as<void>(), fb, hipMemcpyDeviceToHost, stream));
+        if (hid) LHIP_CHECK(hipMemcpyAsync(hid, id.as<void>(), fb, hipMemcpyDeviceToHost, stream));
+        sync();
+    }
+
+    void reset_state() {
+        has_pe = false;
+        acc_ok = true;  // zeroed by the upload
+        tree_ok = costs_ok = false;
+        kick_rec = drift_rec = false;
+        t = lhip_phase_times{};
+    }
+
+    // Accelerations and potential of the last force call, in the backend's precision converted to
+    // float (plain: as computed).
+    void accelerations(float* hax, float* hay, float* haz, float* hpot) {
+        LHIP_CHECK(hipSetDevice(dev));
+        if (n == 0) return;
+        if (mixed) {
+            const std::pair<DevBuf*, DevBuf*> cv[] = {{&axd, &ax}, {&ayd, &ay}, {&azd, &az}, {&potd, &pot}};
+            for (auto& e : cv) launch_f64_to_f32(stream, D(*e.first), F(*e.second), n);
+        }
+        const std::size_t fb = std::size_t(n) * 4;
+        const std::pair<float*, DevBuf*> d[] = {{hax, &ax}, {hay, &ay}, {haz, &az}, {hpot, &pot}};
+        for (auto& e : d)
+            if (e.first) LHIP_CHECK(hipMemcpyAsync(e.first, e.second->as<void>(), fb, hipMemcpyDeviceToHost, stream));
+        sync();
+    }
+
+    void accelerations_f64(double* hax, double* hay, double* haz, double* hpot) {
+        if (!mixed) throw Error(kLogic, "hip accelerations_f64: FP64 accelerations need the mixed-precision mode");
+        LHIP_CHECK(hipSetDevice(dev));
+        if (n == 0) return;
+        const std::size_t db = std::size_t(n) * 8;
+        const std::pair<double*, DevBuf*> d[] = {{hax, &axd}, {hay, &ayd}, {haz, &azd}, {hpot, &potd}};
+        for (auto& e : d)
+            if (e.first) LHIP_CHECK(hipMemcpyAsync(e.first, e.second->as<void>(), db, hipMemcpyDeviceToHost, stream));
+        sync();
     }
 
     void reserve_tree(int nn) {
@@ -122,23 +238,30 @@ struct Context {
         LHIP_CHECK(hipSetDevice(dev));
         reserve_bodies(int(nn));
         n = int(nn);
-        has_pe = false;
-        acc_ok = true;  // zeroed below
-        tree_ok = costs_ok = false;
-        kick_rec = drift_rec = false;
-        t = lhip_phase_times{};
+        reset_state();
         if (n == 0) return;
         const std::size_t fb = std::size_t(n) * 4;
         const std::pair<DevBuf*, const void*> src[] = {{&x, hx}, {&y, hy}, {&z, hz}, {&vx, hvx},
                                                        {&vy, hvy}, {&vz, hvz}, {&m, hm}, {&id, hid}};
         for (auto& s : src) LHIP_CHECK(hipMemcpyAsync(s.first->as<void>(), s.second, fb, hipMemcpyHostToDevice, stream));
         for (DevBuf* b : {&ax, &ay, &az, &pot}) launch_zero_f32(stream, b->as<float>(), n);
+        if (mixed) {  // float bodies promoted to the FP64 state
+            const std::pair<DevBuf*, DevBuf*> cv[] = {{&x, &xd}, {&y, &yd}, {&z, &zd},
+                                                      {&vx, &vxd}, {&vy, &vyd}, {&vz, &vzd}};
+            for (auto& e : cv) launch_f32_to_f64(stream, F(*e.first), D(*e.second), n);
+            for (DevBuf* b : {&axd, &ayd, &azd, &potd}) launch_fill64(stream, D(*b), 0.0, n);
+        }
         sync();
     }
 
     void download(float* hx, float* hy, float* hz, float* hvx, float* hvy, float* hvz, float* hm, uint32_t* hid) {
         LHIP_CHECK(hipSetDevice(dev));
         if (n == 0) return;
+        if (mixed) {  // FP64 state rounded to float
+            const std::pair<DevBuf*, DevBuf*> cv[] = {{&xd, &x}, {&yd, &y}, {&zd, &z},
+                                                      {&vxd, &vx}, {&vyd, &vy}, {&vzd, &vz}};
+            for (auto& e : cv) launch_f64_to_f32(stream, D(*e.first), F(*e.second), n);
+        }
         const std::size_t fb = std::size_t(n) * 4;
         const std::pair<void*, DevBuf*> dst[] = {{hx, &x}, {hy, &y}, {hz, &z}, {hvx, &vx},
                                                  {hvy, &vy}, {hvz, &vz}, {hm, &m}, {hid, &id}};
@@ -152,8 +275,11 @@ struct Context {
         if (n == 0) return;
         if (!acc_ok) throw Error(kLogic, "hip kick: accelerations are stale (build_tree since the last force call)");
         LHIP_CHECK(hipEventRecord(ev[5], stream));
-        launch_kick(stream, vx.as<float>(), vy.as<float>(), vz.as<float>(), ax.as<float>(), ay.as<float>(),
-                    az.as<float>(), float(dt), n);
+        if (mixed)
+            launch_kick(stream, D(vxd), D(vyd), D(vzd), D(axd), D(ayd), D(azd), dt, n);
+        else
+            launch_kick(stream, vx.as<float>(), vy.as<float>(), vz.as<float>(), ax.as<float>(), ay.as<float>(),
+                        az.as<float>(), float(dt), n);
         LHIP_CHECK(hipEventRecord(ev[6], stream));
         kick_rec = true;
     }
@@ -162,8 +288,11 @@ struct Context {
         LHIP_CHECK(hipSetDevice(dev));
         if (n == 0) return;
         LHIP_CHECK(hipEventRecord(ev[7], stream));
-        launch_drift(stream, x.as<float>(), y.as<float>(), z.as<float>(), vx.as<float>(), vy.as<float>(),
-                     vz.as<float>(), float(dt), n);
+        if (mixed)
+            launch_drift(stream, D(xd), D(yd), D(zd), D(vxd), D(vyd), D(vzd), dt, n);
+        else
+            launch_drift(stream, x.as<float>(), y.as<float>(), z.as<float>(), vx.as<float>(), vy.as<float>(),
+                         vz.as<float>(), float(dt), n);
         LHIP_CHECK(hipEventRecord(ev[8], stream));
         drift_rec = true;
         tree_ok = false;
@@ -183,29 +312,45 @@ struct Context {
         acc_ok = false;
         LHIP_CHECK(hipEventRecord(ev[0], stream));
         TreeBufs tb{};
-        tb.x = x.as<float>(); tb.y = y.as<float>(); tb.z = z.as<float>();
-        tb.vx = vx.as<float>(); tb.vy = vy.as<float>(); tb.vz = vz.as<float>();
+        tb.f64 = mixed;
+        if (mixed) {
+            tb.x = D(xd); tb.y = D(yd); tb.z = D(zd); tb.vx = D(vxd); tb.vy = D(vyd); tb.vz = D(vzd);
+            tb.x2 = D(xd2); tb.y2 = D(yd2); tb.z2 = D(zd2); tb.vx2 = D(vxd2); tb.vy2 = D(vyd2); tb.vz2 = D(vzd2);
+            tb.posm = nullptr;
+        } else {
+            tb.x = F(x); tb.y = F(y); tb.z = F(z); tb.vx = F(vx); tb.vy = F(vy); tb.vz = F(vz);
+            tb.x2 = F(x2); tb.y2 = F(y2); tb.z2 = F(z2); tb.vx2 = F(vx2); tb.vy2 = F(vy2); tb.vz2 = F(vz2);
+            tb.posm = posm.as<float4>();
+        }
         tb.m = m.as<float>(); tb.id = id.as<uint32_t>(); tb.cost = cost.as<float>();
-        tb.x2 = x2.as<float>(); tb.y2 = y2.as<float>(); tb.z2 = z2.as<float>();
-        tb.vx2 = vx2.as<float>(); tb.vy2 = vy2.as<float>(); tb.vz2 = vz2.as<float>();
         tb.m2 = m2.as<float>(); tb.id2 = id2.as<uint32_t>(); tb.cost2 = cost2.as<float>();
-        tb.posm = posm.as<float4>();
         tb.key = key.as<uint64_t>(); tb.key2 = key2.as<uint64_t>();
         tb.idx = idx.as<uint32_t>(); tb.idx2 = idx2.as<uint32_t>();
         tb.tile_hist = tile_hist.as<uint32_t>();
         tb.cube = cube.as<Cube>();
-        tb.bbox_part = bbox_part.as<float>();
+        tb.bbox_part = bbox_part.as<void>();
         tree_sort(stream, tb, n, keep_cost);
         // The permute wrote every body array into its alternate: swap the owners.
-        swap(x, x2); swap(y, y2); swap(z, z2); swap(vx, vx2); swap(vy, vy2); swap(vz, vz2);
+        if (mixed) {
+            swap(xd, xd2); swap(yd, yd2); swap(zd, zd2); swap(vxd, vxd2); swap(vyd, vyd2); swap(vzd, vzd2);
+        } else {
+            swap(x, x2); swap(y, y2); swap(z, z2); swap(vx, vx2); swap(vy, vy2); swap(vz, vz2);
+        }
         swap(m, m2); swap(id, id2);
         if (keep_cost) swap(cost, cost2);
         LHIP_CHECK(hipEventRecord(ev[1], stream));
         BuildBufs bb{};
         bb.n = n;
         bb.cap_nodes = int(cap_nodes);
+        bb.gmax = gmax;
         bb.key = key.as<uint64_t>();
         bb.posm = posm.as<float4>();
+        if (mixed) {
+            momd.reserve(std::size_t(cap_nodes) * sizeof(NodeMomD));
+            bb.xd = D(xd); bb.yd = D(yd); bb.zd = D(zd);
+            bb.m = m.as<float>();
+            bb.momd = momd.as<NodeMomD>();
+        }
         bb.cube = cube.as<Cube>();
         bb.bl = bl.as<uint32_t>();
         bb.cnt = cnt.as<uint32_t>();
@@ -245,7 +390,19 @@ struct Context {
         const float G = float(p.G), eps2 = float(p.eps * p.eps);
         LHIP_CHECK(hipEventRecord(ev[3], stream));
         WalkCounters hc{};
-        if (!tree) {
+        if (!tree && mixed) {
+            direct_part.reserve(std::size_t(direct64_split(n)) * 4 * std::size_t(n) * 8);
+            rec64.reserve(std::size_t(n) * sizeof(SimRec));
+            launch_direct64(stream, D(xd), D(yd), D(zd), m.as<float>(), n, p.eps, p.G, potential, D(axd), D(ayd),
+                            D(azd), D(potd), rec64.as<SimRec>(), direct_part.as<double>());
+            LHIP_CHECK(hipEventRecord(ev[4], stream));
+            sync();
+            const uint64_t pairs = uint64_t(n) * uint64_t(n - 1);
+            hc.interactions = pairs;
+            hc.p2p = pairs;
+            hc.listed = uint64_t(n) * uint64_t(n);
+            direct_costs_pending = true;
+        } else if (!tree) {
             launch_pack_posm(stream, x.as<float>(), y.as<float>(), z.as<float>(), m.as<float>(), posm.as<float4>(), n);
             const int S = direct_split(n);
             if (S > 1) direct_part.reserve(std::size_t(S) * std::size_t(n) * sizeof(float4));
@@ -270,6 +427,13 @@ struct Context {
             a.G = G;
             a.ax = ax.as<float>(); a.ay = ay.as<float>(); a.az = az.as<float>(); a.pot = pot.as<float>();
             a.cost = cost.as<float>();
+            if (mixed) {
+                a.xd = D(xd); a.yd = D(yd); a.zd = D(zd);
+                a.m = m.as<float>();
+                a.momd = momd.as<NodeMomD>();
+                a.axd = D(axd); a.ayd = D(ayd); a.azd = D(azd); a.potd = D(potd);
+                a.Gd = p.G;
+            }
             a.counters = counters.as<WalkCounters>();
             a.blocks = walk_blocks;
             tree_walk(stream, a, potential);
@@ -307,9 +471,13 @@ struct Context {
         lhip_diag d{};
         d.has_pe = has_pe ? 1 : 0;
         if (n == 0) return d;
-        launch_diag(stream, x.as<float>(), y.as<float>(), z.as<float>(), vx.as<float>(), vy.as<float>(),
-                    vz.as<float>(), m.as<float>(), pot.as<float>(), has_pe ? 1 : 0, n, red_part.as<double>(),
-                    red_out.as<double>());
+        if (mixed)
+            launch_diag(stream, D(xd), D(yd), D(zd), D(vxd), D(vyd), D(vzd), m.as<float>(), D(potd), has_pe ? 1 : 0, n,
+                        red_part.as<double>(), red_out.as<double>());
+        else
+            launch_diag(stream, x.as<float>(), y.as<float>(), z.as<float>(), vx.as<float>(), vy.as<float>(),
+                        vz.as<float>(), m.as<float>(), pot.as<float>(), has_pe ? 1 : 0, n, red_part.as<double>(),
+                        red_out.as<double>());
         double v[12];
         LHIP_CHECK(hipMemcpyAsync(v, red_out.as<void>(), sizeof v, hipMemcpyDeviceToHost, stream));
         sync();
@@ -325,10 +493,15 @@ struct Context {
     double exact_pe(double G, double eps) {
         LHIP_CHECK(hipSetDevice(dev));
         if (n < 2) return 0.0;
-        launch_pack_posm(stream, x.as<float>(), y.as<float>(), z.as<float>(), m.as<float>(), posm.as<float4>(), n);
         const std::size_t blocks = grid_for(n, kBS);
         red_part.reserve(std::max<std::size_t>(blocks, 12 * kReduceBlocks) * sizeof(double));
-        launch_exact_pe(stream, posm.as<float4>(), n, eps * eps, red_part.as<double>(), red_out.as<double>());
+        if (mixed) {
+            launch_exact_pe(stream, D(xd), D(yd), D(zd), m.as<float>(), n, eps * eps, red_part.as<double>(),
+                            red_out.as<double>());
+        } else {
+            launch_pack_posm(stream, x.as<float>(), y.as<float>(), z.as<float>(), m.as<float>(), posm.as<float4>(), n);
+            launch_exact_pe(stream, posm.as<float4>(), n, eps * eps, red_part.as<double>(), red_out.as<double>());
+        }
         double h = 0;
         LHIP_CHECK(hipMemcpyAsync(&h, red_out.as<void>(), sizeof h, hipMemcpyDeviceToHost, stream));
         sync();
@@ -349,7 +522,8 @@ struct Context {
         LHIP_CHECK(hipSetDevice(dev));
         for (int c = 0; c < 3; ++c) lo[c] = hi[c] = 0.f;
         if (n == 0) return;
-        launch_bbox(stream, x.as<float>(), y.as<float>(), z.as<float>(), n, bbox_part.as<float>(), cube.as<Cube>());
+        if (mixed) launch_bbox(stream, D(xd), D(yd), D(zd), n, bbox_part.as<void>(), cube.as<Cube>());
+        else launch_bbox(stream, F(x), F(y), F(z), n, bbox_part.as<void>(), cube.as<Cube>());
         Cube c{};
         LHIP_CHECK(hipMemcpyAsync(&c, cube.as<void>(), sizeof c, hipMemcpyDeviceToHost, stream));
         sync();
@@ -375,7 +549,9 @@ struct Context {
         std::size_t held = 0;
         for (const DevBuf* b : {&x, &y, &z, &vx, &vy, &vz, &m, &id, &cost, &x2, &y2, &z2, &vx2, &vy2, &vz2, &m2, &id2,
                                 &cost2, &ax, &ay, &az, &pot, &posm, &key, &key2, &idx, &idx2, &tile_hist, &direct_part,
-                                &bl, &cnt, &gflag, &scan_tmp, &pkey, &pkey2, &pval, &pval2, &inv, &node, &mom, &prefix, &gstart})
+                                &bl, &cnt, &gflag, &scan_tmp, &pkey, &pkey2, &pval, &pval2, &inv, &node, &mom, &prefix, &gstart,
+                                &xd, &yd, &zd, &vxd, &vyd, &vzd, &xd2, &yd2, &zd2, &vxd2, &vyd2, &vzd2, &axd, &ayd, &azd,
+                                &potd, &momd, &rec64})
             held += b->capacity();
         *fr = f + held;
         *tot = tt;

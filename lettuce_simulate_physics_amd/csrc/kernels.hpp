@@ -6,22 +6,35 @@
 namespace lhip {
 
 // ---- state (kernels_state.hip) ----
+// float: the plain state; double: the mixed-precision form's FP64 state.
 void launch_kick(hipStream_t s, float* vx, float* vy, float* vz, const float* ax, const float* ay, const float* az,
                  float dt, int n);
 void launch_drift(hipStream_t s, float* x, float* y, float* z, const float* vx, const float* vy, const float* vz,
                   float dt, int n);
+void launch_kick(hipStream_t s, double* vx, double* vy, double* vz, const double* ax, const double* ay,
+                 const double* az, double dt, int n);
+void launch_drift(hipStream_t s, double* x, double* y, double* z, const double* vx, const double* vy,
+                  const double* vz, double dt, int n);
+void launch_f32_to_f64(hipStream_t s, const float* a, double* b, int n);
+void launch_f64_to_f32(hipStream_t s, const double* a, float* b, int n);
 void launch_pack_posm(hipStream_t s, const float* x, const float* y, const float* z, const float* m, float4* posm,
                       int n);
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, long long n);
 void launch_zero_f32(hipStream_t s, float* p, long long n);
-// Bounding box of (x, y, z) into cube (float lo/hi, and the Morton cube in double when want_cube).
-// part: kReduceBlocks * 6 floats of scratch.
-void launch_bbox(hipStream_t s, const float* x, const float* y, const float* z, int n, float* part, Cube* cube);
+// Bounding box of (x, y, z) into cube (float lo/hi, and the Morton cube in double).
+// part: kReduceBlocks * 6 values of scratch (float or double as the positions).
+void launch_bbox(hipStream_t s, const float* x, const float* y, const float* z, int n, void* part, Cube* cube);
+void launch_bbox(hipStream_t s, const double* x, const double* y, const double* z, int n, void* part, Cube* cube);
 // FP64 reductions: mass, ke, sum m*phi, p, L, sum m*x. part: kReduceBlocks * 12 doubles, out: 12 doubles.
 void launch_diag(hipStream_t s, const float* x, const float* y, const float* z, const float* vx, const float* vy,
                  const float* vz, const float* m, const float* pot, int use_pot, int n, double* part, double* out);
+void launch_diag(hipStream_t s, const double* x, const double* y, const double* z, const double* vx,
+                 const double* vy, const double* vz, const float* m, const double* pot, int use_pot, int n,
+                 double* part, double* out);
 // Exact potential energy sum_i m_i sum_{j != i} m_j / s_ij in double. part: grid doubles; out: 1 double.
 void launch_exact_pe(hipStream_t s, const float4* posm, int n, double eps2, double* part, double* out);
+void launch_exact_pe(hipStream_t s, const double* x, const double* y, const double* z, const float* m, int n,
+                     double eps2, double* part, double* out);
 
 // ---- direct summation (kernels_direct.hip) ----
 // Accelerations G sum_j m_j d / (d^2 + eps^2)^1.5 (and phi = -G sum m_j / s), self skipped by index.
@@ -32,21 +45,23 @@ void launch_direct(hipStream_t s, const float4* posm, int n, float eps2, float G
 
 // ---- tree (kernels_tree.hip) ----
 struct TreeBufs {
+    bool f64;  // positions and velocities are double (the mixed form), else float
     // bodies (sorted order after the permute)
-    float *x, *y, *z, *vx, *vy, *vz, *m;
+    void *x, *y, *z, *vx, *vy, *vz;
+    float* m;
     uint32_t* id;
     float* cost;  // per-body cost of the last walk (permuted with the bodies), may be null
     // alternates for the permute (same sizes)
-    float *x2, *y2, *z2, *vx2, *vy2, *vz2, *m2;
+    void *x2, *y2, *z2, *vx2, *vy2, *vz2;
+    float* m2;
     uint32_t* id2;
     float* cost2;
-    float4* posm;
+    float4* posm;  // float (position, mass) in sorted order, may be null
     uint64_t *key, *key2;
     uint32_t *idx, *idx2;
     uint32_t* tile_hist;  // radix tile histograms
-    uint32_t* digit_off;  // radix digit offsets
     Cube* cube;
-    float* bbox_part;
+    void* bbox_part;      // kReduceBlocks * 6 values of the position type
 };
 
 // Morton keys, stable radix sort, permute of every body array into key order (swaps the caller's

@@ -50,6 +50,19 @@ __global__ void __launch_bounds__(kBS) pack_kernel(const double* __restrict__ x,
     }
 }
 
+// rec <- (x, m) with float masses (the mixed-precision backend state).
+__global__ void __launch_bounds__(kBS) pack32_kernel(const double* __restrict__ x, const double* __restrict__ y,
+                                                     const double* __restrict__ z, const float* __restrict__ m,
+                                                     SimRec* __restrict__ rec, int n) {
+    const int i = blockIdx.x * kBS + threadIdx.x;
+    if (i < n) {
+        SimRec r;
+        r.x = x[i]; r.y = y[i]; r.z = z[i];
+        r.m = m[i]; r.ux = r.uy = r.uz = 0.f; r.pad[0] = r.pad[1] = 0.f;
+        rec[i] = r;
+    }
+}
+
 // Opening half of a KDK step: v += h a; x += dt v; rec <- (x, m); optionally a copy of x.
 __global__ void __launch_bounds__(kBS) kick_drift_kernel(double* __restrict__ x, double* __restrict__ y,
                                                          double* __restrict__ z, double* __restrict__ vx,
@@ -159,10 +172,11 @@ __global__ void __launch_bounds__(kBS) fill64_kernel(double* __restrict__ y, dou
 // blockIdx.y selects the source range [split * jchunk, ...), jchunk a multiple of 64, so every block
 // of 64 sources either holds all of the wave's own bodies for some k or none of them.
 
-template <int T, bool GUARD>
+template <int T, bool GUARD, bool POT, bool MASK>
 __device__ __forceinline__ void force_block(const SimRec* __restrict__ rec, int jb, int je, float e2,
                                             const double (&px)[T], const double (&py)[T], const double (&pz)[T],
-                                            const int (&ii)[T], float (&fx)[T], float (&fy)[T], float (&fz)[T]) {
+                                            const int (&ii)[T], float (&fx)[T], float (&fy)[T], float (&fz)[T],
+                                            float (&fp)[T]) {
 #pragma unroll 4
     for (int j = jb; j < je; ++j) {
         const double qx = rec[j].x, qy = rec[j].y, qz = rec[j].z;
@@ -172,51 +186,78 @@ __device__ __forceinline__ void force_block(const SimRec* __restrict__ rec, int 
             const float dx = float(qx - px[k]), dy = float(qy - py[k]), dz = float(qz - pz[k]);
             const float r2 = __fmaf_rn(dz, dz, __fmaf_rn(dy, dy, __fmaf_rn(dx, dx, e2)));
             float inv = __builtin_amdgcn_rsqf(r2);
-            if (GUARD) inv = (j != ii[k] && r2 > 0.f) ? inv : 0.f;
-            const float f = (qm * inv) * (inv * inv);
+            if (MASK) inv = (j != ii[k] && (!GUARD || r2 > 0.f)) ? inv : 0.f;
+            const float mi = qm * inv;
+            const float f = mi * (inv * inv);
             fx[k] = __fmaf_rn(f, dx, fx[k]);
             fy[k] = __fmaf_rn(f, dy, fy[k]);
             fz[k] = __fmaf_rn(f, dz, fz[k]);
+            if (POT) fp[k] += mi;
         }
     }
 }
 
-// Mixed-precision direct force; writes the split's partial sums (without G).
-template <int T, bool GUARD>
+// Mixed-precision direct force (and with POT the potential sum); writes the split's partial sums
+// (without G): 3 components, or 4 with POT.
+template <int T, bool GUARD, bool POT>
 __global__ void __launch_bounds__(kAdjBS) force_mixed_kernel(const SimRec* __restrict__ rec, int n, float e2,
                                                              int jchunk, double* __restrict__ part) {
+    constexpr int C = POT ? 4 : 3;
     const int lane = lane_id();
     const int t0 = (blockIdx.x * kWaves + int(threadIdx.x >> 6)) * 64 * T;
     if (t0 >= n) return;  // wave-uniform
     const int split = blockIdx.y;
-    double px[T], py[T], pz[T], sx[T], sy[T], sz[T];
-    float fx[T], fy[T], fz[T];
+    double px[T], py[T], pz[T], sx[T], sy[T], sz[T], sp[T];
+    float fx[T], fy[T], fz[T], fp[T];
     int ii[T];
 #pragma unroll
     for (int k = 0; k < T; ++k) {
         ii[k] = t0 + 64 * k + lane;
         const int c = ii[k] < n ? ii[k] : n - 1;
         px[k] = rec[c].x; py[k] = rec[c].y; pz[k] = rec[c].z;
-        sx[k] = sy[k] = sz[k] = 0.0;
-        fx[k] = fy[k] = fz[k] = 0.f;
+        sx[k] = sy[k] = sz[k] = sp[k] = 0.0;
+        fx[k] = fy[k] = fz[k] = fp[k] = 0.f;
     }
     const int j0 = split * jchunk, j1 = min(n, j0 + jchunk);
+    const int t1 = t0 + 64 * T;
     for (int jb = j0; jb < j1; jb += kFold) {
-        force_block<T, GUARD>(rec, jb, min(j1, jb + kFold), e2, px, py, pz, ii, fx, fy, fz);
+        const int je = min(j1, jb + kFold);
+        // The potential's self term is not zero: mask the block holding the wave's own bodies.
+        if (GUARD || (POT && jb < t1 && je > t0))
+            force_block<T, GUARD, POT, true>(rec, jb, je, e2, px, py, pz, ii, fx, fy, fz, fp);
+        else
+            force_block<T, false, POT, false>(rec, jb, je, e2, px, py, pz, ii, fx, fy, fz, fp);
 #pragma unroll
         for (int k = 0; k < T; ++k) {
-            sx[k] += fx[k]; sy[k] += fy[k]; sz[k] += fz[k];
-            fx[k] = fy[k] = fz[k] = 0.f;
+            sx[k] += fx[k]; sy[k] += fy[k]; sz[k] += fz[k]; sp[k] += fp[k];
+            fx[k] = fy[k] = fz[k] = fp[k] = 0.f;
         }
     }
 #pragma unroll
     for (int k = 0; k < T; ++k) {
         const int i = ii[k];
         if (i >= n) continue;
-        part[std::size_t(3 * split + 0) * n + i] = sx[k];
-        part[std::size_t(3 * split + 1) * n + i] = sy[k];
-        part[std::size_t(3 * split + 2) * n + i] = sz[k];
+        part[std::size_t(C * split + 0) * n + i] = sx[k];
+        part[std::size_t(C * split + 1) * n + i] = sy[k];
+        part[std::size_t(C * split + 2) * n + i] = sz[k];
+        if (POT) part[std::size_t(C * split + 3) * n + i] = sp[k];
     }
+}
+
+// a = G * sum of the split partials, phi = -G * sum (with the potential); in split order.
+__global__ void __launch_bounds__(kBS) direct64_finish_kernel(const double* __restrict__ part, int n, int S, int C,
+                                                              double G, double* __restrict__ ax,
+                                                              double* __restrict__ ay, double* __restrict__ az,
+                                                              double* __restrict__ pot) {
+    const int i = blockIdx.x * kBS + threadIdx.x;
+    if (i >= n) return;
+    double s[4] = {0, 0, 0, 0};
+    for (int sp = 0; sp < S; ++sp)
+        for (int c = 0; c < C; ++c) s[c] += part[std::size_t(C * sp + c) * n + i];
+    ax[i] = G * s[0];
+    ay[i] = G * s[1];
+    az[i] = G * s[2];
+    if (C == 4) pot[i] = -G * s[3];
 }
 
 // All-FP64 direct force (pair arithmetic in double, correctly rounded sqrt and division; FP64 masses).
@@ -381,11 +422,35 @@ void launch_force_finish(hipStream_t s, const SimRec* rec, const double* m, int 
         S = sp.S;
         const float e2 = float(eps * eps);
         const dim3 grid(wave_blocks(n, kT), S);
-        if (e2 > 0.f) force_mixed_kernel<kT, false><<<grid, kAdjBS, 0, s>>>(rec, n, e2, sp.jchunk, part);
-        else force_mixed_kernel<kT, true><<<grid, kAdjBS, 0, s>>>(rec, n, e2, sp.jchunk, part);
+        if (e2 > 0.f) force_mixed_kernel<kT, false, false><<<grid, kAdjBS, 0, s>>>(rec, n, e2, sp.jchunk, part);
+        else force_mixed_kernel<kT, true, false><<<grid, kAdjBS, 0, s>>>(rec, n, e2, sp.jchunk, part);
     }
     LHIP_LAUNCH_CHECK();
     finish_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(part, n, S, G, h, ax, ay, az, vx, vy, vz);
+    LHIP_LAUNCH_CHECK();
+}
+
+int direct64_split(int n) { return split_for(n, kT).S; }
+
+void launch_direct64(hipStream_t s, const double* x, const double* y, const double* z, const float* m, int n,
+                     double eps, double G, bool potential, double* ax, double* ay, double* az, double* pot,
+                     SimRec* rec, double* part) {
+    if (n <= 0) return;
+    pack32_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(x, y, z, m, rec, n);
+    LHIP_LAUNCH_CHECK();
+    const Split sp = split_for(n, kT);
+    const float e2 = float(eps * eps);
+    const dim3 grid(wave_blocks(n, kT), sp.S);
+    const bool guard = !(e2 > 0.f);
+    if (potential) {
+        if (guard) force_mixed_kernel<kT, true, true><<<grid, kAdjBS, 0, s>>>(rec, n, e2, sp.jchunk, part);
+        else force_mixed_kernel<kT, false, true><<<grid, kAdjBS, 0, s>>>(rec, n, e2, sp.jchunk, part);
+    } else {
+        if (guard) force_mixed_kernel<kT, true, false><<<grid, kAdjBS, 0, s>>>(rec, n, e2, sp.jchunk, part);
+        else force_mixed_kernel<kT, false, false><<<grid, kAdjBS, 0, s>>>(rec, n, e2, sp.jchunk, part);
+    }
+    LHIP_LAUNCH_CHECK();
+    direct64_finish_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(part, n, sp.S, potential ? 4 : 3, G, ax, ay, az, pot);
     LHIP_LAUNCH_CHECK();
 }
 

@@ -29,11 +29,12 @@ constexpr int kLvTile = 256;
 constexpr int kHalo = kGroupMax;  // 128
 
 // Per sorted body i: L = leaf level (1 + max common prefix over 17-body windows holding i, capped at
-// 22 = "more than 16 identical keys"), G = group level (the same with 129-body windows, never deeper
-// than the leaf), c = common prefix with body i - 1 (-1 for i = 0).
+// 22 = "more than 16 identical keys"), G = group level (the same with (gmax + 1)-body windows, never
+// deeper than the leaf; gmax = 128 is the reference's group), c = common prefix with body i - 1
+// (-1 for i = 0).
 // cnt[i] = number of octree cells that start at body i = levels (c, min(L, 21)].
 // gflag[i] = 1 iff the group cell of body i starts at i.
-__global__ void __launch_bounds__(kLvTile) levels_kernel(const uint64_t* __restrict__ key, int n,
+__global__ void __launch_bounds__(kLvTile) levels_kernel(const uint64_t* __restrict__ key, int n, int gmax,
                                                          uint32_t* __restrict__ bl, uint32_t* __restrict__ cnt,
                                                          uint32_t* __restrict__ gflag, WalkCounters* __restrict__ ctr) {
     __shared__ uint64_t sk[kHalo + kLvTile + kHalo];
@@ -49,7 +50,7 @@ __global__ void __launch_bounds__(kLvTile) levels_kernel(const uint64_t* __restr
     for (int t = tid; t < kHalo + kLvTile; t += kLvTile) {
         const long long j = (long long)base - kHalo + t;
         d16[t] = (j >= 0 && j + kLeafMax < n) ? uint8_t(common_octs(sk[t], sk[t + kLeafMax])) : uint8_t(0xff);
-        d128[t] = (j >= 0 && j + kGroupMax < n) ? uint8_t(common_octs(sk[t], sk[t + kGroupMax])) : uint8_t(0xff);
+        d128[t] = (j >= 0 && j + gmax < n) ? uint8_t(common_octs(sk[t], sk[t + gmax])) : uint8_t(0xff);
     }
     __syncthreads();
     const int i = base + tid;
@@ -62,8 +63,8 @@ __global__ void __launch_bounds__(kLvTile) levels_kernel(const uint64_t* __restr
         }
         const int L = best + 1;
         best = -1;
-        for (int u = 0; u <= kGroupMax; ++u) {
-            const int v = d128[tid + u];
+        for (int u = 0; u <= gmax; ++u) {
+            const int v = d128[tid + kHalo - gmax + u];
             if (v != 0xff) best = max(best, v);
         }
         const int Lc = min(L, kMaxLevel);
@@ -278,11 +279,18 @@ __device__ __forceinline__ float rcrit2_of(float cx, float cy, float cz, float g
 // ends at the first body whose boundary level c is below the leaf's level (at most 16 steps except in
 // an over-full finest cell); an internal node's children are the contiguous run of next-level nodes
 // from `first` whose key prefix extends the node's, and the node ends where its last child ends.
+// MIXED: bodies are FP64 positions (xd, yd, zd) with float masses, and each node's FP64 centre of
+// mass, mass and quadrupole go to momd as well; internal nodes combine their children's FP64
+// moments, so the quadrupoles of the root levels are accumulated in FP64 throughout.
+template <bool MIXED>
 __global__ void __launch_bounds__(kBS) moments_level(int level, int n, const float4* __restrict__ posm,
+                                                     const double* __restrict__ xd, const double* __restrict__ yd,
+                                                     const double* __restrict__ zd, const float* __restrict__ bm,
                                                      const uint32_t* __restrict__ bl,
                                                      const uint32_t* __restrict__ levstart,
                                                      const uint64_t* __restrict__ prefix, WalkNode* __restrict__ node,
-                                                     NodeMom* __restrict__ mom, float inv_theta) {
+                                                     NodeMom* __restrict__ mom, NodeMomD* __restrict__ momd,
+                                                     float inv_theta) {
     const int p0 = int(levstart[level]), p1 = int(levstart[level + 1]);
     const int cend = int(levstart[level + 2]);  // end of the next level
     for (int p = p0 + blockIdx.x * kBS + threadIdx.x; p < p1; p += gridDim.x * kBS) {
@@ -292,24 +300,33 @@ __global__ void __launch_bounds__(kBS) moments_level(int level, int n, const flo
         double q[6] = {0, 0, 0, 0, 0, 0};
         double cx, cy, cz;
         int nchild = 0;
+        auto body = [&](int j, double& x, double& y, double& z, double& m) {
+            if (MIXED) {
+                x = xd[j]; y = yd[j]; z = zd[j]; m = bm[j];
+            } else {
+                const float4 b = posm[j];
+                x = b.x; y = b.y; z = b.z; m = b.w;
+            }
+        };
         if (info_leaf(nd.info)) {
             int hi = nd.lo + 1;
             while (hi < n && int((bl[hi] >> 16) & 0xff) - 1 >= level) ++hi;
             nd.hi = hi;
             for (int j = nd.lo; j < nd.hi; ++j) {
-                const float4 b = posm[j];
-                const double m = b.w;
+                double bx, by, bz, m;
+                body(j, bx, by, bz, m);
                 M += m;
-                sx = fma(m, double(b.x), sx);
-                sy = fma(m, double(b.y), sy);
-                sz = fma(m, double(b.z), sz);
+                sx = fma(m, bx, sx);
+                sy = fma(m, by, sy);
+                sz = fma(m, bz, sz);
             }
             cx = M > 0 ? sx / M : double(md.gx);
             cy = M > 0 ? sy / M : double(md.gy);
             cz = M > 0 ? sz / M : double(md.gz);
             for (int j = nd.lo; j < nd.hi; ++j) {
-                const float4 b = posm[j];
-                const double m = b.w, dx = b.x - cx, dy = b.y - cy, dz = b.z - cz;
+                double bx, by, bz, m;
+                body(j, bx, by, bz, m);
+                const double dx = bx - cx, dy = by - cy, dz = bz - cz;
                 const double r2 = fma(dz, dz, fma(dx, dx, dy * dy));
                 q[0] = fma(m, fma(3 * dx, dx, -r2), q[0]);
                 q[3] = fma(m, fma(3 * dy, dy, -r2), q[3]);
@@ -323,12 +340,18 @@ __global__ void __launch_bounds__(kBS) moments_level(int level, int n, const flo
             const uint64_t pre = prefix[p];
             int ch = nd.first;
             while (ch < cend && (prefix[ch] >> 3) == pre) {
-                const NodeMom& c = mom[ch];
-                const double m = c.m;
+                double m, x, y, z;
+                if (MIXED) {
+                    const NodeMomD& c = momd[ch];
+                    m = c.m; x = c.cx; y = c.cy; z = c.cz;
+                } else {
+                    const NodeMom& c = mom[ch];
+                    m = c.m; x = c.cx; y = c.cy; z = c.cz;
+                }
                 M += m;
-                sx = fma(m, double(c.cx), sx);
-                sy = fma(m, double(c.cy), sy);
-                sz = fma(m, double(c.cz), sz);
+                sx = fma(m, x, sx);
+                sy = fma(m, y, sy);
+                sz = fma(m, z, sz);
                 ++ch;
             }
             nchild = ch - nd.first;
@@ -337,16 +360,30 @@ __global__ void __launch_bounds__(kBS) moments_level(int level, int n, const flo
             cy = M > 0 ? sy / M : double(md.gy);
             cz = M > 0 ? sz / M : double(md.gz);
             for (int k = nd.first; k < ch; ++k) {
-                const NodeMom& c = mom[k];
-                const double m = c.m, dx = c.cx - cx, dy = c.cy - cy, dz = c.cz - cz;
+                double m, dx, dy, dz, cq[6];
+                if (MIXED) {
+                    const NodeMomD& c = momd[k];
+                    m = c.m; dx = c.cx - cx; dy = c.cy - cy; dz = c.cz - cz;
+                    for (int u = 0; u < 6; ++u) cq[u] = c.q[u];
+                } else {
+                    const NodeMom& c = mom[k];
+                    m = c.m; dx = c.cx - cx; dy = c.cy - cy; dz = c.cz - cz;
+                    cq[0] = c.qxx; cq[1] = c.qxy; cq[2] = c.qxz; cq[3] = c.qyy; cq[4] = c.qyz; cq[5] = c.qzz;
+                }
                 const double r2 = fma(dz, dz, fma(dx, dx, dy * dy));
-                q[0] += c.qxx + m * (3 * dx * dx - r2);
-                q[3] += c.qyy + m * (3 * dy * dy - r2);
-                q[5] += c.qzz + m * (3 * dz * dz - r2);
-                q[1] += c.qxy + m * 3 * dx * dy;
-                q[2] += c.qxz + m * 3 * dx * dz;
-                q[4] += c.qyz + m * 3 * dy * dz;
+                q[0] += cq[0] + m * (3 * dx * dx - r2);
+                q[3] += cq[3] + m * (3 * dy * dy - r2);
+                q[5] += cq[5] + m * (3 * dz * dz - r2);
+                q[1] += cq[1] + m * 3 * dx * dy;
+                q[2] += cq[2] + m * 3 * dx * dz;
+                q[4] += cq[4] + m * 3 * dy * dz;
             }
+        }
+        if (MIXED) {
+            NodeMomD d;
+            d.cx = cx; d.cy = cy; d.cz = cz; d.m = M;
+            for (int u = 0; u < 6; ++u) d.q[u] = q[u];
+            momd[p] = d;
         }
         md.cx = float(cx); md.cy = float(cy); md.cz = float(cz); md.m = float(M);
         md.qxx = float(q[0]); md.qxy = float(q[1]); md.qxz = float(q[2]);
@@ -370,7 +407,8 @@ void tree_build(hipStream_t s, const BuildBufs& b, float inv_theta) {
     const int n = b.n;
     reset_build<<<1, 64, 0, s>>>(b.counters);
     LHIP_LAUNCH_CHECK();
-    levels_kernel<<<grid_for(n, kLvTile), kLvTile, 0, s>>>(b.key, n, b.bl, b.cnt, b.gflag, b.counters);
+    if (b.gmax < kLeafMax || b.gmax > kGroupMax) throw Error(kInvalid, "walk group size must be in [16, 128]");
+    levels_kernel<<<grid_for(n, kLvTile), kLvTile, 0, s>>>(b.key, n, b.gmax, b.bl, b.cnt, b.gflag, b.counters);
     LHIP_LAUNCH_CHECK();
     scan_exclusive(s, b.cnt, n, b.scan_tmp);
     scan_exclusive(s, b.gflag, n, b.scan_tmp);
@@ -389,7 +427,12 @@ void tree_build(hipStream_t s, const BuildBufs& b, float inv_theta) {
     LHIP_LAUNCH_CHECK();
     const unsigned g = std::min<unsigned>(grid_for(n, kBS), 512u);
     for (int l = kMaxLevel; l >= 0; --l) {
-        moments_level<<<g, kBS, 0, s>>>(l, n, b.posm, b.bl, b.levstart, b.prefix, b.node, b.mom, inv_theta);
+        if (b.momd)
+            moments_level<true><<<g, kBS, 0, s>>>(l, n, nullptr, b.xd, b.yd, b.zd, b.m, b.bl, b.levstart, b.prefix,
+                                                  b.node, b.mom, b.momd, inv_theta);
+        else
+            moments_level<false><<<g, kBS, 0, s>>>(l, n, b.posm, nullptr, nullptr, nullptr, nullptr, b.bl, b.levstart,
+                                                   b.prefix, b.node, b.mom, nullptr, inv_theta);
         LHIP_LAUNCH_CHECK();
     }
 }

@@ -16,10 +16,11 @@
 //    the body just past the cell, and its first child (if any) is k + 1.
 //  * Multipoles level by level, deepest first, each level one launch (kernel boundaries order the
 //    dependent levels without cross-XCD cache maintenance inside a kernel).
-//  * Walk: one wavefront per group. The wave tests 64 consecutive pre-order nodes at once; a node is
-//    skipped iff an earlier node of the window that ends the descent (accepted or leaf) has a skip
-//    pointer past it, which is an exclusive prefix-max over the lanes. The next window starts at the
-//    larger of (window end, largest such skip). No stack.
+//  * Walk: one wavefront per group, a parallel depth-first traversal with an LDS stack
+//    (kernels_walk.hip).
+//
+// Positions are float (the plain state, bit-exact keys with the reference) or double (the
+// mixed-precision form); the key recipe is the same.
 #include <cfloat>
 
 #include "kernels.hpp"
@@ -43,15 +44,17 @@ __device__ __forceinline__ uint64_t dilate21(uint64_t v) {
 
 // Quantisation in double with a clamp, as src/cpu_backend.cpp:419-425: (v - o) * 2^21/side, clamped
 // to [0, 2^21 - 1], truncated. Sub and mul stay separate IEEE operations.
-__device__ __forceinline__ uint64_t quantize(float v, double o, double s) {
+template <class R>
+__device__ __forceinline__ uint64_t quantize(R v, double o, double s) {
     double t = __dmul_rn(__dsub_rn(double(v), o), s);
     t = t < 0.0 ? 0.0 : t;
     t = t > 2097151.0 ? 2097151.0 : t;
     return uint64_t(t);
 }
 
-__global__ void __launch_bounds__(kBS) keys_kernel(const float* __restrict__ x, const float* __restrict__ y,
-                                                   const float* __restrict__ z, const Cube* __restrict__ cube,
+template <class R>
+__global__ void __launch_bounds__(kBS) keys_kernel(const R* __restrict__ x, const R* __restrict__ y,
+                                                   const R* __restrict__ z, const Cube* __restrict__ cube,
                                                    uint64_t* __restrict__ key, uint32_t* __restrict__ idx, int n) {
     const int i = blockIdx.x * kBS + threadIdx.x;
     if (i >= n) return;
@@ -223,26 +226,49 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter(const uint64_t* __
 
 // ---------------------------------------------------------------- permute
 
+template <class R>
 __global__ void __launch_bounds__(kBS) permute_kernel(const uint32_t* __restrict__ perm, int n,
-                                                      const float* __restrict__ x, const float* __restrict__ y,
-                                                      const float* __restrict__ z, const float* __restrict__ vx,
-                                                      const float* __restrict__ vy, const float* __restrict__ vz,
+                                                      const R* __restrict__ x, const R* __restrict__ y,
+                                                      const R* __restrict__ z, const R* __restrict__ vx,
+                                                      const R* __restrict__ vy, const R* __restrict__ vz,
                                                       const float* __restrict__ m, const uint32_t* __restrict__ id,
-                                                      const float* __restrict__ cost, float* __restrict__ x2,
-                                                      float* __restrict__ y2, float* __restrict__ z2,
-                                                      float* __restrict__ vx2, float* __restrict__ vy2,
-                                                      float* __restrict__ vz2, float* __restrict__ m2,
-                                                      uint32_t* __restrict__ id2, float* __restrict__ cost2,
-                                                      float4* __restrict__ posm) {
+                                                      const float* __restrict__ cost, R* __restrict__ x2,
+                                                      R* __restrict__ y2, R* __restrict__ z2, R* __restrict__ vx2,
+                                                      R* __restrict__ vy2, R* __restrict__ vz2,
+                                                      float* __restrict__ m2, uint32_t* __restrict__ id2,
+                                                      float* __restrict__ cost2, float4* __restrict__ posm) {
     const int i = blockIdx.x * kBS + threadIdx.x;
     if (i >= n) return;
     const uint32_t j = perm[i];
-    const float px = x[j], py = y[j], pz = z[j], pm = m[j];
+    const R px = x[j], py = y[j], pz = z[j];
+    const float pm = m[j];
     x2[i] = px; y2[i] = py; z2[i] = pz; m2[i] = pm;
     vx2[i] = vx[j]; vy2[i] = vy[j]; vz2[i] = vz[j];
     id2[i] = id[j];
     if (cost) cost2[i] = cost[j];
-    posm[i] = make_float4(px, py, pz, pm);
+    if (posm) posm[i] = make_float4(float(px), float(py), float(pz), pm);
+}
+
+template <class R>
+void sort_impl(hipStream_t s, TreeBufs& b, int n, bool permute_cost) {
+    const R *x = static_cast<const R*>(b.x), *y = static_cast<const R*>(b.y), *z = static_cast<const R*>(b.z);
+    launch_bbox(s, x, y, z, n, b.bbox_part, b.cube);
+    keys_kernel<R><<<grid_for(n, kBS), kBS, 0, s>>>(x, y, z, b.cube, b.key, b.idx, n);
+    LHIP_LAUNCH_CHECK();
+    uint64_t *ka = b.key, *kb = b.key2;
+    uint32_t *va = b.idx, *vb = b.idx2;
+    for (int shift = 0; shift < 63; shift += 8) {
+        radix_pass(s, ka, va, kb, vb, n, nullptr, shift, b.tile_hist);
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    // Eight passes: the sorted keys and permutation are back in key/idx.
+    auto P = [](void* p) { return static_cast<R*>(p); };
+    permute_kernel<R><<<grid_for(n, kBS), kBS, 0, s>>>(va, n, x, y, z, P(b.vx), P(b.vy), P(b.vz), b.m, b.id,
+                                                      permute_cost ? b.cost : nullptr, P(b.x2), P(b.y2), P(b.z2),
+                                                      P(b.vx2), P(b.vy2), P(b.vz2), b.m2, b.id2,
+                                                      permute_cost ? b.cost2 : nullptr, b.posm);
+    LHIP_LAUNCH_CHECK();
 }
 
 }  // namespace
@@ -266,21 +292,8 @@ std::size_t radix_tile_hist_entries(int n) {
 
 void tree_sort(hipStream_t s, TreeBufs& b, int n, bool permute_cost) {
     if (n <= 0) return;
-    launch_bbox(s, b.x, b.y, b.z, n, b.bbox_part, b.cube);
-    keys_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(b.x, b.y, b.z, b.cube, b.key, b.idx, n);
-    LHIP_LAUNCH_CHECK();
-    uint64_t *ka = b.key, *kb = b.key2;
-    uint32_t *va = b.idx, *vb = b.idx2;
-    for (int shift = 0; shift < 63; shift += 8) {
-        radix_pass(s, ka, va, kb, vb, n, nullptr, shift, b.tile_hist);
-        std::swap(ka, kb);
-        std::swap(va, vb);
-    }
-    // Eight passes: the sorted keys and permutation are back in key/idx.
-    permute_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(va, n, b.x, b.y, b.z, b.vx, b.vy, b.vz, b.m, b.id,
-                                                   permute_cost ? b.cost : nullptr, b.x2, b.y2, b.z2, b.vx2, b.vy2,
-                                                   b.vz2, b.m2, b.id2, permute_cost ? b.cost2 : nullptr, b.posm);
-    LHIP_LAUNCH_CHECK();
+    if (b.f64) sort_impl<double>(s, b, n, permute_cost);
+    else sort_impl<float>(s, b, n, permute_cost);
     std::swap(b.x, b.x2); std::swap(b.y, b.y2); std::swap(b.z, b.z2);
     std::swap(b.vx, b.vx2); std::swap(b.vy, b.vy2); std::swap(b.vz, b.vz2);
     std::swap(b.m, b.m2); std::swap(b.id, b.id2);

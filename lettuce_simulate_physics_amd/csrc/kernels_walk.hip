@@ -28,6 +28,13 @@
 // group meets itself exactly once (the leaves holding the group are always opened), which is how
 // the self pairs are taken out of the interaction counts.
 //
+// Mixed-precision form (REL). Positions and node centres of mass are FP64. The traversal is the
+// same (float group box and float node records); the evaluation runs on FP32 coordinates relative
+// to the group's box centre c: a body enters as float(x - c), a cell as float(com - c), both
+// differences formed in FP64, so a pair's separation keeps FP32 relative precision wherever the
+// group sits. Each slot's FP32 partial sums are folded into FP64 after every list evaluation (at
+// most 64 cells or 128 bodies per share), and the shares are added in FP64.
+//
 // Scheduling. A persistent grid; each wave takes groups from an atomic counter until none are left.
 #include <cfloat>
 
@@ -50,6 +57,11 @@ struct Slot {
     int sp;      // list share (slots without a share read share 0 and are discarded)
     bool on;
     float sx, sy, sz, sp_pot;
+    double dx, dy, dz, dp;  // FP64 carry of the mixed form
+};
+
+struct D4 {
+    double x, y, z, w;
 };
 
 constexpr int kPad = kMaxShare - 1;  // zero entries appended to a list before it is evaluated
@@ -161,9 +173,10 @@ constexpr int kStackCap = 1024;
 // Deepest possible tree: levels 0..21 (no key restarts).
 constexpr int kStackThr = kStackCap - 7 * kLevels - 8;
 
-template <bool POT, bool GUARD>
+template <bool POT, bool GUARD, bool REL>
 __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
     constexpr bool MASK = POT || GUARD;
+    static_assert(sizeof(WaveLds) >= 128 * sizeof(D4), "share reduction buffer");
     __shared__ WaveLds lds[kWalkWaves];
     __shared__ int stk[kWalkWaves][kStackCap];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -196,9 +209,12 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                 sl[s].on = s < NS && sl[s].sp < S;
                 if (!sl[s].on) sl[s].sp = 0;
                 sl[s].i = clo + sl[s].bi;
-                const float4 p = a.posm[sl[s].i];
+                float4 p;
+                if (REL) p = make_float4(float(a.xd[sl[s].i]), float(a.yd[sl[s].i]), float(a.zd[sl[s].i]), 0.f);
+                else p = a.posm[sl[s].i];
                 sl[s].x = p.x; sl[s].y = p.y; sl[s].z = p.z;
                 sl[s].sx = sl[s].sy = sl[s].sz = sl[s].sp_pot = 0.f;
+                sl[s].dx = sl[s].dy = sl[s].dz = sl[s].dp = 0.0;
                 mnx = fminf(mnx, p.x); mny = fminf(mny, p.y); mnz = fminf(mnz, p.z);
                 mxx = fmaxf(mxx, p.x); mxy = fmaxf(mxy, p.y); mxz = fmaxf(mxz, p.z);
             }
@@ -208,6 +224,27 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
             mxx = wave_max(mxx); mxy = wave_max(mxy); mxz = wave_max(mxz);
             const float bx = 0.5f * (mnx + mxx), by = 0.5f * (mny + mxy), bz = 0.5f * (mnz + mxz);
             const float hx = 0.5f * (mxx - mnx), hy = 0.5f * (mxy - mny), hz = 0.5f * (mxz - mnz);
+            // Mixed form: coordinates relative to c = the box centre.
+            const double ccx = bx, ccy = by, ccz = bz;
+            if (REL) {
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    const int i = sl[s].i;
+                    sl[s].x = float(a.xd[i] - ccx);
+                    sl[s].y = float(a.yd[i] - ccy);
+                    sl[s].z = float(a.zd[i] - ccz);
+                }
+            }
+            auto fold = [&]() {
+                if (REL) {
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        sl[s].dx += double(sl[s].sx); sl[s].dy += double(sl[s].sy);
+                        sl[s].dz += double(sl[s].sz); sl[s].dp += double(sl[s].sp_pot);
+                        sl[s].sx = sl[s].sy = sl[s].sz = sl[s].sp_pot = 0.f;
+                    }
+                }
+            };
 
             int nc = 0, nb = 0;
             uint32_t ncell_total = 0, nbody_total = 0;
@@ -220,6 +257,7 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                 __builtin_amdgcn_wave_barrier();
                 if (NS == 2) eval_cells<2, POT, GUARD>(sl, S, nc, L, e2);
                 else eval_cells<1, POT, GUARD>(sl, S, nc, L, e2);
+                fold();
                 __builtin_amdgcn_wave_barrier();
                 ncell_total += uint32_t(nc);
                 nc = 0;
@@ -232,6 +270,7 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                 __builtin_amdgcn_wave_barrier();
                 if (NS == 2) eval_bodies<2, POT, MASK, GUARD>(sl, S, nb, L, e2);
                 else eval_bodies<1, POT, MASK, GUARD>(sl, S, nb, L, e2);
+                fold();
                 __builtin_amdgcn_wave_barrier();
                 nbody_total += uint32_t(nb);
                 nb = 0;
@@ -282,7 +321,12 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                     if (is_cell) {
                         const int slot = nc + int(lanes_below(mc));
                         const NodeMom& md = a.mom[k];
-                        L.c[slot].A = make_float4(md.cx, md.cy, md.cz, md.m);
+                        if (REL) {
+                            const NodeMomD& dm = a.momd[k];
+                            L.c[slot].A = make_float4(float(dm.cx - ccx), float(dm.cy - ccy), float(dm.cz - ccz), md.m);
+                        } else {
+                            L.c[slot].A = make_float4(md.cx, md.cy, md.cz, md.m);
+                        }
                         L.c[slot].B = make_float4(md.qxx, md.qxy, md.qxz, md.qyy);
                         L.c[slot].C = make_float2(md.qyz, md.qzz);
                     }
@@ -302,7 +346,11 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                         const int u0 = max(off, done), u1 = min(off + len, done + take);
                         for (int u = u0; u < u1; ++u) {
                             const int j = nd.lo + (u - off);
-                            L.bP[nb + (u - done)] = a.posm[j];
+                            if (REL)
+                                L.bP[nb + (u - done)] = make_float4(float(a.xd[j] - ccx), float(a.yd[j] - ccy),
+                                                                    float(a.zd[j] - ccz), a.m[j]);
+                            else
+                                L.bP[nb + (u - done)] = a.posm[j];
                             L.bI[nb + (u - done)] = j;
                         }
                         nb += take;
@@ -314,29 +362,56 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
             flush_bodies();
 
             // Add the list shares of each body (slot v = share * nbody + body) through LDS.
-            float4* red = L.bP;  // kBodyCap >= 128 = 64 * NS entries
-            __builtin_amdgcn_wave_barrier();
-#pragma unroll
-            for (int s = 0; s < 2; ++s)
-                if (sl[s].on) red[sl[s].sp * nbody + sl[s].bi] = make_float4(sl[s].sx, sl[s].sy, sl[s].sz, sl[s].sp_pot);
-            __builtin_amdgcn_wave_barrier();
-            const float G = a.G;
             const uint32_t listed = ncell_total + nbody_total;
+            if (REL) {
+                D4* red = reinterpret_cast<D4*>(&L);  // the lists are done: their LDS holds 128 D4
+                __builtin_amdgcn_wave_barrier();
 #pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                const int v = s * 64 + lane;
-                if (v >= nbody) continue;
-                float4 acc4 = red[v];
-                for (int sp = 1; sp < S; ++sp) {
-                    const float4 q = red[sp * nbody + v];
-                    acc4.x += q.x; acc4.y += q.y; acc4.z += q.z; acc4.w += q.w;
+                for (int s = 0; s < 2; ++s)
+                    if (sl[s].on) red[sl[s].sp * nbody + sl[s].bi] = D4{sl[s].dx, sl[s].dy, sl[s].dz, sl[s].dp};
+                __builtin_amdgcn_wave_barrier();
+                const double G = a.Gd;
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    const int v = s * 64 + lane;
+                    if (v >= nbody) continue;
+                    D4 acc4 = red[v];
+                    for (int sp = 1; sp < S; ++sp) {
+                        const D4 q = red[sp * nbody + v];
+                        acc4.x += q.x; acc4.y += q.y; acc4.z += q.z; acc4.w += q.w;
+                    }
+                    const int i = clo + v;
+                    a.axd[i] = G * acc4.x;
+                    a.ayd[i] = G * acc4.y;
+                    a.azd[i] = G * acc4.z;
+                    if (POT) a.potd[i] = -G * acc4.w;
+                    if (a.cost) a.cost[i] = float(listed - 1u);
                 }
-                const int i = clo + v;
-                a.ax[i] = G * acc4.x;
-                a.ay[i] = G * acc4.y;
-                a.az[i] = G * acc4.z;
-                if (POT) a.pot[i] = -G * acc4.w;
-                if (a.cost) a.cost[i] = float(listed - 1u);
+            } else {
+                float4* red = L.bP;  // kBodyCap >= 128 = 64 * NS entries
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+                    if (sl[s].on)
+                        red[sl[s].sp * nbody + sl[s].bi] = make_float4(sl[s].sx, sl[s].sy, sl[s].sz, sl[s].sp_pot);
+                __builtin_amdgcn_wave_barrier();
+                const float G = a.G;
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    const int v = s * 64 + lane;
+                    if (v >= nbody) continue;
+                    float4 acc4 = red[v];
+                    for (int sp = 1; sp < S; ++sp) {
+                        const float4 q = red[sp * nbody + v];
+                        acc4.x += q.x; acc4.y += q.y; acc4.z += q.z; acc4.w += q.w;
+                    }
+                    const int i = clo + v;
+                    a.ax[i] = G * acc4.x;
+                    a.ay[i] = G * acc4.y;
+                    a.az[i] = G * acc4.z;
+                    if (POT) a.pot[i] = -G * acc4.w;
+                    if (a.cost) a.cost[i] = float(listed - 1u);
+                }
             }
             __builtin_amdgcn_wave_barrier();
             if (lane == 0) {
@@ -371,12 +446,25 @@ __global__ void reset_walk(WalkCounters* __restrict__ c) {
 
 int walk_blocks_per_cu() {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(walk_kernel<false, false>),
-                                                     kWalkBS, 0) != hipSuccess || blocks < 1) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks,
+                                                     reinterpret_cast<const void*>(walk_kernel<false, false, false>),
+                                                     kWalkBS, 0) != hipSuccess ||
+        blocks < 1) {
         (void)hipGetLastError();
         blocks = 2;
     }
     return blocks;
+}
+
+template <bool REL>
+void walk_launch(hipStream_t s, const WalkArgs& a, bool potential, bool guard, int blocks) {
+    if (potential) {
+        if (guard) walk_kernel<true, true, REL><<<blocks, kWalkBS, 0, s>>>(a);
+        else walk_kernel<true, false, REL><<<blocks, kWalkBS, 0, s>>>(a);
+    } else {
+        if (guard) walk_kernel<false, true, REL><<<blocks, kWalkBS, 0, s>>>(a);
+        else walk_kernel<false, false, REL><<<blocks, kWalkBS, 0, s>>>(a);
+    }
 }
 
 void tree_walk(hipStream_t s, const WalkArgs& a, bool potential) {
@@ -385,13 +473,8 @@ void tree_walk(hipStream_t s, const WalkArgs& a, bool potential) {
     const int blocks = std::max(1, a.blocks);
     reset_walk<<<1, 1, 0, s>>>(a.counters);
     LHIP_LAUNCH_CHECK();
-    if (potential) {
-        if (guard) walk_kernel<true, true><<<blocks, kWalkBS, 0, s>>>(a);
-        else walk_kernel<true, false><<<blocks, kWalkBS, 0, s>>>(a);
-    } else {
-        if (guard) walk_kernel<false, true><<<blocks, kWalkBS, 0, s>>>(a);
-        else walk_kernel<false, false><<<blocks, kWalkBS, 0, s>>>(a);
-    }
+    if (a.xd) walk_launch<true>(s, a, potential, guard, blocks);
+    else walk_launch<false>(s, a, potential, guard, blocks);
     LHIP_LAUNCH_CHECK();
 }
 

@@ -26,11 +26,23 @@ struct alignas(32) WalkNode {
 };
 static_assert(sizeof(WalkNode) == 32, "WalkNode is one 32-byte record");
 
+// The mixed form's FP64 node moments: centre of mass, mass, traceless quadrupole about the com
+// (xx, xy, xz, yy, yz, zz).
+struct NodeMomD {
+    double cx, cy, cz, m;
+    double q[6];
+};
+static_assert(sizeof(NodeMomD) == 80, "NodeMomD layout");
+
 struct BuildBufs {
     int n;
     int cap_nodes;             // node capacity (upper bound for n bodies)
+    int gmax;                  // walk group size: groups are the topmost cells with <= gmax bodies
     const uint64_t* key;       // sorted keys
-    const float4* posm;        // sorted bodies
+    const float4* posm;        // sorted bodies (plain form)
+    const double *xd, *yd, *zd;  // sorted FP64 positions (mixed form; null in the plain form)
+    const float* m;            // sorted masses (mixed form)
+    NodeMomD* momd;            // cap FP64 moments (mixed form; null in the plain form)
     const Cube* cube;
     uint32_t* bl;              // per body: leaf level | group level << 8 | (c + 1) << 16
     uint32_t* cnt;             // n + 1: nodes starting at each body, then their exclusive scan
@@ -62,7 +74,13 @@ inline long long node_capacity(long long n) { return n + (n / 17 + 1) * kLevels 
 void tree_build(hipStream_t s, const BuildBufs& b, float inv_theta);
 
 struct WalkArgs {
-    const float4* posm;
+    const float4* posm;        // plain form: sorted (position, mass)
+    // mixed form (xd != null): FP64 positions, float masses, FP64 node moments, FP64 outputs
+    const double *xd, *yd, *zd;
+    const float* m;
+    const NodeMomD* momd;
+    double *axd, *ayd, *azd, *potd;
+    double Gd;
     const WalkNode* node;
     const NodeMom* mom;
     const uint32_t* gstart;

@@ -105,5 +105,12 @@ public:
 
 // The HIP factory family, beside the reference's make_cpu_backend/make_cuda_backend.
 std::unique_ptr<Backend> make_hip_backend(int device, std::string& why);
+// Options of the HIP backend with no reference counterpart: the mixed-precision form (FP64 state,
+// FP32 relative-coordinate pair arithmetic) and the walk group size (128 = the reference's).
+struct HipOptions {
+    bool mixed = false;
+    int group_max = 128;
+};
+std::unique_ptr<Backend> make_hip_backend(int device, std::string& why, const HipOptions& opt);
 bool hip_compiled();
 int hip_device_count();

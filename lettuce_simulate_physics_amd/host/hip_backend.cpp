@@ -136,11 +136,20 @@ private:
 }  // namespace
 
 std::unique_ptr<Backend> make_hip_backend(int device, std::string& why) {
+    return make_hip_backend(device, why, HipOptions{});
+}
+
+std::unique_ptr<Backend> make_hip_backend(int device, std::string& why, const HipOptions& opt) {
     lhip_ctx* c = nullptr;
     char buf[512] = {0};
     const int rc = lhip_create(device, &c, buf, sizeof buf);
     if (rc != LHIP_OK || !c) {
         why = buf[0] ? buf : "hip backend unavailable";
+        return nullptr;
+    }
+    if (lhip_set_precision(c, opt.mixed ? LHIP_MODE_MIXED : LHIP_MODE_PLAIN, opt.group_max) != LHIP_OK) {
+        why = lhip_last_error(c);
+        lhip_destroy(c);
         return nullptr;
     }
     return std::make_unique<HipBackend>(c);

@@ -1,0 +1,221 @@
+"""GPU tests of the mixed-precision form (FP64 state, FP32 pair arithmetic on relative coordinates,
+FP64 carry): accelerations and potentials against the oracle's FP64 direct sum, tree topology
+against the plain form, FP64 state round trips and integration, mode switching and error paths.
+No reference counterpart: the reference keeps FP32 state (src/bodies.hpp:8-13)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import lettuce_simulate_physics_amd as L
+import oracle as O
+from helpers import per_body_rel, rel_rms
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def mixed(hip):
+    hip.set_precision("mixed", 64)
+    yield hip
+    hip.set_precision("plain", 128)
+
+
+def accel_pot_by_id(be, b, p, potential=True):
+    be.upload(b)
+    be.compute_forces(p, potential)
+    ax, ay, az, pot = be.accelerations_f64()
+    _, _, _, ids = be.download_f64()
+    a = np.zeros((len(b), 3))
+    phi = np.zeros(len(b))
+    a[ids] = np.stack([ax, ay, az], 1)
+    phi[ids] = pot
+    return a, phi
+
+
+def test_mixed_direct_matches_fp64_direct(mixed):
+    b = O.plummer(4096, 21)
+    a, phi = accel_pot_by_id(mixed, b, L.ForceParams(G=1.0, eps=0.01, method="direct"))
+    ref, rphi = O.direct_f64(b, 1.0, 0.01)
+    e = per_body_rel(a, ref)
+    print(f"mixed direct vs FP64: max {e.max():.3g}, rel_rms {rel_rms(a, ref):.3g}")
+    assert e.max() < 2e-5
+    assert rel_rms(a, ref) < 2e-6
+    assert np.max(np.abs(phi - rphi) / np.abs(rphi)) < 2e-6
+    t = mixed.times()
+    assert t["interactions"] == 4096 * 4095
+
+
+@pytest.mark.parametrize("theta,med_tol,p99_tol", [(0.6, 1e-3, 4e-3), (0.4, 3e-4, 1.5e-3)])
+def test_mixed_tree_matches_fp64_direct(mixed, theta, med_tol, p99_tol):
+    b = O.plummer(30000, 22)
+    a, phi = accel_pot_by_id(mixed, b, L.ForceParams(G=1.0, eps=0.01, theta=theta, method="tree"))
+    sub = np.random.default_rng(5).choice(len(b), 4096, replace=False)
+    ref, rphi = O.direct_f64(b, 1.0, 0.01, sub)
+    e = per_body_rel(a[sub], ref)
+    print(f"mixed tree theta {theta}: median {np.median(e):.3g} p99 {np.quantile(e, 0.99):.3g}")
+    assert np.median(e) < med_tol and np.quantile(e, 0.99) < p99_tol
+    assert np.median(np.abs(phi[sub] - rphi) / np.abs(rphi)) < 1e-3
+
+
+def test_mixed_tree_theta0_equals_direct(mixed):
+    # theta -> 0 opens every cell: the tree walk is the direct sum (reference test pattern,
+    # tests/test_cuda_backend.cpp:200-221), here in the mixed form.
+    b = O.plummer(1000, 23)
+    a, _ = accel_pot_by_id(mixed, b, L.ForceParams(G=1.0, eps=0.01, theta=1e-6, method="tree"))
+    t = mixed.times()
+    assert t["interactions"] == 1000 * 999 and t["p2c"] == 0
+    ref, _ = O.direct_f64(b, 1.0, 0.01)
+    assert np.max(per_body_rel(a, ref)) < 2e-5
+
+
+def test_mixed_topology_equals_plain_with_reference_groups(hip):
+    # Float bodies promoted to FP64 quantise to the same keys, so with the reference's group size the
+    # mixed form builds the plain form's tree exactly.
+    b = O.plummer(50000, 24)
+    p = L.ForceParams(G=1.0, eps=0.01, theta=0.6, method="tree")
+    hip.set_precision("plain", 128)
+    hip.upload(b)
+    hip.compute_forces(p, False)
+    keys, groups, nodes = hip.tree_keys(), hip.tree_groups(), hip.tree_nodes()
+    ids = hip.download().id
+    counts = hip.times()
+    try:
+        hip.set_precision("mixed", 128)
+        hip.upload(b)
+        hip.compute_forces(p, False)
+        assert np.array_equal(hip.tree_keys(), keys)
+        assert np.array_equal(hip.tree_groups(), groups)
+        nd = hip.tree_nodes()
+        for k in ("lo", "hi", "level", "leaf", "nchild", "first", "prefix"):
+            assert np.array_equal(nd[k], nodes[k]), k
+        assert np.array_equal(hip.download_f64()[3], ids)
+        t = hip.times()
+        assert t["listed"] == counts["listed"] and t["p2c"] == counts["p2c"]
+    finally:
+        hip.set_precision("plain", 128)
+
+
+def test_smaller_groups_cut_interactions_at_equal_accuracy(hip):
+    b = O.plummer(30000, 25)
+    p = L.ForceParams(G=1.0, eps=0.01, theta=0.6, method="tree")
+    sub = np.random.default_rng(6).choice(len(b), 2048, replace=False)
+    ref, _ = O.direct_f64(b, 1.0, 0.01, sub)
+    res = {}
+    try:
+        for g in (128, 64, 32):
+            hip.set_precision("mixed", g)
+            a, _ = accel_pot_by_id(hip, b, p, False)
+            res[g] = (hip.times()["interactions"], np.median(per_body_rel(a[sub], ref)))
+    finally:
+        hip.set_precision("plain", 128)
+    print({g: (int(i), f"{m:.3g}") for g, (i, m) in res.items()})
+    assert res[64][0] < res[128][0] and res[32][0] < res[64][0]
+    assert res[32][1] < 1e-3
+
+
+def test_f64_state_round_trip_and_fp64_kick_drift(mixed):
+    rng = np.random.default_rng(7)
+    n = 5000
+    x = rng.standard_normal((n, 3)) + 1e4  # far from the origin: FP32 would lose digits here
+    v = rng.standard_normal((n, 3)) * 1e-3
+    m = np.full(n, 1.0 / n, np.float32)
+    mixed.upload_f64(x, v, m)
+    x2, v2, m2, ids = mixed.download_f64()
+    assert np.array_equal(x2, x) and np.array_equal(v2, v) and np.array_equal(ids, np.arange(n))
+    mixed.drift(0.25)
+    x3, _, _, _ = mixed.download_f64()
+    # dt = 0.25 is a power of two, so dt * v is exact and fma(v, dt, x) = round(x + dt * v).
+    assert np.array_equal(x3, x + 0.25 * v)
+
+
+def test_f64_kick_uses_fp64_accelerations(mixed):
+    b = O.plummer(2000, 26)
+    p = L.ForceParams(G=1.0, eps=0.01, method="direct")
+    mixed.upload(b)
+    mixed.compute_forces(p, False)
+    ax, ay, az, _ = mixed.accelerations_f64()
+    _, v0, _, _ = mixed.download_f64()
+    mixed.kick(0.5)
+    _, v1, _, _ = mixed.download_f64()
+    assert np.array_equal(v1, v0 + 0.5 * np.stack([ax, ay, az], 1))
+
+
+def test_mixed_far_from_origin_keeps_relative_precision(mixed):
+    # A Plummer sphere displaced by 1e4: positions keep their offsets in FP64, so the mixed tree
+    # reproduces the centred sphere's accelerations; an FP32 state would quantise positions at ~1e-3.
+    b = O.plummer(20000, 27)
+    x = np.stack([b.x, b.y, b.z], 1).astype(np.float64)
+    v = np.stack([b.vx, b.vy, b.vz], 1).astype(np.float64)
+    p = L.ForceParams(G=1.0, eps=0.01, theta=0.6, method="tree")
+    out = []
+    for shift in (0.0, 1e4):
+        mixed.upload_f64(x + shift, v, b.m)
+        mixed.compute_forces(p, False)
+        ax, ay, az, _ = mixed.accelerations_f64()
+        ids = mixed.download_f64()[3]
+        a = np.zeros((len(b), 3))
+        a[ids] = np.stack([ax, ay, az], 1)
+        out.append(a)
+    sub = np.random.default_rng(8).choice(len(b), 2048, replace=False)
+    ref, _ = O.direct_f64(b, 1.0, 0.01, sub)
+    e0, e1 = per_body_rel(out[0][sub], ref), per_body_rel(out[1][sub], ref)
+    print(f"median error at the origin {np.median(e0):.3g}, shifted by 1e4 {np.median(e1):.3g}")
+    assert np.median(e1) < 1e-3 and np.median(e1) < 1.5 * np.median(e0) + 1e-5
+
+
+def test_mixed_kdk_conserves_energy_and_momentum(mixed):
+    # Reference KDK conservation bounds (tests/test_cpu_backend.cpp:218-239) in the mixed form.
+    b = O.plummer(20000, 28)
+    p = L.ForceParams(G=1.0, eps=0.01, theta=0.6, method="tree")
+    mixed.upload(b)
+    mixed.compute_forces(p, True)
+    d0 = mixed.diagnostics()
+    e0 = d0["ke"] + d0["pe"]
+    for _ in range(20):
+        mixed.kick(5e-4)
+        mixed.drift(1e-3)
+        mixed.compute_forces(p, True)
+        mixed.kick(5e-4)
+    d = mixed.diagnostics()
+    e1 = d["ke"] + d["pe"]
+    print(f"dE/E {abs(e1 - e0) / abs(e0):.3g}, |p| {np.hypot(np.hypot(d['px'], d['py']), d['pz']):.3g}")
+    assert abs(e1 - e0) / abs(e0) < 2e-3
+    assert np.hypot(np.hypot(d["px"], d["py"]), d["pz"]) < 1e-4
+
+
+def test_mode_switch_converts_state(hip):
+    b = O.plummer(3000, 29)
+    hip.set_precision("plain", 128)
+    hip.upload(b)
+    try:
+        hip.set_precision("mixed", 64)
+        x, v, m, ids = hip.download_f64()
+        assert np.array_equal(x[:, 0], b.x.astype(np.float64)) and np.array_equal(v[:, 2], b.vz.astype(np.float64))
+        assert hip.precision() == ("mixed", 64)
+        hip.set_precision("plain", 128)
+        back = hip.download()
+        assert np.array_equal(back.x, b.x) and np.array_equal(back.id, b.id)
+    finally:
+        hip.set_precision("plain", 128)
+
+
+def test_mixed_mode_errors(hip):
+    hip.set_precision("plain", 128)
+    with pytest.raises(L.LogicError):
+        hip.upload_f64(np.zeros((4, 3)), np.zeros((4, 3)), np.ones(4, np.float32))
+    with pytest.raises(ValueError):
+        hip.set_precision("mixed", 8)
+    assert hip.precision() == ("plain", 128)
+
+
+def test_mixed_disk_sech2_against_fp64_direct(mixed):
+    # The rotating sech^2 disk with a central point mass (small version of the 4M-star workload):
+    # theta 0.5, eps 0.02.
+    b = O.disk_sech2(40000, seed=3)
+    a, _ = accel_pot_by_id(mixed, b, L.ForceParams(G=1.0, eps=0.02, theta=0.5, method="tree"), False)
+    sub = np.random.default_rng(9).choice(len(b), 4096, replace=False)
+    ref, _ = O.direct_f64(b, 1.0, 0.02, sub)
+    e = per_body_rel(a[sub], ref)
+    print(f"disk mixed tree theta 0.5: median {np.median(e):.3g} p99 {np.quantile(e, 0.99):.3g}")
+    assert np.median(e) < 1e-3
