@@ -31,10 +31,11 @@ PEAK_FP32 = 157.3e12  # MI355X published FP32 vector peak (packed issue)
 
 
 def plummer(n: int, seed: int = 1):
-    """The workload's initial conditions (same generator as the reference scenario 'plummer')."""
-    import oracle as O  # the generator only; never the measured path
+    """The workload's initial conditions: the library's generator, bit-exact with the reference
+    scenario 'plummer' (plummer.a = 1, plummer.cut = 10)."""
+    import lettuce_simulate_physics_amd as L
 
-    return O.plummer(n, seed, 1.0, 10.0)
+    return L.plummer(n, seed, 1.0, 10.0)
 
 
 def cpu_baseline(b, steps: int = 2):
@@ -94,6 +95,130 @@ def adjoint_leg(be):
             "loss": float(((xT - xt) ** 2).sum())}
 
 
+def run_steps(be, p, steps: int, dt: float, check=None):
+    """`steps` KDK steps (lettuce-bench order); per-step wall time around each synchronised step.
+    check(k) runs after step k (untimed). Returns timing sums and counts."""
+    import torch
+
+    r = dict(wall=0.0, force=0.0, sort=0.0, build=0.0, walk=0.0, inter=0, p2p=0, p2c=0, p2m=0)
+    for k in range(1, steps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        be.kick(0.5 * dt)
+        be.drift(dt)
+        be.compute_forces(p, False)
+        be.kick(0.5 * dt)
+        t = be.times()  # resolves the kick/drift events (synchronises)
+        torch.cuda.synchronize()
+        r["wall"] += time.perf_counter() - t0
+        for key, name in (("force", "force_ms"), ("sort", "sort_ms"), ("build", "build_ms"), ("walk", "walk_ms")):
+            r[key] += t[name]
+        r["inter"] += t["interactions"]
+        r["p2p"] += t["p2p"]
+        r["p2c"] += t["p2c"]
+        r["p2m"] += t["p2c_mono"]
+        if check:
+            check(k)
+    return r
+
+
+def force_errors(be, eps: float, sample: int, seed: int):
+    """Per-body relative force error of the last force call against the device FP64 direct sum on a
+    seeded sample of bodies: (median, p99)."""
+    n = be.size()
+    rng = np.random.default_rng(seed)
+    idx = np.sort(rng.choice(n, min(sample, n), replace=False)).astype(np.uint32)
+    mode, _ = be.precision()
+    if mode == "mixed":
+        ax, ay, az, _ = be.accelerations_f64()
+    else:
+        ax, ay, az, _ = [a.astype(np.float64) for a in be.accelerations()]
+    a = np.stack([ax[idx], ay[idx], az[idx]], 1)
+    ref, _ = be.direct_f64_targets(idx, 1.0, eps)
+    e = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    return float(np.median(e)), float(np.quantile(e, 0.99))
+
+
+def energy(be, p):
+    be.compute_forces(p, True)
+    d = be.diagnostics()
+    return d["ke"] + d["pe"]
+
+
+def summary(r, steps):
+    return {"ms_per_step": round(r["wall"] * 1e3 / steps, 4), "force_ms": round(r["force"] / steps, 4),
+            "sort_ms": round(r["sort"] / steps, 4), "build_ms": round(r["build"] / steps, 4),
+            "walk_ms": round(r["walk"] / steps, 4),
+            "ginteractions_per_s": round(r["inter"] / (r["force"] * 1e-3) / 1e9, 3),
+            "interactions_per_body_step": None}
+
+
+def mixed_leg(be, b, steps: int, warmup: int, wopt: dict, plain_force_ms: float):
+    """The same 2^20-body Plummer tree step in the mixed-precision form (FP64 state, FP32
+    relative-coordinate pair arithmetic, FP64 carry, and the mixed form's walk options: by default
+    the error-budgeted far field). Force errors of both forms at the initial state against the FP64
+    direct sum on 4096 seeded bodies."""
+    import lettuce_simulate_physics_amd as L
+
+    p = L.ForceParams(G=1.0, eps=EPS, theta=THETA, method="tree")
+    be.set_precision("plain")
+    be.upload(b)
+    be.compute_forces(p, False)
+    pmed, pp99 = force_errors(be, EPS, 4096, 11)
+    be.set_precision("mixed", **wopt)
+    be.upload(b)
+    be.compute_forces(p, False)
+    med, p99 = force_errors(be, EPS, 4096, 11)
+    run_steps(be, p, warmup, DT)
+    r = run_steps(be, p, steps, DT)
+    out = summary(r, steps)
+    out["interactions_per_body_step"] = round(r["inter"] / steps / len(b), 2)
+    out["monopole_fraction_of_p2c"] = round(r["p2m"] / max(r["p2c"], 1), 4)
+    opts = be.precision()[1]
+    be.set_precision("plain")
+    return dict(workload="plummer_tree_1M_mixed", walk_options=opts, **out,
+                force_speedup_vs_plain=round(plain_force_ms / out["force_ms"], 3),
+                median_force_error=med, p99_force_error=p99, plain_median_force_error=pmed,
+                plain_p99_force_error=pp99,
+                error_reference="FP64 direct sum on 4096 seeded bodies at the initial state")
+
+
+def disk_leg(be, nstars: int, steps: int, wopt: dict):
+    """Rotating sech^2 disk, nstars equal-mass stars (total mass 1, Rd 1, cut 10 Rd, z0 0.1,
+    dispersion 0.1 vc) around a point mass 10; tree theta 0.5, eps 0.02, KDK dt 5e-4. Runs the plain
+    and the mixed form from the same initial state; force error vs FP64 direct on 8192 seeded bodies
+    every 10 steps, energy drift over the run (tree potential)."""
+    import lettuce_simulate_physics_amd as L
+
+    b = L.disk_sech2(nstars, seed=1, core=10.0, mass=1.0, rd=1.0, rmax=10.0, z0=0.1, sigma=0.1)
+    p = L.ForceParams(G=1.0, eps=0.02, theta=0.5, method="tree")
+    res = {}
+    for mode, o in (("plain", {}), ("mixed", wopt)):
+        be.set_precision(mode, **o)
+        be.upload(b)
+        e0 = energy(be, p)
+        errs = [force_errors(be, 0.02, 8192, 100)]
+
+        def check(k):
+            if k % 10 == 0:
+                errs.append(force_errors(be, 0.02, 8192, 100 + k))
+
+        r = run_steps(be, p, steps, 5e-4, check)
+        e1 = energy(be, p)
+        out = summary(r, steps)
+        out["interactions_per_body_step"] = round(r["inter"] / steps / len(b), 2)
+        out["walk_options"] = be.precision()[1]
+        out["monopole_fraction_of_p2c"] = round(r["p2m"] / max(r["p2c"], 1), 4)
+        out["force_error_median_per_check"] = [float(f"{m:.4g}") for m, _ in errs]
+        out["force_error_p99_per_check"] = [float(f"{q:.4g}") for _, q in errs]
+        out["energy_drift"] = float(f"{abs(e1 - e0) / abs(e0):.4g}")
+        res[mode] = out
+    be.set_precision("plain")
+    return {"workload": f"disk_sech2_{nstars}_plus_core", "steps": steps, "theta": 0.5, "eps": 0.02, "dt": 5e-4,
+            "checks_every": 10, **res,
+            "mixed_force_speedup_vs_plain": round(res["plain"]["force_ms"] / res["mixed"]["force_ms"], 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +228,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision legs")
+    ap.add_argument("--mixed-group", type=int, default=None, help="walk group size of the mixed form")
+    ap.add_argument("--mixed-small-leaf", type=int, default=None)
+    ap.add_argument("--mixed-tau", type=float, default=None, help="far-field quadrupole threshold (0: all kept)")
+    ap.add_argument("--disk-stars", type=int, default=4194304)
+    ap.add_argument("--disk-steps", type=int, default=50)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy 3 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -129,36 +260,14 @@ def main():
     p = L.ForceParams(G=1.0, eps=EPS, theta=THETA, method="tree")
     be.upload(b)
     be.compute_forces(p, False)
-    for _ in range(args.warmup):
-        be.kick(0.5 * DT)
-        be.drift(DT)
-        be.compute_forces(p, False)
-        be.kick(0.5 * DT)
-    be.times()
+    run_steps(be, p, args.warmup, DT)
     torch.cuda.synchronize()
-
-    force = sort = build = walk = 0.0
-    inter = p2p = p2c = 0
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        be.kick(0.5 * DT)
-        be.drift(DT)
-        be.compute_forces(p, False)
-        be.kick(0.5 * DT)
-        t = be.times()  # resolves the kick/drift events (synchronises)
-        force += t["force_ms"]
-        sort += t["sort_ms"]
-        build += t["build_ms"]
-        walk += t["walk_ms"]
-        inter += t["interactions"]
-        p2p += t["p2p"]
-        p2c += t["p2c"]
-    torch.cuda.synchronize()
-    wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
     k = args.steps
-    force_ms, walk_ms = force / k, walk / k
-    ginter = inter / (force * 1e-3) / 1e9
-    walk_flops = (FLOP_P2P * p2p + FLOP_P2C * p2c) / k
+    r = run_steps(be, p, k, DT)
+    wall_ms = r["wall"] * 1e3 / k
+    force_ms, walk_ms = r["force"] / k, r["walk"] / k
+    ginter = r["inter"] / (r["force"] * 1e-3) / 1e9
+    walk_flops = (FLOP_P2P * r["p2p"] + FLOP_P2C * r["p2c"]) / k
     achieved = walk_flops / (walk_ms * 1e-3)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "walk_traffic.json")
@@ -181,16 +290,23 @@ def main():
         "config": {"workload": "plummer_tree_1M", "n": args.n, "theta": THETA, "eps": EPS, "dt": DT,
                    "leaf_max": 16, "group_max": 128, "plummer_a": 1.0, "plummer_cut": 10.0, "seed": 1},
         "force_ms": round(force_ms, 4),
-        "sort_ms": round(sort / k, 4),
-        "build_ms": round(build / k, 4),
+        "sort_ms": round(r["sort"] / k, 4),
+        "build_ms": round(r["build"] / k, 4),
         "walk_ms": round(walk_ms, 4),
-        "interactions_per_body": round(inter / k / args.n, 2),
+        "interactions_per_body": round(r["inter"] / k / args.n, 2),
         "roofline": {"kernel": "walk_kernel", "bound": "valu", "achieved": round(achieved / 1e12, 3),
                      "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4),
                      "traffic": traffic},
         "cpu_baseline": None,
+        "mixed": None,
+        "disk": None,
         "adjoint": None,
     }
+    if not args.no_mixed:
+        wopt = dict(group_max=args.mixed_group, small_leaf=args.mixed_small_leaf, quad_tau=args.mixed_tau)
+        out["mixed"] = mixed_leg(be, b, k, args.warmup, wopt, force_ms)
+        if args.disk_steps > 0:
+            out["disk"] = disk_leg(be, args.disk_stars, args.disk_steps, wopt)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(be)
         out["adjoint_forward_ratio"] = out["adjoint"]["adjoint_forward_ratio"]

@@ -57,12 +57,14 @@ typedef struct lhip_diag {
 
 /* PhaseTimes (src/backend.hpp:40-49) plus split counters: p2p (body-body) and p2c (body-node)
  * interactions, and `listed`, the reference CPU backend's count (list entries per body, the body's
- * own entry included). interactions = p2p + p2c. */
+ * own entry included). interactions = p2p + p2c. p2c_mono: the body-node interactions evaluated as
+ * monopoles by the error-budgeted far field (part of p2c; 0 with the reference's walk options). */
 typedef struct lhip_phase_times {
     double force_ms, sort_ms, build_ms, walk_ms, step_ms;
     uint64_t interactions;
     double comm_ms, balance_ms;
     uint64_t p2p, p2c, listed;
+    uint64_t p2c_mono;
 } lhip_phase_times;
 
 /* ---- factory (src/backend.hpp:143-148; make_cuda_backend src/cuda_backend.cu:2026-2048) ---- */
@@ -130,11 +132,22 @@ int lhip_body_costs(lhip_ctx* ctx, float* cost);
  * walk group's centre (tree) or on FP64 separations (direct), FP32 partial sums carried into FP64,
  * and FP64 diagnostics. LHIP_MODE_PLAIN is the reference's FP32 state (the default). Switching
  * converts the resident bodies, accelerations and potential and invalidates the tree.
- * group_max: walk group size for later trees, in [16, 128] (128 = the reference's kGroupMax,
- * src/cpu_backend.cpp:27). */
+ *
+ * Walk options (both modes): group_max, the walk group size in [16, 128] (the reference's
+ * kGroupMax = 128, src/cpu_backend.cpp:27); small_leaf, the largest accepted leaf summed body by
+ * body, in [0, 16] (the reference's kSmallLeaf = 4, src/cpu_backend.cpp:29); quad_tau >= 0, the
+ * error-budgeted far field: an accepted cell whose quadrupole term is below quad_tau times its
+ * monopole term at the group box's distance (1.5 |Q|_F / M < quad_tau d^2) is evaluated as a point
+ * mass (0 keeps every quadrupole, as the reference). NULL options select the mode's defaults:
+ * plain {128, 4, 0} (the reference walk), mixed {128, 1, 0.05}. */
 enum { LHIP_MODE_PLAIN = 0, LHIP_MODE_MIXED = 1 };
-int lhip_set_precision(lhip_ctx* ctx, int32_t mode, int32_t group_max);
-int lhip_get_precision(const lhip_ctx* ctx, int32_t* mode, int32_t* group_max);
+typedef struct lhip_walk_options {
+    int32_t group_max;
+    int32_t small_leaf;
+    double quad_tau;
+} lhip_walk_options;
+int lhip_set_precision(lhip_ctx* ctx, int32_t mode, const lhip_walk_options* options);
+int lhip_get_precision(const lhip_ctx* ctx, int32_t* mode, lhip_walk_options* options);
 /* FP64 initial conditions and state (the extra entry Bodies' FP32 fields cannot carry). Mixed
  * mode only (LHIP_LOGIC otherwise). Masses and ids as in lhip_upload. */
 int lhip_upload_f64(lhip_ctx* ctx, uint64_t n, const double* x, const double* y, const double* z, const double* vx,
@@ -145,11 +158,42 @@ int lhip_download_f64(lhip_ctx* ctx, double* x, double* y, double* z, double* vx
  * NULL. */
 int lhip_accelerations_f64(lhip_ctx* ctx, double* ax, double* ay, double* az, double* pot);
 
+/* ---- initial conditions (host code) ----
+ *
+ * The reference's Plummer scenario (src/scenarios.cpp:94-132; keys plummer.a, plummer.cut,
+ * plummer.mass, seed): float outputs are bit-exact with the reference generator; the _f64 form
+ * returns the same values before the final rounding. Any output pointer may be NULL. */
+int lhip_scenario_plummer(uint64_t n, uint64_t seed, double G, double M, double a, double cut, float* x, float* y,
+                          float* z, float* vx, float* vy, float* vz, float* m, uint32_t* id);
+int lhip_scenario_plummer_f64(uint64_t n, uint64_t seed, double G, double M, double a, double cut, double* x,
+                              double* y, double* z, double* vx, double* vy, double* vz, double* m, uint32_t* id);
+/* Rotating exponential disk (scale rd, cut at rmax, total stellar mass `mass`) with a sech^2
+ * vertical profile of scale z0 around a central point mass `core` (body 0; nstars + 1 bodies),
+ * circular speed from the enclosed mass with Gaussian dispersion sigma * vc per component, spin
+ * +1 or -1 about the disk axis; the disk is then rotated by incl about the x axis and moved to
+ * (pos, vel). No reference counterpart: the reference disk (src/scenarios.cpp:173-236) is Gaussian
+ * in z; radii are drawn the same way (src/scenarios.cpp:197-205). */
+typedef struct lhip_disk_params {
+    uint64_t nstars, seed;
+    double G, core, mass, rd, rmax, z0, sigma, incl, spin;
+    double pos[3], vel[3];
+} lhip_disk_params;
+int lhip_scenario_disk_sech2(const lhip_disk_params* p, int32_t com_frame, float* x, float* y, float* z, float* vx,
+                             float* vy, float* vz, float* m, uint32_t* id);
+int lhip_scenario_disk_sech2_f64(const lhip_disk_params* p, int32_t com_frame, double* x, double* y, double* z,
+                                 double* vx, double* vy, double* vz, double* m, uint32_t* id);
+
 /* ---- inspection (no reference counterpart; for parity tests and tools) ---- */
 
 /* Accelerations and potential of the last force call, in the backend's current order (rounded to
  * float in the mixed mode). */
 int lhip_accelerations(lhip_ctx* ctx, float* ax, float* ay, float* az, float* pot);
+/* FP64 direct-sum accelerations and potentials (a = G sum m_j d/s^3, phi = -G sum m_j/s, the
+ * target skipped by index) of the bodies at the given backend-order indices, against every body
+ * at the current positions (FP32 or FP64 state), in double pair arithmetic on the device: the
+ * FP64 direct oracle for force errors at sizes a CPU sum cannot reach. Any output may be NULL. */
+int lhip_direct_f64_targets(lhip_ctx* ctx, uint64_t nt, const uint32_t* targets, double G, double eps, double* ax,
+                            double* ay, double* az, double* pot);
 /* Sorted Morton keys of the current tree (n entries). */
 int lhip_tree_keys(lhip_ctx* ctx, uint64_t* keys);
 /* Node count, walk-group count and the number of bodies in over-full finest cells. */

@@ -27,6 +27,9 @@ OK, RUNTIME, LOGIC, UNAVAILABLE, OUT_OF_MEMORY, INVALID, NOT_PROVIDED = range(7)
 METHOD_DIRECT, METHOD_TREE = 0, 1
 MODE_PLAIN, MODE_MIXED = 0, 1
 REFERENCE_GROUP_MAX = 128  # src/cpu_backend.cpp:27
+# Walk options (group_max, small_leaf, quad_tau) of each mode when none are given: plain is the
+# reference walk (src/cpu_backend.cpp:27,29, every quadrupole kept).
+DEFAULT_WALK = {"plain": (REFERENCE_GROUP_MAX, 4, 0.0), "mixed": (REFERENCE_GROUP_MAX, 1, 0.05)}
 
 
 class LogicError(RuntimeError):
@@ -47,7 +50,12 @@ class DiagC(ctypes.Structure):
 class TimesC(ctypes.Structure):
     _fields_ = [(k, ctypes.c_double) for k in ("force_ms", "sort_ms", "build_ms", "walk_ms", "step_ms")] + \
                [("interactions", ctypes.c_uint64), ("comm_ms", ctypes.c_double), ("balance_ms", ctypes.c_double),
-                ("p2p", ctypes.c_uint64), ("p2c", ctypes.c_uint64), ("listed", ctypes.c_uint64)]
+                ("p2p", ctypes.c_uint64), ("p2c", ctypes.c_uint64), ("listed", ctypes.c_uint64),
+                ("p2c_mono", ctypes.c_uint64)]
+
+
+class WalkOptionsC(ctypes.Structure):
+    _fields_ = [("group_max", ctypes.c_int32), ("small_leaf", ctypes.c_int32), ("quad_tau", ctypes.c_double)]
 
 
 _lib = None
@@ -115,6 +123,60 @@ class ForceParams:
         if self.method not in ("direct", "tree"):
             raise ValueError(f"method must be 'direct' or 'tree', not {self.method!r}")
         return ForceParamsC(self.G, self.eps, self.theta, METHOD_TREE if self.method == "tree" else METHOD_DIRECT, 0)
+
+
+class DiskParamsC(ctypes.Structure):
+    _fields_ = [("nstars", ctypes.c_uint64), ("seed", ctypes.c_uint64)] + \
+               [(k, ctypes.c_double) for k in ("G", "core", "mass", "rd", "rmax", "z0", "sigma", "incl", "spin")] + \
+               [("pos", ctypes.c_double * 3), ("vel", ctypes.c_double * 3)]
+
+
+def _check_gen(rc):
+    if rc != OK:
+        raise ValueError("invalid scenario parameters")
+
+
+def plummer(n: int, seed: int = 1, a: float = 1.0, cut: float = 10.0, M: float = 1.0, G: float = 1.0) -> Bodies:
+    """The reference's Plummer scenario (src/scenarios.cpp:94-132), float bodies bit-exact with it
+    (keys plummer.a, plummer.cut, plummer.mass, seed)."""
+    b = Bodies.empty(n)
+    _check_gen(library().lhip_scenario_plummer(_U64(n), _U64(seed), _D(G), _D(M), _D(a), _D(cut),
+                                               *[_ptr(v) for v in b.arrays()]))
+    return b
+
+
+def plummer_f64(n: int, seed: int = 1, a: float = 1.0, cut: float = 10.0, M: float = 1.0, G: float = 1.0):
+    """The same sphere before rounding to float: (x (n x 3), v (n x 3), m) in FP64."""
+    c = [np.zeros(n) for _ in range(7)]
+    _check_gen(library().lhip_scenario_plummer_f64(_U64(n), _U64(seed), _D(G), _D(M), _D(a), _D(cut),
+                                                   *[_ptr(v) for v in c], None))
+    return np.stack(c[:3], 1), np.stack(c[3:6], 1), c[6]
+
+
+def _disk_params(nstars, seed, core, mass, rd, rmax, z0, sigma, incl, spin, pos, vel, G):
+    return DiskParamsC(nstars, seed, G, core, mass, rd, rmax, z0, sigma, incl, spin, (ctypes.c_double * 3)(*pos),
+                       (ctypes.c_double * 3)(*vel))
+
+
+def disk_sech2(nstars: int, seed: int = 1, core: float = 10.0, mass: float = 1.0, rd: float = 1.0,
+               rmax: float = 10.0, z0: float = 0.1, sigma: float = 0.1, incl: float = 0.0, spin: float = 1.0,
+               pos=(0.0, 0.0, 0.0), vel=(0.0, 0.0, 0.0), com_frame: bool = True, G: float = 1.0) -> Bodies:
+    """Rotating exponential disk with a sech^2 vertical profile around a central point mass (body 0;
+    nstars + 1 bodies). No reference counterpart (the reference disk is Gaussian in z)."""
+    p = _disk_params(nstars, seed, core, mass, rd, rmax, z0, sigma, incl, spin, pos, vel, G)
+    b = Bodies.empty(nstars + 1)
+    _check_gen(library().lhip_scenario_disk_sech2(ctypes.byref(p), int(com_frame), *[_ptr(v) for v in b.arrays()]))
+    return b
+
+
+def disk_sech2_f64(nstars: int, seed: int = 1, core: float = 10.0, mass: float = 1.0, rd: float = 1.0,
+                   rmax: float = 10.0, z0: float = 0.1, sigma: float = 0.1, incl: float = 0.0, spin: float = 1.0,
+                   pos=(0.0, 0.0, 0.0), vel=(0.0, 0.0, 0.0), com_frame: bool = True, G: float = 1.0):
+    """The disk in FP64: (x (n x 3), v (n x 3), m)."""
+    p = _disk_params(nstars, seed, core, mass, rd, rmax, z0, sigma, incl, spin, pos, vel, G)
+    c = [np.zeros(nstars + 1) for _ in range(7)]
+    _check_gen(library().lhip_scenario_disk_sech2_f64(ctypes.byref(p), int(com_frame), *[_ptr(v) for v in c], None))
+    return np.stack(c[:3], 1), np.stack(c[3:6], 1), c[6]
 
 
 def hip_compiled() -> bool:
@@ -257,18 +319,25 @@ class HipBackend:
         return c
 
     # ---- mixed-precision form (no reference counterpart) ----
-    def set_precision(self, mode: str = "plain", group_max: int = REFERENCE_GROUP_MAX) -> None:
+    def set_precision(self, mode: str = "plain", group_max=None, small_leaf=None, quad_tau=None) -> None:
         """'plain': the reference's FP32 state; 'mixed': FP64 state with FP32 relative-coordinate
-        pair arithmetic. group_max: walk group size of later trees (16..128)."""
+        pair arithmetic. Walk options (None: the mode's default, plain 128/4/0 = the reference walk,
+        mixed 128/1/0.05): group_max (16..128), small_leaf (0..16), quad_tau (>= 0, the
+        error-budgeted far field threshold)."""
         if mode not in ("plain", "mixed"):
             raise ValueError("mode must be 'plain' or 'mixed'")
+        d = DEFAULT_WALK[mode]
+        o = WalkOptionsC(int(d[0] if group_max is None else group_max), int(d[1] if small_leaf is None else small_leaf),
+                         float(d[2] if quad_tau is None else quad_tau))
         self._check(self._L.lhip_set_precision(self._h, MODE_MIXED if mode == "mixed" else MODE_PLAIN,
-                                               int(group_max)))
+                                               ctypes.byref(o)))
 
     def precision(self):
-        m, g = ctypes.c_int32(), ctypes.c_int32()
-        self._check(self._L.lhip_get_precision(self._h, ctypes.byref(m), ctypes.byref(g)))
-        return ("mixed" if m.value == MODE_MIXED else "plain"), int(g.value)
+        """(mode, {group_max, small_leaf, quad_tau})."""
+        m, o = ctypes.c_int32(), WalkOptionsC()
+        self._check(self._L.lhip_get_precision(self._h, ctypes.byref(m), ctypes.byref(o)))
+        return ("mixed" if m.value == MODE_MIXED else "plain"), dict(group_max=o.group_max, small_leaf=o.small_leaf,
+                                                                     quad_tau=o.quad_tau)
 
     def upload_f64(self, x, v, m, ids=None) -> None:
         """FP64 positions and velocities (n x 3), float masses, uint32 ids (default 0..n-1)."""
@@ -288,6 +357,14 @@ class HipBackend:
         ids = np.zeros(n, np.uint32)
         self._check(self._L.lhip_download_f64(self._h, *[_ptr(c) for c in cols], _ptr(mm), _ptr(ids)))
         return np.stack(cols[:3], 1), np.stack(cols[3:], 1), mm, ids
+
+    def direct_f64_targets(self, targets, G: float, eps: float):
+        """FP64 direct-sum (a (k x 3), phi (k)) of the bodies at backend indices `targets`."""
+        t = np.ascontiguousarray(targets, np.uint32)
+        out = [np.zeros(len(t)) for _ in range(4)]
+        self._check(self._L.lhip_direct_f64_targets(self._h, _U64(len(t)), _ptr(t), _D(G), _D(eps),
+                                                    *[_ptr(o) for o in out]))
+        return np.stack(out[:3], 1), out[3]
 
     def accelerations_f64(self):
         n = self.size()

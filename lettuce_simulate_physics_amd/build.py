@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "liblettuce_hip.so")
 BUILD = os.path.join(HERE, "_build")
 SOURCES = ["kernels_state.hip", "kernels_direct.hip", "kernels_tree.hip", "kernels_build.hip",
-           "kernels_walk.hip", "kernels_adjoint.hip", "adjoint.hip", "capi.hip"]
+           "kernels_walk.hip", "kernels_adjoint.hip", "adjoint.hip", "capi.hip", "scenarios.cpp"]
 ARCH = os.environ.get("LHIP_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++20", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
@@ -24,7 +24,7 @@ FLAGS = ["-O3", "-std=c++20", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-ta
 
 
 def _compile(src: str) -> str:
-    obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
     cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -153,13 +153,18 @@ int lhip_accelerations(lhip_ctx* ctx, float* ax, float* ay, float* az, float* po
 int lhip_accelerations_f64(lhip_ctx* ctx, double* ax, double* ay, double* az, double* pot) {
     return guarded(ctx, [&](lhip::Context& c) { c.accelerations_f64(ax, ay, az, pot); });
 }
-int lhip_set_precision(lhip_ctx* ctx, int32_t mode, int32_t group_max) {
-    return guarded(ctx, [&](lhip::Context& c) { c.set_precision(mode, group_max); });
+int lhip_direct_f64_targets(lhip_ctx* ctx, uint64_t nt, const uint32_t* targets, double G, double eps, double* ax,
+                            double* ay, double* az, double* pot) {
+    if (nt > 0 && !targets) return LHIP_INVALID;
+    return guarded(ctx, [&](lhip::Context& c) { c.direct_f64_targets(nt, targets, G, eps, ax, ay, az, pot); });
 }
-int lhip_get_precision(const lhip_ctx* ctx, int32_t* mode, int32_t* group_max) {
+int lhip_set_precision(lhip_ctx* ctx, int32_t mode, const lhip_walk_options* options) {
+    return guarded(ctx, [&](lhip::Context& c) { c.set_precision(mode, options); });
+}
+int lhip_get_precision(const lhip_ctx* ctx, int32_t* mode, lhip_walk_options* options) {
     if (!ctx || !ctx->c) return LHIP_INVALID;
     if (mode) *mode = ctx->c->mixed ? LHIP_MODE_MIXED : LHIP_MODE_PLAIN;
-    if (group_max) *group_max = ctx->c->gmax;
+    if (options) *options = ctx->c->wopt;
     return LHIP_OK;
 }
 int lhip_upload_f64(lhip_ctx* ctx, uint64_t n, const double* x, const double* y, const double* z, const double* vx,

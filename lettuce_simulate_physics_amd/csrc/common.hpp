@@ -100,7 +100,7 @@ __host__ __device__ inline int info_nchild(int32_t info) { return (info >> 8) & 
 struct alignas(64) NodeMom {
     float cx, cy, cz, m;
     float qxx, qxy, qxz, qyy;
-    float qyz, qzz, rcrit2, pad0;
+    float qyz, qzz, rcrit2, qrel;  // qrel = 1.5 |Q|_F / M: quadrupole/monopole term ratio times dist^2
     float gx, gy, gz, half;  // geometric cell centre and half side
 };
 static_assert(sizeof(NodeMom) == 64, "NodeMom is one 64-byte record");

@@ -47,7 +47,7 @@ struct Context {
     // FP64 accelerations and potential, FP64 node moments and direct-sum records. Masses and ids
     // stay in m and id; the float position arrays are then only download staging.
     bool mixed = false;
-    int gmax = kGroupMax;  // walk group size (128: the reference's groups)
+    lhip_walk_options wopt{kGroupMax, kSmallLeaf, 0.0};  // the reference's walk
     DevBuf xd, yd, zd, vxd, vyd, vzd, xd2, yd2, zd2, vxd2, vyd2, vzd2, axd, ayd, azd, potd, momd, rec64;
 
     bool has_pe = false, acc_ok = false, tree_ok = false, costs_ok = false;
@@ -103,14 +103,22 @@ struct Context {
     double* D(DevBuf& b) { return b.as<double>(); }
     float* F(DevBuf& b) { return b.as<float>(); }
 
+    static lhip_walk_options default_options(bool mix) {
+        return mix ? lhip_walk_options{kGroupMax, 1, 0.05} : lhip_walk_options{kGroupMax, kSmallLeaf, 0.0};
+    }
+
     // Switches between the plain (FP32 state) and mixed-precision (FP64 state) forms, converting the
-    // resident bodies, accelerations and potential; group is the walk group size for later trees.
-    void set_precision(int mode, int group) {
+    // resident bodies, accelerations and potential; the walk options apply to later trees.
+    void set_precision(int mode, const lhip_walk_options* opt) {
         LHIP_CHECK(hipSetDevice(dev));
         if (mode != LHIP_MODE_PLAIN && mode != LHIP_MODE_MIXED) throw Error(kInvalid, "unknown precision mode");
-        if (group < kLeafMax || group > kGroupMax)
-            throw Error(kInvalid, "walk group size must be in [16, 128], got " + std::to_string(group));
         const bool mix = mode == LHIP_MODE_MIXED;
+        const lhip_walk_options o = opt ? *opt : default_options(mix);
+        if (o.group_max < kLeafMax || o.group_max > kGroupMax)
+            throw Error(kInvalid, "walk group size must be in [16, 128], got " + std::to_string(o.group_max));
+        if (o.small_leaf < 0 || o.small_leaf > kLeafMax)
+            throw Error(kInvalid, "small_leaf must be in [0, 16], got " + std::to_string(o.small_leaf));
+        if (!(o.quad_tau >= 0) || !(o.quad_tau < 1e30)) throw Error(kInvalid, "quad_tau must be finite and >= 0");
         if (mix != mixed && n > 0) {
             if (mix) {
                 reserve_f64(n);
@@ -127,8 +135,8 @@ struct Context {
             sync();
         }
         mixed = mix;
-        gmax = group;
-        tree_ok = false;  // the next tree uses the new form and group size
+        wopt = o;
+        tree_ok = false;  // the next tree uses the new form and options
     }
 
     // FP64 state in and out (mixed form only).
@@ -189,6 +197,37 @@ struct Context {
             if (e.first) LHIP_CHECK(hipMemcpyAsync(e.first, e.second->as<void>(), fb, hipMemcpyDeviceToHost, stream));
         sync();
     }
+
+    // FP64 direct-sum accelerations and potentials of the bodies at backend indices tgt (validation).
+    void direct_f64_targets(uint64_t nt, const uint32_t* tgt, double G, double eps, double* hax, double* hay,
+                            double* haz, double* hpot) {
+        LHIP_CHECK(hipSetDevice(dev));
+        if (nt == 0) return;
+        if (nt >= uint64_t(INT32_MAX)) throw Error(kInvalid, "too many targets");
+        for (uint64_t t = 0; t < nt; ++t)
+            if (tgt[t] >= uint32_t(n)) throw Error(kInvalid, "target index out of range");
+        const int k = int(nt);
+        chk_tgt.reserve(std::size_t(k) * 4);
+        chk_out.reserve(std::size_t(k) * 4 * 8);
+        chk_part.reserve(std::size_t(direct_f64_split(n, k)) * 4 * std::size_t(k) * 8);
+        LHIP_CHECK(hipMemcpyAsync(chk_tgt.as<void>(), tgt, std::size_t(k) * 4, hipMemcpyHostToDevice, stream));
+        double* o = chk_out.as<double>();
+        if (mixed)
+            launch_direct_f64_targets(stream, D(xd), D(yd), D(zd), true, m.as<float>(), n, chk_tgt.as<uint32_t>(), k,
+                                      eps, G, o, o + k, o + 2 * std::size_t(k), o + 3 * std::size_t(k),
+                                      chk_part.as<double>());
+        else
+            launch_direct_f64_targets(stream, F(x), F(y), F(z), false, m.as<float>(), n, chk_tgt.as<uint32_t>(), k,
+                                      eps, G, o, o + k, o + 2 * std::size_t(k), o + 3 * std::size_t(k),
+                                      chk_part.as<double>());
+        double* dst[4] = {hax, hay, haz, hpot};
+        for (int c = 0; c < 4; ++c)
+            if (dst[c])
+                LHIP_CHECK(hipMemcpyAsync(dst[c], o + std::size_t(c) * k, std::size_t(k) * 8, hipMemcpyDeviceToHost,
+                                          stream));
+        sync();
+    }
+    DevBuf chk_tgt, chk_out, chk_part;
 
     void accelerations_f64(double* hax, double* hay, double* haz, double* hpot) {
         if (!mixed) throw Error(kLogic, "hip accelerations_f64: FP64 accelerations need the mixed-precision mode");
@@ -342,7 +381,7 @@ struct Context {
         BuildBufs bb{};
         bb.n = n;
         bb.cap_nodes = int(cap_nodes);
-        bb.gmax = gmax;
+        bb.gmax = wopt.group_max;
         bb.key = key.as<uint64_t>();
         bb.posm = posm.as<float4>();
         if (mixed) {
@@ -376,7 +415,7 @@ struct Context {
     void compute_forces_with(const lhip_force_params& p, bool potential) {
         LHIP_CHECK(hipSetDevice(dev));
         t.walk_ms = t.force_ms = 0;
-        t.interactions = t.p2p = t.p2c = t.listed = 0;
+        t.interactions = t.p2p = t.p2c = t.listed = t.p2c_mono = 0;
         costs_ok = false;
         if (n == 0) {
             t.sort_ms = t.build_ms = 0;
@@ -436,6 +475,8 @@ struct Context {
             }
             a.counters = counters.as<WalkCounters>();
             a.blocks = walk_blocks;
+            a.small_leaf = wopt.small_leaf;
+            a.quad_tau = float(wopt.quad_tau);
             tree_walk(stream, a, potential);
             LHIP_CHECK(hipEventRecord(ev[4], stream));
             LHIP_CHECK(hipMemcpyAsync(&hc, counters.as<void>(), sizeof hc, hipMemcpyDeviceToHost, stream));
@@ -452,6 +493,7 @@ struct Context {
         t.p2p = hc.p2p;
         t.p2c = hc.p2c;
         t.listed = hc.listed;
+        t.p2c_mono = hc.p2m;
         overfull = hc.overfull;
         has_pe = potential;
         acc_ok = true;

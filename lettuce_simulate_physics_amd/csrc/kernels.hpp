@@ -43,6 +43,14 @@ int direct_split(int n);
 void launch_direct(hipStream_t s, const float4* posm, int n, float eps2, float G, bool potential, float* ax, float* ay,
                    float* az, float* pot, float4* part);
 
+// FP64 direct sum for nt target indices (validation): a = G sum m_j d / s^3, phi = -G sum m_j / s,
+// double pair arithmetic on the current positions (float, or double when f64). part: scratch of
+// direct_f64_split(n, nt) * 4 * nt doubles.
+int direct_f64_split(int n, int nt);
+void launch_direct_f64_targets(hipStream_t s, const void* x, const void* y, const void* z, bool f64,
+                               const float* m, int n, const uint32_t* tgt, int nt, double eps, double G, double* ax,
+                               double* ay, double* az, double* pot, double* part);
+
 // ---- tree (kernels_tree.hip) ----
 struct TreeBufs {
     bool f64;  // positions and velocities are double (the mixed form), else float

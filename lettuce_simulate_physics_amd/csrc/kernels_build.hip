@@ -389,7 +389,11 @@ __global__ void __launch_bounds__(kBS) moments_level(int level, int n, const flo
         md.qxx = float(q[0]); md.qxy = float(q[1]); md.qxz = float(q[2]);
         md.qyy = float(q[3]); md.qyz = float(q[4]); md.qzz = float(q[5]);
         md.rcrit2 = rcrit2_of(md.cx, md.cy, md.cz, md.gx, md.gy, md.gz, md.half, inv_theta);
-        md.pad0 = 0.f;
+        {
+            const double qf = sqrt(q[0] * q[0] + q[3] * q[3] + q[5] * q[5] +
+                                   2.0 * (q[1] * q[1] + q[2] * q[2] + q[4] * q[4]));
+            md.qrel = M > 0 ? float(1.5 * qf / M) : 0.f;
+        }
         mom[p] = md;
         nd.cx = md.cx; nd.cy = md.cy; nd.cz = md.cz; nd.rcrit2 = md.rcrit2;
         nd.info |= nchild << 8;

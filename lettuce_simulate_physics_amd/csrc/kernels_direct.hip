@@ -156,6 +156,49 @@ void launch_t(hipStream_t s, const float4* posm, int n, float e2, float G, int S
     }
 }
 
+// FP64 direct sum for a list of targets (validation of the FP32 paths): every body is a source,
+// the target itself skipped by index, pair terms in double with a correctly rounded square root.
+// Thread per target, sources split over blockIdx.y; partials (a_x, a_y, a_z, sum m/s) per split.
+template <class R>
+__global__ void __launch_bounds__(kBS) direct_f64_targets_kernel(const R* __restrict__ x, const R* __restrict__ y,
+                                                                 const R* __restrict__ z, const float* __restrict__ m,
+                                                                 int n, const uint32_t* __restrict__ tgt, int nt,
+                                                                 double e2, int jchunk, double* __restrict__ part) {
+    const int t = blockIdx.x * kBS + threadIdx.x;
+    const int split = blockIdx.y;
+    const int i = t < nt ? int(tgt[t]) : 0;
+    const double px = x[i], py = y[i], pz = z[i];
+    double sx = 0, sy = 0, sz = 0, sp = 0;
+    const int j0 = split * jchunk, j1 = min(n, j0 + jchunk);
+    for (int j = j0; j < j1; ++j) {
+        const double dx = double(x[j]) - px, dy = double(y[j]) - py, dz = double(z[j]) - pz;
+        const double r2 = dx * dx + dy * dy + dz * dz + e2;
+        if (j == i || !(r2 > 0)) continue;
+        const double inv = 1.0 / sqrt(r2), mj = m[j];
+        const double f = mj * inv * inv * inv;
+        sx += f * dx; sy += f * dy; sz += f * dz; sp += mj * inv;
+    }
+    if (t >= nt) return;
+    part[std::size_t(4 * split + 0) * nt + t] = sx;
+    part[std::size_t(4 * split + 1) * nt + t] = sy;
+    part[std::size_t(4 * split + 2) * nt + t] = sz;
+    part[std::size_t(4 * split + 3) * nt + t] = sp;
+}
+
+__global__ void __launch_bounds__(kBS) f64_targets_finish(const double* __restrict__ part, int nt, int S, double G,
+                                                          double* __restrict__ ax, double* __restrict__ ay,
+                                                          double* __restrict__ az, double* __restrict__ pot) {
+    const int t = blockIdx.x * kBS + threadIdx.x;
+    if (t >= nt) return;
+    double s[4] = {0, 0, 0, 0};
+    for (int sp = 0; sp < S; ++sp)
+        for (int c = 0; c < 4; ++c) s[c] += part[std::size_t(4 * sp + c) * nt + t];
+    ax[t] = G * s[0];
+    ay[t] = G * s[1];
+    az[t] = G * s[2];
+    pot[t] = -G * s[3];
+}
+
 }  // namespace
 
 int direct_split(int n) {
@@ -183,6 +226,39 @@ void launch_direct(hipStream_t s, const float4* posm, int n, float eps2, float G
         else { if (guard) LHIP_DIRECT(2, false, true); else LHIP_DIRECT(2, false, false); }
     }
 #undef LHIP_DIRECT
+}
+
+}  // namespace lhip
+
+namespace lhip {
+
+int direct_f64_split(int n, int nt) {
+    const long long blocks = (static_cast<long long>(nt) + kBS - 1) / kBS;
+    long long S = (2048 + blocks - 1) / blocks;
+    S = std::min<long long>(S, std::max(1, n / 256));
+    return int(std::max<long long>(1, std::min<long long>(S, 64)));
+}
+
+void launch_direct_f64_targets(hipStream_t s, const void* x, const void* y, const void* z, bool f64,
+                               const float* m, int n, const uint32_t* tgt, int nt, double eps, double G, double* ax,
+                               double* ay, double* az, double* pot, double* part) {
+    if (n <= 0 || nt <= 0) return;
+    const int S = direct_f64_split(n, nt);
+    const int jchunk = (n + S - 1) / S;
+    const dim3 grid(grid_for(nt, kBS), S);
+    if (f64)
+        direct_f64_targets_kernel<double><<<grid, kBS, 0, s>>>(static_cast<const double*>(x),
+                                                               static_cast<const double*>(y),
+                                                               static_cast<const double*>(z), m, n, tgt, nt,
+                                                               eps * eps, jchunk, part);
+    else
+        direct_f64_targets_kernel<float><<<grid, kBS, 0, s>>>(static_cast<const float*>(x),
+                                                              static_cast<const float*>(y),
+                                                              static_cast<const float*>(z), m, n, tgt, nt, eps * eps,
+                                                              jchunk, part);
+    LHIP_LAUNCH_CHECK();
+    f64_targets_finish<<<grid_for(nt, kBS), kBS, 0, s>>>(part, nt, S, G, ax, ay, az, pot);
+    LHIP_LAUNCH_CHECK();
 }
 
 }  // namespace lhip

@@ -18,15 +18,20 @@
 //
 // Evaluation. Accepted cells are staged in LDS with their moments, bodies of opened leaves with
 // their positions. Walk groups hold 1-128 bodies (39 on average for a Plummer sphere), so one body
-// per lane would idle most lanes. Instead the wave has V = 64 * NS virtual slots (NS = 1 for groups
-// of <= 32 bodies, else 2 per lane); with S = floor(V / nb) list shares, slot v evaluates body
-// v mod nb against list entries e = v div nb (mod S). Partial sums of the S shares are added in
+// per lane would idle most lanes. Instead the wave has V = 64 * NS virtual slots (NS = 1 or 2 per
+// lane, whichever keeps more lanes busy); with S = min(16, floor(V / nb)) list shares, slot v
+// evaluates body v mod nb against list entries e = v div nb (mod S). Partial sums of the S shares are added in
 // LDS at the end. Lists are evaluated whenever the next append would overflow them. Before an
 // evaluation the list is padded with zero-mass entries to a multiple of S, so the loops carry no
 // bounds test; a zero-mass entry adds exactly zero. With eps > 0 a body's own entry adds exactly
 // zero too (d = 0), so only the potential and eps = 0 need the per-pair self test; every body of a
 // group meets itself exactly once (the leaves holding the group are always opened), which is how
 // the self pairs are taken out of the interaction counts.
+//
+// Walk options. small_leaf (the reference's 4) is the largest accepted leaf summed body by body;
+// quad_tau > 0 turns accepted cells whose quadrupole term is small, qrel = 1.5 |Q|_F / M <
+// quad_tau * dist^2(box, com), into point masses in the body list (an error-budgeted far field; 0,
+// the reference's choice, keeps every quadrupole).
 //
 // Mixed-precision form (REL). Positions and node centres of mass are FP64. The traversal is the
 // same (float group box and float node records); the evaluation runs on FP32 coordinates relative
@@ -48,7 +53,7 @@ constexpr int kWalkWaves = 4;
 constexpr int kWalkBS = 64 * kWalkWaves;
 constexpr int kCellCap = 64;    // cells staged per wave
 constexpr int kBodyCap = 128;   // bodies staged per wave (>= 128: also the share-reduction buffer)
-constexpr int kMaxShare = 8;
+constexpr int kMaxShare = 16;
 
 struct Slot {
     float x, y, z;
@@ -169,7 +174,7 @@ __device__ __forceinline__ uint32_t lanes_below(unsigned long long mask) {
     return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
 }
 
-constexpr int kStackCap = 1024;
+constexpr int kStackCap = 896;
 // Deepest possible tree: levels 0..21 (no key restarts).
 constexpr int kStackThr = kStackCap - 7 * kLevels - 8;
 
@@ -184,7 +189,7 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
     int* st = stk[w];
     const int ngroups = int(a.gcount[a.n]);
     const float e2 = a.eps2;
-    unsigned long long t_inter = 0, t_p2p = 0, t_p2c = 0, t_listed = 0;
+    unsigned long long t_inter = 0, t_p2p = 0, t_p2c = 0, t_listed = 0, t_p2m = 0;
     bool overflow = false;
 
     while (true) {
@@ -196,9 +201,12 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
         const bool chunked = ghi - glo > kGroupMax;
         for (int clo = glo; clo < ghi; clo += kGroupMax) {
             const int chi = min(ghi, clo + kGroupMax);
-            const int nbody = chi - clo;                      // 1..128, wave-uniform
-            const int NS = nbody > 32 ? 2 : 1;
-            const int S = min(kMaxShare, (64 * NS) / nbody);  // list shares
+            const int nbody = chi - clo;  // 1..128, wave-uniform
+            // Slots per lane and list shares: the pair (NS, S) that keeps the most lanes busy,
+            // S * nbody of the 64 * NS slots (NS = 2 wins when it more than doubles S).
+            const int S1 = min(kMaxShare, 64 / nbody), S2 = min(kMaxShare, 128 / nbody);
+            const int NS = S2 > 2 * S1 ? 2 : 1;
+            const int S = NS == 2 ? S2 : S1;
             Slot sl[2];
             float mnx = FLT_MAX, mny = FLT_MAX, mnz = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX, mxz = -FLT_MAX;
 #pragma unroll
@@ -247,7 +255,7 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
             };
 
             int nc = 0, nb = 0;
-            uint32_t ncell_total = 0, nbody_total = 0;
+            uint32_t ncell_total = 0, nbody_total = 0, nmono_total = 0;  // body list = bodies + monopoles
             auto flush_cells = [&]() {
                 if (lane < kPad) {
                     L.c[nc + lane].A = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -298,8 +306,12 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                 const float d2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
                 const bool acc = act && !keep_open && d2 >= nd.rcrit2;
                 const bool leaf = act && info_leaf(nd.info);
-                const bool small = leaf && (nd.hi - nd.lo) <= kSmallLeaf;
-                const bool is_cell = acc && !small;
+                const bool small = leaf && (nd.hi - nd.lo) <= a.small_leaf;
+                bool is_cell = acc && !small;
+                // Error-budgeted far field: an accepted cell whose quadrupole term is below quad_tau
+                // of its monopole at the group box's distance enters the body list as a point mass.
+                const bool is_mono = is_cell && a.quad_tau > 0.f && a.mom[k].qrel < a.quad_tau * d2;
+                is_cell = is_cell && !is_mono;
                 const bool is_range = leaf && (!acc || small);
                 const bool open_int = act && !acc && !leaf;
 
@@ -332,6 +344,24 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                     }
                     nc += ncnew;
                 }
+                const unsigned long long mm = __ballot(is_mono);
+                const int nmnew = __popcll(mm);
+                if (nmnew) {
+                    if (nb + nmnew > kBodyCap) flush_bodies();
+                    if (is_mono) {
+                        const int slot = nb + int(lanes_below(mm));
+                        const NodeMom& md = a.mom[k];
+                        if (REL) {
+                            const NodeMomD& dm = a.momd[k];
+                            L.bP[slot] = make_float4(float(dm.cx - ccx), float(dm.cy - ccy), float(dm.cz - ccz), md.m);
+                        } else {
+                            L.bP[slot] = make_float4(md.cx, md.cy, md.cz, md.m);
+                        }
+                        L.bI[slot] = -1;
+                    }
+                    nb += nmnew;
+                    nmono_total += uint32_t(nmnew);
+                }
                 // Bodies of opened leaves: staged in chunks of at most the free buffer space.
                 const unsigned long long mr = __ballot(is_range);
                 if (mr) {
@@ -362,7 +392,7 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
             flush_bodies();
 
             // Add the list shares of each body (slot v = share * nbody + body) through LDS.
-            const uint32_t listed = ncell_total + nbody_total;
+            const uint32_t listed = ncell_total + nbody_total;  // nbody_total includes the monopoles
             if (REL) {
                 D4* red = reinterpret_cast<D4*>(&L);  // the lists are done: their LDS holds 128 D4
                 __builtin_amdgcn_wave_barrier();
@@ -417,8 +447,9 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
             if (lane == 0) {
                 // Each body of the chunk meets itself once in the body list.
                 t_listed += (unsigned long long)listed * nbody;
-                t_p2c += (unsigned long long)ncell_total * nbody;
-                t_p2p += (unsigned long long)nbody_total * nbody - nbody;
+                t_p2c += (unsigned long long)(ncell_total + nmono_total) * nbody;
+                t_p2m += (unsigned long long)nmono_total * nbody;
+                t_p2p += (unsigned long long)(nbody_total - nmono_total) * nbody - nbody;
                 t_inter += (unsigned long long)listed * nbody - nbody;
             }
         }
@@ -428,16 +459,18 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
     t_p2p = wave_sum_u64(t_p2p);
     t_p2c = wave_sum_u64(t_p2c);
     t_listed = wave_sum_u64(t_listed);
+    t_p2m = wave_sum_u64(t_p2m);
     if (lane == 0) {
         atomicAdd(&a.counters->interactions, t_inter);
         atomicAdd(&a.counters->p2p, t_p2p);
         atomicAdd(&a.counters->p2c, t_p2c);
         atomicAdd(&a.counters->listed, t_listed);
+        atomicAdd(&a.counters->p2m, t_p2m);
     }
 }
 
 __global__ void reset_walk(WalkCounters* __restrict__ c) {
-    c->interactions = c->p2p = c->p2c = c->listed = 0;
+    c->interactions = c->p2p = c->p2c = c->listed = c->p2m = 0;
     c->error = 0;
     c->next_group = 0;
 }

@@ -10,6 +10,7 @@ struct alignas(16) WalkCounters {
     unsigned long long p2p;           // body-body (self excluded)
     unsigned long long p2c;           // body-node
     unsigned long long listed;        // list entries per body summed (the reference CPU count)
+    unsigned long long p2m;           // body-node interactions evaluated as monopoles (in p2c)
     int error;                        // nonzero: walk failure (traversal stack overflow)
     int overfull;                     // bodies in finest-level cells with > 16 bodies
     unsigned int next_group;          // walk work counter
@@ -95,6 +96,8 @@ struct WalkArgs {
     float* cost;               // per-body interactions (self excluded), may be null
     WalkCounters* counters;
     int blocks;                // persistent grid size (blocks of 4 waves)
+    int small_leaf;            // accepted leaves with at most this many bodies are summed directly
+    float quad_tau;            // error-budgeted far field threshold (0: every quadrupole kept)
 };
 void tree_walk(hipStream_t s, const WalkArgs& a, bool potential);
 int walk_blocks_per_cu();

@@ -106,10 +106,14 @@ public:
 // The HIP factory family, beside the reference's make_cpu_backend/make_cuda_backend.
 std::unique_ptr<Backend> make_hip_backend(int device, std::string& why);
 // Options of the HIP backend with no reference counterpart: the mixed-precision form (FP64 state,
-// FP32 relative-coordinate pair arithmetic) and the walk group size (128 = the reference's).
+// FP32 relative-coordinate pair arithmetic) and the walk options; negative values keep the mode's
+// defaults (plain: the reference walk, group 128, small leaf 4, every quadrupole; mixed: group 128,
+// small leaf 1, quadrupoles dropped below 0.05 of the monopole term).
 struct HipOptions {
     bool mixed = false;
-    int group_max = 128;
+    int group_max = -1;
+    int small_leaf = -1;
+    double quad_tau = -1;
 };
 std::unique_ptr<Backend> make_hip_backend(int device, std::string& why, const HipOptions& opt);
 bool hip_compiled();

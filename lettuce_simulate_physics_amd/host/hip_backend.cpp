@@ -147,7 +147,14 @@ std::unique_ptr<Backend> make_hip_backend(int device, std::string& why, const Hi
         why = buf[0] ? buf : "hip backend unavailable";
         return nullptr;
     }
-    if (lhip_set_precision(c, opt.mixed ? LHIP_MODE_MIXED : LHIP_MODE_PLAIN, opt.group_max) != LHIP_OK) {
+    lhip_walk_options w{};
+    const int mode = opt.mixed ? LHIP_MODE_MIXED : LHIP_MODE_PLAIN;
+    lhip_set_precision(c, mode, nullptr);  // the mode's defaults, then any explicit option
+    lhip_get_precision(c, nullptr, &w);
+    if (opt.group_max >= 0) w.group_max = opt.group_max;
+    if (opt.small_leaf >= 0) w.small_leaf = opt.small_leaf;
+    if (opt.quad_tau >= 0) w.quad_tau = opt.quad_tau;
+    if (lhip_set_precision(c, mode, &w) != LHIP_OK) {
         why = lhip_last_error(c);
         lhip_destroy(c);
         return nullptr;

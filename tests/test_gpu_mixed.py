@@ -14,11 +14,21 @@ from helpers import per_body_rel, rel_rms
 pytestmark = pytest.mark.gpu
 
 
+REF_WALK = dict(group_max=128, small_leaf=4, quad_tau=0.0)  # the reference walk in the mixed form
+
+
 @pytest.fixture
 def mixed(hip):
-    hip.set_precision("mixed", 64)
+    hip.set_precision("mixed", **REF_WALK)
     yield hip
-    hip.set_precision("plain", 128)
+    hip.set_precision("plain")
+
+
+@pytest.fixture
+def budget(hip):
+    hip.set_precision("mixed")  # the mixed form's defaults: error-budgeted far field
+    yield hip
+    hip.set_precision("plain")
 
 
 def accel_pot_by_id(be, b, p, potential=True):
@@ -74,14 +84,14 @@ def test_mixed_topology_equals_plain_with_reference_groups(hip):
     # mixed form builds the plain form's tree exactly.
     b = O.plummer(50000, 24)
     p = L.ForceParams(G=1.0, eps=0.01, theta=0.6, method="tree")
-    hip.set_precision("plain", 128)
+    hip.set_precision("plain")
     hip.upload(b)
     hip.compute_forces(p, False)
     keys, groups, nodes = hip.tree_keys(), hip.tree_groups(), hip.tree_nodes()
     ids = hip.download().id
     counts = hip.times()
     try:
-        hip.set_precision("mixed", 128)
+        hip.set_precision("mixed", **REF_WALK)
         hip.upload(b)
         hip.compute_forces(p, False)
         assert np.array_equal(hip.tree_keys(), keys)
@@ -93,7 +103,7 @@ def test_mixed_topology_equals_plain_with_reference_groups(hip):
         t = hip.times()
         assert t["listed"] == counts["listed"] and t["p2c"] == counts["p2c"]
     finally:
-        hip.set_precision("plain", 128)
+        hip.set_precision("plain")
 
 
 def test_smaller_groups_cut_interactions_at_equal_accuracy(hip):
@@ -104,11 +114,11 @@ def test_smaller_groups_cut_interactions_at_equal_accuracy(hip):
     res = {}
     try:
         for g in (128, 64, 32):
-            hip.set_precision("mixed", g)
+            hip.set_precision("mixed", group_max=g, small_leaf=4, quad_tau=0.0)
             a, _ = accel_pot_by_id(hip, b, p, False)
             res[g] = (hip.times()["interactions"], np.median(per_body_rel(a[sub], ref)))
     finally:
-        hip.set_precision("plain", 128)
+        hip.set_precision("plain")
     print({g: (int(i), f"{m:.3g}") for g, (i, m) in res.items()})
     assert res[64][0] < res[128][0] and res[32][0] < res[64][0]
     assert res[32][1] < 1e-3
@@ -186,27 +196,31 @@ def test_mixed_kdk_conserves_energy_and_momentum(mixed):
 
 def test_mode_switch_converts_state(hip):
     b = O.plummer(3000, 29)
-    hip.set_precision("plain", 128)
+    hip.set_precision("plain")
     hip.upload(b)
     try:
-        hip.set_precision("mixed", 64)
+        hip.set_precision("mixed", group_max=64)
         x, v, m, ids = hip.download_f64()
         assert np.array_equal(x[:, 0], b.x.astype(np.float64)) and np.array_equal(v[:, 2], b.vz.astype(np.float64))
-        assert hip.precision() == ("mixed", 64)
-        hip.set_precision("plain", 128)
+        assert hip.precision() == ("mixed", dict(group_max=64, small_leaf=1, quad_tau=0.05))
+        hip.set_precision("plain")
         back = hip.download()
         assert np.array_equal(back.x, b.x) and np.array_equal(back.id, b.id)
     finally:
-        hip.set_precision("plain", 128)
+        hip.set_precision("plain")
 
 
 def test_mixed_mode_errors(hip):
-    hip.set_precision("plain", 128)
+    hip.set_precision("plain")
     with pytest.raises(L.LogicError):
         hip.upload_f64(np.zeros((4, 3)), np.zeros((4, 3)), np.ones(4, np.float32))
     with pytest.raises(ValueError):
-        hip.set_precision("mixed", 8)
-    assert hip.precision() == ("plain", 128)
+        hip.set_precision("mixed", group_max=8)
+    with pytest.raises(ValueError):
+        hip.set_precision("mixed", small_leaf=17)
+    with pytest.raises(ValueError):
+        hip.set_precision("mixed", quad_tau=-1.0)
+    assert hip.precision() == ("plain", dict(group_max=128, small_leaf=4, quad_tau=0.0))
 
 
 def test_mixed_disk_sech2_against_fp64_direct(mixed):
@@ -219,3 +233,51 @@ def test_mixed_disk_sech2_against_fp64_direct(mixed):
     e = per_body_rel(a[sub], ref)
     print(f"disk mixed tree theta 0.5: median {np.median(e):.3g} p99 {np.quantile(e, 0.99):.3g}")
     assert np.median(e) < 1e-3
+
+
+def test_direct_f64_targets_matches_the_oracle(hip):
+    # The device FP64 direct sum (the force-error oracle at large N) against the CPU one, in both
+    # state precisions; before any tree the backend order is the upload order.
+    b = L.plummer(20000, 31)
+    tg = np.arange(0, 20000, 7, dtype=np.uint32)
+    ref, rphi = O.direct_f64(b, 1.0, 0.01, tg.astype(np.int64))
+    try:
+        for mode in ("plain", "mixed"):
+            hip.set_precision(mode)
+            hip.upload(b)
+            a, phi = hip.direct_f64_targets(tg, 1.0, 0.01)
+            assert rel_rms(a, ref) < 1e-13, mode
+            assert np.max(np.abs(phi - rphi) / np.abs(rphi)) < 1e-13, mode
+    finally:
+        hip.set_precision("plain")
+
+
+def test_budgeted_far_field_stays_within_the_error_budget(budget):
+    # The mixed form's default walk (quadrupoles kept only where they exceed 0.05 of the monopole
+    # term, accepted leaves as cells): median error <= 1e-3 against FP64 direct, most body-node
+    # interactions evaluated as monopoles.
+    b = O.plummer(60000, 32)
+    a, _ = accel_pot_by_id(budget, b, L.ForceParams(G=1.0, eps=0.01, theta=0.6, method="tree"), False)
+    t = budget.times()
+    sub = np.random.default_rng(10).choice(len(b), 4096, replace=False)
+    ref, _ = O.direct_f64(b, 1.0, 0.01, sub)
+    e = per_body_rel(a[sub], ref)
+    print(f"budgeted far field: median {np.median(e):.3g} p99 {np.quantile(e, 0.99):.3g}, "
+          f"monopoles {t['p2c_mono'] / max(t['p2c'], 1):.2f} of p2c")
+    assert np.median(e) < 1e-3 and np.quantile(e, 0.99) < 5e-3
+    assert t["p2c_mono"] > 0.5 * t["p2c"]
+    assert t["interactions"] == t["p2p"] + t["p2c"]
+
+
+def test_reference_walk_options_reproduce_the_plain_lists(hip):
+    # quad_tau = 0 and small_leaf = 4 in the plain form is the reference walk: no monopoles.
+    b = O.plummer(20000, 33)
+    hip.set_precision("plain")
+    hip.upload(b)
+    hip.compute_forces(L.ForceParams(G=1.0, eps=0.01, theta=0.6, method="tree"), False)
+    t = hip.times()
+    assert t["p2c_mono"] == 0
+    o = O.Oracle()
+    o.upload(b)
+    o.compute_forces(1.0, 0.01, 0.6, 1)
+    assert t["listed"] == o.counts()["listed"] and t["p2c"] == o.counts()["p2c"]
