@@ -48,7 +48,7 @@ struct Context {
     // stay in m and id; the float position arrays are then only download staging.
     bool mixed = false;
     lhip_walk_options wopt{kGroupMax, kSmallLeaf, 0.0};  // the reference's walk
-    DevBuf xd, yd, zd, vxd, vyd, vzd, xd2, yd2, zd2, vxd2, vyd2, vzd2, axd, ayd, azd, potd, momd, rec64;
+    DevBuf xd, yd, zd, vxd, vyd, vzd, xd2, yd2, zd2, vxd2, vyd2, vzd2, axd, ayd, azd, potd, momd, rec64, posd;
 
     bool has_pe = false, acc_ok = false, tree_ok = false, costs_ok = false;
     bool kick_rec = false, drift_rec = false;
@@ -356,10 +356,13 @@ struct Context {
             tb.x = D(xd); tb.y = D(yd); tb.z = D(zd); tb.vx = D(vxd); tb.vy = D(vyd); tb.vz = D(vzd);
             tb.x2 = D(xd2); tb.y2 = D(yd2); tb.z2 = D(zd2); tb.vx2 = D(vxd2); tb.vy2 = D(vyd2); tb.vz2 = D(vzd2);
             tb.posm = nullptr;
+            posd.reserve(std::size_t(n) * sizeof(double4));
+            tb.posd = posd.as<double4>();
         } else {
             tb.x = F(x); tb.y = F(y); tb.z = F(z); tb.vx = F(vx); tb.vy = F(vy); tb.vz = F(vz);
             tb.x2 = F(x2); tb.y2 = F(y2); tb.z2 = F(z2); tb.vx2 = F(vx2); tb.vy2 = F(vy2); tb.vz2 = F(vz2);
             tb.posm = posm.as<float4>();
+            tb.posd = nullptr;
         }
         tb.m = m.as<float>(); tb.id = id.as<uint32_t>(); tb.cost = cost.as<float>();
         tb.m2 = m2.as<float>(); tb.id2 = id2.as<uint32_t>(); tb.cost2 = cost2.as<float>();
@@ -467,8 +470,7 @@ struct Context {
             a.ax = ax.as<float>(); a.ay = ay.as<float>(); a.az = az.as<float>(); a.pot = pot.as<float>();
             a.cost = cost.as<float>();
             if (mixed) {
-                a.xd = D(xd); a.yd = D(yd); a.zd = D(zd);
-                a.m = m.as<float>();
+                a.posd = posd.as<double4>();
                 a.momd = momd.as<NodeMomD>();
                 a.axd = D(axd); a.ayd = D(ayd); a.azd = D(azd); a.potd = D(potd);
                 a.Gd = p.G;
@@ -593,7 +595,7 @@ struct Context {
                                 &cost2, &ax, &ay, &az, &pot, &posm, &key, &key2, &idx, &idx2, &tile_hist, &direct_part,
                                 &bl, &cnt, &gflag, &scan_tmp, &pkey, &pkey2, &pval, &pval2, &inv, &node, &mom, &prefix, &gstart,
                                 &xd, &yd, &zd, &vxd, &vyd, &vzd, &xd2, &yd2, &zd2, &vxd2, &vyd2, &vzd2, &axd, &ayd, &azd,
-                                &potd, &momd, &rec64})
+                                &potd, &momd, &rec64, &posd})
             held += b->capacity();
         *fr = f + held;
         *tot = tt;

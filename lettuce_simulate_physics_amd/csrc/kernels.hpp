@@ -65,6 +65,7 @@ struct TreeBufs {
     uint32_t* id2;
     float* cost2;
     float4* posm;  // float (position, mass) in sorted order, may be null
+    double4* posd; // FP64 (position, mass) in sorted order (mixed form), may be null
     uint64_t *key, *key2;
     uint32_t *idx, *idx2;
     uint32_t* tile_hist;  // radix tile histograms

@@ -236,7 +236,8 @@ __global__ void __launch_bounds__(kBS) permute_kernel(const uint32_t* __restrict
                                                       R* __restrict__ y2, R* __restrict__ z2, R* __restrict__ vx2,
                                                       R* __restrict__ vy2, R* __restrict__ vz2,
                                                       float* __restrict__ m2, uint32_t* __restrict__ id2,
-                                                      float* __restrict__ cost2, float4* __restrict__ posm) {
+                                                      float* __restrict__ cost2, float4* __restrict__ posm,
+                                                      double4* __restrict__ posd) {
     const int i = blockIdx.x * kBS + threadIdx.x;
     if (i >= n) return;
     const uint32_t j = perm[i];
@@ -247,6 +248,7 @@ __global__ void __launch_bounds__(kBS) permute_kernel(const uint32_t* __restrict
     id2[i] = id[j];
     if (cost) cost2[i] = cost[j];
     if (posm) posm[i] = make_float4(float(px), float(py), float(pz), pm);
+    if (posd) posd[i] = make_double4(double(px), double(py), double(pz), double(pm));
 }
 
 template <class R>
@@ -267,7 +269,7 @@ void sort_impl(hipStream_t s, TreeBufs& b, int n, bool permute_cost) {
     permute_kernel<R><<<grid_for(n, kBS), kBS, 0, s>>>(va, n, x, y, z, P(b.vx), P(b.vy), P(b.vz), b.m, b.id,
                                                       permute_cost ? b.cost : nullptr, P(b.x2), P(b.y2), P(b.z2),
                                                       P(b.vx2), P(b.vy2), P(b.vz2), b.m2, b.id2,
-                                                      permute_cost ? b.cost2 : nullptr, b.posm);
+                                                      permute_cost ? b.cost2 : nullptr, b.posm, b.posd);
     LHIP_LAUNCH_CHECK();
 }
 

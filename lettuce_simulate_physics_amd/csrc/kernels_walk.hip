@@ -145,13 +145,17 @@ __device__ __forceinline__ void eval_bodies(Slot (&sl)[2], int S, int nb, const 
     }
 }
 
+// Inclusive prefix sum over the wave: Hillis-Steele within each row of 16 lanes by DPP row shifts
+// (a lane whose source lies outside its row adds 0), then the totals of the rows below.
 __device__ __forceinline__ int prefix_sum_incl(int v, int lane) {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int u = __shfl_up(v, d, 64);
-        if (lane >= d) v += u;
-    }
-    return v;
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    const int r0 = __builtin_amdgcn_readlane(v, 15), r1 = __builtin_amdgcn_readlane(v, 31);
+    const int r2 = __builtin_amdgcn_readlane(v, 47);
+    const int row = lane >> 4;
+    return v + (row == 0 ? 0 : row == 1 ? r0 : row == 2 ? r0 + r1 : r0 + r1 + r2);
 }
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -218,8 +222,12 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                 if (!sl[s].on) sl[s].sp = 0;
                 sl[s].i = clo + sl[s].bi;
                 float4 p;
-                if (REL) p = make_float4(float(a.xd[sl[s].i]), float(a.yd[sl[s].i]), float(a.zd[sl[s].i]), 0.f);
-                else p = a.posm[sl[s].i];
+                if (REL) {
+                    const double4 d = a.posd[sl[s].i];
+                    p = make_float4(float(d.x), float(d.y), float(d.z), 0.f);
+                } else {
+                    p = a.posm[sl[s].i];
+                }
                 sl[s].x = p.x; sl[s].y = p.y; sl[s].z = p.z;
                 sl[s].sx = sl[s].sy = sl[s].sz = sl[s].sp_pot = 0.f;
                 sl[s].dx = sl[s].dy = sl[s].dz = sl[s].dp = 0.0;
@@ -237,10 +245,10 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
             if (REL) {
 #pragma unroll
                 for (int s = 0; s < 2; ++s) {
-                    const int i = sl[s].i;
-                    sl[s].x = float(a.xd[i] - ccx);
-                    sl[s].y = float(a.yd[i] - ccy);
-                    sl[s].z = float(a.zd[i] - ccz);
+                    const double4 d = a.posd[sl[s].i];
+                    sl[s].x = float(d.x - ccx);
+                    sl[s].y = float(d.y - ccy);
+                    sl[s].z = float(d.z - ccz);
                 }
             }
             auto fold = [&]() {
@@ -374,14 +382,27 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                         if (nb == kBodyCap) flush_bodies();
                         const int take = min(kBodyCap - nb, total - done);
                         const int u0 = max(off, done), u1 = min(off + len, done + take);
-                        for (int u = u0; u < u1; ++u) {
-                            const int j = nd.lo + (u - off);
-                            if (REL)
-                                L.bP[nb + (u - done)] = make_float4(float(a.xd[j] - ccx), float(a.yd[j] - ccy),
-                                                                    float(a.zd[j] - ccz), a.m[j]);
-                            else
-                                L.bP[nb + (u - done)] = a.posm[j];
-                            L.bI[nb + (u - done)] = j;
+                        // Four loads in flight per lane before their LDS stores (a load past u1 reads
+                        // the range's last body again and is not stored).
+                        for (int u = u0; u < u1; u += 4) {
+                            float4 q[4];
+#pragma unroll
+                            for (int t = 0; t < 4; ++t) {
+                                const int j = nd.lo + (min(u + t, u1 - 1) - off);
+                                if (REL) {
+                                    const double4 d = a.posd[j];
+                                    q[t] = make_float4(float(d.x - ccx), float(d.y - ccy), float(d.z - ccz),
+                                                       float(d.w));
+                                } else {
+                                    q[t] = a.posm[j];
+                                }
+                            }
+#pragma unroll
+                            for (int t = 0; t < 4; ++t)
+                                if (u + t < u1) {
+                                    L.bP[nb + (u + t - done)] = q[t];
+                                    if (MASK) L.bI[nb + (u + t - done)] = nd.lo + (u + t - off);
+                                }
                         }
                         nb += take;
                         done += take;
@@ -506,7 +527,7 @@ void tree_walk(hipStream_t s, const WalkArgs& a, bool potential) {
     const int blocks = std::max(1, a.blocks);
     reset_walk<<<1, 1, 0, s>>>(a.counters);
     LHIP_LAUNCH_CHECK();
-    if (a.xd) walk_launch<true>(s, a, potential, guard, blocks);
+    if (a.posd) walk_launch<true>(s, a, potential, guard, blocks);
     else walk_launch<false>(s, a, potential, guard, blocks);
     LHIP_LAUNCH_CHECK();
 }

@@ -76,9 +76,8 @@ void tree_build(hipStream_t s, const BuildBufs& b, float inv_theta);
 
 struct WalkArgs {
     const float4* posm;        // plain form: sorted (position, mass)
-    // mixed form (xd != null): FP64 positions, float masses, FP64 node moments, FP64 outputs
-    const double *xd, *yd, *zd;
-    const float* m;
+    // mixed form (posd != null): FP64 (position, mass) records, FP64 node moments, FP64 outputs
+    const double4* posd;
     const NodeMomD* momd;
     double *axd, *ayd, *azd, *potd;
     double Gd;
