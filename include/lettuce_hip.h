@@ -183,6 +183,22 @@ int lhip_scenario_disk_sech2(const lhip_disk_params* p, int32_t com_frame, float
 int lhip_scenario_disk_sech2_f64(const lhip_disk_params* p, int32_t com_frame, double* x, double* y, double* z,
                                  double* vx, double* vy, double* vz, double* m, uint32_t* id);
 
+/* Two-disk encounter (no reference counterpart: the reference's `collision` tilts one disk and
+ * aims by an escape-speed fraction, src/scenarios.cpp:248-271). Two sech^2 disks as above (seeds
+ * seed and seed + 1, spin +1), tilted by incl1 and incl2 about the x axis, their cores on a
+ * parabolic orbit in the xy plane with pericentre `pericentre`, `separation` apart and
+ * approaching, in the cores' centre-of-mass frame. Disk 1 is bodies [0, nstars + 1) with its core
+ * first, disk 2 follows. Outputs in FP64: positions and velocities as n x 3 (AoS), masses, ids. */
+typedef struct lhip_collision_params {
+    uint64_t nstars, seed;
+    double G, core, mass, rd, rmax, z0, sigma;
+    double pericentre, separation, incl1, incl2;
+} lhip_collision_params;
+int lhip_scenario_collision(const lhip_collision_params* p, double* x, double* v, double* m, uint32_t* id);
+/* Vector-Jacobian product of the encounter: given dL/dx and dL/dv (n x 3 each), returns
+ * out = (dL/dpericentre, dL/dincl1, dL/dincl2). */
+int lhip_scenario_collision_vjp(const lhip_collision_params* p, const double* gx, const double* gv, double out[3]);
+
 /* ---- inspection (no reference counterpart; for parity tests and tools) ---- */
 
 /* Accelerations and potential of the last force call, in the backend's current order (rounded to

@@ -179,6 +179,55 @@ def disk_sech2_f64(nstars: int, seed: int = 1, core: float = 10.0, mass: float =
     return np.stack(c[:3], 1), np.stack(c[3:6], 1), c[6]
 
 
+class CollisionParamsC(ctypes.Structure):
+    _fields_ = [("nstars", ctypes.c_uint64), ("seed", ctypes.c_uint64)] + \
+               [(k, ctypes.c_double) for k in ("G", "core", "mass", "rd", "rmax", "z0", "sigma", "pericentre",
+                                               "separation", "incl1", "incl2")]
+
+
+@dataclass
+class Collision:
+    """Two sech^2 disks on a parabolic encounter (see lhip_scenario_collision); the
+    differentiable parameters are pericentre, incl1 and incl2 (radians)."""
+    nstars: int = 524288
+    seed: int = 1
+    pericentre: float = 4.0
+    incl1: float = np.pi / 6
+    incl2: float = np.pi / 3
+    separation: float = 20.0
+    core: float = 10.0
+    mass: float = 1.0
+    rd: float = 1.0
+    rmax: float = 10.0
+    z0: float = 0.1
+    sigma: float = 0.1
+    G: float = 1.0
+
+    def c(self) -> CollisionParamsC:
+        return CollisionParamsC(self.nstars, self.seed, self.G, self.core, self.mass, self.rd, self.rmax, self.z0,
+                                self.sigma, self.pericentre, self.separation, self.incl1, self.incl2)
+
+    @property
+    def n(self) -> int:
+        return 2 * (self.nstars + 1)
+
+    def generate(self):
+        """(x (n x 3), v (n x 3), m) in FP64."""
+        x, v, m = np.zeros((self.n, 3)), np.zeros((self.n, 3)), np.zeros(self.n)
+        p = self.c()
+        _check_gen(library().lhip_scenario_collision(ctypes.byref(p), _ptr(x), _ptr(v), _ptr(m), None))
+        return x, v, m
+
+    def vjp(self, gx, gv):
+        """(dL/dpericentre, dL/dincl1, dL/dincl2) from dL/dx0 and dL/dv0."""
+        gx = np.ascontiguousarray(gx, np.float64).reshape(self.n, 3)
+        gv = np.ascontiguousarray(gv, np.float64).reshape(self.n, 3)
+        out = np.zeros(3)
+        p = self.c()
+        _check_gen(library().lhip_scenario_collision_vjp(ctypes.byref(p), _ptr(gx), _ptr(gv), _ptr(out)))
+        return out
+
+
 def hip_compiled() -> bool:
     try:
         return bool(library().lhip_compiled())

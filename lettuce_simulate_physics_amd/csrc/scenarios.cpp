@@ -151,6 +151,47 @@ void place(Columns& b, double incl, const double p[3], const double v[3]) {
 
 int checked(uint64_t n) { return n >= uint64_t(INT32_MAX) ? LHIP_INVALID : LHIP_OK; }
 
+// Two-disk encounter. Each disk (disk_local, spin +1) is tilted by its inclination about the x axis
+// and its core placed on a parabolic relative orbit in the xy plane: cores r0 apart, pericentre q,
+// approaching, in the frame of the two cores' centre of mass. True anomaly at the start:
+// cos f0 = 2q/r0 - 1, f0 < 0; relative position r0 (cos f0, sin f0, 0); relative velocity
+// sqrt(G Mt / 2q) (-sin f0, 1 + cos f0, 0) (parabola, semi-latus rectum 2q). Disk 1 (bodies
+// [0, n1)) sits at -(M2/Mt) r, disk 2 at +(M1/Mt) r, M_k = core + disk mass.
+struct Encounter {
+    Columns d1, d2;  // disk-local bodies (core first)
+    double M1, M2, Mt;
+    double P1[3], P2[3], V1[3], V2[3];        // core positions and velocities
+    double dP1[3], dP2[3], dV1[3], dV2[3];    // their derivatives in q
+};
+
+Encounter encounter(const lhip_collision_params* p) {
+    Encounter e{disk_local(p->nstars, p->seed, p->G, p->core, p->mass, p->rd, p->rmax, p->z0, p->sigma, 1.0),
+                disk_local(p->nstars, p->seed + 1, p->G, p->core, p->mass, p->rd, p->rmax, p->z0, p->sigma, 1.0),
+                0, 0, 0, {}, {}, {}, {}, {}, {}, {}, {}};
+    e.M1 = e.M2 = p->core + p->mass;
+    e.Mt = e.M1 + e.M2;
+    const double q = p->pericentre, r0 = p->separation;
+    const double cf = 2 * q / r0 - 1, f = -std::acos(cf), sf = std::sin(f);
+    const double dfdq = -2.0 / (r0 * sf);
+    const double k = std::sqrt(p->G * e.Mt / (2 * q)), dkdq = -k / (2 * q);
+    const double r[3] = {r0 * cf, r0 * sf, 0}, dr[3] = {-r0 * sf * dfdq, r0 * cf * dfdq, 0};
+    const double v[3] = {-k * sf, k * (1 + cf), 0};
+    const double dv[3] = {-dkdq * sf - k * cf * dfdq, dkdq * (1 + cf) - k * sf * dfdq, 0};
+    const double w1 = -e.M2 / e.Mt, w2 = e.M1 / e.Mt;
+    for (int c = 0; c < 3; ++c) {
+        e.P1[c] = w1 * r[c]; e.P2[c] = w2 * r[c]; e.V1[c] = w1 * v[c]; e.V2[c] = w2 * v[c];
+        e.dP1[c] = w1 * dr[c]; e.dP2[c] = w2 * dr[c]; e.dV1[c] = w1 * dv[c]; e.dV2[c] = w2 * dv[c];
+    }
+    return e;
+}
+
+int encounter_checked(const lhip_collision_params* p) {
+    if (!p || p->nstars < 1 || checked(2 * (p->nstars + 1)) != LHIP_OK || !(p->rd > 0) || !(p->z0 > 0) ||
+        !(p->pericentre > 0) || !(p->separation > 2 * p->pericentre))
+        return LHIP_INVALID;
+    return LHIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -194,6 +235,65 @@ int lhip_scenario_disk_sech2_f64(const lhip_disk_params* p, int32_t com_frame_fl
     if (com_frame_flag) com_frame(b);
     double* out[7] = {x, y, z, vx, vy, vz, m};
     emit(b, out, id, 0);
+    return LHIP_OK;
+}
+
+int lhip_scenario_collision(const lhip_collision_params* p, double* x, double* v, double* m, uint32_t* id) {
+    if (encounter_checked(p) != LHIP_OK) return LHIP_INVALID;
+    const Encounter e = encounter(p);
+    const std::size_t n1 = e.d1.size();
+    const Columns* d[2] = {&e.d1, &e.d2};
+    const double inc[2] = {p->incl1, p->incl2};
+    const double* P[2] = {e.P1, e.P2};
+    const double* V[2] = {e.V1, e.V2};
+    for (int k = 0; k < 2; ++k) {
+        const double ci = std::cos(inc[k]), si = std::sin(inc[k]);
+        for (std::size_t i = 0; i < d[k]->size(); ++i) {
+            const std::size_t o = k * n1 + i;
+            const auto& c = d[k]->c;
+            if (x) {
+                x[3 * o + 0] = c[0][i] + P[k][0];
+                x[3 * o + 1] = c[1][i] * ci - c[2][i] * si + P[k][1];
+                x[3 * o + 2] = c[1][i] * si + c[2][i] * ci + P[k][2];
+            }
+            if (v) {
+                v[3 * o + 0] = c[3][i] + V[k][0];
+                v[3 * o + 1] = c[4][i] * ci - c[5][i] * si + V[k][1];
+                v[3 * o + 2] = c[4][i] * si + c[5][i] * ci + V[k][2];
+            }
+            if (m) m[o] = c[6][i];
+            if (id) id[o] = uint32_t(o);
+        }
+    }
+    return LHIP_OK;
+}
+
+int lhip_scenario_collision_vjp(const lhip_collision_params* p, const double* gx, const double* gv, double out[3]) {
+    if (encounter_checked(p) != LHIP_OK || !gx || !gv || !out) return LHIP_INVALID;
+    const Encounter e = encounter(p);
+    const std::size_t n1 = e.d1.size();
+    const Columns* d[2] = {&e.d1, &e.d2};
+    const double inc[2] = {p->incl1, p->incl2};
+    const double* dP[2] = {e.dP1, e.dP2};
+    const double* dV[2] = {e.dV1, e.dV2};
+    double gq = 0, gi[2] = {0, 0};
+    for (int k = 0; k < 2; ++k) {
+        // d/di of (y ci - z si, y si + z ci) = (-y si - z ci, y ci - z si)
+        const double ci = std::cos(inc[k]), si = std::sin(inc[k]);
+        for (std::size_t i = 0; i < d[k]->size(); ++i) {
+            const std::size_t o = k * n1 + i;
+            const auto& c = d[k]->c;
+            const double* ax = gx + 3 * o;
+            const double* av = gv + 3 * o;
+            gq += ax[0] * dP[k][0] + ax[1] * dP[k][1] + ax[2] * dP[k][2];
+            gq += av[0] * dV[k][0] + av[1] * dV[k][1] + av[2] * dV[k][2];
+            gi[k] += ax[1] * (-c[1][i] * si - c[2][i] * ci) + ax[2] * (c[1][i] * ci - c[2][i] * si);
+            gi[k] += av[1] * (-c[4][i] * si - c[5][i] * ci) + av[2] * (c[4][i] * ci - c[5][i] * si);
+        }
+    }
+    out[0] = gq;
+    out[1] = gi[0];
+    out[2] = gi[1];
     return LHIP_OK;
 }
 

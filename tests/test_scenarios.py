@@ -62,3 +62,33 @@ def test_generator_rejects_invalid_parameters():
         L.plummer(10, 1, a=-1.0)
     with pytest.raises(ValueError):
         L.disk_sech2(0)
+
+
+def test_collision_orbit_and_vjp_against_finite_differences():
+    C = L.Collision(nstars=2000, seed=3, pericentre=4.0, incl1=0.5, incl2=1.0)
+    x, v, m = C.generate()
+    n1 = C.nstars + 1
+    # cores: separation 20, relative energy zero (parabolic), relative angular momentum sqrt(2 G Mt q)
+    r = x[n1] - x[0]
+    u = v[n1] - v[0]
+    Mt = 2 * (C.core + C.mass)
+    assert abs(np.linalg.norm(r) - 20.0) < 1e-12
+    assert abs(0.5 * u @ u - Mt / np.linalg.norm(r)) < 1e-12
+    assert abs(np.linalg.norm(np.cross(r, u)) - np.sqrt(2 * Mt * 4.0)) < 1e-12
+    assert r @ u < 0  # approaching
+    # disk 2's spin axis is tilted by incl2 from +z about x
+    d = x[n1 + 1:] - x[n1]
+    w = v[n1 + 1:] - v[n1]
+    Lz = (m[n1 + 1:, None] * np.cross(d, w)).sum(0)
+    assert abs(np.arctan2(-Lz[1], Lz[2]) - 1.0) < 0.02
+    # VJP against central differences of a random linear functional
+    rng = np.random.default_rng(1)
+    gx, gv = rng.standard_normal(x.shape), rng.standard_normal(v.shape)
+    g = C.vjp(gx, gv)
+    for k, name in enumerate(("pericentre", "incl1", "incl2")):
+        h = 1e-6
+        a, b = L.Collision(**{**C.__dict__, name: getattr(C, name) + h}), L.Collision(**{**C.__dict__, name: getattr(C, name) - h})
+        xa, va, _ = a.generate()
+        xb, vb, _ = b.generate()
+        fd = ((gx * (xa - xb)).sum() + (gv * (va - vb)).sum()) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-6 * max(1.0, abs(fd)), (name, fd, g[k])
