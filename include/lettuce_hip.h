@@ -230,20 +230,29 @@ int lhip_tree_groups(lhip_ctx* ctx, uint32_t* gstart);
  * checkpoints every `checkpoint_every` steps on the device; the backward recomputes each segment from
  * its checkpoint and runs the adjoint sweep. Arrays are n x 3 (row-major) or n, host memory. */
 enum { LHIP_PRECISION_MIXED = 0, LHIP_PRECISION_F64 = 1 };
+/* Tree method: LHIP_TREE_BUILD builds a tree per force call; LHIP_TREE_RECORD also keeps each
+ * call's topology (sorted order and positions); LHIP_TREE_REPLAY runs the forward from new initial
+ * conditions on the recorded topology (the frozen-topology forward whose finite differences the
+ * adjoint matches) and keeps the recorded forward's checkpoints for the backward. */
+enum { LHIP_TREE_BUILD = 0, LHIP_TREE_RECORD = 1, LHIP_TREE_REPLAY = 2 };
 
 typedef struct lhip_sim_params {
     double G, eps, dt, theta;
     int32_t steps;
     int32_t checkpoint_every;
-    int32_t method;     /* LHIP_METHOD_DIRECT */
-    int32_t precision;  /* LHIP_PRECISION_MIXED or LHIP_PRECISION_F64 */
+    int32_t method;     /* LHIP_METHOD_DIRECT, or LHIP_METHOD_TREE (mixed-precision walk, theta) */
+    int32_t precision;  /* direct: LHIP_PRECISION_MIXED or LHIP_PRECISION_F64; tree: mixed */
+    int32_t tree_mode;  /* tree: LHIP_TREE_BUILD, LHIP_TREE_RECORD or LHIP_TREE_REPLAY */
+    int32_t reserved;
+    lhip_walk_options walk;  /* tree: walk options; group_max 0 selects the mixed mode's defaults */
 } lhip_sim_params;
 
 /* Runs the forward from (x0, v0, m); writes the final state (xT, vT may be NULL). */
 int lhip_sim_forward(lhip_ctx* ctx, const lhip_sim_params* p, uint64_t n, const double* x0, const double* v0,
                      const double* m, double* xT, double* vT);
 /* Given dL/dxT and dL/dvT, writes dL/dx0, dL/dv0, dL/dm (any output may be NULL). LHIP_LOGIC without
- * a forward. */
+ * a forward. The tree method runs on the context's bodies (the forward replaces them), keeps its
+ * checkpoints per body id, and gives no mass gradient (dL/dm must be NULL). */
 int lhip_sim_backward(lhip_ctx* ctx, const double* gxT, const double* gvT, double* gx0, double* gv0, double* gm);
 /* Device time of the last forward and backward (HIP events on the context's stream). */
 int lhip_sim_times(lhip_ctx* ctx, double* forward_ms, double* backward_ms);

@@ -12,15 +12,18 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import HipBackend, LogicError, METHOD_DIRECT, OK, _D, _P, _U64, library
+from . import (DEFAULT_WALK, HipBackend, LogicError, METHOD_DIRECT, METHOD_TREE, OK, WalkOptionsC, _D, _P, _U64,
+               library)
 
 PRECISION_MIXED, PRECISION_F64 = 0, 1
+TREE_MODES = {"build": 0, "record": 1, "replay": 2}
 
 
 class SimParamsC(ctypes.Structure):
     _fields_ = [("G", ctypes.c_double), ("eps", ctypes.c_double), ("dt", ctypes.c_double), ("theta", ctypes.c_double),
                 ("steps", ctypes.c_int32), ("checkpoint_every", ctypes.c_int32), ("method", ctypes.c_int32),
-                ("precision", ctypes.c_int32)]
+                ("precision", ctypes.c_int32), ("tree_mode", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("walk", WalkOptionsC)]
 
 
 @dataclass
@@ -30,13 +33,26 @@ class SimParams:
     dt: float = 1e-3
     steps: int = 200
     checkpoint_every: int = 20
-    precision: str = "mixed"  # "mixed" (FP32 pair arithmetic on FP64 separations) or "f64"
+    precision: str = "mixed"  # direct: "mixed" (FP32 pair arithmetic on FP64 separations) or "f64"
+    method: str = "direct"    # or "tree": the mixed-precision walk with frozen topology in the adjoint
+    theta: float = 0.6
+    tree_mode: str = "build"  # tree: "build", "record" (keep each step's topology) or "replay"
+    walk: dict | None = None  # tree walk options (group_max, small_leaf, quad_tau); None: mixed defaults
 
     def c(self) -> SimParamsC:
         if self.precision not in ("mixed", "f64"):
             raise ValueError("precision must be 'mixed' or 'f64'")
-        return SimParamsC(self.G, self.eps, self.dt, 0.0, int(self.steps), int(self.checkpoint_every), METHOD_DIRECT,
-                          PRECISION_F64 if self.precision == "f64" else PRECISION_MIXED)
+        if self.method not in ("direct", "tree"):
+            raise ValueError("method must be 'direct' or 'tree'")
+        w = WalkOptionsC(0, 0, 0.0)
+        if self.walk is not None:
+            d = DEFAULT_WALK["mixed"]
+            w = WalkOptionsC(int(self.walk.get("group_max", d[0])), int(self.walk.get("small_leaf", d[1])),
+                             float(self.walk.get("quad_tau", d[2])))
+        return SimParamsC(self.G, self.eps, self.dt, self.theta, int(self.steps), int(self.checkpoint_every),
+                          METHOD_TREE if self.method == "tree" else METHOD_DIRECT,
+                          PRECISION_F64 if self.precision == "f64" else PRECISION_MIXED, TREE_MODES[self.tree_mode], 0,
+                          w)
 
 
 def _f64(a, shape):
@@ -62,12 +78,15 @@ class Simulation:
         return xT, vT
 
     def backward(self, gxT, gvT):
+        """(dL/dx0, dL/dv0, dL/dm); dL/dm is None for the tree method (masses are not differentiated
+        through the tree)."""
         n = getattr(self, "n", None) or int(np.asarray(gxT).reshape(-1, 3).shape[0])
         gxT, gvT = _f64(gxT, (n, 3)), _f64(gvT, (n, 3))
-        gx0, gv0, gm = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+        gx0, gv0 = np.zeros((n, 3)), np.zeros((n, 3))
+        gm = None if self.p.method == "tree" else np.zeros(n)
         self.be._check(library().lhip_sim_backward(self.be._h, gxT.ctypes.data_as(_P), gvT.ctypes.data_as(_P),
                                                    gx0.ctypes.data_as(_P), gv0.ctypes.data_as(_P),
-                                                   gm.ctypes.data_as(_P)))
+                                                   None if gm is None else gm.ctypes.data_as(_P)))
         return gx0, gv0, gm
 
     def times(self):
@@ -96,7 +115,7 @@ def leapfrog_function():
             gxT = np.zeros((n, 3)) if gxT is None else gxT.detach().cpu().numpy()
             gvT = np.zeros((n, 3)) if gvT is None else gvT.detach().cpu().numpy()
             gx0, gv0, gm = ctx.sim.backward(gxT, gvT)
-            t = lambda a: torch.from_numpy(a).to(ctx.dev)  # noqa: E731
+            t = lambda a: None if a is None else torch.from_numpy(a).to(ctx.dev)  # noqa: E731
             return t(gx0), t(gv0), t(gm), None
 
     return Leapfrog

@@ -80,6 +80,11 @@ struct AdjSim {
 
 void adj_destroy(AdjSim* a) { delete a; }
 
+void treesim_forward(Context& c, const lhip_sim_params& p, int n, const double* x0, const double* v0, const double* m,
+                     double* xT, double* vT);
+void treesim_backward(Context& c, const double* gxT, const double* gvT, double* gx0, double* gv0);
+void treesim_times(Context& c, double* f, double* b);
+
 namespace {
 
 void to_soa(const double* aos, std::size_t n, std::vector<double>& soa) {
@@ -120,9 +125,13 @@ int lhip_sim_forward(lhip_ctx* ctx, const lhip_sim_params* p, uint64_t n, const 
         using namespace lhip;
         if (n == 0 || n >= uint64_t(INT32_MAX)) throw Error(kInvalid, "sim: n must be in [1, 2^31-1)");
         if (p->steps < 0 || p->checkpoint_every < 1) throw Error(kInvalid, "sim: steps >= 0, checkpoint_every >= 1");
-        if (p->method != LHIP_METHOD_DIRECT)
-            throw Error(kInvalid, "sim: the differentiable path supports the direct method");
         LHIP_CHECK(hipSetDevice(c.dev));
+        if (p->method == LHIP_METHOD_TREE) {
+            treesim_forward(c, *p, int(n), x0, v0, m, xT, vT);
+            c.last_sim_tree = true;
+            return;
+        }
+        if (p->method != LHIP_METHOD_DIRECT) throw Error(kInvalid, "sim: unknown method");
         if (!c.adj) c.adj = new AdjSim();
         AdjSim& A = *c.adj;
         A.n = int(n);
@@ -166,6 +175,7 @@ int lhip_sim_forward(lhip_ctx* ctx, const lhip_sim_params* p, uint64_t n, const 
         LHIP_CHECK(hipEventElapsedTime(&ms, A.ev[0], A.ev[1]));
         A.fwd_ms = ms;
         A.have_forward = true;
+        c.last_sim_tree = false;
     });
 }
 
@@ -173,6 +183,12 @@ int lhip_sim_backward(lhip_ctx* ctx, const double* gxT, const double* gvT, doubl
     if (!gxT || !gvT) return LHIP_INVALID;
     return lhip::guarded(ctx, [&](lhip::Context& c) {
         using namespace lhip;
+        if (c.last_sim_tree) {
+            if (gm) throw Error(kInvalid, "sim backward: the tree adjoint gives no mass gradient (pass NULL)");
+            LHIP_CHECK(hipSetDevice(c.dev));
+            treesim_backward(c, gxT, gvT, gx0, gv0);
+            return;
+        }
         if (!c.adj || !c.adj->have_forward) throw Error(kLogic, "sim backward: call lhip_sim_forward first");
         LHIP_CHECK(hipSetDevice(c.dev));
         AdjSim& A = *c.adj;
@@ -231,6 +247,10 @@ int lhip_sim_backward(lhip_ctx* ctx, const double* gxT, const double* gvT, doubl
 
 int lhip_sim_times(lhip_ctx* ctx, double* forward_ms, double* backward_ms) {
     return lhip::guarded(ctx, [&](lhip::Context& c) {
+        if (c.last_sim_tree) {
+            lhip::treesim_times(c, forward_ms, backward_ms);
+            return;
+        }
         if (!c.adj) throw lhip::Error(LHIP_LOGIC, "sim: no simulation run yet");
         if (forward_ms) *forward_ms = c.adj->fwd_ms;
         if (backward_ms) *backward_ms = c.adj->bwd_ms;

@@ -38,6 +38,8 @@ void launch_vjp_update(hipStream_t s, const SimRec* rec, int n, double eps, doub
                        double* ly, double* lz, double* lm, double* lvx, double* lvy, double* lvz, double drift,
                        const double* npos, const double* m, double c_next, SimRec* next_rec);
 void launch_fill64(hipStream_t s, double* y, double v, int n);
+// y += c * x.
+void launch_axpy_f64(hipStream_t s, double* y, const double* x, double c, int n);
 
 // Direct summation on FP64 positions with float masses (the mixed-precision backend form): FP32
 // pair arithmetic on FP64 separations, FP64 carry every 64 sources; a = G sum, phi = -G sum m/s.

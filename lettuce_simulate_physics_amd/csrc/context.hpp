@@ -21,9 +21,13 @@ namespace lhip {
 
 struct AdjSim;  // differentiable FP64-state simulation (adjoint.hip)
 void adj_destroy(AdjSim* a);
+struct TreeSim;  // differentiable simulation through the tree (treesim.hip)
+void treesim_destroy(TreeSim* t);
 
 struct Context {
     AdjSim* adj = nullptr;
+    TreeSim* tsim = nullptr;
+    bool last_sim_tree = false;
     int dev = 0;
     hipStream_t stream = nullptr;
     std::string name, err;
@@ -49,6 +53,10 @@ struct Context {
     bool mixed = false;
     lhip_walk_options wopt{kGroupMax, kSmallLeaf, 0.0};  // the reference's walk
     DevBuf xd, yd, zd, vxd, vyd, vzd, xd2, yd2, zd2, vxd2, vyd2, vzd2, axd, ayd, azd, potd, momd, rec64, posd;
+
+    // Frozen-topology replay of the differentiable tree simulation: group boxes from these recorded
+    // positions instead of the current ones (null otherwise).
+    const double4* posd_box = nullptr;
 
     bool has_pe = false, acc_ok = false, tree_ok = false, costs_ok = false;
     bool kick_rec = false, drift_rec = false;
@@ -77,6 +85,7 @@ struct Context {
     }
     ~Context() {
         adj_destroy(adj);
+        treesim_destroy(tsim);
         if (stream) {
             (void)hipStreamSynchronize(stream);
             (void)hipStreamDestroy(stream);
@@ -381,6 +390,15 @@ struct Context {
         swap(m, m2); swap(id, id2);
         if (keep_cost) swap(cost, cost2);
         LHIP_CHECK(hipEventRecord(ev[1], stream));
+        BuildBufs bb = build_bufs();
+        // inv_theta as the reference CPU forms it (src/cpu_backend.cpp:595).
+        const float inv_theta = p.theta > 0 ? float(1.0 / p.theta) : FLT_MAX;
+        tree_build(stream, bb, inv_theta);
+        LHIP_CHECK(hipEventRecord(ev[2], stream));
+        tree_ok = true;
+    }
+
+    BuildBufs build_bufs() {
         BuildBufs bb{};
         bb.n = n;
         bb.cap_nodes = int(cap_nodes);
@@ -408,11 +426,7 @@ struct Context {
         bb.prefix = prefix.as<uint64_t>();
         bb.gstart = gstart.as<uint32_t>();
         bb.counters = counters.as<WalkCounters>();
-        // inv_theta as the reference CPU forms it (src/cpu_backend.cpp:595).
-        const float inv_theta = p.theta > 0 ? float(1.0 / p.theta) : FLT_MAX;
-        tree_build(stream, bb, inv_theta);
-        LHIP_CHECK(hipEventRecord(ev[2], stream));
-        tree_ok = true;
+        return bb;
     }
 
     void compute_forces_with(const lhip_force_params& p, bool potential) {
@@ -471,6 +485,7 @@ struct Context {
             a.cost = cost.as<float>();
             if (mixed) {
                 a.posd = posd.as<double4>();
+                a.posd_box = posd_box ? posd_box : a.posd;
                 a.momd = momd.as<NodeMomD>();
                 a.axd = D(axd); a.ayd = D(ayd); a.azd = D(azd); a.potd = D(potd);
                 a.Gd = p.G;

@@ -76,6 +76,10 @@ struct TreeBufs {
 // Morton keys, stable radix sort, permute of every body array into key order (swaps the caller's
 // pointers: after the call b.x etc. hold the sorted arrays). Keys stay in b.key.
 void tree_sort(hipStream_t s, TreeBufs& b, int n, bool permute_cost);
+// Keys of FP64 positions already in key order (a recorded sorted state): bounding cube and keys,
+// no sort (idx = identity).
+void tree_keys_sorted(hipStream_t s, const double* x, const double* y, const double* z, int n, void* bbox_part,
+                      Cube* cube, uint64_t* key, uint32_t* idx);
 std::size_t radix_tile_hist_entries(int n);
 // One stable 8-bit LSD pass over bits [shift, shift + 8) of n (key, value) pairs; with n_dev the
 // count is read on the device and n is the capacity. The digit totals follow the tile histograms:

@@ -161,6 +161,12 @@ __global__ void __launch_bounds__(kBS) adj_update_kernel(const double* __restric
     }
 }
 
+__global__ void __launch_bounds__(kBS) axpy_f64_kernel(double* __restrict__ y, const double* __restrict__ x, double c,
+                                                       int n) {
+    const int i = blockIdx.x * kBS + threadIdx.x;
+    if (i < n) y[i] = fma(x[i], c, y[i]);
+}
+
 __global__ void __launch_bounds__(kBS) fill64_kernel(double* __restrict__ y, double v, int n) {
     const int i = blockIdx.x * kBS + threadIdx.x;
     if (i < n) y[i] = v;
@@ -473,6 +479,12 @@ void launch_vjp_update(hipStream_t s, const SimRec* rec, int n, double eps, doub
     LHIP_LAUNCH_CHECK();
     adj_update_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(part, n, sp.S, G, lx, ly, lz, lm, lvx, lvy, lvz, drift, npos, m,
                                                        c_next, next_rec);
+    LHIP_LAUNCH_CHECK();
+}
+
+void launch_axpy_f64(hipStream_t s, double* y, const double* x, double c, int n) {
+    if (n <= 0) return;
+    axpy_f64_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(y, x, c, n);
     LHIP_LAUNCH_CHECK();
 }
 

@@ -282,7 +282,10 @@ __device__ __forceinline__ float rcrit2_of(float cx, float cy, float cz, float g
 // MIXED: bodies are FP64 positions (xd, yd, zd) with float masses, and each node's FP64 centre of
 // mass, mass and quadrupole go to momd as well; internal nodes combine their children's FP64
 // moments, so the quadrupoles of the root levels are accumulated in FP64 throughout.
-template <bool MIXED>
+// KEEP: a second pass on other positions over a built tree (frozen-topology replay): node records,
+// rcrit2 and qrel (the walk's decisions) stay as the first pass wrote them; com, mass and
+// quadrupole are recomputed.
+template <bool MIXED, bool KEEP = false>
 __global__ void __launch_bounds__(kBS) moments_level(int level, int n, const float4* __restrict__ posm,
                                                      const double* __restrict__ xd, const double* __restrict__ yd,
                                                      const double* __restrict__ zd, const float* __restrict__ bm,
@@ -388,6 +391,10 @@ __global__ void __launch_bounds__(kBS) moments_level(int level, int n, const flo
         md.cx = float(cx); md.cy = float(cy); md.cz = float(cz); md.m = float(M);
         md.qxx = float(q[0]); md.qxy = float(q[1]); md.qxz = float(q[2]);
         md.qyy = float(q[3]); md.qyz = float(q[4]); md.qzz = float(q[5]);
+        if (KEEP) {
+            mom[p] = md;
+            continue;
+        }
         md.rcrit2 = rcrit2_of(md.cx, md.cy, md.cz, md.gx, md.gy, md.gz, md.half, inv_theta);
         {
             const double qf = sqrt(q[0] * q[0] + q[3] * q[3] + q[5] * q[5] +
@@ -402,6 +409,15 @@ __global__ void __launch_bounds__(kBS) moments_level(int level, int n, const flo
 }
 
 }  // namespace
+
+void tree_moments_keep(hipStream_t s, const BuildBufs& b) {
+    const unsigned g = std::min<unsigned>(grid_for(b.n, kBS), 512u);
+    for (int l = kMaxLevel; l >= 0; --l) {
+        moments_level<true, true><<<g, kBS, 0, s>>>(l, b.n, nullptr, b.xd, b.yd, b.zd, b.m, b.bl, b.levstart,
+                                                    b.prefix, b.node, b.mom, b.momd, 1.f);
+        LHIP_LAUNCH_CHECK();
+    }
+}
 
 std::size_t scan_tmp_entries(int n) {
     return std::size_t((static_cast<long long>(n) + kScanTile) / kScanTile) + 1;

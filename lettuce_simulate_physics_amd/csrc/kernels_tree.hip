@@ -292,6 +292,14 @@ std::size_t radix_tile_hist_entries(int n) {
     return ntiles * kRadix + kRadix;  // + digit totals
 }
 
+void tree_keys_sorted(hipStream_t s, const double* x, const double* y, const double* z, int n, void* bbox_part,
+                      Cube* cube, uint64_t* key, uint32_t* idx) {
+    if (n <= 0) return;
+    launch_bbox(s, x, y, z, n, bbox_part, cube);
+    keys_kernel<double><<<grid_for(n, kBS), kBS, 0, s>>>(x, y, z, cube, key, idx, n);
+    LHIP_LAUNCH_CHECK();
+}
+
 void tree_sort(hipStream_t s, TreeBufs& b, int n, bool permute_cost) {
     if (n <= 0) return;
     if (b.f64) sort_impl<double>(s, b, n, permute_cost);

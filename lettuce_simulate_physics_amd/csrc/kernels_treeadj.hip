@@ -1,0 +1,435 @@
+// Vector-Jacobian product of the tree force with frozen topology (no reference counterpart: the
+// reference has no differentiation).
+//
+// The force of the mixed-precision walk on body i is a sum over its group's list:
+//   body or monopole entries y (mass m):  a_i += m K(y - x_i),     K(d) = d / r^3, r^2 = |d|^2 + eps^2
+//   multipole cells (com C, mass M, Q):   a_i += -grad Psi(d),     d = C - x_i,
+//                                         Psi(d) = M / r + (1/2) d.Q.d / r^5 (the walk's p2c).
+// With the lists, the tree and the decisions frozen, and a cotangent u_i on each a_i, the VJP is
+//   point entries:  x_i gets -m T u_i,  y gets +m T u_i,   T = I / r^3 - 3 d d^T / r^5
+//   cells:          x_i gets  H u_i,    C gets -H u_i,     H = Hessian of Psi,
+//                   Q gets G = -(u d^T + d u^T) / (2 r^5) + (5/2) (d.u) d d^T / r^7
+// with H u = M (3 d (d.u) / r^5 - u / r^3) + Q u / r^5 - 5 Qd (d.u) / r^7 - 5 d (Qd.u) / r^7
+//            - (5/2) (d.Q.d) u / r^7 + (35/2) (d.Q.d) (d.u) d / r^9.
+// A cell's com and quadrupole depend on its bodies: com = sum m_j x_j / M, and
+// Q = sum m_j (3 s s^T - |s|^2 I), s = x_j - com (whose com derivative vanishes), so body j of cell c
+// gets m_j [gC / M + (6 G - 2 tr(G) I) (x_j - com)]. Summed over the cells containing j, that is
+// m_j (A + B x_j) with A, B accumulated from the root down (down_level) and applied at the leaves.
+//
+// Evaluation is entry-major: each lane owns a list entry (one cell, or two bodies) and loops over
+// the group's targets, whose positions and cotangents come from the slot registers by readlane. The
+// lane keeps its entry's source-side sums; each target's side is a wave sum (DPP butterflies). The
+// source sums go out as FP64 atomic adds per (group, entry) (hardware float atomics on this
+// library's coarse-grained device buffers); FP32 partial sums per list batch are carried into FP64.
+#include <cfloat>
+
+#include "kernels.hpp"
+#include "tree.hpp"
+#include "wave.hpp"
+
+namespace lhip {
+namespace {
+
+constexpr int kWaves = 4;
+constexpr int kBS4 = 64 * kWaves;
+constexpr int kCells = 64;     // cells per batch (one per lane)
+constexpr int kBodies = 128;   // bodies per batch (two per lane)
+constexpr int kStack = 896;
+constexpr int kThr = kStack - 7 * kLevels - 8;
+
+struct VjpLds {
+    float4 cA[kCells];  // com relative to the group centre, mass
+    float4 cB[kCells];  // qxx, qxy, qxz, qyy
+    float2 cC[kCells];  // qyz, qzz
+    int cI[kCells];     // node index
+    float4 bP[kBodies]; // position relative to the group centre, mass
+    int bI[kBodies];    // body index; -(k + 2) for a monopole cell k
+};
+
+struct Target {
+    float x, y, z, ux, uy, uz;
+    int i;
+    float fx, fy, fz;     // FP32 sums of the current batch
+    double gx, gy, gz;    // FP64 totals
+};
+
+__global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
+    __shared__ VjpLds lds[kWaves];
+    __shared__ int stk[kWaves][kStack];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    VjpLds& L = lds[w];
+    int* st = stk[w];
+    const int ngroups = int(a.gcount[a.n]);
+    const float e2 = a.eps2;
+    bool overflow = false;
+
+    while (true) {
+        int g = 0;
+        if (lane == 0) g = int(atomicAdd(&a.counters->next_group, 1u));
+        g = __builtin_amdgcn_readfirstlane(__shfl(g, 0, 64));
+        if (g >= ngroups) break;
+        const int glo = int(a.gstart[g]), ghi = int(a.gstart[g + 1]);
+        const bool chunked = ghi - glo > kGroupMax;
+        for (int clo = glo; clo < ghi; clo += kGroupMax) {
+            const int chi = min(ghi, clo + kGroupMax);
+            const int nbody = chi - clo;  // 1..128, wave-uniform
+            Target tg[2];
+            float mnx = FLT_MAX, mny = FLT_MAX, mnz = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX, mxz = -FLT_MAX;
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int t = s * 64 + lane;
+                tg[s].i = clo + min(t, nbody - 1);  // lanes past the chunk repeat its last body
+                const double4 d = a.posd[tg[s].i];
+                mnx = fminf(mnx, float(d.x)); mny = fminf(mny, float(d.y)); mnz = fminf(mnz, float(d.z));
+                mxx = fmaxf(mxx, float(d.x)); mxy = fmaxf(mxy, float(d.y)); mxz = fmaxf(mxz, float(d.z));
+                tg[s].fx = tg[s].fy = tg[s].fz = 0.f;
+                tg[s].gx = tg[s].gy = tg[s].gz = 0.0;
+            }
+            // The forward walk's group box (same float rounding), and its centre for relative
+            // coordinates.
+            mnx = wave_min(mnx); mny = wave_min(mny); mnz = wave_min(mnz);
+            mxx = wave_max(mxx); mxy = wave_max(mxy); mxz = wave_max(mxz);
+            const float bx = 0.5f * (mnx + mxx), by = 0.5f * (mny + mxy), bz = 0.5f * (mnz + mxz);
+            const float hx = 0.5f * (mxx - mnx), hy = 0.5f * (mxy - mny), hz = 0.5f * (mxz - mnz);
+            const double ccx = bx, ccy = by, ccz = bz;
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const double4 d = a.posd[tg[s].i];
+                tg[s].x = float(d.x - ccx); tg[s].y = float(d.y - ccy); tg[s].z = float(d.z - ccz);
+                tg[s].ux = float(a.ux[tg[s].i]); tg[s].uy = float(a.uy[tg[s].i]); tg[s].uz = float(a.uz[tg[s].i]);
+            }
+            auto fold = [&]() {
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    tg[s].gx += double(tg[s].fx); tg[s].gy += double(tg[s].fy); tg[s].gz += double(tg[s].fz);
+                    tg[s].fx = tg[s].fy = tg[s].fz = 0.f;
+                }
+            };
+            // Adds a wave-uniform vector to target t's FP32 sums.
+            auto add_target = [&](int t, float vx, float vy, float vz) {
+                const bool mine = lane == (t & 63);
+                if (t < 64) {
+                    if (mine) { tg[0].fx += vx; tg[0].fy += vy; tg[0].fz += vz; }
+                } else {
+                    if (mine) { tg[1].fx += vx; tg[1].fy += vy; tg[1].fz += vz; }
+                }
+            };
+            auto target_of = [&](int t, float& x, float& y, float& z, float& ux, float& uy, float& uz) {
+                const int l = t & 63;
+                const Target& T = t < 64 ? tg[0] : tg[1];
+                x = readlane_f(T.x, l); y = readlane_f(T.y, l); z = readlane_f(T.z, l);
+                ux = readlane_f(T.ux, l); uy = readlane_f(T.uy, l); uz = readlane_f(T.uz, l);
+            };
+
+            int nc = 0, nb = 0;
+            auto flush_cells = [&]() {
+                __builtin_amdgcn_wave_barrier();
+                const bool valid = lane < nc;
+                float4 A = make_float4(0.f, 0.f, 0.f, 0.f), B = A;
+                float2 C = make_float2(0.f, 0.f);
+                if (valid) { A = L.cA[lane]; B = L.cB[lane]; C = L.cC[lane]; }
+                const float qxx = B.x, qxy = B.y, qxz = B.z, qyy = B.w, qyz = C.x, qzz = C.y;
+                float sCx = 0.f, sCy = 0.f, sCz = 0.f;
+                float Gxx = 0.f, Gxy = 0.f, Gxz = 0.f, Gyy = 0.f, Gyz = 0.f, Gzz = 0.f;
+                for (int t = 0; t < nbody; ++t) {
+                    float x, y, z, ux, uy, uz;
+                    target_of(t, x, y, z, ux, uy, uz);
+                    const float dx = A.x - x, dy = A.y - y, dz = A.z - z;
+                    const float r2 = __fmaf_rn(dz, dz, __fmaf_rn(dy, dy, __fmaf_rn(dx, dx, e2)));
+                    const float inv = __builtin_amdgcn_rsqf(r2);
+                    const float i2 = inv * inv, i3 = inv * i2, i5 = i3 * i2, i7 = i5 * i2, i9 = i7 * i2;
+                    const float Qdx = qxx * dx + qxy * dy + qxz * dz;
+                    const float Qdy = qxy * dx + qyy * dy + qyz * dz;
+                    const float Qdz = qxz * dx + qyz * dy + qzz * dz;
+                    const float Qux = qxx * ux + qxy * uy + qxz * uz;
+                    const float Quy = qxy * ux + qyy * uy + qyz * uz;
+                    const float Quz = qxz * ux + qyz * uy + qzz * uz;
+                    const float dQd = dx * Qdx + dy * Qdy + dz * Qdz;
+                    const float du = dx * ux + dy * uy + dz * uz;
+                    const float Qdu = Qdx * ux + Qdy * uy + Qdz * uz;
+                    // H u = c_d d + c_u u + i5 Q u - 5 i7 (d.u) Q d
+                    const float c_d = 3.f * A.w * du * i5 - 5.f * Qdu * i7 + 17.5f * dQd * du * i9;
+                    const float c_u = -A.w * i3 - 2.5f * dQd * i7;
+                    const float c_q = -5.f * du * i7;
+                    const float Hx = c_d * dx + c_u * ux + i5 * Qux + c_q * Qdx;
+                    const float Hy = c_d * dy + c_u * uy + i5 * Quy + c_q * Qdy;
+                    const float Hz = c_d * dz + c_u * uz + i5 * Quz + c_q * Qdz;
+                    add_target(t, wave_sum(Hx), wave_sum(Hy), wave_sum(Hz));
+                    sCx -= Hx; sCy -= Hy; sCz -= Hz;
+                    const float h5 = -0.5f * i5, k7 = 2.5f * du * i7;
+                    Gxx += 2.f * h5 * ux * dx + k7 * dx * dx;
+                    Gyy += 2.f * h5 * uy * dy + k7 * dy * dy;
+                    Gzz += 2.f * h5 * uz * dz + k7 * dz * dz;
+                    Gxy += h5 * (ux * dy + dx * uy) + k7 * dx * dy;
+                    Gxz += h5 * (ux * dz + dx * uz) + k7 * dx * dz;
+                    Gyz += h5 * (uy * dz + dy * uz) + k7 * dy * dz;
+                }
+                if (valid) {
+                    double* o = a.gcell + std::size_t(L.cI[lane]) * 9;
+                    unsafeAtomicAdd(o + 0, double(sCx)); unsafeAtomicAdd(o + 1, double(sCy)); unsafeAtomicAdd(o + 2, double(sCz));
+                    unsafeAtomicAdd(o + 3, double(Gxx)); unsafeAtomicAdd(o + 4, double(Gxy)); unsafeAtomicAdd(o + 5, double(Gxz));
+                    unsafeAtomicAdd(o + 6, double(Gyy)); unsafeAtomicAdd(o + 7, double(Gyz)); unsafeAtomicAdd(o + 8, double(Gzz));
+                }
+                fold();
+                __builtin_amdgcn_wave_barrier();
+                nc = 0;
+            };
+            auto flush_bodies = [&]() {
+                __builtin_amdgcn_wave_barrier();
+                float4 P[2];
+                int J[2];
+                bool valid[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int e = h * 64 + lane;
+                    valid[h] = e < nb;
+                    P[h] = valid[h] ? L.bP[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    J[h] = valid[h] ? L.bI[e] : -1;
+                }
+                float s[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+                for (int t = 0; t < nbody; ++t) {
+                    float x, y, z, ux, uy, uz;
+                    target_of(t, x, y, z, ux, uy, uz);
+                    const int it = clo + t;
+                    float tx = 0.f, ty = 0.f, tz = 0.f;
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const float dx = P[h].x - x, dy = P[h].y - y, dz = P[h].z - z;
+                        const float r2 = __fmaf_rn(dz, dz, __fmaf_rn(dy, dy, __fmaf_rn(dx, dx, e2)));
+                        const float inv = __builtin_amdgcn_rsqf(r2);
+                        const float i3 = inv * inv * inv;
+                        const float m = J[h] == it ? 0.f : P[h].w;  // the body's own entry adds nothing
+                        const float mi3 = m * i3;
+                        const float c = -3.f * mi3 * inv * inv * (dx * ux + dy * uy + dz * uz);
+                        const float wx = mi3 * ux + c * dx, wy = mi3 * uy + c * dy, wz = mi3 * uz + c * dz;
+                        s[h][0] += wx; s[h][1] += wy; s[h][2] += wz;
+                        tx -= wx; ty -= wy; tz -= wz;
+                    }
+                    add_target(t, wave_sum(tx), wave_sum(ty), wave_sum(tz));
+                }
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    if (!valid[h]) continue;
+                    if (J[h] >= 0) {
+                        unsafeAtomicAdd(a.gsx + J[h], double(s[h][0]));
+                        unsafeAtomicAdd(a.gsy + J[h], double(s[h][1]));
+                        unsafeAtomicAdd(a.gsz + J[h], double(s[h][2]));
+                    } else if (J[h] <= -2) {
+                        double* o = a.gcell + std::size_t(-J[h] - 2) * 9;
+                        unsafeAtomicAdd(o + 0, double(s[h][0]));
+                        unsafeAtomicAdd(o + 1, double(s[h][1]));
+                        unsafeAtomicAdd(o + 2, double(s[h][2]));
+                    }
+                }
+                fold();
+                __builtin_amdgcn_wave_barrier();
+                nb = 0;
+            };
+
+            // Traversal: the forward walk's (kernels_walk.hip), same decisions.
+            if (lane == 0) st[0] = 0;
+            int top = 1;
+            while (top > 0) {
+                const int room = (kThr - top) / 7;
+                const int P = min(64, min(top, max(1, room)));
+                __builtin_amdgcn_wave_barrier();
+                const bool act = lane < P;
+                const int k = act ? st[top - P + lane] : 0;
+                __builtin_amdgcn_wave_barrier();
+                top -= P;
+                WalkNode nd{0.f, 0.f, 0.f, 0.f, 0, 0, 0, -1};
+                if (act) nd = a.node[k];
+                const bool keep_open = chunked ? (nd.lo < chi && clo < nd.hi) : (nd.lo <= clo && chi <= nd.hi);
+                const float dx = fmaxf(fabsf(nd.cx - bx) - hx, 0.f);
+                const float dy = fmaxf(fabsf(nd.cy - by) - hy, 0.f);
+                const float dz = fmaxf(fabsf(nd.cz - bz) - hz, 0.f);
+                const float d2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+                const bool acc = act && !keep_open && d2 >= nd.rcrit2;
+                const bool leaf = act && info_leaf(nd.info);
+                const bool small = leaf && (nd.hi - nd.lo) <= a.small_leaf;
+                bool is_cell = acc && !small;
+                const bool is_mono = is_cell && a.quad_tau > 0.f && a.mom[k].qrel < a.quad_tau * d2;
+                is_cell = is_cell && !is_mono;
+                const bool is_range = leaf && (!acc || small);
+                const bool open_int = act && !acc && !leaf;
+
+                const int nch = open_int ? info_nchild(nd.info) : 0;
+                const int cend = prefix_sum_incl(nch, lane);
+                const int pushed = __builtin_amdgcn_readlane(cend, 63);
+                if (top + pushed > kStack) {
+                    overflow = true;
+                    break;
+                }
+                for (int c = 0; c < nch; ++c) st[top + cend - nch + c] = nd.first + c;
+                top += pushed;
+
+                const unsigned long long mc = __ballot(is_cell);
+                const int ncnew = __popcll(mc);
+                if (ncnew) {
+                    if (nc + ncnew > kCells) flush_cells();
+                    if (is_cell) {
+                        const int slot = nc + int(lanes_below(mc));
+                        const NodeMom& md = a.mom[k];
+                        const NodeMomD& dm = a.momd[k];
+                        L.cA[slot] = make_float4(float(dm.cx - ccx), float(dm.cy - ccy), float(dm.cz - ccz), md.m);
+                        L.cB[slot] = make_float4(md.qxx, md.qxy, md.qxz, md.qyy);
+                        L.cC[slot] = make_float2(md.qyz, md.qzz);
+                        L.cI[slot] = k;
+                    }
+                    nc += ncnew;
+                }
+                const unsigned long long mm = __ballot(is_mono);
+                const int nmnew = __popcll(mm);
+                if (nmnew) {
+                    if (nb + nmnew > kBodies) flush_bodies();
+                    if (is_mono) {
+                        const int slot = nb + int(lanes_below(mm));
+                        const NodeMomD& dm = a.momd[k];
+                        L.bP[slot] = make_float4(float(dm.cx - ccx), float(dm.cy - ccy), float(dm.cz - ccz),
+                                                 a.mom[k].m);
+                        L.bI[slot] = -(k + 2);
+                    }
+                    nb += nmnew;
+                }
+                const unsigned long long mr = __ballot(is_range);
+                if (mr) {
+                    const int len = is_range ? nd.hi - nd.lo : 0;
+                    const int end = prefix_sum_incl(len, lane);
+                    const int off = end - len;
+                    const int total = __builtin_amdgcn_readlane(end, 63);
+                    int done = 0;
+                    while (done < total) {
+                        if (nb == kBodies) flush_bodies();
+                        const int take = min(kBodies - nb, total - done);
+                        const int u0 = max(off, done), u1 = min(off + len, done + take);
+                        for (int u = u0; u < u1; ++u) {
+                            const int j = nd.lo + (u - off);
+                            const double4 d = a.posd[j];
+                            L.bP[nb + (u - done)] = make_float4(float(d.x - ccx), float(d.y - ccy), float(d.z - ccz),
+                                                                float(d.w));
+                            L.bI[nb + (u - done)] = j;
+                        }
+                        nb += take;
+                        done += take;
+                    }
+                }
+            }
+            if (nc) flush_cells();
+            if (nb) flush_bodies();
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int t = s * 64 + lane;
+                if (t < nbody) {
+                    a.gtx[tg[s].i] = tg[s].gx;
+                    a.gty[tg[s].i] = tg[s].gy;
+                    a.gtz[tg[s].i] = tg[s].gz;
+                }
+            }
+        }
+    }
+    if (overflow && lane == 0) atomicOr(&a.counters->error, 1);
+}
+
+__global__ void reset_vjp(WalkCounters* __restrict__ c) {
+    c->interactions = c->p2p = c->p2c = c->listed = c->p2m = 0;
+    c->error = 0;
+    c->next_group = 0;
+}
+
+// Per-node A (3) and B (6: xx, xy, xz, yy, yz, zz) of the downward pass: the node's own terms
+// B = 6 G - 2 tr(G) I and A = gC / M - B com.
+__device__ __forceinline__ void own_terms(const double* __restrict__ gc, const NodeMomD& md, double o[9]) {
+    if (!(md.m > 0)) {
+        for (int c = 0; c < 9; ++c) o[c] = 0.0;
+        return;
+    }
+    const double t = gc[3] + gc[6] + gc[8];
+    const double Bxx = 6 * gc[3] - 2 * t, Bxy = 6 * gc[4], Bxz = 6 * gc[5];
+    const double Byy = 6 * gc[6] - 2 * t, Byz = 6 * gc[7], Bzz = 6 * gc[8] - 2 * t;
+    o[0] = gc[0] / md.m - (Bxx * md.cx + Bxy * md.cy + Bxz * md.cz);
+    o[1] = gc[1] / md.m - (Bxy * md.cx + Byy * md.cy + Byz * md.cz);
+    o[2] = gc[2] / md.m - (Bxz * md.cx + Byz * md.cy + Bzz * md.cz);
+    o[3] = Bxx; o[4] = Bxy; o[5] = Bxz; o[6] = Byy; o[7] = Byz; o[8] = Bzz;
+}
+
+__global__ void __launch_bounds__(kBS) down_root(const double* __restrict__ gcell, const NodeMomD* __restrict__ momd,
+                                                 double* __restrict__ cum) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double o[9];
+        own_terms(gcell, momd[0], o);
+        for (int c = 0; c < 9; ++c) cum[c] = o[c];
+    }
+}
+
+// Children of the nodes of one level: cum(child) = cum(parent) + own(child).
+__global__ void __launch_bounds__(kBS) down_level(int level, const uint32_t* __restrict__ levstart,
+                                                  const WalkNode* __restrict__ node,
+                                                  const double* __restrict__ gcell,
+                                                  const NodeMomD* __restrict__ momd, double* __restrict__ cum) {
+    const int p0 = int(levstart[level]), p1 = int(levstart[level + 1]);
+    for (int p = p0 + blockIdx.x * kBS + threadIdx.x; p < p1; p += gridDim.x * kBS) {
+        const WalkNode nd = node[p];
+        const int nch = info_nchild(nd.info);
+        if (info_leaf(nd.info) || nch == 0) continue;
+        double cp[9];
+        for (int c = 0; c < 9; ++c) cp[c] = cum[std::size_t(p) * 9 + c];
+        for (int ch = nd.first; ch < nd.first + nch; ++ch) {
+            double o[9];
+            own_terms(gcell + std::size_t(ch) * 9, momd[ch], o);
+            for (int c = 0; c < 9; ++c) cum[std::size_t(ch) * 9 + c] = cp[c] + o[c];
+        }
+    }
+}
+
+// Bodies of each leaf: g_j += m_j (A + B x_j) with the leaf's accumulated A, B.
+__global__ void __launch_bounds__(kBS) down_leaves(const uint32_t* __restrict__ node_count,
+                                                   const WalkNode* __restrict__ node,
+                                                   const double* __restrict__ cum, const double4* __restrict__ posd,
+                                                   double* __restrict__ gx, double* __restrict__ gy,
+                                                   double* __restrict__ gz) {
+    const int p = blockIdx.x * kBS + threadIdx.x;
+    if (p >= int(*node_count)) return;
+    const WalkNode nd = node[p];
+    if (!info_leaf(nd.info)) return;
+    const double* c = cum + std::size_t(p) * 9;
+    for (int j = nd.lo; j < nd.hi; ++j) {
+        const double4 b = posd[j];
+        gx[j] += b.w * (c[0] + c[3] * b.x + c[4] * b.y + c[5] * b.z);
+        gy[j] += b.w * (c[1] + c[4] * b.x + c[6] * b.y + c[7] * b.z);
+        gz[j] += b.w * (c[2] + c[5] * b.x + c[7] * b.y + c[8] * b.z);
+    }
+}
+
+}  // namespace
+
+void tree_vjp(hipStream_t s, const TreeVjpArgs& a, const uint32_t* levstart, const uint32_t* node_count, int nodes,
+              double* cum) {
+    if (a.n <= 0) return;
+    reset_vjp<<<1, 1, 0, s>>>(a.counters);
+    LHIP_LAUNCH_CHECK();
+    tree_vjp_kernel<<<std::max(1, a.blocks), kBS4, 0, s>>>(a);
+    LHIP_LAUNCH_CHECK();
+    down_root<<<1, 64, 0, s>>>(a.gcell, a.momd, cum);
+    LHIP_LAUNCH_CHECK();
+    const unsigned g = std::min<unsigned>(grid_for(a.n, kBS), 512u);
+    for (int l = 0; l < kMaxLevel; ++l) {
+        down_level<<<g, kBS, 0, s>>>(l, levstart, a.node, a.gcell, a.momd, cum);
+        LHIP_LAUNCH_CHECK();
+    }
+    down_leaves<<<grid_for(nodes, kBS), kBS, 0, s>>>(node_count, a.node, cum, a.posd, a.gsx, a.gsy, a.gsz);
+    LHIP_LAUNCH_CHECK();
+}
+
+int tree_vjp_blocks_per_cu() {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(tree_vjp_kernel), kBS4,
+                                                     0) != hipSuccess ||
+        blocks < 1) {
+        (void)hipGetLastError();
+        blocks = 2;
+    }
+    return blocks;
+}
+
+}  // namespace lhip

@@ -45,6 +45,7 @@
 
 #include "kernels.hpp"
 #include "tree.hpp"
+#include "wave.hpp"
 
 namespace lhip {
 namespace {
@@ -145,39 +146,6 @@ __device__ __forceinline__ void eval_bodies(Slot (&sl)[2], int S, int nb, const 
     }
 }
 
-// Inclusive prefix sum over the wave: Hillis-Steele within each row of 16 lanes by DPP row shifts
-// (a lane whose source lies outside its row adds 0), then the totals of the rows below.
-__device__ __forceinline__ int prefix_sum_incl(int v, int lane) {
-    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
-    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
-    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
-    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
-    const int r0 = __builtin_amdgcn_readlane(v, 15), r1 = __builtin_amdgcn_readlane(v, 31);
-    const int r2 = __builtin_amdgcn_readlane(v, 47);
-    const int row = lane >> 4;
-    return v + (row == 0 ? 0 : row == 1 ? r0 : row == 2 ? r0 + r1 : r0 + r1 + r2);
-}
-
-__device__ __forceinline__ float wave_min(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-__device__ __forceinline__ uint32_t lanes_below(unsigned long long mask) {
-    return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
-}
-
 constexpr int kStackCap = 896;
 // Deepest possible tree: levels 0..21 (no key restarts).
 constexpr int kStackThr = kStackCap - 7 * kLevels - 8;
@@ -223,7 +191,7 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                 sl[s].i = clo + sl[s].bi;
                 float4 p;
                 if (REL) {
-                    const double4 d = a.posd[sl[s].i];
+                    const double4 d = a.posd_box[sl[s].i];
                     p = make_float4(float(d.x), float(d.y), float(d.z), 0.f);
                 } else {
                     p = a.posm[sl[s].i];

@@ -73,11 +73,16 @@ inline long long node_capacity(long long n) { return n + (n / 17 + 1) * kLevels 
 // Leaf levels, node enumeration, level-order layout, groups and multipoles. inv_theta as the
 // reference CPU computes it.
 void tree_build(hipStream_t s, const BuildBufs& b, float inv_theta);
+// Mixed form, frozen topology: recompute com, mass and quadrupole (node and momd) from the
+// positions in b over the tree tree_build made, keeping every decision input (node records,
+// rcrit2, qrel).
+void tree_moments_keep(hipStream_t s, const BuildBufs& b);
 
 struct WalkArgs {
     const float4* posm;        // plain form: sorted (position, mass)
     // mixed form (posd != null): FP64 (position, mass) records, FP64 node moments, FP64 outputs
     const double4* posd;
+    const double4* posd_box;   // positions for the group boxes (decisions); posd unless replaying
     const NodeMomD* momd;
     double *axd, *ayd, *azd, *potd;
     double Gd;
@@ -100,5 +105,33 @@ struct WalkArgs {
 };
 void tree_walk(hipStream_t s, const WalkArgs& a, bool potential);
 int walk_blocks_per_cu();
+
+// Vector-Jacobian product of the mixed-precision walk's forces with frozen topology
+// (kernels_treeadj.hip). u = cotangent on the accelerations (already scaled by G and the kick
+// weight), sorted order. Target-side terms are written to gt*; body source terms and the
+// cell-moment terms carried down the tree are added to gs* (zeroed by the caller), gcell (9 per
+// node, zeroed by the caller) and cum (9 per node) are scratch.
+struct TreeVjpArgs {
+    const WalkNode* node;
+    const NodeMom* mom;
+    const NodeMomD* momd;
+    const double4* posd;
+    const double *ux, *uy, *uz;
+    const uint32_t* gstart;
+    const uint32_t* gcount;
+    int n;
+    float eps2;
+    int small_leaf;
+    float quad_tau;
+    double *gtx, *gty, *gtz;
+    double *gsx, *gsy, *gsz;
+    double* gcell;
+    WalkCounters* counters;
+    int blocks;
+};
+// node_count: the device node count of the tree (nodes = its capacity, the launch bound).
+void tree_vjp(hipStream_t s, const TreeVjpArgs& a, const uint32_t* levstart, const uint32_t* node_count, int nodes,
+              double* cum);
+int tree_vjp_blocks_per_cu();
 
 }  // namespace lhip

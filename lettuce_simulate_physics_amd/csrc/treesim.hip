@@ -1,0 +1,454 @@
+// Differentiable simulation through the tree: FP64-state KDK with the mixed-precision walk, step
+// checkpoints, and the adjoint with frozen topology (no reference counterpart: the reference has no
+// differentiation; the forward is the reference's KDK, src/simulation.cpp:406-438).
+//
+// The simulation runs on the context's own state in the mixed mode (it replaces the resident
+// bodies). Each force call sorts the bodies, so the body order changes every step; the cotangents
+// are kept per body id.
+//
+// Forward: a tree per force call. RECORD also keeps, per force call, the bodies' sorted order and
+// positions; REPLAY (the frozen-topology forward that finite differences need) rebuilds each
+// step's tree and walk decisions from those records and evaluates the forces at the current
+// positions: the state is brought into the recorded order, keys, levels, nodes, groups and the
+// decision moments (rcrit2, qrel, node com) come from the recorded positions, and a second moments
+// pass takes com, mass and quadrupole from the current ones; the group boxes are the recorded ones.
+//
+// Backward: segment by segment from the checkpoints (x, v, a, id, m at a segment start), a
+// recompute that keeps each step's positions (in the order its tree was built from), then the
+// merged-kick reverse KDK of the direct-sum adjoint: VJP(x_T, dt/2); for n = T-1 .. 0
+// { lambda_v += dt lambda_x; VJP(x_n, n > 0 ? dt : dt/2) }. A VJP rebuilds the tree at x_n (the
+// forward's tree: the same positions in the same order sort the same way), gathers
+// u = G w lambda_v in sorted order and runs tree_vjp; the result is added to lambda_x by id.
+#include "adjoint.hpp"
+#include "context.hpp"
+
+namespace lhip {
+namespace {
+
+__global__ void __launch_bounds__(kBS) gather_u_kernel(const uint32_t* __restrict__ id, const double* __restrict__ lx,
+                                                       const double* __restrict__ ly, const double* __restrict__ lz,
+                                                       double c, double* __restrict__ ux, double* __restrict__ uy,
+                                                       double* __restrict__ uz, int n) {
+    const int k = blockIdx.x * kBS + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t j = id[k];
+    ux[k] = c * lx[j];
+    uy[k] = c * ly[j];
+    uz[k] = c * lz[j];
+}
+
+// lambda_x[id_k] += gt_k + gs_k.
+__global__ void __launch_bounds__(kBS) scatter_add_kernel(const uint32_t* __restrict__ id, const double* __restrict__ gtx,
+                                                          const double* __restrict__ gty,
+                                                          const double* __restrict__ gtz,
+                                                          const double* __restrict__ gsx,
+                                                          const double* __restrict__ gsy,
+                                                          const double* __restrict__ gsz, double* __restrict__ lx,
+                                                          double* __restrict__ ly, double* __restrict__ lz, int n) {
+    const int k = blockIdx.x * kBS + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t j = id[k];
+    lx[j] += gtx[k] + gsx[k];
+    ly[j] += gty[k] + gsy[k];
+    lz[j] += gtz[k] + gsz[k];
+}
+
+__global__ void __launch_bounds__(kBS) inverse_kernel(const uint32_t* __restrict__ id, uint32_t* __restrict__ inv, int n) {
+    const int k = blockIdx.x * kBS + threadIdx.x;
+    if (k < n) inv[id[k]] = uint32_t(k);
+}
+
+// Moves the state into a recorded order: new position k holds the body with id rec[k], currently
+// at inv[rec[k]].
+__global__ void __launch_bounds__(kBS) reorder_kernel(const uint32_t* __restrict__ rec, const uint32_t* __restrict__ inv,
+                                                      const double* __restrict__ src, double* __restrict__ dst,
+                                                      int arrays, int n) {
+    const int k = blockIdx.x * kBS + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t j = inv[rec[k]];
+    for (int c = 0; c < arrays; ++c) dst[std::size_t(c) * n + k] = src[std::size_t(c) * n + j];
+}
+
+__global__ void __launch_bounds__(kBS) reorder_small_kernel(const uint32_t* __restrict__ rec,
+                                                            const uint32_t* __restrict__ inv,
+                                                            const float* __restrict__ m, float* __restrict__ m2,
+                                                            uint32_t* __restrict__ id2, int n) {
+    const int k = blockIdx.x * kBS + threadIdx.x;
+    if (k >= n) return;
+    m2[k] = m[inv[rec[k]]];
+    id2[k] = rec[k];
+}
+
+__global__ void __launch_bounds__(kBS) pack_posd_kernel(const double* __restrict__ x, const double* __restrict__ y,
+                                                        const double* __restrict__ z, const float* __restrict__ m,
+                                                        double4* __restrict__ p, int n) {
+    const int k = blockIdx.x * kBS + threadIdx.x;
+    if (k < n) p[k] = make_double4(x[k], y[k], z[k], double(m[k]));
+}
+
+}  // namespace
+
+struct TreeSim {
+    int n = 0;
+    lhip_sim_params p{};
+    int nck = 0;
+    bool have_forward = false, have_record = false;
+    int rec_steps = 0;
+    double fwd_ms = 0, bwd_ms = 0;
+    hipEvent_t ev[4] = {};
+    // checkpoints: 9 double arrays (x, y, z, vx, vy, vz, ax, ay, az) + id + m per segment start
+    DevBuf ck_d, ck_id, ck_m;
+    // recompute of a segment: positions (3 arrays), id, m per step
+    DevBuf seg_d, seg_id, seg_m;
+    // records per force call: sorted ids and sorted positions
+    DevBuf rec_id, rec_x;
+    // cotangents (id order) and VJP scratch (sorted order)
+    DevBuf lx, lvx, u, gt, gs, gcell, cum, inv, tmp, tmp_m, tmp_id, posd_rec;
+
+    TreeSim() {
+        for (auto& e : ev) LHIP_CHECK(hipEventCreate(&e));
+    }
+    ~TreeSim() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+void treesim_destroy(TreeSim* t) { delete t; }
+
+namespace {
+
+struct Run {
+    Context& c;
+    TreeSim& S;
+    hipStream_t s;
+    int n;
+    lhip_force_params fp;
+
+    double* D(DevBuf& b) { return b.as<double>(); }
+
+    void force() { c.compute_forces(fp, false); }
+
+    // State arrays of the context in checkpoint order.
+    DevBuf* state(int k) {
+        DevBuf* a[9] = {&c.xd, &c.yd, &c.zd, &c.vxd, &c.vyd, &c.vzd, &c.axd, &c.ayd, &c.azd};
+        return a[k];
+    }
+
+    void save_checkpoint(int k) {
+        const std::size_t db = std::size_t(n) * 8, fb = std::size_t(n) * 4;
+        double* base = D(S.ck_d) + std::size_t(k) * 9 * n;
+        for (int a = 0; a < 9; ++a)
+            LHIP_CHECK(hipMemcpyAsync(base + std::size_t(a) * n, state(a)->as<void>(), db, hipMemcpyDeviceToDevice, s));
+        LHIP_CHECK(hipMemcpyAsync(S.ck_id.as<uint32_t>() + std::size_t(k) * n, c.id.as<void>(), fb,
+                                  hipMemcpyDeviceToDevice, s));
+        LHIP_CHECK(hipMemcpyAsync(S.ck_m.as<float>() + std::size_t(k) * n, c.m.as<void>(), fb, hipMemcpyDeviceToDevice, s));
+    }
+
+    void load_checkpoint(int k) {
+        const std::size_t db = std::size_t(n) * 8, fb = std::size_t(n) * 4;
+        const double* base = D(S.ck_d) + std::size_t(k) * 9 * n;
+        for (int a = 0; a < 9; ++a)
+            LHIP_CHECK(hipMemcpyAsync(state(a)->as<void>(), base + std::size_t(a) * n, db, hipMemcpyDeviceToDevice, s));
+        LHIP_CHECK(hipMemcpyAsync(c.id.as<void>(), S.ck_id.as<uint32_t>() + std::size_t(k) * n, fb,
+                                  hipMemcpyDeviceToDevice, s));
+        LHIP_CHECK(hipMemcpyAsync(c.m.as<void>(), S.ck_m.as<float>() + std::size_t(k) * n, fb, hipMemcpyDeviceToDevice, s));
+        c.tree_ok = false;
+        c.acc_ok = true;
+    }
+
+    // Positions, ids and masses of the current state into slot k of a (positions, id, m) store.
+    void save_positions(DevBuf& dd, DevBuf& did, DevBuf& dm, int k) {
+        const std::size_t db = std::size_t(n) * 8, fb = std::size_t(n) * 4;
+        double* base = D(dd) + std::size_t(k) * 3 * n;
+        for (int a = 0; a < 3; ++a)
+            LHIP_CHECK(hipMemcpyAsync(base + std::size_t(a) * n, state(a)->as<void>(), db, hipMemcpyDeviceToDevice, s));
+        LHIP_CHECK(hipMemcpyAsync(did.as<uint32_t>() + std::size_t(k) * n, c.id.as<void>(), fb, hipMemcpyDeviceToDevice, s));
+        LHIP_CHECK(hipMemcpyAsync(dm.as<float>() + std::size_t(k) * n, c.m.as<void>(), fb, hipMemcpyDeviceToDevice, s));
+    }
+
+    void load_positions(DevBuf& dd, DevBuf& did, DevBuf& dm, int k) {
+        const std::size_t db = std::size_t(n) * 8, fb = std::size_t(n) * 4;
+        const double* base = D(dd) + std::size_t(k) * 3 * n;
+        for (int a = 0; a < 3; ++a)
+            LHIP_CHECK(hipMemcpyAsync(state(a)->as<void>(), base + std::size_t(a) * n, db, hipMemcpyDeviceToDevice, s));
+        LHIP_CHECK(hipMemcpyAsync(c.id.as<void>(), did.as<uint32_t>() + std::size_t(k) * n, fb, hipMemcpyDeviceToDevice, s));
+        LHIP_CHECK(hipMemcpyAsync(c.m.as<void>(), dm.as<float>() + std::size_t(k) * n, fb, hipMemcpyDeviceToDevice, s));
+        c.tree_ok = false;
+    }
+
+    // The force at the current positions with the topology of recorded force call r: state into
+    // the recorded order, tree and decisions from the recorded positions, moments at the current.
+    void replay_force(int r) {
+        const uint32_t* rid = S.rec_id.as<uint32_t>() + std::size_t(r) * n;
+        const double* rx = D(S.rec_x) + std::size_t(r) * 3 * n;
+        inverse_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(c.id.as<uint32_t>(), S.inv.as<uint32_t>(), n);
+        LHIP_LAUNCH_CHECK();
+        // 6 state arrays (x, v) through tmp; masses and ids through tmp_m / tmp_id
+        double* t = D(S.tmp);
+        for (int a = 0; a < 6; ++a)
+            LHIP_CHECK(hipMemcpyAsync(t + std::size_t(a) * n, state(a)->as<void>(), std::size_t(n) * 8,
+                                      hipMemcpyDeviceToDevice, s));
+        double* t2 = t + 6 * std::size_t(n);
+        reorder_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(rid, S.inv.as<uint32_t>(), t, t2, 6, n);
+        LHIP_LAUNCH_CHECK();
+        for (int a = 0; a < 6; ++a)
+            LHIP_CHECK(hipMemcpyAsync(state(a)->as<void>(), t2 + std::size_t(a) * n, std::size_t(n) * 8,
+                                      hipMemcpyDeviceToDevice, s));
+        reorder_small_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(rid, S.inv.as<uint32_t>(), c.m.as<float>(),
+                                                              S.tmp_m.as<float>(), c.id.as<uint32_t>(), n);
+        LHIP_LAUNCH_CHECK();
+        LHIP_CHECK(hipMemcpyAsync(c.m.as<void>(), S.tmp_m.as<void>(), std::size_t(n) * 4, hipMemcpyDeviceToDevice, s));
+        // tree from the recorded positions (already in key order)
+        c.reserve_tree(n);
+        c.momd.reserve(std::size_t(c.cap_nodes) * sizeof(NodeMomD));
+        c.posd.reserve(std::size_t(n) * sizeof(double4));
+        S.posd_rec.reserve(std::size_t(n) * sizeof(double4));
+        LHIP_CHECK(hipEventRecord(c.ev[0], s));
+        tree_keys_sorted(s, rx, rx + n, rx + 2 * std::size_t(n), n, c.bbox_part.as<void>(), c.cube.as<Cube>(),
+                         c.key.as<uint64_t>(), c.idx.as<uint32_t>());
+        LHIP_CHECK(hipEventRecord(c.ev[1], s));
+        BuildBufs bb = c.build_bufs();
+        bb.xd = rx; bb.yd = rx + n; bb.zd = rx + 2 * std::size_t(n);
+        tree_build(s, bb, fp.theta > 0 ? float(1.0 / fp.theta) : FLT_MAX);
+        bb.xd = D(c.xd); bb.yd = D(c.yd); bb.zd = D(c.zd);
+        tree_moments_keep(s, bb);
+        pack_posd_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(rx, rx + n, rx + 2 * std::size_t(n), c.m.as<float>(),
+                                                          S.posd_rec.as<double4>(), n);
+        LHIP_LAUNCH_CHECK();
+        pack_posd_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(D(c.xd), D(c.yd), D(c.zd), c.m.as<float>(),
+                                                          c.posd.as<double4>(), n);
+        LHIP_LAUNCH_CHECK();
+        LHIP_CHECK(hipEventRecord(c.ev[2], s));
+        c.tree_ok = true;
+        c.posd_box = S.posd_rec.as<double4>();
+        c.compute_forces_with(fp, false);
+        c.posd_box = nullptr;
+    }
+
+    void record(int r) {
+        const std::size_t db = std::size_t(n) * 8, fb = std::size_t(n) * 4;
+        double* base = D(S.rec_x) + std::size_t(r) * 3 * n;
+        for (int a = 0; a < 3; ++a)
+            LHIP_CHECK(hipMemcpyAsync(base + std::size_t(a) * n, state(a)->as<void>(), db, hipMemcpyDeviceToDevice, s));
+        LHIP_CHECK(hipMemcpyAsync(S.rec_id.as<uint32_t>() + std::size_t(r) * n, c.id.as<void>(), fb,
+                                  hipMemcpyDeviceToDevice, s));
+    }
+
+    // One force evaluation of the forward: r = index of the force call.
+    void forward_force(int r, int mode) {
+        if (mode == LHIP_TREE_REPLAY) {
+            replay_force(r);
+        } else {
+            force();
+            if (mode == LHIP_TREE_RECORD) record(r);
+        }
+    }
+};
+
+void host_soa(const double* aos, std::size_t n, std::vector<double>& soa) {
+    soa.resize(3 * n);
+    for (std::size_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) soa[c * n + i] = aos[3 * i + c];
+}
+
+// Device SoA (3 arrays of n) in the body order given by ids -> host AoS in id order.
+void to_host_by_id(hipStream_t s, const double* x, const double* y, const double* z, const uint32_t* id, int n,
+                   double* out) {
+    std::vector<double> h(3 * std::size_t(n));
+    std::vector<uint32_t> ids(n);
+    LHIP_CHECK(hipMemcpyAsync(h.data(), x, std::size_t(n) * 8, hipMemcpyDeviceToHost, s));
+    LHIP_CHECK(hipMemcpyAsync(h.data() + n, y, std::size_t(n) * 8, hipMemcpyDeviceToHost, s));
+    LHIP_CHECK(hipMemcpyAsync(h.data() + 2 * std::size_t(n), z, std::size_t(n) * 8, hipMemcpyDeviceToHost, s));
+    LHIP_CHECK(hipMemcpyAsync(ids.data(), id, std::size_t(n) * 4, hipMemcpyDeviceToHost, s));
+    LHIP_CHECK(hipStreamSynchronize(s));
+    for (int k = 0; k < n; ++k)
+        for (int c = 0; c < 3; ++c) out[3 * std::size_t(ids[k]) + c] = h[std::size_t(c) * n + k];
+}
+
+}  // namespace
+
+void treesim_forward(Context& c, const lhip_sim_params& p, int n, const double* x0, const double* v0, const double* m,
+                     double* xT, double* vT) {
+    if (!c.tsim) c.tsim = new TreeSim();
+    TreeSim& S = *c.tsim;
+    const int mode = p.tree_mode;
+    if (mode != LHIP_TREE_BUILD && mode != LHIP_TREE_RECORD && mode != LHIP_TREE_REPLAY)
+        throw Error(kInvalid, "sim: unknown tree mode");
+    if (mode == LHIP_TREE_REPLAY && (!S.have_record || S.n != n || S.rec_steps != p.steps))
+        throw Error(kLogic, "sim: replay needs a recorded forward of the same size and length");
+    if (!(p.eps > 0)) throw Error(kInvalid, "sim: the tree method needs eps > 0");
+    const lhip_walk_options* wo = p.walk.group_max > 0 ? &p.walk : nullptr;
+    c.set_precision(LHIP_MODE_MIXED, wo);
+    const int T = p.steps, K = p.checkpoint_every;
+    S.n = n;
+    S.p = p;
+    S.have_forward = false;
+    if (mode != LHIP_TREE_REPLAY) S.have_record = false;
+    S.nck = T > 0 ? (T + K - 1) / K : 1;
+    const std::size_t N = std::size_t(n);
+    if (mode != LHIP_TREE_REPLAY) {
+        S.ck_d.reserve(std::size_t(S.nck) * 9 * N * 8);
+        S.ck_id.reserve(std::size_t(S.nck) * N * 4);
+        S.ck_m.reserve(std::size_t(S.nck) * N * 4);
+    }
+    if (mode == LHIP_TREE_RECORD) {
+        S.rec_x.reserve(std::size_t(T + 1) * 3 * N * 8);
+        S.rec_id.reserve(std::size_t(T + 1) * N * 4);
+    }
+    S.inv.reserve(N * 4);
+    S.tmp.reserve(12 * N * 8);
+    S.tmp_m.reserve(N * 4);
+    // initial state: FP64 upload with ids 0..n-1 and float masses
+    std::vector<double> sx, sv;
+    host_soa(x0, N, sx);
+    host_soa(v0, N, sv);
+    std::vector<float> mf(N);
+    std::vector<uint32_t> ids(N);
+    for (std::size_t i = 0; i < N; ++i) { mf[i] = float(m[i]); ids[i] = uint32_t(i); }
+    c.upload_f64(N, sx.data(), sx.data() + N, sx.data() + 2 * N, sv.data(), sv.data() + N, sv.data() + 2 * N,
+                 mf.data(), ids.data());
+    hipStream_t s = c.stream;
+    lhip_force_params fp{p.G, p.eps, p.theta, LHIP_METHOD_TREE, 0};
+    Run R{c, S, s, n, fp};
+    const double h = 0.5 * p.dt;
+    LHIP_CHECK(hipEventRecord(S.ev[0], s));
+    R.forward_force(0, mode);
+    if (mode != LHIP_TREE_REPLAY) R.save_checkpoint(0);
+    for (int st = 0; st < T; ++st) {
+        c.kick(h);
+        c.drift(p.dt);
+        R.forward_force(st + 1, mode);
+        c.kick(h);
+        if (mode != LHIP_TREE_REPLAY && (st + 1) % K == 0 && st + 1 < T) R.save_checkpoint((st + 1) / K);
+    }
+    LHIP_CHECK(hipEventRecord(S.ev[1], s));
+    if (xT) to_host_by_id(s, R.D(c.xd), R.D(c.yd), R.D(c.zd), c.id.as<uint32_t>(), n, xT);
+    if (vT) to_host_by_id(s, R.D(c.vxd), R.D(c.vyd), R.D(c.vzd), c.id.as<uint32_t>(), n, vT);
+    c.sync();
+    float ms = 0;
+    LHIP_CHECK(hipEventElapsedTime(&ms, S.ev[0], S.ev[1]));
+    S.fwd_ms = ms;
+    if (mode == LHIP_TREE_RECORD) {
+        S.have_record = true;
+        S.rec_steps = T;
+    }
+    // A replay leaves the checkpoints of the recorded forward in place: the adjoint stays that of
+    // the recorded trajectory.
+    if (mode != LHIP_TREE_REPLAY) S.have_forward = true;
+}
+
+void treesim_backward(Context& c, const double* gxT, const double* gvT, double* gx0, double* gv0) {
+    if (!c.tsim || !c.tsim->have_forward) throw Error(kLogic, "sim backward: call lhip_sim_forward first");
+    TreeSim& S = *c.tsim;
+    const int n = S.n, T = S.p.steps, K = S.p.checkpoint_every;
+    const std::size_t N = std::size_t(n);
+    const double dt = S.p.dt;
+    hipStream_t s = c.stream;
+    lhip_force_params fp{S.p.G, S.p.eps, S.p.theta, LHIP_METHOD_TREE, 0};
+    const lhip_walk_options* wo = S.p.walk.group_max > 0 ? &S.p.walk : nullptr;
+    c.set_precision(LHIP_MODE_MIXED, wo);
+    Run R{c, S, s, n, fp};
+    S.lx.reserve(3 * N * 8);
+    S.lvx.reserve(3 * N * 8);
+    S.u.reserve(3 * N * 8);
+    S.gt.reserve(3 * N * 8);
+    S.gs.reserve(3 * N * 8);
+    S.seg_d.reserve(std::size_t(K + 1) * 3 * N * 8);
+    S.seg_id.reserve(std::size_t(K + 1) * N * 4);
+    S.seg_m.reserve(std::size_t(K + 1) * N * 4);
+    double* lx = R.D(S.lx);
+    double* lv = R.D(S.lvx);
+    {
+        std::vector<double> h;
+        host_soa(gxT, N, h);
+        LHIP_CHECK(hipMemcpyAsync(lx, h.data(), 3 * N * 8, hipMemcpyHostToDevice, s));
+        c.sync();
+        host_soa(gvT, N, h);
+        LHIP_CHECK(hipMemcpyAsync(lv, h.data(), 3 * N * 8, hipMemcpyHostToDevice, s));
+        c.sync();
+    }
+    const int blocks = c.cus * tree_vjp_blocks_per_cu();
+    auto vjp_at = [&](int k, double w) {
+        R.load_positions(S.seg_d, S.seg_id, S.seg_m, k);
+        c.build_tree(fp);  // the forward's tree at these positions
+        c.posd.reserve(N * sizeof(double4));
+        const int nodes = int(c.cap_nodes);
+        S.gcell.reserve(std::size_t(nodes) * 9 * 8);
+        S.cum.reserve(std::size_t(nodes) * 9 * 8);
+        double* u = R.D(S.u);
+        gather_u_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(c.id.as<uint32_t>(), lv, lv + N, lv + 2 * N, S.p.G * w, u,
+                                                         u + N, u + 2 * N, n);
+        LHIP_LAUNCH_CHECK();
+        double* gs = R.D(S.gs);
+        double* gt = R.D(S.gt);
+        launch_fill64(s, gs, 0.0, 3 * n);
+        launch_fill64(s, R.D(S.gcell), 0.0, nodes * 9);
+        TreeVjpArgs a{};
+        a.node = c.node.as<WalkNode>();
+        a.mom = c.mom.as<NodeMom>();
+        a.momd = c.momd.as<NodeMomD>();
+        a.posd = c.posd.as<double4>();
+        a.ux = u; a.uy = u + N; a.uz = u + 2 * N;
+        a.gstart = c.gstart.as<uint32_t>();
+        a.gcount = c.gflag.as<uint32_t>();
+        a.n = n;
+        a.eps2 = float(S.p.eps * S.p.eps);
+        a.small_leaf = c.wopt.small_leaf;
+        a.quad_tau = float(c.wopt.quad_tau);
+        a.gtx = gt; a.gty = gt + N; a.gtz = gt + 2 * N;
+        a.gsx = gs; a.gsy = gs + N; a.gsz = gs + 2 * N;
+        a.gcell = R.D(S.gcell);
+        a.counters = c.counters.as<WalkCounters>();
+        a.blocks = blocks;
+        tree_vjp(s, a, c.levstart.as<uint32_t>(), c.cnt.as<uint32_t>() + n, nodes, R.D(S.cum));
+        scatter_add_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(c.id.as<uint32_t>(), gt, gt + N, gt + 2 * N, gs, gs + N,
+                                                            gs + 2 * N, lx, lx + N, lx + 2 * N, n);
+        LHIP_LAUNCH_CHECK();
+    };
+    auto weight = [&](int q) { return (q == T || q == 0) ? 0.5 * dt : dt; };
+    LHIP_CHECK(hipEventRecord(S.ev[2], s));
+    for (int sg = S.nck - 1; sg >= 0 && T > 0; --sg) {
+        const int start = sg * K, end = std::min(T, start + K);
+        R.load_checkpoint(sg);
+        R.save_positions(S.seg_d, S.seg_id, S.seg_m, 0);
+        for (int t = start; t < end; ++t) {
+            c.kick(0.5 * dt);
+            c.drift(dt);
+            if (t + 1 < end) {
+                R.force();
+                R.save_positions(S.seg_d, S.seg_id, S.seg_m, t + 1 - start);
+                c.kick(0.5 * dt);
+            } else {
+                R.save_positions(S.seg_d, S.seg_id, S.seg_m, t + 1 - start);
+            }
+        }
+        const int first = end == T ? end : end - 1;
+        for (int q = first; q >= start; --q) {
+            vjp_at(q - start, weight(q));
+            if (q > 0) launch_axpy_f64(s, lv, lx, dt, 3 * n);
+        }
+    }
+    LHIP_CHECK(hipEventRecord(S.ev[3], s));
+    std::vector<double> h(3 * N);
+    auto out = [&](const double* d, double* dst) {
+        LHIP_CHECK(hipMemcpyAsync(h.data(), d, 3 * N * 8, hipMemcpyDeviceToHost, s));
+        c.sync();
+        for (std::size_t i = 0; i < N; ++i)
+            for (int k = 0; k < 3; ++k) dst[3 * i + k] = h[k * N + i];
+    };
+    if (gx0) out(lx, gx0);
+    if (gv0) out(lv, gv0);
+    c.sync();
+    float ms = 0;
+    LHIP_CHECK(hipEventElapsedTime(&ms, S.ev[2], S.ev[3]));
+    S.bwd_ms = ms;
+}
+
+void treesim_times(Context& c, double* f, double* b) {
+    if (!c.tsim) throw Error(kLogic, "sim: no tree simulation run yet");
+    if (f) *f = c.tsim->fwd_ms;
+    if (b) *b = c.tsim->bwd_ms;
+}
+
+}  // namespace lhip

@@ -1,0 +1,166 @@
+"""GPU tests of the differentiable simulation through the tree (frozen-topology adjoint of the
+mixed-precision walk): replay of a recorded forward, the adjoint against central differences of the
+frozen-topology forward (parameters of the two-disk encounter and single coordinates), against the
+direct-sum adjoint, and checkpoint invariance. No reference counterpart: the reference has no
+differentiation."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import lettuce_simulate_physics_amd as L
+from lettuce_simulate_physics_amd.autograd import SimParams, Simulation
+
+pytestmark = pytest.mark.gpu
+
+PRM = dict(method="tree", theta=0.6, eps=0.02, dt=5e-3, steps=40, checkpoint_every=8)
+
+
+@pytest.fixture(autouse=True)
+def plain_after(hip):
+    # the tree simulation runs on the context's bodies in the mixed mode; restore the plain mode
+    yield
+    hip.set_precision("plain")
+
+
+def encounter(nstars, seed=3):
+    return L.Collision(nstars=nstars, seed=seed, pericentre=4.0, incl1=0.5, incl2=1.0)
+
+
+def com2_loss(C, m, target):
+    """Squared distance of disk 2's centre of mass from target: loss(xT) and dL/dxT."""
+    n1 = C.nstars + 1
+    w = np.zeros(len(m))
+    w[n1:] = m[n1:] / m[n1:].sum()
+
+    def f(xT):
+        c = (w[:, None] * xT).sum(0)
+        return float(((c - target) ** 2).sum()), 2 * w[:, None] * (c - target)[None, :]
+    return f
+
+
+def test_replay_reproduces_the_recorded_forward(hip):
+    C = encounter(3000)
+    x0, v0, m = C.generate()
+    rec = Simulation(hip, SimParams(**{**PRM, "steps": 12}, tree_mode="record"))
+    xa, va = rec.forward(x0, v0, m)
+    rep = Simulation(hip, SimParams(**{**PRM, "steps": 12}, tree_mode="replay"))
+    xb, vb = rep.forward(x0, v0, m)
+    assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+    # and the plain tree forward is the same trajectory
+    plain = Simulation(hip, SimParams(**{**PRM, "steps": 12}))
+    xc, vc = plain.forward(x0, v0, m)
+    assert np.array_equal(xa, xc) and np.array_equal(va, vc)
+
+
+def test_tree_adjoint_parameter_gradient_matches_frozen_forward_fd(hip):
+    # dL/d(pericentre, incl1, incl2) through the generator's VJP against central differences of the
+    # frozen-topology forward (replay of the recorded topology with perturbed initial conditions).
+    C = encounter(2000)
+    x0, v0, m = C.generate()
+    target = np.array([1.0, -2.0, 0.5])
+    loss = com2_loss(C, m, target)
+    rec = Simulation(hip, SimParams(**PRM, tree_mode="record"))
+    xT, _ = rec.forward(x0, v0, m)
+    L0, gxT = loss(xT)
+    gx0, gv0, gm = rec.backward(gxT, np.zeros_like(gxT))
+    assert gm is None
+    g = C.vjp(gx0, gv0)
+    rep = Simulation(hip, SimParams(**PRM, tree_mode="replay"))
+    res = []
+    for k, (name, h) in enumerate((("pericentre", 1e-3), ("incl1", 1e-3), ("incl2", 1e-3))):
+        vals = []
+        for sgn in (1, -1):
+            Cp = L.Collision(**{**C.__dict__, name: getattr(C, name) + sgn * h})
+            xp, vp, mp = Cp.generate()
+            xq, _ = rep.forward(xp, vp, mp)
+            vals.append(loss(xq)[0])
+        fd = (vals[0] - vals[1]) / (2 * h)
+        res.append((name, fd, g[k]))
+    print("parameter gradients (FD, adjoint):", [(n, f"{a:.6g}", f"{b:.6g}") for n, a, b in res])
+    for name, fd, ad in res:
+        assert abs(fd - ad) <= 1e-4 * max(abs(fd), 1e-3 * max(abs(r[1]) for r in res)), name
+
+
+def test_tree_adjoint_single_coordinates_match_frozen_forward_fd(hip):
+    # Position and velocity coordinates of single bodies, with a loss on every body's final state, so
+    # that the force Jacobian (tidal and quadrupole terms, cell moments) carries the gradients.
+    C = encounter(1500, seed=5)
+    x0, v0, m = C.generate()
+    n = len(m)
+    rng = np.random.default_rng(2)
+    cx, cv = rng.standard_normal((n, 3)), rng.standard_normal((n, 3))
+    rec = Simulation(hip, SimParams(**PRM, tree_mode="record"))
+    xT, vT = rec.forward(x0, v0, m)
+    gx0, gv0, _ = rec.backward(cx, cv)
+    rep = Simulation(hip, SimParams(**PRM, tree_mode="replay"))
+
+    def loss(x, v):
+        a, b = rep.forward(x, v, m)
+        return float((cx * a).sum() + (cv * b).sum())
+
+    worst = 0.0
+    for k in range(8):
+        i, c = int(rng.integers(n)), int(rng.integers(3))
+        for arr, g, base in ((0, gx0, x0), (1, gv0, v0)):
+            h = 1e-3
+            a, b = base.copy(), base.copy()
+            a[i, c] += h
+            b[i, c] -= h
+            fd = (loss(a, v0) - loss(b, v0)) / (2 * h) if arr == 0 else (loss(x0, a) - loss(x0, b)) / (2 * h)
+            worst = max(worst, abs(fd - g[i, c]) / max(abs(fd), 1.0))
+    print(f"single coordinates: worst relative {worst:.3g}")
+    assert worst < 1e-3
+
+
+def test_tree_adjoint_agrees_with_the_direct_sum_adjoint(hip):
+    # The same encounter, loss and horizon through the tree (theta 0.6) and through the direct sum:
+    # the parameter gradients agree to the tree's force accuracy.
+    C = encounter(4096)
+    x0, v0, m = C.generate()
+    target = np.array([1.0, -2.0, 0.5])
+    loss = com2_loss(C, m, target)
+    out = {}
+    for method in ("tree", "direct"):
+        prm = {**PRM, "method": method}
+        sim = Simulation(hip, SimParams(**prm, tree_mode="build"))
+        xT, _ = sim.forward(x0, v0, m)
+        _, gxT = loss(xT)
+        gx0, gv0, _ = sim.backward(gxT, np.zeros_like(gxT))
+        out[method] = C.vjp(gx0, gv0)
+    rel = np.abs(out["tree"] - out["direct"]) / np.abs(out["direct"])
+    print(f"tree vs direct adjoint: {out['tree']} vs {out['direct']}, relative {rel}")
+    assert np.all(rel < 1e-2)
+
+
+def test_tree_checkpoint_interval_does_not_change_gradients(hip):
+    C = encounter(1000, seed=7)
+    x0, v0, m = C.generate()
+    out = []
+    for K in (1, 5, 40):
+        sim = Simulation(hip, SimParams(**{**PRM, "checkpoint_every": K}))
+        xT, vT = sim.forward(x0, v0, m)
+        out.append(sim.backward(np.ones_like(xT), vT)[:2])
+    for g in out[1:]:
+        for a, b in zip(g, out[0]):
+            # source-side sums arrive by atomic adds: equal up to summation order
+            assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(b))
+
+
+def test_tree_backward_without_mass_gradient_and_replay_errors(hip):
+    C = encounter(500, seed=9)
+    x0, v0, m = C.generate()
+    sim = Simulation(hip, SimParams(**{**PRM, "steps": 4}))
+    xT, vT = sim.forward(x0, v0, m)
+    lib = L.library()
+    import ctypes
+    n = len(m)
+    z = np.zeros((n, 3))
+    gm = np.zeros(n)
+    rc = lib.lhip_sim_backward(hip._h, z.ctypes.data_as(ctypes.c_void_p), z.ctypes.data_as(ctypes.c_void_p),
+                               None, None, gm.ctypes.data_as(ctypes.c_void_p))
+    assert rc == L.INVALID
+    other = encounter(600, seed=9)
+    x1, v1, m1 = other.generate()
+    with pytest.raises(L.LogicError):
+        Simulation(hip, SimParams(**{**PRM, "steps": 4}, tree_mode="replay")).forward(x1, v1, m1)
