@@ -219,6 +219,35 @@ def disk_leg(be, nstars: int, steps: int, wopt: dict):
             "mixed_force_speedup_vs_plain": round(res["plain"]["force_ms"] / res["mixed"]["force_ms"], 3)}
 
 
+def tree_adjoint_leg(be, nstars: int, steps: int):
+    """Adjoint through the frozen-topology tree walk: two sech^2 disks of nstars stars and a core of 10
+    each, inclined 30 and 60 degrees, 20 Rd apart on a parabolic approach with pericentre 4 Rd; tree
+    theta 0.6 (mixed form, its default walk options), eps 0.02, KDK dt 1e-3, checkpoints every 20
+    steps. Loss: squared distance of disk 2's final centre of mass from a seeded target point.
+    Times one forward and one backward (device events); gradients in (pericentre, incl1, incl2)."""
+    import lettuce_simulate_physics_amd as L
+    from lettuce_simulate_physics_amd.autograd import SimParams, Simulation
+
+    C = L.Collision(nstars=nstars, seed=1, pericentre=4.0, incl1=np.pi / 6, incl2=np.pi / 3)
+    x0, v0, m = C.generate()
+    n1 = nstars + 1
+    w = np.zeros(len(m))
+    w[n1:] = m[n1:] / m[n1:].sum()
+    target = np.random.default_rng(2027).standard_normal(3)
+    sim = Simulation(be, SimParams(method="tree", theta=0.6, eps=0.02, dt=1e-3, steps=steps, checkpoint_every=20))
+    xT, vT = sim.forward(x0, v0, m)
+    fwd_ms, _ = sim.times()
+    com = (w[:, None] * xT).sum(0)
+    gx, gv, _ = sim.backward(2 * w[:, None] * (com - target)[None, :], np.zeros_like(vT))
+    _, bwd_ms = sim.times()
+    g = C.vjp(gx, gv)
+    be.set_precision("plain")
+    return {"workload": f"two_disk_encounter_2x{nstars}_tree_adjoint", "steps": steps, "checkpoint_every": 20,
+            "forward_ms_per_step": round(fwd_ms / steps, 4), "adjoint_ms_per_step": round(bwd_ms / steps, 4),
+            "adjoint_forward_ratio": round(bwd_ms / fwd_ms, 3), "loss": float(((com - target) ** 2).sum()),
+            "grad_pericentre": float(g[0]), "grad_incl1": float(g[1]), "grad_incl2": float(g[2])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -234,6 +263,8 @@ def main():
     ap.add_argument("--mixed-tau", type=float, default=None, help="far-field quadrupole threshold (0: all kept)")
     ap.add_argument("--disk-stars", type=int, default=4194304)
     ap.add_argument("--disk-steps", type=int, default=50)
+    ap.add_argument("--encounter-stars", type=int, default=524288, help="stars per disk of the tree adjoint leg")
+    ap.add_argument("--encounter-steps", type=int, default=100, help="0 skips the tree adjoint leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy 3 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -310,6 +341,8 @@ def main():
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(be)
         out["adjoint_forward_ratio"] = out["adjoint"]["adjoint_forward_ratio"]
+        if args.encounter_steps > 0:
+            out["tree_adjoint"] = tree_adjoint_leg(be, args.encounter_stars, args.encounter_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(b, args.cpu_steps)
         out["cpu_baseline"]["value"] = round(out["cpu_baseline"]["value"], 4)

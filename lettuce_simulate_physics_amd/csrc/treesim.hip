@@ -342,6 +342,7 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
     if (!c.tsim || !c.tsim->have_forward) throw Error(kLogic, "sim backward: call lhip_sim_forward first");
     TreeSim& S = *c.tsim;
     const int n = S.n, T = S.p.steps, K = S.p.checkpoint_every;
+    if (c.n != n) throw Error(kLogic, "sim backward: the context's bodies changed since the forward");
     const std::size_t N = std::size_t(n);
     const double dt = S.p.dt;
     hipStream_t s = c.stream;
