@@ -16,6 +16,10 @@
 // gets m_j [gC / M + (6 G - 2 tr(G) I) (x_j - com)]. Summed over the cells containing j, that is
 // m_j (A + B x_j) with A, B accumulated from the root down (down_level) and applied at the leaves.
 //
+// Contention. The top of the tree (the first kHot nodes in level order) sits in nearly every
+// group's list, so its cotangents go to a private copy per block (gpriv, blocks x kHot x 9) that
+// hot_reduce adds up afterwards; deeper cells and bodies take global atomics directly.
+//
 // Evaluation is entry-major: each lane owns a list entry (one cell, or two bodies) and loops over
 // the group's targets, whose positions and cotangents come from the slot registers by readlane. The
 // lane keeps its entry's source-side sums; each target's side is a wave sum (DPP butterflies). The
@@ -62,6 +66,14 @@ __global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
     const int ngroups = int(a.gcount[a.n]);
     const float e2 = a.eps2;
     bool overflow = false;
+
+    double* const priv = a.gpriv + std::size_t(blockIdx.x) * kHot * 12;
+    auto cell_out = [&](int k) -> double* {  // quadrupole cell k: gC (3), G (6)
+        return k < kHot ? priv + std::size_t(k) * 9 : a.gcell + std::size_t(k) * 9;
+    };
+    auto mono_out = [&](int k) -> double* {  // monopole cell k: gC (3)
+        return k < kHot ? priv + 9 * kHot + std::size_t(k) * 3 : a.gmono + std::size_t(k) * 3;
+    };
 
     while (true) {
         int g = 0;
@@ -164,11 +176,19 @@ __global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
                     Gxz += h5 * (ux * dz + dx * uz) + k7 * dx * dz;
                     Gyz += h5 * (uy * dz + dy * uz) + k7 * dy * dz;
                 }
+                // Component-major atomics: the 9 values of each cell go through LDS (cA..cC, already
+                // in registers) so that neighbouring lanes add neighbouring doubles of one cell.
+                float* tb = reinterpret_cast<float*>(L.cA);
+                __builtin_amdgcn_wave_barrier();
                 if (valid) {
-                    double* o = a.gcell + std::size_t(L.cI[lane]) * 9;
-                    unsafeAtomicAdd(o + 0, double(sCx)); unsafeAtomicAdd(o + 1, double(sCy)); unsafeAtomicAdd(o + 2, double(sCz));
-                    unsafeAtomicAdd(o + 3, double(Gxx)); unsafeAtomicAdd(o + 4, double(Gxy)); unsafeAtomicAdd(o + 5, double(Gxz));
-                    unsafeAtomicAdd(o + 6, double(Gyy)); unsafeAtomicAdd(o + 7, double(Gyz)); unsafeAtomicAdd(o + 8, double(Gzz));
+                    float* q = tb + lane * 9;
+                    q[0] = sCx; q[1] = sCy; q[2] = sCz; q[3] = Gxx; q[4] = Gxy;
+                    q[5] = Gxz; q[6] = Gyy; q[7] = Gyz; q[8] = Gzz;
+                }
+                __builtin_amdgcn_wave_barrier();
+                for (int item = lane; item < 9 * nc; item += 64) {
+                    const int c = item / 9;
+                    unsafeAtomicAdd(cell_out(L.cI[c]) + (item - 9 * c), double(tb[item]));
                 }
                 fold();
                 __builtin_amdgcn_wave_barrier();
@@ -207,18 +227,32 @@ __global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
                     }
                     add_target(t, wave_sum(tx), wave_sum(ty), wave_sum(tz));
                 }
+                // Bodies: consecutive entries are consecutive bodies of a leaf, so the adds coalesce.
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    if (!valid[h]) continue;
+                for (int h = 0; h < 2; ++h)
                     if (J[h] >= 0) {
                         unsafeAtomicAdd(a.gsx + J[h], double(s[h][0]));
                         unsafeAtomicAdd(a.gsy + J[h], double(s[h][1]));
                         unsafeAtomicAdd(a.gsz + J[h], double(s[h][2]));
-                    } else if (J[h] <= -2) {
-                        double* o = a.gcell + std::size_t(-J[h] - 2) * 9;
-                        unsafeAtomicAdd(o + 0, double(s[h][0]));
-                        unsafeAtomicAdd(o + 1, double(s[h][1]));
-                        unsafeAtomicAdd(o + 2, double(s[h][2]));
+                    }
+                // Monopole cells: compacted through LDS (bP, bI, already in registers) and added
+                // component-major.
+                const unsigned long long m0 = __ballot(J[0] <= -2), m1 = __ballot(J[1] <= -2);
+                const int n0 = __popcll(m0), nm = n0 + __popcll(m1);
+                if (nm) {
+                    float* mb = reinterpret_cast<float*>(L.bP);
+                    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+                        if (J[h] <= -2) {
+                            const int slot = (h ? n0 : 0) + int(lanes_below(h ? m1 : m0));
+                            L.bI[slot] = -J[h] - 2;
+                            mb[3 * slot] = s[h][0]; mb[3 * slot + 1] = s[h][1]; mb[3 * slot + 2] = s[h][2];
+                        }
+                    __builtin_amdgcn_wave_barrier();
+                    for (int item = lane; item < 3 * nm; item += 64) {
+                        const int c = item / 3;
+                        unsafeAtomicAdd(mono_out(L.bI[c]) + (item - 3 * c), double(mb[item]));
                     }
                 }
                 fold();
@@ -302,12 +336,21 @@ __global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
                         if (nb == kBodies) flush_bodies();
                         const int take = min(kBodies - nb, total - done);
                         const int u0 = max(off, done), u1 = min(off + len, done + take);
-                        for (int u = u0; u < u1; ++u) {
-                            const int j = nd.lo + (u - off);
-                            const double4 d = a.posd[j];
-                            L.bP[nb + (u - done)] = make_float4(float(d.x - ccx), float(d.y - ccy), float(d.z - ccz),
-                                                                float(d.w));
-                            L.bI[nb + (u - done)] = j;
+                        // Four loads in flight per lane before their LDS stores (a load past u1 reads
+                        // the range's last body again and is not stored).
+                        for (int u = u0; u < u1; u += 4) {
+                            float4 q[4];
+#pragma unroll
+                            for (int t = 0; t < 4; ++t) {
+                                const double4 d = a.posd[nd.lo + (min(u + t, u1 - 1) - off)];
+                                q[t] = make_float4(float(d.x - ccx), float(d.y - ccy), float(d.z - ccz), float(d.w));
+                            }
+#pragma unroll
+                            for (int t = 0; t < 4; ++t)
+                                if (u + t < u1) {
+                                    L.bP[nb + (u + t - done)] = q[t];
+                                    L.bI[nb + (u + t - done)] = nd.lo + (u + t - off);
+                                }
                         }
                         nb += take;
                         done += take;
@@ -331,6 +374,21 @@ __global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
     if (overflow && lane == 0) atomicOr(&a.counters->error, 1);
 }
 
+// The per-block copies of the first kHot nodes (9 quadrupole-cell then 3 monopole-cell values
+// per node) added into gcell and gmono; node_count bounds the nodes.
+__global__ void __launch_bounds__(kBS) hot_reduce(const double* __restrict__ gpriv, int blocks,
+                                                  const uint32_t* __restrict__ node_count,
+                                                  double* __restrict__ gcell, double* __restrict__ gmono) {
+    const int t = blockIdx.x * kBS + threadIdx.x;
+    if (t >= kHot * 12) return;
+    const int k = t < 9 * kHot ? t / 9 : (t - 9 * kHot) / 3;
+    if (k >= int(*node_count)) return;
+    double s = 0;
+    for (int b = 0; b < blocks; ++b) s += gpriv[std::size_t(b) * kHot * 12 + t];
+    if (t < 9 * kHot) gcell[t] += s;
+    else gmono[t - 9 * kHot] += s;
+}
+
 __global__ void reset_vjp(WalkCounters* __restrict__ c) {
     c->interactions = c->p2p = c->p2c = c->listed = c->p2m = 0;
     c->error = 0;
@@ -338,8 +396,10 @@ __global__ void reset_vjp(WalkCounters* __restrict__ c) {
 }
 
 // Per-node A (3) and B (6: xx, xy, xz, yy, yz, zz) of the downward pass: the node's own terms
-// B = 6 G - 2 tr(G) I and A = gC / M - B com.
-__device__ __forceinline__ void own_terms(const double* __restrict__ gc, const NodeMomD& md, double o[9]) {
+// B = 6 G - 2 tr(G) I and A = gC / M - B com, gC summed over the quadrupole (gc) and monopole
+// (gm) entries of the node.
+__device__ __forceinline__ void own_terms(const double* __restrict__ gc, const double* __restrict__ gm,
+                                          const NodeMomD& md, double o[9]) {
     if (!(md.m > 0)) {
         for (int c = 0; c < 9; ++c) o[c] = 0.0;
         return;
@@ -347,17 +407,17 @@ __device__ __forceinline__ void own_terms(const double* __restrict__ gc, const N
     const double t = gc[3] + gc[6] + gc[8];
     const double Bxx = 6 * gc[3] - 2 * t, Bxy = 6 * gc[4], Bxz = 6 * gc[5];
     const double Byy = 6 * gc[6] - 2 * t, Byz = 6 * gc[7], Bzz = 6 * gc[8] - 2 * t;
-    o[0] = gc[0] / md.m - (Bxx * md.cx + Bxy * md.cy + Bxz * md.cz);
-    o[1] = gc[1] / md.m - (Bxy * md.cx + Byy * md.cy + Byz * md.cz);
-    o[2] = gc[2] / md.m - (Bxz * md.cx + Byz * md.cy + Bzz * md.cz);
+    o[0] = (gc[0] + gm[0]) / md.m - (Bxx * md.cx + Bxy * md.cy + Bxz * md.cz);
+    o[1] = (gc[1] + gm[1]) / md.m - (Bxy * md.cx + Byy * md.cy + Byz * md.cz);
+    o[2] = (gc[2] + gm[2]) / md.m - (Bxz * md.cx + Byz * md.cy + Bzz * md.cz);
     o[3] = Bxx; o[4] = Bxy; o[5] = Bxz; o[6] = Byy; o[7] = Byz; o[8] = Bzz;
 }
 
-__global__ void __launch_bounds__(kBS) down_root(const double* __restrict__ gcell, const NodeMomD* __restrict__ momd,
-                                                 double* __restrict__ cum) {
+__global__ void __launch_bounds__(kBS) down_root(const double* __restrict__ gcell, const double* __restrict__ gmono,
+                                                 const NodeMomD* __restrict__ momd, double* __restrict__ cum) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         double o[9];
-        own_terms(gcell, momd[0], o);
+        own_terms(gcell, gmono, momd[0], o);
         for (int c = 0; c < 9; ++c) cum[c] = o[c];
     }
 }
@@ -366,6 +426,7 @@ __global__ void __launch_bounds__(kBS) down_root(const double* __restrict__ gcel
 __global__ void __launch_bounds__(kBS) down_level(int level, const uint32_t* __restrict__ levstart,
                                                   const WalkNode* __restrict__ node,
                                                   const double* __restrict__ gcell,
+                                                  const double* __restrict__ gmono,
                                                   const NodeMomD* __restrict__ momd, double* __restrict__ cum) {
     const int p0 = int(levstart[level]), p1 = int(levstart[level + 1]);
     for (int p = p0 + blockIdx.x * kBS + threadIdx.x; p < p1; p += gridDim.x * kBS) {
@@ -376,7 +437,7 @@ __global__ void __launch_bounds__(kBS) down_level(int level, const uint32_t* __r
         for (int c = 0; c < 9; ++c) cp[c] = cum[std::size_t(p) * 9 + c];
         for (int ch = nd.first; ch < nd.first + nch; ++ch) {
             double o[9];
-            own_terms(gcell + std::size_t(ch) * 9, momd[ch], o);
+            own_terms(gcell + std::size_t(ch) * 9, gmono + std::size_t(ch) * 3, momd[ch], o);
             for (int c = 0; c < 9; ++c) cum[std::size_t(ch) * 9 + c] = cp[c] + o[c];
         }
     }
@@ -410,11 +471,14 @@ void tree_vjp(hipStream_t s, const TreeVjpArgs& a, const uint32_t* levstart, con
     LHIP_LAUNCH_CHECK();
     tree_vjp_kernel<<<std::max(1, a.blocks), kBS4, 0, s>>>(a);
     LHIP_LAUNCH_CHECK();
-    down_root<<<1, 64, 0, s>>>(a.gcell, a.momd, cum);
+    hot_reduce<<<grid_for(kHot * 12, kBS), kBS, 0, s>>>(a.gpriv, std::max(1, a.blocks), node_count, a.gcell,
+                                                         a.gmono);
+    LHIP_LAUNCH_CHECK();
+    down_root<<<1, 64, 0, s>>>(a.gcell, a.gmono, a.momd, cum);
     LHIP_LAUNCH_CHECK();
     const unsigned g = std::min<unsigned>(grid_for(a.n, kBS), 512u);
     for (int l = 0; l < kMaxLevel; ++l) {
-        down_level<<<g, kBS, 0, s>>>(l, levstart, a.node, a.gcell, a.momd, cum);
+        down_level<<<g, kBS, 0, s>>>(l, levstart, a.node, a.gcell, a.gmono, a.momd, cum);
         LHIP_LAUNCH_CHECK();
     }
     down_leaves<<<grid_for(nodes, kBS), kBS, 0, s>>>(node_count, a.node, cum, a.posd, a.gsx, a.gsy, a.gsz);

@@ -109,8 +109,8 @@ int walk_blocks_per_cu();
 // Vector-Jacobian product of the mixed-precision walk's forces with frozen topology
 // (kernels_treeadj.hip). u = cotangent on the accelerations (already scaled by G and the kick
 // weight), sorted order. Target-side terms are written to gt*; body source terms and the
-// cell-moment terms carried down the tree are added to gs* (zeroed by the caller), gcell (9 per
-// node, zeroed by the caller) and cum (9 per node) are scratch.
+// cell-moment terms carried down the tree are added to gs* (zeroed by the caller); gcell, gmono and
+// gpriv (zeroed by the caller) and cum (9 per node) are scratch.
 struct TreeVjpArgs {
     const WalkNode* node;
     const NodeMom* mom;
@@ -125,10 +125,14 @@ struct TreeVjpArgs {
     float quad_tau;
     double *gtx, *gty, *gtz;
     double *gsx, *gsy, *gsz;
-    double* gcell;
+    double* gcell;  // per node: gC (3) and G (6) of its quadrupole-cell entries
+    double* gmono;  // per node: gC (3) of its monopole-cell entries
+    double* gpriv;  // blocks x kHot x 12 doubles (per-block gcell, gmono of the first kHot nodes), zeroed
     WalkCounters* counters;
     int blocks;
 };
+// Nodes at the top of the tree (in level order) whose VJP cotangents each block accumulates privately.
+constexpr int kHot = 4096;
 // node_count: the device node count of the tree (nodes = its capacity, the launch bound).
 void tree_vjp(hipStream_t s, const TreeVjpArgs& a, const uint32_t* levstart, const uint32_t* node_count, int nodes,
               double* cum);

@@ -103,7 +103,7 @@ struct TreeSim {
     // records per force call: sorted ids and sorted positions
     DevBuf rec_id, rec_x;
     // cotangents (id order) and VJP scratch (sorted order)
-    DevBuf lx, lvx, u, gt, gs, gcell, cum, inv, tmp, tmp_m, tmp_id, posd_rec;
+    DevBuf lx, lvx, u, gt, gs, gcell, gmono, gpriv, cum, inv, tmp, tmp_m, tmp_id, posd_rec;
 
     TreeSim() {
         for (auto& e : ev) LHIP_CHECK(hipEventCreate(&e));
@@ -376,6 +376,7 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
         c.posd.reserve(N * sizeof(double4));
         const int nodes = int(c.cap_nodes);
         S.gcell.reserve(std::size_t(nodes) * 9 * 8);
+        S.gmono.reserve(std::size_t(nodes) * 3 * 8);
         S.cum.reserve(std::size_t(nodes) * 9 * 8);
         double* u = R.D(S.u);
         gather_u_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(c.id.as<uint32_t>(), lv, lv + N, lv + 2 * N, S.p.G * w, u,
@@ -385,6 +386,9 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
         double* gt = R.D(S.gt);
         launch_fill64(s, gs, 0.0, 3 * n);
         launch_fill64(s, R.D(S.gcell), 0.0, nodes * 9);
+        launch_fill64(s, R.D(S.gmono), 0.0, nodes * 3);
+        S.gpriv.reserve(std::size_t(blocks) * kHot * 12 * 8);
+        launch_fill64(s, R.D(S.gpriv), 0.0, blocks * kHot * 12);
         TreeVjpArgs a{};
         a.node = c.node.as<WalkNode>();
         a.mom = c.mom.as<NodeMom>();
@@ -400,6 +404,8 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
         a.gtx = gt; a.gty = gt + N; a.gtz = gt + 2 * N;
         a.gsx = gs; a.gsy = gs + N; a.gsz = gs + 2 * N;
         a.gcell = R.D(S.gcell);
+        a.gmono = R.D(S.gmono);
+        a.gpriv = R.D(S.gpriv);
         a.counters = c.counters.as<WalkCounters>();
         a.blocks = blocks;
         tree_vjp(s, a, c.levstart.as<uint32_t>(), c.cnt.as<uint32_t>() + n, nodes, R.D(S.cum));
