@@ -17,15 +17,18 @@
 // m_j (A + B x_j) with A, B accumulated from the root down (down_level) and applied at the leaves.
 //
 // Contention. The top of the tree (the first kHot nodes in level order) sits in nearly every
-// group's list, so its cotangents go to a private copy per block (gpriv, blocks x kHot x 9) that
+// group's list, so its cotangents go to a private copy per block (gpriv, blocks x kHot x 12) that
 // hot_reduce adds up afterwards; deeper cells and bodies take global atomics directly.
 //
 // Evaluation is entry-major: each lane owns a list entry (one cell, or two bodies) and loops over
-// the group's targets, whose positions and cotangents come from the slot registers by readlane. The
-// lane keeps its entry's source-side sums; each target's side is a wave sum (DPP butterflies). The
-// source sums go out as FP64 atomic adds per (group, entry) (hardware float atomics on this
-// library's coarse-grained device buffers); FP32 partial sums per list batch are carried into FP64.
+// the group's targets four at a time, whose positions and cotangents come from the slot registers
+// by readlane. The lane keeps its entry's source-side sums; the target sides of four targets are
+// reduced over the wave together (wave_sum4: 32- and 16-lane swaps, then DPP butterflies) and
+// carried into FP64 once per list batch. The source sums go out as FP64 atomic adds per
+// (group, entry), component-major through LDS so that neighbouring lanes hit neighbouring
+// doubles (hardware float atomics on this library's coarse-grained device buffers).
 #include <cfloat>
+#include <type_traits>
 
 #include "kernels.hpp"
 #include "tree.hpp"
@@ -51,10 +54,9 @@ struct VjpLds {
 };
 
 struct Target {
-    float x, y, z, ux, uy, uz;
+    float x, y, z, ux, uy, uz;  // position relative to the group centre, cotangent
     int i;
-    float fx, fy, fz;     // FP32 sums of the current batch
-    double gx, gy, gz;    // FP64 totals
+    double gx, gy, gz;          // FP64 totals of the target-side terms
 };
 
 __global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
@@ -94,7 +96,6 @@ __global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
                 const double4 d = a.posd[tg[s].i];
                 mnx = fminf(mnx, float(d.x)); mny = fminf(mny, float(d.y)); mnz = fminf(mnz, float(d.z));
                 mxx = fmaxf(mxx, float(d.x)); mxy = fmaxf(mxy, float(d.y)); mxz = fmaxf(mxz, float(d.z));
-                tg[s].fx = tg[s].fy = tg[s].fz = 0.f;
                 tg[s].gx = tg[s].gy = tg[s].gz = 0.0;
             }
             // The forward walk's group box (same float rounding), and its centre for relative
@@ -110,27 +111,26 @@ __global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
                 tg[s].x = float(d.x - ccx); tg[s].y = float(d.y - ccy); tg[s].z = float(d.z - ccz);
                 tg[s].ux = float(a.ux[tg[s].i]); tg[s].uy = float(a.uy[tg[s].i]); tg[s].uz = float(a.uz[tg[s].i]);
             }
-            auto fold = [&]() {
+            // Targets go four at a time (never across the two 64-target slots); wave_sum4 leaves
+            // their sums in one row each, and lanes 0-2 of the row write them to out[3 t + c]. out
+            // is the flush's own LDS, already unpacked into registers. Each target's owner lane
+            // then adds its sums to its FP64 totals.
+            auto put_sums = [&](float* out, int t, float Sx, float Sy, float Sz) {
+                const int c = lane & 15;
+                if (c < 3) out[3 * (t + sum4_target(lane >> 4)) + c] = c == 0 ? Sx : (c == 1 ? Sy : Sz);
+            };
+            auto take_sums = [&](const float* out) {
+                __builtin_amdgcn_wave_barrier();
 #pragma unroll
-                for (int s = 0; s < 2; ++s) {
-                    tg[s].gx += double(tg[s].fx); tg[s].gy += double(tg[s].fy); tg[s].gz += double(tg[s].fz);
-                    tg[s].fx = tg[s].fy = tg[s].fz = 0.f;
+                for (int sl = 0; sl < 2; ++sl) {
+                    const int t = sl * 64 + lane;
+                    if (t < nbody) {
+                        tg[sl].gx += double(out[3 * t]);
+                        tg[sl].gy += double(out[3 * t + 1]);
+                        tg[sl].gz += double(out[3 * t + 2]);
+                    }
                 }
-            };
-            // Adds a wave-uniform vector to target t's FP32 sums.
-            auto add_target = [&](int t, float vx, float vy, float vz) {
-                const bool mine = lane == (t & 63);
-                if (t < 64) {
-                    if (mine) { tg[0].fx += vx; tg[0].fy += vy; tg[0].fz += vz; }
-                } else {
-                    if (mine) { tg[1].fx += vx; tg[1].fy += vy; tg[1].fz += vz; }
-                }
-            };
-            auto target_of = [&](int t, float& x, float& y, float& z, float& ux, float& uy, float& uz) {
-                const int l = t & 63;
-                const Target& T = t < 64 ? tg[0] : tg[1];
-                x = readlane_f(T.x, l); y = readlane_f(T.y, l); z = readlane_f(T.z, l);
-                ux = readlane_f(T.ux, l); uy = readlane_f(T.uy, l); uz = readlane_f(T.uz, l);
+                __builtin_amdgcn_wave_barrier();
             };
 
             int nc = 0, nb = 0;
@@ -143,43 +143,58 @@ __global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
                 const float qxx = B.x, qxy = B.y, qxz = B.z, qyy = B.w, qyz = C.x, qzz = C.y;
                 float sCx = 0.f, sCy = 0.f, sCz = 0.f;
                 float Gxx = 0.f, Gxy = 0.f, Gxz = 0.f, Gyy = 0.f, Gyz = 0.f, Gzz = 0.f;
-                for (int t = 0; t < nbody; ++t) {
-                    float x, y, z, ux, uy, uz;
-                    target_of(t, x, y, z, ux, uy, uz);
-                    const float dx = A.x - x, dy = A.y - y, dz = A.z - z;
-                    const float r2 = __fmaf_rn(dz, dz, __fmaf_rn(dy, dy, __fmaf_rn(dx, dx, e2)));
-                    const float inv = __builtin_amdgcn_rsqf(r2);
-                    const float i2 = inv * inv, i3 = inv * i2, i5 = i3 * i2, i7 = i5 * i2, i9 = i7 * i2;
-                    const float Qdx = qxx * dx + qxy * dy + qxz * dz;
-                    const float Qdy = qxy * dx + qyy * dy + qyz * dz;
-                    const float Qdz = qxz * dx + qyz * dy + qzz * dz;
-                    const float Qux = qxx * ux + qxy * uy + qxz * uz;
-                    const float Quy = qxy * ux + qyy * uy + qyz * uz;
-                    const float Quz = qxz * ux + qyz * uy + qzz * uz;
-                    const float dQd = dx * Qdx + dy * Qdy + dz * Qdz;
-                    const float du = dx * ux + dy * uy + dz * uz;
-                    const float Qdu = Qdx * ux + Qdy * uy + Qdz * uz;
-                    // H u = c_d d + c_u u + i5 Q u - 5 i7 (d.u) Q d
-                    const float c_d = 3.f * A.w * du * i5 - 5.f * Qdu * i7 + 17.5f * dQd * du * i9;
-                    const float c_u = -A.w * i3 - 2.5f * dQd * i7;
-                    const float c_q = -5.f * du * i7;
-                    const float Hx = c_d * dx + c_u * ux + i5 * Qux + c_q * Qdx;
-                    const float Hy = c_d * dy + c_u * uy + i5 * Quy + c_q * Qdy;
-                    const float Hz = c_d * dz + c_u * uz + i5 * Quz + c_q * Qdz;
-                    add_target(t, wave_sum(Hx), wave_sum(Hy), wave_sum(Hz));
-                    sCx -= Hx; sCy -= Hy; sCz -= Hz;
-                    const float h5 = -0.5f * i5, k7 = 2.5f * du * i7;
-                    Gxx += 2.f * h5 * ux * dx + k7 * dx * dx;
-                    Gyy += 2.f * h5 * uy * dy + k7 * dy * dy;
-                    Gzz += 2.f * h5 * uz * dz + k7 * dz * dz;
-                    Gxy += h5 * (ux * dy + dx * uy) + k7 * dx * dy;
-                    Gxz += h5 * (ux * dz + dx * uz) + k7 * dx * dz;
-                    Gyz += h5 * (uy * dz + dy * uz) + k7 * dy * dz;
-                }
-                // Component-major atomics: the 9 values of each cell go through LDS (cA..cC, already
-                // in registers) so that neighbouring lanes add neighbouring doubles of one cell.
-                float* tb = reinterpret_cast<float*>(L.cA);
+                float* out = reinterpret_cast<float*>(L.cA);  // 640 floats (cA..cC)
                 __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int sl = 0; sl < 2; ++sl) {
+                    const int tend = min(nbody - sl * 64, 64);
+                    for (int t0 = 0; t0 < tend; t0 += 4) {
+                        float Hx[4], Hy[4], Hz[4];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            Hx[k] = Hy[k] = Hz[k] = 0.f;
+                            if (t0 + k >= tend) continue;
+                            const Target& T = tg[sl];
+                            const float x = readlane_f(T.x, t0 + k), y = readlane_f(T.y, t0 + k);
+                            const float z = readlane_f(T.z, t0 + k), ux = readlane_f(T.ux, t0 + k);
+                            const float uy = readlane_f(T.uy, t0 + k), uz = readlane_f(T.uz, t0 + k);
+                            const float dx = A.x - x, dy = A.y - y, dz = A.z - z;
+                            const float r2 = __fmaf_rn(dz, dz, __fmaf_rn(dy, dy, __fmaf_rn(dx, dx, e2)));
+                            const float inv = __builtin_amdgcn_rsqf(r2);
+                            const float i2 = inv * inv, i3 = inv * i2, i5 = i3 * i2, i7 = i5 * i2, i9 = i7 * i2;
+                            const float Qdx = qxx * dx + qxy * dy + qxz * dz;
+                            const float Qdy = qxy * dx + qyy * dy + qyz * dz;
+                            const float Qdz = qxz * dx + qyz * dy + qzz * dz;
+                            const float Qux = qxx * ux + qxy * uy + qxz * uz;
+                            const float Quy = qxy * ux + qyy * uy + qyz * uz;
+                            const float Quz = qxz * ux + qyz * uy + qzz * uz;
+                            const float dQd = dx * Qdx + dy * Qdy + dz * Qdz;
+                            const float du = dx * ux + dy * uy + dz * uz;
+                            const float Qdu = Qdx * ux + Qdy * uy + Qdz * uz;
+                            // H u = c_d d + c_u u + i5 Q u - 5 i7 (d.u) Q d
+                            const float c_d = 3.f * A.w * du * i5 - 5.f * Qdu * i7 + 17.5f * dQd * du * i9;
+                            const float c_u = -A.w * i3 - 2.5f * dQd * i7;
+                            const float c_q = -5.f * du * i7;
+                            Hx[k] = c_d * dx + c_u * ux + i5 * Qux + c_q * Qdx;
+                            Hy[k] = c_d * dy + c_u * uy + i5 * Quy + c_q * Qdy;
+                            Hz[k] = c_d * dz + c_u * uz + i5 * Quz + c_q * Qdz;
+                            sCx -= Hx[k]; sCy -= Hy[k]; sCz -= Hz[k];
+                            const float h5 = -0.5f * i5, k7 = 2.5f * du * i7;
+                            Gxx += 2.f * h5 * ux * dx + k7 * dx * dx;
+                            Gyy += 2.f * h5 * uy * dy + k7 * dy * dy;
+                            Gzz += 2.f * h5 * uz * dz + k7 * dz * dz;
+                            Gxy += h5 * (ux * dy + dx * uy) + k7 * dx * dy;
+                            Gxz += h5 * (ux * dz + dx * uz) + k7 * dx * dz;
+                            Gyz += h5 * (uy * dz + dy * uz) + k7 * dy * dz;
+                        }
+                        put_sums(out, sl * 64 + t0, wave_sum4(Hx[0], Hx[1], Hx[2], Hx[3]),
+                                 wave_sum4(Hy[0], Hy[1], Hy[2], Hy[3]), wave_sum4(Hz[0], Hz[1], Hz[2], Hz[3]));
+                    }
+                }
+                take_sums(out);
+                // Component-major atomics: the 9 values of each cell go through LDS (cA..cC) so that
+                // neighbouring lanes add neighbouring doubles of one cell.
+                float* tb = out;
                 if (valid) {
                     float* q = tb + lane * 9;
                     q[0] = sCx; q[1] = sCy; q[2] = sCz; q[3] = Gxx; q[4] = Gxy;
@@ -190,7 +205,6 @@ __global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
                     const int c = item / 9;
                     unsafeAtomicAdd(cell_out(L.cI[c]) + (item - 9 * c), double(tb[item]));
                 }
-                fold();
                 __builtin_amdgcn_wave_barrier();
                 nc = 0;
             };
@@ -198,35 +212,56 @@ __global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
                 __builtin_amdgcn_wave_barrier();
                 float4 P[2];
                 int J[2];
-                bool valid[2];
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
                     const int e = h * 64 + lane;
-                    valid[h] = e < nb;
-                    P[h] = valid[h] ? L.bP[e] : make_float4(0.f, 0.f, 0.f, 0.f);
-                    J[h] = valid[h] ? L.bI[e] : -1;
+                    const bool valid = e < nb;
+                    P[h] = valid ? L.bP[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    J[h] = valid ? L.bI[e] : -1;
                 }
                 float s[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
-                for (int t = 0; t < nbody; ++t) {
-                    float x, y, z, ux, uy, uz;
-                    target_of(t, x, y, z, ux, uy, uz);
-                    const int it = clo + t;
-                    float tx = 0.f, ty = 0.f, tz = 0.f;
+                float* out = reinterpret_cast<float*>(L.bP);  // 512 floats
+                __builtin_amdgcn_wave_barrier();
+                // MASK: the batch holds bodies of the group itself, whose own entry adds nothing.
+                auto run = [&](auto mask) {
+                    constexpr bool MASK = decltype(mask)::value;
 #pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        const float dx = P[h].x - x, dy = P[h].y - y, dz = P[h].z - z;
-                        const float r2 = __fmaf_rn(dz, dz, __fmaf_rn(dy, dy, __fmaf_rn(dx, dx, e2)));
-                        const float inv = __builtin_amdgcn_rsqf(r2);
-                        const float i3 = inv * inv * inv;
-                        const float m = J[h] == it ? 0.f : P[h].w;  // the body's own entry adds nothing
-                        const float mi3 = m * i3;
-                        const float c = -3.f * mi3 * inv * inv * (dx * ux + dy * uy + dz * uz);
-                        const float wx = mi3 * ux + c * dx, wy = mi3 * uy + c * dy, wz = mi3 * uz + c * dz;
-                        s[h][0] += wx; s[h][1] += wy; s[h][2] += wz;
-                        tx -= wx; ty -= wy; tz -= wz;
+                    for (int sl = 0; sl < 2; ++sl) {
+                        const int tend = min(nbody - sl * 64, 64);
+                        for (int t0 = 0; t0 < tend; t0 += 4) {
+                            float tx[4], ty[4], tz[4];
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) {
+                                tx[k] = ty[k] = tz[k] = 0.f;
+                                if (t0 + k >= tend) continue;
+                                const Target& T = tg[sl];
+                                const float x = readlane_f(T.x, t0 + k), y = readlane_f(T.y, t0 + k);
+                                const float z = readlane_f(T.z, t0 + k), ux = readlane_f(T.ux, t0 + k);
+                                const float uy = readlane_f(T.uy, t0 + k), uz = readlane_f(T.uz, t0 + k);
+                                const int it = clo + sl * 64 + t0 + k;
+#pragma unroll
+                                for (int h = 0; h < 2; ++h) {
+                                    const float dx = P[h].x - x, dy = P[h].y - y, dz = P[h].z - z;
+                                    const float r2 = __fmaf_rn(dz, dz, __fmaf_rn(dy, dy, __fmaf_rn(dx, dx, e2)));
+                                    const float inv = __builtin_amdgcn_rsqf(r2);
+                                    const float i3 = inv * inv * inv;
+                                    const float m = (MASK && J[h] == it) ? 0.f : P[h].w;
+                                    const float mi3 = m * i3;
+                                    const float c = -3.f * mi3 * inv * inv * (dx * ux + dy * uy + dz * uz);
+                                    const float wx = mi3 * ux + c * dx, wy = mi3 * uy + c * dy, wz = mi3 * uz + c * dz;
+                                    s[h][0] += wx; s[h][1] += wy; s[h][2] += wz;
+                                    tx[k] -= wx; ty[k] -= wy; tz[k] -= wz;
+                                }
+                            }
+                            put_sums(out, sl * 64 + t0, wave_sum4(tx[0], tx[1], tx[2], tx[3]),
+                                     wave_sum4(ty[0], ty[1], ty[2], ty[3]), wave_sum4(tz[0], tz[1], tz[2], tz[3]));
+                        }
                     }
-                    add_target(t, wave_sum(tx), wave_sum(ty), wave_sum(tz));
-                }
+                };
+                const bool own = (J[0] >= clo && J[0] < chi) || (J[1] >= clo && J[1] < chi);
+                if (__ballot(own)) run(std::true_type{});
+                else run(std::false_type{});
+                take_sums(out);
                 // Bodies: consecutive entries are consecutive bodies of a leaf, so the adds coalesce.
 #pragma unroll
                 for (int h = 0; h < 2; ++h)
@@ -235,13 +270,11 @@ __global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
                         unsafeAtomicAdd(a.gsy + J[h], double(s[h][1]));
                         unsafeAtomicAdd(a.gsz + J[h], double(s[h][2]));
                     }
-                // Monopole cells: compacted through LDS (bP, bI, already in registers) and added
-                // component-major.
+                // Monopole cells: compacted through LDS (bP, bI) and added component-major.
                 const unsigned long long m0 = __ballot(J[0] <= -2), m1 = __ballot(J[1] <= -2);
                 const int n0 = __popcll(m0), nm = n0 + __popcll(m1);
                 if (nm) {
-                    float* mb = reinterpret_cast<float*>(L.bP);
-                    __builtin_amdgcn_wave_barrier();
+                    float* mb = out;
 #pragma unroll
                     for (int h = 0; h < 2; ++h)
                         if (J[h] <= -2) {
@@ -255,7 +288,6 @@ __global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
                         unsafeAtomicAdd(mono_out(L.bI[c]) + (item - 3 * c), double(mb[item]));
                     }
                 }
-                fold();
                 __builtin_amdgcn_wave_barrier();
                 nb = 0;
             };
@@ -375,7 +407,9 @@ __global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
 }
 
 // The per-block copies of the first kHot nodes (9 quadrupole-cell then 3 monopole-cell values
-// per node) added into gcell and gmono; node_count bounds the nodes.
+// per node) added into gcell and gmono: grid.y splits the copies into runs of kHotRun, each summed
+// by one thread and added atomically; node_count bounds the nodes.
+constexpr int kHotRun = 32;
 __global__ void __launch_bounds__(kBS) hot_reduce(const double* __restrict__ gpriv, int blocks,
                                                   const uint32_t* __restrict__ node_count,
                                                   double* __restrict__ gcell, double* __restrict__ gmono) {
@@ -383,10 +417,12 @@ __global__ void __launch_bounds__(kBS) hot_reduce(const double* __restrict__ gpr
     if (t >= kHot * 12) return;
     const int k = t < 9 * kHot ? t / 9 : (t - 9 * kHot) / 3;
     if (k >= int(*node_count)) return;
+    const int b0 = blockIdx.y * kHotRun, b1 = min(blocks, b0 + kHotRun);
     double s = 0;
-    for (int b = 0; b < blocks; ++b) s += gpriv[std::size_t(b) * kHot * 12 + t];
-    if (t < 9 * kHot) gcell[t] += s;
-    else gmono[t - 9 * kHot] += s;
+#pragma unroll 8
+    for (int b = b0; b < b1; ++b) s += gpriv[std::size_t(b) * kHot * 12 + t];
+    if (s == 0.0) return;
+    unsafeAtomicAdd(t < 9 * kHot ? gcell + t : gmono + (t - 9 * kHot), s);
 }
 
 __global__ void reset_vjp(WalkCounters* __restrict__ c) {
@@ -471,8 +507,9 @@ void tree_vjp(hipStream_t s, const TreeVjpArgs& a, const uint32_t* levstart, con
     LHIP_LAUNCH_CHECK();
     tree_vjp_kernel<<<std::max(1, a.blocks), kBS4, 0, s>>>(a);
     LHIP_LAUNCH_CHECK();
-    hot_reduce<<<grid_for(kHot * 12, kBS), kBS, 0, s>>>(a.gpriv, std::max(1, a.blocks), node_count, a.gcell,
-                                                         a.gmono);
+    const int nb = std::max(1, a.blocks);
+    hot_reduce<<<dim3(grid_for(kHot * 12, kBS), (nb + kHotRun - 1) / kHotRun), kBS, 0, s>>>(a.gpriv, nb, node_count,
+                                                                                         a.gcell, a.gmono);
     LHIP_LAUNCH_CHECK();
     down_root<<<1, 64, 0, s>>>(a.gcell, a.gmono, a.momd, cum);
     LHIP_LAUNCH_CHECK();

@@ -131,8 +131,9 @@ struct TreeVjpArgs {
     WalkCounters* counters;
     int blocks;
 };
-// Nodes at the top of the tree (in level order) whose VJP cotangents each block accumulates privately.
-constexpr int kHot = 4096;
+// Nodes at the top of the tree (levels 0-3 and part of 4 in level order) whose VJP cotangents each
+// block accumulates privately.
+constexpr int kHot = 512;
 // node_count: the device node count of the tree (nodes = its capacity, the launch bound).
 void tree_vjp(hipStream_t s, const TreeVjpArgs& a, const uint32_t* levstart, const uint32_t* node_count, int nodes,
               double* cum);

@@ -39,6 +39,26 @@ __device__ __forceinline__ float wave_sum(float v) {
     return (a + b) + (c + d);
 }
 
+// Sums over the wave of four values at once: lanes 0-31 of v0 + v1 after a 32-lane swap hold v0's
+// half sums and lanes 32-63 v1's (likewise v2, v3), a 16-lane swap of the two leaves one target per
+// row, and butterflies within the row finish it. Row r (lanes 16r..16r+15) of the result holds
+// the sum of v_k with k = sum4_target(r).
+__device__ __forceinline__ float wave_sum4(float v0, float v1, float v2, float v3) {
+    const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(v0), __float_as_uint(v1), false, false);
+    const float a0 = __uint_as_float(p[0]) + __uint_as_float(p[1]);
+    const auto q = __builtin_amdgcn_permlane32_swap(__float_as_uint(v2), __float_as_uint(v3), false, false);
+    const float a1 = __uint_as_float(q[0]) + __uint_as_float(q[1]);
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a0), __float_as_uint(a1), false, false);
+    float b = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    b += dpp_f<0xB1>(b);   // quad_perm(1, 0, 3, 2)
+    b += dpp_f<0x4E>(b);   // quad_perm(2, 3, 0, 1)
+    b += dpp_f<0x141>(b);  // row_half_mirror
+    b += dpp_f<0x140>(b);  // row_mirror
+    return b;
+}
+// The value index whose sum wave_sum4 leaves in row r: {0, 2, 1, 3}[r].
+__device__ __forceinline__ int sum4_target(int row) { return ((row & 1) << 1) | (row >> 1); }
+
 __device__ __forceinline__ float readlane_f(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
