@@ -127,27 +127,29 @@ def test_tree_adjoint_single_coordinates_match_frozen_forward_fd(hip, theta, wal
 
 
 def test_tree_adjoint_agrees_with_the_direct_sum_adjoint(hip):
-    # The same encounter, loss and horizon through the tree (theta 0.6) and through the direct sum:
-    # the parameter gradients agree to the tree's force accuracy.
-    C = encounter(4096)
+    # The two-disk encounter at 2 x 16384 stars with the benchmark's settings (inclinations 30 and 60
+    # degrees, pericentre 4, theta 0.6, eps 0.02, dt 1e-3, 100 steps, checkpoints every 20) through
+    # the tree and through the direct sum: the parameter gradients agree to the tree's force accuracy.
+    C = L.Collision(nstars=16384, seed=1, pericentre=4.0, incl1=np.pi / 6, incl2=np.pi / 3)
     x0, v0, m = C.generate()
-    target = np.array([1.0, -2.0, 0.5])
+    target = np.random.default_rng(2027).standard_normal(3)
     loss = com2_loss(C, m, target)
     out = {}
     for method in ("tree", "direct"):
-        prm = {**PRM, "method": method}
-        sim = Simulation(hip, SimParams(**prm, tree_mode="build"))
+        sim = Simulation(hip, SimParams(method=method, theta=0.6, eps=0.02, dt=1e-3, steps=100,
+                                        checkpoint_every=20))
         xT, _ = sim.forward(x0, v0, m)
         _, gxT = loss(xT)
         gx0, gv0, _ = sim.backward(gxT, np.zeros_like(gxT))
         out[method] = C.vjp(gx0, gv0)
     err = np.abs(out["tree"] - out["direct"]) / np.abs(out["direct"]).max()
     print(f"tree vs direct adjoint: {out['tree']} vs {out['direct']}, error / largest component {err}")
-    # The incl1 component (~1e-5 here) comes from the tree's small violation of momentum conservation
-    # inside disk 1, absent from the direct sum; the scale is the gradient's largest component.
-    assert np.all(err < 1e-2)
+    # Components far below the largest (incl1 here, which moves only disk 1) carry the tree's small
+    # violation of momentum conservation inside disk 1, absent from the direct sum; the scale is the
+    # gradient's largest component.
+    assert np.all(err < 1e-3)
     big = np.abs(out["direct"]) > 1e-3 * np.abs(out["direct"]).max()
-    assert np.all(np.abs(out["tree"] - out["direct"])[big] / np.abs(out["direct"])[big] < 1e-2)
+    assert np.all(np.abs(out["tree"] - out["direct"])[big] / np.abs(out["direct"])[big] < 1e-3)
 
 
 def test_tree_checkpoint_interval_does_not_change_gradients(hip):
