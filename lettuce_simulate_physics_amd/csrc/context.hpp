@@ -45,6 +45,8 @@ struct Context {
     DevBuf cube, bbox_part, red_part, red_out, direct_part;
     // tree
     DevBuf bl, cnt, gflag, scan_tmp, pkey, pkey2, pval, pval2, inv, levstart, node, mom, prefix, gstart, counters;
+    DevBuf pflag, pstart;  // walk packs of sibling groups (pack_max > 0)
+    int pack_max = 0;
     long long cap_nodes = 0;
 
     // Mixed-precision form: FP64 positions and velocities (backend order) with permute alternates,
@@ -259,6 +261,8 @@ struct Context {
         cnt.reserve((N + 1) * 4);
         gflag.reserve((N + 1) * 4);
         gstart.reserve((N + 1) * 4);
+        pflag.reserve((N + 1) * 4);
+        pstart.reserve((N + 1) * 4);
         scan_tmp.reserve(scan_tmp_entries(nn) * 4);
         const std::size_t C = std::size_t(cap_nodes);
         pkey.reserve(C * 8); pkey2.reserve(C * 8);
@@ -425,9 +429,15 @@ struct Context {
         bb.mom = mom.as<NodeMom>();
         bb.prefix = prefix.as<uint64_t>();
         bb.gstart = gstart.as<uint32_t>();
+        bb.pmax = pack_max;
+        bb.pflag = pflag.as<uint32_t>();
+        bb.pstart = pstart.as<uint32_t>();
         bb.counters = counters.as<WalkCounters>();
         return bb;
     }
+    // The walk's units: the groups, or their packs. The count is at [n].
+    const uint32_t* walk_starts() const { return pack_max > 0 ? pstart.as<uint32_t>() : gstart.as<uint32_t>(); }
+    const uint32_t* walk_count() const { return pack_max > 0 ? pflag.as<uint32_t>() : gflag.as<uint32_t>(); }
 
     void compute_forces_with(const lhip_force_params& p, bool potential) {
         LHIP_CHECK(hipSetDevice(dev));
@@ -476,8 +486,8 @@ struct Context {
             a.posm = posm.as<float4>();
             a.node = node.as<WalkNode>();
             a.mom = mom.as<NodeMom>();
-            a.gstart = gstart.as<uint32_t>();
-            a.gcount = gflag.as<uint32_t>();
+            a.gstart = walk_starts();
+            a.gcount = walk_count();
             a.n = n;
             a.eps2 = eps2;
             a.G = G;

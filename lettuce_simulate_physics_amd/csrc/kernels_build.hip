@@ -408,6 +408,49 @@ __global__ void __launch_bounds__(kBS) moments_level(int level, int n, const flo
     }
 }
 
+// Walk packs: runs of consecutive sibling groups (same group level G, same level-(G-1) parent: the
+// first body of the later group shares exactly G - 1 octal digits with the body before it), each
+// run cut greedily into packs of at most pmax bodies. A pack's box lies inside the parent cell.
+// pflag[g] = 1 iff group g starts a pack (g < group count); 0 for g in [group count, n). The first
+// group of each run writes the run's flags.
+__global__ void __launch_bounds__(kBS) packs_kernel(const uint32_t* __restrict__ gstart,
+                                                    const uint32_t* __restrict__ gscan, const uint32_t* __restrict__ bl,
+                                                    int n, int pmax, uint32_t* __restrict__ pflag) {
+    const int g = blockIdx.x * kBS + threadIdx.x;
+    if (g >= n) return;
+    const int ng = int(gscan[n]);
+    if (g >= ng) {
+        pflag[g] = 0u;
+        return;
+    }
+    auto level = [&](int h) { return int((bl[gstart[h]] >> 8) & 0xff); };
+    auto continues = [&](int h) {  // group h is a sibling of group h - 1
+        if (h == 0) return false;
+        const uint32_t b = bl[gstart[h]];
+        const int G = int((b >> 8) & 0xff), c = int((b >> 16) & 0xff) - 1;
+        return G > 0 && c == G - 1 && level(h - 1) == G;
+    };
+    if (continues(g)) return;
+    int acc = 0;
+    for (int h = g; h < ng && (h == g || continues(h)); ++h) {
+        const int size = int(gstart[h + 1] - gstart[h]);
+        const bool start = h == g || acc + size > pmax;
+        pflag[h] = start ? 1u : 0u;
+        acc = start ? size : acc + size;
+    }
+}
+
+// Pack starts in body order from the scanned flags; pstart[count] = n.
+__global__ void __launch_bounds__(kBS) pstarts_kernel(const uint32_t* __restrict__ gstart,
+                                                      const uint32_t* __restrict__ gscan,
+                                                      const uint32_t* __restrict__ pscan, int n,
+                                                      uint32_t* __restrict__ pstart) {
+    const int g = blockIdx.x * kBS + threadIdx.x;
+    if (g >= n) return;
+    if (g < int(gscan[n]) && pscan[g + 1] != pscan[g]) pstart[pscan[g]] = gstart[g];
+    if (g == n - 1) pstart[pscan[n]] = uint32_t(n);
+}
+
 }  // namespace
 
 void tree_moments_keep(hipStream_t s, const BuildBufs& b) {
@@ -445,6 +488,13 @@ void tree_build(hipStream_t s, const BuildBufs& b, float inv_theta) {
     LHIP_LAUNCH_CHECK();
     groups_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(b.gflag, n, b.bl, b.gstart);
     LHIP_LAUNCH_CHECK();
+    if (b.pmax > 0) {
+        packs_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(b.gstart, b.gflag, b.bl, n, b.pmax, b.pflag);
+        LHIP_LAUNCH_CHECK();
+        scan_exclusive(s, b.pflag, n, b.scan_tmp);
+        pstarts_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(b.gstart, b.gflag, b.pflag, n, b.pstart);
+        LHIP_LAUNCH_CHECK();
+    }
     const unsigned g = std::min<unsigned>(grid_for(n, kBS), 512u);
     for (int l = kMaxLevel; l >= 0; --l) {
         if (b.momd)

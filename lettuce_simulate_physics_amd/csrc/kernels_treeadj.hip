@@ -304,7 +304,12 @@ __global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
                 __builtin_amdgcn_wave_barrier();
                 top -= P;
                 WalkNode nd{0.f, 0.f, 0.f, 0.f, 0, 0, 0, -1};
-                if (act) nd = a.node[k];
+                // qrel (the far-field decision) loads with the node.
+                float qr = 0.f;
+                if (act) {
+                    nd = a.node[k];
+                    if (a.quad_tau > 0.f) qr = a.mom[k].qrel;
+                }
                 const bool keep_open = chunked ? (nd.lo < chi && clo < nd.hi) : (nd.lo <= clo && chi <= nd.hi);
                 const float dx = fmaxf(fabsf(nd.cx - bx) - hx, 0.f);
                 const float dy = fmaxf(fabsf(nd.cy - by) - hy, 0.f);
@@ -314,7 +319,7 @@ __global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
                 const bool leaf = act && info_leaf(nd.info);
                 const bool small = leaf && (nd.hi - nd.lo) <= a.small_leaf;
                 bool is_cell = acc && !small;
-                const bool is_mono = is_cell && a.quad_tau > 0.f && a.mom[k].qrel < a.quad_tau * d2;
+                const bool is_mono = is_cell && a.quad_tau > 0.f && qr < a.quad_tau * d2;
                 is_cell = is_cell && !is_mono;
                 const bool is_range = leaf && (!acc || small);
                 const bool open_int = act && !acc && !leaf;
@@ -352,7 +357,7 @@ __global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
                         const int slot = nb + int(lanes_below(mm));
                         const NodeMomD& dm = a.momd[k];
                         L.bP[slot] = make_float4(float(dm.cx - ccx), float(dm.cy - ccy), float(dm.cz - ccz),
-                                                 a.mom[k].m);
+                                                 float(dm.m));
                         L.bI[slot] = -(k + 2);
                     }
                     nb += nmnew;

@@ -271,7 +271,19 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                 __builtin_amdgcn_wave_barrier();  // the popped entries are read before pushes overwrite them
                 top -= P;
                 WalkNode nd{0.f, 0.f, 0.f, 0.f, 0, 0, 0, -1};
-                if (act) nd = a.node[k];
+                // Mixed form: the FP64 com and mass (what a monopole or a cell stages) and qrel (the
+                // far-field decision) load together with the node, so that deciding and staging
+                // wait for one round trip.
+                double4 pc = make_double4(0.0, 0.0, 0.0, 0.0);
+                float qr = 0.f;
+                if (act) {
+                    nd = a.node[k];
+                    if (REL) {
+                        const NodeMomD& dm = a.momd[k];
+                        pc = make_double4(dm.cx, dm.cy, dm.cz, dm.m);
+                        if (a.quad_tau > 0.f) qr = a.mom[k].qrel;
+                    }
+                }
                 // A node holding bodies of this (sub)group is always opened: the reference's
                 // "contains" test for whole groups; any overlap for chunks of an over-full group.
                 const bool keep_open = chunked ? (nd.lo < chi && clo < nd.hi) : (nd.lo <= clo && chi <= nd.hi);
@@ -286,7 +298,7 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                 bool is_cell = acc && !small;
                 // Error-budgeted far field: an accepted cell whose quadrupole term is below quad_tau
                 // of its monopole at the group box's distance enters the body list as a point mass.
-                const bool is_mono = is_cell && a.quad_tau > 0.f && a.mom[k].qrel < a.quad_tau * d2;
+                const bool is_mono = is_cell && a.quad_tau > 0.f && (REL ? qr : a.mom[k].qrel) < a.quad_tau * d2;
                 is_cell = is_cell && !is_mono;
                 const bool is_range = leaf && (!acc || small);
                 const bool open_int = act && !acc && !leaf;
@@ -310,8 +322,8 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                         const int slot = nc + int(lanes_below(mc));
                         const NodeMom& md = a.mom[k];
                         if (REL) {
-                            const NodeMomD& dm = a.momd[k];
-                            L.c[slot].A = make_float4(float(dm.cx - ccx), float(dm.cy - ccy), float(dm.cz - ccz), md.m);
+                            L.c[slot].A = make_float4(float(pc.x - ccx), float(pc.y - ccy), float(pc.z - ccz),
+                                                      float(pc.w));
                         } else {
                             L.c[slot].A = make_float4(md.cx, md.cy, md.cz, md.m);
                         }
@@ -326,11 +338,11 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                     if (nb + nmnew > kBodyCap) flush_bodies();
                     if (is_mono) {
                         const int slot = nb + int(lanes_below(mm));
-                        const NodeMom& md = a.mom[k];
                         if (REL) {
-                            const NodeMomD& dm = a.momd[k];
-                            L.bP[slot] = make_float4(float(dm.cx - ccx), float(dm.cy - ccy), float(dm.cz - ccz), md.m);
+                            L.bP[slot] = make_float4(float(pc.x - ccx), float(pc.y - ccy), float(pc.z - ccz),
+                                                     float(pc.w));
                         } else {
+                            const NodeMom& md = a.mom[k];
                             L.bP[slot] = make_float4(md.cx, md.cy, md.cz, md.m);
                         }
                         L.bI[slot] = -1;

@@ -62,6 +62,10 @@ struct BuildBufs {
     NodeMom* mom;              // cap
     uint64_t* prefix;          // cap
     uint32_t* gstart;          // n + 1
+    // walk packs (pmax > 0): sibling groups packed up to pmax bodies
+    int pmax;
+    uint32_t* pflag;           // n + 1: pack starts per group, then their exclusive scan (count at [n])
+    uint32_t* pstart;          // n + 1
     WalkCounters* counters;
 };
 

@@ -395,8 +395,8 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
         a.momd = c.momd.as<NodeMomD>();
         a.posd = c.posd.as<double4>();
         a.ux = u; a.uy = u + N; a.uz = u + 2 * N;
-        a.gstart = c.gstart.as<uint32_t>();
-        a.gcount = c.gflag.as<uint32_t>();
+        a.gstart = c.walk_starts();
+        a.gcount = c.walk_count();
         a.n = n;
         a.eps2 = float(S.p.eps * S.p.eps);
         a.small_leaf = c.wopt.small_leaf;
