@@ -210,9 +210,12 @@ int lhip_accelerations(lhip_ctx* ctx, float* ax, float* ay, float* az, float* po
  * FP64 direct oracle for force errors at sizes a CPU sum cannot reach. Any output may be NULL. */
 int lhip_direct_f64_targets(lhip_ctx* ctx, uint64_t nt, const uint32_t* targets, double G, double eps, double* ax,
                             double* ay, double* az, double* pot);
-/* Sorted Morton keys of the current tree (n entries). */
+/* Morton keys of the current tree in body order (n entries); a body of a restarted cell has the key
+ * of its deepest frame, as the reference's key array. */
 int lhip_tree_keys(lhip_ctx* ctx, uint64_t* keys);
-/* Node count, walk-group count and the number of bodies in over-full finest cells. */
+/* Node count, walk-group count and the number of bodies in over-full finest cells: cells whose
+ * bodies coincide (exact leaves, as the reference keeps them), or restarted beyond the fourth key
+ * frame. Over-full cells whose bodies differ restart their keys (src/cpu_backend.cpp:501-544). */
 int lhip_tree_counts(lhip_ctx* ctx, uint64_t* nodes, uint64_t* groups, uint64_t* overfull);
 /* Node records in level order (children of a node contiguous): link = (lo, hi, info, first child)
  * per node with info = level | leaf << 5 | children << 8 and first = -1 for a leaf; prefix = key

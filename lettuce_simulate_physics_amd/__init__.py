@@ -444,7 +444,7 @@ class HipBackend:
         prefix = np.zeros(M, np.uint64)
         mom = np.zeros((M, 16), np.float32)
         self._check(self._L.lhip_tree_nodes(self._h, _ptr(link), _ptr(prefix), _ptr(mom)))
-        return dict(lo=link[:, 0], hi=link[:, 1], level=link[:, 2] & 31, leaf=(link[:, 2] >> 5) & 1,
+        return dict(lo=link[:, 0], hi=link[:, 1], level=link[:, 2] & 127, leaf=(link[:, 2] >> 7) & 1,
                     nchild=(link[:, 2] >> 8) & 15, first=link[:, 3], prefix=prefix, com=mom[:, 0:3],
                     m=mom[:, 3], q=mom[:, 4:10], rcrit2=mom[:, 10], geo=mom[:, 12:16])
 

@@ -1,4 +1,6 @@
 // C ABI of include/lettuce_hip.h over lhip::Context (context.hpp).
+#include <vector>
+
 #include "context.hpp"
 
 // ---------------------------------------------------------------- C ABI
@@ -179,8 +181,22 @@ int lhip_download_f64(lhip_ctx* ctx, double* x, double* y, double* z, double* vx
 int lhip_tree_keys(lhip_ctx* ctx, uint64_t* keys) {
     return guarded(ctx, [&](lhip::Context& c) {
         if (!c.tree_ok) throw lhip::Error(LHIP_LOGIC, "no current tree");
-        LHIP_CHECK(hipMemcpyAsync(keys, c.key.as<void>(), std::size_t(c.n) * 8, hipMemcpyDeviceToHost, c.stream));
+        const std::size_t n = std::size_t(c.n);
+        LHIP_CHECK(hipMemcpyAsync(keys, c.key.as<void>(), n * 8, hipMemcpyDeviceToHost, c.stream));
         c.sync();
+        if (c.frames > 1) {
+            // A body of a restarted cell carries the key of its deepest frame, as the reference's
+            // key array does after its restarts (src/cpu_backend.cpp:501-529).
+            std::vector<uint64_t> fk(std::size_t(c.frames - 1) * n);
+            std::vector<uint32_t> cap(n);
+            LHIP_CHECK(hipMemcpyAsync(fk.data(), c.fkey.as<void>(), fk.size() * 8, hipMemcpyDeviceToHost, c.stream));
+            LHIP_CHECK(hipMemcpyAsync(cap.data(), c.capb.as<void>(), n * 4, hipMemcpyDeviceToHost, c.stream));
+            c.sync();
+            for (std::size_t i = 0; i < n; ++i) {
+                const int r = int(cap[i]) / lhip::kMaxLevel - 1;  // restarts of body i
+                if (r > 0) keys[i] = fk[std::size_t(r - 1) * n + i];
+            }
+        }
     });
 }
 

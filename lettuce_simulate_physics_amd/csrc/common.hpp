@@ -80,6 +80,12 @@ constexpr int kGroupMax = 128;      // walk group size (reference src/cpu_backen
 constexpr int kMaxLevel = 21;       // Morton bits per axis (reference src/cpu_backend.cpp:28)
 constexpr int kSmallLeaf = 4;       // accepted leaves this small are summed directly (src/cpu_backend.cpp:29)
 constexpr int kLevels = kMaxLevel + 1;
+// Key restarts (reference src/cpu_backend.cpp:501-544): an over-full level-21 cell whose bodies do
+// not coincide restarts the keys in its own bounding cube, one more 21-level key frame. At most
+// kMaxFrames frames (tree depth kMaxDepth); deeper over-full cells stay leaves and are counted.
+constexpr int kMaxFrames = 4;
+constexpr int kMaxDepth = kMaxLevel * kMaxFrames;
+constexpr int kDepthLevels = kMaxDepth + 1;
 
 inline unsigned grid_for(long long n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
 
@@ -90,9 +96,9 @@ struct Cube {
     float lo[3], hi[3];
 };
 
-// Node info word: level in bits 0-4, leaf flag in bit 5, child count in bits 8-11.
-constexpr int32_t kInfoLeaf = 1 << 5;
-__host__ __device__ inline int info_level(int32_t info) { return info & 31; }
+// Node info word: level (depth) in bits 0-6, leaf flag in bit 7, child count in bits 8-11.
+constexpr int32_t kInfoLeaf = 1 << 7;
+__host__ __device__ inline int info_level(int32_t info) { return info & 127; }
 __host__ __device__ inline bool info_leaf(int32_t info) { return (info & kInfoLeaf) != 0; }
 __host__ __device__ inline int info_nchild(int32_t info) { return (info >> 8) & 15; }
 

@@ -48,6 +48,11 @@ struct Context {
     DevBuf pflag, pstart;  // walk packs of sibling groups (pack_max > 0)
     int pack_max = 0;
     long long cap_nodes = 0;
+    // Key restarts: key frames of the current tree (1: none), frame keys, depth caps, restarted
+    // cells and their cubes; the pinned flag the level pass sets; restart scratch.
+    int frames = 1;
+    int* host_over = nullptr;
+    DevBuf fkey, fseg, capb, fcube, rs_seg, rs_bbox, rs_cstart, rs_gkey, rs_ka, rs_kb, rs_va, rs_vb;
 
     // Mixed-precision form: FP64 positions and velocities (backend order) with permute alternates,
     // FP64 accelerations and potential, FP64 node moments and direct-sum records. Masses and ids
@@ -84,6 +89,8 @@ struct Context {
         red_out.reserve(sizeof(double) * 16);
         counters.reserve(sizeof(WalkCounters));
         walk_blocks = cus * walk_blocks_per_cu();
+        LHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_over), sizeof(int), hipHostMallocDefault));
+        *host_over = 0;
     }
     ~Context() {
         adj_destroy(adj);
@@ -94,6 +101,7 @@ struct Context {
         }
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        if (host_over) (void)hipHostFree(host_over);
     }
 
     void reserve_bodies(int nn) {
@@ -267,10 +275,24 @@ struct Context {
         const std::size_t C = std::size_t(cap_nodes);
         pkey.reserve(C * 8); pkey2.reserve(C * 8);
         pval.reserve(C * 4); pval2.reserve(C * 4); inv.reserve(C * 4);
-        levstart.reserve((kLevels + 2) * 4);
+        levstart.reserve((kDepthLevels + 2) * 4);
         node.reserve(C * sizeof(WalkNode));
         mom.reserve(C * sizeof(NodeMom));
         prefix.reserve(C * 8);
+    }
+
+    // Node arrays for a tree of M nodes (key restarts can exceed the single-frame bound).
+    void grow_nodes(long long M) {
+        if (M <= cap_nodes) return;
+        cap_nodes = M;
+        const std::size_t C = std::size_t(cap_nodes);
+        tile_hist.reserve(radix_tile_hist_entries(int(std::max<long long>(n, cap_nodes))) * 4);
+        pkey.reserve(C * 8); pkey2.reserve(C * 8);
+        pval.reserve(C * 4); pval2.reserve(C * 4); inv.reserve(C * 4);
+        node.reserve(C * sizeof(WalkNode));
+        mom.reserve(C * sizeof(NodeMom));
+        prefix.reserve(C * 8);
+        if (mixed) momd.reserve(C * sizeof(NodeMomD));
     }
 
     float ms(int a, int b) const {
@@ -363,6 +385,28 @@ struct Context {
         has_pe = false;
         acc_ok = false;
         LHIP_CHECK(hipEventRecord(ev[0], stream));
+        TreeBufs tb = tree_bufs();
+        tree_sort(stream, tb, n, keep_cost);
+        swap_permuted(keep_cost);
+        LHIP_CHECK(hipEventRecord(ev[1], stream));
+        frames = 1;
+        BuildBufs bb = build_bufs();
+        // inv_theta as the reference CPU forms it (src/cpu_backend.cpp:595).
+        const float inv_theta = p.theta > 0 ? float(1.0 / p.theta) : FLT_MAX;
+        // The level pass flags over-full finest cells; the host reads the flag while the rest of the
+        // build runs, and only then takes the key-restart path (a second build).
+        *reinterpret_cast<volatile int*>(host_over) = 0;
+        tree_build(stream, bb, inv_theta, ev[9]);
+        LHIP_CHECK(hipEventSynchronize(ev[9]));
+        if (*reinterpret_cast<volatile int*>(host_over)) {
+            restart_keys(nullptr, nullptr, nullptr, true);
+            rebuild(inv_theta, nullptr);
+        }
+        LHIP_CHECK(hipEventRecord(ev[2], stream));
+        tree_ok = true;
+    }
+
+    TreeBufs tree_bufs() {
         TreeBufs tb{};
         tb.f64 = mixed;
         if (mixed) {
@@ -384,22 +428,133 @@ struct Context {
         tb.tile_hist = tile_hist.as<uint32_t>();
         tb.cube = cube.as<Cube>();
         tb.bbox_part = bbox_part.as<void>();
-        tree_sort(stream, tb, n, keep_cost);
-        // The permute wrote every body array into its alternate: swap the owners.
+        return tb;
+    }
+
+    // The permute wrote every body array into its alternate: swap the owners.
+    void swap_permuted(bool with_cost) {
         if (mixed) {
             swap(xd, xd2); swap(yd, yd2); swap(zd, zd2); swap(vxd, vxd2); swap(vyd, vyd2); swap(vzd, vzd2);
         } else {
             swap(x, x2); swap(y, y2); swap(z, z2); swap(vx, vx2); swap(vy, vy2); swap(vz, vz2);
         }
         swap(m, m2); swap(id, id2);
-        if (keep_cost) swap(cost, cost2);
-        LHIP_CHECK(hipEventRecord(ev[1], stream));
-        BuildBufs bb = build_bufs();
-        // inv_theta as the reference CPU forms it (src/cpu_backend.cpp:595).
-        const float inv_theta = p.theta > 0 ? float(1.0 / p.theta) : FLT_MAX;
-        tree_build(stream, bb, inv_theta);
-        LHIP_CHECK(hipEventRecord(ev[2], stream));
-        tree_ok = true;
+        if (with_cost) swap(cost, cost2);
+    }
+
+    // Key restarts (src/cpu_backend.cpp:501-544), the rare path after a level pass that found an
+    // over-full finest cell. Frame by frame: the over-full cells of the deepest frame whose bodies do
+    // not coincide get keys in their own cubes, and (permute) the bodies are re-sorted stably by
+    // (cell, new key). kx, ky, kz: the key positions in the current order (null: the state's own);
+    // without permute the order must already be the restarted one (a replayed topology). Leaves
+    // bl, cnt, gflag of the final frames; the caller rebuilds the tree (rebuild).
+    void restart_keys(const void* kx, const void* ky, const void* kz, bool permute) {
+        sync();
+        const std::size_t N = std::size_t(n);
+        const std::size_t maxcells = N / (kLeafMax + 1) + 1;
+        fkey.reserve((kMaxFrames - 1) * N * 8);
+        fseg.reserve((kMaxFrames - 1) * N * 4);
+        capb.reserve(N * 4);
+        fcube.reserve((kMaxFrames - 1) * maxcells * sizeof(double4));
+        rs_seg.reserve((N + 1) * 4);
+        rs_bbox.reserve(6 * maxcells * 8);
+        rs_cstart.reserve(maxcells * 4);
+        for (DevBuf* b : {&rs_gkey, &rs_ka, &rs_kb}) b->reserve(N * 8);
+        for (DevBuf* b : {&rs_va, &rs_vb}) b->reserve(N * 4);
+        fill_u32(stream, capb.as<uint32_t>(), kMaxLevel, n);
+        frames = 1;
+        auto bufs = [&]() {
+            BuildBufs b = build_bufs();
+            b.frames = frames;
+            b.cap = capb.as<uint32_t>();
+            b.fkey = fkey.as<uint64_t>();
+            b.fseg = fseg.as<uint32_t>();
+            b.fcube = fcube.as<double4>();
+            b.host_over = nullptr;
+            return b;
+        };
+        auto pos = [&](int c) -> const void* {
+            if (c == 0 && kx) return kx;
+            if (c == 1 && ky) return ky;
+            if (c == 2 && kz) return kz;
+            if (mixed) return c == 0 ? D(xd) : c == 1 ? D(yd) : D(zd);
+            return c == 0 ? F(x) : c == 1 ? F(y) : F(z);
+        };
+        BuildBufs b = bufs();
+        tree_levels(stream, b);
+        int cubes_used = 0;
+        for (int f = 1; f < kMaxFrames; ++f) {
+            const int nc = tree_restart_cells(stream, b, rs_seg.as<uint32_t>());
+            if (nc == 0) break;
+            double4* cubes = fcube.as<double4>() + cubes_used;
+            tree_restart_cubes(stream, b, pos(0), pos(1), pos(2), mixed, rs_seg.as<uint32_t>(), nc,
+                               rs_bbox.as<unsigned long long>(), cubes);
+            std::vector<double4> hc(nc);
+            LHIP_CHECK(hipMemcpyAsync(hc.data(), cubes, std::size_t(nc) * sizeof(double4), hipMemcpyDeviceToHost, stream));
+            sync();
+            bool any = false;
+            for (const double4& q : hc) any = any || q.w > 0.0;
+            if (!any) break;  // every over-full cell's bodies coincide: they stay leaves
+            uint64_t* fk = fkey.as<uint64_t>() + std::size_t(f - 1) * N;
+            uint32_t* fs = fseg.as<uint32_t>() + std::size_t(f - 1) * N;
+            tree_restart_keys(stream, b, pos(0), pos(1), pos(2), mixed, rs_seg.as<uint32_t>(), cubes, cubes_used, f,
+                              fk, fs, capb.as<uint32_t>(), rs_gkey.as<uint64_t>(), rs_cstart.as<uint32_t>());
+            cubes_used += nc;
+            frames = f + 1;
+            if (permute) {
+                // Stable sort by the new key, then stably by the grouping key (the cell's first body,
+                // or the body itself outside restarted cells): cells stay in place, sorted inside.
+                uint64_t *ka = rs_ka.as<uint64_t>(), *kb = rs_kb.as<uint64_t>();
+                uint32_t *va = rs_va.as<uint32_t>(), *vb = rs_vb.as<uint32_t>();
+                iota_u32(stream, va, n);
+                LHIP_CHECK(hipMemcpyAsync(ka, fk, N * 8, hipMemcpyDeviceToDevice, stream));
+                for (int shift = 0; shift < 63; shift += 8) {
+                    radix_pass(stream, ka, va, kb, vb, n, nullptr, shift, tile_hist.as<uint32_t>());
+                    std::swap(ka, kb);
+                    std::swap(va, vb);
+                }
+                gather_u64(stream, va, rs_gkey.as<uint64_t>(), ka, n);
+                int gbits = 0;
+                while (gbits < 32 && (1ll << gbits) < (long long)n) gbits += 8;
+                for (int shift = 0; shift < gbits; shift += 8) {
+                    radix_pass(stream, ka, va, kb, vb, n, nullptr, shift, tile_hist.as<uint32_t>());
+                    std::swap(ka, kb);
+                    std::swap(va, vb);
+                }
+                const uint32_t* perm = va;
+                TreeBufs tb = tree_bufs();
+                tree_permute(stream, tb, perm, n, false);
+                swap_permuted(false);
+                gather_u64(stream, perm, key.as<uint64_t>(), key2.as<uint64_t>(), n);
+                swap(key, key2);
+                uint64_t* k2 = kb;  // free scratch of N keys
+                for (int g = 1; g <= f; ++g) {
+                    uint64_t* fg = fkey.as<uint64_t>() + std::size_t(g - 1) * N;
+                    gather_u64(stream, perm, fg, k2, n);
+                    LHIP_CHECK(hipMemcpyAsync(fg, k2, N * 8, hipMemcpyDeviceToDevice, stream));
+                    uint32_t* sg = fseg.as<uint32_t>() + std::size_t(g - 1) * N;
+                    uint32_t* t2 = perm == rs_va.as<uint32_t>() ? rs_vb.as<uint32_t>() : rs_va.as<uint32_t>();
+                    gather_u32(stream, perm, sg, t2, n);
+                    LHIP_CHECK(hipMemcpyAsync(sg, t2, N * 4, hipMemcpyDeviceToDevice, stream));
+                }
+                uint32_t* t2 = perm == rs_va.as<uint32_t>() ? rs_vb.as<uint32_t>() : rs_va.as<uint32_t>();
+                gather_u32(stream, perm, capb.as<uint32_t>(), t2, n);
+                LHIP_CHECK(hipMemcpyAsync(capb.as<uint32_t>(), t2, N * 4, hipMemcpyDeviceToDevice, stream));
+            }
+            b = bufs();
+            tree_levels(stream, b);
+        }
+    }
+
+    // The build after restart_keys: node arrays sized for the tree, then the whole build over the
+    // key frames. box (optional) replaces the positions the moments read (a replayed topology).
+    void rebuild(float inv_theta, const double* const* box) {
+        BuildBufs b = build_bufs();
+        if (frames > 1) grow_nodes(tree_node_count(stream, b) + 1);
+        b = build_bufs();
+        b.host_over = nullptr;
+        if (box) { b.xd = box[0]; b.yd = box[1]; b.zd = box[2]; }
+        tree_build(stream, b, inv_theta);
     }
 
     BuildBufs build_bufs() {
@@ -433,6 +588,14 @@ struct Context {
         bb.pflag = pflag.as<uint32_t>();
         bb.pstart = pstart.as<uint32_t>();
         bb.counters = counters.as<WalkCounters>();
+        bb.frames = frames;
+        if (frames > 1) {
+            bb.fkey = fkey.as<uint64_t>();
+            bb.cap = capb.as<uint32_t>();
+            bb.fseg = fseg.as<uint32_t>();
+            bb.fcube = fcube.as<double4>();
+        }
+        bb.host_over = host_over;
         return bb;
     }
     // The walk's units: the groups, or their packs. The count is at [n].
@@ -504,6 +667,7 @@ struct Context {
             a.blocks = walk_blocks;
             a.small_leaf = wopt.small_leaf;
             a.quad_tau = float(wopt.quad_tau);
+            a.depth = tree_depth(frames);
             tree_walk(stream, a, potential);
             LHIP_CHECK(hipEventRecord(ev[4], stream));
             LHIP_CHECK(hipMemcpyAsync(&hc, counters.as<void>(), sizeof hc, hipMemcpyDeviceToHost, stream));

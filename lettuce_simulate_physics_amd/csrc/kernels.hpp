@@ -76,6 +76,8 @@ struct TreeBufs {
 // Morton keys, stable radix sort, permute of every body array into key order (swaps the caller's
 // pointers: after the call b.x etc. hold the sorted arrays). Keys stay in b.key.
 void tree_sort(hipStream_t s, TreeBufs& b, int n, bool permute_cost);
+// Every body array into the order perm gives (new i <- old perm[i]); swaps like tree_sort.
+void tree_permute(hipStream_t s, TreeBufs& b, const uint32_t* perm, int n, bool permute_cost);
 // Keys of FP64 positions already in key order (a recorded sorted state): bounding cube and keys,
 // no sort (idx = identity).
 void tree_keys_sorted(hipStream_t s, const double* x, const double* y, const double* z, int n, void* bbox_part,

@@ -7,6 +7,7 @@
 #include <cfloat>
 
 #include "kernels.hpp"
+#include "morton.hpp"
 #include "tree.hpp"
 
 namespace lhip {
@@ -15,6 +16,23 @@ namespace {
 // Number of leading octal digits two 63-bit keys share (21 when equal).
 __device__ __forceinline__ int common_octs(uint64_t a, uint64_t b) {
     return a == b ? 21 : (__clzll(static_cast<long long>(a ^ b)) - 1) / 3;
+}
+
+// Key frames of a build: frame 0 keys, frame f >= 1 keys at fkey[(f - 1) n + i].
+struct Frames {
+    const uint64_t* key;
+    const uint64_t* fkey;
+    int frames;
+    int n;
+};
+
+// Leading octal digits sorted bodies a and b share over all key frames: a later frame counts only
+// where the earlier ones agree entirely (both bodies then lie in the same restarted cell).
+__device__ __forceinline__ int ext_common(const Frames& F, int a, int b) {
+    int c = common_octs(F.key[a], F.key[b]);
+    for (int f = 1; f < F.frames && c == kMaxLevel * f; ++f)
+        c = kMaxLevel * f + common_octs(F.fkey[std::size_t(f - 1) * F.n + a], F.fkey[std::size_t(f - 1) * F.n + b]);
+    return c;
 }
 
 // ---------------------------------------------------------------- state reset
@@ -36,7 +54,8 @@ constexpr int kHalo = kGroupMax;  // 128
 // gflag[i] = 1 iff the group cell of body i starts at i.
 __global__ void __launch_bounds__(kLvTile) levels_kernel(const uint64_t* __restrict__ key, int n, int gmax,
                                                          uint32_t* __restrict__ bl, uint32_t* __restrict__ cnt,
-                                                         uint32_t* __restrict__ gflag, WalkCounters* __restrict__ ctr) {
+                                                         uint32_t* __restrict__ gflag, WalkCounters* __restrict__ ctr,
+                                                         int* __restrict__ host_over) {
     __shared__ uint64_t sk[kHalo + kLvTile + kHalo];
     __shared__ uint8_t d16[kHalo + kLvTile];
     __shared__ uint8_t d128[kHalo + kLvTile];
@@ -76,7 +95,41 @@ __global__ void __launch_bounds__(kLvTile) levels_kernel(const uint64_t* __restr
         over = L > kMaxLevel;
     }
     const unsigned long long ov = __ballot(over);
-    if ((tid & 63) == 0 && ov) atomicAdd(&ctr->overfull, __popcll(ov));
+    if ((tid & 63) == 0 && ov) {
+        atomicAdd(&ctr->overfull, __popcll(ov));
+        if (host_over) __hip_atomic_store(host_over, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+// The level pass over key frames (the restart path): the same windows on the extended keys, each
+// body's levels capped at its deepest frame (cap[i]). Bodies whose deepest cell still holds more
+// than 16 bodies are counted (coincident bodies, or restarts beyond the last frame).
+__global__ void __launch_bounds__(kBS) levels_ext_kernel(Frames F, const uint32_t* __restrict__ cap, int gmax,
+                                                         uint32_t* __restrict__ bl, uint32_t* __restrict__ cnt,
+                                                         uint32_t* __restrict__ gflag, WalkCounters* __restrict__ ctr,
+                                                         int* __restrict__ host_over) {
+    const int i = blockIdx.x * kBS + threadIdx.x;
+    bool over = false;
+    if (i < F.n) {
+        const int n = F.n;
+        int best = -1;
+        for (int j = max(0, i - kLeafMax); j <= i && j + kLeafMax < n; ++j) best = max(best, ext_common(F, j, j + kLeafMax));
+        const int L = best + 1;
+        best = -1;
+        for (int j = max(0, i - gmax); j <= i && j + gmax < n; ++j) best = max(best, ext_common(F, j, j + gmax));
+        const int Lc = min(L, int(cap[i]));
+        const int Gc = min(best + 1, Lc);
+        const int c = i == 0 ? -1 : ext_common(F, i - 1, i);
+        bl[i] = uint32_t(L) | uint32_t(Gc) << 8 | uint32_t(c + 1) << 16;
+        cnt[i] = uint32_t(max(0, Lc - c));
+        gflag[i] = c < Gc ? 1u : 0u;
+        over = L > int(cap[i]);
+    }
+    const unsigned long long ov = __ballot(over);
+    if ((threadIdx.x & 63) == 0 && ov) {
+        atomicAdd(&ctr->overfull, __popcll(ov));
+        if (host_over) __hip_atomic_store(host_over, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 // ---------------------------------------------------------------- exclusive scan (u32, in place)
@@ -167,12 +220,12 @@ void scan_exclusive(hipStream_t s, uint32_t* a, int n, uint32_t* tmp) {
 // level yields the level order (within a level, pre-order = first-body order).
 __global__ void __launch_bounds__(kBS) enumerate_kernel(int n, const uint32_t* __restrict__ bl,
                                                         const uint32_t* __restrict__ scan, uint64_t* __restrict__ pkey,
-                                                        uint32_t* __restrict__ pval) {
+                                                        uint32_t* __restrict__ pval, const uint32_t* __restrict__ cap) {
     const int i = blockIdx.x * kBS + threadIdx.x;
     if (i >= n) return;
     const uint32_t b = bl[i];
     const int L = int(b & 0xff), c = int((b >> 16) & 0xff) - 1;
-    const int Lc = min(L, kMaxLevel);
+    const int Lc = min(L, cap ? int(cap[i]) : kMaxLevel);
     const uint32_t first = scan[i];
     for (int l = c + 1; l <= Lc; ++l) {
         const uint32_t k = first + uint32_t(l - c - 1);
@@ -190,35 +243,51 @@ __global__ void __launch_bounds__(kBS) invert_kernel(const uint32_t* __restrict_
     if (p < M) inv[perm[p]] = uint32_t(p);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         uint32_t s = 0;
-        for (int l = 0; l < kLevels; ++l) {
+        for (int l = 0; l < kDepthLevels; ++l) {
             levstart[l] = s;
             s += level_tot[l];
         }
-        levstart[kLevels] = s;
-        levstart[kLevels + 1] = s;
+        levstart[kDepthLevels] = s;
+        levstart[kDepthLevels + 1] = s;
     }
 }
 
 // Level-order node records: body range start, level, leaf flag, first child, key prefix and the
 // geometric cell. The centre follows the reference's halving in double (src/cpu_backend.cpp:576-581):
 // child centre = parent centre +- half/2 per axis bit, from the root centre at origin + side/2.
+// EXT (key frames): a restarted level-21 cell takes its own cube's centre and half side, and its
+// descendants halve from there by the next frame's key digits (src/cpu_backend.cpp:541-545).
+template <bool EXT>
 __global__ void __launch_bounds__(kBS) nodes_kernel(const uint64_t* __restrict__ key, int n,
                                                     const uint32_t* __restrict__ bl, const uint32_t* __restrict__ scan,
                                                     const uint32_t* __restrict__ inv, const Cube* __restrict__ cube,
                                                     WalkNode* __restrict__ node, NodeMom* __restrict__ mom,
-                                                    uint64_t* __restrict__ prefix) {
+                                                    uint64_t* __restrict__ prefix, Frames F,
+                                                    const uint32_t* __restrict__ cap, const uint32_t* __restrict__ fseg,
+                                                    const double4* __restrict__ fcube) {
     const int i = blockIdx.x * kBS + threadIdx.x;
     if (i >= n) return;
     const uint32_t b = bl[i];
     const int L = int(b & 0xff), c = int((b >> 16) & 0xff) - 1;
-    const int Lc = min(L, kMaxLevel);
+    const int top = EXT ? int(cap[i]) : kMaxLevel;
+    const int Lc = min(L, top);
     if (Lc <= c) return;
-    const uint64_t ki = key[i];
+    uint64_t kw = key[i];  // key word of the current frame
     const uint32_t first = scan[i];
     const Cube cb = *cube;
     double half = 0.5 * cb.side;
     double gx = cb.ox + half, gy = cb.oy + half, gz = cb.oz + half;
     for (int l = 0; l <= Lc; ++l) {
+        // the prefix within the frame (a frame's level-21 cell carries that frame's whole key)
+        const int jl = l - kMaxLevel * ((l - 1) / kMaxLevel);
+        const uint64_t pre = l == 0 ? 0ull : (kw >> (63 - 3 * jl));
+        if (EXT && l > 0 && l % kMaxLevel == 0 && top > l) {
+            const int f = l / kMaxLevel;
+            const double4 q = fcube[fseg[std::size_t(f - 1) * n + i]];
+            half = 0.5 * q.w;
+            gx = q.x + half; gy = q.y + half; gz = q.z + half;
+            kw = F.fkey[std::size_t(f - 1) * n + i];
+        }
         if (l > c) {
             const uint32_t k = first + uint32_t(l - c - 1);
             const int p = int(inv[k]);
@@ -231,7 +300,7 @@ __global__ void __launch_bounds__(kBS) nodes_kernel(const uint64_t* __restrict__
             // The first child is the next pre-order node (same first body, one level deeper).
             nd.first = l == Lc ? -1 : int(inv[k + 1]);
             node[p] = nd;
-            prefix[p] = l == 0 ? 0ull : (ki >> (63 - 3 * l));
+            prefix[p] = pre;
             NodeMom& md = mom[p];
             md.gx = float(gx);
             md.gy = float(gy);
@@ -239,7 +308,8 @@ __global__ void __launch_bounds__(kBS) nodes_kernel(const uint64_t* __restrict__
             md.half = float(half);
         }
         if (l < Lc) {
-            const int oct = int((ki >> (3 * (kMaxLevel - 1 - l))) & 7);
+            const int j = l % kMaxLevel;
+            const int oct = int((kw >> (3 * (kMaxLevel - 1 - j))) & 7);
             const double q = 0.5 * half;
             gx = gx + ((oct & 4) ? q : -q);
             gy = gy + ((oct & 2) ? q : -q);
@@ -285,7 +355,7 @@ __device__ __forceinline__ float rcrit2_of(float cx, float cy, float cz, float g
 // KEEP: a second pass on other positions over a built tree (frozen-topology replay): node records,
 // rcrit2 and qrel (the walk's decisions) stay as the first pass wrote them; com, mass and
 // quadrupole are recomputed.
-template <bool MIXED, bool KEEP = false>
+template <bool MIXED, bool KEEP = false, bool EXT = false>
 __global__ void __launch_bounds__(kBS) moments_level(int level, int n, const float4* __restrict__ posm,
                                                      const double* __restrict__ xd, const double* __restrict__ yd,
                                                      const double* __restrict__ zd, const float* __restrict__ bm,
@@ -293,7 +363,7 @@ __global__ void __launch_bounds__(kBS) moments_level(int level, int n, const flo
                                                      const uint32_t* __restrict__ levstart,
                                                      const uint64_t* __restrict__ prefix, WalkNode* __restrict__ node,
                                                      NodeMom* __restrict__ mom, NodeMomD* __restrict__ momd,
-                                                     float inv_theta) {
+                                                     float inv_theta, Frames F) {
     const int p0 = int(levstart[level]), p1 = int(levstart[level + 1]);
     const int cend = int(levstart[level + 2]);  // end of the next level
     for (int p = p0 + blockIdx.x * kBS + threadIdx.x; p < p1; p += gridDim.x * kBS) {
@@ -342,7 +412,12 @@ __global__ void __launch_bounds__(kBS) moments_level(int level, int n, const flo
         } else {
             const uint64_t pre = prefix[p];
             int ch = nd.first;
-            while (ch < cend && (prefix[ch] >> 3) == pre) {
+            // Children: the run of next-level nodes from `first` inside this cell (key prefix; over
+            // key frames, the first bodies' common extended prefix).
+            auto child = [&](int q) {
+                return EXT ? ext_common(F, nd.lo, node[q].lo) >= level : (prefix[q] >> 3) == pre;
+            };
+            while (ch < cend && child(ch)) {
                 double m, x, y, z;
                 if (MIXED) {
                     const NodeMomD& c = momd[ch];
@@ -451,13 +526,229 @@ __global__ void __launch_bounds__(kBS) pstarts_kernel(const uint32_t* __restrict
     if (g == n - 1) pstart[pscan[n]] = uint32_t(n);
 }
 
+// ---------------------------------------------------------------- key restarts (the rare path)
+// src/cpu_backend.cpp:501-544: an over-full level-21 cell (of the deepest key frame so far) whose
+// bodies do not all coincide gets keys in its own bounding cube (float minimum, side = largest
+// extent x (1 + 1e-6)); its bodies are re-sorted stably by them and the octree continues below it
+// in that frame.
+
+__device__ __forceinline__ bool over_full(const uint32_t* bl, const uint32_t* cap, int i) {
+    return int(bl[i] & 0xff) > (cap ? int(cap[i]) : kMaxLevel);
+}
+
+// seg[i] = 1 at the first body of each over-full deepest cell.
+__global__ void __launch_bounds__(kBS) restart_cells_kernel(Frames F, const uint32_t* __restrict__ cap,
+                                                            const uint32_t* __restrict__ bl, uint32_t* __restrict__ seg) {
+    const int i = blockIdx.x * kBS + threadIdx.x;
+    if (i >= F.n) return;
+    bool start = over_full(bl, cap, i);
+    if (start && i > 0 && over_full(bl, cap, i - 1))
+        start = ext_common(F, i - 1, i) < (cap ? int(cap[i]) : kMaxLevel);
+    seg[i] = start ? 1u : 0u;
+}
+
+// Order-preserving 64-bit image of a double, for atomic min and max.
+__device__ __forceinline__ unsigned long long ord64(double v) {
+    const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(v));
+    return (u >> 63) ? ~u : (u | (1ull << 63));
+}
+__device__ __forceinline__ double unord64(unsigned long long o) {
+    const unsigned long long u = (o >> 63) ? (o & ~(1ull << 63)) : ~o;
+    return __longlong_as_double(static_cast<long long>(u));
+}
+
+__global__ void __launch_bounds__(kBS) restart_bbox_init(unsigned long long* __restrict__ bbox, int ncell) {
+    const int t = blockIdx.x * kBS + threadIdx.x;
+    if (t < 6 * ncell) bbox[t] = (t % 6) < 3 ? ~0ull : 0ull;
+}
+
+// bbox[6 c .. 6 c + 6) = min x, y, z, max x, y, z of cell c (seg is the scanned start flags).
+template <class R>
+__global__ void __launch_bounds__(kBS) restart_bbox_kernel(int n, const uint32_t* __restrict__ cap,
+                                                           const uint32_t* __restrict__ bl,
+                                                           const uint32_t* __restrict__ seg, const R* __restrict__ x,
+                                                           const R* __restrict__ y, const R* __restrict__ z,
+                                                           unsigned long long* __restrict__ bbox) {
+    const int i = blockIdx.x * kBS + threadIdx.x;
+    if (i >= n || !over_full(bl, cap, i)) return;
+    unsigned long long* q = bbox + 6 * std::size_t(seg[i + 1] - 1);
+    const unsigned long long ox = ord64(double(x[i])), oy = ord64(double(y[i])), oz = ord64(double(z[i]));
+    atomicMin(q + 0, ox); atomicMin(q + 1, oy); atomicMin(q + 2, oz);
+    atomicMax(q + 3, ox); atomicMax(q + 4, oy); atomicMax(q + 5, oz);
+}
+
+// Cube of each cell: origin = the minimum, side = largest extent x (1 + 1e-6) (0: all coincide).
+__global__ void __launch_bounds__(kBS) restart_cube_kernel(const unsigned long long* __restrict__ bbox, int ncell,
+                                                           double4* __restrict__ cubes) {
+    const int c = blockIdx.x * kBS + threadIdx.x;
+    if (c >= ncell) return;
+    const unsigned long long* q = bbox + 6 * std::size_t(c);
+    const double mnx = unord64(q[0]), mny = unord64(q[1]), mnz = unord64(q[2]);
+    const double sx = __dsub_rn(unord64(q[3]), mnx), sy = __dsub_rn(unord64(q[4]), mny), sz = __dsub_rn(unord64(q[5]), mnz);
+    double side = fmax(sx, fmax(sy, sz));
+    side = side == 0.0 ? 0.0 : __dmul_rn(side, 1.0 + 1e-6);
+    cubes[c] = make_double4(mnx, mny, mnz, side);
+}
+
+// Frame keys of the bodies of restarted cells (side > 0), their cube index and cap; gkey = the
+// first body of the body's restarted cell, else the body itself.
+template <class R>
+__global__ void __launch_bounds__(kBS) restart_keys_kernel(int n, const uint32_t* __restrict__ bl,
+                                                           const uint32_t* __restrict__ seg,
+                                                           const double4* __restrict__ cubes, int cube0,
+                                                           const R* __restrict__ x, const R* __restrict__ y,
+                                                           const R* __restrict__ z, uint64_t* __restrict__ fkey,
+                                                           uint32_t* __restrict__ fseg, uint32_t* __restrict__ cap,
+                                                           uint64_t* __restrict__ gkey,
+                                                           const uint32_t* __restrict__ cstart) {
+    const int i = blockIdx.x * kBS + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k = 0, g = uint64_t(i);
+    uint32_t fs = ~0u;
+    if (over_full(bl, cap, i)) {
+        const int c = int(seg[i + 1]) - 1;
+        const double4 q = cubes[c];
+        if (q.w > 0.0) {
+            k = morton_key(x[i], y[i], z[i], q.x, q.y, q.z, __ddiv_rn(2097152.0, q.w));
+            fs = uint32_t(cube0 + c);
+            g = uint64_t(cstart[c]);
+            cap[i] += kMaxLevel;
+        }
+    }
+    fkey[i] = k;
+    fseg[i] = fs;
+    gkey[i] = g;
+}
+
+// cstart[c] = the first body of cell c.
+__global__ void __launch_bounds__(kBS) restart_starts_kernel(int n, const uint32_t* __restrict__ seg,
+                                                             uint32_t* __restrict__ cstart) {
+    const int i = blockIdx.x * kBS + threadIdx.x;
+    if (i < n && seg[i + 1] != seg[i]) cstart[seg[i]] = uint32_t(i);
+}
+
+__global__ void __launch_bounds__(kBS) gather_u64_kernel(const uint32_t* __restrict__ perm,
+                                                         const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+                                                         int n) {
+    const int i = blockIdx.x * kBS + threadIdx.x;
+    if (i < n) out[i] = in[perm[i]];
+}
+__global__ void __launch_bounds__(kBS) gather_u32_kernel(const uint32_t* __restrict__ perm,
+                                                         const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                         int n) {
+    const int i = blockIdx.x * kBS + threadIdx.x;
+    if (i < n) out[i] = in[perm[i]];
+}
+__global__ void __launch_bounds__(kBS) iota_u32_kernel(uint32_t* __restrict__ a, int n) {
+    const int i = blockIdx.x * kBS + threadIdx.x;
+    if (i < n) a[i] = uint32_t(i);
+}
+__global__ void __launch_bounds__(kBS) fill_u32_kernel(uint32_t* __restrict__ a, uint32_t v, long long n) {
+    const long long i = (long long)blockIdx.x * kBS + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+Frames frames_of(const BuildBufs& b) { return Frames{b.key, b.fkey, b.frames > 1 ? b.frames : 1, b.n}; }
+
 }  // namespace
+
+void tree_levels(hipStream_t s, const BuildBufs& b) {
+    const int n = b.n;
+    if (b.frames > 1 || b.cap)
+        levels_ext_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(frames_of(b), b.cap, b.gmax, b.bl, b.cnt, b.gflag,
+                                                           b.counters, b.host_over);
+    else
+        levels_kernel<<<grid_for(n, kLvTile), kLvTile, 0, s>>>(b.key, n, b.gmax, b.bl, b.cnt, b.gflag, b.counters,
+                                                               b.host_over);
+    LHIP_LAUNCH_CHECK();
+}
+
+int tree_restart_cells(hipStream_t s, const BuildBufs& b, uint32_t* seg) {
+    const int n = b.n;
+    restart_cells_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(frames_of(b), b.cap, b.bl, seg);
+    LHIP_LAUNCH_CHECK();
+    scan_exclusive(s, seg, n, b.scan_tmp);
+    uint32_t count = 0;
+    LHIP_CHECK(hipMemcpyAsync(&count, seg + n, 4, hipMemcpyDeviceToHost, s));
+    LHIP_CHECK(hipStreamSynchronize(s));
+    return int(count);
+}
+
+int tree_node_count(hipStream_t s, const BuildBufs& b) {
+    scan_exclusive(s, b.cnt, b.n, b.scan_tmp);
+    uint32_t M = 0;
+    LHIP_CHECK(hipMemcpyAsync(&M, b.cnt + b.n, 4, hipMemcpyDeviceToHost, s));
+    LHIP_CHECK(hipStreamSynchronize(s));
+    return int(M);
+}
+
+void tree_restart_cubes(hipStream_t s, const BuildBufs& b, const void* x, const void* y, const void* z, bool f64,
+                        const uint32_t* seg, int ncell, unsigned long long* bbox, double4* cubes) {
+    if (ncell <= 0) return;
+    const int n = b.n;
+    restart_bbox_init<<<grid_for(6ll * ncell, kBS), kBS, 0, s>>>(bbox, ncell);
+    LHIP_LAUNCH_CHECK();
+    if (f64)
+        restart_bbox_kernel<double><<<grid_for(n, kBS), kBS, 0, s>>>(
+            n, b.cap, b.bl, seg, static_cast<const double*>(x), static_cast<const double*>(y),
+            static_cast<const double*>(z), bbox);
+    else
+        restart_bbox_kernel<float><<<grid_for(n, kBS), kBS, 0, s>>>(n, b.cap, b.bl, seg, static_cast<const float*>(x),
+                                                                    static_cast<const float*>(y),
+                                                                    static_cast<const float*>(z), bbox);
+    LHIP_LAUNCH_CHECK();
+    restart_cube_kernel<<<grid_for(ncell, kBS), kBS, 0, s>>>(bbox, ncell, cubes);
+    LHIP_LAUNCH_CHECK();
+}
+
+void tree_restart_keys(hipStream_t s, const BuildBufs& b, const void* x, const void* y, const void* z, bool f64,
+                       const uint32_t* seg, const double4* cubes, int cube0, int frame, uint64_t* fkey_frame,
+                       uint32_t* fseg_frame, uint32_t* cap, uint64_t* gkey, uint32_t* cstart) {
+    (void)frame;
+    const int n = b.n;
+    restart_starts_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(n, seg, cstart);
+    LHIP_LAUNCH_CHECK();
+    if (f64)
+        restart_keys_kernel<double><<<grid_for(n, kBS), kBS, 0, s>>>(
+            n, b.bl, seg, cubes, cube0, static_cast<const double*>(x), static_cast<const double*>(y),
+            static_cast<const double*>(z), fkey_frame, fseg_frame, cap, gkey, cstart);
+    else
+        restart_keys_kernel<float><<<grid_for(n, kBS), kBS, 0, s>>>(
+            n, b.bl, seg, cubes, cube0, static_cast<const float*>(x), static_cast<const float*>(y),
+            static_cast<const float*>(z), fkey_frame, fseg_frame, cap, gkey, cstart);
+    LHIP_LAUNCH_CHECK();
+}
+
+void gather_u64(hipStream_t s, const uint32_t* perm, const uint64_t* in, uint64_t* out, int n) {
+    gather_u64_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(perm, in, out, n);
+    LHIP_LAUNCH_CHECK();
+}
+void gather_u32(hipStream_t s, const uint32_t* perm, const uint32_t* in, uint32_t* out, int n) {
+    gather_u32_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(perm, in, out, n);
+    LHIP_LAUNCH_CHECK();
+}
+void iota_u32(hipStream_t s, uint32_t* a, int n) {
+    if (n <= 0) return;
+    iota_u32_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(a, n);
+    LHIP_LAUNCH_CHECK();
+}
+void fill_u32(hipStream_t s, uint32_t* a, uint32_t v, long long n) {
+    if (n <= 0) return;
+    fill_u32_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(a, v, n);
+    LHIP_LAUNCH_CHECK();
+}
 
 void tree_moments_keep(hipStream_t s, const BuildBufs& b) {
     const unsigned g = std::min<unsigned>(grid_for(b.n, kBS), 512u);
-    for (int l = kMaxLevel; l >= 0; --l) {
-        moments_level<true, true><<<g, kBS, 0, s>>>(l, b.n, nullptr, b.xd, b.yd, b.zd, b.m, b.bl, b.levstart,
-                                                    b.prefix, b.node, b.mom, b.momd, 1.f);
+    const bool ext = b.frames > 1;
+    for (int l = tree_depth(ext ? b.frames : 1); l >= 0; --l) {
+        if (ext)
+            moments_level<true, true, true><<<g, kBS, 0, s>>>(l, b.n, nullptr, b.xd, b.yd, b.zd, b.m, b.bl,
+                                                              b.levstart, b.prefix, b.node, b.mom, b.momd, 1.f,
+                                                              frames_of(b));
+        else
+            moments_level<true, true><<<g, kBS, 0, s>>>(l, b.n, nullptr, b.xd, b.yd, b.zd, b.m, b.bl, b.levstart,
+                                                        b.prefix, b.node, b.mom, b.momd, 1.f, frames_of(b));
         LHIP_LAUNCH_CHECK();
     }
 }
@@ -466,25 +757,31 @@ std::size_t scan_tmp_entries(int n) {
     return std::size_t((static_cast<long long>(n) + kScanTile) / kScanTile) + 1;
 }
 
-void tree_build(hipStream_t s, const BuildBufs& b, float inv_theta) {
+void tree_build(hipStream_t s, const BuildBufs& b, float inv_theta, hipEvent_t levels_done) {
     const int n = b.n;
+    const bool ext = b.frames > 1;
     reset_build<<<1, 64, 0, s>>>(b.counters);
     LHIP_LAUNCH_CHECK();
     if (b.gmax < kLeafMax || b.gmax > kGroupMax) throw Error(kInvalid, "walk group size must be in [16, 128]");
-    levels_kernel<<<grid_for(n, kLvTile), kLvTile, 0, s>>>(b.key, n, b.gmax, b.bl, b.cnt, b.gflag, b.counters);
-    LHIP_LAUNCH_CHECK();
+    tree_levels(s, b);
+    if (levels_done) LHIP_CHECK(hipEventRecord(levels_done, s));
     scan_exclusive(s, b.cnt, n, b.scan_tmp);
     scan_exclusive(s, b.gflag, n, b.scan_tmp);
     const uint32_t* M_dev = b.cnt + n;  // node count
-    enumerate_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(n, b.bl, b.cnt, b.pkey, b.pval);
+    enumerate_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(n, b.bl, b.cnt, b.pkey, b.pval, ext ? b.cap : nullptr);
     LHIP_LAUNCH_CHECK();
-    // Stable sort of the pre-order nodes by level (one 8-bit pass: levels < 32).
+    // Stable sort of the pre-order nodes by level (one 8-bit pass: levels <= kMaxDepth < 256).
     radix_pass(s, b.pkey, b.pval, b.pkey2, b.pval2, b.cap_nodes, M_dev, 0, b.tile_hist);
     const std::size_t ntiles = (static_cast<std::size_t>(b.cap_nodes) + kRadixTile - 1) / kRadixTile;
     const uint32_t* level_tot = b.tile_hist + ntiles * 256;
     invert_kernel<<<grid_for(b.cap_nodes, kBS), kBS, 0, s>>>(b.pval2, M_dev, b.inv, level_tot, b.levstart);
     LHIP_LAUNCH_CHECK();
-    nodes_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(b.key, n, b.bl, b.cnt, b.inv, b.cube, b.node, b.mom, b.prefix);
+    if (ext)
+        nodes_kernel<true><<<grid_for(n, kBS), kBS, 0, s>>>(b.key, n, b.bl, b.cnt, b.inv, b.cube, b.node, b.mom,
+                                                            b.prefix, frames_of(b), b.cap, b.fseg, b.fcube);
+    else
+        nodes_kernel<false><<<grid_for(n, kBS), kBS, 0, s>>>(b.key, n, b.bl, b.cnt, b.inv, b.cube, b.node, b.mom,
+                                                             b.prefix, frames_of(b), nullptr, nullptr, nullptr);
     LHIP_LAUNCH_CHECK();
     groups_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(b.gflag, n, b.bl, b.gstart);
     LHIP_LAUNCH_CHECK();
@@ -496,13 +793,25 @@ void tree_build(hipStream_t s, const BuildBufs& b, float inv_theta) {
         LHIP_LAUNCH_CHECK();
     }
     const unsigned g = std::min<unsigned>(grid_for(n, kBS), 512u);
-    for (int l = kMaxLevel; l >= 0; --l) {
-        if (b.momd)
-            moments_level<true><<<g, kBS, 0, s>>>(l, n, nullptr, b.xd, b.yd, b.zd, b.m, b.bl, b.levstart, b.prefix,
-                                                  b.node, b.mom, b.momd, inv_theta);
-        else
-            moments_level<false><<<g, kBS, 0, s>>>(l, n, b.posm, nullptr, nullptr, nullptr, nullptr, b.bl, b.levstart,
-                                                   b.prefix, b.node, b.mom, nullptr, inv_theta);
+    const Frames F = frames_of(b);
+    for (int l = tree_depth(ext ? b.frames : 1); l >= 0; --l) {
+        if (b.momd) {
+            if (ext)
+                moments_level<true, false, true><<<g, kBS, 0, s>>>(l, n, nullptr, b.xd, b.yd, b.zd, b.m, b.bl,
+                                                                   b.levstart, b.prefix, b.node, b.mom, b.momd,
+                                                                   inv_theta, F);
+            else
+                moments_level<true><<<g, kBS, 0, s>>>(l, n, nullptr, b.xd, b.yd, b.zd, b.m, b.bl, b.levstart,
+                                                      b.prefix, b.node, b.mom, b.momd, inv_theta, F);
+        } else {
+            if (ext)
+                moments_level<false, false, true><<<g, kBS, 0, s>>>(l, n, b.posm, nullptr, nullptr, nullptr, nullptr,
+                                                                    b.bl, b.levstart, b.prefix, b.node, b.mom,
+                                                                    nullptr, inv_theta, F);
+            else
+                moments_level<false><<<g, kBS, 0, s>>>(l, n, b.posm, nullptr, nullptr, nullptr, nullptr, b.bl,
+                                                       b.levstart, b.prefix, b.node, b.mom, nullptr, inv_theta, F);
+        }
         LHIP_LAUNCH_CHECK();
     }
 }

@@ -24,6 +24,7 @@
 #include <cfloat>
 
 #include "kernels.hpp"
+#include "morton.hpp"
 #include "tree.hpp"
 
 namespace lhip {
@@ -31,36 +32,13 @@ namespace {
 
 // ---------------------------------------------------------------- keys
 
-// Bit dilation for 21-bit coordinates: the standard mask ladder that moves bit b to bit 3b.
-__device__ __forceinline__ uint64_t dilate21(uint64_t v) {
-    v &= 0x1fffffull;
-    v = (v | v << 32) & 0x1f00000000ffffull;
-    v = (v | v << 16) & 0x1f0000ff0000ffull;
-    v = (v | v << 8) & 0x100f00f00f00f00full;
-    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
-    v = (v | v << 2) & 0x1249249249249249ull;
-    return v;
-}
-
-// Quantisation in double with a clamp, as src/cpu_backend.cpp:419-425: (v - o) * 2^21/side, clamped
-// to [0, 2^21 - 1], truncated. Sub and mul stay separate IEEE operations.
-template <class R>
-__device__ __forceinline__ uint64_t quantize(R v, double o, double s) {
-    double t = __dmul_rn(__dsub_rn(double(v), o), s);
-    t = t < 0.0 ? 0.0 : t;
-    t = t > 2097151.0 ? 2097151.0 : t;
-    return uint64_t(t);
-}
-
 template <class R>
 __global__ void __launch_bounds__(kBS) keys_kernel(const R* __restrict__ x, const R* __restrict__ y,
                                                    const R* __restrict__ z, const Cube* __restrict__ cube,
                                                    uint64_t* __restrict__ key, uint32_t* __restrict__ idx, int n) {
     const int i = blockIdx.x * kBS + threadIdx.x;
     if (i >= n) return;
-    const double s = cube->scale;
-    key[i] = dilate21(quantize(x[i], cube->ox, s)) << 2 | dilate21(quantize(y[i], cube->oy, s)) << 1 |
-             dilate21(quantize(z[i], cube->oz, s));
+    key[i] = morton_key(x[i], y[i], z[i], cube->ox, cube->oy, cube->oz, cube->scale);
     idx[i] = uint32_t(i);
 }
 
@@ -298,6 +276,25 @@ void tree_keys_sorted(hipStream_t s, const double* x, const double* y, const dou
     launch_bbox(s, x, y, z, n, bbox_part, cube);
     keys_kernel<double><<<grid_for(n, kBS), kBS, 0, s>>>(x, y, z, cube, key, idx, n);
     LHIP_LAUNCH_CHECK();
+}
+
+void tree_permute(hipStream_t s, TreeBufs& b, const uint32_t* perm, int n, bool permute_cost) {
+    if (n <= 0) return;
+    auto run = [&](auto tag) {
+        using R = decltype(tag);
+        auto P = [](void* p) { return static_cast<R*>(p); };
+        permute_kernel<R><<<grid_for(n, kBS), kBS, 0, s>>>(perm, n, P(b.x), P(b.y), P(b.z), P(b.vx), P(b.vy), P(b.vz),
+                                                          b.m, b.id, permute_cost ? b.cost : nullptr, P(b.x2), P(b.y2),
+                                                          P(b.z2), P(b.vx2), P(b.vy2), P(b.vz2), b.m2, b.id2,
+                                                          permute_cost ? b.cost2 : nullptr, b.posm, b.posd);
+        LHIP_LAUNCH_CHECK();
+    };
+    if (b.f64) run(double{});
+    else run(float{});
+    std::swap(b.x, b.x2); std::swap(b.y, b.y2); std::swap(b.z, b.z2);
+    std::swap(b.vx, b.vx2); std::swap(b.vy, b.vy2); std::swap(b.vz, b.vz2);
+    std::swap(b.m, b.m2); std::swap(b.id, b.id2);
+    if (permute_cost) std::swap(b.cost, b.cost2);
 }
 
 void tree_sort(hipStream_t s, TreeBufs& b, int n, bool permute_cost) {

@@ -42,7 +42,6 @@ constexpr int kBS4 = 64 * kWaves;
 constexpr int kCells = 64;     // cells per batch (one per lane)
 constexpr int kBodies = 128;   // bodies per batch (two per lane)
 constexpr int kStack = 896;
-constexpr int kThr = kStack - 7 * kLevels - 8;
 
 struct VjpLds {
     float4 cA[kCells];  // com relative to the group centre, mass
@@ -68,6 +67,8 @@ __global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
     const int ngroups = int(a.gcount[a.n]);
     const float e2 = a.eps2;
     bool overflow = false;
+    // the walk's stack budget for this tree depth (kernels_walk.hip)
+    const int thr = kStack - 7 * (max(a.depth, kMaxLevel) + 1) - 8;
 
     double* const priv = a.gpriv + std::size_t(blockIdx.x) * kHot * 12;
     auto cell_out = [&](int k) -> double* {  // quadrupole cell k: gC (3), G (6)
@@ -296,7 +297,7 @@ __global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
             if (lane == 0) st[0] = 0;
             int top = 1;
             while (top > 0) {
-                const int room = (kThr - top) / 7;
+                const int room = (thr - top) / 7;
                 const int P = min(64, min(top, max(1, room)));
                 __builtin_amdgcn_wave_barrier();
                 const bool act = lane < P;
@@ -519,7 +520,7 @@ void tree_vjp(hipStream_t s, const TreeVjpArgs& a, const uint32_t* levstart, con
     down_root<<<1, 64, 0, s>>>(a.gcell, a.gmono, a.momd, cum);
     LHIP_LAUNCH_CHECK();
     const unsigned g = std::min<unsigned>(grid_for(a.n, kBS), 512u);
-    for (int l = 0; l < kMaxLevel; ++l) {
+    for (int l = 0; l < std::max(a.depth, kMaxLevel); ++l) {
         down_level<<<g, kBS, 0, s>>>(l, levstart, a.node, a.gcell, a.gmono, a.momd, cum);
         LHIP_LAUNCH_CHECK();
     }

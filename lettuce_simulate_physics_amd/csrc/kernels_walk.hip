@@ -147,8 +147,9 @@ __device__ __forceinline__ void eval_bodies(Slot (&sl)[2], int S, int nb, const 
 }
 
 constexpr int kStackCap = 896;
-// Deepest possible tree: levels 0..21 (no key restarts).
-constexpr int kStackThr = kStackCap - 7 * kLevels - 8;
+// Stack budget: a step popping P nodes pushes at most 8P, so P is held to (thr - top) / 7 with thr
+// = capacity - 7 (depth + 1) - 8; at or above thr one node per step adds at most 7 per tree level.
+__device__ __forceinline__ int stack_threshold(int depth) { return kStackCap - 7 * (max(depth, kMaxLevel) + 1) - 8; }
 
 template <bool POT, bool GUARD, bool REL>
 __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
@@ -161,6 +162,7 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
     int* st = stk[w];
     const int ngroups = int(a.gcount[a.n]);
     const float e2 = a.eps2;
+    const int stack_thr = stack_threshold(a.depth);
     unsigned long long t_inter = 0, t_p2p = 0, t_p2c = 0, t_listed = 0, t_p2m = 0;
     bool overflow = false;
 
@@ -263,7 +265,7 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
             if (lane == 0) st[0] = 0;  // the root
             int top = 1;
             while (top > 0) {
-                const int room = (kStackThr - top) / 7;
+                const int room = (stack_thr - top) / 7;
                 const int P = min(64, min(top, max(1, room)));
                 __builtin_amdgcn_wave_barrier();
                 const bool act = lane < P;

@@ -67,6 +67,15 @@ struct BuildBufs {
     uint32_t* pflag;           // n + 1: pack starts per group, then their exclusive scan (count at [n])
     uint32_t* pstart;          // n + 1
     WalkCounters* counters;
+    // Key restarts (frames > 1). Frame f >= 1 keys of body i at fkey[(f - 1) n + i] (0 outside
+    // restarted cells); cap[i] = 21 x the frames body i has (its deepest level); fseg[(f - 1) n + i]
+    // = the body's restarted cell at frame f, an index into fcube (ox, oy, oz, side).
+    int frames;
+    const uint64_t* fkey;
+    const uint32_t* cap;
+    const uint32_t* fseg;
+    const double4* fcube;
+    int* host_over;            // pinned host flag: set by the level pass if a finest cell is over-full
 };
 
 std::size_t scan_tmp_entries(int n);
@@ -75,8 +84,35 @@ std::size_t scan_tmp_entries(int n);
 inline long long node_capacity(long long n) { return n + (n / 17 + 1) * kLevels + 1; }
 
 // Leaf levels, node enumeration, level-order layout, groups and multipoles. inv_theta as the
-// reference CPU computes it.
-void tree_build(hipStream_t s, const BuildBufs& b, float inv_theta);
+// reference CPU computes it. levels_done (optional) is recorded after the level pass, which sets
+// *b.host_over when a deepest cell is over-full (key restarts needed, or coincident bodies).
+void tree_build(hipStream_t s, const BuildBufs& b, float inv_theta, hipEvent_t levels_done = nullptr);
+// Deepest level of a tree with b.frames key frames.
+inline int tree_depth(int frames) { return kMaxLevel * frames; }
+
+// ---- key restarts (the rare path; the host drives it, src/cpu_backend.cpp:501-544)
+// Level pass with frames: bl, cnt, gflag from the extended keys and caps.
+void tree_levels(hipStream_t s, const BuildBufs& b);
+// Over-full deepest cells: seg[i] = 1 at each one's first body, then the exclusive scan (count at
+// [n], read back; the stream is synchronised). Returns the count.
+int tree_restart_cells(hipStream_t s, const BuildBufs& b, uint32_t* seg);
+// Node count of the tree the levels in b describe (scans b.cnt; read back).
+int tree_node_count(hipStream_t s, const BuildBufs& b);
+// Bounding cube of each over-full cell (side 0: its bodies coincide, no restart), from positions
+// (double if f64). bbox: 6 x ncell scratch.
+void tree_restart_cubes(hipStream_t s, const BuildBufs& b, const void* x, const void* y, const void* z, bool f64,
+                        const uint32_t* seg, int ncell, unsigned long long* bbox, double4* cubes);
+// Frame-`frame` keys of the bodies of restarted cells (cube index cube0 + cell), their caps and
+// cells; gkey[i] = the position of the first body of i's restarted cell, else i (the grouping key of
+// the stable re-sort); cstart: ncell scratch. fkey/fseg of the frame are written for every body.
+void tree_restart_keys(hipStream_t s, const BuildBufs& b, const void* x, const void* y, const void* z, bool f64,
+                       const uint32_t* seg, const double4* cubes, int cube0, int frame, uint64_t* fkey_frame,
+                       uint32_t* fseg_frame, uint32_t* cap, uint64_t* gkey, uint32_t* cstart);
+// out[i] = in[perm[i]] (u64 and u32 arrays of n).
+void gather_u64(hipStream_t s, const uint32_t* perm, const uint64_t* in, uint64_t* out, int n);
+void gather_u32(hipStream_t s, const uint32_t* perm, const uint32_t* in, uint32_t* out, int n);
+void fill_u32(hipStream_t s, uint32_t* a, uint32_t v, long long n);
+void iota_u32(hipStream_t s, uint32_t* a, int n);
 // Mixed form, frozen topology: recompute com, mass and quadrupole (node and momd) from the
 // positions in b over the tree tree_build made, keeping every decision input (node records,
 // rcrit2, qrel).
@@ -106,6 +142,7 @@ struct WalkArgs {
     int blocks;                // persistent grid size (blocks of 4 waves)
     int small_leaf;            // accepted leaves with at most this many bodies are summed directly
     float quad_tau;            // error-budgeted far field threshold (0: every quadrupole kept)
+    int depth;                 // deepest tree level (21 x key frames)
 };
 void tree_walk(hipStream_t s, const WalkArgs& a, bool potential);
 int walk_blocks_per_cu();
@@ -134,6 +171,7 @@ struct TreeVjpArgs {
     double* gpriv;  // blocks x kHot x 12 doubles (per-block gcell, gmono of the first kHot nodes), zeroed
     WalkCounters* counters;
     int blocks;
+    int depth;      // deepest tree level (21 x key frames)
 };
 // Nodes at the top of the tree (levels 0-3 and part of 4 in level order) whose VJP cotangents each
 // block accumulates privately.

@@ -208,9 +208,20 @@ struct Run {
         tree_keys_sorted(s, rx, rx + n, rx + 2 * std::size_t(n), n, c.bbox_part.as<void>(), c.cube.as<Cube>(),
                          c.key.as<uint64_t>(), c.idx.as<uint32_t>());
         LHIP_CHECK(hipEventRecord(c.ev[1], s));
+        c.frames = 1;
         BuildBufs bb = c.build_bufs();
         bb.xd = rx; bb.yd = rx + n; bb.zd = rx + 2 * std::size_t(n);
-        tree_build(s, bb, fp.theta > 0 ? float(1.0 / fp.theta) : FLT_MAX);
+        const float inv_theta = fp.theta > 0 ? float(1.0 / fp.theta) : FLT_MAX;
+        *reinterpret_cast<volatile int*>(c.host_over) = 0;
+        tree_build(s, bb, inv_theta, c.ev[9]);
+        LHIP_CHECK(hipEventSynchronize(c.ev[9]));
+        if (*reinterpret_cast<volatile int*>(c.host_over)) {
+            // key restarts of the recorded call: its order is already the restarted one
+            const double* box[3] = {rx, rx + n, rx + 2 * std::size_t(n)};
+            c.restart_keys(box[0], box[1], box[2], false);
+            c.rebuild(inv_theta, box);
+        }
+        bb = c.build_bufs();
         bb.xd = D(c.xd); bb.yd = D(c.yd); bb.zd = D(c.zd);
         tree_moments_keep(s, bb);
         pack_posd_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(rx, rx + n, rx + 2 * std::size_t(n), c.m.as<float>(),
@@ -408,6 +419,7 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
         a.gpriv = R.D(S.gpriv);
         a.counters = c.counters.as<WalkCounters>();
         a.blocks = blocks;
+        a.depth = tree_depth(c.frames);
         tree_vjp(s, a, c.levstart.as<uint32_t>(), c.cnt.as<uint32_t>() + n, nodes, R.D(S.cum));
         scatter_add_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(c.id.as<uint32_t>(), gt, gt + N, gt + 2 * N, gs, gs + N,
                                                             gs + 2 * N, lx, lx + N, lx + 2 * N, n);

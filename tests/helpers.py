@@ -89,3 +89,32 @@ def morton_keys_np(x, y, z):
         return r
 
     return (dil(q(x, lo[0])) << np.uint64(2)) | (dil(q(y, lo[1])) << np.uint64(1)) | dil(q(z, lo[2]))
+
+
+def restart_positions(kind: str, seed: int = 1) -> np.ndarray:
+    """3000 float32 positions in [-1, 1]^3 with finest-level cells holding more than 16 bodies that do
+    not coincide, so the build restarts keys in those cells' own cubes (src/cpu_backend.cpp:501-544):
+    "clump": 200 bodies within 1e-6 of a point (one restart frame);
+    "nested": a body at 1000 sets a large root cube; 399 bodies in [0, 2e-3]^3, 50 within 2e-9 of a
+    point and 9 copies at each of 21 consecutive float x values (restarts inside a restarted cell);
+    "coincident_in_restart": 100 bodies within 1e-6 of a point, 40 more on one of them (a coincident
+    leaf inside a restarted cell)."""
+    rng = np.random.default_rng(seed)
+    n = 3000
+    x = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    if kind == "clump":
+        x[:200] = (np.float32([0.3, -0.2, 0.1]) + rng.uniform(0, 1e-6, (200, 3))).astype(np.float32)
+    elif kind == "nested":
+        x[0] = [1000.0, 1000.0, 1000.0]
+        x[1:400] = rng.uniform(0, 2e-3, (399, 3)).astype(np.float32)
+        x[400:450] = (np.float32(1e-3) + rng.uniform(0, 2e-9, (50, 3))).astype(np.float32)
+        v = np.float32(1.5e-3)
+        for k in range(21):
+            x[450 + 9 * k: 459 + 9 * k] = [v, np.float32(1.5e-3), np.float32(1.5e-3)]
+            v = np.nextafter(v, np.float32(1))
+    elif kind == "coincident_in_restart":
+        x[:100] = (np.float32([0.5, 0.5, 0.5]) + rng.uniform(0, 1e-6, (100, 3))).astype(np.float32)
+        x[100:140] = x[0]
+    else:
+        raise ValueError(kind)
+    return x

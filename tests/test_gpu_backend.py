@@ -12,7 +12,7 @@ import pytest
 
 import lettuce_simulate_physics_amd as L
 import oracle as O
-from helpers import canon_from_hip, canon_from_oracle, digest, per_body_rel, rel_rms
+from helpers import canon_from_hip, canon_from_oracle, digest, per_body_rel, rel_rms, restart_positions
 
 pytestmark = pytest.mark.gpu
 
@@ -284,8 +284,6 @@ def test_small_sets_parity_with_reference(hip, name):
         theta = float(key[1:])
         a, d, t = got(hip, b, P("tree", eps, theta), True)
         assert rel_rms(a, SMALL[f"{name}__{key}__acc"]) < 1e-5
-        if "coincident" in name:
-            continue  # the reference restarts keys in over-full finest cells; not built here yet
         o = hip.download()
         # the kick(1) above does not reorder: ids are still in tree order
         assert np.array_equal(o.id, SMALL[f"{name}__{key}__ids"])
@@ -295,14 +293,14 @@ def test_small_sets_parity_with_reference(hip, name):
         assert t["listed"] == meta[key]["interactions_listed"]
 
 
-@pytest.mark.parametrize("n,seed", [(3000, 21), (50000, 22)])
-def test_tree_bit_exact_with_oracle(hip, n, seed):
-    b = O.plummer(n, seed)
+def check_tree_against_oracle(hip, b, eps=0.01, theta=0.6):
+    """Permutation, keys, canonical octree, walk groups bit-exact; moments, counts and forces from the
+    same lists."""
     o = O.Oracle()
     o.upload(b)
-    o.compute_forces(1.0, 0.01, 0.6, 1, True)
+    o.compute_forces(1.0, eps, theta, 1, True)
     hip.upload(b)
-    hip.compute_forces(P("tree", 0.01, 0.6), True)
+    hip.compute_forces(P("tree", eps, theta), True)
     keys_o, _ = o.keys()
     so = o.download()
     sh = hip.download()
@@ -312,14 +310,14 @@ def test_tree_bit_exact_with_oracle(hip, n, seed):
     assert np.array_equal(canon_from_hip(nh), canon_from_oracle(no))  # octree topology
     g = hip.tree_groups()
     assert np.array_equal(np.stack([g[:-1], g[1:]], 1), o.groups())  # walk groups
-    # moments: same cells, FP64 accumulation on both sides
-    ch, co = canon_from_hip(nh), canon_from_oracle(no)
+    # moments and cell geometry: same cells, FP64 accumulation on both sides
     ih = np.lexsort((nh["level"], nh["lo"]))
     io = np.lexsort((no["depth"], no["lo"]))
     assert np.allclose(nh["m"][ih], no["m"][io], rtol=1e-6, atol=0)
     assert np.allclose(nh["com"][ih], no["com"][io], rtol=0, atol=1e-6)
     assert np.allclose(nh["q"][ih], no["q"][io], rtol=1e-4, atol=1e-6 * np.abs(no["q"]).max())
     assert np.allclose(nh["rcrit2"][ih], no["rcrit2"][io], rtol=1e-6)
+    assert np.array_equal(nh["geo"][ih], no["geo"][io])
     # interaction lists: per-body counts and totals (reference convention includes the self entry)
     t = hip.times()
     c = o.counts()
@@ -333,6 +331,30 @@ def test_tree_bit_exact_with_oracle(hip, n, seed):
     ah, ao = np.stack([ax, ay, az], 1), np.stack([bx, by, bz], 1)
     assert rel_rms(ah, ao) < 1e-5
     assert rel_rms(pot, bpot) < 1e-5
+    return no
+
+
+@pytest.mark.parametrize("n,seed", [(3000, 21), (50000, 22)])
+def test_tree_bit_exact_with_oracle(hip, n, seed):
+    check_tree_against_oracle(hip, O.plummer(n, seed))
+
+
+@pytest.mark.parametrize("kind", ["clump", "nested", "coincident_in_restart"])
+def test_tree_key_restarts_bit_exact_with_oracle(hip, kind):
+    # Over-full finest cells whose bodies do not coincide restart their keys in their own cubes
+    # (src/cpu_backend.cpp:501-544), nested where a restarted cell holds one again: keys of the
+    # deepest frame, permutation, topology, groups, cell geometry and lists equal the oracle's.
+    x = restart_positions(kind)
+    n = len(x)
+    b = O.Bodies(x[:, 0].copy(), x[:, 1].copy(), x[:, 2].copy(), np.zeros(n, np.float32), np.zeros(n, np.float32),
+                 np.zeros(n, np.float32), np.full(n, 1.0 / n, np.float32), np.arange(n, dtype=np.uint32))
+    no = check_tree_against_oracle(hip, b, 0.01, 0.6)
+    depth = int(no["depth"].max())
+    assert depth > 21 and (kind != "nested" or depth > 42)
+    # theta 0: every cell opened, the direct sum with its count
+    a0, _, t0 = got(hip, b, P("tree", 0.01, 0.0), False)
+    ad, _, _ = got(hip, b, P("direct", 0.01), False)
+    assert rel_rms(a0, ad) < 1e-5 and t0["interactions"] == n * (n - 1)
 
 
 def test_c1_million_parity_with_reference(hip):

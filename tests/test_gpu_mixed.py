@@ -9,7 +9,7 @@ import pytest
 
 import lettuce_simulate_physics_amd as L
 import oracle as O
-from helpers import per_body_rel, rel_rms
+from helpers import per_body_rel, rel_rms, restart_positions
 
 pytestmark = pytest.mark.gpu
 
@@ -79,10 +79,19 @@ def test_mixed_tree_theta0_equals_direct(mixed):
     assert np.max(per_body_rel(a, ref)) < 2e-5
 
 
-def test_mixed_topology_equals_plain_with_reference_groups(hip):
-    # Float bodies promoted to FP64 quantise to the same keys, so with the reference's group size the
-    # mixed form builds the plain form's tree exactly.
-    b = O.plummer(50000, 24)
+def restart_bodies(kind):
+    x = restart_positions(kind)
+    n = len(x)
+    return O.Bodies(x[:, 0].copy(), x[:, 1].copy(), x[:, 2].copy(), np.zeros(n, np.float32), np.zeros(n, np.float32),
+                    np.zeros(n, np.float32), np.full(n, 1.0 / n, np.float32), np.arange(n, dtype=np.uint32))
+
+
+@pytest.mark.parametrize("kind", ["plummer", "clump", "nested", "coincident_in_restart"])
+def test_mixed_topology_equals_plain_with_reference_groups(hip, kind):
+    # Float bodies promoted to FP64 quantise to the same keys (key restarts included: the same cubes
+    # from the same extents), so with the reference's group size the mixed form builds the plain
+    # form's tree exactly.
+    b = O.plummer(50000, 24) if kind == "plummer" else restart_bodies(kind)
     p = L.ForceParams(G=1.0, eps=0.01, theta=0.6, method="tree")
     hip.set_precision("plain")
     hip.upload(b)
