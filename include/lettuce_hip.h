@@ -218,9 +218,10 @@ int lhip_tree_keys(lhip_ctx* ctx, uint64_t* keys);
  * frame. Over-full cells whose bodies differ restart their keys (src/cpu_backend.cpp:501-544). */
 int lhip_tree_counts(lhip_ctx* ctx, uint64_t* nodes, uint64_t* groups, uint64_t* overfull);
 /* Node records in level order (children of a node contiguous): link = (lo, hi, info, first child)
- * per node with info = level | leaf << 5 | children << 8 and first = -1 for a leaf; prefix = key
- * prefix (3*level bits); mom = 16 floats per node (cx, cy, cz, m, qxx, qxy, qxz, qyy, qyz, qzz,
- * rcrit2, 0, gx, gy, gz, half). Any pointer may be NULL. */
+ * per node with info = level | leaf << 7 | children << 8 (level = depth from the root, up to 84 with
+ * key restarts) and first = -1 for a leaf; prefix = key prefix within the node's key frame; mom = 16
+ * floats per node (cx, cy, cz, m, qxx, qxy, qxz, qyy, qyz, qzz, rcrit2, qrel, gx, gy, gz, half). Any
+ * pointer may be NULL. */
 int lhip_tree_nodes(lhip_ctx* ctx, int32_t* link, uint64_t* prefix, float* mom);
 /* Walk-group starts (groups + 1 entries; the last is n). */
 int lhip_tree_groups(lhip_ctx* ctx, uint32_t* gstart);
