@@ -19,6 +19,8 @@
 // { lambda_v += dt lambda_x; VJP(x_n, n > 0 ? dt : dt/2) }. A VJP rebuilds the tree at x_n (the
 // forward's tree: the same positions in the same order sort the same way), gathers
 // u = G w lambda_v in sorted order and runs tree_vjp; the result is added to lambda_x by id.
+#include <vector>
+
 #include "adjoint.hpp"
 #include "context.hpp"
 
@@ -86,6 +88,25 @@ __global__ void __launch_bounds__(kBS) pack_posd_kernel(const double* __restrict
     if (k < n) p[k] = make_double4(x[k], y[k], z[k], double(m[k]));
 }
 
+// A recomputed step's tree into a store slot: the node records of the device node count (at most
+// cap; flag = 1 and nothing copied beyond that).
+__global__ void __launch_bounds__(kBS) store_nodes_kernel(const uint32_t* __restrict__ node_count,
+                                                          const WalkNode* __restrict__ node,
+                                                          const NodeMom* __restrict__ mom,
+                                                          const NodeMomD* __restrict__ momd, long long cap,
+                                                          WalkNode* __restrict__ snode, NodeMom* __restrict__ smom,
+                                                          NodeMomD* __restrict__ smomd, uint32_t* __restrict__ flag) {
+    const long long M = *node_count;
+    const long long i0 = (long long)blockIdx.x * kBS + threadIdx.x;
+    if (i0 == 0) *flag = M > cap ? 1u : 0u;
+    if (M > cap) return;
+    for (long long i = i0; i < M; i += (long long)gridDim.x * kBS) {
+        snode[i] = node[i];
+        smom[i] = mom[i];
+        smomd[i] = momd[i];
+    }
+}
+
 }  // namespace
 
 struct TreeSim {
@@ -104,6 +125,13 @@ struct TreeSim {
     DevBuf rec_id, rec_x;
     // cotangents (id order) and VJP scratch (sorted order)
     DevBuf lx, lvx, u, gt, gs, gcell, gmono, gpriv, cum, inv, tmp, tmp_m, tmp_id, posd_rec;
+    // Trees of a segment's recomputed steps (slot k = step start + k), reused by their VJPs instead
+    // of a second build: node records (at most tr_cap per slot), level starts, node count, walk
+    // units and their count; tr_flag[k] = 1 where a tree did not fit.
+    DevBuf tr_node, tr_mom, tr_momd, tr_lev, tr_cnt, tr_starts, tr_count, tr_flag;
+    long long tr_cap = 0;
+    int tr_slots = 0;
+    std::vector<int> tr_frames;
 
     TreeSim() {
         for (auto& e : ev) LHIP_CHECK(hipEventCreate(&e));
@@ -381,11 +409,72 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
         c.sync();
     }
     const int blocks = c.cus * tree_vjp_blocks_per_cu();
+    // Tree store for the K - 1 inner steps of a segment, sized on the first recomputed tree (twice
+    // its node count per slot) and limited to kTreeStoreBytes; off when K < 2 or over the limit.
+    constexpr long long kTreeStoreBytes = 16ll << 30;
+    constexpr int kLev = kDepthLevels + 2;
+    S.tr_slots = 0;
+    bool tr_sized = K < 2;
+    auto size_store = [&]() {
+        tr_sized = true;
+        uint32_t M0 = 0;
+        LHIP_CHECK(hipMemcpyAsync(&M0, c.cnt.as<uint32_t>() + n, 4, hipMemcpyDeviceToHost, s));
+        c.sync();
+        const long long cap = std::max<long long>(2ll * M0, 1024);
+        const long long per = cap * (long long)(sizeof(WalkNode) + sizeof(NodeMom) + sizeof(NodeMomD)) +
+                              2ll * (n + 1) * 4 + kLev * 4 + 8;
+        if (per * (K - 1) > kTreeStoreBytes) return;
+        S.tr_cap = cap;
+        S.tr_slots = K - 1;
+        const std::size_t sl = std::size_t(S.tr_slots);
+        S.tr_node.reserve(sl * cap * sizeof(WalkNode));
+        S.tr_mom.reserve(sl * cap * sizeof(NodeMom));
+        S.tr_momd.reserve(sl * cap * sizeof(NodeMomD));
+        S.tr_lev.reserve(sl * kLev * 4);
+        S.tr_cnt.reserve(sl * 4);
+        S.tr_starts.reserve(sl * (N + 1) * 4);
+        S.tr_count.reserve(sl * (N + 1) * 4);
+        S.tr_flag.reserve(sl * 4);
+        S.tr_frames.assign(sl, 1);
+    };
+    auto slot_node = [&](int j) { return S.tr_node.as<WalkNode>() + std::size_t(j) * S.tr_cap; };
+    auto slot_mom = [&](int j) { return S.tr_mom.as<NodeMom>() + std::size_t(j) * S.tr_cap; };
+    auto slot_momd = [&](int j) { return S.tr_momd.as<NodeMomD>() + std::size_t(j) * S.tr_cap; };
+    // the tree the recompute just built at segment step k (1 <= k <= K - 1) into slot k - 1
+    auto store_tree = [&](int k) {
+        if (!tr_sized) size_store();
+        if (k < 1 || k > S.tr_slots) return;
+        const int j = k - 1;
+        const unsigned g = std::min<unsigned>(grid_for(S.tr_cap, kBS), 2048u);
+        store_nodes_kernel<<<g, kBS, 0, s>>>(c.cnt.as<uint32_t>() + n, c.node.as<WalkNode>(), c.mom.as<NodeMom>(),
+                                             c.momd.as<NodeMomD>(), S.tr_cap, slot_node(j), slot_mom(j),
+                                             slot_momd(j), S.tr_flag.as<uint32_t>() + j);
+        LHIP_LAUNCH_CHECK();
+        LHIP_CHECK(hipMemcpyAsync(S.tr_lev.as<uint32_t>() + std::size_t(j) * kLev, c.levstart.as<void>(), kLev * 4,
+                                  hipMemcpyDeviceToDevice, s));
+        LHIP_CHECK(hipMemcpyAsync(S.tr_cnt.as<uint32_t>() + j, c.cnt.as<uint32_t>() + n, 4, hipMemcpyDeviceToDevice, s));
+        LHIP_CHECK(hipMemcpyAsync(S.tr_starts.as<uint32_t>() + std::size_t(j) * (N + 1), c.walk_starts(), (N + 1) * 4,
+                                  hipMemcpyDeviceToDevice, s));
+        LHIP_CHECK(hipMemcpyAsync(S.tr_count.as<uint32_t>() + std::size_t(j) * (N + 1), c.walk_count(), (N + 1) * 4,
+                                  hipMemcpyDeviceToDevice, s));
+        S.tr_frames[j] = c.frames;
+    };
+    std::vector<uint32_t> tr_ok;  // per slot of the current segment: 1 if its tree was stored whole
     auto vjp_at = [&](int k, double w) {
         R.load_positions(S.seg_d, S.seg_id, S.seg_m, k);
-        c.build_tree(fp);  // the forward's tree at these positions
         c.posd.reserve(N * sizeof(double4));
-        const int nodes = int(c.cap_nodes);
+        const int j = k - 1;
+        const bool stored = j >= 0 && j < int(tr_ok.size()) && tr_ok[j];
+        if (stored) {
+            // the recompute's tree at these positions; only the FP64 records are repacked
+            pack_posd_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(R.D(c.xd), R.D(c.yd), R.D(c.zd), c.m.as<float>(),
+                                                              c.posd.as<double4>(), n);
+            LHIP_LAUNCH_CHECK();
+        } else {
+            c.build_tree(fp);  // the forward's tree at these positions
+        }
+        // node scratch covers the tree in use (a stored tree may exceed the last build's bound)
+        const int nodes = int(stored ? std::max<long long>(c.cap_nodes, S.tr_cap) : c.cap_nodes);
         S.gcell.reserve(std::size_t(nodes) * 9 * 8);
         S.gmono.reserve(std::size_t(nodes) * 3 * 8);
         S.cum.reserve(std::size_t(nodes) * 9 * 8);
@@ -401,13 +490,13 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
         S.gpriv.reserve(std::size_t(blocks) * kHot * 12 * 8);
         launch_fill64(s, R.D(S.gpriv), 0.0, blocks * kHot * 12);
         TreeVjpArgs a{};
-        a.node = c.node.as<WalkNode>();
-        a.mom = c.mom.as<NodeMom>();
-        a.momd = c.momd.as<NodeMomD>();
+        a.node = stored ? slot_node(j) : c.node.as<WalkNode>();
+        a.mom = stored ? slot_mom(j) : c.mom.as<NodeMom>();
+        a.momd = stored ? slot_momd(j) : c.momd.as<NodeMomD>();
         a.posd = c.posd.as<double4>();
         a.ux = u; a.uy = u + N; a.uz = u + 2 * N;
-        a.gstart = c.walk_starts();
-        a.gcount = c.walk_count();
+        a.gstart = stored ? S.tr_starts.as<uint32_t>() + std::size_t(j) * (N + 1) : c.walk_starts();
+        a.gcount = stored ? S.tr_count.as<uint32_t>() + std::size_t(j) * (N + 1) : c.walk_count();
         a.n = n;
         a.eps2 = float(S.p.eps * S.p.eps);
         a.small_leaf = c.wopt.small_leaf;
@@ -419,8 +508,9 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
         a.gpriv = R.D(S.gpriv);
         a.counters = c.counters.as<WalkCounters>();
         a.blocks = blocks;
-        a.depth = tree_depth(c.frames);
-        tree_vjp(s, a, c.levstart.as<uint32_t>(), c.cnt.as<uint32_t>() + n, nodes, R.D(S.cum));
+        a.depth = tree_depth(stored ? S.tr_frames[j] : c.frames);
+        tree_vjp(s, a, stored ? S.tr_lev.as<uint32_t>() + std::size_t(j) * kLev : c.levstart.as<uint32_t>(),
+                 stored ? S.tr_cnt.as<uint32_t>() + j : c.cnt.as<uint32_t>() + n, nodes, R.D(S.cum));
         scatter_add_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(c.id.as<uint32_t>(), gt, gt + N, gt + 2 * N, gs, gs + N,
                                                             gs + 2 * N, lx, lx + N, lx + 2 * N, n);
         LHIP_LAUNCH_CHECK();
@@ -437,10 +527,21 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
             if (t + 1 < end) {
                 R.force();
                 R.save_positions(S.seg_d, S.seg_id, S.seg_m, t + 1 - start);
+                store_tree(t + 1 - start);
                 c.kick(0.5 * dt);
             } else {
                 R.save_positions(S.seg_d, S.seg_id, S.seg_m, t + 1 - start);
             }
+        }
+        // which of this segment's stored trees fit their slots (one read per segment)
+        const int stored_steps = std::min(S.tr_slots, end - start - 1);
+        tr_ok.assign(std::max(stored_steps, 0), 0u);
+        if (stored_steps > 0) {
+            std::vector<uint32_t> fl(stored_steps);
+            LHIP_CHECK(hipMemcpyAsync(fl.data(), S.tr_flag.as<void>(), std::size_t(stored_steps) * 4,
+                                      hipMemcpyDeviceToHost, s));
+            c.sync();
+            for (int q = 0; q < stored_steps; ++q) tr_ok[q] = fl[q] == 0u;
         }
         const int first = end == T ? end : end - 1;
         for (int q = first; q >= start; --q) {
