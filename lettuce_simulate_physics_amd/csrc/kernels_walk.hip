@@ -18,8 +18,9 @@
 //
 // Evaluation. Accepted cells are staged in LDS with their moments, bodies of opened leaves with
 // their positions. Walk groups hold 1-128 bodies (39 on average for a Plummer sphere), so one body
-// per lane would idle most lanes. Instead the wave has V = 64 * NS virtual slots (NS = 1 or 2 per
-// lane, whichever keeps more lanes busy); with S = min(16, floor(V / nb)) list shares, slot v
+// per lane would idle most lanes. Instead the wave has V = 64 * NS virtual slots (NS = 1-3 per lane
+// in the plain form, 1-2 in the mixed form, whichever keeps most slots busy); with
+// S = min(16, floor(V / nb)) list shares, slot v
 // evaluates body v mod nb against list entries e = v div nb (mod S). Partial sums of the S shares are added in
 // LDS at the end. Lists are evaluated whenever the next append would overflow them. Before an
 // evaluation the list is padded with zero-mass entries to a multiple of S, so the loops carry no
@@ -122,8 +123,8 @@ __device__ __forceinline__ void p2c(Slot& b, float4 A, float4 B, float2 C, float
 }
 
 // Every slot evaluates entries sp, sp + S, ... < steps * S (the list is padded to that length).
-template <int NS, bool POT, bool GUARD>
-__device__ __forceinline__ void eval_cells(Slot (&sl)[2], int S, int nc, const WaveLds& L, float e2) {
+template <int NS, bool POT, bool GUARD, int NX>
+__device__ __forceinline__ void eval_cells(Slot (&sl)[NX], int S, int nc, const WaveLds& L, float e2) {
     const int steps = (nc + S - 1) / S;
     for (int t = 0; t < steps; ++t) {
 #pragma unroll
@@ -134,8 +135,8 @@ __device__ __forceinline__ void eval_cells(Slot (&sl)[2], int S, int nc, const W
     }
 }
 
-template <int NS, bool POT, bool MASK, bool GUARD>
-__device__ __forceinline__ void eval_bodies(Slot (&sl)[2], int S, int nb, const WaveLds& L, float e2) {
+template <int NS, bool POT, bool MASK, bool GUARD, int NX>
+__device__ __forceinline__ void eval_bodies(Slot (&sl)[NX], int S, int nb, const WaveLds& L, float e2) {
     const int steps = (nb + S - 1) / S;
     for (int t = 0; t < steps; ++t) {
 #pragma unroll
@@ -154,7 +155,10 @@ __device__ __forceinline__ int stack_threshold(int depth) { return kStackCap - 7
 template <bool POT, bool GUARD, bool REL>
 __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
     constexpr bool MASK = POT || GUARD;
-    static_assert(sizeof(WaveLds) >= 128 * sizeof(D4), "share reduction buffer");
+    // Slots per lane at most: 3 in the plain form (4 would spill at 4 waves per SIMD), 2 in the mixed
+    // form (a third slot with its FP64 carries takes it past 128 VGPRs).
+    constexpr int NX = REL ? 2 : 3;
+    static_assert(sizeof(WaveLds) >= 64 * NX * sizeof(D4), "share reduction buffer");
     __shared__ WaveLds lds[kWalkWaves];
     __shared__ int stk[kWalkWaves][kStackCap];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -178,13 +182,18 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
             const int nbody = chi - clo;  // 1..128, wave-uniform
             // Slots per lane and list shares: the pair (NS, S) that keeps the most lanes busy,
             // S * nbody of the 64 * NS slots (NS = 2 wins when it more than doubles S).
-            const int S1 = min(kMaxShare, 64 / nbody), S2 = min(kMaxShare, 128 / nbody);
-            const int NS = S2 > 2 * S1 ? 2 : 1;
-            const int S = NS == 2 ? S2 : S1;
-            Slot sl[2];
+            // Slots per lane NS (up to NX) and list shares S = min(16, 64 NS / nbody): the pair
+            // that keeps the most slots busy (nbody S of 64 NS), the fewest slots on a tie.
+            int NS = 1, S = min(kMaxShare, 64 / nbody);
+#pragma unroll
+            for (int ns = 2; ns <= NX; ++ns) {
+                const int Sn = min(kMaxShare, 64 * ns / nbody);
+                if (Sn * NS > S * ns) { NS = ns; S = Sn; }
+            }
+            Slot sl[NX];
             float mnx = FLT_MAX, mny = FLT_MAX, mnz = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX, mxz = -FLT_MAX;
 #pragma unroll
-            for (int s = 0; s < 2; ++s) {
+            for (int s = 0; s < NX; ++s) {
                 const int v = s * 64 + lane;
                 sl[s].bi = v % nbody;
                 sl[s].sp = v / nbody;
@@ -214,7 +223,7 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
             const double ccx = bx, ccy = by, ccz = bz;
             if (REL) {
 #pragma unroll
-                for (int s = 0; s < 2; ++s) {
+                for (int s = 0; s < NX; ++s) {
                     const double4 d = a.posd[sl[s].i];
                     sl[s].x = float(d.x - ccx);
                     sl[s].y = float(d.y - ccy);
@@ -224,7 +233,7 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
             auto fold = [&]() {
                 if (REL) {
 #pragma unroll
-                    for (int s = 0; s < 2; ++s) {
+                    for (int s = 0; s < NX; ++s) {
                         sl[s].dx += double(sl[s].sx); sl[s].dy += double(sl[s].sy);
                         sl[s].dz += double(sl[s].sz); sl[s].dp += double(sl[s].sp_pot);
                         sl[s].sx = sl[s].sy = sl[s].sz = sl[s].sp_pot = 0.f;
@@ -241,7 +250,9 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                     L.c[nc + lane].C = make_float2(0.f, 0.f);
                 }
                 __builtin_amdgcn_wave_barrier();
-                if (NS == 2) eval_cells<2, POT, GUARD>(sl, S, nc, L, e2);
+                if (NX >= 4 && NS == 4) eval_cells<4, POT, GUARD>(sl, S, nc, L, e2);
+                else if (NX >= 3 && NS == 3) eval_cells<3, POT, GUARD>(sl, S, nc, L, e2);
+                else if (NS == 2) eval_cells<2, POT, GUARD>(sl, S, nc, L, e2);
                 else eval_cells<1, POT, GUARD>(sl, S, nc, L, e2);
                 fold();
                 __builtin_amdgcn_wave_barrier();
@@ -254,7 +265,9 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                     L.bI[nb + lane] = -1;
                 }
                 __builtin_amdgcn_wave_barrier();
-                if (NS == 2) eval_bodies<2, POT, MASK, GUARD>(sl, S, nb, L, e2);
+                if (NX >= 4 && NS == 4) eval_bodies<4, POT, MASK, GUARD>(sl, S, nb, L, e2);
+                else if (NX >= 3 && NS == 3) eval_bodies<3, POT, MASK, GUARD>(sl, S, nb, L, e2);
+                else if (NS == 2) eval_bodies<2, POT, MASK, GUARD>(sl, S, nb, L, e2);
                 else eval_bodies<1, POT, MASK, GUARD>(sl, S, nb, L, e2);
                 fold();
                 __builtin_amdgcn_wave_barrier();
@@ -397,10 +410,10 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
             // Add the list shares of each body (slot v = share * nbody + body) through LDS.
             const uint32_t listed = ncell_total + nbody_total;  // nbody_total includes the monopoles
             if (REL) {
-                D4* red = reinterpret_cast<D4*>(&L);  // the lists are done: their LDS holds 128 D4
+                D4* red = reinterpret_cast<D4*>(&L);  // the lists are done: their LDS holds 64 NX D4
                 __builtin_amdgcn_wave_barrier();
 #pragma unroll
-                for (int s = 0; s < 2; ++s)
+                for (int s = 0; s < NX; ++s)
                     if (sl[s].on) red[sl[s].sp * nbody + sl[s].bi] = D4{sl[s].dx, sl[s].dy, sl[s].dz, sl[s].dp};
                 __builtin_amdgcn_wave_barrier();
                 const double G = a.Gd;
@@ -421,10 +434,10 @@ __global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
                     if (a.cost) a.cost[i] = float(listed - 1u);
                 }
             } else {
-                float4* red = L.bP;  // kBodyCap >= 128 = 64 * NS entries
+                float4* red = reinterpret_cast<float4*>(&L);  // the lists are done: 64 NX float4
                 __builtin_amdgcn_wave_barrier();
 #pragma unroll
-                for (int s = 0; s < 2; ++s)
+                for (int s = 0; s < NX; ++s)
                     if (sl[s].on)
                         red[sl[s].sp * nbody + sl[s].bi] = make_float4(sl[s].sx, sl[s].sy, sl[s].sz, sl[s].sp_pot);
                 __builtin_amdgcn_wave_barrier();
