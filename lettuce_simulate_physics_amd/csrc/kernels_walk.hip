@@ -18,8 +18,8 @@
 //
 // Evaluation. Accepted cells are staged in LDS with their moments, bodies of opened leaves with
 // their positions. Walk groups hold 1-128 bodies (39 on average for a Plummer sphere), so one body
-// per lane would idle most lanes. Instead the wave has V = 64 * NS virtual slots (NS = 1-3 per lane
-// in the plain form, 1-2 in the mixed form, whichever keeps most slots busy); with
+// per lane would idle most lanes. Instead the wave has V = 64 * NS virtual slots (NS = 1-3 per lane,
+// 1-2 with the potential or the self test, whichever keeps most slots busy); with
 // S = min(16, floor(V / nb)) list shares, slot v
 // evaluates body v mod nb against list entries e = v div nb (mod S). Partial sums of the S shares are added in
 // LDS at the end. Lists are evaluated whenever the next append would overflow them. Before an
@@ -153,11 +153,11 @@ constexpr int kStackCap = 896;
 __device__ __forceinline__ int stack_threshold(int depth) { return kStackCap - 7 * (max(depth, kMaxLevel) + 1) - 8; }
 
 template <bool POT, bool GUARD, bool REL>
-__global__ void __launch_bounds__(kWalkBS) walk_kernel(WalkArgs a) {
+__global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
     constexpr bool MASK = POT || GUARD;
-    // Slots per lane at most: 3 in the plain form (4 would spill at 4 waves per SIMD), 2 in the mixed
-    // form (a third slot with its FP64 carries takes it past 128 VGPRs).
-    constexpr int NX = REL ? 2 : 3;
+    // Slots per lane at most: 3 (4 would spill at 4 waves per SIMD, the occupancy the bounds ask
+    // for); 2 with the potential or the self test, whose extra registers would spill a third.
+    constexpr int NX = MASK ? 2 : 3;
     static_assert(sizeof(WaveLds) >= 64 * NX * sizeof(D4), "share reduction buffer");
     __shared__ WaveLds lds[kWalkWaves];
     __shared__ int stk[kWalkWaves][kStackCap];
