@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
     // Slots per lane at most: 3 (4 would spill at 4 waves per SIMD, the occupancy the bounds ask
     // for); 2 with the potential or the self test, whose extra registers would spill a third.
     constexpr int NX = MASK ? 2 : 3;
-    static_assert(sizeof(WaveLds) >= 64 * NX * sizeof(D4), "share reduction buffer");
+    static_assert(sizeof(WaveLds) >= 64 * NX * (REL ? sizeof(D4) : sizeof(float4)), "share reduction buffer");
     __shared__ WaveLds lds[kWalkWaves];
     __shared__ int stk[kWalkWaves][kStackCap];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
