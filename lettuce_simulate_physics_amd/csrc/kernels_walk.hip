@@ -289,14 +289,24 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                 // Mixed form: the FP64 com and mass (what a monopole or a cell stages) and qrel (the
                 // far-field decision) load together with the node, so that deciding and staging
                 // wait for one round trip.
+                // Plain form: the mass, quadrupole and qrel (the com is the node record's) likewise.
                 double4 pc = make_double4(0.0, 0.0, 0.0, 0.0);
-                float qr = 0.f;
+                float qr = 0.f, pm = 0.f;
+                float4 pq = make_float4(0.f, 0.f, 0.f, 0.f);
+                float2 pq2 = make_float2(0.f, 0.f);
                 if (act) {
                     nd = a.node[k];
                     if (REL) {
                         const NodeMomD& dm = a.momd[k];
                         pc = make_double4(dm.cx, dm.cy, dm.cz, dm.m);
                         if (a.quad_tau > 0.f) qr = a.mom[k].qrel;
+                    } else {
+                        const float4* r = reinterpret_cast<const float4*>(a.mom + k);
+                        const float4 r2 = r[2];
+                        pm = r[0].w;
+                        pq = r[1];
+                        pq2 = make_float2(r2.x, r2.y);
+                        qr = r2.w;
                     }
                 }
                 // A node holding bodies of this (sub)group is always opened: the reference's
@@ -313,7 +323,7 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                 bool is_cell = acc && !small;
                 // Error-budgeted far field: an accepted cell whose quadrupole term is below quad_tau
                 // of its monopole at the group box's distance enters the body list as a point mass.
-                const bool is_mono = is_cell && a.quad_tau > 0.f && (REL ? qr : a.mom[k].qrel) < a.quad_tau * d2;
+                const bool is_mono = is_cell && a.quad_tau > 0.f && qr < a.quad_tau * d2;
                 is_cell = is_cell && !is_mono;
                 const bool is_range = leaf && (!acc || small);
                 const bool open_int = act && !acc && !leaf;
@@ -335,15 +345,17 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                     if (nc + ncnew > kCellCap) flush_cells();
                     if (is_cell) {
                         const int slot = nc + int(lanes_below(mc));
-                        const NodeMom& md = a.mom[k];
                         if (REL) {
+                            const NodeMom& md = a.mom[k];
                             L.c[slot].A = make_float4(float(pc.x - ccx), float(pc.y - ccy), float(pc.z - ccz),
                                                       float(pc.w));
+                            L.c[slot].B = make_float4(md.qxx, md.qxy, md.qxz, md.qyy);
+                            L.c[slot].C = make_float2(md.qyz, md.qzz);
                         } else {
-                            L.c[slot].A = make_float4(md.cx, md.cy, md.cz, md.m);
+                            L.c[slot].A = make_float4(nd.cx, nd.cy, nd.cz, pm);
+                            L.c[slot].B = pq;
+                            L.c[slot].C = pq2;
                         }
-                        L.c[slot].B = make_float4(md.qxx, md.qxy, md.qxz, md.qyy);
-                        L.c[slot].C = make_float2(md.qyz, md.qzz);
                     }
                     nc += ncnew;
                 }
@@ -357,8 +369,7 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                             L.bP[slot] = make_float4(float(pc.x - ccx), float(pc.y - ccy), float(pc.z - ccz),
                                                      float(pc.w));
                         } else {
-                            const NodeMom& md = a.mom[k];
-                            L.bP[slot] = make_float4(md.cx, md.cy, md.cz, md.m);
+                            L.bP[slot] = make_float4(nd.cx, nd.cy, nd.cz, pm);
                         }
                         L.bI[slot] = -1;
                     }
