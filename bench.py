@@ -225,19 +225,20 @@ def tree_adjoint_leg(be, nstars: int, steps: int):
     theta 0.6 (mixed form, its default walk options), eps 0.02, KDK dt 1e-3, checkpoints every 20
     steps. Loss: squared distance of disk 2's final centre of mass from a seeded target point.
     Times one forward and one backward (device events); gradients in (pericentre, incl1, incl2).
-    Cross-check: the same problem at 2 x 16384 stars through the tree and through the direct-sum
-    adjoint, the largest gradient difference relative to the largest component."""
+    Also with a checkpoint every step (no recompute). Cross-check: the same problem at 2 x 16384 stars
+    through the tree and through the direct-sum adjoint, the largest gradient difference relative to
+    the largest component."""
     import lettuce_simulate_physics_amd as L
     from lettuce_simulate_physics_amd.autograd import SimParams, Simulation
 
-    def run(ns, method, k):
+    def run(ns, method, k, every=20):
         C = L.Collision(nstars=ns, seed=1, pericentre=4.0, incl1=np.pi / 6, incl2=np.pi / 3)
         x0, v0, m = C.generate()
         n1 = ns + 1
         w = np.zeros(len(m))
         w[n1:] = m[n1:] / m[n1:].sum()
         target = np.random.default_rng(2027).standard_normal(3)
-        sim = Simulation(be, SimParams(method=method, theta=0.6, eps=0.02, dt=1e-3, steps=k, checkpoint_every=20))
+        sim = Simulation(be, SimParams(method=method, theta=0.6, eps=0.02, dt=1e-3, steps=k, checkpoint_every=every))
         xT, vT = sim.forward(x0, v0, m)
         fwd_ms, _ = sim.times()
         com = (w[:, None] * xT).sum(0)
@@ -246,6 +247,8 @@ def tree_adjoint_leg(be, nstars: int, steps: int):
         return fwd_ms, bwd_ms, float(((com - target) ** 2).sum()), C.vjp(gx, gv)
 
     fwd_ms, bwd_ms, loss, g = run(nstars, "tree", steps)
+    # the same with a checkpoint every step (no recompute: HBM for time)
+    fwd1, bwd1, _, g1 = run(nstars, "tree", steps, 1)
     _, _, _, gt = run(16384, "tree", 100)
     _, _, _, gd = run(16384, "direct", 100)
     be.set_precision("plain")
@@ -253,6 +256,10 @@ def tree_adjoint_leg(be, nstars: int, steps: int):
             "forward_ms_per_step": round(fwd_ms / steps, 4), "adjoint_ms_per_step": round(bwd_ms / steps, 4),
             "adjoint_forward_ratio": round(bwd_ms / fwd_ms, 3), "loss": loss,
             "grad_pericentre": float(g[0]), "grad_incl1": float(g[1]), "grad_incl2": float(g[2]),
+            "checkpoint_every_step": {"forward_ms_per_step": round(fwd1 / steps, 4),
+                                      "adjoint_ms_per_step": round(bwd1 / steps, 4),
+                                      "adjoint_forward_ratio": round(bwd1 / fwd1, 3),
+                                      "max_grad_diff": float(np.abs(g1 - g).max())},
             "cross_check_2x16384_100_steps": {
                 "tree": [float(v) for v in gt], "direct": [float(v) for v in gd],
                 "max_diff_over_largest": float(np.abs(gt - gd).max() / np.abs(gd).max())}}
