@@ -58,7 +58,7 @@ struct Target {
     double gx, gy, gz;          // FP64 totals of the target-side terms
 };
 
-__global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
+__global__ void __launch_bounds__(kBS4, 4) tree_vjp_kernel(TreeVjpArgs a) {
     __shared__ VjpLds lds[kWaves];
     __shared__ int stk[kWaves][kStack];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -305,10 +305,13 @@ __global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
                 __builtin_amdgcn_wave_barrier();
                 top -= P;
                 WalkNode nd{0.f, 0.f, 0.f, 0.f, 0, 0, 0, -1};
-                // qrel (the far-field decision) loads with the node.
+                // The FP64 com and mass and qrel load with the node (one round trip), as in the walk.
+                double4 pc = make_double4(0.0, 0.0, 0.0, 0.0);
                 float qr = 0.f;
                 if (act) {
                     nd = a.node[k];
+                    const NodeMomD& dm0 = a.momd[k];
+                    pc = make_double4(dm0.cx, dm0.cy, dm0.cz, dm0.m);
                     if (a.quad_tau > 0.f) qr = a.mom[k].qrel;
                 }
                 const bool keep_open = chunked ? (nd.lo < chi && clo < nd.hi) : (nd.lo <= clo && chi <= nd.hi);
@@ -342,8 +345,7 @@ __global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
                     if (is_cell) {
                         const int slot = nc + int(lanes_below(mc));
                         const NodeMom& md = a.mom[k];
-                        const NodeMomD& dm = a.momd[k];
-                        L.cA[slot] = make_float4(float(dm.cx - ccx), float(dm.cy - ccy), float(dm.cz - ccz), md.m);
+                        L.cA[slot] = make_float4(float(pc.x - ccx), float(pc.y - ccy), float(pc.z - ccz), float(pc.w));
                         L.cB[slot] = make_float4(md.qxx, md.qxy, md.qxz, md.qyy);
                         L.cC[slot] = make_float2(md.qyz, md.qzz);
                         L.cI[slot] = k;
@@ -356,9 +358,8 @@ __global__ void __launch_bounds__(kBS4) tree_vjp_kernel(TreeVjpArgs a) {
                     if (nb + nmnew > kBodies) flush_bodies();
                     if (is_mono) {
                         const int slot = nb + int(lanes_below(mm));
-                        const NodeMomD& dm = a.momd[k];
-                        L.bP[slot] = make_float4(float(dm.cx - ccx), float(dm.cy - ccy), float(dm.cz - ccz),
-                                                 float(dm.m));
+                        L.bP[slot] = make_float4(float(pc.x - ccx), float(pc.y - ccy), float(pc.z - ccz),
+                                                 float(pc.w));
                         L.bI[slot] = -(k + 2);
                     }
                     nb += nmnew;
