@@ -497,8 +497,8 @@ struct Context {
             if (!any) break;  // every over-full cell's bodies coincide: they stay leaves
             uint64_t* fk = fkey.as<uint64_t>() + std::size_t(f - 1) * N;
             uint32_t* fs = fseg.as<uint32_t>() + std::size_t(f - 1) * N;
-            tree_restart_keys(stream, b, pos(0), pos(1), pos(2), mixed, rs_seg.as<uint32_t>(), cubes, cubes_used, f,
-                              fk, fs, capb.as<uint32_t>(), rs_gkey.as<uint64_t>(), rs_cstart.as<uint32_t>());
+            tree_restart_keys(stream, b, pos(0), pos(1), pos(2), mixed, rs_seg.as<uint32_t>(), cubes, cubes_used, fk,
+                              fs, capb.as<uint32_t>(), rs_gkey.as<uint64_t>(), rs_cstart.as<uint32_t>());
             cubes_used += nc;
             frames = f + 1;
             if (permute) {

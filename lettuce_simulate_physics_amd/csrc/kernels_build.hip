@@ -702,9 +702,8 @@ void tree_restart_cubes(hipStream_t s, const BuildBufs& b, const void* x, const 
 }
 
 void tree_restart_keys(hipStream_t s, const BuildBufs& b, const void* x, const void* y, const void* z, bool f64,
-                       const uint32_t* seg, const double4* cubes, int cube0, int frame, uint64_t* fkey_frame,
+                       const uint32_t* seg, const double4* cubes, int cube0, uint64_t* fkey_frame,
                        uint32_t* fseg_frame, uint32_t* cap, uint64_t* gkey, uint32_t* cstart) {
-    (void)frame;
     const int n = b.n;
     restart_starts_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(n, seg, cstart);
     LHIP_LAUNCH_CHECK();

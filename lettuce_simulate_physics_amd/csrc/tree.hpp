@@ -102,11 +102,11 @@ int tree_node_count(hipStream_t s, const BuildBufs& b);
 // (double if f64). bbox: 6 x ncell scratch.
 void tree_restart_cubes(hipStream_t s, const BuildBufs& b, const void* x, const void* y, const void* z, bool f64,
                         const uint32_t* seg, int ncell, unsigned long long* bbox, double4* cubes);
-// Frame-`frame` keys of the bodies of restarted cells (cube index cube0 + cell), their caps and
+// The next frame's keys of the bodies of restarted cells (cube index cube0 + cell), their caps and
 // cells; gkey[i] = the position of the first body of i's restarted cell, else i (the grouping key of
 // the stable re-sort); cstart: ncell scratch. fkey/fseg of the frame are written for every body.
 void tree_restart_keys(hipStream_t s, const BuildBufs& b, const void* x, const void* y, const void* z, bool f64,
-                       const uint32_t* seg, const double4* cubes, int cube0, int frame, uint64_t* fkey_frame,
+                       const uint32_t* seg, const double4* cubes, int cube0, uint64_t* fkey_frame,
                        uint32_t* fseg_frame, uint32_t* cap, uint64_t* gkey, uint32_t* cstart);
 // out[i] = in[perm[i]] (u64 and u32 arrays of n).
 void gather_u64(hipStream_t s, const uint32_t* perm, const uint64_t* in, uint64_t* out, int n);
