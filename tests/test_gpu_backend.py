@@ -119,6 +119,77 @@ def test_tree_coincident_bodies(hip):
     assert abs(d1["pe"] / pe - 1) <= 2e-4
 
 
+def with_escaper(b, x=1e8):
+    """b plus one body at (x, 0, 0) at rest with b's first mass (the reference tests' push)."""
+    n = len(b)
+    e = O.Bodies.empty(n + 1)
+    for src, dst in zip(b.arrays(), e.arrays()):
+        dst[:n] = src
+    e.x[n], e.y[n], e.z[n] = np.float32(x), 0.0, 0.0
+    e.vx[n] = e.vy[n] = e.vz[n] = 0.0
+    e.m[n] = b.m[0]
+    e.id[n] = n
+    return e
+
+
+def test_tree_far_escaper_vs_direct(hip):
+    # tests/test_cuda_backend.cpp:489-536: one body at 1e8 puts the whole cluster into one finest
+    # cell; the build restarts its keys (src/cpu_backend.cpp:501-544), so the walk stays O(n log n).
+    n = 20000
+    base = O.plummer(n, 10, 3 * math.pi / 16, 20.0)
+    b = with_escaper(base)
+    eps = 0.01
+    p = P("tree", eps, 0.6)
+    o = O.Oracle()
+    ref = o.accel_by_id(b, 1.0, eps, 0.6, 0, True)
+    o.upload(b)
+    o.compute_forces(1.0, eps, 0.6, 0, True)
+    ref_pe = o.diagnostics()["pe"]
+    _, _, plain = got(hip, base, p, False)
+    a, d, t = got(hip, b, p, True)
+    e = rel_rms(a, ref)
+    print(f"escaper: rel rms {e:.3g}, pe-1 {d['pe'] / ref_pe - 1:.3g}, {t['interactions'] / (n + 1):.1f} int/body "
+          f"(without {plain['interactions'] / n:.1f}), build {t['build_ms']:.3f} ms")
+    assert np.all(np.isfinite(a))
+    assert e <= 1e-3
+    assert abs(d["pe"] / ref_pe - 1) <= 1e-3
+    assert t["interactions"] <= 2 * plain["interactions"] + 2 * (n + 1)
+    # a few steps through restarted builds keep every body exactly once
+    hip.upload(b)
+    hip.compute_forces(p, False)
+    for _ in range(3):
+        hip.kick(0.005)
+        hip.drift(0.01)
+        hip.compute_forces(p, False)
+        hip.kick(0.005)
+    out = hip.download()
+    assert len(out) == len(b)
+    assert np.array_equal(np.sort(out.id), np.arange(len(b), dtype=np.uint32))
+    assert np.all(np.isfinite(out.x)) and np.all(out.m == b.m[out.id])
+
+
+def test_tree_theta0_exact_with_escaper_and_coincident(hip):
+    # tests/test_cuda_backend.cpp:538-574
+    b = cube(3000, 11)
+    b.x[:100], b.y[:100], b.z[:100] = 0.25, -0.5, 0.125
+    b = with_escaper(b)
+    N = len(b)
+    eps = 0.05
+    ref, pe = brute(b, eps)
+    for theta in (0.0, 0.6):
+        a, d, t = got(hip, b, P("tree", eps, theta), True)
+        e = rel_rms(a, ref)
+        print(f"theta {theta}: rel rms {e:.3g}, pe-1 {d['pe'] / pe - 1:.3g}, interactions {t['interactions']}")
+        assert np.all(np.isfinite(a))
+        if theta == 0.0:
+            assert e <= 1e-5
+            assert t["interactions"] == N * (N - 1)
+            assert abs(d["pe"] / pe - 1) <= 1e-5
+        else:
+            assert e <= 1e-2
+            assert abs(d["pe"] / pe - 1) <= 2e-4
+
+
 def test_tree_vs_direct_20k(hip):
     b = O.plummer(20000, 2, 3 * math.pi / 16, 20.0)
     ad, dd, _ = got(hip, b, P("direct", 0.01), True)
