@@ -115,6 +115,48 @@ int lhip_build_tree(lhip_ctx* ctx, const lhip_force_params* p);
 /* Backend::compute_forces_with with an empty remote list (the single-rank case). LHIP_LOGIC for
  * the tree method without a current tree (src/cuda_backend.cu:1399-1400). */
 int lhip_compute_forces_with(lhip_ctx* ctx, const lhip_force_params* p, int potential);
+
+/* LetNode (src/backend.hpp:63-76): one node of a locally essential tree (LET), 60 bytes, the
+ * backend-neutral wire format LETs travel in between ranks. Children of a node are contiguous at
+ * first_child (-1: none), its bodies at body_first (-1: none) in the tree's body arrays. A node with
+ * neither is terminal (accepted by any target in the box it was exported for). rcrit2 = 3.0e38:
+ * always opened. */
+typedef struct lhip_let_node {
+    float cx, cy, cz, mass;
+    float qxx, qxy, qxz, qyy, qyz, qzz;
+    float rcrit2;
+    int32_t first_child, child_count, body_first, body_count;
+} lhip_let_node;
+
+/* LetTree (src/backend.hpp:84-87) as plain arrays: n_nodes records, n_bodies bodies (SoA). */
+typedef struct lhip_let {
+    const lhip_let_node* nodes;
+    uint64_t n_nodes;
+    const float *bx, *by, *bz, *bm;
+    uint64_t n_bodies;
+} lhip_let;
+
+/* Backend::compute_forces_with (src/backend.hpp:131-136) with remote LETs: the forces of the local
+ * bodies plus every remote tree. Each walk group also walks every remote tree with the receiver
+ * rules of src/backend.hpp:78-83 (bodies enter with no self term); the direct method walks them
+ * per block of 128 consecutive bodies (src/cpu_backend.cpp:691-713). Costs and interaction counts
+ * include the remote entries. The trees are validated before they reach the device: every body
+ * range in bounds, children after their parent (as both reference exporters emit them) and in
+ * bounds, at most 2^31 - 1 nodes and bodies in all, at most 33554431 bodies per node; LHIP_INVALID
+ * otherwise. LHIP_RUNTIME if the remote trees are too deep for the walk's stack. Empty trees
+ * (n_nodes = 0) are skipped. */
+int lhip_compute_forces_remote(lhip_ctx* ctx, const lhip_force_params* p, int potential, const lhip_let* remote,
+                               int n_remote);
+/* Backend::export_let (src/backend.hpp:125-130): the part of the local tree a target anywhere in
+ * the box [lo, hi] needs at p->theta, built on the device. Returns its sizes; lhip_export_let_copy
+ * then fills the caller's arrays (n_nodes records, n_bodies entries per body array). Tree method:
+ * needs a current tree (build_tree after the last upload or drift; LHIP_LOGIC otherwise). Direct
+ * method: one always-open root holding every local body in the current order. n = 0 or a root of
+ * zero mass: an empty LET. */
+int lhip_export_let(lhip_ctx* ctx, const float lo[3], const float hi[3], const lhip_force_params* p,
+                    uint64_t* n_nodes, uint64_t* n_bodies);
+/* Copies the LET of the last lhip_export_let; LHIP_LOGIC if there is none. */
+int lhip_export_let_copy(lhip_ctx* ctx, lhip_let_node* nodes, float* bx, float* by, float* bz, float* bm);
 /* Backend::memory_need (device and host bytes for n bodies). */
 int lhip_memory_need(uint64_t n, int32_t method, uint64_t* device_bytes, uint64_t* host_bytes);
 /* Backend::device_memory: free (including what this context holds) and total device bytes. */

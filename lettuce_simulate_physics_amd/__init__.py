@@ -228,6 +228,35 @@ class Collision:
         return out
 
 
+# LetNode (src/backend.hpp:63-76): the 60-byte wire record of a locally essential tree.
+LET_NODE = np.dtype([(k, "<f4") for k in ("cx", "cy", "cz", "mass", "qxx", "qxy", "qxz", "qyy", "qyz", "qzz",
+                                          "rcrit2")] +
+                    [(k, "<i4") for k in ("first_child", "child_count", "body_first", "body_count")])
+assert LET_NODE.itemsize == 60
+
+
+@dataclass
+class LetTree:
+    """LetTree (src/backend.hpp:84-87): LET_NODE records and the exported bodies (float SoA)."""
+    nodes: np.ndarray
+    bx: np.ndarray
+    by: np.ndarray
+    bz: np.ndarray
+    bm: np.ndarray
+
+    @staticmethod
+    def empty() -> "LetTree":
+        return LetTree(np.zeros(0, LET_NODE), *[np.zeros(0, np.float32) for _ in range(4)])
+
+    def __len__(self):
+        return len(self.nodes)
+
+
+class LetC(ctypes.Structure):
+    _fields_ = [("nodes", _P), ("n_nodes", ctypes.c_uint64), ("bx", _P), ("by", _P), ("bz", _P), ("bm", _P),
+                ("n_bodies", ctypes.c_uint64)]
+
+
 def hip_compiled() -> bool:
     try:
         return bool(library().lhip_compiled())
@@ -340,10 +369,32 @@ class HipBackend:
         self._check(self._L.lhip_build_tree(self._h, ctypes.byref(c)))
 
     def compute_forces_with(self, p: ForceParams, potential: bool = False, remote=()) -> None:
-        if remote:
-            raise NotImplementedError("remote locally essential trees (multi-rank) are not supported on one GPU")
+        """Forces from the local bodies plus every remote LetTree (src/backend.hpp:131-136)."""
         c = p.c()
-        self._check(self._L.lhip_compute_forces_with(self._h, ctypes.byref(c), int(potential)))
+        if not remote:
+            self._check(self._L.lhip_compute_forces_with(self._h, ctypes.byref(c), int(potential)))
+            return
+        keep, lets = [], (LetC * len(remote))()
+        for k, t in enumerate(remote):
+            nodes = np.ascontiguousarray(t.nodes, LET_NODE)
+            cols = [np.ascontiguousarray(v, np.float32) for v in (t.bx, t.by, t.bz, t.bm)]
+            if len({len(v) for v in cols}) != 1:
+                raise ValueError("LET body arrays differ in length")
+            keep += [nodes, *cols]
+            lets[k] = LetC(_ptr(nodes), len(nodes), *[_ptr(v) for v in cols], len(cols[0]))
+        self._check(self._L.lhip_compute_forces_remote(self._h, ctypes.byref(c), int(potential), lets, len(remote)))
+
+    def export_let(self, box, p: ForceParams) -> LetTree:
+        """The part of the local tree a target anywhere in box = (lo, hi) needs (src/backend.hpp:125-130)."""
+        lo = (ctypes.c_float * 3)(*[float(v) for v in box[0]])
+        hi = (ctypes.c_float * 3)(*[float(v) for v in box[1]])
+        c = p.c()
+        nn, nb = _U64(), _U64()
+        self._check(self._L.lhip_export_let(self._h, lo, hi, ctypes.byref(c), ctypes.byref(nn), ctypes.byref(nb)))
+        t = LetTree(np.zeros(nn.value, LET_NODE), *[np.zeros(nb.value, np.float32) for _ in range(4)])
+        self._check(self._L.lhip_export_let_copy(self._h, _ptr(t.nodes), _ptr(t.bx), _ptr(t.by), _ptr(t.bz),
+                                                 _ptr(t.bm)))
+        return t
 
     @staticmethod
     def memory_need(n: int, method: str):

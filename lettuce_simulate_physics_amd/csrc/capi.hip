@@ -131,6 +131,22 @@ int lhip_compute_forces_with(lhip_ctx* ctx, const lhip_force_params* p, int pote
     return guarded(ctx, [&](lhip::Context& c) { c.compute_forces_with(*p, potential != 0); });
 }
 
+int lhip_compute_forces_remote(lhip_ctx* ctx, const lhip_force_params* p, int potential, const lhip_let* remote,
+                               int n_remote) {
+    if (!p || n_remote < 0 || (n_remote > 0 && !remote)) return LHIP_INVALID;
+    return guarded(ctx, [&](lhip::Context& c) { c.compute_forces_with(*p, potential != 0, remote, n_remote); });
+}
+
+int lhip_export_let(lhip_ctx* ctx, const float lo[3], const float hi[3], const lhip_force_params* p,
+                    uint64_t* n_nodes, uint64_t* n_bodies) {
+    if (!lo || !hi || !p || !n_nodes || !n_bodies) return LHIP_INVALID;
+    return guarded(ctx, [&](lhip::Context& c) { c.export_let(lo, hi, *p, n_nodes, n_bodies); });
+}
+
+int lhip_export_let_copy(lhip_ctx* ctx, lhip_let_node* nodes, float* bx, float* by, float* bz, float* bm) {
+    return guarded(ctx, [&](lhip::Context& c) { c.export_let_copy(nodes, bx, by, bz, bm); });
+}
+
 int lhip_memory_need(uint64_t n, int32_t method, uint64_t* device_bytes, uint64_t* host_bytes) {
     if (!device_bytes || !host_bytes) return LHIP_INVALID;
     lhip::memory_need(n, method, device_bytes, host_bytes);

@@ -602,7 +602,141 @@ struct Context {
     const uint32_t* walk_starts() const { return pack_max > 0 ? pstart.as<uint32_t>() : gstart.as<uint32_t>(); }
     const uint32_t* walk_count() const { return pack_max > 0 ? pflag.as<uint32_t>() : gflag.as<uint32_t>(); }
 
-    void compute_forces_with(const lhip_force_params& p, bool potential) {
+    // The walk's arguments for the current bodies: the local tree (tree method) and the outputs.
+    WalkArgs walk_args(const lhip_force_params& p) {
+        WalkArgs a{};
+        a.posm = posm.as<float4>();
+        a.node = node.as<WalkNode>();
+        a.mom = mom.as<NodeMom>();
+        a.gstart = walk_starts();
+        a.gcount = walk_count();
+        a.n = n;
+        a.eps2 = float(p.eps * p.eps);
+        a.G = float(p.G);
+        a.ax = ax.as<float>(); a.ay = ay.as<float>(); a.az = az.as<float>(); a.pot = pot.as<float>();
+        a.cost = cost.as<float>();
+        if (mixed) {
+            a.posd = posd.as<double4>();
+            a.posd_box = posd_box ? posd_box : a.posd;
+            a.momd = momd.as<NodeMomD>();
+            a.axd = D(axd); a.ayd = D(ayd); a.azd = D(azd); a.potd = D(potd);
+            a.Gd = p.G;
+        }
+        a.counters = counters.as<WalkCounters>();
+        a.blocks = walk_blocks;
+        a.small_leaf = wopt.small_leaf;
+        a.quad_tau = float(wopt.quad_tau);
+        a.depth = tree_depth(frames);
+        return a;
+    }
+
+    // ---- remote trees (LETs, src/backend.hpp:63-87)
+    DevBuf let_rec, let_body, let_root;
+    std::vector<LetRec> let_hrec;
+    std::vector<float4> let_hbody;
+    std::vector<int> let_hroot;
+    int let_thr = 0, let_fan = 1;
+    // Most bodies one remote node may hold: 64 of them (one walk step) stay below 2^31.
+    static constexpr int32_t kLetMaxBodies = INT32_MAX / 64;
+
+    // Validates the remote trees, converts them to device records (indices offset into the
+    // concatenated arrays) and uploads them. Returns the number of non-empty trees. Children must
+    // follow their parent (both reference exporters emit breadth-first): no cycles, and one forward
+    // pass gives every node's depth, which with the largest child count bounds the walk's stack.
+    int upload_remote(const lhip_let* remote, int nremote) {
+        if (nremote < 0 || (nremote > 0 && !remote)) throw Error(kInvalid, "hip compute_forces_with: bad remote list");
+        uint64_t nn = 0, nbt = 0;
+        for (int t = 0; t < nremote; ++t) {
+            if (remote[t].n_nodes == 0) continue;
+            nn += remote[t].n_nodes;
+            nbt += remote[t].n_bodies;
+        }
+        if (nn == 0) return 0;
+        if (nn >= uint64_t(INT32_MAX) || nbt >= uint64_t(INT32_MAX))
+            throw Error(kInvalid, "hip compute_forces_with: remote LETs exceed 2^31 - 1 nodes or bodies");
+        let_hrec.resize(nn);
+        let_hbody.resize(nbt);
+        let_hroot.clear();
+        std::vector<int> depth;
+        int height = 0, fan = 1;
+        uint64_t off = 0, boff = 0;
+        for (int t = 0; t < nremote; ++t) {
+            const lhip_let& L = remote[t];
+            if (L.n_nodes == 0) continue;
+            const uint64_t m = L.n_nodes, nb = L.n_bodies;
+            if (!L.nodes || (nb > 0 && (!L.bx || !L.by || !L.bz || !L.bm)))
+                throw Error(kInvalid, "hip compute_forces_with: remote LET with null arrays");
+            depth.assign(m, 0);
+            let_hroot.push_back(int(off));
+            for (uint64_t i = 0; i < m; ++i) {
+                const lhip_let_node& d = L.nodes[i];
+                LetRec r;
+                r.A = make_float4(d.cx, d.cy, d.cz, d.mass);
+                r.B = make_float4(d.qxx, d.qxy, d.qxz, d.qyy);
+                const double qf = std::sqrt(double(d.qxx) * d.qxx + double(d.qyy) * d.qyy + double(d.qzz) * d.qzz +
+                                            2.0 * (double(d.qxy) * d.qxy + double(d.qxz) * d.qxz + double(d.qyz) * d.qyz));
+                r.C = make_float4(d.qyz, d.qzz, d.rcrit2, d.mass > 0.f ? float(1.5 * qf / d.mass) : 0.f);
+                int32_t bf = 0;
+                if (d.body_count > 0) {
+                    if (d.body_first < 0 || uint64_t(d.body_first) + uint64_t(d.body_count) > nb)
+                        throw Error(kInvalid, "hip compute_forces_with: remote LET node body range out of bounds");
+                    if (d.body_count > kLetMaxBodies)
+                        throw Error(kInvalid, "hip compute_forces_with: remote LET node holds more than " +
+                                                  std::to_string(kLetMaxBodies) + " bodies");
+                    bf = int32_t(boff + uint64_t(d.body_first));
+                }
+                // first child: -1 only where the record has none (with no bodies: terminal).
+                int32_t fc = d.first_child < 0 ? -1 : 0, cc = 0;
+                if (d.body_count <= 0 && d.child_count > 0) {
+                    if (d.first_child < 0 || uint64_t(d.first_child) <= i ||
+                        uint64_t(d.first_child) + uint64_t(d.child_count) > m)
+                        throw Error(kInvalid, "hip compute_forces_with: remote LET node children out of order or bounds");
+                    fc = int32_t(off + uint64_t(d.first_child));
+                    cc = d.child_count;
+                    fan = std::max(fan, int(cc));
+                    for (int32_t c = d.first_child; c < d.first_child + d.child_count; ++c)
+                        depth[std::size_t(c)] = std::max(depth[std::size_t(c)], depth[i] + 1);
+                }
+                height = std::max(height, depth[i]);
+                r.D = make_int4(fc, cc, bf, d.body_count);
+                let_hrec[off + i] = r;
+            }
+            for (uint64_t j = 0; j < nb; ++j) let_hbody[boff + j] = make_float4(L.bx[j], L.by[j], L.bz[j], L.bm[j]);
+            off += m;
+            boff += nb;
+        }
+        // A step popping P nodes pushes at most fan P; one node at a time (above the threshold) the
+        // stack grows by at most (fan - 1) per level below the top node.
+        const long long thr = kWalkStack - (long long)fan * (height + 1);
+        if (thr < 64)
+            throw Error(kRuntime, "hip compute_forces_with: remote LETs too deep for the walk stack (height " +
+                                      std::to_string(height) + ", up to " + std::to_string(fan) + " children)");
+        let_thr = int(thr);
+        let_fan = fan;
+        let_rec.reserve(nn * sizeof(LetRec));
+        let_body.reserve(std::max<uint64_t>(nbt, 1) * sizeof(float4));
+        let_root.reserve(let_hroot.size() * sizeof(int));
+        LHIP_CHECK(hipMemcpyAsync(let_rec.as<void>(), let_hrec.data(), nn * sizeof(LetRec), hipMemcpyHostToDevice, stream));
+        if (nbt)
+            LHIP_CHECK(hipMemcpyAsync(let_body.as<void>(), let_hbody.data(), nbt * sizeof(float4), hipMemcpyHostToDevice,
+                                      stream));
+        LHIP_CHECK(hipMemcpyAsync(let_root.as<void>(), let_hroot.data(), let_hroot.size() * sizeof(int),
+                                  hipMemcpyHostToDevice, stream));
+        return int(let_hroot.size());
+    }
+
+    void remote_args(WalkArgs& a, int trees, bool direct) {
+        a.let = let_rec.as<LetRec>();
+        a.let_body = let_body.as<float4>();
+        a.let_root = let_root.as<int>();
+        a.let_trees = trees;
+        a.let_thr = let_thr;
+        a.let_fan = let_fan;
+        a.direct_blocks = direct ? 1 : 0;
+    }
+
+    void compute_forces_with(const lhip_force_params& p, bool potential, const lhip_let* remote = nullptr,
+                             int nremote = 0) {
         LHIP_CHECK(hipSetDevice(dev));
         t.walk_ms = t.force_ms = 0;
         t.interactions = t.p2p = t.p2c = t.listed = t.p2c_mono = 0;
@@ -617,65 +751,57 @@ struct Context {
         if (tree && !tree_ok)
             throw Error(kLogic, "hip compute_forces_with: call build_tree after the last upload/drift first");
         const float G = float(p.G), eps2 = float(p.eps * p.eps);
+        // Remote trees are validated and uploaded before any force work is enqueued.
+        const int trees = upload_remote(remote, nremote);
         LHIP_CHECK(hipEventRecord(ev[3], stream));
         WalkCounters hc{};
-        if (!tree && mixed) {
-            direct_part.reserve(std::size_t(direct64_split(n)) * 4 * std::size_t(n) * 8);
-            rec64.reserve(std::size_t(n) * sizeof(SimRec));
-            launch_direct64(stream, D(xd), D(yd), D(zd), m.as<float>(), n, p.eps, p.G, potential, D(axd), D(ayd),
-                            D(azd), D(potd), rec64.as<SimRec>(), direct_part.as<double>());
-            LHIP_CHECK(hipEventRecord(ev[4], stream));
-            sync();
-            const uint64_t pairs = uint64_t(n) * uint64_t(n - 1);
-            hc.interactions = pairs;
-            hc.p2p = pairs;
-            hc.listed = uint64_t(n) * uint64_t(n);
-            direct_costs_pending = true;
-        } else if (!tree) {
-            launch_pack_posm(stream, x.as<float>(), y.as<float>(), z.as<float>(), m.as<float>(), posm.as<float4>(), n);
-            const int S = direct_split(n);
-            if (S > 1) direct_part.reserve(std::size_t(S) * std::size_t(n) * sizeof(float4));
-            launch_direct(stream, posm.as<float4>(), n, eps2, G, potential, ax.as<float>(), ay.as<float>(),
-                          az.as<float>(), pot.as<float>(), direct_part.as<float4>());
-            LHIP_CHECK(hipEventRecord(ev[4], stream));
-            sync();
-            const uint64_t pairs = uint64_t(n) * uint64_t(n - 1);
-            hc.interactions = pairs;
-            hc.p2p = pairs;
-            hc.listed = uint64_t(n) * uint64_t(n);
-            direct_costs_pending = true;
-        } else {
-            WalkArgs a{};
-            a.posm = posm.as<float4>();
-            a.node = node.as<WalkNode>();
-            a.mom = mom.as<NodeMom>();
-            a.gstart = walk_starts();
-            a.gcount = walk_count();
-            a.n = n;
-            a.eps2 = eps2;
-            a.G = G;
-            a.ax = ax.as<float>(); a.ay = ay.as<float>(); a.az = az.as<float>(); a.pot = pot.as<float>();
-            a.cost = cost.as<float>();
+        if (!tree) {
             if (mixed) {
-                a.posd = posd.as<double4>();
-                a.posd_box = posd_box ? posd_box : a.posd;
-                a.momd = momd.as<NodeMomD>();
-                a.axd = D(axd); a.ayd = D(ayd); a.azd = D(azd); a.potd = D(potd);
-                a.Gd = p.G;
+                direct_part.reserve(std::size_t(direct64_split(n)) * 4 * std::size_t(n) * 8);
+                rec64.reserve(std::size_t(n) * sizeof(SimRec));
+                launch_direct64(stream, D(xd), D(yd), D(zd), m.as<float>(), n, p.eps, p.G, potential, D(axd), D(ayd),
+                                D(azd), D(potd), rec64.as<SimRec>(), direct_part.as<double>());
+            } else {
+                launch_pack_posm(stream, x.as<float>(), y.as<float>(), z.as<float>(), m.as<float>(), posm.as<float4>(),
+                                 n);
+                const int S = direct_split(n);
+                if (S > 1) direct_part.reserve(std::size_t(S) * std::size_t(n) * sizeof(float4));
+                launch_direct(stream, posm.as<float4>(), n, eps2, G, potential, ax.as<float>(), ay.as<float>(),
+                              az.as<float>(), pot.as<float>(), direct_part.as<float4>());
             }
-            a.counters = counters.as<WalkCounters>();
-            a.blocks = walk_blocks;
-            a.small_leaf = wopt.small_leaf;
-            a.quad_tau = float(wopt.quad_tau);
-            a.depth = tree_depth(frames);
+            if (trees > 0) {
+                // Remote trees per block of kGroupMax consecutive bodies (src/cpu_backend.cpp:691-713).
+                if (mixed) {
+                    posd.reserve(std::size_t(n) * sizeof(double4));
+                    launch_pack_posd(stream, D(xd), D(yd), D(zd), m.as<float>(), posd.as<double4>(), n);
+                }
+                WalkArgs a = walk_args(p);
+                if (mixed) a.posd_box = a.posd;
+                remote_args(a, trees, true);
+                tree_walk_remote(stream, a, potential);
+                LHIP_CHECK(hipMemcpyAsync(&hc, counters.as<void>(), sizeof hc, hipMemcpyDeviceToHost, stream));
+            }
+            LHIP_CHECK(hipEventRecord(ev[4], stream));
+            sync();
+            const uint64_t pairs = uint64_t(n) * uint64_t(n - 1);
+            hc.interactions += pairs;
+            hc.p2p += pairs;
+            hc.listed += uint64_t(n) * uint64_t(n);
+            direct_costs_pending = trees == 0;
+        } else {
+            WalkArgs a = walk_args(p);
             tree_walk(stream, a, potential);
+            if (trees > 0) {
+                remote_args(a, trees, false);
+                tree_walk_remote(stream, a, potential);
+            }
             LHIP_CHECK(hipEventRecord(ev[4], stream));
             LHIP_CHECK(hipMemcpyAsync(&hc, counters.as<void>(), sizeof hc, hipMemcpyDeviceToHost, stream));
             sync();
-            if (hc.error)
-                throw Error(kRuntime, "hip tree walk: traversal stack overflow; accelerations are invalid");
             direct_costs_pending = false;
         }
+        if (hc.error)
+            throw Error(kRuntime, "hip tree walk: traversal stack overflow; accelerations are invalid");
         t.sort_ms = tree ? ms(0, 1) : 0.0;
         t.build_ms = tree ? ms(1, 2) : 0.0;
         t.walk_ms = ms(3, 4);
@@ -761,6 +887,105 @@ struct Context {
         LHIP_CHECK(hipMemcpyAsync(&c, cube.as<void>(), sizeof c, hipMemcpyDeviceToHost, stream));
         sync();
         for (int k = 0; k < 3; ++k) { lo[k] = c.lo[k]; hi[k] = c.hi[k]; }
+    }
+
+    // ---- LET export (src/backend.hpp:125-130)
+    DevBuf let_state, let_ex, let_bcnt, let_tmp, let_out, let_bods;
+    bool let_ready = false, let_host = false;
+    uint64_t let_nn = 0, let_nb = 0;
+    std::size_t let_stride = 0;  // entries per body array in let_bods
+    lhip_let_node let_hnode{};
+    std::vector<float> let_hx, let_hy, let_hz, let_hm;
+
+    void export_let(const float lo[3], const float hi[3], const lhip_force_params& p, uint64_t* nn, uint64_t* nb) {
+        static_assert(sizeof(lhip_let_node) == sizeof(LetWire), "LetWire is the ABI's LetNode");
+        LHIP_CHECK(hipSetDevice(dev));
+        let_ready = let_host = false;
+        let_nn = let_nb = 0;
+        if (n > 0 && p.method != LHIP_METHOD_TREE) {
+            // Direct method: one always-open root holding every body in the current order, its com
+            // summed in double over that order (src/cpu_backend.cpp:727-745).
+            for (auto* v : {&let_hx, &let_hy, &let_hz, &let_hm}) v->resize(std::size_t(n));
+            download(let_hx.data(), let_hy.data(), let_hz.data(), nullptr, nullptr, nullptr, let_hm.data(), nullptr);
+            double M = 0, sx = 0, sy = 0, sz = 0;
+            for (int i = 0; i < n; ++i) {
+                const double w = let_hm[i];
+                M += w; sx += w * let_hx[i]; sy += w * let_hy[i]; sz += w * let_hz[i];
+            }
+            lhip_let_node r{};
+            if (M > 0) { r.cx = float(sx / M); r.cy = float(sy / M); r.cz = float(sz / M); }
+            r.mass = float(M);
+            r.rcrit2 = 3.0e38f;  // always opened: the quadrupole is never read and stays zero
+            r.first_child = -1; r.child_count = 0;
+            r.body_first = 0; r.body_count = n;
+            let_hnode = r;
+            let_host = true;
+            let_nn = 1;
+            let_nb = uint64_t(n);
+        } else if (n > 0) {
+            if (!tree_ok) throw Error(kLogic, "hip export_let: call build_tree after the last upload/drift first");
+            uint32_t count = 0;
+            LHIP_CHECK(hipMemcpyAsync(&count, cnt.as<uint32_t>() + n, 4, hipMemcpyDeviceToHost, stream));
+            sync();
+            const std::size_t c1 = std::size_t(count) + 1;
+            for (DevBuf* b : {&let_state, &let_ex, &let_bcnt}) b->reserve(c1 * 4);
+            let_tmp.reserve(scan_tmp_entries(int(count)) * 4);
+            let_out.reserve(std::size_t(count) * sizeof(LetWire));
+            let_bods.reserve(std::size_t(n) * 16);
+            LetExport e{};
+            e.node = node.as<WalkNode>();
+            e.mom = mom.as<NodeMom>();
+            e.levstart = levstart.as<uint32_t>();
+            e.count = int(count);
+            e.depth = tree_depth(frames);
+            e.f64 = mixed;
+            if (mixed) { e.x = D(xd); e.y = D(yd); e.z = D(zd); }
+            else { e.x = F(x); e.y = F(y); e.z = F(z); }
+            e.m = m.as<float>();
+            for (int k = 0; k < 3; ++k) { e.lo[k] = lo[k]; e.hi[k] = hi[k]; }
+            e.state = let_state.as<uint32_t>();
+            e.ex = let_ex.as<uint32_t>();
+            e.bcnt = let_bcnt.as<uint32_t>();
+            e.tmp = let_tmp.as<uint32_t>();
+            e.out = let_out.as<LetWire>();
+            float* bods = let_bods.as<float>();
+            let_stride = std::size_t(n);
+            e.bx = bods; e.by = bods + let_stride; e.bz = bods + 2 * let_stride; e.bm = bods + 3 * let_stride;
+            let_export(stream, e);
+            uint32_t tot[2] = {0, 0};
+            LHIP_CHECK(hipMemcpyAsync(&tot[0], let_ex.as<uint32_t>() + count, 4, hipMemcpyDeviceToHost, stream));
+            LHIP_CHECK(hipMemcpyAsync(&tot[1], let_bcnt.as<uint32_t>() + count, 4, hipMemcpyDeviceToHost, stream));
+            sync();
+            let_nn = tot[0];
+            let_nb = tot[1];
+        }
+        let_ready = true;
+        *nn = let_nn;
+        *nb = let_nb;
+    }
+
+    void export_let_copy(lhip_let_node* nodes, float* bx, float* by, float* bz, float* bm) {
+        LHIP_CHECK(hipSetDevice(dev));
+        if (!let_ready) throw Error(kLogic, "hip export_let_copy: call export_let first");
+        if ((let_nn && !nodes) || (let_nb && (!bx || !by || !bz || !bm)))
+            throw Error(kInvalid, "hip export_let_copy: null output array");
+        if (let_host) {
+            nodes[0] = let_hnode;
+            const std::size_t fb = std::size_t(let_nb) * 4;
+            std::memcpy(bx, let_hx.data(), fb);
+            std::memcpy(by, let_hy.data(), fb);
+            std::memcpy(bz, let_hz.data(), fb);
+            std::memcpy(bm, let_hm.data(), fb);
+            return;
+        }
+        if (let_nn) LHIP_CHECK(hipMemcpyAsync(nodes, let_out.as<void>(), let_nn * sizeof(LetWire), hipMemcpyDeviceToHost, stream));
+        const std::size_t fb = std::size_t(let_nb) * 4;
+        const float* bods = let_bods.as<float>();
+        float* dst[4] = {bx, by, bz, bm};
+        if (let_nb)
+            for (int c = 0; c < 4; ++c)
+                LHIP_CHECK(hipMemcpyAsync(dst[c], bods + c * let_stride, fb, hipMemcpyDeviceToHost, stream));
+        sync();
     }
 
     bool body_costs(float* out) {

@@ -19,6 +19,9 @@ void launch_f32_to_f64(hipStream_t s, const float* a, double* b, int n);
 void launch_f64_to_f32(hipStream_t s, const double* a, float* b, int n);
 void launch_pack_posm(hipStream_t s, const float* x, const float* y, const float* z, const float* m, float4* posm,
                       int n);
+// FP64 positions and float masses into (x, y, z, m) records.
+void launch_pack_posd(hipStream_t s, const double* x, const double* y, const double* z, const float* m, double4* posd,
+                      int n);
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, long long n);
 void launch_zero_f32(hipStream_t s, float* p, long long n);
 // Bounding box of (x, y, z) into cube (float lo/hi, and the Morton cube in double).

@@ -756,6 +756,8 @@ std::size_t scan_tmp_entries(int n) {
     return std::size_t((static_cast<long long>(n) + kScanTile) / kScanTile) + 1;
 }
 
+void scan_u32(hipStream_t s, uint32_t* a, int n, uint32_t* tmp) { scan_exclusive(s, a, n, tmp); }
+
 void tree_build(hipStream_t s, const BuildBufs& b, float inv_theta, hipEvent_t levels_done) {
     const int n = b.n;
     const bool ext = b.frames > 1;

@@ -56,6 +56,13 @@ __global__ void __launch_bounds__(kBS) pack_kernel(const float* __restrict__ x, 
     if (i < n) posm[i] = make_float4(x[i], y[i], z[i], m[i]);
 }
 
+__global__ void __launch_bounds__(kBS) pack_posd_kernel(const double* __restrict__ x, const double* __restrict__ y,
+                                                        const double* __restrict__ z, const float* __restrict__ m,
+                                                        double4* __restrict__ posd, int n) {
+    const int i = blockIdx.x * kBS + threadIdx.x;
+    if (i < n) posd[i] = make_double4(x[i], y[i], z[i], double(m[i]));
+}
+
 __global__ void __launch_bounds__(kBS) fill_u32_kernel(uint32_t* __restrict__ p, uint32_t v, long long n) {
     const long long i = (long long)blockIdx.x * kBS + threadIdx.x;
     if (i < n) p[i] = v;
@@ -279,6 +286,13 @@ void launch_pack_posm(hipStream_t s, const float* x, const float* y, const float
                       int n) {
     if (n <= 0) return;
     pack_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(x, y, z, m, posm, n);
+    LHIP_LAUNCH_CHECK();
+}
+
+void launch_pack_posd(hipStream_t s, const double* x, const double* y, const double* z, const float* m, double4* posd,
+                      int n) {
+    if (n <= 0) return;
+    pack_posd_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(x, y, z, m, posd, n);
     LHIP_LAUNCH_CHECK();
 }
 

@@ -147,12 +147,15 @@ __device__ __forceinline__ void eval_bodies(Slot (&sl)[NX], int S, int nb, const
     }
 }
 
-constexpr int kStackCap = 896;
+constexpr int kStackCap = kWalkStack;
 // Stack budget: a step popping P nodes pushes at most 8P, so P is held to (thr - top) / 7 with thr
 // = capacity - 7 (depth + 1) - 8; at or above thr one node per step adds at most 7 per tree level.
 __device__ __forceinline__ int stack_threshold(int depth) { return kStackCap - 7 * (max(depth, kMaxLevel) + 1) - 8; }
 
-template <bool POT, bool GUARD, bool REL>
+// LET: the remote trees' pass (a.let): the same groups (or, with direct_blocks, blocks of
+// kGroupMax consecutive bodies) walk every remote tree instead of the local one, and the forces,
+// costs and counts are added to what is there.
+template <bool POT, bool GUARD, bool REL, bool LET>
 __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
     constexpr bool MASK = POT || GUARD;
     // Slots per lane at most: 3 (4 would spill at 4 waves per SIMD, the occupancy the bounds ask
@@ -164,9 +167,10 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     WaveLds& L = lds[w];
     int* st = stk[w];
-    const int ngroups = int(a.gcount[a.n]);
+    const bool blocks = LET && a.direct_blocks;
+    const int ngroups = blocks ? (a.n + kGroupMax - 1) / kGroupMax : int(a.gcount[a.n]);
     const float e2 = a.eps2;
-    const int stack_thr = stack_threshold(a.depth);
+    const int stack_thr = LET ? a.let_thr : stack_threshold(a.depth);
     unsigned long long t_inter = 0, t_p2p = 0, t_p2c = 0, t_listed = 0, t_p2m = 0;
     bool overflow = false;
 
@@ -175,7 +179,8 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
         if (lane == 0) g = int(atomicAdd(&a.counters->next_group, 1u));
         g = __builtin_amdgcn_readfirstlane(__shfl(g, 0, 64));
         if (g >= ngroups) break;
-        const int glo = int(a.gstart[g]), ghi = int(a.gstart[g + 1]);
+        const int glo = blocks ? g * kGroupMax : int(a.gstart[g]);
+        const int ghi = blocks ? min(a.n, glo + kGroupMax) : int(a.gstart[g + 1]);
         const bool chunked = ghi - glo > kGroupMax;
         for (int clo = glo; clo < ghi; clo += kGroupMax) {
             const int chi = min(ghi, clo + kGroupMax);
@@ -275,111 +280,42 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                 nb = 0;
             };
 
-            if (lane == 0) st[0] = 0;  // the root
-            int top = 1;
-            while (top > 0) {
-                const int room = (stack_thr - top) / 7;
-                const int P = min(64, min(top, max(1, room)));
-                __builtin_amdgcn_wave_barrier();
-                const bool act = lane < P;
-                const int k = act ? st[top - P + lane] : 0;
-                __builtin_amdgcn_wave_barrier();  // the popped entries are read before pushes overwrite them
-                top -= P;
-                WalkNode nd{0.f, 0.f, 0.f, 0.f, 0, 0, 0, -1};
-                // Mixed form: the FP64 com and mass (what a monopole or a cell stages) and qrel (the
-                // far-field decision) load together with the node, so that deciding and staging
-                // wait for one round trip.
-                // Plain form: the mass, quadrupole and qrel (the com is the node record's) likewise.
-                double4 pc = make_double4(0.0, 0.0, 0.0, 0.0);
-                float qr = 0.f, pm = 0.f;
-                float4 pq = make_float4(0.f, 0.f, 0.f, 0.f);
-                float2 pq2 = make_float2(0.f, 0.f);
-                if (act) {
-                    nd = a.node[k];
-                    if (REL) {
-                        const NodeMomD& dm = a.momd[k];
-                        pc = make_double4(dm.cx, dm.cy, dm.cz, dm.m);
-                        if (a.quad_tau > 0.f) qr = a.mom[k].qrel;
-                    } else {
-                        const float4* r = reinterpret_cast<const float4*>(a.mom + k);
-                        const float4 r2 = r[2];
-                        pm = r[0].w;
-                        pq = r[1];
-                        pq2 = make_float2(r2.x, r2.y);
-                        qr = r2.w;
-                    }
-                }
-                // A node holding bodies of this (sub)group is always opened: the reference's
-                // "contains" test for whole groups; any overlap for chunks of an over-full group.
-                const bool keep_open = chunked ? (nd.lo < chi && clo < nd.hi) : (nd.lo <= clo && chi <= nd.hi);
-                const float dx = fmaxf(fabsf(nd.cx - bx) - hx, 0.f);
-                const float dy = fmaxf(fabsf(nd.cy - by) - hy, 0.f);
-                const float dz = fmaxf(fabsf(nd.cz - bz) - hz, 0.f);
-                // Rounded as the reference build evaluates it: three products, then two sums.
-                const float d2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
-                const bool acc = act && !keep_open && d2 >= nd.rcrit2;
-                const bool leaf = act && info_leaf(nd.info);
-                const bool small = leaf && (nd.hi - nd.lo) <= a.small_leaf;
-                bool is_cell = acc && !small;
-                // Error-budgeted far field: an accepted cell whose quadrupole term is below quad_tau
-                // of its monopole at the group box's distance enters the body list as a point mass.
-                const bool is_mono = is_cell && a.quad_tau > 0.f && qr < a.quad_tau * d2;
-                is_cell = is_cell && !is_mono;
-                const bool is_range = leaf && (!acc || small);
-                const bool open_int = act && !acc && !leaf;
-
-                // push the child blocks of opened internal nodes
-                const int nch = open_int ? info_nchild(nd.info) : 0;
-                const int cend = prefix_sum_incl(nch, lane);
-                const int pushed = __builtin_amdgcn_readlane(cend, 63);
-                if (top + pushed > kStackCap) {
-                    overflow = true;
-                    break;
-                }
-                for (int c = 0; c < nch; ++c) st[top + cend - nch + c] = nd.first + c;
-                top += pushed;
-
+            // Staging into the LDS lists; get() and load() run on the staging lanes only.
+            auto stage_cells = [&](bool is_cell, auto&& get) {
                 const unsigned long long mc = __ballot(is_cell);
                 const int ncnew = __popcll(mc);
                 if (ncnew) {
                     if (nc + ncnew > kCellCap) flush_cells();
                     if (is_cell) {
-                        const int slot = nc + int(lanes_below(mc));
-                        if (REL) {
-                            const NodeMom& md = a.mom[k];
-                            L.c[slot].A = make_float4(float(pc.x - ccx), float(pc.y - ccy), float(pc.z - ccz),
-                                                      float(pc.w));
-                            L.c[slot].B = make_float4(md.qxx, md.qxy, md.qxz, md.qyy);
-                            L.c[slot].C = make_float2(md.qyz, md.qzz);
-                        } else {
-                            L.c[slot].A = make_float4(nd.cx, nd.cy, nd.cz, pm);
-                            L.c[slot].B = pq;
-                            L.c[slot].C = pq2;
-                        }
+                        const CellL c = get();
+                        CellL& d = L.c[nc + int(lanes_below(mc))];
+                        d.A = c.A;
+                        d.B = c.B;
+                        d.C = c.C;
                     }
                     nc += ncnew;
                 }
+            };
+            auto stage_monos = [&](bool is_mono, auto&& get) {
                 const unsigned long long mm = __ballot(is_mono);
                 const int nmnew = __popcll(mm);
                 if (nmnew) {
                     if (nb + nmnew > kBodyCap) flush_bodies();
                     if (is_mono) {
                         const int slot = nb + int(lanes_below(mm));
-                        if (REL) {
-                            L.bP[slot] = make_float4(float(pc.x - ccx), float(pc.y - ccy), float(pc.z - ccz),
-                                                     float(pc.w));
-                        } else {
-                            L.bP[slot] = make_float4(nd.cx, nd.cy, nd.cz, pm);
-                        }
+                        L.bP[slot] = get();
                         L.bI[slot] = -1;
                     }
                     nb += nmnew;
                     nmono_total += uint32_t(nmnew);
                 }
-                // Bodies of opened leaves: staged in chunks of at most the free buffer space.
+            };
+            // Body ranges [lo, lo + len) of the staging lanes, in chunks of at most the free buffer
+            // space; index(j) is what the self test compares (-1: never the target).
+            auto stage_ranges = [&](bool is_range, int lo, int len_in, auto&& load, auto&& index) {
                 const unsigned long long mr = __ballot(is_range);
                 if (mr) {
-                    const int len = is_range ? nd.hi - nd.lo : 0;
+                    const int len = is_range ? len_in : 0;
                     const int end = prefix_sum_incl(len, lane);
                     const int off = end - len;
                     const int total = __builtin_amdgcn_readlane(end, 63);
@@ -393,25 +329,171 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                         for (int u = u0; u < u1; u += 4) {
                             float4 q[4];
 #pragma unroll
-                            for (int t = 0; t < 4; ++t) {
-                                const int j = nd.lo + (min(u + t, u1 - 1) - off);
-                                if (REL) {
-                                    const double4 d = a.posd[j];
-                                    q[t] = make_float4(float(d.x - ccx), float(d.y - ccy), float(d.z - ccz),
-                                                       float(d.w));
-                                } else {
-                                    q[t] = a.posm[j];
-                                }
-                            }
+                            for (int t = 0; t < 4; ++t) q[t] = load(lo + (min(u + t, u1 - 1) - off));
 #pragma unroll
                             for (int t = 0; t < 4; ++t)
                                 if (u + t < u1) {
                                     L.bP[nb + (u + t - done)] = q[t];
-                                    if (MASK) L.bI[nb + (u + t - done)] = nd.lo + (u + t - off);
+                                    if (MASK) L.bI[nb + (u + t - done)] = index(lo + (u + t - off));
                                 }
                         }
                         nb += take;
                         done += take;
+                    }
+                }
+            };
+
+            if constexpr (!LET) {
+                if (lane == 0) st[0] = 0;  // the root
+                int top = 1;
+                while (top > 0) {
+                    const int room = (stack_thr - top) / 7;
+                    const int P = min(64, min(top, max(1, room)));
+                    __builtin_amdgcn_wave_barrier();
+                    const bool act = lane < P;
+                    const int k = act ? st[top - P + lane] : 0;
+                    __builtin_amdgcn_wave_barrier();  // the popped entries are read before pushes overwrite them
+                    top -= P;
+                    WalkNode nd{0.f, 0.f, 0.f, 0.f, 0, 0, 0, -1};
+                    // Mixed form: the FP64 com and mass (what a monopole or a cell stages) and qrel
+                    // (the far-field decision) load together with the node, so that deciding and
+                    // staging wait for one round trip.
+                    // Plain form: the mass, quadrupole and qrel (the com is the node record's) likewise.
+                    double4 pc = make_double4(0.0, 0.0, 0.0, 0.0);
+                    float qr = 0.f, pm = 0.f;
+                    float4 pq = make_float4(0.f, 0.f, 0.f, 0.f);
+                    float2 pq2 = make_float2(0.f, 0.f);
+                    if (act) {
+                        nd = a.node[k];
+                        if (REL) {
+                            const NodeMomD& dm = a.momd[k];
+                            pc = make_double4(dm.cx, dm.cy, dm.cz, dm.m);
+                            if (a.quad_tau > 0.f) qr = a.mom[k].qrel;
+                        } else {
+                            const float4* r = reinterpret_cast<const float4*>(a.mom + k);
+                            const float4 r2 = r[2];
+                            pm = r[0].w;
+                            pq = r[1];
+                            pq2 = make_float2(r2.x, r2.y);
+                            qr = r2.w;
+                        }
+                    }
+                    // A node holding bodies of this (sub)group is always opened: the reference's
+                    // "contains" test for whole groups; any overlap for chunks of an over-full group.
+                    const bool keep_open = chunked ? (nd.lo < chi && clo < nd.hi) : (nd.lo <= clo && chi <= nd.hi);
+                    const float dx = fmaxf(fabsf(nd.cx - bx) - hx, 0.f);
+                    const float dy = fmaxf(fabsf(nd.cy - by) - hy, 0.f);
+                    const float dz = fmaxf(fabsf(nd.cz - bz) - hz, 0.f);
+                    // Rounded as the reference build evaluates it: three products, then two sums.
+                    const float d2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+                    const bool acc = act && !keep_open && d2 >= nd.rcrit2;
+                    const bool leaf = act && info_leaf(nd.info);
+                    const bool small = leaf && (nd.hi - nd.lo) <= a.small_leaf;
+                    bool is_cell = acc && !small;
+                    // Error-budgeted far field: an accepted cell whose quadrupole term is below
+                    // quad_tau of its monopole at the group box's distance enters the body list as a
+                    // point mass.
+                    const bool is_mono = is_cell && a.quad_tau > 0.f && qr < a.quad_tau * d2;
+                    is_cell = is_cell && !is_mono;
+                    const bool is_range = leaf && (!acc || small);
+                    const bool open_int = act && !acc && !leaf;
+
+                    // push the child blocks of opened internal nodes
+                    const int nch = open_int ? info_nchild(nd.info) : 0;
+                    const int cend = prefix_sum_incl(nch, lane);
+                    const int pushed = __builtin_amdgcn_readlane(cend, 63);
+                    if (top + pushed > kStackCap) {
+                        overflow = true;
+                        break;
+                    }
+                    for (int c = 0; c < nch; ++c) st[top + cend - nch + c] = nd.first + c;
+                    top += pushed;
+
+                    stage_cells(is_cell, [&]() {
+                        CellL c;
+                        if (REL) {
+                            const NodeMom& md = a.mom[k];
+                            c.A = make_float4(float(pc.x - ccx), float(pc.y - ccy), float(pc.z - ccz), float(pc.w));
+                            c.B = make_float4(md.qxx, md.qxy, md.qxz, md.qyy);
+                            c.C = make_float2(md.qyz, md.qzz);
+                        } else {
+                            c.A = make_float4(nd.cx, nd.cy, nd.cz, pm);
+                            c.B = pq;
+                            c.C = pq2;
+                        }
+                        return c;
+                    });
+                    stage_monos(is_mono, [&]() {
+                        return REL ? make_float4(float(pc.x - ccx), float(pc.y - ccy), float(pc.z - ccz), float(pc.w))
+                                   : make_float4(nd.cx, nd.cy, nd.cz, pm);
+                    });
+                    stage_ranges(
+                        is_range, nd.lo, nd.hi - nd.lo,
+                        [&](int j) {
+                            if (REL) {
+                                const double4 d = a.posd[j];
+                                return make_float4(float(d.x - ccx), float(d.y - ccy), float(d.z - ccz), float(d.w));
+                            }
+                            return a.posm[j];
+                        },
+                        [](int j) { return j; });
+                }
+            } else {
+                // Remote trees (src/backend.hpp:78-83): a terminal node (no children, no bodies) or
+                // one whose com is at rcrit2 or beyond is accepted: its bodies if it has 1 to
+                // small_leaf of them, else its multipole; an opened node contributes its bodies if it
+                // has any, else its children. Remote bodies are never the target (index -1).
+                for (int t = 0; t < a.let_trees && !overflow; ++t) {
+                    if (lane == 0) st[0] = a.let_root[t];
+                    int top = 1;
+                    while (top > 0) {
+                        const int room = (stack_thr - top) / a.let_fan;
+                        const int P = min(64, min(top, max(1, room)));
+                        __builtin_amdgcn_wave_barrier();
+                        const bool act = lane < P;
+                        const int k = act ? st[top - P + lane] : 0;
+                        __builtin_amdgcn_wave_barrier();
+                        top -= P;
+                        LetRec r;
+                        r.A = r.B = r.C = make_float4(0.f, 0.f, 0.f, 0.f);
+                        r.D = make_int4(-1, 0, 0, 0);
+                        if (act) r = a.let[k];
+                        const bool terminal = r.D.x < 0 && r.D.w == 0;
+                        const float dx = fmaxf(fabsf(r.A.x - bx) - hx, 0.f);
+                        const float dy = fmaxf(fabsf(r.A.y - by) - hy, 0.f);
+                        const float dz = fmaxf(fabsf(r.A.z - bz) - hz, 0.f);
+                        const float d2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+                        const bool acc = act && (terminal || d2 >= r.C.z);
+                        const bool small = r.D.w > 0 && r.D.w <= a.small_leaf;
+                        bool is_cell = acc && !small;
+                        const bool is_mono = is_cell && a.quad_tau > 0.f && r.C.w < a.quad_tau * d2;
+                        is_cell = is_cell && !is_mono;
+                        const bool is_range = act && r.D.w > 0 && (!acc || small);
+                        const int nch = (act && !acc && r.D.w <= 0) ? r.D.y : 0;
+                        const int cend = prefix_sum_incl(nch, lane);
+                        const int pushed = __builtin_amdgcn_readlane(cend, 63);
+                        if (top + pushed > kStackCap) {
+                            overflow = true;
+                            break;
+                        }
+                        for (int c = 0; c < nch; ++c) st[top + cend - nch + c] = r.D.x + c;
+                        top += pushed;
+
+                        auto rel = [&](float4 p) {
+                            return REL ? make_float4(float(double(p.x) - ccx), float(double(p.y) - ccy),
+                                                     float(double(p.z) - ccz), p.w)
+                                       : p;
+                        };
+                        stage_cells(is_cell, [&]() {
+                            CellL c;
+                            c.A = rel(r.A);
+                            c.B = r.B;
+                            c.C = make_float2(r.C.x, r.C.y);
+                            return c;
+                        });
+                        stage_monos(is_mono, [&]() { return rel(r.A); });
+                        stage_ranges(
+                            is_range, r.D.z, r.D.w, [&](int j) { return rel(a.let_body[j]); }, [](int) { return -1; });
                     }
                 }
             }
@@ -420,6 +502,13 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
 
             // Add the list shares of each body (slot v = share * nbody + body) through LDS.
             const uint32_t listed = ncell_total + nbody_total;  // nbody_total includes the monopoles
+            // Per-body cost, self excluded: the local pass's list less the body's own entry; the
+            // remote pass adds its entries to the local cost, or to n - 1 after the direct sum.
+            auto cost_of = [&](int i, uint32_t len) {
+                if (!LET) return float(len - 1u);
+                if (blocks) return float((unsigned long long)(a.n - 1) + len);
+                return float((unsigned long long)a.cost[i] + len);
+            };
             if (REL) {
                 D4* red = reinterpret_cast<D4*>(&L);  // the lists are done: their LDS holds 64 NX D4
                 __builtin_amdgcn_wave_barrier();
@@ -438,11 +527,18 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                         acc4.x += q.x; acc4.y += q.y; acc4.z += q.z; acc4.w += q.w;
                     }
                     const int i = clo + v;
-                    a.axd[i] = G * acc4.x;
-                    a.ayd[i] = G * acc4.y;
-                    a.azd[i] = G * acc4.z;
-                    if (POT) a.potd[i] = -G * acc4.w;
-                    if (a.cost) a.cost[i] = float(listed - 1u);
+                    if (LET) {
+                        a.axd[i] += G * acc4.x;
+                        a.ayd[i] += G * acc4.y;
+                        a.azd[i] += G * acc4.z;
+                        if (POT) a.potd[i] -= G * acc4.w;
+                    } else {
+                        a.axd[i] = G * acc4.x;
+                        a.ayd[i] = G * acc4.y;
+                        a.azd[i] = G * acc4.z;
+                        if (POT) a.potd[i] = -G * acc4.w;
+                    }
+                    if (a.cost) a.cost[i] = cost_of(i, listed);
                 }
             } else {
                 float4* red = reinterpret_cast<float4*>(&L);  // the lists are done: 64 NX float4
@@ -463,21 +559,30 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                         acc4.x += q.x; acc4.y += q.y; acc4.z += q.z; acc4.w += q.w;
                     }
                     const int i = clo + v;
-                    a.ax[i] = G * acc4.x;
-                    a.ay[i] = G * acc4.y;
-                    a.az[i] = G * acc4.z;
-                    if (POT) a.pot[i] = -G * acc4.w;
-                    if (a.cost) a.cost[i] = float(listed - 1u);
+                    if (LET) {
+                        a.ax[i] += G * acc4.x;
+                        a.ay[i] += G * acc4.y;
+                        a.az[i] += G * acc4.z;
+                        if (POT) a.pot[i] -= G * acc4.w;
+                    } else {
+                        a.ax[i] = G * acc4.x;
+                        a.ay[i] = G * acc4.y;
+                        a.az[i] = G * acc4.z;
+                        if (POT) a.pot[i] = -G * acc4.w;
+                    }
+                    if (a.cost) a.cost[i] = cost_of(i, listed);
                 }
             }
             __builtin_amdgcn_wave_barrier();
             if (lane == 0) {
-                // Each body of the chunk meets itself once in the body list.
+                // Each body of the chunk meets itself once in the local body list (never in a
+                // remote one).
+                const unsigned long long self = LET ? 0ull : (unsigned long long)nbody;
                 t_listed += (unsigned long long)listed * nbody;
                 t_p2c += (unsigned long long)(ncell_total + nmono_total) * nbody;
                 t_p2m += (unsigned long long)nmono_total * nbody;
-                t_p2p += (unsigned long long)(nbody_total - nmono_total) * nbody - nbody;
-                t_inter += (unsigned long long)listed * nbody - nbody;
+                t_p2p += (unsigned long long)(nbody_total - nmono_total) * nbody - self;
+                t_inter += (unsigned long long)listed * nbody - self;
             }
         }
     }
@@ -502,12 +607,14 @@ __global__ void reset_walk(WalkCounters* __restrict__ c) {
     c->next_group = 0;
 }
 
+__global__ void reset_next_group(WalkCounters* __restrict__ c) { c->next_group = 0; }
+
 }  // namespace
 
 int walk_blocks_per_cu() {
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks,
-                                                     reinterpret_cast<const void*>(walk_kernel<false, false, false>),
+                                                     reinterpret_cast<const void*>(walk_kernel<false, false, false, false>),
                                                      kWalkBS, 0) != hipSuccess ||
         blocks < 1) {
         (void)hipGetLastError();
@@ -516,14 +623,14 @@ int walk_blocks_per_cu() {
     return blocks;
 }
 
-template <bool REL>
+template <bool REL, bool LET>
 void walk_launch(hipStream_t s, const WalkArgs& a, bool potential, bool guard, int blocks) {
     if (potential) {
-        if (guard) walk_kernel<true, true, REL><<<blocks, kWalkBS, 0, s>>>(a);
-        else walk_kernel<true, false, REL><<<blocks, kWalkBS, 0, s>>>(a);
+        if (guard) walk_kernel<true, true, REL, LET><<<blocks, kWalkBS, 0, s>>>(a);
+        else walk_kernel<true, false, REL, LET><<<blocks, kWalkBS, 0, s>>>(a);
     } else {
-        if (guard) walk_kernel<false, true, REL><<<blocks, kWalkBS, 0, s>>>(a);
-        else walk_kernel<false, false, REL><<<blocks, kWalkBS, 0, s>>>(a);
+        if (guard) walk_kernel<false, true, REL, LET><<<blocks, kWalkBS, 0, s>>>(a);
+        else walk_kernel<false, false, REL, LET><<<blocks, kWalkBS, 0, s>>>(a);
     }
 }
 
@@ -533,8 +640,20 @@ void tree_walk(hipStream_t s, const WalkArgs& a, bool potential) {
     const int blocks = std::max(1, a.blocks);
     reset_walk<<<1, 1, 0, s>>>(a.counters);
     LHIP_LAUNCH_CHECK();
-    if (a.posd) walk_launch<true>(s, a, potential, guard, blocks);
-    else walk_launch<false>(s, a, potential, guard, blocks);
+    if (a.posd) walk_launch<true, false>(s, a, potential, guard, blocks);
+    else walk_launch<false, false>(s, a, potential, guard, blocks);
+    LHIP_LAUNCH_CHECK();
+}
+
+void tree_walk_remote(hipStream_t s, const WalkArgs& a, bool potential) {
+    if (a.n <= 0 || a.let_trees <= 0) return;
+    const bool guard = !(a.eps2 > 0.f);
+    const int blocks = std::max(1, a.blocks);
+    if (a.direct_blocks) reset_walk<<<1, 1, 0, s>>>(a.counters);
+    else reset_next_group<<<1, 1, 0, s>>>(a.counters);
+    LHIP_LAUNCH_CHECK();
+    if (a.posd) walk_launch<true, true>(s, a, potential, guard, blocks);
+    else walk_launch<false, true>(s, a, potential, guard, blocks);
     LHIP_LAUNCH_CHECK();
 }
 

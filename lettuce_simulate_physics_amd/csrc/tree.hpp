@@ -35,6 +35,26 @@ struct NodeMomD {
 };
 static_assert(sizeof(NodeMomD) == 80, "NodeMomD layout");
 
+// A remote LET node on the device: com and mass; quadrupole; (qyz, qzz, rcrit2, qrel); first child,
+// child count, first body, body count (indices into the concatenated remote arrays; first child -1
+// only for a node without children, which with no bodies is terminal).
+struct alignas(16) LetRec {
+    float4 A;
+    float4 B;
+    float4 C;
+    int4 D;
+};
+static_assert(sizeof(LetRec) == 64, "LetRec is one 64-byte record");
+
+// The LET wire record (src/backend.hpp:63-76), as written by the export.
+struct LetWire {
+    float cx, cy, cz, mass;
+    float qxx, qxy, qxz, qyy, qyz, qzz;
+    float rcrit2;
+    int32_t first_child, child_count, body_first, body_count;
+};
+static_assert(sizeof(LetWire) == 60, "LetWire is the 60-byte LetNode");
+
 struct BuildBufs {
     int n;
     int cap_nodes;             // node capacity (upper bound for n bodies)
@@ -113,6 +133,29 @@ void gather_u64(hipStream_t s, const uint32_t* perm, const uint64_t* in, uint64_
 void gather_u32(hipStream_t s, const uint32_t* perm, const uint32_t* in, uint32_t* out, int n);
 void fill_u32(hipStream_t s, uint32_t* a, uint32_t v, long long n);
 void iota_u32(hipStream_t s, uint32_t* a, int n);
+// In-place exclusive scan of n entries; a[n] = total. tmp: scan_tmp_entries(n) entries.
+void scan_u32(hipStream_t s, uint32_t* a, int n, uint32_t* tmp);
+
+// LET export (kernels_let.hip): the nodes of the tree in [0, count) a target box needs, in the LET's
+// breadth-first order. state, ex, bcnt: count + 1 entries of scratch; tmp: scan_tmp_entries(count).
+// Positions (float, or double when f64) and masses in sorted order. Writes the records to out and
+// the bodies of opened leaves to bx..bm; the totals (nodes, bodies) end at ex[count], bcnt[count].
+struct LetExport {
+    const WalkNode* node;
+    const NodeMom* mom;
+    const uint32_t* levstart;
+    int count;
+    int depth;                // deepest level
+    const void *x, *y, *z;
+    bool f64;
+    const float* m;
+    float lo[3], hi[3];       // target box
+    uint32_t *state, *ex, *bcnt, *tmp;
+    LetWire* out;
+    float *bx, *by, *bz, *bm;
+};
+void let_export(hipStream_t s, const LetExport& e);
+
 // Mixed form, frozen topology: recompute com, mass and quadrupole (node and momd) from the
 // positions in b over the tree tree_build made, keeping every decision input (node records,
 // rcrit2, qrel).
@@ -143,8 +186,25 @@ struct WalkArgs {
     int small_leaf;            // accepted leaves with at most this many bodies are summed directly
     float quad_tau;            // error-budgeted far field threshold (0: every quadrupole kept)
     int depth;                 // deepest tree level (21 x key frames)
+    // Remote trees (tree_walk_remote): node records, bodies (x, y, z, m) and the root of each
+    // non-empty tree; the stack threshold for their height and fan-out. direct_blocks: the groups
+    // are blocks of kGroupMax consecutive bodies in the current order (the direct method's remote
+    // pass) instead of gstart.
+    const LetRec* let;
+    const float4* let_body;
+    const int* let_root;
+    int let_trees;
+    int let_thr;
+    int let_fan;
+    int direct_blocks;
 };
 void tree_walk(hipStream_t s, const WalkArgs& a, bool potential);
+// Entries of a walk wave's traversal stack (LDS).
+constexpr int kWalkStack = 896;
+// The remote trees' pass: every group walks every remote tree with the receiver rules of the LET
+// format and adds the forces, costs and counts to what the local pass (or the direct sum) left.
+// After tree_walk (same counters; only its group counter is reset) or, with direct_blocks, alone.
+void tree_walk_remote(hipStream_t s, const WalkArgs& a, bool potential);
 int walk_blocks_per_cu();
 
 // Vector-Jacobian product of the mixed-precision walk's forces with frozen topology

@@ -3,6 +3,7 @@
 // Errors keep the reference's exception types: device/runtime failures std::runtime_error,
 // protocol misuse std::logic_error (src/cuda_backend.cu:45-51, 1399-1400, 1570-1571); the factory
 // never throws and reports `why` (src/cuda_backend.cu:2026-2048).
+#include <cstddef>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -11,6 +12,12 @@
 #include "lettuce_hip.h"
 
 namespace {
+
+static_assert(sizeof(LetNode) == sizeof(lhip_let_node), "LetNode and lhip_let_node are one record");
+static_assert(offsetof(LetNode, rcrit2) == offsetof(lhip_let_node, rcrit2) &&
+                  offsetof(LetNode, first_child) == offsetof(lhip_let_node, first_child) &&
+                  offsetof(LetNode, body_count) == offsetof(lhip_let_node, body_count),
+              "LetNode and lhip_let_node share their layout");
 
 lhip_force_params to_c(const ForceParams& p) {
     lhip_force_params c{};
@@ -90,16 +97,37 @@ public:
         ck(lhip_build_tree(c_, &c));
     }
 
-    // The locally essential tree exchange is a multi-rank feature; one GPU never calls it.
-    LetTree export_let(const Box3&, const ForceParams&) override {
-        throw std::logic_error("hip export_let: remote trees (multi-rank) are not supported by this backend");
+    // LETs travel as the reference's LetNode records; lhip_let_node has the same layout, so the
+    // records pass through the ABI without conversion.
+    LetTree export_let(const Box3& target, const ForceParams& p) override {
+        const lhip_force_params c = to_c(p);
+        uint64_t nn = 0, nb = 0;
+        ck(lhip_export_let(c_, target.lo, target.hi, &c, &nn, &nb));
+        LetTree out;
+        out.nodes.resize(nn);
+        out.bx.resize(nb); out.by.resize(nb); out.bz.resize(nb); out.bm.resize(nb);
+        ck(lhip_export_let_copy(c_, reinterpret_cast<lhip_let_node*>(out.nodes.data()), out.bx.data(), out.by.data(),
+                                out.bz.data(), out.bm.data()));
+        return out;
     }
 
     void compute_forces_with(const ForceParams& p, bool potential, const std::vector<const LetTree*>& remote) override {
-        if (!remote.empty())
-            throw std::logic_error("hip compute_forces_with: remote trees (multi-rank) are not supported");
         const lhip_force_params c = to_c(p);
-        ck(lhip_compute_forces_with(c_, &c, potential ? 1 : 0));
+        if (remote.empty()) {
+            ck(lhip_compute_forces_with(c_, &c, potential ? 1 : 0));
+            return;
+        }
+        std::vector<lhip_let> lets;
+        lets.reserve(remote.size());
+        for (const LetTree* t : remote) {
+            if (!t) continue;
+            const std::size_t nb = t->bx.size();
+            if (t->by.size() != nb || t->bz.size() != nb || t->bm.size() != nb)
+                throw std::runtime_error("hip compute_forces_with: LET body arrays differ in length");
+            lets.push_back(lhip_let{reinterpret_cast<const lhip_let_node*>(t->nodes.data()), t->nodes.size(),
+                                    t->bx.data(), t->by.data(), t->bz.data(), t->bm.data(), nb});
+        }
+        ck(lhip_compute_forces_remote(c_, &c, potential ? 1 : 0, lets.data(), int(lets.size())));
     }
 
     MemoryNeed memory_need(uint64_t n, ForceMethod m) const override {

@@ -165,6 +165,42 @@ def sim_backward(gxT, gvT):
     return gx0, gv0, gm
 
 
+# LetNode (src/backend.hpp:63-76): the 60-byte wire record of a locally essential tree.
+LET_NODE = np.dtype([(k, "<f4") for k in ("cx", "cy", "cz", "mass", "qxx", "qxy", "qxz", "qyy", "qyz", "qzz",
+                                          "rcrit2")] +
+                    [(k, "<i4") for k in ("first_child", "child_count", "body_first", "body_count")])
+
+
+@dataclass
+class Let:
+    """LetTree (src/backend.hpp:84-87): LET_NODE records and the exported bodies."""
+    nodes: np.ndarray
+    bx: np.ndarray
+    by: np.ndarray
+    bz: np.ndarray
+    bm: np.ndarray
+
+    @staticmethod
+    def sized(nn: int, nb: int) -> "Let":
+        return Let(np.zeros(nn, LET_NODE), *[np.zeros(nb, np.float32) for _ in range(4)])
+
+
+class _FlatLet(ctypes.Structure):
+    _fields_ = [("nodes", _P), ("n_nodes", ctypes.c_uint64), ("bx", _P), ("by", _P), ("bz", _P), ("bm", _P),
+                ("n_bodies", ctypes.c_uint64)]
+
+
+def _flat(remote):
+    """(ctypes array of flat LETs, arrays to keep alive) for objects with nodes, bx, by, bz, bm."""
+    keep, arr = [], (_FlatLet * max(len(remote), 1))()
+    for k, t in enumerate(remote):
+        nodes = np.ascontiguousarray(t.nodes, LET_NODE)
+        cols = [np.ascontiguousarray(v, np.float32) for v in (t.bx, t.by, t.bz, t.bm)]
+        keep += [nodes, *cols]
+        arr[k] = _FlatLet(_ptr(nodes), len(nodes), *[_ptr(v) for v in cols], len(cols[0]))
+    return arr, keep
+
+
 class _Common:
     """Accelerations are read as the reference tests do: upload with v = 0, compute, kick(1), read v
     (tests/test_cpu_backend.cpp:45-60)."""
@@ -271,6 +307,30 @@ class Oracle(_Common):
         self.L.orc_groups(self.h, _ptr(g))
         return g
 
+    # ---- distributed support (src/backend.hpp:114-136) ----
+    def local_bounds(self):
+        lo, hi = (ctypes.c_float * 3)(), (ctypes.c_float * 3)()
+        self.L.orc_local_bounds(self.h, lo, hi)
+        return list(lo), list(hi)
+
+    def build_tree(self, theta=0.6, method=1):
+        self.L.orc_build_tree(self.h, _D(theta), int(method))
+
+    def export_let(self, box, method=1) -> Let:
+        lo = (ctypes.c_float * 3)(*[float(v) for v in box[0]])
+        hi = (ctypes.c_float * 3)(*[float(v) for v in box[1]])
+        nn, nb = _U64(), _U64()
+        if self.L.orc_export_let(self.h, lo, hi, int(method), ctypes.byref(nn), ctypes.byref(nb)):
+            raise RuntimeError(self.L.orc_last_error().decode())
+        t = Let.sized(nn.value, nb.value)
+        self.L.orc_export_let_copy(self.h, _ptr(t.nodes), _ptr(t.bx), _ptr(t.by), _ptr(t.bz), _ptr(t.bm))
+        return t
+
+    def compute_forces_with(self, G=1.0, eps=0.01, method=1, potential=False, remote=()):
+        arr, _keep = _flat(remote)
+        if self.L.orc_forces_with(self.h, _D(G), _D(eps), int(method), int(potential), arr, len(remote)):
+            raise RuntimeError(self.L.orc_last_error().decode())
+
 
 class RefCpu(_Common):
     """The reference's own CPU backend (oracle/_ref/libref.so), where it was built."""
@@ -325,6 +385,32 @@ class RefCpu(_Common):
     def body_costs(self):
         c = np.zeros(self.n, np.float32)
         return c if self.R.ref_body_costs(self.h, _ptr(c)) else None
+
+    # ---- distributed support (src/backend.hpp:114-136) ----
+    def local_bounds(self):
+        lo, hi = (ctypes.c_float * 3)(), (ctypes.c_float * 3)()
+        self.R.ref_local_bounds(self.h, lo, hi)
+        return list(lo), list(hi)
+
+    def build_tree(self, G=1.0, eps=0.01, theta=0.6, method=1):
+        if self.R.ref_build_tree(self.h, _D(G), _D(eps), _D(theta), int(method)):
+            raise RuntimeError(self.R.ref_last_error().decode())
+
+    def export_let(self, box, G=1.0, eps=0.01, theta=0.6, method=1) -> Let:
+        lo = (ctypes.c_float * 3)(*[float(v) for v in box[0]])
+        hi = (ctypes.c_float * 3)(*[float(v) for v in box[1]])
+        nn, nb = _U64(), _U64()
+        if self.R.ref_export_let(self.h, lo, hi, _D(G), _D(eps), _D(theta), int(method), ctypes.byref(nn),
+                                 ctypes.byref(nb)):
+            raise RuntimeError(self.R.ref_last_error().decode())
+        t = Let.sized(nn.value, nb.value)
+        self.R.ref_export_let_copy(_ptr(t.nodes), _ptr(t.bx), _ptr(t.by), _ptr(t.bz), _ptr(t.bm))
+        return t
+
+    def compute_forces_with(self, G=1.0, eps=0.01, theta=0.6, method=1, potential=False, remote=()):
+        arr, _keep = _flat(remote)
+        if self.R.ref_forces_with(self.h, _D(G), _D(eps), _D(theta), int(method), int(potential), arr, len(remote)):
+            raise RuntimeError(self.R.ref_last_error().decode())
 
     def export_tree(self, G=1.0, eps=0.01, theta=0.6) -> dict:
         """The whole octree, breadth first (build_tree + export_let over a box holding every body)."""

@@ -15,6 +15,9 @@
 //   walk forces          reference accelerations (rel rms), interaction counts and per-body costs
 //                        exactly; theta = 0 equals direct; FP64 brute force bounds
 //   direct forces        FP64 brute force (rel rms <= 1e-6), two-body closed form
+//   export_let / remote  the reference's own export_let and compute_forces_with(remote) (oracle/_ref):
+//     LETs               LETs field for field, forces, costs and counts on two-rank splits; two ranks
+//                        against one direct sum; an empty LET changes nothing
 //   kick/drift/diag      closed-form sums in numpy
 //   scenario_disk_sech2  NOT a reference function (the reference disk is Gaussian in z): pinned by
 //                        closed-form moments of the exponential / sech^2 profiles only
@@ -96,6 +99,81 @@ struct Node {
 };
 
 struct Group { float bx, by, bz, hx, hy, hz; uint32_t lo, hi; };
+
+// Locally essential tree in the reference's wire format (LetNode, LetTree: src/backend.hpp:63-87).
+struct LetNodeO {
+    float cx, cy, cz, mass;
+    float qxx, qxy, qxz, qyy, qyz, qzz;
+    float rcrit2;
+    int32_t first_child, child_count, body_first, body_count;
+};
+static_assert(sizeof(LetNodeO) == 60, "LetNode is a 60-byte record");
+struct LetO {
+    std::vector<LetNodeO> nodes;
+    std::vector<float> bx, by, bz, bm;
+};
+struct RBody { float x, y, z, m; };
+
+// One list entry's terms for a target at (xi, yi, zi): a softened body (inv = 0 for r2 = 0, as the
+// reference's masked rsqrt) and a monopole + quadrupole cell (src/cpu_backend.cpp:816-882).
+inline void pair_term(float px, float py, float pz, float pm, float xi, float yi, float zi, float e2, float& sx,
+                      float& sy, float& sz, float& sp) {
+    const float dx = px - xi, dy = py - yi, dz = pz - zi;
+    const float r2 = dx * dx + dy * dy + dz * dz + e2;
+    const float inv = r2 > 0.f ? 1.f / std::sqrt(r2) : 0.f;
+    const float mi = pm * inv, f = mi * inv * inv;
+    sx += f * dx; sy += f * dy; sz += f * dz; sp += mi;
+}
+inline void cell_term(float cx, float cy, float cz, float cm, const float* q, float xi, float yi, float zi, float e2,
+                      float& sx, float& sy, float& sz, float& sp) {
+    const float dx = cx - xi, dy = cy - yi, dz = cz - zi;
+    const float r2 = dx * dx + dy * dy + dz * dz + e2;
+    const float inv = r2 > 0.f ? 1.f / std::sqrt(r2) : 0.f;
+    const float inv2 = inv * inv, inv3 = inv * inv2, inv5 = inv3 * inv2;
+    // q: xx, xy, xz, yy, yz, zz
+    const float qx = q[0] * dx + q[1] * dy + q[2] * dz;
+    const float qy = q[1] * dx + q[3] * dy + q[4] * dz;
+    const float qz = q[2] * dx + q[4] * dy + q[5] * dz;
+    const float dqd = dx * qx + dy * qy + dz * qz;
+    const float f = cm * inv3 + 2.5f * dqd * inv5 * inv2;
+    sx += f * dx - qx * inv5; sy += f * dy - qy * inv5; sz += f * dz - qz * inv5;
+    sp += cm * inv + 0.5f * dqd * inv5;
+}
+
+// Receiver walk of a remote LET against a group box (src/cpu_backend.cpp:788-814, rules of
+// src/backend.hpp:78-83): a terminal node (no children, no bodies) or one whose com is at rcrit2 or
+// beyond is accepted, as its bodies if it has 1..kSmallLeaf of them, else as its multipole; an
+// opened node adds its bodies if it has any, else its children (depth-first, first child first).
+void let_list(const LetO& t, const Group& g, std::vector<RBody>& rb, std::vector<LetNodeO>& rc,
+              std::vector<int32_t>& st) {
+    if (t.nodes.empty()) return;
+    st.clear();
+    st.push_back(0);
+    auto bodies = [&](const LetNodeO& nd) {
+        for (int32_t j = nd.body_first; j < nd.body_first + nd.body_count; ++j)
+            rb.push_back({t.bx[j], t.by[j], t.bz[j], t.bm[j]});
+    };
+    while (!st.empty()) {
+        const LetNodeO& nd = t.nodes[std::size_t(st.back())];
+        st.pop_back();
+        const bool terminal = nd.first_child < 0 && nd.body_count == 0;
+        bool accept = terminal;
+        if (!terminal) {
+            const float dx = std::max(std::fabs(nd.cx - g.bx) - g.hx, 0.f);
+            const float dy = std::max(std::fabs(nd.cy - g.by) - g.hy, 0.f);
+            const float dz = std::max(std::fabs(nd.cz - g.bz) - g.hz, 0.f);
+            accept = dx * dx + dy * dy + dz * dz >= nd.rcrit2;
+        }
+        if (accept) {
+            if (nd.body_count > 0 && nd.body_count <= kSmallLeaf) bodies(nd);
+            else rc.push_back(nd);
+        } else if (nd.body_count > 0) {
+            bodies(nd);
+        } else {
+            for (int32_t c = nd.first_child + nd.child_count; c-- > nd.first_child;) st.push_back(c);
+        }
+    }
+}
 
 struct Oracle {
     int nt = 1;
@@ -389,13 +467,18 @@ struct Oracle {
     // Per group: the interaction list against the group's bounding box. A node is accepted when it
     // does not contain the group and dist^2(box, com) >= rcrit2; accepted leaves with <= 4 bodies and
     // opened leaves contribute their bodies, other accepted nodes their multipole.
-    void walk(float G, float e2, bool potential) {
+    // Remote LETs (src/cpu_backend.cpp:667-689): each group's list continues with every remote
+    // tree's entries, bodies after the local bodies and cells after the local cells.
+    void walk(float G, float e2, bool potential, const std::vector<const LetO*>& remote = {}) {
         cost.assign(n, 0.f);
         uint64_t inter = 0, lst = 0, npp = 0, npc = 0;
 #pragma omp parallel num_threads(nt) reduction(+ : inter, lst, npp, npc)
         {
             std::vector<uint32_t> st, cells;
             std::vector<uint32_t> bods;
+            std::vector<RBody> rb;
+            std::vector<LetNodeO> rc;
+            std::vector<int32_t> lst_stack;
 #pragma omp for schedule(dynamic, 4)
             for (long gi = 0; gi < long(groups.size()); ++gi) {
                 const Group& g = groups[gi];
@@ -420,39 +503,33 @@ struct Oracle {
                         for (uint32_t c = nd.first + nd.nchild; c-- > nd.first;) st.push_back(c);
                     }
                 }
+                rb.clear(); rc.clear();
+                for (const LetO* t : remote) let_list(*t, g, rb, rc, lst_stack);
                 for (uint32_t i = g.lo; i < g.hi; ++i) {
                     float sx = 0, sy = 0, sz = 0, sp = 0;
                     uint64_t self = 0;
                     for (uint32_t j : bods) {
                         if (j == i) { ++self; continue; }
-                        const float ddx = x[j] - x[i], ddy = y[j] - y[i], ddz = z[j] - z[i];
-                        const float r2 = ddx * ddx + ddy * ddy + ddz * ddz + e2;
-                        const float inv = r2 > 0.f ? 1.f / std::sqrt(r2) : 0.f;
-                        const float mi = m[j] * inv, f = mi * inv * inv;
-                        sx += f * ddx; sy += f * ddy; sz += f * ddz; sp += mi;
+                        pair_term(x[j], y[j], z[j], m[j], x[i], y[i], z[i], e2, sx, sy, sz, sp);
                     }
+                    for (const RBody& r : rb) pair_term(r.x, r.y, r.z, r.m, x[i], y[i], z[i], e2, sx, sy, sz, sp);
                     for (uint32_t c : cells) {
                         const Node& nd = nodes[c];
-                        const float ddx = nd.cx - x[i], ddy = nd.cy - y[i], ddz = nd.cz - z[i];
-                        const float r2 = ddx * ddx + ddy * ddy + ddz * ddz + e2;
-                        const float inv = r2 > 0.f ? 1.f / std::sqrt(r2) : 0.f;
-                        const float inv2 = inv * inv, inv3 = inv * inv2, inv5 = inv3 * inv2;
-                        const float qx = nd.qxx * ddx + nd.qxy * ddy + nd.qxz * ddz;
-                        const float qy = nd.qxy * ddx + nd.qyy * ddy + nd.qyz * ddz;
-                        const float qz = nd.qxz * ddx + nd.qyz * ddy + nd.qzz * ddz;
-                        const float dqd = ddx * qx + ddy * qy + ddz * qz;
-                        const float f = nd.m * inv3 + 2.5f * dqd * inv5 * inv2;
-                        sx += f * ddx - qx * inv5; sy += f * ddy - qy * inv5; sz += f * ddz - qz * inv5;
-                        sp += nd.m * inv + 0.5f * dqd * inv5;
+                        const float q[6] = {nd.qxx, nd.qxy, nd.qxz, nd.qyy, nd.qyz, nd.qzz};
+                        cell_term(nd.cx, nd.cy, nd.cz, nd.m, q, x[i], y[i], z[i], e2, sx, sy, sz, sp);
+                    }
+                    for (const LetNodeO& nd : rc) {
+                        const float q[6] = {nd.qxx, nd.qxy, nd.qxz, nd.qyy, nd.qyz, nd.qzz};
+                        cell_term(nd.cx, nd.cy, nd.cz, nd.mass, q, x[i], y[i], z[i], e2, sx, sy, sz, sp);
                     }
                     ax[i] = G * sx; ay[i] = G * sy; az[i] = G * sz;
                     if (potential) pot[i] = -G * sp;
-                    const uint64_t len = bods.size() + cells.size();
+                    const uint64_t len = bods.size() + cells.size() + rb.size() + rc.size();
                     cost[i] = float(len);
                     lst += len;
                     inter += len - self;
-                    npp += bods.size() - self;
-                    npc += cells.size();
+                    npp += bods.size() + rb.size() - self;
+                    npc += cells.size() + rc.size();
                 }
             }
         }
@@ -468,7 +545,145 @@ struct Oracle {
         build_tree(theta);
         walk(g, e2, potential);
     }
+
+    // ---- distributed support (src/backend.hpp:114-136, src/cpu_backend.cpp:165-258) ----
+    void build(double theta, int method) {
+        tree_ok = false;
+        if (method == 0 || n == 0) return;
+        sort_bodies();
+        build_tree(theta);
+    }
+
+    // Direct method with remote LETs: local pairs, then the remote trees per block of kGroupMax
+    // consecutive bodies in the current order, each block's box from its bodies
+    // (src/cpu_backend.cpp:691-725); costs n + the remote list length.
+    void forces_with(double G, double eps, int method, bool potential, const std::vector<const LetO*>& remote) {
+        has_pot = potential;
+        if (n == 0) { interactions = listed = p2p = p2c = 0; return; }
+        const float g = float(G), e2 = float(eps * eps);
+        if (method == 1) {
+            if (!tree_ok) throw std::logic_error("oracle compute_forces_with: build the tree first");
+            walk(g, e2, potential, remote);
+            return;
+        }
+        direct(g, e2, potential);
+        uint64_t inter = 0, npp = 0, npc = 0;
+        const long nblk = long((n + kGroupMax - 1) / kGroupMax);
+#pragma omp parallel num_threads(nt) reduction(+ : inter, npp, npc)
+        {
+            std::vector<RBody> rb;
+            std::vector<LetNodeO> rc;
+            std::vector<int32_t> st;
+#pragma omp for schedule(dynamic, 1)
+            for (long b = 0; b < nblk; ++b) {
+                Group gr{};
+                gr.lo = uint32_t(b * kGroupMax);
+                gr.hi = uint32_t(std::min<std::size_t>(n, std::size_t(b + 1) * kGroupMax));
+                float mn[3], mx[3];
+                bbox(gr.lo, gr.hi, mn, mx);
+                gr.bx = 0.5f * (mn[0] + mx[0]); gr.by = 0.5f * (mn[1] + mx[1]); gr.bz = 0.5f * (mn[2] + mx[2]);
+                gr.hx = 0.5f * (mx[0] - mn[0]); gr.hy = 0.5f * (mx[1] - mn[1]); gr.hz = 0.5f * (mx[2] - mn[2]);
+                rb.clear(); rc.clear();
+                for (const LetO* t : remote) let_list(*t, gr, rb, rc, st);
+                for (uint32_t i = gr.lo; i < gr.hi; ++i) {
+                    float sx = 0, sy = 0, sz = 0, sp = 0;
+                    for (const RBody& r : rb) pair_term(r.x, r.y, r.z, r.m, x[i], y[i], z[i], e2, sx, sy, sz, sp);
+                    for (const LetNodeO& nd : rc) {
+                        const float q[6] = {nd.qxx, nd.qxy, nd.qxz, nd.qyy, nd.qyz, nd.qzz};
+                        cell_term(nd.cx, nd.cy, nd.cz, nd.mass, q, x[i], y[i], z[i], e2, sx, sy, sz, sp);
+                    }
+                    ax[i] += g * sx; ay[i] += g * sy; az[i] += g * sz;
+                    if (potential) pot[i] -= g * sp;
+                    cost[i] += float(rb.size() + rc.size());
+                }
+                const uint64_t nb = gr.hi - gr.lo;
+                inter += (rb.size() + rc.size()) * nb;
+                npp += rb.size() * nb;
+                npc += rc.size() * nb;
+            }
+        }
+        interactions += inter; listed += inter; p2p += npp; p2c += npc;
+    }
+
+    // export_let (src/cpu_backend.cpp:222-258): breadth first from the root, the output doubling as
+    // the queue; a node is terminal when min over the box of |x - com|^2 >= rcrit2 (box as float
+    // centre and half extent), an opened leaf exports its bodies, an opened internal node appends
+    // its children of positive mass as one block. Direct method: export_direct
+    // (src/cpu_backend.cpp:727-745), one always-open root with every body.
+    LetO export_let(const float* lo, const float* hi, int method) const {
+        LetO out;
+        if (n == 0) return out;
+        if (method == 0) {
+            out.bx = x; out.by = y; out.bz = z; out.bm = m;
+            double M = 0, sx = 0, sy = 0, sz = 0;
+            for (std::size_t i = 0; i < n; ++i) {
+                const double w = m[i];
+                M += w; sx += w * x[i]; sy += w * y[i]; sz += w * z[i];
+            }
+            LetNodeO r{};
+            if (M > 0) { r.cx = float(sx / M); r.cy = float(sy / M); r.cz = float(sz / M); }
+            r.mass = float(M);
+            r.rcrit2 = 3.0e38f;
+            r.first_child = -1; r.child_count = 0; r.body_first = 0; r.body_count = int32_t(n);
+            out.nodes.push_back(r);
+            return out;
+        }
+        if (!tree_ok) throw std::logic_error("oracle export_let: build the tree first");
+        if (!(nodes[0].m > 0)) return out;
+        const float bcx = 0.5f * (lo[0] + hi[0]), bhx = 0.5f * (hi[0] - lo[0]);
+        const float bcy = 0.5f * (lo[1] + hi[1]), bhy = 0.5f * (hi[1] - lo[1]);
+        const float bcz = 0.5f * (lo[2] + hi[2]), bhz = 0.5f * (hi[2] - lo[2]);
+        std::vector<uint32_t> src;
+        auto emit = [&](uint32_t ni) {
+            const Node& d = nodes[ni];
+            out.nodes.push_back(LetNodeO{d.cx, d.cy, d.cz, d.m, d.qxx, d.qxy, d.qxz, d.qyy, d.qyz, d.qzz, d.rcrit2,
+                                         -1, 0, -1, 0});
+            src.push_back(ni);
+        };
+        emit(0);
+        for (std::size_t k = 0; k < out.nodes.size(); ++k) {
+            const Node& d = nodes[src[k]];
+            const float dx = std::max(std::fabs(d.cx - bcx) - bhx, 0.f);
+            const float dy = std::max(std::fabs(d.cy - bcy) - bhy, 0.f);
+            const float dz = std::max(std::fabs(d.cz - bcz) - bhz, 0.f);
+            if (dx * dx + dy * dy + dz * dz >= d.rcrit2) continue;
+            if (d.nchild == 0) {
+                out.nodes[k].body_first = int32_t(out.bx.size());
+                out.nodes[k].body_count = int32_t(d.hi - d.lo);
+                for (uint32_t j = d.lo; j < d.hi; ++j) {
+                    out.bx.push_back(x[j]); out.by.push_back(y[j]); out.bz.push_back(z[j]); out.bm.push_back(m[j]);
+                }
+                continue;
+            }
+            const int32_t first = int32_t(out.nodes.size());
+            for (uint32_t c = d.first; c < d.first + d.nchild; ++c)
+                if (nodes[c].m > 0) emit(c);
+            out.nodes[k].first_child = first;
+            out.nodes[k].child_count = int32_t(out.nodes.size()) - first;
+        }
+        return out;
+    }
+    LetO exported;
 };
+
+// LETs as flat arrays (the layout of lhip_let in include/lettuce_hip.h).
+struct FlatLet {
+    const void* nodes;
+    uint64_t n_nodes;
+    const float *bx, *by, *bz, *bm;
+    uint64_t n_bodies;
+};
+std::vector<LetO> unflatten(const FlatLet* lets, int nlets) {
+    std::vector<LetO> v(std::size_t(std::max(nlets, 0)));
+    for (int k = 0; k < nlets; ++k) {
+        const FlatLet& f = lets[k];
+        const auto* nd = static_cast<const LetNodeO*>(f.nodes);
+        v[k].nodes.assign(nd, nd + f.n_nodes);
+        v[k].bx.assign(f.bx, f.bx + f.n_bodies); v[k].by.assign(f.by, f.by + f.n_bodies);
+        v[k].bz.assign(f.bz, f.bz + f.n_bodies); v[k].bm.assign(f.bm, f.bm + f.n_bodies);
+    }
+    return v;
+}
 
 thread_local std::string g_err;
 
@@ -747,6 +962,53 @@ void orc_keys(void* h, uint64_t* keys, uint32_t* perm) {
 void orc_cube(void* h, double* out) {
     const Oracle& o = *static_cast<Oracle*>(h);
     out[0] = o.ox; out[1] = o.oy; out[2] = o.oz; out[3] = o.side;
+}
+
+// ---- distributed support ----
+void orc_local_bounds(void* h, float* lo, float* hi) {
+    const Oracle& o = *static_cast<Oracle*>(h);
+    for (int c = 0; c < 3; ++c) lo[c] = hi[c] = 0.f;
+    if (o.n) o.bbox(0, o.n, lo, hi);
+}
+
+int orc_build_tree(void* h, double theta, int method) {
+    static_cast<Oracle*>(h)->build(theta, method);
+    return 0;
+}
+
+int orc_forces_with(void* h, double G, double eps, int method, int potential, const FlatLet* lets, int nlets) {
+    try {
+        const std::vector<LetO> v = unflatten(lets, nlets);
+        std::vector<const LetO*> r;
+        for (const LetO& t : v) r.push_back(&t);
+        static_cast<Oracle*>(h)->forces_with(G, eps, method, potential != 0, r);
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+// export_let for the box [lo, hi]: returns the sizes; orc_export_let_copy copies the LET out.
+int orc_export_let(void* h, const float* lo, const float* hi, int method, uint64_t* nn, uint64_t* nb) {
+    try {
+        Oracle& o = *static_cast<Oracle*>(h);
+        o.exported = o.export_let(lo, hi, method);
+        *nn = o.exported.nodes.size();
+        *nb = o.exported.bx.size();
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+void orc_export_let_copy(void* h, void* nodes, float* bx, float* by, float* bz, float* bm) {
+    const LetO& t = static_cast<Oracle*>(h)->exported;
+    std::memcpy(nodes, t.nodes.data(), t.nodes.size() * sizeof(LetNodeO));
+    const std::size_t b = t.bx.size() * 4;
+    std::memcpy(bx, t.bx.data(), b); std::memcpy(by, t.by.data(), b);
+    std::memcpy(bz, t.bz.data(), b); std::memcpy(bm, t.bm.data(), b);
 }
 
 uint64_t orc_node_count(void* h) { return static_cast<Oracle*>(h)->nodes.size(); }

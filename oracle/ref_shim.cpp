@@ -163,4 +163,74 @@ int ref_export_tree(void* h, double G, double eps, double theta, uint64_t* nnode
     }
 }
 
+// ---- distributed support: local_bounds, build_tree, export_let, compute_forces_with
+// (src/backend.hpp:114-136). LETs cross as flat arrays (the layout of lhip_let in
+// include/lettuce_hip.h).
+struct FlatLet {
+    const void* nodes;
+    uint64_t n_nodes;
+    const float *bx, *by, *bz, *bm;
+    uint64_t n_bodies;
+};
+static thread_local LetTree g_let;
+
+void ref_local_bounds(void* h, float* lo, float* hi) {
+    const Box3 b = static_cast<RefCtx*>(h)->be->local_bounds();
+    for (int c = 0; c < 3; ++c) { lo[c] = b.lo[c]; hi[c] = b.hi[c]; }
+}
+
+int ref_build_tree(void* h, double G, double eps, double theta, int method) {
+    try {
+        static_cast<RefCtx*>(h)->be->build_tree(make_params(G, eps, theta, method));
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+// export_let for the box [lo, hi]: returns the sizes; ref_export_let_copy copies the LET out.
+int ref_export_let(void* h, const float* lo, const float* hi, double G, double eps, double theta, int method,
+                   uint64_t* nn, uint64_t* nb) {
+    try {
+        Box3 b;
+        for (int c = 0; c < 3; ++c) { b.lo[c] = lo[c]; b.hi[c] = hi[c]; }
+        g_let = static_cast<RefCtx*>(h)->be->export_let(b, make_params(G, eps, theta, method));
+        *nn = g_let.nodes.size();
+        *nb = g_let.bx.size();
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+void ref_export_let_copy(void* nodes, float* bx, float* by, float* bz, float* bm) {
+    std::memcpy(nodes, g_let.nodes.data(), g_let.nodes.size() * sizeof(LetNode));
+    const std::size_t b = g_let.bx.size() * 4;
+    std::memcpy(bx, g_let.bx.data(), b); std::memcpy(by, g_let.by.data(), b);
+    std::memcpy(bz, g_let.bz.data(), b); std::memcpy(bm, g_let.bm.data(), b);
+}
+
+int ref_forces_with(void* h, double G, double eps, double theta, int method, int potential, const FlatLet* lets,
+                    int nlets) {
+    try {
+        std::vector<LetTree> v(std::size_t(nlets > 0 ? nlets : 0));
+        std::vector<const LetTree*> r;
+        for (int k = 0; k < nlets; ++k) {
+            const FlatLet& f = lets[k];
+            const auto* nd = static_cast<const LetNode*>(f.nodes);
+            v[k].nodes.assign(nd, nd + f.n_nodes);
+            v[k].bx.assign(f.bx, f.bx + f.n_bodies); v[k].by.assign(f.by, f.by + f.n_bodies);
+            v[k].bz.assign(f.bz, f.bz + f.n_bodies); v[k].bm.assign(f.bm, f.bm + f.n_bodies);
+            r.push_back(&v[k]);
+        }
+        static_cast<RefCtx*>(h)->be->compute_forces_with(make_params(G, eps, theta, method), potential != 0, r);
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
 }  // extern "C"

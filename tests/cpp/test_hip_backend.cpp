@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <functional>
 #include <stdexcept>
 #include <string>
@@ -196,6 +197,95 @@ TEST("hip: build_tree invalidates accelerations and potential") {
     CHECK(be->body_costs(cost) && cost.size() == b.size());
     uint64_t fr = 0, tot = 0;
     CHECK(be->device_memory(fr, tot) && tot > 0 && fr <= tot + (uint64_t(1) << 40));
+}
+
+// ---- distributed hooks (the reference's tests/test_cuda_let.cpp): two contexts stand in for two ranks.
+
+TEST("hip let: two halves with LETs vs one direct sum, 20k Plummer") {
+    auto ref = backend();
+    auto r0 = backend();
+    auto r1 = backend();
+    Bodies all = plummer(20000, 13, 1.0, 10.0);
+    std::fill(all.vx.begin(), all.vx.end(), 0.f);
+    std::fill(all.vy.begin(), all.vy.end(), 0.f);
+    std::fill(all.vz.begin(), all.vz.end(), 0.f);
+    const Got d = accel(*ref, all, params(ForceMethod::direct, 0.01), true);
+    Bodies h[2];
+    for (std::size_t i = 0; i < all.size(); ++i) {
+        Bodies& t = h[all.x[i] < 0.f ? 0 : 1];
+        t.push(all.x[i], all.y[i], all.z[i], 0, 0, 0, all.m[i]);
+        t.id.back() = all.id[i];
+    }
+    for (ForceMethod m : {ForceMethod::tree, ForceMethod::direct}) {
+        const bool tree = m == ForceMethod::tree;
+        const ForceParams p = params(m, 0.01);
+        Backend* be[2] = {r0.get(), r1.get()};
+        for (int k = 0; k < 2; ++k) { be[k]->upload(h[k]); be[k]->build_tree(p); }
+        const LetTree from1 = be[1]->export_let(be[0]->local_bounds(), p);
+        const LetTree from0 = be[0]->export_let(be[1]->local_bounds(), p);
+        be[0]->compute_forces_with(p, true, {&from1});
+        be[1]->compute_forces_with(p, true, {&from0});
+        std::vector<double> a(3 * all.size(), 0.0);
+        double pe = 0;
+        uint64_t inter = 0;
+        for (int k = 0; k < 2; ++k) {
+            inter += be[k]->times().interactions;
+            pe += be[k]->diagnostics().pe;
+            std::vector<float> c;
+            CHECK(be[k]->body_costs(c) && c.size() == h[k].size());
+            double cs = 0;
+            for (float v : c) cs += v;
+            CHECK(cs == double(be[k]->times().interactions));
+            be[k]->kick(1.0);
+            Bodies o;
+            be[k]->download(o);
+            for (std::size_t i = 0; i < o.size(); ++i) {
+                a[3 * o.id[i]] = o.vx[i]; a[3 * o.id[i] + 1] = o.vy[i]; a[3 * o.id[i] + 2] = o.vz[i];
+            }
+        }
+        const double e = rel_rms(a, d.a);
+        std::fprintf(stderr, "  %s: LETs %zu/%zu and %zu/%zu, rel rms %.3g\n", tree ? "tree" : "direct",
+                     from1.nodes.size(), from1.bx.size(), from0.nodes.size(), from0.bx.size(), e);
+        CHECK(e <= (tree ? 1.5e-3 : 1e-5));
+        CHECK_NEAR(pe / d.d.pe, 1.0, tree ? 1.5e-3 : 1e-5);
+        if (!tree) CHECK(inter == uint64_t(all.size()) * (all.size() - 1));
+    }
+}
+
+TEST("hip let: empty LETs change nothing; a rank without bodies exports none") {
+    auto be = backend();
+    const ForceParams p = params(ForceMethod::tree, 0.01);
+    const Bodies b = plummer(5000, 14, 1.0, 10.0);
+    const Got g0 = accel(*be, b, p, true);
+    Bodies z = b;
+    std::fill(z.vx.begin(), z.vx.end(), 0.f);
+    std::fill(z.vy.begin(), z.vy.end(), 0.f);
+    std::fill(z.vz.begin(), z.vz.end(), 0.f);
+    be->upload(z);
+    be->build_tree(p);
+    const LetTree empty;
+    be->compute_forces_with(p, true, {&empty, &empty});
+    CHECK(be->diagnostics().pe == g0.d.pe);
+    be->kick(1.0);
+    Bodies o;
+    be->download(o);
+    bool same = true;
+    for (std::size_t i = 0; i < o.size(); ++i)
+        same = same && o.vx[i] == g0.a[3 * o.id[i]] && o.vy[i] == g0.a[3 * o.id[i] + 1] &&
+               o.vz[i] == g0.a[3 * o.id[i] + 2];
+    CHECK(same);
+    Bodies none;
+    be->upload(none);
+    be->build_tree(p);
+    CHECK(be->export_let(Box3{}, p).nodes.empty());
+    bool threw = false;
+    try {
+        be->upload(b);
+        (void)be->export_let(Box3{}, p);  // no tree since the upload
+    } catch (const std::logic_error&) {
+        threw = true;
+    }
+    CHECK(threw);
 }
 
 int main(int argc, char** argv) {

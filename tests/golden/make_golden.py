@@ -2,6 +2,7 @@
 generator (oracle/_ref/libref.so, compiled from the reference sources by oracle/Makefile).
 
 Run in the container that has the reference sources:  python tests/golden/make_golden.py
+(``--let`` writes only the LET fixture, let.npz).
 The fixtures are data only: inputs (as generator parameters plus digests of the generated arrays)
 and the reference's outputs (accelerations of sampled bodies, energies, interaction counts, digests
 of the sort permutation, keys and canonical octree).
@@ -156,5 +157,69 @@ def main():
     print("wrote fixtures to", HERE)
 
 
+LET_CASE = (4096, 5, 1.0, 10.0)  # Plummer n, seed, a, cut
+LET_FAR_BOX = ([24.8, -0.2, -0.2], [25.2, 0.2, 0.2])
+
+
+def zero_v(b: O.Bodies) -> O.Bodies:
+    b = b.copy()
+    b.vx[:] = 0
+    b.vy[:] = 0
+    b.vz[:] = 0
+    return b
+
+
+def let_arrays(t: O.Let, tag: str) -> dict:
+    return {f"{tag}__nodes": t.nodes, f"{tag}__bodies": np.stack([t.bx, t.by, t.bz, t.bm], 1)}
+
+
+def write_let():
+    """LET fixtures (src/backend.hpp:114-136): a Plummer sphere split at x = 0 into two ranks; rank 1
+    exports its LET for rank 0's box (tree at theta 0.6, and direct); rank 0 computes its forces with
+    it. Outputs per method: the LET, rank 0's accelerations, costs (rows in rank 0's upload order),
+    pe and interactions. Also the whole sphere's LETs for a far box and for a point box."""
+    b = zero_v(ref_plummer(*LET_CASE))
+    h0, h1 = b.subset(b.x < 0), b.subset(b.x >= 0)
+    arrays = {}
+    for method, tag in ((1, "tree"), (0, "direct")):
+        r0, r1 = O.RefCpu(), O.RefCpu()
+        r0.upload(h0)
+        r1.upload(h1)
+        r0.build_tree(1.0, 0.01, 0.6, method)
+        r1.build_tree(1.0, 0.01, 0.6, method)
+        box0 = r0.local_bounds()
+        let = r1.export_let(box0, 1.0, 0.01, 0.6, method)
+        r0.compute_forces_with(1.0, 0.01, 0.6, method, True, [let])
+        pe = r0.diagnostics()["pe"]
+        inter = r0.times()["interactions"]
+        costs = r0.body_costs()
+        r0.kick(1.0)
+        o = r0.download()
+        row = np.empty(int(h0.id.max()) + 1, np.int64)
+        row[h0.id] = np.arange(len(h0))
+        acc = np.zeros((len(h0), 3), np.float32)
+        cst = np.zeros(len(h0), np.float32)
+        acc[row[o.id]] = np.stack([o.vx, o.vy, o.vz], 1)
+        cst[row[o.id]] = costs
+        arrays.update(let_arrays(let, tag))
+        arrays[f"{tag}__box0"] = np.array(box0, np.float32)
+        arrays[f"{tag}__acc"] = acc
+        arrays[f"{tag}__costs"] = cst
+        arrays[f"{tag}__pe"] = np.float64(pe)
+        arrays[f"{tag}__interactions"] = np.uint64(inter)
+    R = O.RefCpu()
+    R.upload(b)
+    R.build_tree(1.0, 0.01, 0.6, 1)
+    arrays.update(let_arrays(R.export_let(LET_FAR_BOX, 1.0, 0.01, 0.6, 1), "far"))
+    p = [float(h0.x[0]), float(h0.y[0]), float(h0.z[0])]
+    arrays.update(let_arrays(R.export_let((p, p), 1.0, 0.01, 0.6, 1), "point"))
+    np.savez_compressed(os.path.join(HERE, "let.npz"), **arrays)
+    print("wrote", os.path.join(HERE, "let.npz"))
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["--let"]:
+        write_let()
+    else:
+        main()
+        write_let()

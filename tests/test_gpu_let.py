@@ -1,0 +1,494 @@
+"""GPU tests of the distributed hooks (src/backend.hpp:114-136) on the HIP backend: local_bounds,
+build_tree, export_let and compute_forces_with with remote LETs. Two backend contexts stand in for
+two ranks, as the reference's tests/test_cuda_let.cpp does; the oracle (whose LETs match the
+reference's record for record, tests/test_oracle.py) is the checker."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+import lettuce_simulate_physics_amd as L
+import oracle as O
+from helpers import rel_rms
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+LETZ = np.load(os.path.join(GOLD, "let.npz"))
+
+
+def P(method, eps=0.01, theta=0.6):
+    return L.ForceParams(G=1.0, eps=eps, theta=theta, method=method)
+
+
+def zero_v(b: O.Bodies) -> O.Bodies:
+    b = b.copy()
+    b.vx[:] = 0
+    b.vy[:] = 0
+    b.vz[:] = 0
+    return b
+
+
+def split(b: O.Bodies, by_x: bool):
+    """Halves keep their ids: x < 0 / x >= 0 (disjoint boxes) or random (overlapping boxes)."""
+    first = b.x < 0 if by_x else np.random.default_rng(99).integers(0, 2, len(b)).astype(bool)
+    return b.subset(first), b.subset(~first)
+
+
+def read_accel(be, a):
+    """Accelerations after the force call on be (bodies uploaded with v = 0), scattered by id."""
+    be.kick(1.0)
+    o = be.download()
+    a[o.id] = np.stack([o.vx, o.vy, o.vz], 1)
+
+
+def as_oracle_let(t) -> O.Let:
+    return O.Let(np.ascontiguousarray(t.nodes).view(O.LET_NODE), t.bx, t.by, t.bz, t.bm)
+
+
+def as_hip_let(t) -> L.LetTree:
+    return L.LetTree(np.ascontiguousarray(t.nodes).view(L.LET_NODE), t.bx, t.by, t.bz, t.bm)
+
+
+def check_let_equal(h, o, exact_floats=False):
+    """Same records (structure exactly; moments to the node tolerance unless exact) and bodies."""
+    assert len(h.nodes) == len(o.nodes)
+    for f in ("first_child", "child_count", "body_first", "body_count"):
+        assert np.array_equal(h.nodes[f], o.nodes[f]), f
+    assert np.array_equal(np.stack([h.bx, h.by, h.bz, h.bm], 1), np.stack([o.bx, o.by, o.bz, o.bm], 1))
+    if exact_floats:
+        assert h.nodes.tobytes() == o.nodes.tobytes()
+        return
+    assert np.allclose(h.nodes["mass"], o.nodes["mass"], rtol=1e-6, atol=0)
+    com_h = np.stack([h.nodes[k] for k in ("cx", "cy", "cz")], 1)
+    com_o = np.stack([o.nodes[k] for k in ("cx", "cy", "cz")], 1)
+    assert np.allclose(com_h, com_o, rtol=0, atol=1e-6)
+    q = ("qxx", "qxy", "qxz", "qyy", "qyz", "qzz")
+    qh, qo = np.stack([h.nodes[k] for k in q], 1), np.stack([o.nodes[k] for k in q], 1)
+    assert np.allclose(qh, qo, rtol=1e-4, atol=1e-6 * max(np.abs(qo).max(), 1e-30))
+    r = o.nodes["rcrit2"] < 1e30
+    assert np.allclose(h.nodes["rcrit2"][r], o.nodes["rcrit2"][r], rtol=1e-6)
+
+
+@pytest.fixture(scope="module")
+def ranks():
+    """Two more backend contexts (ranks 0 and 1 of the two-rank tests)."""
+    a, why = L.make_hip_backend(0)
+    b, why2 = L.make_hip_backend(0)
+    if a is None or b is None:
+        pytest.fail(f"HIP backend unavailable on a GPU test run: {why or why2}")
+    yield a, b
+    a.close()
+    b.close()
+
+
+# ---------------------------------------------------------------- reference suite (test_cuda_let.cpp)
+
+def test_local_bounds_is_the_bounding_box(hip):
+    b = O.plummer(10000, 11)
+    hip.upload(b)
+    lo, hi = hip.local_bounds()
+    for c, v in enumerate((b.x, b.y, b.z)):
+        assert lo[c] == v.min() and hi[c] == v.max()
+
+
+@pytest.mark.parametrize("method", ["tree", "direct"])
+def test_compute_forces_equals_build_tree_plus_compute_forces_with(hip, method):
+    b = zero_v(O.plummer(20000, 12))
+    a0, a1 = np.zeros((len(b), 3)), np.zeros((len(b), 3))
+    hip.upload(b)
+    hip.compute_forces(P(method), True)
+    pe0 = hip.diagnostics()["pe"]
+    read_accel(hip, a0)
+    hip.upload(b)
+    hip.build_tree(P(method))
+    hip.compute_forces_with(P(method), True, [])
+    pe1 = hip.diagnostics()["pe"]
+    read_accel(hip, a1)
+    assert np.array_equal(a0, a1) and pe0 == pe1
+
+
+def two_rank(r0, r1, h0, h1, nid, p, pot):
+    """Each half's forces from its own bodies plus the other half's LET, as the MPI layer does."""
+    out = dict(a=np.zeros((nid, 3)), pe=0.0, inter=[0, 0], cost_sum=[0.0, 0.0], cost_len=[0, 0], lets=[])
+    be = (r0, r1)
+    r0.upload(h0)
+    r1.upload(h1)
+    r0.build_tree(p)
+    r1.build_tree(p)
+    box0, box1 = r0.local_bounds(), r1.local_bounds()
+    from1, from0 = r1.export_let(box0, p), r0.export_let(box1, p)
+    out["lets"] = [from1, from0]
+    r0.compute_forces_with(p, pot, [from1])
+    r1.compute_forces_with(p, pot, [from0])
+    for k in range(2):
+        out["inter"][k] = be[k].times()["interactions"]
+        if pot:
+            out["pe"] += be[k].diagnostics()["pe"]
+        c = be[k].body_costs()
+        out["cost_len"][k] = len(c)
+        out["cost_sum"][k] = float(c.astype(np.float64).sum())
+        read_accel(be[k], out["a"])
+    return out
+
+
+@pytest.mark.parametrize("by_x", [True, False], ids=["x_split", "random_split"])
+@pytest.mark.parametrize("method", ["tree", "direct"])
+@pytest.mark.parametrize("pot", [False, True], ids=["acc", "pot"])
+def test_two_halves_with_lets_vs_single_direct(hip, ranks, by_x, method, pot):
+    b = zero_v(O.plummer(20000, 13))
+    d = np.zeros((len(b), 3))
+    hip.upload(b)
+    hip.compute_forces(P("direct"), True)
+    pe_d = hip.diagnostics()["pe"]
+    read_accel(hip, d)
+    h0, h1 = split(b, by_x)
+    tree = method == "tree"
+    r = two_rank(*ranks, h0, h1, len(b), P(method), pot)
+    e = rel_rms(r["a"], d)
+    print(f"{'x' if by_x else 'random'} split, {method}, pot {pot}: rel rms {e:.3g}, LETs "
+          f"{len(r['lets'][0].nodes)}/{len(r['lets'][0].bx)} and {len(r['lets'][1].nodes)}/{len(r['lets'][1].bx)}")
+    assert np.isfinite(r["a"]).all()
+    assert e <= (1.5e-3 if tree else 1e-5)
+    if pot:
+        assert abs(r["pe"] / pe_d - 1) <= (1.5e-3 if tree else 1e-5)
+    for k, n in enumerate((len(h0), len(h1))):
+        assert r["cost_len"][k] == n and r["cost_sum"][k] > 0
+        assert r["cost_sum"][k] == float(r["inter"][k])  # per-body counts, self excluded, add up
+    if not tree:
+        assert r["inter"][0] + r["inter"][1] == len(b) * (len(b) - 1)
+
+
+def test_empty_let_and_far_box(hip, ranks):
+    p = P("tree")
+    b = zero_v(O.plummer(5000, 14))
+    a0, a1 = np.zeros((len(b), 3)), np.zeros((len(b), 3))
+    hip.upload(b)
+    hip.compute_forces(p, True)
+    pe0 = hip.diagnostics()["pe"]
+    read_accel(hip, a0)
+    empty = L.LetTree.empty()
+    hip.upload(b)
+    hip.build_tree(p)
+    hip.compute_forces_with(p, True, [empty, empty])
+    pe1 = hip.diagnostics()["pe"]
+    read_accel(hip, a1)
+    assert np.array_equal(a0, a1) and pe0 == pe1
+    src, _ = ranks
+    src.upload(O.Bodies.empty(0))
+    src.build_tree(p)
+    assert len(src.export_let(([0, 0, 0], [0, 0, 0]), p).nodes) == 0
+    # Sources: a 20k sphere (cut at 20). Targets: a massless compact cluster 60 units away, whose
+    # forces come from the LET alone; most exported nodes are terminal.
+    srcb = zero_v(O.plummer(20000, 15, 1.0, 20.0))
+    t = zero_v(O.plummer(2000, 16))
+    t.x[:] = t.x * np.float32(0.2) + np.float32(60.0)
+    t.y *= np.float32(0.2)
+    t.z *= np.float32(0.2)
+    t.m[:] = 0
+    src.upload(srcb)
+    src.build_tree(p)
+    hip.upload(t)
+    hip.build_tree(p)
+    let = src.export_let(hip.local_bounds(), p)
+    terminal = int(((let.nodes["first_child"] < 0) & (let.nodes["body_count"] == 0)).sum())
+    print(f"far LET: {len(let.nodes)} nodes ({terminal} terminal), {len(let.bx)} bodies")
+    assert len(let.nodes) > 0 and 2 * terminal > len(let.nodes)
+    hip.compute_forces_with(p, True, [let])
+    a = np.zeros((len(t), 3))
+    read_accel(hip, a)
+    ref = np.zeros((len(t), 3))
+    sx = np.stack([srcb.x, srcb.y, srcb.z], 1).astype(np.float64)
+    tx = np.stack([t.x, t.y, t.z], 1).astype(np.float64)
+    for i0 in range(0, len(t), 250):
+        d = sx[None, :, :] - tx[i0:i0 + 250, None, :]
+        inv = 1.0 / np.sqrt((d ** 2).sum(-1) + 1e-4)
+        ref[t.id[i0:i0 + 250]] = (d * (srcb.m[None, :, None] * inv[..., None] ** 3)).sum(1)
+    e = rel_rms(a, ref)
+    print(f"far LET rel rms {e:.3g}")
+    assert np.isfinite(a).all() and e <= 1.5e-3
+
+
+# ---------------------------------------------------------------- parity with the oracle
+
+def oracle_pair(h0, h1, theta, method):
+    o = [O.Oracle(), O.Oracle()]
+    for k, h in enumerate((h0, h1)):
+        o[k].upload(h)
+        o[k].build_tree(theta, method)
+    return o
+
+
+@pytest.mark.parametrize("method", ["tree", "direct"])
+def test_let_golden_fixture(ranks, method):
+    # The reference's own LET and forces for the fixture's two ranks (tests/golden/let.npz).
+    tag = method
+    b = zero_v(O.plummer(4096, 5))
+    h0, h1 = b.subset(b.x < 0), b.subset(b.x >= 0)
+    r0, r1 = ranks
+    p = P(method)
+    r0.upload(h0)
+    r1.upload(h1)
+    r0.build_tree(p)
+    r1.build_tree(p)
+    box0 = r0.local_bounds()
+    assert np.array_equal(np.array(box0, np.float32), LETZ[f"{tag}__box0"])
+    let = r1.export_let(box0, p)
+    gold = O.Let(LETZ[f"{tag}__nodes"], *[LETZ[f"{tag}__bodies"][:, c] for c in range(4)])
+    if method == "direct":
+        # the bodies in this backend's order; the root record (com summed over that order) to rounding
+        order = np.lexsort((let.bz, let.by, let.bx))
+        order_g = np.lexsort((gold.bz, gold.by, gold.bx))
+        assert np.array_equal(np.stack([let.bx, let.by, let.bz, let.bm], 1)[order],
+                              np.stack([gold.bx, gold.by, gold.bz, gold.bm], 1)[order_g])
+        for f in ("first_child", "child_count", "body_first", "body_count", "rcrit2"):
+            assert let.nodes[f][0] == gold.nodes[f][0]
+        assert np.isclose(let.nodes["mass"][0], gold.nodes["mass"][0], rtol=1e-6)
+    else:
+        check_let_equal(let, gold)
+    r0.compute_forces_with(p, True, [let])
+    pe = r0.diagnostics()["pe"]
+    inter = r0.times()["interactions"]
+    costs = r0.body_costs()
+    r0.kick(1.0)
+    o = r0.download()
+    row = np.zeros(4096, np.int64)
+    row[h0.id] = np.arange(len(h0))
+    a = np.zeros((len(h0), 3))
+    c = np.zeros(len(h0), np.float32)
+    a[row[o.id]] = np.stack([o.vx, o.vy, o.vz], 1)
+    c[row[o.id]] = costs
+    # The reference counts include each body's own entry: one per body in the local list.
+    assert np.array_equal(c + 1, LETZ[f"{tag}__costs"])
+    assert inter + len(h0) == int(LETZ[f"{tag}__interactions"])
+    assert rel_rms(a, LETZ[f"{tag}__acc"]) < 1e-5
+    assert abs(pe / float(LETZ[f"{tag}__pe"]) - 1) < 1e-5
+
+
+@pytest.mark.parametrize("by_x", [True, False], ids=["x_split", "random_split"])
+@pytest.mark.parametrize("theta", [0.6, 0.3])
+def test_let_export_and_walk_match_oracle(ranks, by_x, theta):
+    # Both directions: each rank's LET for the other's box equals the oracle's, and the receiver's
+    # lists (costs, counts) are the oracle's; forces and potential from the same lists.
+    b = zero_v(O.plummer(30000, 21))
+    h = split(b, by_x)
+    o = oracle_pair(*h, theta, 1)
+    p = P("tree", 0.01, theta)
+    for k in range(2):
+        ranks[k].upload(h[k])
+        ranks[k].build_tree(p)
+    for k in range(2):
+        box = ranks[k].local_bounds()
+        assert box == o[k].local_bounds()
+        lh = ranks[1 - k].export_let(box, p)
+        lo_ = o[1 - k].export_let(box, 1)
+        check_let_equal(lh, lo_)
+        ranks[k].compute_forces_with(p, True, [lh])
+        o[k].compute_forces_with(1.0, 0.01, 1, True, [lo_])
+        t, c = ranks[k].times(), o[k].counts()
+        assert t["listed"] == c["listed"] and t["interactions"] == c["interactions"]
+        assert t["p2p"] == c["p2p"] and t["p2c"] == c["p2c"]
+        assert np.array_equal(ranks[k].body_costs() + 1, o[k].body_costs())
+        assert abs(ranks[k].diagnostics()["pe"] / o[k].diagnostics()["pe"] - 1) < 1e-5
+        ranks[k].kick(1.0)
+        o[k].kick(1.0)
+        dh, do = ranks[k].download(), o[k].download()
+        assert np.array_equal(dh.id, do.id)
+        assert rel_rms(np.stack([dh.vx, dh.vy, dh.vz], 1), np.stack([do.vx, do.vy, do.vz], 1)) < 1e-5
+
+
+def test_let_point_and_far_boxes_match_oracle(ranks):
+    b = zero_v(O.plummer(4096, 5))
+    src = ranks[1]
+    p = P("tree")
+    src.upload(b)
+    src.build_tree(p)
+    o = O.Oracle()
+    o.upload(b)
+    o.build_tree(0.6, 1)
+    far = ([24.8, -0.2, -0.2], [25.2, 0.2, 0.2])
+    check_let_equal(src.export_let(far, p), o.export_let(far, 1))
+    gold = O.Let(LETZ["far__nodes"], *[LETZ["far__bodies"][:, c] for c in range(4)])
+    check_let_equal(src.export_let(far, p), gold)
+    q = [float(b.x[17]), float(b.y[17]), float(b.z[17])]
+    check_let_equal(src.export_let((q, q), p), o.export_let((q, q), 1))
+    lo, hi = src.local_bounds()
+    whole = src.export_let((lo, hi), p)
+    assert len(whole.bx) == len(b) and whole.nodes["body_count"].sum() == len(b)
+
+
+def test_lets_cross_between_oracle_and_hip(hip, ranks):
+    # Direct LETs (one always-open root with every body) and tree LETs for single-body receivers
+    # (the target box is a point: the receiver makes exactly the exporter's decisions), exported by
+    # one implementation and walked by the other.
+    b = zero_v(O.plummer(20000, 17))
+    h0, h1 = b.subset(b.x < 0), b.subset(b.x >= 0)
+    rx = h0.copy()
+    rx.m[:] = 0
+    exp_h = ranks[1]
+    # direct LETs: exact sums, massless receivers
+    p = P("direct")
+    exp_h.upload(h1)
+    exp_h.build_tree(p)
+    o_exp = O.Oracle()
+    o_exp.upload(h1)
+    o_exp.build_tree(0.6, 0)
+    box = (list(np.array([rx.x.min(), rx.y.min(), rx.z.min()])), list(np.array([rx.x.max(), rx.y.max(), rx.z.max()])))
+    lets = {"hip": exp_h.export_let(box, p), "oracle": as_hip_let(o_exp.export_let(box, 0))}
+    for name, let in lets.items():
+        ag, ao = np.zeros((len(b), 3)), np.zeros((len(b), 3))
+        hip.upload(rx)
+        hip.build_tree(p)
+        hip.compute_forces_with(p, False, [let])
+        read_accel(hip, ag)
+        o = O.Oracle()
+        o.upload(rx)
+        o.build_tree(0.6, 0)
+        o.compute_forces_with(1.0, 0.01, 0, False, [as_oracle_let(let)])
+        o.kick(1.0)
+        d = o.download()
+        ao[d.id] = np.stack([d.vx, d.vy, d.vz], 1)
+        e = rel_rms(ag, ao)
+        print(f"direct LET from {name}: hip vs oracle receiver rel rms {e:.3g}")
+        assert np.isfinite(ag).all() and e <= 1e-5
+    # tree LETs for single bodies
+    p = P("tree")
+    exp_h.upload(h1)
+    exp_h.build_tree(p)
+    o_exp.upload(h1)
+    o_exp.build_tree(0.6, 1)
+    for s in range(0, len(h0), len(h0) // 16):
+        one = h0.subset([s])
+        one.id[0] = 0
+        q = [float(one.x[0]), float(one.y[0]), float(one.z[0])]
+        lh = exp_h.export_let((q, q), p)
+        lo_ = o_exp.export_let((q, q), 1)
+        check_let_equal(lh, lo_)
+        for let in (lh, as_hip_let(lo_)):
+            g = np.zeros((1, 3))
+            hip.upload(one)
+            hip.build_tree(p)
+            hip.compute_forces_with(p, False, [let])
+            read_accel(hip, g)
+            o = O.Oracle()
+            o.upload(one)
+            o.build_tree(0.6, 1)
+            o.compute_forces_with(1.0, 0.01, 1, False, [as_oracle_let(let)])
+            o.kick(1.0)
+            d = o.download()
+            ref = np.array([[d.vx[0], d.vy[0], d.vz[0]]])
+            assert rel_rms(g, ref) <= 1e-5
+
+
+# ---------------------------------------------------------------- mixed precision and robustness
+
+@pytest.mark.parametrize("method", ["tree", "direct"])
+def test_mixed_two_halves_vs_fp64_direct(hip, ranks, method):
+    # FP64 state on both ranks; the LETs are float records. Forces against the FP64 direct sum over
+    # all bodies: the mixed form's tolerance.
+    x, v, m = L.plummer_f64(40000, 23)
+    first = x[:, 0] < 0
+    idx = [np.flatnonzero(first), np.flatnonzero(~first)]
+    p = P(method)
+    for k in range(2):
+        ranks[k].set_precision("mixed")
+        ranks[k].upload_f64(x[idx[k]], np.zeros((len(idx[k]), 3)), m[idx[k]], idx[k].astype(np.uint32))
+        ranks[k].build_tree(p)
+    try:
+        lets = [ranks[1].export_let(ranks[0].local_bounds(), p), ranks[0].export_let(ranks[1].local_bounds(), p)]
+        a = np.zeros((len(m), 3))
+        for k in range(2):
+            ranks[k].compute_forces_with(p, False, [lets[k]])
+            ak = ranks[k].accelerations_f64()
+            _, _, _, ids = ranks[k].download_f64()
+            a[ids] = np.stack(ak[:3], 1)
+        hip.set_precision("mixed")
+        hip.upload_f64(x, np.zeros_like(x), m)
+        sample = np.arange(0, len(m), 7)
+        ref, _ = hip.direct_f64_targets(sample, 1.0, 0.01)
+        _, _, _, hid = hip.download_f64()
+        e = rel_rms(a[hid[sample]], ref)
+        print(f"mixed two ranks, {method}: rel rms vs FP64 direct {e:.3g}")
+        assert e <= (1.5e-3 if method == "tree" else 1e-6)
+    finally:
+        hip.set_precision("plain")
+        for r in ranks:
+            r.set_precision("plain")
+
+
+def test_remote_lets_are_validated(hip):
+    b = zero_v(O.plummer(3000, 31))
+    p = P("tree")
+    hip.upload(b)
+    hip.build_tree(p)
+
+    def let_of(nodes, nb=0):
+        t = L.LetTree.empty()
+        t.nodes = np.array(nodes, L.LET_NODE)
+        t.bx = t.by = t.bz = np.zeros(nb, np.float32)
+        t.bm = np.full(nb, 1e-3, np.float32)
+        return t
+
+    def node(first=-1, cnt=0, bf=-1, bc=0, r2=3e38):
+        return (0, 0, 0, 1e-3, 0, 0, 0, 0, 0, 0, r2, first, cnt, bf, bc)
+
+    bad = [let_of([node(0, 1)]),                       # a child that is its parent
+           let_of([node(1, 2), node()]),               # children past the end
+           let_of([node(bf=0, bc=3)], nb=2),           # bodies past the end
+           let_of([node(bf=-1, bc=1)], nb=2)]          # bodies before the start
+    for t in bad:
+        with pytest.raises(ValueError):
+            hip.compute_forces_with(p, False, [t])
+    # 120 levels of 8 children (node 8k opens into 8k+1..8k+8, the last one continuing) would need
+    # more stack than the walk has: refused on the host, before any launch
+    nodes = [node() for _ in range(8 * 120 + 1)]
+    for k in range(120):
+        nodes[8 * k] = node(8 * k + 1, 8)
+    nodes[8 * 120] = node(bf=0, bc=1)
+    with pytest.raises(RuntimeError):
+        hip.compute_forces_with(p, False, [let_of(nodes, nb=1)])
+    # the context still works
+    a = np.zeros((len(b), 3))
+    hip.compute_forces_with(p, False, [])
+    read_accel(hip, a)
+    assert np.isfinite(a).all() and np.abs(a).sum() > 0
+
+
+def test_deep_binary_let_is_walked(hip):
+    # A remote tree of another shape than an octree (a chain of 60 binary splits, as a binary-tree
+    # exporter makes), every node always open: the receiver sums every body.
+    rng = np.random.default_rng(5)
+    depth = 60
+    nodes = []
+    bx, by, bz, bm = [], [], [], []
+    for k in range(depth):
+        # node 2k: internal with children 2k+1 (a leaf with one body) and 2k+2 (the next level)
+        nodes.append((0, 0, 0, 1.0, 0, 0, 0, 0, 0, 0, 3e38, 2 * k + 1, 2, -1, 0))
+        pos = rng.uniform(-1, 1, 3).astype(np.float32) + np.float32(5.0)
+        nodes.append((*pos, 1e-3, 0, 0, 0, 0, 0, 0, 3e38, -1, 0, len(bx), 1))
+        bx.append(pos[0]); by.append(pos[1]); bz.append(pos[2]); bm.append(1e-3)
+    nodes.append((0, 0, 0, 1e-3, 0, 0, 0, 0, 0, 0, 3e38, -1, 0, len(bx), 1))
+    bx.append(np.float32(5)); by.append(np.float32(5)); bz.append(np.float32(5)); bm.append(1e-3)
+    t = L.LetTree(np.array(nodes, L.LET_NODE), *[np.array(v, np.float32) for v in (bx, by, bz, bm)])
+    tgt = zero_v(O.plummer(500, 32))
+    tgt.m[:] = 0
+    p = P("tree")
+    hip.upload(tgt)
+    hip.build_tree(p)
+    hip.compute_forces_with(p, False, [])
+    local = hip.times()
+    hip.compute_forces_with(p, False, [t])
+    remote = hip.times()
+    a = np.zeros((len(tgt), 3))
+    read_accel(hip, a)
+    s = np.stack([bx, by, bz], 1).astype(np.float64)
+    x = np.stack([tgt.x, tgt.y, tgt.z], 1).astype(np.float64)
+    d = s[None] - x[:, None]
+    inv = 1 / np.sqrt((d ** 2).sum(-1) + 1e-4)
+    ref = np.zeros_like(a)
+    ref[tgt.id] = (d * (np.array(bm)[None, :, None] * inv[..., None] ** 3)).sum(1)
+    assert rel_rms(a, ref) < 1e-5
+    # every remote body once per target, on top of the local lists
+    assert remote["p2p"] - local["p2p"] == len(tgt) * len(bx) and remote["p2c"] == local["p2c"]

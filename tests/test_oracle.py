@@ -292,6 +292,193 @@ def test_disk_sech2_closed_form_moments():
     assert abs(np.mean(vphi[order] / vc) - 1) < 0.01
 
 
+# ---------------------------------------------------------------- locally essential trees
+
+LETZ = np.load(os.path.join(GOLD, "let.npz"))
+
+
+def zero_v(b: O.Bodies) -> O.Bodies:
+    b = b.copy()
+    b.vx[:] = 0
+    b.vy[:] = 0
+    b.vz[:] = 0
+    return b
+
+
+def let_halves():
+    """The LET fixture's inputs: a 4096-body Plummer sphere (seed 5) split at x = 0."""
+    b = zero_v(O.plummer(4096, 5))
+    return b, b.subset(b.x < 0), b.subset(b.x >= 0)
+
+
+def let_body_table(t) -> np.ndarray:
+    return np.stack([t.bx, t.by, t.bz, t.bm], 1)
+
+
+def oracle_accel(o: O.Oracle, rows_of_id, n):
+    """Accelerations (and costs) after a force call on bodies uploaded with v = 0, by upload row."""
+    costs = o.body_costs()
+    o.kick(1.0)
+    d = o.download()
+    a = np.zeros((n, 3), np.float32)
+    c = np.zeros(n, np.float32)
+    a[rows_of_id[d.id]] = np.stack([d.vx, d.vy, d.vz], 1)
+    c[rows_of_id[d.id]] = costs
+    return a, c
+
+
+@pytest.mark.parametrize("method,tag", [(1, "tree"), (0, "direct")])
+def test_let_export_and_remote_walk_match_reference_fixture(method, tag):
+    # Two ranks (src/backend.hpp:114-136): rank 1's LET for rank 0's box is the reference's record for
+    # record, and rank 0's forces with it have the reference's lists (costs, counts) and values.
+    _, h0, h1 = let_halves()
+    o0, o1 = O.Oracle(), O.Oracle()
+    o0.upload(h0)
+    o1.upload(h1)
+    o0.build_tree(0.6, method)
+    o1.build_tree(0.6, method)
+    box0 = o0.local_bounds()
+    assert np.array_equal(np.array(box0, np.float32), LETZ[f"{tag}__box0"])
+    let = o1.export_let(box0, method)
+    assert let.nodes.tobytes() == LETZ[f"{tag}__nodes"].tobytes()
+    assert np.array_equal(let_body_table(let), LETZ[f"{tag}__bodies"])
+    o0.compute_forces_with(1.0, 0.01, method, True, [let])
+    pe = o0.diagnostics()["pe"]
+    c = o0.counts()
+    rows = np.zeros(4096, np.int64)
+    rows[h0.id] = np.arange(len(h0))
+    a, costs = oracle_accel(o0, rows, len(h0))
+    assert np.array_equal(costs, LETZ[f"{tag}__costs"])
+    assert c["listed"] == int(LETZ[f"{tag}__interactions"])
+    assert rel_rms(a, LETZ[f"{tag}__acc"]) < 5e-6
+    assert abs(pe / float(LETZ[f"{tag}__pe"]) - 1) < 1e-6
+
+
+def test_let_far_and_point_boxes_match_reference_fixture():
+    b, h0, _ = let_halves()
+    o = O.Oracle()
+    o.upload(b)
+    o.build_tree(0.6, 1)
+    far = o.export_let(([24.8, -0.2, -0.2], [25.2, 0.2, 0.2]), 1)
+    assert far.nodes.tobytes() == LETZ["far__nodes"].tobytes() and len(far.bx) == 0
+    p = [float(h0.x[0]), float(h0.y[0]), float(h0.z[0])]
+    pt = o.export_let((p, p), 1)
+    assert pt.nodes.tobytes() == LETZ["point__nodes"].tobytes()
+    assert np.array_equal(let_body_table(pt), LETZ["point__bodies"])
+
+
+def test_let_box_holding_everything_exports_the_whole_tree():
+    # Every node's com lies in the box, so no node is terminal: the LET is the whole tree (every
+    # node has positive mass here) and every body, leaf by leaf in breadth-first order.
+    b = zero_v(O.plummer(3000, 8))
+    o = O.Oracle()
+    o.upload(b)
+    o.build_tree(0.6, 1)
+    lo, hi = o.local_bounds()
+    let = o.export_let((lo, hi), 1)
+    nodes = o.nodes()
+    assert len(let.nodes) == len(nodes["m"])
+    assert len(let.bx) == len(b)
+    leaf = let.nodes["body_count"] > 0
+    assert np.array_equal(leaf, let.nodes["first_child"] < 0)
+    assert let.nodes["body_count"].sum() == len(b)
+    assert np.array_equal(np.sort(let.bm), np.sort(b.m)) and np.isclose(let.nodes["mass"][0], 1.0, rtol=1e-6)
+    # children follow their parent, siblings contiguous and in order
+    for k in np.flatnonzero(~leaf):
+        f, c = let.nodes["first_child"][k], let.nodes["child_count"][k]
+        assert f > k and c >= 1 and np.isclose(let.nodes["mass"][f:f + c].sum(), let.nodes["mass"][k], rtol=1e-5)
+
+
+def test_let_theta0_two_ranks_equal_one_direct_sum():
+    # theta 0 opens every remote node: the two ranks' forces are the direct sum over all bodies, and
+    # the pair counts add up to n (n - 1).
+    b = zero_v(O.plummer(2000, 9))
+    h0, h1 = b.subset(b.x < 0.1), b.subset(b.x >= 0.1)
+    ad = O.Oracle().accel_by_id(b, 1.0, 0.01, 0.6, 0)
+    a = np.zeros((len(b), 3))
+    inter = 0
+    for method in (1, 0):
+        o = [O.Oracle(), O.Oracle()]
+        for k, h in enumerate((h0, h1)):
+            o[k].upload(h)
+            o[k].build_tree(0.0, method)
+        lets = [o[1].export_let(o[0].local_bounds(), method), o[0].export_let(o[1].local_bounds(), method)]
+        inter = 0
+        for k in range(2):
+            o[k].compute_forces_with(1.0, 0.01, method, False, [lets[k]])
+            inter += o[k].counts()["interactions"]
+            o[k].kick(1.0)
+            d = o[k].download()
+            a[d.id] = np.stack([d.vx, d.vy, d.vz], 1)
+        assert inter == len(b) * (len(b) - 1)
+        assert rel_rms(a, ad) < 1e-6
+
+
+def test_let_empty_and_zero_mass():
+    # An empty LET changes nothing; a root of zero mass exports nothing; zero-mass children are left out.
+    b = zero_v(O.plummer(3000, 10))
+    o = O.Oracle()
+    a0 = o.accel_by_id(b, 1.0, 0.01, 0.6, 1, True)
+    o.upload(b)
+    o.build_tree(0.6, 1)
+    empty = O.Let.sized(0, 0)
+    o.compute_forces_with(1.0, 0.01, 1, True, [empty, empty])
+    o.kick(1.0)
+    d = o.download()
+    a1 = np.zeros_like(a0)
+    a1[d.id] = np.stack([d.vx, d.vy, d.vz], 1)
+    assert np.array_equal(a0, a1)
+    z = b.copy()
+    z.m[:] = 0
+    o.upload(z)
+    o.build_tree(0.6, 1)
+    assert len(o.export_let(([-1, -1, -1], [1, 1, 1]), 1).nodes) == 0
+    h = b.copy()
+    h.m[h.x > 0] = 0  # every cell on the x > 0 side is massless
+    o.upload(h)
+    o.build_tree(0.6, 1)
+    let = o.export_let(([-100, -100, -100], [100, 100, 100]), 1)
+    assert (let.nodes["mass"] > 0).all()
+    leaves = np.flatnonzero(let.nodes["body_count"] > 0)
+    for k in leaves:  # every exported leaf holds mass; massless leaves stay home
+        f, c = let.nodes["body_first"][k], let.nodes["body_count"][k]
+        assert let.bm[f:f + c].sum() > 0
+    assert len(let.bx) < len(b) and (let.bm > 0).sum() == (h.m > 0).sum()
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("method", [1, 0])
+def test_oracle_let_matches_reference_build_live(method):
+    # Random halves (overlapping boxes) of a 20k sphere, both directions, with the potential.
+    b = zero_v(O.plummer(20000, 13))
+    pick = np.random.default_rng(99).integers(0, 2, len(b)).astype(bool)
+    halves = (b.subset(pick), b.subset(~pick))
+    o = [O.Oracle(), O.Oracle()]
+    r = [O.RefCpu(), O.RefCpu()]
+    for k in range(2):
+        o[k].upload(halves[k])
+        r[k].upload(halves[k])
+        o[k].build_tree(0.6, method)
+        r[k].build_tree(1.0, 0.01, 0.6, method)
+    for k in range(2):
+        box = r[k].local_bounds()
+        assert o[k].local_bounds() == box
+        lo_ = o[1 - k].export_let(box, method)
+        lr = r[1 - k].export_let(box, 1.0, 0.01, 0.6, method)
+        assert lo_.nodes.tobytes() == lr.nodes.tobytes()
+        assert np.array_equal(let_body_table(lo_), let_body_table(lr))
+        o[k].compute_forces_with(1.0, 0.01, method, True, [lo_])
+        r[k].compute_forces_with(1.0, 0.01, 0.6, method, True, [lr])
+        assert np.array_equal(o[k].body_costs(), r[k].body_costs())
+        assert o[k].counts()["listed"] == r[k].times()["interactions"]
+        assert abs(o[k].diagnostics()["pe"] / r[k].diagnostics()["pe"] - 1) < 1e-6
+        o[k].kick(1.0)
+        r[k].kick(1.0)
+        do, dr = o[k].download(), r[k].download()
+        assert np.array_equal(do.id, dr.id)
+        assert rel_rms(np.stack([do.vx, do.vy, do.vz], 1), np.stack([dr.vx, dr.vy, dr.vz], 1)) < 5e-6
+
+
 # ---------------------------------------------------------------- live reference (this container only)
 
 @pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) not present")
