@@ -265,6 +265,57 @@ def tree_adjoint_leg(be, nstars: int, steps: int):
                 "max_diff_over_largest": float(np.abs(gt - gd).max() / np.abs(gd).max())}}
 
 
+def let_leg(b, reps: int = 5):
+    """Two contexts on this GPU as two ranks of the reference's distributed tree step
+    (src/parallel.cpp:260-295): the workload's sphere split at x = 0. Each rank builds its tree,
+    exports its LET for the other's box and computes its forces with the other's LET. Medians over
+    `reps` (after one untimed round) of the export (host clock: device passes, two small read-backs
+    and the copy out) and of the walk with and without the remote tree (device time)."""
+    import lettuce_simulate_physics_amd as L
+
+    p = L.ForceParams(G=1.0, eps=EPS, theta=THETA, method="tree")
+    halves = [b.x < 0, b.x >= 0]
+    r = [L.HipBackend(0), L.HipBackend(0)]
+    for k in range(2):
+        r[k].upload(L.Bodies(*[np.ascontiguousarray(a[halves[k]]) for a in b.arrays()]))
+    rows = []
+    for rep in range(reps + 1):
+        row = {}
+        for k in range(2):
+            r[k].build_tree(p)
+        boxes = [r[k].local_bounds() for k in range(2)]
+        t0 = time.perf_counter()
+        lets = [r[1].export_let(boxes[0], p), r[0].export_let(boxes[1], p)]
+        row["export_ms"] = (time.perf_counter() - t0) * 1e3 / 2
+        local_ms, total_ms, inter_local, inter_remote = 0.0, 0.0, 0, 0
+        for k in range(2):
+            r[k].compute_forces_with(p, False, [])
+            t = r[k].times()
+            local_ms += t["walk_ms"]
+            inter_local += t["interactions"]
+            r[k].compute_forces_with(p, False, [lets[k]])
+            t = r[k].times()
+            total_ms += t["walk_ms"]
+            inter_remote += t["interactions"]
+        row.update(walk_local_ms=local_ms / 2, walk_with_let_ms=total_ms / 2,
+                   remote_inter=inter_remote - inter_local, nodes=[len(t_.nodes) for t_ in lets],
+                   bodies=[len(t_.bx) for t_ in lets])
+        if rep > 0:
+            rows.append(row)
+    med = lambda k: float(np.median([x[k] for x in rows]))  # noqa: E731
+    remote_ms = med("walk_with_let_ms") - med("walk_local_ms")
+    out = {"workload": "plummer_1M_two_ranks_x_split", "ranks_bodies": [int(h.sum()) for h in halves],
+           "let_nodes": rows[0]["nodes"], "let_bodies": rows[0]["bodies"],
+           "let_bytes": [60 * a + 16 * c for a, c in zip(rows[0]["nodes"], rows[0]["bodies"])],
+           "export_ms": round(med("export_ms"), 4), "walk_local_ms": round(med("walk_local_ms"), 4),
+           "walk_with_let_ms": round(med("walk_with_let_ms"), 4), "remote_pass_ms": round(remote_ms, 4),
+           "remote_interactions": int(rows[0]["remote_inter"]),
+           "remote_ginteractions_per_s": round(rows[0]["remote_inter"] / 2 / max(remote_ms, 1e-9) / 1e6, 2)}
+    for be in r:
+        be.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -282,6 +333,7 @@ def main():
     ap.add_argument("--disk-steps", type=int, default=50)
     ap.add_argument("--encounter-stars", type=int, default=524288, help="stars per disk of the tree adjoint leg")
     ap.add_argument("--encounter-steps", type=int, default=100, help="0 skips the tree adjoint leg")
+    ap.add_argument("--no-let", action="store_true", help="skip the two-rank LET leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy 3 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -355,6 +407,8 @@ def main():
         out["mixed"] = mixed_leg(be, b, k, args.warmup, wopt, force_ms)
         if args.disk_steps > 0:
             out["disk"] = disk_leg(be, args.disk_stars, args.disk_steps, wopt)
+    if not args.no_let:
+        out["let"] = let_leg(b)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(be)
         out["adjoint_forward_ratio"] = out["adjoint"]["adjoint_forward_ratio"]

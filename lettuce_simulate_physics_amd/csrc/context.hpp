@@ -778,7 +778,7 @@ struct Context {
                 WalkArgs a = walk_args(p);
                 if (mixed) a.posd_box = a.posd;
                 remote_args(a, trees, true);
-                tree_walk_remote(stream, a, potential);
+                tree_walk(stream, a, potential);
                 LHIP_CHECK(hipMemcpyAsync(&hc, counters.as<void>(), sizeof hc, hipMemcpyDeviceToHost, stream));
             }
             LHIP_CHECK(hipEventRecord(ev[4], stream));
@@ -790,11 +790,8 @@ struct Context {
             direct_costs_pending = trees == 0;
         } else {
             WalkArgs a = walk_args(p);
+            if (trees > 0) remote_args(a, trees, false);  // one pass: local, then remote trees
             tree_walk(stream, a, potential);
-            if (trees > 0) {
-                remote_args(a, trees, false);
-                tree_walk_remote(stream, a, potential);
-            }
             LHIP_CHECK(hipEventRecord(ev[4], stream));
             LHIP_CHECK(hipMemcpyAsync(&hc, counters.as<void>(), sizeof hc, hipMemcpyDeviceToHost, stream));
             sync();

@@ -41,6 +41,14 @@
 // group sits. Each slot's FP32 partial sums are folded into FP64 after every list evaluation (at
 // most 64 cells or 128 bodies per share), and the shares are added in FP64.
 //
+// Remote trees (LETs from other ranks, src/backend.hpp:78-83; src/cpu_backend.cpp:667-725, 788-814).
+// Each group continues its lists with every remote tree, walked with the same stack and staging: a
+// terminal node (no children, no bodies) or one whose com is at rcrit2 or beyond is accepted (its
+// bodies if it has 1 to small_leaf of them, else its multipole); an opened node contributes its
+// bodies if it has any, else its children. Remote bodies carry index -1, so the self test never
+// drops one. The direct method's remote pass runs the same kernel over blocks of kGroupMax
+// consecutive bodies with the remote trees only and adds to the direct sum.
+//
 // Scheduling. A persistent grid; each wave takes groups from an atomic counter until none are left.
 #include <cfloat>
 
@@ -152,9 +160,10 @@ constexpr int kStackCap = kWalkStack;
 // = capacity - 7 (depth + 1) - 8; at or above thr one node per step adds at most 7 per tree level.
 __device__ __forceinline__ int stack_threshold(int depth) { return kStackCap - 7 * (max(depth, kMaxLevel) + 1) - 8; }
 
-// LET: the remote trees' pass (a.let): the same groups (or, with direct_blocks, blocks of
-// kGroupMax consecutive bodies) walk every remote tree instead of the local one, and the forces,
-// costs and counts are added to what is there.
+// LET: every group also walks every remote tree (a.let) after the local one, into the same lists
+// (the reference's order: src/cpu_backend.cpp:667-689). With direct_blocks the groups are blocks of
+// kGroupMax consecutive bodies that walk the remote trees only, and the forces, costs and counts are
+// added to the direct sum's.
 template <bool POT, bool GUARD, bool REL, bool LET>
 __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
     constexpr bool MASK = POT || GUARD;
@@ -170,7 +179,8 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
     const bool blocks = LET && a.direct_blocks;
     const int ngroups = blocks ? (a.n + kGroupMax - 1) / kGroupMax : int(a.gcount[a.n]);
     const float e2 = a.eps2;
-    const int stack_thr = LET ? a.let_thr : stack_threshold(a.depth);
+    const int stack_thr = stack_threshold(a.depth);
+    const int let_thr = LET ? a.let_thr : 0;
     unsigned long long t_inter = 0, t_p2p = 0, t_p2c = 0, t_listed = 0, t_p2m = 0;
     bool overflow = false;
 
@@ -343,7 +353,7 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                 }
             };
 
-            if constexpr (!LET) {
+            if (!blocks) {
                 if (lane == 0) st[0] = 0;  // the root
                 int top = 1;
                 while (top > 0) {
@@ -438,7 +448,8 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                         },
                         [](int j) { return j; });
                 }
-            } else {
+            }
+            if constexpr (LET) {
                 // Remote trees (src/backend.hpp:78-83): a terminal node (no children, no bodies) or
                 // one whose com is at rcrit2 or beyond is accepted: its bodies if it has 1 to
                 // small_leaf of them, else its multipole; an opened node contributes its bodies if it
@@ -447,7 +458,7 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                     if (lane == 0) st[0] = a.let_root[t];
                     int top = 1;
                     while (top > 0) {
-                        const int room = (stack_thr - top) / a.let_fan;
+                        const int room = (let_thr - top) / a.let_fan;
                         const int P = min(64, min(top, max(1, room)));
                         __builtin_amdgcn_wave_barrier();
                         const bool act = lane < P;
@@ -502,12 +513,11 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
 
             // Add the list shares of each body (slot v = share * nbody + body) through LDS.
             const uint32_t listed = ncell_total + nbody_total;  // nbody_total includes the monopoles
-            // Per-body cost, self excluded: the local pass's list less the body's own entry; the
-            // remote pass adds its entries to the local cost, or to n - 1 after the direct sum.
-            auto cost_of = [&](int i, uint32_t len) {
-                if (!LET) return float(len - 1u);
+            // Per-body cost, self excluded: the list less the body's own entry; remote blocks after
+            // the direct sum add their list to n - 1.
+            auto cost_of = [&](int, uint32_t len) {
                 if (blocks) return float((unsigned long long)(a.n - 1) + len);
-                return float((unsigned long long)a.cost[i] + len);
+                return float(len - 1u);
             };
             if (REL) {
                 D4* red = reinterpret_cast<D4*>(&L);  // the lists are done: their LDS holds 64 NX D4
@@ -527,7 +537,7 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                         acc4.x += q.x; acc4.y += q.y; acc4.z += q.z; acc4.w += q.w;
                     }
                     const int i = clo + v;
-                    if (LET) {
+                    if (blocks) {
                         a.axd[i] += G * acc4.x;
                         a.ayd[i] += G * acc4.y;
                         a.azd[i] += G * acc4.z;
@@ -559,7 +569,7 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                         acc4.x += q.x; acc4.y += q.y; acc4.z += q.z; acc4.w += q.w;
                     }
                     const int i = clo + v;
-                    if (LET) {
+                    if (blocks) {
                         a.ax[i] += G * acc4.x;
                         a.ay[i] += G * acc4.y;
                         a.az[i] += G * acc4.z;
@@ -577,7 +587,7 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
             if (lane == 0) {
                 // Each body of the chunk meets itself once in the local body list (never in a
                 // remote one).
-                const unsigned long long self = LET ? 0ull : (unsigned long long)nbody;
+                const unsigned long long self = blocks ? 0ull : (unsigned long long)nbody;
                 t_listed += (unsigned long long)listed * nbody;
                 t_p2c += (unsigned long long)(ncell_total + nmono_total) * nbody;
                 t_p2m += (unsigned long long)nmono_total * nbody;
@@ -607,7 +617,6 @@ __global__ void reset_walk(WalkCounters* __restrict__ c) {
     c->next_group = 0;
 }
 
-__global__ void reset_next_group(WalkCounters* __restrict__ c) { c->next_group = 0; }
 
 }  // namespace
 
@@ -640,20 +649,11 @@ void tree_walk(hipStream_t s, const WalkArgs& a, bool potential) {
     const int blocks = std::max(1, a.blocks);
     reset_walk<<<1, 1, 0, s>>>(a.counters);
     LHIP_LAUNCH_CHECK();
-    if (a.posd) walk_launch<true, false>(s, a, potential, guard, blocks);
-    else walk_launch<false, false>(s, a, potential, guard, blocks);
-    LHIP_LAUNCH_CHECK();
-}
-
-void tree_walk_remote(hipStream_t s, const WalkArgs& a, bool potential) {
-    if (a.n <= 0 || a.let_trees <= 0) return;
-    const bool guard = !(a.eps2 > 0.f);
-    const int blocks = std::max(1, a.blocks);
-    if (a.direct_blocks) reset_walk<<<1, 1, 0, s>>>(a.counters);
-    else reset_next_group<<<1, 1, 0, s>>>(a.counters);
-    LHIP_LAUNCH_CHECK();
-    if (a.posd) walk_launch<true, true>(s, a, potential, guard, blocks);
-    else walk_launch<false, true>(s, a, potential, guard, blocks);
+    const bool let = a.let_trees > 0;
+    if (a.posd) let ? walk_launch<true, true>(s, a, potential, guard, blocks)
+                    : walk_launch<true, false>(s, a, potential, guard, blocks);
+    else let ? walk_launch<false, true>(s, a, potential, guard, blocks)
+             : walk_launch<false, false>(s, a, potential, guard, blocks);
     LHIP_LAUNCH_CHECK();
 }
 

@@ -186,10 +186,11 @@ struct WalkArgs {
     int small_leaf;            // accepted leaves with at most this many bodies are summed directly
     float quad_tau;            // error-budgeted far field threshold (0: every quadrupole kept)
     int depth;                 // deepest tree level (21 x key frames)
-    // Remote trees (tree_walk_remote): node records, bodies (x, y, z, m) and the root of each
-    // non-empty tree; the stack threshold for their height and fan-out. direct_blocks: the groups
-    // are blocks of kGroupMax consecutive bodies in the current order (the direct method's remote
-    // pass) instead of gstart.
+    // Remote trees (let_trees > 0): node records, bodies (x, y, z, m) and the root of each
+    // non-empty tree; the stack threshold for their height and fan-out. Every group walks them
+    // after the local tree, into the same lists. direct_blocks: the groups are blocks of kGroupMax
+    // consecutive bodies in the current order that walk the remote trees only and add to the
+    // direct sum's forces (the direct method's remote pass).
     const LetRec* let;
     const float4* let_body;
     const int* let_root;
@@ -201,10 +202,6 @@ struct WalkArgs {
 void tree_walk(hipStream_t s, const WalkArgs& a, bool potential);
 // Entries of a walk wave's traversal stack (LDS).
 constexpr int kWalkStack = 896;
-// The remote trees' pass: every group walks every remote tree with the receiver rules of the LET
-// format and adds the forces, costs and counts to what the local pass (or the direct sum) left.
-// After tree_walk (same counters; only its group counter is reset) or, with direct_blocks, alone.
-void tree_walk_remote(hipStream_t s, const WalkArgs& a, bool potential);
 int walk_blocks_per_cu();
 
 // Vector-Jacobian product of the mixed-precision walk's forces with frozen topology
