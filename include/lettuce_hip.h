@@ -2,7 +2,7 @@
  *
  * Every entry point mirrors one member of the reference's Backend interface (src/backend.hpp) or
  * its factory functions, with plain pointers and sizes. C++ callers use the HipBackend class
- * (lettuce_simulate_physics_amd/csrc/hip_backend.hpp), which implements Backend on top of this ABI
+ * (lettuce_simulate_physics_amd/host/hip_backend.cpp), which implements Backend on top of this ABI
  * and rethrows errors as the reference's exception types.
  *
  * Status codes: LHIP_OK on success; LHIP_RUNTIME for device/runtime failures (reference:
@@ -106,7 +106,7 @@ int lhip_diagnostics(lhip_ctx* ctx, lhip_diag* out);
 int lhip_exact_potential_energy(lhip_ctx* ctx, double G, double eps, double* out);
 int lhip_times(lhip_ctx* ctx, lhip_phase_times* out);
 
-/* ---- distributed hooks used on one rank (src/backend.hpp:114-140) ---- */
+/* ---- distributed hooks (src/backend.hpp:114-140) ---- */
 
 /* Backend::local_bounds: float bounding box of the current positions. */
 int lhip_local_bounds(lhip_ctx* ctx, float lo[3], float hi[3]);
@@ -199,6 +199,50 @@ int lhip_download_f64(lhip_ctx* ctx, double* x, double* y, double* z, double* vx
 /* FP64 accelerations and potential of the last force call (mixed mode only). Any pointer may be
  * NULL. */
 int lhip_accelerations_f64(lhip_ctx* ctx, double* ax, double* ay, double* az, double* pot);
+
+/* ---- snapshots (host code; the reference's LTCSNAP3 files, src/snapshot.hpp:1-27) ----
+ *
+ * The reference's layout: a 256-byte little-endian header, then planar arrays on 4096-byte
+ * boundaries (f32 x, y, z, vx, vy, vz, m; u32 id). With FP64 state (the mixed-precision form) the
+ * file also carries f64 x, y, z, vx, vy, vz arrays after those, marked in spare header bytes (no
+ * reference counterpart: its snapshots are f32 only). The f32 arrays then hold the FP64 values
+ * rounded to nearest, so the reference reads such a file as an ordinary snapshot, and these
+ * readers restore the FP64 state bit for bit. Writes go to <path>.tmp, are fsync'ed and renamed
+ * into place. Failures return LHIP_RUNTIME (the reference: std::runtime_error naming the path),
+ * with the message in lhip_snapshot_error(); LHIP_INVALID for bad arguments. */
+
+/* SnapshotHeader (src/snapshot.hpp:9-20). scenario: at most 31 bytes are stored. fp64 (read only):
+ * 1 when the file holds the FP64 state arrays. */
+typedef struct lhip_snapshot_header {
+    uint64_t n, step;
+    double time, G, eps, dt;
+    uint64_t seed;
+    char scenario[32];
+    uint32_t pieces;
+    uint32_t fp64;
+} lhip_snapshot_header;
+
+/* Message of the last failed snapshot call on this thread. */
+const char* lhip_snapshot_error(void);
+/* read_snapshot_header (src/snapshot.hpp:26): the header, validated against the file size. */
+int lhip_snapshot_read_header(const char* path, lhip_snapshot_header* h);
+/* write_snapshot (src/snapshot.hpp:23) of n bodies from host arrays; n is the arrays' length
+ * (h->n is not used). xd..vzd: the FP64 state, or all NULL for an f32-only file. */
+int lhip_snapshot_save(const char* path, const lhip_snapshot_header* h, uint64_t n, const float* x, const float* y,
+                       const float* z, const float* vx, const float* vy, const float* vz, const float* m,
+                       const uint32_t* id, const double* xd, const double* yd, const double* zd, const double* vxd,
+                       const double* vyd, const double* vzd);
+/* read_snapshot (src/snapshot.hpp:24) into host arrays of `capacity` entries (LHIP_INVALID when
+ * the file holds more bodies). xd..vzd are filled when the file has FP64 arrays (NULL: skipped). */
+int lhip_snapshot_load(const char* path, lhip_snapshot_header* h, uint64_t capacity, float* x, float* y, float* z,
+                       float* vx, float* vy, float* vz, float* m, uint32_t* id, double* xd, double* yd, double* zd,
+                       double* vxd, double* vyd, double* vzd);
+/* The context's bodies (download order) to a snapshot; in the mixed-precision mode with the FP64
+ * state. h->n is not used. */
+int lhip_snapshot_write(lhip_ctx* ctx, const char* path, const lhip_snapshot_header* h);
+/* A snapshot into the context (as upload): the FP64 state when the file has it and the context is
+ * in the mixed-precision mode, else the f32 arrays (promoted in the mixed mode). *h: its header. */
+int lhip_snapshot_restore(lhip_ctx* ctx, const char* path, lhip_snapshot_header* h);
 
 /* ---- initial conditions (host code) ----
  *

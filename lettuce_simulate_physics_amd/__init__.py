@@ -74,6 +74,7 @@ def library():
         L.lhip_last_error.argtypes = [_P]
         L.lhip_size.restype = _U64
         L.lhip_size.argtypes = [_P]
+        L.lhip_snapshot_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
 
@@ -257,6 +258,75 @@ class LetC(ctypes.Structure):
                 ("n_bodies", ctypes.c_uint64)]
 
 
+class SnapshotHeaderC(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint64), ("step", ctypes.c_uint64), ("time", ctypes.c_double), ("G", ctypes.c_double),
+                ("eps", ctypes.c_double), ("dt", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("scenario", ctypes.c_char * 32), ("pieces", ctypes.c_uint32), ("fp64", ctypes.c_uint32)]
+
+
+SNAPSHOT_FIELDS = ("step", "time", "G", "eps", "dt", "seed", "scenario", "pieces")
+
+
+def _snap_header(h: dict | None) -> SnapshotHeaderC:
+    h = dict(h or {})
+    unknown = set(h) - set(SNAPSHOT_FIELDS) - {"n", "fp64"}
+    if unknown:
+        raise ValueError(f"unknown snapshot header fields: {sorted(unknown)}")
+    c = SnapshotHeaderC()
+    c.step, c.time, c.G = int(h.get("step", 0)), float(h.get("time", 0.0)), float(h.get("G", 1.0))
+    c.eps, c.dt, c.seed = float(h.get("eps", 0.0)), float(h.get("dt", 0.0)), int(h.get("seed", 0))
+    c.scenario = str(h.get("scenario", "")).encode()[:31]
+    c.pieces = int(h.get("pieces", 0))
+    return c
+
+
+def _snap_dict(c: SnapshotHeaderC) -> dict:
+    return dict(n=c.n, step=c.step, time=c.time, G=c.G, eps=c.eps, dt=c.dt, seed=c.seed,
+                scenario=c.scenario.decode(), pieces=c.pieces, fp64=bool(c.fp64))
+
+
+def _snap_check(rc):
+    if rc == OK:
+        return
+    msg = library().lhip_snapshot_error().decode()
+    if rc == INVALID:
+        raise ValueError(msg or "invalid snapshot argument")
+    raise RuntimeError(msg or f"snapshot error {rc}")
+
+
+def snapshot_header(path: str) -> dict:
+    """read_snapshot_header (src/snapshot.hpp:26): the header, validated against the file size."""
+    c = SnapshotHeaderC()
+    _snap_check(library().lhip_snapshot_read_header(os.fsencode(path), ctypes.byref(c)))
+    return _snap_dict(c)
+
+
+def snapshot_save(path: str, header: dict | None, b, f64=None) -> None:
+    """write_snapshot (src/snapshot.hpp:23): bodies b; f64 = (x (n x 3), v (n x 3)) adds the FP64
+    state (the f32 arrays should then be its rounding, as download() gives)."""
+    b = Bodies.of(b)
+    cols = [None] * 6
+    if f64 is not None:
+        x, v = (np.asarray(a, np.float64).reshape(len(b), 3) for a in f64)
+        cols = [np.ascontiguousarray(a[:, c]) for a in (x, v) for c in range(3)]
+    c = _snap_header(header)
+    _snap_check(library().lhip_snapshot_save(os.fsencode(path), ctypes.byref(c), _U64(len(b)),
+                                             *[_ptr(a) for a in b.arrays()], *[_ptr(a) for a in cols]))
+
+
+def snapshot_load(path: str):
+    """read_snapshot (src/snapshot.hpp:24): (header, Bodies, (x, v) in FP64 or None)."""
+    n = snapshot_header(path)["n"]
+    b = Bodies.empty(n)
+    cols = [np.zeros(n) for _ in range(6)]
+    c = SnapshotHeaderC()
+    _snap_check(library().lhip_snapshot_load(os.fsencode(path), ctypes.byref(c), _U64(n),
+                                             *[_ptr(a) for a in b.arrays()], *[_ptr(a) for a in cols]))
+    h = _snap_dict(c)
+    f64 = (np.stack(cols[:3], 1), np.stack(cols[3:], 1)) if h["fp64"] else None
+    return h, b, f64
+
+
 def hip_compiled() -> bool:
     try:
         return bool(library().lhip_compiled())
@@ -417,6 +487,18 @@ class HipBackend:
             return None
         self._check(rc)
         return c
+
+    # ---- snapshots (src/snapshot.hpp; FP64 arrays in the mixed-precision mode) ----
+    def write_snapshot(self, path: str, **header) -> None:
+        """The bodies (download order) to an LTCSNAP3 file, with the FP64 state in the mixed mode."""
+        c = _snap_header(header)
+        self._check(self._L.lhip_snapshot_write(self._h, os.fsencode(path), ctypes.byref(c)))
+
+    def restore_snapshot(self, path: str) -> dict:
+        """A snapshot into this backend (as upload); the FP64 state when present and in the mixed mode."""
+        c = SnapshotHeaderC()
+        self._check(self._L.lhip_snapshot_restore(self._h, os.fsencode(path), ctypes.byref(c)))
+        return _snap_dict(c)
 
     # ---- mixed-precision form (no reference counterpart) ----
     def set_precision(self, mode: str = "plain", group_max=None, small_leaf=None, quad_tau=None) -> None:

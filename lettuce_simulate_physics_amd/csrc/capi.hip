@@ -147,6 +147,16 @@ int lhip_export_let_copy(lhip_ctx* ctx, lhip_let_node* nodes, float* bx, float* 
     return guarded(ctx, [&](lhip::Context& c) { c.export_let_copy(nodes, bx, by, bz, bm); });
 }
 
+int lhip_snapshot_write(lhip_ctx* ctx, const char* path, const lhip_snapshot_header* h) {
+    if (!path || !h) return LHIP_INVALID;
+    return guarded(ctx, [&](lhip::Context& c) { c.snapshot_write(path, *h); });
+}
+
+int lhip_snapshot_restore(lhip_ctx* ctx, const char* path, lhip_snapshot_header* h) {
+    if (!path || !h) return LHIP_INVALID;
+    return guarded(ctx, [&](lhip::Context& c) { c.snapshot_restore(path, h); });
+}
+
 int lhip_memory_need(uint64_t n, int32_t method, uint64_t* device_bytes, uint64_t* host_bytes) {
     if (!device_bytes || !host_bytes) return LHIP_INVALID;
     lhip::memory_need(n, method, device_bytes, host_bytes);

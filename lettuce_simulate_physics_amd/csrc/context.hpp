@@ -985,6 +985,52 @@ struct Context {
         sync();
     }
 
+    // ---- snapshots (src/simulation.cpp:311-325 writes download() order; restart uploads it)
+    void snapshot_write(const char* path, const lhip_snapshot_header& h) {
+        const std::size_t k = std::size_t(n);
+        std::vector<float> f[7];
+        for (auto& v : f) v.resize(k);
+        std::vector<uint32_t> ids(k);
+        download(f[0].data(), f[1].data(), f[2].data(), f[3].data(), f[4].data(), f[5].data(), f[6].data(), ids.data());
+        std::vector<double> d[6];
+        if (mixed) {
+            for (auto& v : d) v.resize(k);
+            download_f64(d[0].data(), d[1].data(), d[2].data(), d[3].data(), d[4].data(), d[5].data(), nullptr,
+                         nullptr);
+        }
+        const int rc = lhip_snapshot_save(path, &h, k, f[0].data(), f[1].data(), f[2].data(), f[3].data(), f[4].data(),
+                                          f[5].data(), f[6].data(), ids.data(), mixed ? d[0].data() : nullptr,
+                                          d[1].data(), d[2].data(), d[3].data(), d[4].data(), d[5].data());
+        if (rc != LHIP_OK) throw Error(rc, lhip_snapshot_error());
+    }
+
+    void snapshot_restore(const char* path, lhip_snapshot_header* out) {
+        lhip_snapshot_header h{};
+        int rc = lhip_snapshot_read_header(path, &h);
+        if (rc != LHIP_OK) throw Error(rc, lhip_snapshot_error());
+        if (h.n >= uint64_t(INT32_MAX))
+            throw Error(kInvalid, "hip backend: n = " + std::to_string(h.n) + " exceeds the 2^31-1 body limit");
+        const std::size_t k = std::size_t(h.n);
+        std::vector<float> f[7];
+        for (auto& v : f) v.resize(k);
+        std::vector<uint32_t> ids(k);
+        const bool want64 = mixed && h.fp64;
+        std::vector<double> d[6];
+        if (want64)
+            for (auto& v : d) v.resize(k);
+        auto p64 = [&](int c) { return want64 ? d[c].data() : nullptr; };
+        rc = lhip_snapshot_load(path, &h, k, f[0].data(), f[1].data(), f[2].data(), f[3].data(), f[4].data(),
+                                f[5].data(), f[6].data(), ids.data(), p64(0), p64(1), p64(2), p64(3), p64(4), p64(5));
+        if (rc != LHIP_OK) throw Error(rc, lhip_snapshot_error());
+        if (want64)
+            upload_f64(h.n, d[0].data(), d[1].data(), d[2].data(), d[3].data(), d[4].data(), d[5].data(), f[6].data(),
+                       ids.data());
+        else
+            upload(h.n, f[0].data(), f[1].data(), f[2].data(), f[3].data(), f[4].data(), f[5].data(), f[6].data(),
+                   ids.data());
+        *out = h;
+    }
+
     bool body_costs(float* out) {
         LHIP_CHECK(hipSetDevice(dev));
         if (!costs_ok || n == 0) return false;

@@ -118,3 +118,8 @@ struct HipOptions {
 std::unique_ptr<Backend> make_hip_backend(int device, std::string& why, const HipOptions& opt);
 bool hip_compiled();
 int hip_device_count();
+// The C ABI context behind a backend that make_hip_backend made (nullptr for any other backend), for
+// the calls that have no Backend member: the FP64 state and its snapshots (lhip_upload_f64,
+// lhip_snapshot_write, lhip_snapshot_restore).
+struct lhip_ctx;
+lhip_ctx* hip_context(Backend& be);

@@ -31,6 +31,7 @@ lhip_force_params to_c(const ForceParams& p) {
 class HipBackend final : public Backend {
 public:
     explicit HipBackend(lhip_ctx* c) : c_(c) {}
+    lhip_ctx* ctx() const { return c_; }
     ~HipBackend() override { lhip_destroy(c_); }
     HipBackend(const HipBackend&) = delete;
     HipBackend& operator=(const HipBackend&) = delete;
@@ -188,6 +189,11 @@ std::unique_ptr<Backend> make_hip_backend(int device, std::string& why, const Hi
         return nullptr;
     }
     return std::make_unique<HipBackend>(c);
+}
+
+lhip_ctx* hip_context(Backend& be) {
+    auto* h = dynamic_cast<HipBackend*>(&be);
+    return h ? h->ctx() : nullptr;
 }
 
 bool hip_compiled() { return lhip_compiled() != 0; }

@@ -201,6 +201,33 @@ def _flat(remote):
     return arr, keep
 
 
+def ref_write_snapshot(path: str, header: dict, b: Bodies) -> None:
+    """The reference's write_snapshot (src/snapshot.hpp:23), from oracle/_ref."""
+    R = ref()
+    hd = np.zeros(8, np.float64)
+    u = hd.view(np.uint64)
+    u[1], u[6], u[7] = header.get("step", 0), header.get("seed", 0), header.get("pieces", 0)
+    hd[2], hd[3], hd[4], hd[5] = header.get("time", 0.0), header.get("G", 1.0), header.get("eps", 0.0), header.get("dt", 0.0)
+    if R.ref_write_snapshot(os.fsencode(path), _ptr(hd), str(header.get("scenario", "")).encode(), _U64(len(b)),
+                            *[_ptr(v) for v in b.arrays()]):
+        raise RuntimeError(R.ref_last_error().decode())
+
+
+def ref_read_snapshot(path: str):
+    """The reference's read_snapshot (src/snapshot.hpp:24): (header, Bodies)."""
+    R = ref()
+    hd = np.zeros(8, np.float64)
+    sc = ctypes.create_string_buffer(32)
+    if R.ref_read_snapshot(os.fsencode(path), _ptr(hd), sc):
+        raise RuntimeError(R.ref_last_error().decode())
+    u = hd.view(np.uint64)
+    h = dict(n=int(u[0]), step=int(u[1]), time=float(hd[2]), G=float(hd[3]), eps=float(hd[4]), dt=float(hd[5]),
+             seed=int(u[6]), pieces=int(u[7]), scenario=sc.value.decode())
+    b = Bodies.empty(h["n"])
+    R.ref_snapshot_copy(*[_ptr(v) for v in b.arrays()])
+    return h, b
+
+
 class _Common:
     """Accelerations are read as the reference tests do: upload with v = 0, compute, kick(1), read v
     (tests/test_cpu_backend.cpp:45-60)."""

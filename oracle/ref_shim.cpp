@@ -5,7 +5,9 @@
 //
 // Interfaces used (reference files): Backend, ForceParams, Diag, PhaseTimes, LetTree
 // (src/backend.hpp:12-148), make_cpu_backend (src/cpu_backend.cpp:911), Config and
-// build_scenario (src/config.hpp:14-70, src/scenarios.cpp:411-420).
+// build_scenario (src/config.hpp:14-70, src/scenarios.cpp:411-420), write_snapshot and
+// read_snapshot (src/snapshot.hpp:22-27).
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <exception>
@@ -16,6 +18,7 @@
 #include "backend.hpp"
 #include "config.hpp"
 #include "scenarios.hpp"
+#include "snapshot.hpp"
 
 namespace {
 thread_local std::string g_err;
@@ -231,6 +234,57 @@ int ref_forces_with(void* h, double G, double eps, double theta, int method, int
         g_err = e.what();
         return 1;
     }
+}
+
+// ---- snapshots (src/snapshot.hpp:22-27). hdr: n, step, seed, pieces as u64 and time, G, eps, dt as
+// f64 (8 slots of 8 bytes: n, step, time, G, eps, dt, seed, pieces); scenario: 32 bytes.
+static thread_local Bodies g_snap;
+
+int ref_write_snapshot(const char* path, const double* hd, const char* scenario, uint64_t n, const float* x,
+                       const float* y, const float* z, const float* vx, const float* vy, const float* vz,
+                       const float* m, const uint32_t* id) {
+    try {
+        SnapshotHeader h;
+        std::memcpy(&h.step, hd + 1, 8);
+        h.time = hd[2]; h.G = hd[3]; h.eps = hd[4]; h.dt = hd[5];
+        std::memcpy(&h.seed, hd + 6, 8);
+        uint64_t pieces;
+        std::memcpy(&pieces, hd + 7, 8);
+        h.pieces = uint32_t(pieces);
+        h.scenario = scenario;
+        Bodies b;
+        b.x.assign(x, x + n); b.y.assign(y, y + n); b.z.assign(z, z + n);
+        b.vx.assign(vx, vx + n); b.vy.assign(vy, vy + n); b.vz.assign(vz, vz + n);
+        b.m.assign(m, m + n); b.id.assign(id, id + n);
+        write_snapshot(path, h, b);
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+// Reads the file (keeping the bodies for ref_snapshot_copy); fills hd and scenario as above.
+int ref_read_snapshot(const char* path, double* hd, char* scenario) {
+    try {
+        SnapshotHeader h = read_snapshot(path, g_snap);
+        std::memcpy(hd, &h.n, 8);
+        std::memcpy(hd + 1, &h.step, 8);
+        hd[2] = h.time; hd[3] = h.G; hd[4] = h.eps; hd[5] = h.dt;
+        std::memcpy(hd + 6, &h.seed, 8);
+        const uint64_t pieces = h.pieces;
+        std::memcpy(hd + 7, &pieces, 8);
+        std::memset(scenario, 0, 32);
+        std::memcpy(scenario, h.scenario.data(), std::min<std::size_t>(h.scenario.size(), 31));
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+void ref_snapshot_copy(float* x, float* y, float* z, float* vx, float* vy, float* vz, float* m, uint32_t* id) {
+    copy_out(g_snap, x, y, z, vx, vy, vz, m, id);
 }
 
 }  // extern "C"

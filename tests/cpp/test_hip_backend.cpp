@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "backend.hpp"
+#include "lettuce_hip.h"
 
 extern "C" {
 void orc_plummer(uint64_t n, uint64_t seed, double G, double M, double a, double cut, float* x, float* y, float* z,
@@ -286,6 +287,44 @@ TEST("hip let: empty LETs change nothing; a rank without bodies exports none") {
         threw = true;
     }
     CHECK(threw);
+}
+
+TEST("hip snapshot: a mixed-precision run restarts bit for bit from its FP64 snapshot") {
+    std::string why;
+    HipOptions opt;
+    opt.mixed = true;
+    auto a = make_hip_backend(0, why, opt);
+    auto b = make_hip_backend(0, why, opt);
+    if (!a || !b) throw Skip{why};
+    CHECK(hip_context(*a) != nullptr);
+    const ForceParams p = params(ForceMethod::tree, 0.01);
+    a->upload(plummer(8192, 21, 1.0, 10.0));
+    a->compute_forces(p, false);
+    auto steps = [&](Backend& be, int k) {
+        for (int s = 0; s < k; ++s) {
+            be.kick(5e-4); be.drift(1e-3); be.compute_forces(p, false); be.kick(5e-4);
+        }
+    };
+    steps(*a, 4);
+    lhip_snapshot_header h{};
+    h.step = 4;
+    const char* tmp = std::getenv("TMPDIR");
+    const std::string path = std::string(tmp && *tmp ? tmp : "/tmp") + "/test_hip_snapshot_fp64.snap";
+    CHECK(lhip_snapshot_write(hip_context(*a), path.c_str(), &h) == LHIP_OK);
+    steps(*a, 4);
+    lhip_snapshot_header g{};
+    CHECK(lhip_snapshot_restore(hip_context(*b), path.c_str(), &g) == LHIP_OK && g.fp64 == 1 && g.step == 4);
+    b->compute_forces(p, false);
+    steps(*b, 4);
+    const uint64_t n = lhip_size(hip_context(*a));
+    std::vector<double> xa(n), xb(n), va(n), vb(n), d(n);
+    std::vector<uint32_t> ia(n), ib(n);
+    lhip_download_f64(hip_context(*a), xa.data(), d.data(), d.data(), va.data(), d.data(), d.data(), nullptr, ia.data());
+    lhip_download_f64(hip_context(*b), xb.data(), d.data(), d.data(), vb.data(), d.data(), d.data(), nullptr, ib.data());
+    bool same = ia == ib;
+    for (uint64_t i = 0; same && i < n; ++i) same = xa[i] == xb[i] && va[i] == vb[i];
+    CHECK(same);
+    std::remove(path.c_str());
 }
 
 int main(int argc, char** argv) {
