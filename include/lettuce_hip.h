@@ -157,9 +157,13 @@ int lhip_export_let(lhip_ctx* ctx, const float lo[3], const float hi[3], const l
                     uint64_t* n_nodes, uint64_t* n_bodies);
 /* Copies the LET of the last lhip_export_let; LHIP_LOGIC if there is none. */
 int lhip_export_let_copy(lhip_ctx* ctx, lhip_let_node* nodes, float* bx, float* by, float* bz, float* bm);
-/* Backend::memory_need (device and host bytes for n bodies). */
+/* Backend::memory_need (src/backend.hpp:137): device and host bytes a force call on n bodies
+ * reserves in the plain mode, buffer by buffer as the reserves size them, plus 64 MiB for allocator
+ * rounding; lhip_memory_need_mode (below) for either mode. Remote LETs and their exports (sized by
+ * the other ranks) and the differentiable simulations are not included. */
 int lhip_memory_need(uint64_t n, int32_t method, uint64_t* device_bytes, uint64_t* host_bytes);
-/* Backend::device_memory: free (including what this context holds) and total device bytes. */
+/* Backend::device_memory (src/backend.hpp:138-143): free device bytes counting every buffer this
+ * context's force path holds (lhip_device_bytes_held), and the total. */
 int lhip_device_memory(lhip_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 /* Backend::body_costs: interactions per body of the last force call (self excluded), in download
  * order; LHIP_NOT_PROVIDED when there is none. */
@@ -190,6 +194,8 @@ typedef struct lhip_walk_options {
 } lhip_walk_options;
 int lhip_set_precision(lhip_ctx* ctx, int32_t mode, const lhip_walk_options* options);
 int lhip_get_precision(const lhip_ctx* ctx, int32_t* mode, lhip_walk_options* options);
+/* Backend::memory_need for the given mode (LHIP_MODE_PLAIN or LHIP_MODE_MIXED). */
+int lhip_memory_need_mode(uint64_t n, int32_t method, int32_t mode, uint64_t* device_bytes, uint64_t* host_bytes);
 /* FP64 initial conditions and state (the extra entry Bodies' FP32 fields cannot carry). Mixed
  * mode only (LHIP_LOGIC otherwise). Masses and ids as in lhip_upload. */
 int lhip_upload_f64(lhip_ctx* ctx, uint64_t n, const double* x, const double* y, const double* z, const double* vx,
@@ -286,6 +292,9 @@ int lhip_scenario_collision(const lhip_collision_params* p, double* x, double* v
 int lhip_scenario_collision_vjp(const lhip_collision_params* p, const double* gx, const double* gv, double out[3]);
 
 /* ---- inspection (no reference counterpart; for parity tests and tools) ---- */
+
+/* Device bytes this context's force path holds (the buffers device_memory counts as free). */
+int lhip_device_bytes_held(lhip_ctx* ctx, uint64_t* held);
 
 /* Accelerations and potential of the last force call, in the backend's current order (rounded to
  * float in the mixed mode). */

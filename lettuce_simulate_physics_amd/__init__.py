@@ -467,13 +467,23 @@ class HipBackend:
         return t
 
     @staticmethod
-    def memory_need(n: int, method: str):
+    def memory_need(n: int, method: str, mode: str = "plain"):
+        """(device, host) bytes a force call on n bodies reserves (src/backend.hpp:137)."""
+        if mode not in ("plain", "mixed"):
+            raise ValueError("mode must be 'plain' or 'mixed'")
         d, h = _U64(), _U64()
-        rc = library().lhip_memory_need(_U64(n), METHOD_TREE if method == "tree" else METHOD_DIRECT,
-                                        ctypes.byref(d), ctypes.byref(h))
+        rc = library().lhip_memory_need_mode(_U64(n), METHOD_TREE if method == "tree" else METHOD_DIRECT,
+                                             MODE_MIXED if mode == "mixed" else MODE_PLAIN, ctypes.byref(d),
+                                             ctypes.byref(h))
         if rc != OK:
             raise ValueError("memory_need failed")
         return int(d.value), int(h.value)
+
+    def device_bytes_held(self) -> int:
+        """Device bytes this backend's force path holds (counted as free by device_memory)."""
+        v = _U64()
+        self._check(self._L.lhip_device_bytes_held(self._h, ctypes.byref(v)))
+        return int(v.value)
 
     def device_memory(self):
         f, t = _U64(), _U64()

@@ -158,9 +158,18 @@ int lhip_snapshot_restore(lhip_ctx* ctx, const char* path, lhip_snapshot_header*
 }
 
 int lhip_memory_need(uint64_t n, int32_t method, uint64_t* device_bytes, uint64_t* host_bytes) {
-    if (!device_bytes || !host_bytes) return LHIP_INVALID;
-    lhip::memory_need(n, method, device_bytes, host_bytes);
+    return lhip_memory_need_mode(n, method, LHIP_MODE_PLAIN, device_bytes, host_bytes);
+}
+
+int lhip_memory_need_mode(uint64_t n, int32_t method, int32_t mode, uint64_t* device_bytes, uint64_t* host_bytes) {
+    if (!device_bytes || !host_bytes || (mode != LHIP_MODE_PLAIN && mode != LHIP_MODE_MIXED)) return LHIP_INVALID;
+    lhip::memory_need(n, method, mode == LHIP_MODE_MIXED, device_bytes, host_bytes);
     return LHIP_OK;
+}
+
+int lhip_device_bytes_held(lhip_ctx* ctx, uint64_t* held) {
+    if (!held) return LHIP_INVALID;
+    return guarded(ctx, [&](lhip::Context& c) { *held = c.held_bytes(); });
 }
 
 int lhip_device_memory(lhip_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes) {

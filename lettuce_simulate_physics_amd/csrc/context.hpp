@@ -1043,37 +1043,75 @@ struct Context {
         return true;
     }
 
+    // Every buffer of the force path: the state, the sort, the tree, the walk, LETs and checks (not
+    // the differentiable simulations' own, which the force path never reuses).
+    std::size_t held_bytes() const {
+        std::size_t held = 0;
+        for (const DevBuf* b :
+             {&x, &y, &z, &vx, &vy, &vz, &m, &id, &cost, &x2, &y2, &z2, &vx2, &vy2, &vz2, &m2, &id2, &cost2, &ax, &ay,
+              &az, &pot, &posm, &key, &key2, &idx, &idx2, &tile_hist, &cube, &bbox_part, &red_part, &red_out,
+              &direct_part, &bl, &cnt, &gflag, &scan_tmp, &pkey, &pkey2, &pval, &pval2, &inv, &levstart, &node, &mom,
+              &prefix, &gstart, &counters, &pflag, &pstart, &fkey, &fseg, &capb, &fcube, &rs_seg, &rs_bbox,
+              &rs_cstart, &rs_gkey, &rs_ka, &rs_kb, &rs_va, &rs_vb, &xd, &yd, &zd, &vxd, &vyd, &vzd, &xd2, &yd2, &zd2,
+              &vxd2, &vyd2, &vzd2, &axd, &ayd, &azd, &potd, &momd, &rec64, &posd, &chk_tgt, &chk_out, &chk_part,
+              &let_rec, &let_body, &let_root, &let_state, &let_ex, &let_bcnt, &let_tmp, &let_out, &let_bods})
+            held += b->capacity();
+        return held;
+    }
+
+    // Free device bytes counting what this context holds (a grow-only buffer is freed before it is
+    // reallocated), and the total (src/backend.hpp:138-143).
     void device_memory(uint64_t* fr, uint64_t* tot) {
         LHIP_CHECK(hipSetDevice(dev));
         std::size_t f = 0, tt = 0;
         LHIP_CHECK(hipMemGetInfo(&f, &tt));
-        std::size_t held = 0;
-        for (const DevBuf* b : {&x, &y, &z, &vx, &vy, &vz, &m, &id, &cost, &x2, &y2, &z2, &vx2, &vy2, &vz2, &m2, &id2,
-                                &cost2, &ax, &ay, &az, &pot, &posm, &key, &key2, &idx, &idx2, &tile_hist, &direct_part,
-                                &bl, &cnt, &gflag, &scan_tmp, &pkey, &pkey2, &pval, &pval2, &inv, &node, &mom, &prefix, &gstart,
-                                &xd, &yd, &zd, &vxd, &vyd, &vzd, &xd2, &yd2, &zd2, &vxd2, &vyd2, &vzd2, &axd, &ayd, &azd,
-                                &potd, &momd, &rec64, &posd})
-            held += b->capacity();
-        *fr = f + held;
+        *fr = f + held_bytes();
         *tot = tt;
     }
 };
 
-// Device bytes per body: 22 four-byte state/alternate arrays + posm; the tree adds keys and indices
-// (double-buffered) and per-body build arrays, plus node records for the bound node_capacity(n)
-// (WalkNode 32 + NodeMom 64 + prefix 8 + level-sort scratch 28 = 132 B per node).
-inline void memory_need(uint64_t n, int method, uint64_t* dev, uint64_t* host) {
-    uint64_t d = n * (22 * 4 + 16) + (uint64_t(64) << 20);
+// Device bytes a force call on n bodies reserves (Backend::memory_need, src/backend.hpp:137): the
+// same sizes the reserve calls use, buffer by buffer, each at least 256 bytes, plus 64 MiB for
+// allocator rounding. Not included: remote LETs and their exports, whose sizes depend on the other
+// ranks, and the differentiable simulations. Host: nothing per body beyond the caller's arrays.
+inline void memory_need(uint64_t n, int method, bool mixed, uint64_t* dev, uint64_t* host) {
+    const uint64_t N = std::min<uint64_t>(n, INT32_MAX - 1);
+    const int ni = int(N);
+    uint64_t d = 0;
+    auto buf = [&d](uint64_t bytes) { d += std::max<uint64_t>(bytes, 256); };
+    for (int k = 0; k < 22; ++k) buf(N * 4);  // state, permute alternates, a, phi, costs
+    buf(N * 16);                               // posm
+    buf(sizeof(Cube));
+    buf(sizeof(double) * 6 * kReduceBlocks);
+    buf(std::max<uint64_t>(grid_for(ni, kBS), 12 * kReduceBlocks) * 8);  // reductions, exact pe
+    buf(sizeof(double) * 16);
+    buf(sizeof(WalkCounters));
+    if (mixed)
+        for (int k = 0; k < 16; ++k) buf(N * 8);  // FP64 state, alternates, a, phi
     if (method == LHIP_METHOD_TREE) {
-        d += n * (16 + 8 + 4 + 4 * 4) + radix_tile_hist_entries(int(std::min<uint64_t>(n, INT32_MAX))) * 4;
-        d += uint64_t(node_capacity((long long)n)) * 132;
+        const uint64_t C = uint64_t(node_capacity((long long)N));
+        buf(N * 8); buf(N * 8); buf(N * 4); buf(N * 4);  // keys, sort index (double-buffered)
+        buf(radix_tile_hist_entries(int(std::max<uint64_t>(N, C))) * 4);
+        buf(N * 4);                                                        // levels word
+        for (int k = 0; k < 5; ++k) buf((N + 1) * 4);                      // counts, groups, packs
+        buf(scan_tmp_entries(ni) * 4);
+        buf(C * 8); buf(C * 8); buf(C * 4); buf(C * 4); buf(C * 4);        // level sort, inverse
+        buf((kDepthLevels + 2) * 4);
+        buf(C * sizeof(WalkNode)); buf(C * sizeof(NodeMom)); buf(C * 8);  // records, key prefixes
+        if (mixed) {
+            buf(C * sizeof(NodeMomD));
+            buf(N * sizeof(double4));  // posd
+        }
+    } else if (mixed) {
+        buf(uint64_t(direct64_split(ni)) * 4 * N * 8);
+        buf(N * sizeof(SimRec));
     } else {
-        d += n * 16 * 64;  // worst-case split partial sums
+        const int S = direct_split(ni);
+        if (S > 1) buf(uint64_t(S) * N * sizeof(float4));
     }
-    *dev = d;
+    *dev = d + (uint64_t(64) << 20);
     *host = 0;
 }
-
 
 }  // namespace lhip
 

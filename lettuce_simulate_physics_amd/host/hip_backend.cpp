@@ -131,9 +131,13 @@ public:
         ck(lhip_compute_forces_remote(c_, &c, potential ? 1 : 0, lets.data(), int(lets.size())));
     }
 
+    // For this backend's mode: the mixed-precision form holds the FP64 state besides the f32 one.
     MemoryNeed memory_need(uint64_t n, ForceMethod m) const override {
         MemoryNeed r;
-        lhip_memory_need(n, m == ForceMethod::tree ? LHIP_METHOD_TREE : LHIP_METHOD_DIRECT, &r.device, &r.host);
+        int32_t mode = LHIP_MODE_PLAIN;
+        lhip_get_precision(c_, &mode, nullptr);
+        lhip_memory_need_mode(n, m == ForceMethod::tree ? LHIP_METHOD_TREE : LHIP_METHOD_DIRECT, mode, &r.device,
+                              &r.host);
         return r;
     }
 
