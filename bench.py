@@ -265,6 +265,53 @@ def tree_adjoint_leg(be, nstars: int, steps: int):
                 "max_diff_over_largest": float(np.abs(gt - gd).max() / np.abs(gd).max())}}
 
 
+# Algorithmic HBM bytes per body of the force step's memory-bound phases (the walk is VALU-bound):
+# the sort phase is the bounding box (12), keys (12 in + 12 out), one pass of (key, index) in and
+# out (24, the single-pass lower bound) and the permute (32 + 32 + 4); the build reads the keys (8)
+# and posm (16) and writes each node's 32 + 64 B records and reads the 64-B moments of each child
+# once (96 + 64 per node).
+SORT_BYTES_PER_BODY = 12 + 24 + 24 + 68
+BUILD_BYTES_PER_BODY = 8 + 16
+BUILD_BYTES_PER_NODE = 96 + 64
+PEAK_HBM = 8.0e12
+
+
+def hbm_phases(r, steps: int, n: int, nodes: int):
+    """Achieved HBM rates of the sort and build phases from their event-timed averages."""
+    out = {}
+    for name, ms, nbytes in (("sort", r["sort"] / steps, SORT_BYTES_PER_BODY * n),
+                             ("build", r["build"] / steps, BUILD_BYTES_PER_BODY * n + BUILD_BYTES_PER_NODE * nodes)):
+        rate = nbytes / (ms * 1e-3)
+        out[name] = {"ms": round(ms, 4), "algorithmic_bytes": int(nbytes), "GB_s": round(rate / 1e9, 1),
+                     "frac": round(rate / PEAK_HBM, 4)}
+    out["nodes"] = nodes
+    return out
+
+
+def direct_leg(sizes=(16384, 65536), reps: int = 5):
+    """The direct sum (src/cpu_backend.cpp:353-392) at the 16,384- and 65,536-body Plummer sizes:
+    device time of the force call, pair rate and FP32 rate (20 flop per pair) against the peak."""
+    import lettuce_simulate_physics_amd as L
+
+    be = L.HipBackend(0)
+    out = {}
+    p = L.ForceParams(G=1.0, eps=EPS, theta=THETA, method="direct")
+    for n in sizes:
+        be.upload(plummer(n))
+        be.compute_forces(p, False)
+        ms = []
+        for _ in range(reps):
+            be.compute_forces(p, False)
+            ms.append(be.times()["walk_ms"])
+        t = float(np.median(ms))
+        pairs = n * (n - 1)
+        out[str(n)] = {"ms": round(t, 4), "gpairs_per_s": round(pairs / (t * 1e-3) / 1e9, 1),
+                       "tflops": round(FLOP_P2P * pairs / (t * 1e-3) / 1e12, 2),
+                       "frac": round(FLOP_P2P * pairs / (t * 1e-3) / PEAK_FP32, 4)}
+    be.close()
+    return out
+
+
 def let_leg(b, reps: int = 5):
     """Two contexts on this GPU as two ranks of the reference's distributed tree step
     (src/parallel.cpp:260-295): the workload's sphere split at x = 0. Each rank builds its tree,
@@ -334,6 +381,7 @@ def main():
     ap.add_argument("--encounter-stars", type=int, default=524288, help="stars per disk of the tree adjoint leg")
     ap.add_argument("--encounter-steps", type=int, default=100, help="0 skips the tree adjoint leg")
     ap.add_argument("--no-let", action="store_true", help="skip the two-rank LET leg")
+    ap.add_argument("--no-direct", action="store_true", help="skip the direct-sum leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy 3 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -397,6 +445,7 @@ def main():
         "roofline": {"kernel": "walk_kernel", "bound": "valu", "achieved": round(achieved / 1e12, 3),
                      "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4),
                      "traffic": traffic},
+        "hbm_phases": hbm_phases(r, k, args.n, be.tree_counts()[0]),
         "cpu_baseline": None,
         "mixed": None,
         "disk": None,
@@ -407,6 +456,8 @@ def main():
         out["mixed"] = mixed_leg(be, b, k, args.warmup, wopt, force_ms)
         if args.disk_steps > 0:
             out["disk"] = disk_leg(be, args.disk_stars, args.disk_steps, wopt)
+    if not args.no_direct:
+        out["direct"] = direct_leg()
     if not args.no_let:
         out["let"] = let_leg(b)
     if not args.no_adjoint:

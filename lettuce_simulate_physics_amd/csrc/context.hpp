@@ -89,8 +89,8 @@ struct Context {
         red_out.reserve(sizeof(double) * 16);
         counters.reserve(sizeof(WalkCounters));
         walk_blocks = cus * walk_blocks_per_cu();
-        LHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_over), sizeof(int), hipHostMallocDefault));
-        *host_over = 0;
+        LHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_over), 2 * sizeof(int), hipHostMallocDefault));
+        host_over[0] = host_over[1] = 0;
     }
     ~Context() {
         adj_destroy(adj);
@@ -396,7 +396,7 @@ struct Context {
         // The level pass flags over-full finest cells; the host reads the flag while the rest of the
         // build runs, and only then takes the key-restart path (a second build).
         *reinterpret_cast<volatile int*>(host_over) = 0;
-        tree_build(stream, bb, inv_theta, ev[9]);
+        built_depth = tree_build(stream, bb, inv_theta, ev[9]);
         LHIP_CHECK(hipEventSynchronize(ev[9]));
         if (*reinterpret_cast<volatile int*>(host_over)) {
             restart_keys(nullptr, nullptr, nullptr, true);
@@ -554,8 +554,10 @@ struct Context {
         b = build_bufs();
         b.host_over = nullptr;
         if (box) { b.xd = box[0]; b.yd = box[1]; b.zd = box[2]; }
-        tree_build(stream, b, inv_theta);
+        built_depth = tree_build(stream, b, inv_theta);
     }
+    // Deepest level of the last build's multipoles (tree_moments_keep needs no more).
+    int built_depth = -1;
 
     BuildBufs build_bufs() {
         BuildBufs bb{};
@@ -596,6 +598,7 @@ struct Context {
             bb.fcube = fcube.as<double4>();
         }
         bb.host_over = host_over;
+        bb.depth_hint = -1;
         return bb;
     }
     // The walk's units: the groups, or their packs. The count is at [n].

@@ -74,6 +74,7 @@ __global__ void __launch_bounds__(kLvTile) levels_kernel(const uint64_t* __restr
     __syncthreads();
     const int i = base + tid;
     bool over = false;
+    int deepest = 0;
     if (i < n) {
         int best = -1;
         for (int u = 0; u <= kLeafMax; ++u) {
@@ -93,12 +94,24 @@ __global__ void __launch_bounds__(kLvTile) levels_kernel(const uint64_t* __restr
         cnt[i] = uint32_t(max(0, Lc - c));
         gflag[i] = c < Gc ? 1u : 0u;
         over = L > kMaxLevel;
+        deepest = Lc;
     }
     const unsigned long long ov = __ballot(over);
     if ((tid & 63) == 0 && ov) {
         atomicAdd(&ctr->overfull, __popcll(ov));
         if (host_over) __hip_atomic_store(host_over, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
+    // Deepest leaf level: a wave maximum, then one atomic per wave that raises the running value.
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) deepest = max(deepest, __shfl_xor(deepest, o, 64));
+    if ((tid & 63) == 0 && deepest > __hip_atomic_load(&ctr->max_level, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(&ctr->max_level, deepest);
+}
+
+// Publishes the level pass's deepest leaf level to the pinned host word the build waits on.
+__global__ void publish_depth_kernel(const WalkCounters* __restrict__ ctr, int* __restrict__ host_over) {
+    if (threadIdx.x == 0)
+        __hip_atomic_store(host_over + 1, ctr->max_level, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // The level pass over key frames (the restart path): the same windows on the extended keys, each
@@ -381,56 +394,100 @@ __global__ void __launch_bounds__(kBS) moments_level(int level, int n, const flo
                 x = b.x; y = b.y; z = b.z; m = b.w;
             }
         };
+        // Loads are issued in batches (the leaf-end words, four bodies, the child prefixes, four
+        // children) before the sums that use them, which keep the one-at-a-time order.
         if (info_leaf(nd.info)) {
+            // The leaf ends at the first body past lo whose cells start above this level: within
+            // kLeafMax bodies, unless its bodies coincide (then the scan continues one by one).
             int hi = nd.lo + 1;
-            while (hi < n && int((bl[hi] >> 16) & 0xff) - 1 >= level) ++hi;
+            {
+                uint32_t w[kLeafMax];
+#pragma unroll
+                for (int t = 0; t < kLeafMax; ++t) w[t] = nd.lo + 1 + t < n ? bl[nd.lo + 1 + t] : 0u;
+                bool run = true;
+#pragma unroll
+                for (int t = 0; t < kLeafMax; ++t) {
+                    run = run && nd.lo + 1 + t < n && int((w[t] >> 16) & 0xff) - 1 >= level;
+                    hi += run ? 1 : 0;
+                }
+            }
+            if (hi == nd.lo + 1 + kLeafMax)
+                while (hi < n && int((bl[hi] >> 16) & 0xff) - 1 >= level) ++hi;
             nd.hi = hi;
-            for (int j = nd.lo; j < nd.hi; ++j) {
-                double bx, by, bz, m;
-                body(j, bx, by, bz, m);
-                M += m;
-                sx = fma(m, bx, sx);
-                sy = fma(m, by, sy);
-                sz = fma(m, bz, sz);
+            for (int j0 = nd.lo; j0 < hi; j0 += 4) {
+                double bx[4], by[4], bz[4], m[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) body(min(j0 + t, hi - 1), bx[t], by[t], bz[t], m[t]);
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    if (j0 + t < hi) {
+                        M += m[t];
+                        sx = fma(m[t], bx[t], sx);
+                        sy = fma(m[t], by[t], sy);
+                        sz = fma(m[t], bz[t], sz);
+                    }
             }
             cx = M > 0 ? sx / M : double(md.gx);
             cy = M > 0 ? sy / M : double(md.gy);
             cz = M > 0 ? sz / M : double(md.gz);
-            for (int j = nd.lo; j < nd.hi; ++j) {
-                double bx, by, bz, m;
-                body(j, bx, by, bz, m);
-                const double dx = bx - cx, dy = by - cy, dz = bz - cz;
-                const double r2 = fma(dz, dz, fma(dx, dx, dy * dy));
-                q[0] = fma(m, fma(3 * dx, dx, -r2), q[0]);
-                q[3] = fma(m, fma(3 * dy, dy, -r2), q[3]);
-                q[5] += m * fma(3 * dz, dz, -r2);
-                const double m3 = m * 3;
-                q[1] = fma(m3 * dx, dy, q[1]);
-                q[2] = fma(m3 * dx, dz, q[2]);
-                q[4] += m3 * dy * dz;
+            for (int j0 = nd.lo; j0 < hi; j0 += 4) {
+                double bx[4], by[4], bz[4], mm[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) body(min(j0 + t, hi - 1), bx[t], by[t], bz[t], mm[t]);
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    if (j0 + t < hi) {
+                        const double m = mm[t];
+                        const double dx = bx[t] - cx, dy = by[t] - cy, dz = bz[t] - cz;
+                        const double r2 = fma(dz, dz, fma(dx, dx, dy * dy));
+                        q[0] = fma(m, fma(3 * dx, dx, -r2), q[0]);
+                        q[3] = fma(m, fma(3 * dy, dy, -r2), q[3]);
+                        q[5] += m * fma(3 * dz, dz, -r2);
+                        const double m3 = m * 3;
+                        q[1] = fma(m3 * dx, dy, q[1]);
+                        q[2] = fma(m3 * dx, dz, q[2]);
+                        q[4] += m3 * dy * dz;
+                    }
             }
         } else {
             const uint64_t pre = prefix[p];
             int ch = nd.first;
             // Children: the run of next-level nodes from `first` inside this cell (key prefix; over
-            // key frames, the first bodies' common extended prefix).
-            auto child = [&](int q) {
-                return EXT ? ext_common(F, nd.lo, node[q].lo) >= level : (prefix[q] >> 3) == pre;
-            };
-            while (ch < cend && child(ch)) {
-                double m, x, y, z;
-                if (MIXED) {
-                    const NodeMomD& c = momd[ch];
-                    m = c.m; x = c.cx; y = c.cy; z = c.cz;
-                } else {
-                    const NodeMom& c = mom[ch];
-                    m = c.m; x = c.cx; y = c.cy; z = c.cz;
+            // key frames, the first bodies' common extended prefix). At most 8.
+            if (EXT) {
+                while (ch < cend && ext_common(F, nd.lo, node[ch].lo) >= level) ++ch;
+            } else {
+                uint64_t pr[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) pr[t] = nd.first + t < cend ? prefix[nd.first + t] : 0ull;
+                bool run = true;
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    run = run && nd.first + t < cend && (pr[t] >> 3) == pre;
+                    ch += run ? 1 : 0;
                 }
-                M += m;
-                sx = fma(m, x, sx);
-                sy = fma(m, y, sy);
-                sz = fma(m, z, sz);
-                ++ch;
+            }
+            for (int k0 = nd.first; k0 < ch; k0 += 4) {
+                double m[4], x[4], y[4], z[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int k = min(k0 + t, ch - 1);
+                    if (MIXED) {
+                        const NodeMomD& c = momd[k];
+                        m[t] = c.m; x[t] = c.cx; y[t] = c.cy; z[t] = c.cz;
+                    } else {
+                        const NodeMom& c = mom[k];
+                        m[t] = c.m; x[t] = c.cx; y[t] = c.cy; z[t] = c.cz;
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    if (k0 + t < ch) {
+                        M += m[t];
+                        sx = fma(m[t], x[t], sx);
+                        sy = fma(m[t], y[t], sy);
+                        sz = fma(m[t], z[t], sz);
+                    }
             }
             nchild = ch - nd.first;
             nd.hi = node[ch - 1].hi;
@@ -740,7 +797,8 @@ void fill_u32(hipStream_t s, uint32_t* a, uint32_t v, long long n) {
 void tree_moments_keep(hipStream_t s, const BuildBufs& b) {
     const unsigned g = std::min<unsigned>(grid_for(b.n, kBS), 512u);
     const bool ext = b.frames > 1;
-    for (int l = tree_depth(ext ? b.frames : 1); l >= 0; --l) {
+    const int full = tree_depth(ext ? b.frames : 1);
+    for (int l = b.depth_hint >= 0 ? std::min(b.depth_hint, full) : full; l >= 0; --l) {
         if (ext)
             moments_level<true, true, true><<<g, kBS, 0, s>>>(l, b.n, nullptr, b.xd, b.yd, b.zd, b.m, b.bl,
                                                               b.levstart, b.prefix, b.node, b.mom, b.momd, 1.f,
@@ -758,13 +816,18 @@ std::size_t scan_tmp_entries(int n) {
 
 void scan_u32(hipStream_t s, uint32_t* a, int n, uint32_t* tmp) { scan_exclusive(s, a, n, tmp); }
 
-void tree_build(hipStream_t s, const BuildBufs& b, float inv_theta, hipEvent_t levels_done) {
+int tree_build(hipStream_t s, const BuildBufs& b, float inv_theta, hipEvent_t levels_done) {
     const int n = b.n;
     const bool ext = b.frames > 1;
     reset_build<<<1, 64, 0, s>>>(b.counters);
     LHIP_LAUNCH_CHECK();
     if (b.gmax < kLeafMax || b.gmax > kGroupMax) throw Error(kInvalid, "walk group size must be in [16, 128]");
     tree_levels(s, b);
+    const bool wait_levels = levels_done && b.host_over && !ext;
+    if (wait_levels) {
+        publish_depth_kernel<<<1, 64, 0, s>>>(b.counters, b.host_over);
+        LHIP_LAUNCH_CHECK();
+    }
     if (levels_done) LHIP_CHECK(hipEventRecord(levels_done, s));
     scan_exclusive(s, b.cnt, n, b.scan_tmp);
     scan_exclusive(s, b.gflag, n, b.scan_tmp);
@@ -793,9 +856,17 @@ void tree_build(hipStream_t s, const BuildBufs& b, float inv_theta, hipEvent_t l
         pstarts_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(b.gstart, b.gflag, b.pflag, n, b.pstart);
         LHIP_LAUNCH_CHECK();
     }
+    int depth = tree_depth(ext ? b.frames : 1);
+    if (wait_levels) {
+        // The rest of the build above is queued; the multipoles wait for the level pass's report.
+        LHIP_CHECK(hipEventSynchronize(levels_done));
+        if (*reinterpret_cast<volatile int*>(b.host_over)) return -1;  // key restarts: the caller rebuilds
+        const int deepest = *reinterpret_cast<volatile int*>(b.host_over + 1);
+        depth = std::min(depth, std::max(0, deepest));
+    }
     const unsigned g = std::min<unsigned>(grid_for(n, kBS), 512u);
     const Frames F = frames_of(b);
-    for (int l = tree_depth(ext ? b.frames : 1); l >= 0; --l) {
+    for (int l = depth; l >= 0; --l) {
         if (b.momd) {
             if (ext)
                 moments_level<true, false, true><<<g, kBS, 0, s>>>(l, n, nullptr, b.xd, b.yd, b.zd, b.m, b.bl,
@@ -815,6 +886,7 @@ void tree_build(hipStream_t s, const BuildBufs& b, float inv_theta, hipEvent_t l
         }
         LHIP_LAUNCH_CHECK();
     }
+    return depth;
 }
 
 }  // namespace lhip

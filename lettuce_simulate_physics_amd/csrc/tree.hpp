@@ -14,7 +14,7 @@ struct alignas(16) WalkCounters {
     int error;                        // nonzero: walk failure (traversal stack overflow)
     int overfull;                     // bodies in finest-level cells with > 16 bodies
     unsigned int next_group;          // walk work counter
-    int pad;
+    int max_level;                    // build: deepest leaf level of the level pass (frames = 1)
 };
 
 // The walk's node record, in level order (all cells of level 0, then level 1, ...; within a level
@@ -95,7 +95,9 @@ struct BuildBufs {
     const uint32_t* cap;
     const uint32_t* fseg;
     const double4* fcube;
-    int* host_over;            // pinned host flag: set by the level pass if a finest cell is over-full
+    int* host_over;            // pinned host words: [0] set by the level pass if a finest cell is
+                               // over-full; [1] the deepest leaf level, published after the pass
+    int depth_hint;            // tree_moments_keep: deepest level of the tree (< 0: every level)
 };
 
 std::size_t scan_tmp_entries(int n);
@@ -105,8 +107,11 @@ inline long long node_capacity(long long n) { return n + (n / 17 + 1) * kLevels 
 
 // Leaf levels, node enumeration, level-order layout, groups and multipoles. inv_theta as the
 // reference CPU computes it. levels_done (optional) is recorded after the level pass, which sets
-// *b.host_over when a deepest cell is over-full (key restarts needed, or coincident bodies).
-void tree_build(hipStream_t s, const BuildBufs& b, float inv_theta, hipEvent_t levels_done = nullptr);
+// host_over[0] when a deepest cell is over-full (key restarts needed, or coincident bodies) and
+// host_over[1] to the deepest leaf level. With both, the host waits for the pass before enqueuing
+// the multipoles: it launches only the levels the tree has, and none when host_over[0] is set (the
+// caller then restarts the keys and rebuilds). Returns the deepest level built (-1: no multipoles).
+int tree_build(hipStream_t s, const BuildBufs& b, float inv_theta, hipEvent_t levels_done = nullptr);
 // Deepest level of a tree with b.frames key frames.
 inline int tree_depth(int frames) { return kMaxLevel * frames; }
 

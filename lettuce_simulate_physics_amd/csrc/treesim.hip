@@ -241,7 +241,7 @@ struct Run {
         bb.xd = rx; bb.yd = rx + n; bb.zd = rx + 2 * std::size_t(n);
         const float inv_theta = fp.theta > 0 ? float(1.0 / fp.theta) : FLT_MAX;
         *reinterpret_cast<volatile int*>(c.host_over) = 0;
-        tree_build(s, bb, inv_theta, c.ev[9]);
+        c.built_depth = tree_build(s, bb, inv_theta, c.ev[9]);
         LHIP_CHECK(hipEventSynchronize(c.ev[9]));
         if (*reinterpret_cast<volatile int*>(c.host_over)) {
             // key restarts of the recorded call: its order is already the restarted one
@@ -251,6 +251,7 @@ struct Run {
         }
         bb = c.build_bufs();
         bb.xd = D(c.xd); bb.yd = D(c.yd); bb.zd = D(c.zd);
+        bb.depth_hint = c.built_depth;
         tree_moments_keep(s, bb);
         pack_posd_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(rx, rx + n, rx + 2 * std::size_t(n), c.m.as<float>(),
                                                           S.posd_rec.as<double4>(), n);
