@@ -11,7 +11,7 @@ import pytest
 
 import lettuce_simulate_physics_amd as L
 import oracle as O
-from helpers import rel_rms
+from helpers import rel_rms, restart_positions
 
 pytestmark = pytest.mark.gpu
 
@@ -380,6 +380,49 @@ def test_lets_cross_between_oracle_and_hip(hip, ranks):
             d = o.download()
             ref = np.array([[d.vx[0], d.vy[0], d.vz[0]]])
             assert rel_rms(g, ref) <= 1e-5
+
+
+@pytest.mark.parametrize("kind", ["clump", "nested", "coincident_in_restart"])
+def test_let_export_from_trees_with_key_restarts(ranks, kind):
+    # Trees deeper than 21 levels (restarted key frames): the export's per-depth passes cover every
+    # frame. Boxes: the whole set (every node) and a box around the restarted region (deep nodes
+    # opened, far ones terminal). Against the oracle's breadth-first export of its restarted tree.
+    x = restart_positions(kind)
+    n = len(x)
+    b = O.Bodies(x[:, 0].copy(), x[:, 1].copy(), x[:, 2].copy(), np.zeros(n, np.float32), np.zeros(n, np.float32),
+                 np.zeros(n, np.float32), np.full(n, 1.0 / n, np.float32), np.arange(n, dtype=np.uint32))
+    src = ranks[0]
+    p = P("tree")
+    src.upload(b)
+    src.build_tree(p)
+    o = O.Oracle()
+    o.upload(b)
+    o.build_tree(0.6, 1)
+    c = x[100]  # a body of the clustered region in every kind
+    boxes = [src.local_bounds(), ([float(c[0]) - 1e-4] * 3, [float(c[0]) + 1e-4] * 3),
+             ([float(v) - 1e-5 for v in c], [float(v) + 1e-5 for v in c])]
+    for box in boxes:
+        lh, lo_ = src.export_let(box, p), o.export_let(box, 1)
+        check_let_equal(lh, lo_)
+    assert len(src.export_let(boxes[0], p).bx) == n
+
+
+def test_let_export_in_the_mixed_form_matches_the_plain_one(ranks):
+    # Float bodies promoted to FP64 build the same tree; the exported records round the FP64 moments
+    # to float, so the LET equals the plain form's within the node tolerance, structure exactly.
+    b = zero_v(O.plummer(20000, 41))
+    h0, h1 = split(b, True)
+    p = P("tree")
+    lets = {}
+    for mode in ("plain", "mixed"):
+        for k, h in enumerate((h0, h1)):
+            ranks[k].set_precision(mode)
+            ranks[k].upload(h)
+            ranks[k].build_tree(p)
+        lets[mode] = ranks[1].export_let(ranks[0].local_bounds(), p)
+    for r in ranks:
+        r.set_precision("plain")
+    check_let_equal(lets["mixed"], lets["plain"])
 
 
 # ---------------------------------------------------------------- mixed precision and robustness
