@@ -56,9 +56,12 @@ __global__ void __launch_bounds__(kLvTile) levels_kernel(const uint64_t* __restr
                                                          uint32_t* __restrict__ bl, uint32_t* __restrict__ cnt,
                                                          uint32_t* __restrict__ gflag, WalkCounters* __restrict__ ctr,
                                                          int* __restrict__ host_over) {
+    constexpr int kA = kHalo + kLvTile;  // window starts held per tile
     __shared__ uint64_t sk[kHalo + kLvTile + kHalo];
-    __shared__ uint8_t d16[kHalo + kLvTile];
-    __shared__ uint8_t d128[kHalo + kLvTile];
+    // Window values as 1 + common octal prefix (0 where the window leaves the bodies), so that a
+    // window maximum is directly 1 + the largest prefix, and then range maxima over them.
+    __shared__ uint8_t d16[kA];
+    __shared__ uint8_t d128[kA];
     const int tid = threadIdx.x;
     const int base = blockIdx.x * kLvTile;
     for (int t = tid; t < kHalo + kLvTile + kHalo; t += kLvTile) {
@@ -66,29 +69,52 @@ __global__ void __launch_bounds__(kLvTile) levels_kernel(const uint64_t* __restr
         sk[t] = (j >= 0 && j < n) ? key[j] : 0ull;
     }
     __syncthreads();
-    for (int t = tid; t < kHalo + kLvTile; t += kLvTile) {
+    for (int t = tid; t < kA; t += kLvTile) {
         const long long j = (long long)base - kHalo + t;
-        d16[t] = (j >= 0 && j + kLeafMax < n) ? uint8_t(common_octs(sk[t], sk[t + kLeafMax])) : uint8_t(0xff);
-        d128[t] = (j >= 0 && j + gmax < n) ? uint8_t(common_octs(sk[t], sk[t + gmax])) : uint8_t(0xff);
+        d16[t] = (j >= 0 && j + kLeafMax < n) ? uint8_t(1 + common_octs(sk[t], sk[t + kLeafMax])) : uint8_t(0);
+        d128[t] = (j >= 0 && j + gmax < n) ? uint8_t(1 + common_octs(sk[t], sk[t + gmax])) : uint8_t(0);
+    }
+    // Range maxima by doubling, in place: after step s, entry t holds the maximum over
+    // [t, t + 2^s). The leaf windows (17 starts) need 16-wide ranges, the group windows (gmax + 1
+    // starts) the widest power of two not above gmax + 1. A query only reads ranges inside the
+    // array, so the entries whose range runs past its end (read clamped) are never used.
+    const int k128 = 31 - __builtin_clz(unsigned(gmax + 1));
+    for (int step = 0; step < max(4, k128); ++step) {
+        const int h = 1 << step;
+        uint8_t a16[2], b16[2], a128[2], b128[2];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int t = tid + r * kLvTile;
+            if (t < kA) {
+                const int u = min(t + h, kA - 1);
+                a16[r] = d16[t]; b16[r] = d16[u];
+                a128[r] = d128[t]; b128[r] = d128[u];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int t = tid + r * kLvTile;
+            if (t < kA) {
+                if (step < 4) d16[t] = max(a16[r], b16[r]);
+                if (step < k128) d128[t] = max(a128[r], b128[r]);
+            }
+        }
     }
     __syncthreads();
     const int i = base + tid;
     bool over = false;
     int deepest = 0;
     if (i < n) {
-        int best = -1;
-        for (int u = 0; u <= kLeafMax; ++u) {
-            const int v = d16[tid + kHalo - kLeafMax + u];
-            if (v != 0xff) best = max(best, v);
-        }
-        const int L = best + 1;
-        best = -1;
-        for (int u = 0; u <= gmax; ++u) {
-            const int v = d128[tid + kHalo - gmax + u];
-            if (v != 0xff) best = max(best, v);
-        }
+        // Leaf window starts [i - 16, i]: two overlapping 16-wide ranges.
+        const int a = tid + kHalo - kLeafMax;
+        const int L = max(int(d16[a]), int(d16[a + 1]));
+        // Group window starts [i - gmax, i]: two overlapping 2^k128-wide ranges.
+        const int ag = tid + kHalo - gmax;
+        const int Gw = max(int(d128[ag]), int(d128[ag + gmax + 1 - (1 << k128)]));
         const int Lc = min(L, kMaxLevel);
-        const int Gc = min(best + 1, Lc);
+        const int Gc = min(Gw, Lc);
         const int c = i == 0 ? -1 : common_octs(sk[tid + kHalo - 1], sk[tid + kHalo]);
         bl[i] = uint32_t(L) | uint32_t(Gc) << 8 | uint32_t(c + 1) << 16;
         cnt[i] = uint32_t(max(0, Lc - c));
