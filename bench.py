@@ -28,6 +28,22 @@ N_BODIES = 1 << 20
 THETA, EPS, DT = 0.6, 0.01, 1e-3
 FLOP_P2P, FLOP_P2C = 20, 51
 PEAK_FP32 = 157.3e12  # MI355X published FP32 vector peak (packed issue)
+# Issue-slot ceiling of the pair loops: one wave64 VALU instruction per SIMD every 4 cycles, i.e.
+# 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz lane-instructions per second. The inner loops of the built
+# kernels take 12 VALU instructions per direct pair, about 13.3 per walk body pair and 32.3 per walk
+# cell term (gfx950 ISA of the 1-3 slot loops); v_rsq_f32 runs on the transcendental unit at half
+# rate beside the VALU stream (a 12 FMA + 1 rsq chain runs at the rate of its 12 FMAs alone), so it
+# takes no VALU slot.
+VALU_LANE_RATE = 256 * 4 * 16 * 2.4e9
+VALU_DIRECT, VALU_P2P, VALU_P2C = 12.0, 13.3, 32.3
+
+
+def issue_ceiling(n_pp: float, n_pc: float, valu_pp: float, seconds: float) -> dict:
+    """FP32 rate the instruction mix allows at full VALU issue, and the share of it achieved."""
+    t_min = (valu_pp * n_pp + VALU_P2C * n_pc) / VALU_LANE_RATE
+    flops = FLOP_P2P * n_pp + FLOP_P2C * n_pc
+    return {"valu_per_p2p": valu_pp, "valu_per_p2c": VALU_P2C, "clock_ghz": 2.4,
+            "ceiling": round(flops / t_min / 1e12, 2), "unit": "TFLOP/s", "frac": round(t_min / seconds, 4)}
 
 
 def plummer(n: int, seed: int = 1):
@@ -307,7 +323,8 @@ def direct_leg(sizes=(16384, 65536), reps: int = 5):
         pairs = n * (n - 1)
         out[str(n)] = {"ms": round(t, 4), "gpairs_per_s": round(pairs / (t * 1e-3) / 1e9, 1),
                        "tflops": round(FLOP_P2P * pairs / (t * 1e-3) / 1e12, 2),
-                       "frac": round(FLOP_P2P * pairs / (t * 1e-3) / PEAK_FP32, 4)}
+                       "frac": round(FLOP_P2P * pairs / (t * 1e-3) / PEAK_FP32, 4),
+                       "issue_ceiling": issue_ceiling(pairs, 0, VALU_DIRECT, t * 1e-3)}
     be.close()
     return out
 
@@ -444,7 +461,8 @@ def main():
         "interactions_per_body": round(r["inter"] / k / args.n, 2),
         "roofline": {"kernel": "walk_kernel", "bound": "valu", "achieved": round(achieved / 1e12, 3),
                      "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4),
-                     "traffic": traffic},
+                     "traffic": traffic,
+                     "issue_ceiling": issue_ceiling(r["p2p"] / k, r["p2c"] / k, VALU_P2P, walk_ms * 1e-3)},
         "hbm_phases": hbm_phases(r, k, args.n, be.tree_counts()[0]),
         "cpu_baseline": None,
         "mixed": None,

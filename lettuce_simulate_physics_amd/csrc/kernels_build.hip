@@ -55,7 +55,7 @@ constexpr int kHalo = kGroupMax;  // 128
 __global__ void __launch_bounds__(kLvTile) levels_kernel(const uint64_t* __restrict__ key, int n, int gmax,
                                                          uint32_t* __restrict__ bl, uint32_t* __restrict__ cnt,
                                                          uint32_t* __restrict__ gflag, WalkCounters* __restrict__ ctr,
-                                                         int* __restrict__ host_over) {
+                                                         int* __restrict__ host_over, uint32_t* __restrict__ bdeep) {
     constexpr int kA = kHalo + kLvTile;  // window starts held per tile
     __shared__ uint64_t sk[kHalo + kLvTile + kHalo];
     // Window values as 1 + common octal prefix (0 where the window leaves the bodies), so that a
@@ -64,9 +64,17 @@ __global__ void __launch_bounds__(kLvTile) levels_kernel(const uint64_t* __restr
     __shared__ uint8_t d128[kA];
     const int tid = threadIdx.x;
     const int base = blockIdx.x * kLvTile;
-    for (int t = tid; t < kHalo + kLvTile + kHalo; t += kLvTile) {
-        const long long j = (long long)base - kHalo + t;
-        sk[t] = (j >= 0 && j < n) ? key[j] : 0ull;
+    {
+        // Both loads of the tile are issued before either is stored, so their latencies overlap.
+        static_assert(kHalo + kLvTile + kHalo == 2 * kLvTile, "the key tile is two loads per thread");
+        uint64_t kv[2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const long long j = (long long)base - kHalo + tid + r * kLvTile;
+            kv[r] = (j >= 0 && j < n) ? key[j] : 0ull;
+        }
+        sk[tid] = kv[0];
+        sk[tid + kLvTile] = kv[1];
     }
     __syncthreads();
     for (int t = tid; t < kA; t += kLvTile) {
@@ -127,17 +135,39 @@ __global__ void __launch_bounds__(kLvTile) levels_kernel(const uint64_t* __restr
         atomicAdd(&ctr->overfull, __popcll(ov));
         if (host_over) __hip_atomic_store(host_over, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
-    // Deepest leaf level: a wave maximum, then one atomic per wave that raises the running value.
+    // Deepest leaf level of the block: wave maxima, then one plain store per block (no shared
+    // counter word that every wave would wait on); the publish kernel reduces the blocks.
+    __shared__ int wdeep[kLvTile / 64];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) deepest = max(deepest, __shfl_xor(deepest, o, 64));
-    if ((tid & 63) == 0 && deepest > __hip_atomic_load(&ctr->max_level, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-        atomicMax(&ctr->max_level, deepest);
+    if ((tid & 63) == 0) wdeep[tid >> 6] = deepest;
+    __syncthreads();
+    if (tid == 0) {
+        int d = wdeep[0];
+#pragma unroll
+        for (int w = 1; w < kLvTile / 64; ++w) d = max(d, wdeep[w]);
+        bdeep[blockIdx.x] = uint32_t(d);
+    }
 }
 
-// Publishes the level pass's deepest leaf level to the pinned host word the build waits on.
-__global__ void publish_depth_kernel(const WalkCounters* __restrict__ ctr, int* __restrict__ host_over) {
-    if (threadIdx.x == 0)
-        __hip_atomic_store(host_over + 1, ctr->max_level, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+// Reduces the level pass's per-block deepest levels and publishes the result to the counters and
+// to the pinned host word the build waits on. One block of kPublish threads.
+constexpr int kPublish = 256;
+__global__ void __launch_bounds__(kPublish) publish_depth_kernel(const uint32_t* __restrict__ bdeep, int nblocks,
+                                                                  WalkCounters* __restrict__ ctr,
+                                                                  int* __restrict__ host_over) {
+    __shared__ int wdeep[kPublish / 64];
+    int d = 0;
+    for (int i = threadIdx.x; i < nblocks; i += kPublish) d = max(d, int(bdeep[i]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) d = max(d, __shfl_xor(d, o, 64));
+    if ((threadIdx.x & 63) == 0) wdeep[threadIdx.x >> 6] = d;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kPublish / 64; ++w) d = max(d, wdeep[w]);
+        ctr->max_level = d;
+        __hip_atomic_store(host_over + 1, d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 // The level pass over key frames (the restart path): the same windows on the extended keys, each
@@ -742,7 +772,7 @@ void tree_levels(hipStream_t s, const BuildBufs& b) {
                                                            b.counters, b.host_over);
     else
         levels_kernel<<<grid_for(n, kLvTile), kLvTile, 0, s>>>(b.key, n, b.gmax, b.bl, b.cnt, b.gflag, b.counters,
-                                                               b.host_over);
+                                                               b.host_over, b.gstart);
     LHIP_LAUNCH_CHECK();
 }
 
@@ -851,7 +881,7 @@ int tree_build(hipStream_t s, const BuildBufs& b, float inv_theta, hipEvent_t le
     tree_levels(s, b);
     const bool wait_levels = levels_done && b.host_over && !ext;
     if (wait_levels) {
-        publish_depth_kernel<<<1, 64, 0, s>>>(b.counters, b.host_over);
+        publish_depth_kernel<<<1, kPublish, 0, s>>>(b.gstart, grid_for(n, kLvTile), b.counters, b.host_over);
         LHIP_LAUNCH_CHECK();
     }
     if (levels_done) LHIP_CHECK(hipEventRecord(levels_done, s));

@@ -61,13 +61,20 @@ __global__ void __launch_bounds__(kSortThreads) radix_count(const uint64_t* __re
                                                             uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[kRadix];
     if (n_dev) n = min(n, int(*n_dev));
-    h[threadIdx.x] = 0;
-    __syncthreads();
     const int base = blockIdx.x * kSortTile;
+    // All of the tile's loads are issued before the first use, so their latencies overlap.
+    uint64_t kk[kSortKPT];
 #pragma unroll
     for (int r = 0; r < kSortKPT; ++r) {
         const int i = base + r * kSortThreads + threadIdx.x;
-        if (i < n) atomicAdd(&h[(key[i] >> shift) & 0xff], 1u);
+        kk[r] = i < n ? key[i] : 0ull;
+    }
+    h[threadIdx.x] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortKPT; ++r) {
+        const int i = base + r * kSortThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(kk[r] >> shift) & 0xff], 1u);
     }
     __syncthreads();
     hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
@@ -114,11 +121,25 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter(const uint64_t* __
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int base = blockIdx.x * kSortTile;
     if (n_dev) n = min(n, int(*n_dev));
+    // All global loads of the block are issued here, before the first barrier, so that their
+    // latencies overlap each other and the digit-base scan.
+    uint64_t k[kSortKPT];
+    uint32_t v[kSortKPT], pos[kSortKPT];
+    int dg[kSortKPT];
+#pragma unroll
+    for (int r = 0; r < kSortKPT; ++r) {
+        const int i = base + r * kSortThreads + tid;
+        const bool ok = i < n;
+        k[r] = ok ? kin[i] : ~0ull;
+        v[r] = ok ? vin[i] : 0u;
+    }
+    const uint32_t my_tot = tot[tid];
+    const uint32_t my_hist = hist[tid * ntiles + blockIdx.x];
     run[tid] = 0;
     for (int w = 0; w < kSortThreads / 64; ++w) cnt[w][tid] = 0;
     // Digit base = exclusive scan of the digit totals (block scan of 256 values).
     {
-        dstart[tid] = tot[tid];
+        dstart[tid] = my_tot;
         __syncthreads();
         for (int o = 1; o < kRadix; o <<= 1) {
             const uint32_t a = tid >= o ? dstart[tid - o] : 0u;
@@ -126,19 +147,13 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter(const uint64_t* __
             dstart[tid] += a;
             __syncthreads();
         }
-        gbase[tid] = dstart[tid] - tot[tid] + hist[tid * ntiles + blockIdx.x];
+        gbase[tid] = dstart[tid] - my_tot + my_hist;
         __syncthreads();
     }
-    uint64_t k[kSortKPT];
-    uint32_t v[kSortKPT], pos[kSortKPT];
-    int dg[kSortKPT];
     const uint64_t lt = (1ull << lane) - 1ull;
 #pragma unroll
     for (int r = 0; r < kSortKPT; ++r) {
-        const int i = base + r * kSortThreads + tid;
-        const bool ok = i < n;
-        k[r] = ok ? kin[i] : ~0ull;
-        v[r] = ok ? vin[i] : 0u;
+        const bool ok = base + r * kSortThreads + tid < n;
         // Out-of-range slots take digit 255 and sort after every real key of the tile (they are last
         // in tile order), so real keys keep their ranks.
         dg[r] = ok ? int((k[r] >> shift) & 0xff) : 0xff;

@@ -81,7 +81,7 @@ struct BuildBufs {
     WalkNode* node;            // cap
     NodeMom* mom;              // cap
     uint64_t* prefix;          // cap
-    uint32_t* gstart;          // n + 1
+    uint32_t* gstart;          // n + 1 (until the group pass writes it: the level pass's per-block deepest levels)
     // walk packs (pmax > 0): sibling groups packed up to pmax bodies
     int pmax;
     uint32_t* pflag;           // n + 1: pack starts per group, then their exclusive scan (count at [n])
