@@ -424,8 +424,12 @@ __device__ __forceinline__ float rcrit2_of(float cx, float cy, float cz, float g
 // KEEP: a second pass on other positions over a built tree (frozen-topology replay): node records,
 // rcrit2 and qrel (the walk's decisions) stay as the first pass wrote them; com, mass and
 // quadrupole are recomputed.
+// part: kMomLeaves takes the leaves of levels 0..level in one launch (a leaf needs only its bodies);
+// kMomInternal takes the internal nodes of `level` (their children are done by then). Each node's
+// sums are the same either way.
+constexpr int kMomLeaves = 1, kMomInternal = 2;
 template <bool MIXED, bool KEEP = false, bool EXT = false>
-__global__ void __launch_bounds__(kBS) moments_level(int level, int n, const float4* __restrict__ posm,
+__global__ void __launch_bounds__(kBS) moments_level(int part, int level, int n, const float4* __restrict__ posm,
                                                      const double* __restrict__ xd, const double* __restrict__ yd,
                                                      const double* __restrict__ zd, const float* __restrict__ bm,
                                                      const uint32_t* __restrict__ bl,
@@ -433,10 +437,12 @@ __global__ void __launch_bounds__(kBS) moments_level(int level, int n, const flo
                                                      const uint64_t* __restrict__ prefix, WalkNode* __restrict__ node,
                                                      NodeMom* __restrict__ mom, NodeMomD* __restrict__ momd,
                                                      float inv_theta, Frames F) {
-    const int p0 = int(levstart[level]), p1 = int(levstart[level + 1]);
+    const bool leaves = part == kMomLeaves;
+    const int p0 = leaves ? 0 : int(levstart[level]), p1 = int(levstart[level + 1]);
     const int cend = int(levstart[level + 2]);  // end of the next level
     for (int p = p0 + blockIdx.x * kBS + threadIdx.x; p < p1; p += gridDim.x * kBS) {
         WalkNode nd = node[p];
+        if (info_leaf(nd.info) != leaves) continue;
         NodeMom md = mom[p];
         double M = 0, sx = 0, sy = 0, sz = 0;
         double q[6] = {0, 0, 0, 0, 0, 0};
@@ -452,9 +458,10 @@ __global__ void __launch_bounds__(kBS) moments_level(int level, int n, const flo
         };
         // Loads are issued in batches (the leaf-end words, four bodies, the child prefixes, four
         // children) before the sums that use them, which keep the one-at-a-time order.
-        if (info_leaf(nd.info)) {
-            // The leaf ends at the first body past lo whose cells start above this level: within
+        if (leaves) {
+            // The leaf ends at the first body past lo whose cells start above its level: within
             // kLeafMax bodies, unless its bodies coincide (then the scan continues one by one).
+            const int level = info_level(nd.info);
             int hi = nd.lo + 1;
             {
                 uint32_t w[kLeafMax];
@@ -850,20 +857,28 @@ void fill_u32(hipStream_t s, uint32_t* a, uint32_t v, long long n) {
     LHIP_LAUNCH_CHECK();
 }
 
+// Launch grids of the multipole pass: the leaves of every level at once, then one level of internal
+// nodes per launch (grid-stride loops, so any grid covers the nodes).
+inline unsigned leaf_grid(int n) { return std::min<unsigned>(grid_for(n, kBS), 2048u); }
+inline unsigned level_grid(int n) { return std::min<unsigned>(grid_for(n, kBS), 512u); }
+
 void tree_moments_keep(hipStream_t s, const BuildBufs& b) {
-    const unsigned g = std::min<unsigned>(grid_for(b.n, kBS), 512u);
     const bool ext = b.frames > 1;
     const int full = tree_depth(ext ? b.frames : 1);
-    for (int l = b.depth_hint >= 0 ? std::min(b.depth_hint, full) : full; l >= 0; --l) {
+    const int depth = b.depth_hint >= 0 ? std::min(b.depth_hint, full) : full;
+    auto launch = [&](int part, int l, unsigned g) {
         if (ext)
-            moments_level<true, true, true><<<g, kBS, 0, s>>>(l, b.n, nullptr, b.xd, b.yd, b.zd, b.m, b.bl,
+            moments_level<true, true, true><<<g, kBS, 0, s>>>(part, l, b.n, nullptr, b.xd, b.yd, b.zd, b.m, b.bl,
                                                               b.levstart, b.prefix, b.node, b.mom, b.momd, 1.f,
                                                               frames_of(b));
         else
-            moments_level<true, true><<<g, kBS, 0, s>>>(l, b.n, nullptr, b.xd, b.yd, b.zd, b.m, b.bl, b.levstart,
-                                                        b.prefix, b.node, b.mom, b.momd, 1.f, frames_of(b));
+            moments_level<true, true><<<g, kBS, 0, s>>>(part, l, b.n, nullptr, b.xd, b.yd, b.zd, b.m, b.bl,
+                                                        b.levstart, b.prefix, b.node, b.mom, b.momd, 1.f,
+                                                        frames_of(b));
         LHIP_LAUNCH_CHECK();
-    }
+    };
+    launch(kMomLeaves, depth, leaf_grid(b.n));
+    for (int l = depth - 1; l >= 0; --l) launch(kMomInternal, l, level_grid(b.n));
 }
 
 std::size_t scan_tmp_entries(int n) {
@@ -920,28 +935,30 @@ int tree_build(hipStream_t s, const BuildBufs& b, float inv_theta, hipEvent_t le
         const int deepest = *reinterpret_cast<volatile int*>(b.host_over + 1);
         depth = std::min(depth, std::max(0, deepest));
     }
-    const unsigned g = std::min<unsigned>(grid_for(n, kBS), 512u);
     const Frames F = frames_of(b);
-    for (int l = depth; l >= 0; --l) {
+    // The deepest level present holds only leaves, so the internal levels start one above it.
+    auto launch = [&](int part, int l, unsigned g) {
         if (b.momd) {
             if (ext)
-                moments_level<true, false, true><<<g, kBS, 0, s>>>(l, n, nullptr, b.xd, b.yd, b.zd, b.m, b.bl,
+                moments_level<true, false, true><<<g, kBS, 0, s>>>(part, l, n, nullptr, b.xd, b.yd, b.zd, b.m, b.bl,
                                                                    b.levstart, b.prefix, b.node, b.mom, b.momd,
                                                                    inv_theta, F);
             else
-                moments_level<true><<<g, kBS, 0, s>>>(l, n, nullptr, b.xd, b.yd, b.zd, b.m, b.bl, b.levstart,
+                moments_level<true><<<g, kBS, 0, s>>>(part, l, n, nullptr, b.xd, b.yd, b.zd, b.m, b.bl, b.levstart,
                                                       b.prefix, b.node, b.mom, b.momd, inv_theta, F);
         } else {
             if (ext)
-                moments_level<false, false, true><<<g, kBS, 0, s>>>(l, n, b.posm, nullptr, nullptr, nullptr, nullptr,
-                                                                    b.bl, b.levstart, b.prefix, b.node, b.mom,
-                                                                    nullptr, inv_theta, F);
+                moments_level<false, false, true><<<g, kBS, 0, s>>>(part, l, n, b.posm, nullptr, nullptr, nullptr,
+                                                                    nullptr, b.bl, b.levstart, b.prefix, b.node,
+                                                                    b.mom, nullptr, inv_theta, F);
             else
-                moments_level<false><<<g, kBS, 0, s>>>(l, n, b.posm, nullptr, nullptr, nullptr, nullptr, b.bl,
+                moments_level<false><<<g, kBS, 0, s>>>(part, l, n, b.posm, nullptr, nullptr, nullptr, nullptr, b.bl,
                                                        b.levstart, b.prefix, b.node, b.mom, nullptr, inv_theta, F);
         }
         LHIP_LAUNCH_CHECK();
-    }
+    };
+    launch(kMomLeaves, depth, leaf_grid(n));
+    for (int l = depth - 1; l >= 0; --l) launch(kMomInternal, l, level_grid(n));
     return depth;
 }
 
