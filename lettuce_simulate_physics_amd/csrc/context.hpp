@@ -45,6 +45,7 @@ struct Context {
     DevBuf cube, bbox_part, red_part, red_out, direct_part;
     // tree
     DevBuf bl, cnt, gflag, scan_tmp, pkey, pkey2, pval, pval2, inv, levstart, node, mom, prefix, gstart, counters;
+    DevBuf worder;  // the walk's unit order, written by every build
     DevBuf pflag, pstart;  // walk packs of sibling groups (pack_max > 0)
     int pack_max = 0;
     long long cap_nodes = 0;
@@ -269,6 +270,7 @@ struct Context {
         cnt.reserve((N + 1) * 4);
         gflag.reserve((N + 1) * 4);
         gstart.reserve((N + 1) * 4);
+        worder.reserve((N + 1) * 4);
         pflag.reserve((N + 1) * 4);
         pstart.reserve((N + 1) * 4);
         scan_tmp.reserve(scan_tmp_entries(nn) * 4);
@@ -378,9 +380,11 @@ struct Context {
         tree_ok = false;
         t.sort_ms = t.build_ms = 0;
         if (p.method != LHIP_METHOD_TREE || n == 0) return;
-        // The sort reorders the bodies but not a, phi or costs: stale until the next force call.
+        // The sort reorders the bodies; a, phi and the costs are stale until the next force call.
+        // The costs are carried with the bodies all the same: the build orders the walk's groups by
+        // them (longest first), which changes no result.
         reserve_tree(n);
-        const bool keep_cost = false;
+        const bool keep_cost = true;
         costs_ok = false;
         has_pe = false;
         acc_ok = false;
@@ -586,6 +590,8 @@ struct Context {
         bb.mom = mom.as<NodeMom>();
         bb.prefix = prefix.as<uint64_t>();
         bb.gstart = gstart.as<uint32_t>();
+        bb.wcost = cost.as<float>();
+        bb.worder = worder.as<uint32_t>();
         bb.pmax = pack_max;
         bb.pflag = pflag.as<uint32_t>();
         bb.pstart = pstart.as<uint32_t>();
@@ -613,6 +619,7 @@ struct Context {
         a.mom = mom.as<NodeMom>();
         a.gstart = walk_starts();
         a.gcount = walk_count();
+        a.order = p.method == LHIP_METHOD_TREE ? worder.as<uint32_t>() : nullptr;
         a.n = n;
         a.eps2 = float(p.eps * p.eps);
         a.G = float(p.G);
@@ -1054,7 +1061,7 @@ struct Context {
              {&x, &y, &z, &vx, &vy, &vz, &m, &id, &cost, &x2, &y2, &z2, &vx2, &vy2, &vz2, &m2, &id2, &cost2, &ax, &ay,
               &az, &pot, &posm, &key, &key2, &idx, &idx2, &tile_hist, &cube, &bbox_part, &red_part, &red_out,
               &direct_part, &bl, &cnt, &gflag, &scan_tmp, &pkey, &pkey2, &pval, &pval2, &inv, &levstart, &node, &mom,
-              &prefix, &gstart, &counters, &pflag, &pstart, &fkey, &fseg, &capb, &fcube, &rs_seg, &rs_bbox,
+              &prefix, &gstart, &worder, &counters, &pflag, &pstart, &fkey, &fseg, &capb, &fcube, &rs_seg, &rs_bbox,
               &rs_cstart, &rs_gkey, &rs_ka, &rs_kb, &rs_va, &rs_vb, &xd, &yd, &zd, &vxd, &vyd, &vzd, &xd2, &yd2, &zd2,
               &vxd2, &vyd2, &vzd2, &axd, &ayd, &azd, &potd, &momd, &rec64, &posd, &chk_tgt, &chk_out, &chk_part,
               &let_rec, &let_body, &let_root, &let_state, &let_ex, &let_bcnt, &let_tmp, &let_out, &let_bods})
@@ -1096,7 +1103,7 @@ inline void memory_need(uint64_t n, int method, bool mixed, uint64_t* dev, uint6
         buf(N * 8); buf(N * 8); buf(N * 4); buf(N * 4);  // keys, sort index (double-buffered)
         buf(radix_tile_hist_entries(int(std::max<uint64_t>(N, C))) * 4);
         buf(N * 4);                                                        // levels word
-        for (int k = 0; k < 5; ++k) buf((N + 1) * 4);                      // counts, groups, packs
+        for (int k = 0; k < 6; ++k) buf((N + 1) * 4);                      // counts, groups, packs, order
         buf(scan_tmp_entries(ni) * 4);
         buf(C * 8); buf(C * 8); buf(C * 4); buf(C * 4); buf(C * 4);        // level sort, inverse
         buf((kDepthLevels + 2) * 4);

@@ -189,8 +189,9 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
         if (lane == 0) g = int(atomicAdd(&a.counters->next_group, 1u));
         g = __builtin_amdgcn_readfirstlane(__shfl(g, 0, 64));
         if (g >= ngroups) break;
-        const int glo = blocks ? g * kGroupMax : int(a.gstart[g]);
-        const int ghi = blocks ? min(a.n, glo + kGroupMax) : int(a.gstart[g + 1]);
+        const int gu = !blocks && a.order ? int(a.order[g]) : g;  // the g-th group to take
+        const int glo = blocks ? g * kGroupMax : int(a.gstart[gu]);
+        const int ghi = blocks ? min(a.n, glo + kGroupMax) : int(a.gstart[gu + 1]);
         const bool chunked = ghi - glo > kGroupMax;
         for (int clo = glo; clo < ghi; clo += kGroupMax) {
             const int chi = min(ghi, clo + kGroupMax);

@@ -82,6 +82,10 @@ struct BuildBufs {
     NodeMom* mom;              // cap
     uint64_t* prefix;          // cap
     uint32_t* gstart;          // n + 1 (until the group pass writes it: the level pass's per-block deepest levels)
+    // Walk order (optional): the walk units sorted longest first by an estimate from per-body costs
+    // (wcost, the previous walk's list lengths carried with the bodies through the sort).
+    const float* wcost;
+    uint32_t* worder;          // n + 1
     // walk packs (pmax > 0): sibling groups packed up to pmax bodies
     int pmax;
     uint32_t* pflag;           // n + 1: pack starts per group, then their exclusive scan (count at [n])
@@ -178,6 +182,7 @@ struct WalkArgs {
     const NodeMom* mom;
     const uint32_t* gstart;
     const uint32_t* gcount;    // group count = gcount[n]
+    const uint32_t* order;     // optional: the order in which the waves take the groups (a permutation)
     int n;
     float eps2;
     float G;

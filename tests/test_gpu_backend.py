@@ -322,6 +322,33 @@ def test_body_costs_sum_to_interactions(hip):
         assert t["p2p"] + t["p2c"] == t["interactions"]
 
 
+@pytest.mark.parametrize("mode", ["plain", "mixed"])
+def test_walk_order_changes_no_result(mode):
+    """Each build orders the walk's groups longest first by the costs the previous walk left
+    (carried with the bodies through the sort); before any walk the cost array holds whatever the
+    allocation held. The order only schedules the waves: forces, potentials, counts and costs are
+    bit-identical under two different orders."""
+    be, why = L.make_hip_backend(0)
+    if be is None:
+        pytest.fail(f"HIP backend unavailable on a GPU test run: {why}")
+    try:
+        be.set_precision(mode)
+        b = O.plummer(50000, 21, 3 * math.pi / 16, 20.0)
+        p = P("tree", 0.01, 0.6)
+        be.upload(b)
+        runs = []
+        for _ in range(3):  # bodies stay sorted: the same tree each time, the order from new costs
+            be.compute_forces(p, True)
+            t = be.times()
+            acc = be.accelerations_f64() if mode == "mixed" else be.accelerations()
+            runs.append((acc, be.body_costs(), (t["interactions"], t["p2p"], t["p2c"])))
+        for acc, cost, counts in runs[1:]:
+            assert all(np.array_equal(x, y) for x, y in zip(acc, runs[0][0]))
+            assert np.array_equal(cost, runs[0][1]) and counts == runs[0][2]
+    finally:
+        be.close()
+
+
 def test_huge_theta_finite_and_momentum_sane(hip):
     b = O.plummer(5000, 6, 3 * math.pi / 16, 20.0)
     for theta in (5.0, 1e6, 1e30):
