@@ -592,6 +592,7 @@ struct Context {
         bb.gstart = gstart.as<uint32_t>();
         bb.wcost = cost.as<float>();
         bb.worder = worder.as<uint32_t>();
+        bb.order_thr = walk_order_threshold();
         bb.pmax = pack_max;
         bb.pflag = pflag.as<uint32_t>();
         bb.pstart = pstart.as<uint32_t>();
@@ -607,6 +608,17 @@ struct Context {
         bb.depth_hint = -1;
         return bb;
     }
+    // Walk units estimated above this share of one wave's work are taken first (longest first): a
+    // costly unit taken late would leave the other waves idle at the end. One wave's work is the
+    // previous walk's interactions (x 1.2 for the traversal term of the estimate) over the waves of
+    // the grid; with no previous walk every unit keeps its natural order.
+    static constexpr double kOrderShare = 0.125;
+    float walk_order_threshold() const {
+        if (t.interactions == 0) return FLT_MAX;
+        const double wave = 1.2 * double(t.interactions) / (double(walk_blocks) * kWalkBlockWaves);
+        return float(std::max(1.0, kOrderShare * wave));
+    }
+
     // The walk's units: the groups, or their packs. The count is at [n].
     const uint32_t* walk_starts() const { return pack_max > 0 ? pstart.as<uint32_t>() : gstart.as<uint32_t>(); }
     const uint32_t* walk_count() const { return pack_max > 0 ? pflag.as<uint32_t>() : gflag.as<uint32_t>(); }

@@ -635,21 +635,23 @@ __global__ void __launch_bounds__(kBS) packs_kernel(const uint32_t* __restrict__
     }
 }
 
-// Keys that order the walk units longest first. A unit's bodies share one interaction list, so its
+// Keys that take the costly walk units first. A unit's bodies share one interaction list, so its
 // work is estimated as (list length + 1) x (bodies + 8), the list length taken from the previous
 // walk's cost of its first body (costs travel with the bodies through the sort; before any walk
-// they may hold anything, which only changes the order). Key: 255 - 8 log2(estimate), 8 buckets per
-// octave, so one stable 8-bit radix pass sorts the units by descending estimate.
+// they may hold anything, which only changes the order). Units estimated at thr or more get
+// 255 - 8 log2(estimate) (8 buckets per octave), the others 255, so one stable 8-bit radix pass
+// puts the costly units first, longest first, and leaves the rest in their natural (Morton) order,
+// which keeps neighbouring groups, and the tree nodes they read, together in time.
 __global__ void __launch_bounds__(kBS) unit_order_keys(const uint32_t* __restrict__ ustart,
                                                        const uint32_t* __restrict__ ucount, int n,
-                                                       const float* __restrict__ cost, uint64_t* __restrict__ key,
-                                                       uint32_t* __restrict__ val) {
+                                                       const float* __restrict__ cost, float thr,
+                                                       uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
     const int g = blockIdx.x * kBS + threadIdx.x;
     if (g >= int(ucount[n])) return;
     const int lo = int(ustart[g]), hi = int(ustart[g + 1]);
     float est = (fmaxf(cost[lo], 0.f) + 1.f) * float(hi - lo + 8);
     est = est >= 1.f ? fminf(est, 1e30f) : 1.f;  // NaN and garbage included
-    const int bucket = min(255, int(8.f * log2f(est)));
+    const int bucket = est >= thr ? min(255, int(8.f * log2f(est))) : 0;
     key[g] = uint64_t(255 - bucket);
     val[g] = uint32_t(g);
 }
@@ -881,6 +883,13 @@ void fill_u32(hipStream_t s, uint32_t* a, uint32_t v, long long n) {
 inline unsigned leaf_grid(int n) { return std::min<unsigned>(grid_for(n, kBS), 2048u); }
 inline unsigned level_grid(int n) { return std::min<unsigned>(grid_for(n, kBS), 512u); }
 
+void unit_order(hipStream_t s, const BuildBufs& b, const uint32_t* ustart, const uint32_t* ucount, float thr,
+                uint32_t* order) {
+    unit_order_keys<<<grid_for(b.n, kBS), kBS, 0, s>>>(ustart, ucount, b.n, b.wcost, thr, b.pkey, b.pval);
+    LHIP_LAUNCH_CHECK();
+    radix_pass(s, b.pkey, b.pval, b.pkey2, order, b.n, ucount + b.n, 0, b.tile_hist);
+}
+
 void tree_moments_keep(hipStream_t s, const BuildBufs& b) {
     const bool ext = b.frames > 1;
     const int full = tree_depth(ext ? b.frames : 1);
@@ -948,11 +957,7 @@ int tree_build(hipStream_t s, const BuildBufs& b, float inv_theta, hipEvent_t le
     }
     if (b.worder && b.wcost) {
         // The walk order of this tree's units (queued before the host waits on the level pass).
-        const uint32_t* us = b.pmax > 0 ? b.pstart : b.gstart;
-        const uint32_t* uc = b.pmax > 0 ? b.pflag : b.gflag;
-        unit_order_keys<<<grid_for(n, kBS), kBS, 0, s>>>(us, uc, n, b.wcost, b.pkey, b.pval);
-        LHIP_LAUNCH_CHECK();
-        radix_pass(s, b.pkey, b.pval, b.pkey2, b.worder, n, uc + n, 0, b.tile_hist);
+        unit_order(s, b, b.pmax > 0 ? b.pstart : b.gstart, b.pmax > 0 ? b.pflag : b.gflag, b.order_thr, b.worder);
     }
     int depth = tree_depth(ext ? b.frames : 1);
     if (wait_levels) {

@@ -83,7 +83,8 @@ __global__ void __launch_bounds__(kBS4, 4) tree_vjp_kernel(TreeVjpArgs a) {
         if (lane == 0) g = int(atomicAdd(&a.counters->next_group, 1u));
         g = __builtin_amdgcn_readfirstlane(__shfl(g, 0, 64));
         if (g >= ngroups) break;
-        const int glo = int(a.gstart[g]), ghi = int(a.gstart[g + 1]);
+        const int gu = a.order ? int(a.order[g]) : g;  // the g-th group to take
+        const int glo = int(a.gstart[gu]), ghi = int(a.gstart[gu + 1]);
         const bool chunked = ghi - glo > kGroupMax;
         for (int clo = glo; clo < ghi; clo += kGroupMax) {
             const int chi = min(ghi, clo + kGroupMax);

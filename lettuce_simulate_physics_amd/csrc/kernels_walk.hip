@@ -59,7 +59,7 @@
 namespace lhip {
 namespace {
 
-constexpr int kWalkWaves = 4;
+constexpr int kWalkWaves = kWalkBlockWaves;
 constexpr int kWalkBS = 64 * kWalkWaves;
 constexpr int kCellCap = 64;    // cells staged per wave
 constexpr int kBodyCap = 128;   // bodies staged per wave (>= 128: also the share-reduction buffer)

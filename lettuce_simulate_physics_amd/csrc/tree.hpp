@@ -86,6 +86,7 @@ struct BuildBufs {
     // (wcost, the previous walk's list lengths carried with the bodies through the sort).
     const float* wcost;
     uint32_t* worder;          // n + 1
+    float order_thr;           // units estimated below this keep their natural (Morton) order
     // walk packs (pmax > 0): sibling groups packed up to pmax bodies
     int pmax;
     uint32_t* pflag;           // n + 1: pack starts per group, then their exclusive scan (count at [n])
@@ -170,6 +171,12 @@ void let_export(hipStream_t s, const LetExport& e);
 // rcrit2, qrel).
 void tree_moments_keep(hipStream_t s, const BuildBufs& b);
 
+// The order in which a walk takes the units of a tree (ustart, count at ucount[n]): units estimated
+// at thr or more first, longest first, then the rest in their natural order (thr = 1: all sorted).
+// Estimates from b.wcost; scratch b.pkey, b.pval, b.pkey2, b.tile_hist.
+void unit_order(hipStream_t s, const BuildBufs& b, const uint32_t* ustart, const uint32_t* ucount, float thr,
+                uint32_t* order);
+
 struct WalkArgs {
     const float4* posm;        // plain form: sorted (position, mass)
     // mixed form (posd != null): FP64 (position, mass) records, FP64 node moments, FP64 outputs
@@ -213,6 +220,7 @@ void tree_walk(hipStream_t s, const WalkArgs& a, bool potential);
 // Entries of a walk wave's traversal stack (LDS).
 constexpr int kWalkStack = 896;
 int walk_blocks_per_cu();
+constexpr int kWalkBlockWaves = 4;  // waves per walk block
 
 // Vector-Jacobian product of the mixed-precision walk's forces with frozen topology
 // (kernels_treeadj.hip). u = cotangent on the accelerations (already scaled by G and the kick
@@ -227,6 +235,7 @@ struct TreeVjpArgs {
     const double *ux, *uy, *uz;
     const uint32_t* gstart;
     const uint32_t* gcount;
+    const uint32_t* order;     // optional: the order in which the waves take the groups (a permutation)
     int n;
     float eps2;
     int small_leaf;

@@ -128,7 +128,8 @@ struct TreeSim {
     // Trees of a segment's recomputed steps (slot k = step start + k), reused by their VJPs instead
     // of a second build: node records (at most tr_cap per slot), level starts, node count, walk
     // units and their count; tr_flag[k] = 1 where a tree did not fit.
-    DevBuf tr_node, tr_mom, tr_momd, tr_lev, tr_cnt, tr_starts, tr_count, tr_flag;
+    DevBuf tr_node, tr_mom, tr_momd, tr_lev, tr_cnt, tr_starts, tr_count, tr_order, tr_flag;
+    DevBuf vjp_order;  // the VJP's group order on a tree built for it
     long long tr_cap = 0;
     int tr_slots = 0;
     std::vector<int> tr_frames;
@@ -423,7 +424,7 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
         c.sync();
         const long long cap = std::max<long long>(2ll * M0, 1024);
         const long long per = cap * (long long)(sizeof(WalkNode) + sizeof(NodeMom) + sizeof(NodeMomD)) +
-                              2ll * (n + 1) * 4 + kLev * 4 + 8;
+                              3ll * (n + 1) * 4 + kLev * 4 + 8;
         if (per * (K - 1) > kTreeStoreBytes) return;
         S.tr_cap = cap;
         S.tr_slots = K - 1;
@@ -435,6 +436,7 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
         S.tr_cnt.reserve(sl * 4);
         S.tr_starts.reserve(sl * (N + 1) * 4);
         S.tr_count.reserve(sl * (N + 1) * 4);
+        S.tr_order.reserve(sl * (N + 1) * 4);
         S.tr_flag.reserve(sl * 4);
         S.tr_frames.assign(sl, 1);
     };
@@ -458,6 +460,10 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
                                   hipMemcpyDeviceToDevice, s));
         LHIP_CHECK(hipMemcpyAsync(S.tr_count.as<uint32_t>() + std::size_t(j) * (N + 1), c.walk_count(), (N + 1) * 4,
                                   hipMemcpyDeviceToDevice, s));
+        // The VJP takes every group longest first (its waves gain more from balance than from the
+        // forward walk's Morton locality).
+        unit_order(s, c.build_bufs(), c.walk_starts(), c.walk_count(), 1.f,
+                   S.tr_order.as<uint32_t>() + std::size_t(j) * (N + 1));
         S.tr_frames[j] = c.frames;
     };
     std::vector<uint32_t> tr_ok;  // per slot of the current segment: 1 if its tree was stored whole
@@ -498,6 +504,11 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
         a.ux = u; a.uy = u + N; a.uz = u + 2 * N;
         a.gstart = stored ? S.tr_starts.as<uint32_t>() + std::size_t(j) * (N + 1) : c.walk_starts();
         a.gcount = stored ? S.tr_count.as<uint32_t>() + std::size_t(j) * (N + 1) : c.walk_count();
+        if (!stored) {
+            S.vjp_order.reserve((N + 1) * 4);
+            unit_order(s, c.build_bufs(), c.walk_starts(), c.walk_count(), 1.f, S.vjp_order.as<uint32_t>());
+        }
+        a.order = stored ? S.tr_order.as<uint32_t>() + std::size_t(j) * (N + 1) : S.vjp_order.as<uint32_t>();
         a.n = n;
         a.eps2 = float(S.p.eps * S.p.eps);
         a.small_leaf = c.wopt.small_leaf;
