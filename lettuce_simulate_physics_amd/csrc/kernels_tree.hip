@@ -62,6 +62,7 @@ __global__ void __launch_bounds__(kSortThreads) radix_count(const uint64_t* __re
     __shared__ uint32_t h[kRadix];
     if (n_dev) n = min(n, int(*n_dev));
     const int base = blockIdx.x * kSortTile;
+    if (base >= n) return;  // a tile past the elements (n_dev below the capacity): never scanned
     // All of the tile's loads are issued before the first use, so their latencies overlap.
     uint64_t kk[kSortKPT];
 #pragma unroll
@@ -80,11 +81,14 @@ __global__ void __launch_bounds__(kSortThreads) radix_count(const uint64_t* __re
     hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// One block per digit: exclusive scan of hist[d][0..ntiles) in place; total into tot[d].
+// One block per digit: exclusive scan of hist[d][0..tiles) in place, over the tiles that hold
+// elements (all ntiles, or those below n_dev's count); total into tot[d].
 __global__ void __launch_bounds__(kSortThreads) radix_row_scan(uint32_t* __restrict__ hist, int ntiles,
-                                                               uint32_t* __restrict__ tot) {
+                                                               uint32_t* __restrict__ tot, int n,
+                                                               const uint32_t* __restrict__ n_dev) {
     __shared__ uint32_t sh[kSortThreads];
     uint32_t* row = hist + blockIdx.x * ntiles;
+    if (n_dev) ntiles = min(ntiles, int((min(n, int(*n_dev)) + kSortTile - 1) / kSortTile));
     uint32_t carry = 0;
     for (int b = 0; b < ntiles; b += kSortThreads) {
         const int t = b + threadIdx.x;
@@ -121,6 +125,7 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter(const uint64_t* __
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int base = blockIdx.x * kSortTile;
     if (n_dev) n = min(n, int(*n_dev));
+    if (base >= n) return;  // nothing to place (the row scans skipped this tile)
     // All global loads of the block are issued here, before the first barrier, so that their
     // latencies overlap each other and the digit-base scan.
     uint64_t k[kSortKPT];
@@ -274,7 +279,7 @@ void radix_pass(hipStream_t s, const uint64_t* kin, const uint32_t* vin, uint64_
     uint32_t* tot = tile_hist + static_cast<std::size_t>(ntiles) * kRadix;
     radix_count<<<ntiles, kSortThreads, 0, s>>>(kin, n, n_dev, shift, ntiles, tile_hist);
     LHIP_LAUNCH_CHECK();
-    radix_row_scan<<<kRadix, kSortThreads, 0, s>>>(tile_hist, ntiles, tot);
+    radix_row_scan<<<kRadix, kSortThreads, 0, s>>>(tile_hist, ntiles, tot, n, n_dev);
     LHIP_LAUNCH_CHECK();
     radix_scatter<<<ntiles, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, n_dev, shift, ntiles, tile_hist, tot);
     LHIP_LAUNCH_CHECK();
