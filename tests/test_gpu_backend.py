@@ -455,6 +455,55 @@ def test_tree_key_restarts_bit_exact_with_oracle(hip, kind):
     assert rel_rms(a0, ad) < 1e-5 and t0["interactions"] == n * (n - 1)
 
 
+def test_c0_direct_16384_ten_kdk_steps_match_reference(hip):
+    """Config: 16384-body Plummer (seed 1, a = 1, cut 10), direct, eps 0.01, 10 KDK steps of dt 1e-3,
+    against the reference CPU backend's outputs (tests/golden/c0_direct.npz, written from the
+    reference build by tests/golden/make_golden.py): sampled accelerations, the exact potential
+    energy, the per-step diagnostics series and the sampled final state. Pattern of the reference's
+    tests/test_cuda_backend.cpp:290-343 (tolerance 1e-5), at this config's size and step count."""
+    g = np.load(os.path.join(GOLD, "c0_direct.npz"))
+    b = O.plummer(16384, 1, 1.0, 10.0)
+    p = P("direct", 0.01)
+    s = g["sample"]
+    a, _, t = got(hip, b, p, False)
+    assert t["interactions"] == 16384 * 16383
+    assert rel_rms(a[s], g["acc"]) <= 1e-5
+    assert np.max(per_body_rel(a[s], g["acc"])) <= 1e-5
+    a64, _ = O.direct_f64(b, 1.0, 0.01, s)
+    assert rel_rms(a[s], a64) <= 1e-5  # tests/test_cpu_backend.cpp:126
+    hip.upload(b)
+    assert abs(hip.exact_potential_energy(1.0, 0.01) / float(g["exact_pe"]) - 1) <= 1e-9
+    hip.compute_forces(p, True)
+    keys = ("mass", "ke", "pe", "px", "py", "pz", "lx", "ly", "lz", "cx", "cy", "cz")
+    rows = [hip.diagnostics()]
+    for _ in range(10):
+        hip.kick(5e-4)
+        hip.drift(1e-3)
+        hip.compute_forces(p, True)
+        hip.kick(5e-4)
+        rows.append(hip.diagnostics())
+    assert all(r["has_pe"] for r in rows)
+    got_series = np.array([[r[k] for k in keys] for r in rows])
+    ref = g["series"]
+    assert abs(float(ref[0, 2]) / float(g["pe"]) - 1) < 1e-12
+    e, e_ref = got_series[:, 1] + got_series[:, 2], ref[:, 1] + ref[:, 2]
+    print(f"c0 energy series: max |E/E_ref - 1| = {np.max(np.abs(e / e_ref - 1)):.3g}")
+    assert np.max(np.abs(e / e_ref - 1)) <= 1e-6
+    for col in (0, 1, 2):
+        assert np.max(np.abs(got_series[:, col] / ref[:, col] - 1)) <= 1e-5
+    r = np.sqrt(b.x.astype(np.float64) ** 2 + b.y ** 2 + b.z ** 2)
+    v = np.sqrt(b.vx.astype(np.float64) ** 2 + b.vy ** 2 + b.vz ** 2)
+    m = b.m.astype(np.float64)
+    for cols, scale in (((3, 4, 5), (m * v).sum()), ((6, 7, 8), (m * r * v).sum()), ((9, 10, 11), (m * r).sum())):
+        assert np.max(np.abs(got_series[:, cols] - ref[:, cols])) <= 1e-5 * scale
+    o = hip.download()
+    assert np.array_equal(np.sort(o.id), np.arange(16384, dtype=np.uint32))
+    at = np.argsort(o.id)[s]
+    fin = np.stack([getattr(o, f)[at] for f in ("x", "y", "z", "vx", "vy", "vz")], 1).astype(np.float64)
+    w = g["final_pos"].astype(np.float64)
+    assert np.all(np.abs(fin - w) <= 1e-5 * (np.abs(w) + 1e-2))
+
+
 def test_c1_million_parity_with_reference(hip):
     """Config: 2^20-body Plummer (a = 1, cut 10), theta 0.6, eps 0.01, leaf <= 16: keys, permutation
     and topology bit-exact with the reference; accelerations against the reference and FP64."""
