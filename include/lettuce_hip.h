@@ -353,6 +353,15 @@ int lhip_sim_forward(lhip_ctx* ctx, const lhip_sim_params* p, uint64_t n, const 
  * a forward. The tree method runs on the context's bodies (the forward replaces them), keeps its
  * checkpoints per body id, and gives no mass gradient (dL/dm must be NULL). */
 int lhip_sim_backward(lhip_ctx* ctx, const double* gxT, const double* gvT, double* gx0, double* gv0, double* gm);
+/* Generation of the forward whose adjoint lhip_sim_backward runs (0 before the first forward). Every
+ * lhip_sim_forward other than a tree replay starts a new generation: the checkpoints of the previous
+ * one are gone. A replay keeps the recorded forward's checkpoints, parameters and generation. */
+int lhip_sim_generation(const lhip_ctx* ctx, uint64_t* generation);
+/* lhip_sim_backward for the forward of the given generation: LHIP_LOGIC if another forward has run
+ * on this context since (a caller holding several forwards, e.g. an autograd graph, gets an error
+ * instead of another trajectory's gradients). generation 0 skips the check (lhip_sim_backward). */
+int lhip_sim_backward_gen(lhip_ctx* ctx, uint64_t generation, const double* gxT, const double* gvT, double* gx0,
+                          double* gv0, double* gm);
 /* Device time of the last forward and backward (HIP events on the context's stream). */
 int lhip_sim_times(lhip_ctx* ctx, double* forward_ms, double* backward_ms);
 

@@ -71,22 +71,37 @@ class Simulation:
         x0, v0, m = _f64(x0, (n, 3)), _f64(v0, (n, 3)), _f64(m, (n,))
         xT, vT = np.zeros((n, 3)), np.zeros((n, 3))
         c = self.p.c()
+        self.gen = None
         self.be._check(library().lhip_sim_forward(self.be._h, ctypes.byref(c), _U64(n), x0.ctypes.data_as(_P),
                                                   v0.ctypes.data_as(_P), m.ctypes.data_as(_P),
                                                   xT.ctypes.data_as(_P), vT.ctypes.data_as(_P)))
         self.n = n
+        if self.p.method != "tree" or self.p.tree_mode != "replay":
+            gen = ctypes.c_uint64()
+            self.be._check(library().lhip_sim_generation(self.be._h, ctypes.byref(gen)))
+            self.gen = gen.value  # the forward whose checkpoints backward() needs
         return xT, vT
 
     def backward(self, gxT, gvT):
-        """(dL/dx0, dL/dv0, dL/dm); dL/dm is None for the tree method (masses are not differentiated
-        through the tree)."""
-        n = getattr(self, "n", None) or int(np.asarray(gxT).reshape(-1, 3).shape[0])
+        """(dL/dx0, dL/dv0, dL/dm) of this object's last forward; dL/dm is None for the tree method
+        (masses are not differentiated through the tree). Raises LogicError when another forward has
+        run on the same backend since (its checkpoints are gone), and for a replay, which has no
+        adjoint of its own (the recording Simulation's backward is the frozen-topology adjoint)."""
+        return self.backward_of(getattr(self, "gen", None), gxT, gvT)
+
+    def backward_of(self, gen, gxT, gvT):
+        """backward() for the forward of generation `gen` (what forward() left in self.gen)."""
+        if gen is None:
+            raise LogicError("sim backward: no forward of this Simulation to differentiate "
+                             "(none ran, or it was a replay)")
+        n = int(np.asarray(gxT).reshape(-1, 3).shape[0])
         gxT, gvT = _f64(gxT, (n, 3)), _f64(gvT, (n, 3))
         gx0, gv0 = np.zeros((n, 3)), np.zeros((n, 3))
         gm = None if self.p.method == "tree" else np.zeros(n)
-        self.be._check(library().lhip_sim_backward(self.be._h, gxT.ctypes.data_as(_P), gvT.ctypes.data_as(_P),
-                                                   gx0.ctypes.data_as(_P), gv0.ctypes.data_as(_P),
-                                                   None if gm is None else gm.ctypes.data_as(_P)))
+        self.be._check(library().lhip_sim_backward_gen(self.be._h, _U64(gen), gxT.ctypes.data_as(_P),
+                                                       gvT.ctypes.data_as(_P), gx0.ctypes.data_as(_P),
+                                                       gv0.ctypes.data_as(_P),
+                                                       None if gm is None else gm.ctypes.data_as(_P)))
         return gx0, gv0, gm
 
     def times(self):
@@ -105,16 +120,18 @@ def leapfrog_function():
         @staticmethod
         def forward(ctx, x0, v0, m, sim: Simulation):
             xT, vT = sim.forward(x0.detach().cpu().numpy(), v0.detach().cpu().numpy(), m.detach().cpu().numpy())
-            ctx.sim = sim
+            # This graph's forward: a later forward on the same backend makes its backward raise
+            # rather than return another trajectory's gradients.
+            ctx.sim, ctx.gen, ctx.n = sim, sim.gen, sim.n
             ctx.dev = x0.device
             return (torch.from_numpy(xT).to(x0.device), torch.from_numpy(vT).to(x0.device))
 
         @staticmethod
         def backward(ctx, gxT, gvT):
-            n = ctx.sim.n
+            n = ctx.n
             gxT = np.zeros((n, 3)) if gxT is None else gxT.detach().cpu().numpy()
             gvT = np.zeros((n, 3)) if gvT is None else gvT.detach().cpu().numpy()
-            gx0, gv0, gm = ctx.sim.backward(gxT, gvT)
+            gx0, gv0, gm = ctx.sim.backward_of(ctx.gen, gxT, gvT)
             t = lambda a: None if a is None else torch.from_numpy(a).to(ctx.dev)  # noqa: E731
             return t(gx0), t(gv0), t(gm), None
 

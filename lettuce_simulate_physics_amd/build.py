@@ -20,7 +20,7 @@ SOURCES = ["kernels_state.hip", "kernels_direct.hip", "kernels_tree.hip", "kerne
            "treesim.hip", "capi.hip", "scenarios.cpp", "snapshot.cpp"]
 ARCH = os.environ.get("LHIP_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-FLAGS = ["-O3", "-std=c++20", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+FLAGS = ["-O3", "-std=c++23", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
          "-packed-fp32-ops", "-Wall", "-Wno-unused-command-line-argument", "-Wno-unused-function"]
 
 

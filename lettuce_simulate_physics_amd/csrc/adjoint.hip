@@ -127,13 +127,20 @@ int lhip_sim_forward(lhip_ctx* ctx, const lhip_sim_params* p, uint64_t n, const 
         if (p->steps < 0 || p->checkpoint_every < 1) throw Error(kInvalid, "sim: steps >= 0, checkpoint_every >= 1");
         LHIP_CHECK(hipSetDevice(c.dev));
         if (p->method == LHIP_METHOD_TREE) {
+            // A replay evaluates the recorded topology from other initial conditions and keeps the
+            // recorded forward's checkpoints: the adjoint on offer stays the recorded forward's.
+            if (p->tree_mode != LHIP_TREE_REPLAY) {
+                ++c.sim_gen;
+                c.last_sim_tree = true;
+            }
             treesim_forward(c, *p, int(n), x0, v0, m, xT, vT);
-            c.last_sim_tree = true;
             return;
         }
         if (p->method != LHIP_METHOD_DIRECT) throw Error(kInvalid, "sim: unknown method");
         if (!c.adj) c.adj = new AdjSim();
         AdjSim& A = *c.adj;
+        ++c.sim_gen;
+        c.last_sim_tree = false;
         A.n = int(n);
         A.p = *p;
         A.have_forward = false;
@@ -175,14 +182,23 @@ int lhip_sim_forward(lhip_ctx* ctx, const lhip_sim_params* p, uint64_t n, const 
         LHIP_CHECK(hipEventElapsedTime(&ms, A.ev[0], A.ev[1]));
         A.fwd_ms = ms;
         A.have_forward = true;
-        c.last_sim_tree = false;
     });
 }
 
-int lhip_sim_backward(lhip_ctx* ctx, const double* gxT, const double* gvT, double* gx0, double* gv0, double* gm) {
+int lhip_sim_generation(const lhip_ctx* ctx, uint64_t* generation) {
+    if (!ctx || !ctx->c || !generation) return LHIP_INVALID;
+    *generation = ctx->c->sim_gen;
+    return LHIP_OK;
+}
+
+int lhip_sim_backward_gen(lhip_ctx* ctx, uint64_t generation, const double* gxT, const double* gvT, double* gx0,
+                          double* gv0, double* gm) {
     if (!gxT || !gvT) return LHIP_INVALID;
     return lhip::guarded(ctx, [&](lhip::Context& c) {
         using namespace lhip;
+        if (generation != 0 && generation != c.sim_gen)
+            throw Error(kLogic, "sim backward: another forward has run on this context since (its checkpoints are "
+                                "gone); run the forward again");
         if (c.last_sim_tree) {
             if (gm) throw Error(kInvalid, "sim backward: the tree adjoint gives no mass gradient (pass NULL)");
             LHIP_CHECK(hipSetDevice(c.dev));
@@ -243,6 +259,10 @@ int lhip_sim_backward(lhip_ctx* ctx, const double* gxT, const double* gvT, doubl
         LHIP_CHECK(hipEventElapsedTime(&ms, A.ev[2], A.ev[3]));
         A.bwd_ms = ms;
     });
+}
+
+int lhip_sim_backward(lhip_ctx* ctx, const double* gxT, const double* gvT, double* gx0, double* gv0, double* gm) {
+    return lhip_sim_backward_gen(ctx, 0, gxT, gvT, gx0, gv0, gm);
 }
 
 int lhip_sim_times(lhip_ctx* ctx, double* forward_ms, double* backward_ms) {

@@ -28,6 +28,9 @@ struct Context {
     AdjSim* adj = nullptr;
     TreeSim* tsim = nullptr;
     bool last_sim_tree = false;
+    // Generation of the differentiable forward whose adjoint lhip_sim_backward runs: every forward
+    // other than a tree replay starts a new one, so an older caller's backward can be refused.
+    uint64_t sim_gen = 0;
     int dev = 0;
     hipStream_t stream = nullptr;
     std::string name, err;

@@ -316,19 +316,25 @@ void treesim_forward(Context& c, const lhip_sim_params& p, int n, const double* 
     const int mode = p.tree_mode;
     if (mode != LHIP_TREE_BUILD && mode != LHIP_TREE_RECORD && mode != LHIP_TREE_REPLAY)
         throw Error(kInvalid, "sim: unknown tree mode");
-    if (mode == LHIP_TREE_REPLAY && (!S.have_record || S.n != n || S.rec_steps != p.steps))
+    const bool replay = mode == LHIP_TREE_REPLAY;
+    if (replay && (!S.have_record || S.n != n || S.rec_steps != p.steps))
         throw Error(kLogic, "sim: replay needs a recorded forward of the same size and length");
+    // The topology a replay freezes is the recorded forward's: same opening angle and walk options.
+    if (replay && (p.theta != S.p.theta || p.walk.group_max != S.p.walk.group_max ||
+                   p.walk.small_leaf != S.p.walk.small_leaf || p.walk.quad_tau != S.p.walk.quad_tau))
+        throw Error(kLogic, "sim: a replay takes the recorded forward's theta and walk options");
     if (!(p.eps > 0)) throw Error(kInvalid, "sim: the tree method needs eps > 0");
     const lhip_walk_options* wo = p.walk.group_max > 0 ? &p.walk : nullptr;
     c.set_precision(LHIP_MODE_MIXED, wo);
     const int T = p.steps, K = p.checkpoint_every;
-    S.n = n;
-    S.p = p;
-    S.have_forward = false;
-    if (mode != LHIP_TREE_REPLAY) S.have_record = false;
-    S.nck = T > 0 ? (T + K - 1) / K : 1;
     const std::size_t N = std::size_t(n);
-    if (mode != LHIP_TREE_REPLAY) {
+    // A replay leaves the recorded forward's parameters, checkpoints and adjoint in place.
+    if (!replay) {
+        S.n = n;
+        S.p = p;
+        S.have_forward = false;
+        S.have_record = false;
+        S.nck = T > 0 ? (T + K - 1) / K : 1;
         S.ck_d.reserve(std::size_t(S.nck) * 9 * N * 8);
         S.ck_id.reserve(std::size_t(S.nck) * N * 4);
         S.ck_m.reserve(std::size_t(S.nck) * N * 4);
@@ -374,9 +380,7 @@ void treesim_forward(Context& c, const lhip_sim_params& p, int n, const double* 
         S.have_record = true;
         S.rec_steps = T;
     }
-    // A replay leaves the checkpoints of the recorded forward in place: the adjoint stays that of
-    // the recorded trajectory.
-    if (mode != LHIP_TREE_REPLAY) S.have_forward = true;
+    if (!replay) S.have_forward = true;
 }
 
 void treesim_backward(Context& c, const double* gxT, const double* gvT, double* gx0, double* gv0) {

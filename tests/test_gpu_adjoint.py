@@ -3,6 +3,8 @@ integrator, adjoint against the oracle's FP64 CPU adjoint and against central fi
 the all-FP64 GPU forward, checkpoint invariance, and the torch.autograd.Function harness."""
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -143,6 +145,37 @@ def test_adjoint_fd_on_64_coordinates_of_the_65536_body_run_shape(hip):
     assert max(worst) <= 1e-4
 
 
+def test_c3_adjoint_matches_oracle_fp64_adjoint_at_4096(hip):
+    # The adjoint benchmark's problem (Plummer seed 1, eps 0.05, 200 KDK steps of 1e-3, checkpoints
+    # every 20; target = the forward from positions perturbed by N(0, 0.01^2) and masses scaled by
+    # (1 + 0.01 N(0, 1)); loss = sum |x_T - x_target|^2) at N = 4096, against the oracle's FP64 CPU
+    # adjoint (every step kept, two unmerged half-kick VJPs per step from explicit Jacobian blocks).
+    # Tolerances: mixed 1e-5 relative (x, v) and 1e-4 (m); all-FP64 forces 1e-8 (x, v) and 1e-6 (m).
+    # The VJP reads each source's mass and cotangent as FP32 (its 48-byte source records) in both
+    # precisions, so the mass gradient, which comes from the VJP alone, agrees to ~2e-8, and x and v,
+    # where the force Jacobian is a small part of the gradient, to ~6e-10 (measured on an MI355X).
+    # The CPU adjoint takes 5-30 s on the host's cores.
+    n = 4096
+    b = O.plummer(n, 1)
+    x0 = np.stack([b.x, b.y, b.z], 1).astype(np.float64)
+    v0 = np.stack([b.vx, b.vy, b.vz], 1).astype(np.float64)
+    m = b.m.astype(np.float64)
+    rng = np.random.default_rng(2026)
+    xt, _ = O.sim_forward(x0 + 0.01 * rng.standard_normal(x0.shape), v0, m * (1 + 0.01 * rng.standard_normal(n)),
+                          1.0, 0.05, 1e-3, 200)
+    xo, vo = O.sim_forward(x0, v0, m, 1.0, 0.05, 1e-3, 200)
+    ox, ov, om = O.sim_backward(2 * (xo - xt), np.zeros_like(vo))
+    for prec, tol, tol_m in (("mixed", 1e-5, 1e-4), ("f64", 1e-8, 1e-6)):
+        sim = Simulation(hip, SimParams(eps=0.05, dt=1e-3, steps=200, checkpoint_every=20, precision=prec))
+        xT, vT = sim.forward(x0, v0, m)
+        assert rel(xT - x0, xo - x0) < tol, prec
+        # the same cotangent on both sides (the oracle's final state), so only the adjoints differ
+        gx, gv, gm = sim.backward(2 * (xo - xt), np.zeros_like(vT))
+        print(f"c3 at N = {n}, {prec}: adjoint vs FP64 CPU adjoint x {rel(gx, ox):.3g} v {rel(gv, ov):.3g} "
+              f"m {rel(gm, om):.3g}")
+        assert rel(gx, ox) < tol and rel(gv, ov) < tol and rel(gm, om) < tol_m, prec
+
+
 def test_checkpoint_interval_does_not_change_gradients(hip):
     x0, v0, m = plummer_state(1024, 6)
     out = []
@@ -162,8 +195,51 @@ def test_backward_before_forward_is_a_logic_error(hip):
     try:
         with pytest.raises(L.LogicError):
             Simulation(fresh, SimParams()).backward(np.zeros((4, 3)), np.zeros((4, 3)))
+        z = np.zeros((4, 3))
+        rc = L.library().lhip_sim_backward(fresh._h, z.ctypes.data_as(ctypes.c_void_p),
+                                           z.ctypes.data_as(ctypes.c_void_p), None, None, None)
+        assert rc == L.LOGIC
     finally:
         fresh.close()
+
+
+def test_backward_of_an_older_forward_is_a_logic_error(hip):
+    # Each forward replaces the context's checkpoints: the backward of an earlier forward (another
+    # Simulation, or an earlier graph of the autograd function) raises instead of returning the
+    # later trajectory's gradients, also across methods (a direct forward after a tree forward).
+    import lettuce_simulate_physics_amd as L
+
+    x0, v0, m = plummer_state(256, 8)
+    a = Simulation(hip, SimParams(eps=0.05, dt=1e-3, steps=4, checkpoint_every=2))
+    b = Simulation(hip, SimParams(eps=0.05, dt=1e-3, steps=6, checkpoint_every=3))
+    xa, _ = a.forward(x0, v0, m)
+    ga = a.backward(xa, np.zeros_like(xa))
+    xb, _ = b.forward(x0 + 0.01, v0, m)
+    with pytest.raises(L.LogicError):
+        a.backward(xa, np.zeros_like(xa))
+    gb = b.backward(xb, np.zeros_like(xb))
+    assert np.all(np.isfinite(gb[0]))
+    xa2, _ = a.forward(x0, v0, m)
+    assert np.array_equal(xa2, xa)
+    for u, w in zip(a.backward(xa, np.zeros_like(xa)), ga):
+        assert np.array_equal(u, w)
+    tree = Simulation(hip, SimParams(method="tree", eps=0.05, dt=1e-3, steps=4, checkpoint_every=2))
+    xt, _ = tree.forward(x0, v0, m)
+    a.forward(x0, v0, m)
+    with pytest.raises(L.LogicError):
+        tree.backward(xt, np.zeros_like(xt))
+    hip.set_precision("plain")
+
+    torch = pytest.importorskip("torch")
+    F = leapfrog_function()
+    X1, X2 = (torch.tensor(x, requires_grad=True) for x in (x0, x0 + 0.01))
+    V, M = torch.tensor(v0), torch.tensor(m)
+    y1, _ = F.apply(X1, V, M, a)
+    y2, _ = F.apply(X2, V, M, b)
+    with pytest.raises(RuntimeError, match="another forward"):
+        y1.sum().backward()
+    y2.sum().backward()
+    assert X2.grad is not None and torch.all(torch.isfinite(X2.grad))
 
 
 def test_torch_autograd_function(hip):

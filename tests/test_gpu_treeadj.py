@@ -147,30 +147,68 @@ def test_tree_adjoint_single_coordinates_match_frozen_forward_fd(hip, theta, wal
     assert worst < 2e-3
 
 
+BENCH_ENCOUNTER = dict(nstars=16384, seed=1, pericentre=4.0, incl1=np.pi / 6, incl2=np.pi / 3)
+BENCH_PRM = dict(theta=0.6, eps=0.02, dt=1e-3, steps=100, checkpoint_every=20)
+REF_WALK = dict(group_max=128, small_leaf=4, quad_tau=0.0)  # every quadrupole kept
+
+
+def encounter_gradient(hip, method, walk=None, tree_mode="build"):
+    """dL/d(pericentre, incl1, incl2) of the benchmark's encounter at 2 x 16384 stars."""
+    C = L.Collision(**BENCH_ENCOUNTER)
+    x0, v0, m = C.generate()
+    loss = com2_loss(C, m, np.random.default_rng(2027).standard_normal(3))
+    sim = Simulation(hip, SimParams(method=method, walk=walk, tree_mode=tree_mode, **BENCH_PRM))
+    xT, _ = sim.forward(x0, v0, m)
+    _, gxT = loss(xT)
+    gx0, gv0, _ = sim.backward(gxT, np.zeros_like(gxT))
+    return C, loss, C.vjp(gx0, gv0)
+
+
 def test_tree_adjoint_agrees_with_the_direct_sum_adjoint(hip):
     # The two-disk encounter at 2 x 16384 stars with the benchmark's settings (inclinations 30 and 60
-    # degrees, pericentre 4, theta 0.6, eps 0.02, dt 1e-3, 100 steps, checkpoints every 20) through
-    # the tree and through the direct sum: the parameter gradients agree to the tree's force accuracy.
-    C = L.Collision(nstars=16384, seed=1, pericentre=4.0, incl1=np.pi / 6, incl2=np.pi / 3)
-    x0, v0, m = C.generate()
-    target = np.random.default_rng(2027).standard_normal(3)
-    loss = com2_loss(C, m, target)
-    out = {}
-    for method in ("tree", "direct"):
-        sim = Simulation(hip, SimParams(method=method, theta=0.6, eps=0.02, dt=1e-3, steps=100,
-                                        checkpoint_every=20))
-        xT, _ = sim.forward(x0, v0, m)
-        _, gxT = loss(xT)
-        gx0, gv0, _ = sim.backward(gxT, np.zeros_like(gxT))
-        out[method] = C.vjp(gx0, gv0)
-    err = np.abs(out["tree"] - out["direct"]) / np.abs(out["direct"]).max()
-    print(f"tree vs direct adjoint: {out['tree']} vs {out['direct']}, error / largest component {err}")
-    # Components far below the largest (incl1 here, which moves only disk 1) carry the tree's small
-    # violation of momentum conservation inside disk 1, absent from the direct sum; the scale is the
-    # gradient's largest component.
-    assert np.all(err < 1e-3)
-    big = np.abs(out["direct"]) > 1e-3 * np.abs(out["direct"]).max()
-    assert np.all(np.abs(out["tree"] - out["direct"])[big] / np.abs(out["direct"])[big] < 1e-3)
+    # degrees, pericentre 4, theta 0.6, eps 0.02, dt 1e-3, 100 steps, checkpoints every 20) through the
+    # tree and through the direct sum, per component. With every quadrupole kept (the reference walk)
+    # the components agree to the tree's force accuracy: dL/d(incl1), which reaches disk 2 only through
+    # disk 1's quadrupole and higher moments, to 3% (measured 3.1%), the others to 1e-3. With the mixed
+    # form's error-budgeted far field (cells whose quadrupole term is below 5% of their monopole term
+    # evaluated as point masses) dL/d(incl1) is about a third smaller: that far field drops the
+    # quadrupoles that carry disk 1's orientation. The adjoint itself matches central differences of
+    # its own frozen-topology forward in both walks (next test), so the gap is the force model's.
+    _, _, direct = encounter_gradient(hip, "direct")
+    _, _, full = encounter_gradient(hip, "tree", REF_WALK)
+    _, _, budget = encounter_gradient(hip, "tree")
+    rel = lambda a: np.abs(a - direct) / np.abs(direct)  # noqa: E731
+    print(f"direct {direct}\ntree, every quadrupole {full} (relative {rel(full)})\n"
+          f"tree, mixed far field {budget} (relative {rel(budget)})")
+    assert rel(full)[0] < 1e-3 and rel(full)[2] < 1e-3 and rel(full)[1] < 0.05
+    assert rel(budget)[0] < 1e-3 and rel(budget)[2] < 1e-3
+
+
+@pytest.mark.parametrize("walk", [None, REF_WALK], ids=["mixed-far-field", "every-quadrupole"])
+def test_tree_adjoint_matches_frozen_forward_fd_at_benchmark_size(hip, walk):
+    # The same encounter: the tree adjoint's dL/d(pericentre, incl1, incl2) against central
+    # differences of the frozen-topology forward (replays of the recorded topology from the
+    # generator's perturbed initial conditions), Richardson-extrapolated from h = 1e-2 and 5e-3.
+    # Per-component relative tolerance: 1e-5 (pericentre; measured 1e-7), 1e-4 (incl2; 2e-6) and 2e-2
+    # (incl1, a gradient 1e5 times smaller than the pericentre's, where FP32 rounding in the forward
+    # leaves ~0.5% of noise in the differences; measured 0.4% and 0.08%).
+    C, loss, ad = encounter_gradient(hip, "tree", walk, tree_mode="record")
+    rep = Simulation(hip, SimParams(method="tree", walk=walk, tree_mode="replay", **BENCH_PRM))
+    fd = []
+    for name in ("pericentre", "incl1", "incl2"):
+        est = []
+        for h in (1e-2, 5e-3):
+            vals = []
+            for sgn in (1, -1):
+                xp, vp, mp = L.Collision(**{**BENCH_ENCOUNTER, name: getattr(C, name) + sgn * h}).generate()
+                xq, _ = rep.forward(xp, vp, mp)
+                vals.append(loss(xq)[0])
+            est.append((vals[0] - vals[1]) / (2 * h))
+        fd.append((4 * est[1] - est[0]) / 3)
+    fd = np.array(fd)
+    err = np.abs(ad - fd) / np.abs(fd)
+    print(f"adjoint {ad}\nfrozen-forward FD {fd}\nrelative {err}")
+    assert err[0] < 1e-5 and err[1] < 2e-2 and err[2] < 1e-4
 
 
 def test_tree_checkpoint_interval_does_not_change_gradients(hip):
@@ -185,6 +223,31 @@ def test_tree_checkpoint_interval_does_not_change_gradients(hip):
         for a, b in zip(g, out[0]):
             # source-side sums arrive by atomic adds: equal up to summation order
             assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(b))
+
+
+def test_backward_after_a_replay_is_the_recorded_forwards_adjoint(hip):
+    # A replay (other initial conditions, other step count per checkpoint) leaves the recorded
+    # forward's checkpoints and parameters in place: the recording's backward after it returns the
+    # same gradients as before it (up to the order of the atomic source-side sums); the replay
+    # itself has no adjoint of its own, and one with another theta or walk options is refused.
+    C = encounter(1500, seed=11)
+    x0, v0, m = C.generate()
+    rec = Simulation(hip, SimParams(**{**PRM, "steps": 8}, tree_mode="record"))
+    xT, vT = rec.forward(x0, v0, m)
+    g1 = rec.backward(xT, vT)[:2]
+    rep = Simulation(hip, SimParams(**{**PRM, "steps": 8, "checkpoint_every": 3, "dt": 4e-3}, tree_mode="replay"))
+    rep.forward(x0 * 1.001, v0, m)
+    with pytest.raises(L.LogicError):
+        rep.backward(xT, vT)
+    g2 = rec.backward(xT, vT)[:2]
+    for a, b in zip(g2, g1):
+        assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(b))
+    for bad in (dict(theta=0.5), dict(walk=dict(group_max=64, small_leaf=1, quad_tau=0.05))):
+        with pytest.raises(L.LogicError):
+            Simulation(hip, SimParams(**{**PRM, "steps": 8, **bad}, tree_mode="replay")).forward(x0, v0, m)
+    g3 = rec.backward(xT, vT)[:2]
+    for a, b in zip(g3, g1):
+        assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(b))
 
 
 def test_tree_backward_without_mass_gradient_and_replay_errors(hip):
