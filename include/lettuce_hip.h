@@ -321,6 +321,23 @@ int lhip_tree_nodes(lhip_ctx* ctx, int32_t* link, uint64_t* prefix, float* mom);
 /* Walk-group starts (groups + 1 entries; the last is n). */
 int lhip_tree_groups(lhip_ctx* ctx, uint32_t* gstart);
 
+/* ---- allocation-free direct sum (no counterpart in the reference) ----
+ *
+ * For callers that cannot allocate device memory (code under HIP stream capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the direct sum of lhip_compute_forces with method
+ * LHIP_METHOD_DIRECT (its kernels and split count, so the results are the same bit for bit) on
+ * caller-owned device arrays, enqueued on `stream` (a hipStream_t; NULL: the null stream) of the
+ * current device without synchronising. posm: n float4 records (x, y, z, m), 16-byte aligned;
+ * ax, ay, az (and pot when potential != 0): n floats each. a_i = G sum_j m_j d_ij / s^3 and
+ * phi_i = -G sum_j m_j / s with s^2 = d^2 + eps^2, the target skipped by index. The partial sums of
+ * the split source range live in a 32 MiB device array the library holds per device for the
+ * process's lifetime (2,097,152 float4; every n below 2^31 - 1 fits). Nothing is allocated or freed.
+ * Returns LHIP_INVALID for n >= 2^31 - 1, NULL or misaligned arrays; LHIP_RUNTIME for launch
+ * failures. Calls that overlap on one device share that array and race: serialise them (one
+ * stream, or events). */
+int lhip_direct_forces_noalloc(const float* posm, uint64_t n, double G, double eps, int potential, float* ax,
+                               float* ay, float* az, float* pot, void* stream);
+
 /* ---- differentiable simulation (no reference counterpart: the reference has no differentiation) ----
  *
  * FP64 state (positions, velocities, masses) integrated with the reference's KDK leapfrog

@@ -341,6 +341,21 @@ def hip_device_count() -> int:
         return 0
 
 
+def direct_forces_noalloc(posm: int, n: int, G: float, eps: float, potential: bool, ax: int, ay: int, az: int,
+                          pot: int | None = None, stream: int | None = None) -> None:
+    """lhip_direct_forces_noalloc: the context's direct sum on caller-owned device arrays (integer
+    device addresses: posm holds n float4 (x, y, z, m), ax/ay/az/pot n floats), enqueued on `stream`
+    (None: the null stream) without allocating or synchronising. Raises ValueError for arguments the
+    library refuses, RuntimeError for launch failures."""
+    rc = library().lhip_direct_forces_noalloc(_P(posm), _U64(n), _D(G), _D(eps), 1 if potential else 0, _P(ax),
+                                               _P(ay), _P(az), _P(pot) if pot else None, _P(stream) if stream else None)
+    if rc == INVALID:
+        raise ValueError("lhip_direct_forces_noalloc: n must be below 2^31 - 1, arrays non-null (pot with the "
+                         "potential), posm 16-byte aligned")
+    if rc != OK:
+        raise RuntimeError(f"lhip_direct_forces_noalloc failed with status {rc}")
+
+
 def make_hip_backend(device: int = 0):
     """Factory with the reference's contract: (backend, "") or (None, why). Never raises."""
     try:

@@ -17,7 +17,7 @@ OUT = os.path.join(HERE, "liblettuce_hip.so")
 BUILD = os.path.join(HERE, "_build")
 SOURCES = ["kernels_state.hip", "kernels_direct.hip", "kernels_tree.hip", "kernels_build.hip",
            "kernels_walk.hip", "kernels_let.hip", "kernels_adjoint.hip", "kernels_treeadj.hip", "adjoint.hip",
-           "treesim.hip", "capi.hip", "scenarios.cpp", "snapshot.cpp"]
+           "treesim.hip", "direct_noalloc.hip", "capi.hip", "scenarios.cpp", "snapshot.cpp"]
 ARCH = os.environ.get("LHIP_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++23", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
