@@ -423,6 +423,52 @@ def let_leg(b, reps: int = 5):
     return out
 
 
+# The reference's published CUDA numbers (docs/running.md:71-81; H200 SXM, lettuce-bench method=tree
+# n=1e6,1e7,1e8 steps=3, Hénon-unit Plummer a = 3 pi / 16, cut 20, seed 1, eps 0.01, theta 0.6,
+# dt 1e-3): force ms per step and Ginteractions/s.
+PUBLISHED_TREE = {1000000: (5.40, 472.0), 10000000: (50.9, 532.0), 100000000: (532.0, 538.0)}
+
+
+def henon_leg(sizes, steps: int = 3):
+    """The reference's published benchmark configuration, like for like: lettuce-bench's default
+    Plummer scenario (Hénon units: a = 3 pi / 16, cut 20, seed 1, masses 1/N), tree theta 0.6, eps
+    0.01, dt 1e-3; one warm-up force call, then `steps` timed KDK steps (src/bench.cpp:169-198).
+    Force ms and Ginteractions/s beside the published H200 figures; bodies resident on the device
+    before timing (the PCIe upload is reported apart)."""
+    import lettuce_simulate_physics_amd as L
+
+    be = L.HipBackend(0)  # its own context: the 1e8 buffers go with it
+    p = L.ForceParams(G=1.0, eps=EPS, theta=THETA, method="tree")
+    rows = {}
+    for n in sizes:
+        t0 = time.perf_counter()
+        b = L.plummer(n, 1, 3 * np.pi / 16, 20.0)
+        gen_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        be.upload(b)
+        upload_s = time.perf_counter() - t0
+        del b
+        be.compute_forces(p, False)
+        r = run_steps(be, p, steps, DT)
+        force_ms = r["force"] / steps
+        ginter = r["inter"] / (r["force"] * 1e-3) / 1e9
+        row = {"n": n, "steps": steps, "ms_per_step": round(r["wall"] * 1e3 / steps, 3), "force_ms": round(force_ms, 3),
+               "sort_ms": round(r["sort"] / steps, 3), "build_ms": round(r["build"] / steps, 3),
+               "walk_ms": round(r["walk"] / steps, 3), "ginteractions_per_s": round(ginter, 1),
+               "interactions_per_body": round(r["inter"] / steps / n, 1),
+               "upload_s": round(upload_s, 3), "generate_s": round(gen_s, 1)}
+        if n in PUBLISHED_TREE:
+            ms, gi = PUBLISHED_TREE[n]
+            row.update(published_h200_force_ms=ms, published_h200_ginteractions_per_s=gi,
+                       force_time_speedup_vs_h200=round(ms / force_ms, 3))
+        rows[str(n)] = row
+        print(f"henon n={n}: force {force_ms:.3f} ms, {ginter:.1f} Gint/s", file=sys.stderr, flush=True)
+    be.close()
+    return {"workload": "henon_plummer_tree", "plummer_a": round(3 * np.pi / 16, 6), "plummer_cut": 20.0, "seed": 1,
+            "theta": THETA, "eps": EPS, "dt": DT, "published": "docs/running.md:71-81 (H200 SXM, CUDA backend)",
+            "sizes": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -442,6 +488,8 @@ def main():
     ap.add_argument("--encounter-steps", type=int, default=100, help="0 skips the tree adjoint leg")
     ap.add_argument("--no-let", action="store_true", help="skip the two-rank LET leg")
     ap.add_argument("--no-direct", action="store_true", help="skip the direct-sum leg")
+    ap.add_argument("--henon-sizes", type=str, default="1e6,1e7,1e8",
+                    help="sizes of the like-for-like leg against the published numbers ('' skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy 3 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -526,6 +574,16 @@ def main():
         out["adjoint_forward_ratio"] = out["adjoint"]["adjoint_forward_ratio"]
         if args.encounter_steps > 0:
             out["tree_adjoint"] = tree_adjoint_leg(be, args.encounter_stars, args.encounter_steps)
+    if args.henon_sizes:
+        out["henon"] = henon_leg([int(float(v)) for v in args.henon_sizes.split(",")])
+        h1 = out["henon"]["sizes"].get("1000000")
+        if h1:
+            # value's metric on the published configuration (the Hénon-unit 1e6 sphere) over the
+            # published H200 figure for it
+            out["vs_baseline"] = round(h1["ginteractions_per_s"] / PUBLISHED_TREE[1000000][1], 3)
+            out["vs_baseline_basis"] = ("Ginteractions/s of the henon leg at n=1e6 (a=3pi/16, cut 20) over the "
+                                        "published H200 472 Gint/s (docs/running.md:78); force ms "
+                                        f"{h1['force_ms']} vs 5.40")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(out), flush=True)
