@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <cstdlib>
 #include <vector>
 
 #include "../../include/lettuce_hip.h"
@@ -23,6 +24,12 @@ struct AdjSim;  // differentiable FP64-state simulation (adjoint.hip)
 void adj_destroy(AdjSim* a);
 struct TreeSim;  // differentiable simulation through the tree (treesim.hip)
 void treesim_destroy(TreeSim* t);
+
+// The list pool's first size for n bodies (chunks of kListChunk entries): 1 chunk per 4 bodies.
+constexpr uint64_t kListChunksMax = uint64_t(INT32_MAX) / kListChunk;
+inline uint32_t list_chunks_initial(long long n) {
+    return uint32_t(std::min<uint64_t>(uint64_t(n) / 4 + 1024, kListChunksMax));
+}
 
 struct Context {
     AdjSim* adj = nullptr;
@@ -69,6 +76,16 @@ struct Context {
     // positions instead of the current ones (null otherwise).
     const double4* posd_box = nullptr;
 
+    // Interaction lists (the walk split into a list traversal and a list evaluation; measured slower
+    // than the fused walk on both forms, so off unless LETTUCE_HIP_WALK=lists asks for it): per mode
+    // whether the local walk uses them, the pool (list_cap chunks of kListChunk entries), chunk links
+    // and per-group heads, and the chunks the last walk took (the next pool's size).
+    bool lists_plain = false, lists_mixed = false;
+    DevBuf lpool, lnext, lhead;
+    uint32_t list_cap = 0, list_used = 0;
+    int list_blocks = 1;
+    bool lists_now() const { return mixed ? lists_mixed : lists_plain; }
+
     bool has_pe = false, acc_ok = false, tree_ok = false, costs_ok = false;
     bool kick_rec = false, drift_rec = false;
     lhip_phase_times t{};
@@ -93,6 +110,12 @@ struct Context {
         red_out.reserve(sizeof(double) * 16);
         counters.reserve(sizeof(WalkCounters));
         walk_blocks = cus * walk_blocks_per_cu();
+        list_blocks = cus * walk_lists_blocks_per_cu();
+        if (const char* w = std::getenv("LETTUCE_HIP_WALK")) {
+            const std::string v = w;
+            if (v == "lists") lists_plain = lists_mixed = true;
+            else if (v == "fused") lists_plain = lists_mixed = false;
+        }
         LHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_over), 2 * sizeof(int), hipHostMallocDefault));
         host_over[0] = host_over[1] = 0;
     }
@@ -649,6 +672,7 @@ struct Context {
         }
         a.counters = counters.as<WalkCounters>();
         a.blocks = walk_blocks;
+        a.list_blocks = list_blocks;
         a.small_leaf = wopt.small_leaf;
         a.quad_tau = float(wopt.quad_tau);
         a.depth = tree_depth(frames);
@@ -815,11 +839,26 @@ struct Context {
             direct_costs_pending = trees == 0;
         } else {
             WalkArgs a = walk_args(p);
+            const bool lists = trees == 0 && lists_now() && lists_fit(cap_nodes);
+            if (lists) list_args(a, list_reserve());
             if (trees > 0) remote_args(a, trees, false);  // one pass: local, then remote trees
             tree_walk(stream, a, potential);
             LHIP_CHECK(hipEventRecord(ev[4], stream));
             LHIP_CHECK(hipMemcpyAsync(&hc, counters.as<void>(), sizeof hc, hipMemcpyDeviceToHost, stream));
             sync();
+            if (lists && !hc.error && hc.list_chunks > a.list_cap) {
+                // The pool was too small (the first call, or lists that grew): the evaluation left
+                // everything alone; run the walk again with a pool of the size the traversal asked for.
+                list_used = hc.list_chunks;
+                list_args(a, list_reserve());
+                tree_walk(stream, a, potential);
+                LHIP_CHECK(hipEventRecord(ev[4], stream));
+                LHIP_CHECK(hipMemcpyAsync(&hc, counters.as<void>(), sizeof hc, hipMemcpyDeviceToHost, stream));
+                sync();
+                if (!hc.error && hc.list_chunks > a.list_cap)
+                    throw Error(kRuntime, "hip tree walk: interaction lists outgrew their pool");
+            }
+            if (lists) list_used = hc.list_chunks;
             direct_costs_pending = false;
         }
         if (hc.error)
@@ -841,6 +880,24 @@ struct Context {
 
     bool direct_costs_pending = false;
     int overfull = 0;
+
+    // Chunks of the list pool: 1 per 4 bodies to begin with (list_chunks_initial), and a quarter
+    // more than a walk asked for whenever one outgrew the pool.
+    uint32_t list_reserve() {
+        uint32_t c = std::max(list_cap, list_chunks_initial(n));
+        if (list_used > c) c = uint32_t(std::min<uint64_t>(uint64_t(list_used) + list_used / 4 + 64, kListChunksMax));
+        lpool.reserve(std::size_t(c) * kListChunk * 4);
+        lnext.reserve(std::size_t(c) * 4);
+        lhead.reserve((std::size_t(n) + 1) * 4);
+        list_cap = c;
+        return c;
+    }
+    void list_args(WalkArgs& a, uint32_t cap) {
+        a.lpool = lpool.as<uint32_t>();
+        a.lnext = lnext.as<uint32_t>();
+        a.lhead = lhead.as<uint32_t>();
+        a.list_cap = cap;
+    }
 
     void compute_forces(const lhip_force_params& p, bool potential) {
         build_tree(p);
@@ -1079,7 +1136,8 @@ struct Context {
               &prefix, &gstart, &worder, &counters, &pflag, &pstart, &fkey, &fseg, &capb, &fcube, &rs_seg, &rs_bbox,
               &rs_cstart, &rs_gkey, &rs_ka, &rs_kb, &rs_va, &rs_vb, &xd, &yd, &zd, &vxd, &vyd, &vzd, &xd2, &yd2, &zd2,
               &vxd2, &vyd2, &vzd2, &axd, &ayd, &azd, &potd, &momd, &rec64, &posd, &chk_tgt, &chk_out, &chk_part,
-              &let_rec, &let_body, &let_root, &let_state, &let_ex, &let_bcnt, &let_tmp, &let_out, &let_bods})
+              &let_rec, &let_body, &let_root, &let_state, &let_ex, &let_bcnt, &let_tmp, &let_out, &let_bods, &lpool, &lnext,
+              &lhead})
             held += b->capacity();
         return held;
     }
@@ -1098,7 +1156,8 @@ struct Context {
 // Device bytes a force call on n bodies reserves (Backend::memory_need, src/backend.hpp:137): the
 // same sizes the reserve calls use, buffer by buffer, each at least 256 bytes, plus 64 MiB for
 // allocator rounding. Not included: remote LETs and their exports, whose sizes depend on the other
-// ranks, and the differentiable simulations. Host: nothing per body beyond the caller's arrays.
+// ranks, the differentiable simulations, and the interaction-list pool of LETTUCE_HIP_WALK=lists
+// (about 1 KiB per 4 bodies). Host: nothing per body beyond the caller's arrays.
 inline void memory_need(uint64_t n, int method, bool mixed, uint64_t* dev, uint64_t* host) {
     const uint64_t N = std::min<uint64_t>(n, INT32_MAX - 1);
     const int ni = int(N);

@@ -1,5 +1,6 @@
-// Barnes-Hut tree on the device: Morton keys, stable LSD radix sort, permute, octree enumeration in
-// pre-order, bottom-up multipoles, walk groups, and the wavefront-parallel stackless walk.
+// Barnes-Hut tree on the device, first stage: Morton keys, the stable LSD radix sort and the permute.
+// The octree itself (pre-order enumeration, level order, multipoles, walk groups) is built in
+// kernels_build.hip and walked in kernels_walk.hip; the overview below covers all three.
 //
 // Reference algorithm: CpuBackend::sort_bodies / make_keys / radix_sort / permute / build_node /
 // build_octree / build_list / eval_group (src/cpu_backend.cpp:395-882). The device form differs in

@@ -158,24 +158,205 @@ __device__ __forceinline__ void eval_bodies(Slot (&sl)[NX], int S, int nb, const
 constexpr int kStackCap = kWalkStack;
 // Stack budget: a step popping P nodes pushes at most 8P, so P is held to (thr - top) / 7 with thr
 // = capacity - 7 (depth + 1) - 8; at or above thr one node per step adds at most 7 per tree level.
-__device__ __forceinline__ int stack_threshold(int depth) { return kStackCap - 7 * (max(depth, kMaxLevel) + 1) - 8; }
+__device__ __forceinline__ int stack_threshold(int depth, int cap = kStackCap) {
+    return cap - 7 * (max(depth, kMaxLevel) + 1) - 8;
+}
+
+// What a local node contributes to the list of the bodies [clo, chi) with box centre b and half
+// extent h (reference build_list, src/cpu_backend.cpp:816-849): a quadrupole cell, a monopole (the
+// error-budgeted far field), the bodies of a leaf, or its children (opened internal node). A node
+// holding bodies of this (sub)group is always opened: the reference's "contains" test for whole
+// groups; any overlap for chunks of an over-full group. The distance is rounded as the reference
+// build evaluates it: three products, then two sums. Both the list traversal and the fused walk
+// decide through this one function, so their lists are the same.
+struct Pick {
+    bool cell, mono, range, open;
+};
+__device__ __forceinline__ Pick decide(bool act, const WalkNode& nd, float qr, float bx, float by, float bz, float hx,
+                                       float hy, float hz, int clo, int chi, bool chunked, int small_leaf,
+                                       float quad_tau) {
+    const bool keep_open = chunked ? (nd.lo < chi && clo < nd.hi) : (nd.lo <= clo && chi <= nd.hi);
+    const float dx = fmaxf(fabsf(nd.cx - bx) - hx, 0.f);
+    const float dy = fmaxf(fabsf(nd.cy - by) - hy, 0.f);
+    const float dz = fmaxf(fabsf(nd.cz - bz) - hz, 0.f);
+    const float d2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+    const bool acc = act && !keep_open && d2 >= nd.rcrit2;
+    const bool leaf = act && info_leaf(nd.info);
+    const bool small = leaf && (nd.hi - nd.lo) <= small_leaf;
+    const bool cell = acc && !small;
+    Pick r;
+    r.mono = cell && quad_tau > 0.f && qr < quad_tau * d2;
+    r.cell = cell && !r.mono;
+    r.range = leaf && (!acc || small);
+    r.open = act && !acc && !leaf;
+    return r;
+}
+
+// ---------------------------------------------------------------- interaction lists
+//
+// walk_lists_kernel traverses like walk_kernel (same stack discipline, same decisions) but only
+// writes each group's list: accepted cells, monopoles and opened leaves as tagged node indices. The
+// walk then evaluates the lists. Split this way, the traversal runs at twice the waves per SIMD of
+// the evaluation (it holds no list buffers and few registers), so its dependent node loads
+// overlap each other, and the evaluation streams its lists with no traversal latency between the
+// pair loops.
+
+constexpr int kListWaves = 4;
+constexpr int kListBS = 64 * kListWaves;
+constexpr int kListStack = kStackCap;  // traversal stack entries per wave
+
+// Pool chunks a wave takes from the shared counter at a time (one atomic per slab, not per chunk:
+// every wave of the chip adds to that one counter).
+constexpr uint32_t kListSlab = 8;
+
+// Appends the entries of the lanes with `has` to the unit list being written (chunk, fill), taking
+// a new pool chunk whenever the current one fills up, from the wave's slab [slab, slab_end);
+// writes beyond the pool are dropped (the chunk counter then exceeds the pool's and the host runs
+// the walk again with a larger pool).
+struct ListWriter {
+    uint32_t chunk, fill, slab, slab_end;
+    __device__ __forceinline__ uint32_t take(const WalkArgs& a, int lane) {
+        if (slab == slab_end) {
+            uint32_t c = 0;
+            if (lane == 0) c = atomicAdd(&a.counters->list_chunks, kListSlab);
+            slab = uint32_t(__builtin_amdgcn_readfirstlane(__shfl(int(c), 0, 64)));
+            slab_end = slab + kListSlab;
+        }
+        return slab++;
+    }
+    __device__ __forceinline__ void append(const WalkArgs& a, int lane, bool has, uint32_t ent) {
+        const unsigned long long m = __ballot(has);
+        const uint32_t cnt = uint32_t(__popcll(m));
+        if (!cnt) return;
+        uint32_t next = chunk;
+        if (fill + cnt >= uint32_t(kListChunk)) {
+            next = take(a, lane);
+            if (lane == 0 && chunk < a.list_cap && next < a.list_cap) a.lnext[chunk] = next;
+        }
+        if (has) {
+            const uint32_t slot = fill + lanes_below(m);
+            const uint32_t c = slot < uint32_t(kListChunk) ? chunk : next;
+            if (c < a.list_cap) a.lpool[std::size_t(c) * kListChunk + (slot & (kListChunk - 1))] = ent;
+        }
+        fill += cnt;
+        if (fill >= uint32_t(kListChunk)) {
+            fill -= kListChunk;
+            chunk = next;
+        }
+    }
+};
+
+template <bool REL>
+__global__ void __launch_bounds__(kListBS, 8) walk_lists_kernel(WalkArgs a) {
+    __shared__ int stk[kListWaves][kListStack];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int* st = stk[w];
+    const int ngroups = int(a.gcount[a.n]);
+    const int stack_thr = stack_threshold(a.depth, kListStack);
+    const bool need_qrel = a.quad_tau > 0.f;
+    bool overflow = false;
+    ListWriter wr;
+    wr.slab = wr.slab_end = 0;
+    while (!overflow) {
+        int g = 0;
+        if (lane == 0) g = int(atomicAdd(&a.counters->next_list, 1u));
+        g = __builtin_amdgcn_readfirstlane(__shfl(g, 0, 64));
+        if (g >= ngroups) break;
+        const int gu = a.order ? int(a.order[g]) : g;
+        const int glo = int(a.gstart[gu]), ghi = int(a.gstart[gu + 1]);
+        const bool chunked = ghi - glo > kGroupMax;
+        {
+            wr.chunk = wr.take(a, lane);
+            wr.fill = 0;
+            if (lane == 0 && wr.chunk < a.list_cap) a.lhead[gu] = wr.chunk;
+        }
+        for (int clo = glo; clo < ghi && !overflow; clo += kGroupMax) {
+            const int chi = min(ghi, clo + kGroupMax);
+            const int nbody = chi - clo;
+            // The group box of walk_kernel: float min/max over the chunk's bodies.
+            float mnx = FLT_MAX, mny = FLT_MAX, mnz = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX, mxz = -FLT_MAX;
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int v = t * 64 + lane;
+                if (v < nbody) {
+                    float px, py, pz;
+                    if (REL) {
+                        const double4 d = a.posd_box[clo + v];
+                        px = float(d.x); py = float(d.y); pz = float(d.z);
+                    } else {
+                        const float4 q = a.posm[clo + v];
+                        px = q.x; py = q.y; pz = q.z;
+                    }
+                    mnx = fminf(mnx, px); mny = fminf(mny, py); mnz = fminf(mnz, pz);
+                    mxx = fmaxf(mxx, px); mxy = fmaxf(mxy, py); mxz = fmaxf(mxz, pz);
+                }
+            }
+            mnx = wave_min(mnx); mny = wave_min(mny); mnz = wave_min(mnz);
+            mxx = wave_max(mxx); mxy = wave_max(mxy); mxz = wave_max(mxz);
+            const float bx = 0.5f * (mnx + mxx), by = 0.5f * (mny + mxy), bz = 0.5f * (mnz + mxz);
+            const float hx = 0.5f * (mxx - mnx), hy = 0.5f * (mxy - mny), hz = 0.5f * (mxz - mnz);
+            // The unit's count entry, filled in at its end.
+            const uint32_t hchunk = wr.chunk, hslot = wr.fill;
+            wr.append(a, lane, lane == 0, 0u);
+            uint32_t count = 0;
+            if (lane == 0) st[0] = 0;  // the root
+            int top = 1;
+            while (top > 0) {
+                const int room = (stack_thr - top) / 7;
+                const int P = min(64, min(top, max(1, room)));
+                __builtin_amdgcn_wave_barrier();
+                const bool act = lane < P;
+                const int k = act ? st[top - P + lane] : 0;
+                __builtin_amdgcn_wave_barrier();
+                top -= P;
+                WalkNode nd{0.f, 0.f, 0.f, 0.f, 0, 0, 0, -1};
+                float qr = 0.f;
+                if (act) {
+                    nd = a.node[k];
+                    if (need_qrel) qr = a.mom[k].qrel;
+                }
+                const Pick pk = decide(act, nd, qr, bx, by, bz, hx, hy, hz, clo, chi, chunked, a.small_leaf,
+                                       a.quad_tau);
+                const int nch = pk.open ? info_nchild(nd.info) : 0;
+                const int cend = prefix_sum_incl(nch, lane);
+                const int pushed = __builtin_amdgcn_readlane(cend, 63);
+                if (top + pushed > kListStack) {
+                    overflow = true;
+                    break;
+                }
+                for (int c = 0; c < nch; ++c) st[top + cend - nch + c] = nd.first + c;
+                top += pushed;
+                const bool has = pk.cell || pk.mono || pk.range;
+                const uint32_t tag = pk.cell ? kEntCell : (pk.mono ? kEntMono : kEntRange);
+                count += uint32_t(__popcll(__ballot(has)));
+                wr.append(a, lane, has, tag | uint32_t(k));
+            }
+            if (lane == 0 && hchunk < a.list_cap) a.lpool[std::size_t(hchunk) * kListChunk + hslot] = count;
+        }
+    }
+    if (overflow && lane == 0) atomicOr(&a.counters->error, 1);
+}
 
 // LET: every group also walks every remote tree (a.let) after the local one, into the same lists
 // (the reference's order: src/cpu_backend.cpp:667-689). With direct_blocks the groups are blocks of
 // kGroupMax consecutive bodies that walk the remote trees only, and the forces, costs and counts are
 // added to the direct sum's.
-template <bool POT, bool GUARD, bool REL, bool LET>
+template <bool POT, bool GUARD, bool REL, bool LET, bool LISTS>
 __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
+    static_assert(!(LISTS && LET), "remote trees are walked, not listed");
     constexpr bool MASK = POT || GUARD;
     // Slots per lane at most: 3 (4 would spill at 4 waves per SIMD, the occupancy the bounds ask
     // for); 2 with the potential or the self test, whose extra registers would spill a third.
     constexpr int NX = MASK ? 2 : 3;
     static_assert(sizeof(WaveLds) >= 64 * NX * (REL ? sizeof(D4) : sizeof(float4)), "share reduction buffer");
     __shared__ WaveLds lds[kWalkWaves];
-    __shared__ int stk[kWalkWaves][kStackCap];
+    __shared__ int stk[kWalkWaves][LISTS ? 1 : kStackCap];  // the lists need no traversal stack
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     WaveLds& L = lds[w];
     int* st = stk[w];
+    // A list traversal that overflowed its stack or the pool left no usable lists: the host raises,
+    // or runs the walk again with a larger pool.
+    if (LISTS && (a.counters->error != 0 || a.counters->list_chunks > a.list_cap)) return;
     const bool blocks = LET && a.direct_blocks;
     const int ngroups = blocks ? (a.n + kGroupMax - 1) / kGroupMax : int(a.gcount[a.n]);
     const float e2 = a.eps2;
@@ -193,6 +374,9 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
         const int glo = blocks ? g * kGroupMax : int(a.gstart[gu]);
         const int ghi = blocks ? min(a.n, glo + kGroupMax) : int(a.gstart[gu + 1]);
         const bool chunked = ghi - glo > kGroupMax;
+        // List reader: chunk and offset in the pool (indices clamped into the pool).
+        uint32_t lc = 0, loff = 0;
+        if (LISTS) lc = min(a.lhead[gu], a.list_cap - 1u);
         for (int clo = glo; clo < ghi; clo += kGroupMax) {
             const int chi = min(ghi, clo + kGroupMax);
             const int nbody = chi - clo;  // 1..128, wave-uniform
@@ -354,7 +538,82 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                 }
             };
 
-            if (!blocks) {
+            if constexpr (LISTS) {
+                // This unit's list, 64 entries at a time, staged as the traversal would stage them.
+                const uint32_t last = a.list_cap - 1u;
+                const uint32_t units = uint32_t(__builtin_amdgcn_readfirstlane(
+                    int(a.lpool[std::size_t(lc) * kListChunk + loff])));
+                if (++loff == uint32_t(kListChunk)) {
+                    lc = min(a.lnext[lc], last);
+                    loff = 0;
+                }
+                for (uint32_t done = 0; done < units; done += 64) {
+                    const uint32_t take = min(64u, units - done);
+                    const bool act = uint32_t(lane) < take;
+                    uint32_t nc = lc;
+                    if (loff + take >= uint32_t(kListChunk)) nc = min(a.lnext[lc], last);
+                    uint32_t e = 0;
+                    if (act) {
+                        const uint32_t o = loff + uint32_t(lane);
+                        e = o < uint32_t(kListChunk) ? a.lpool[std::size_t(lc) * kListChunk + o]
+                                                     : a.lpool[std::size_t(nc) * kListChunk + (o - kListChunk)];
+                    }
+                    loff += take;
+                    if (loff >= uint32_t(kListChunk)) {
+                        loff -= kListChunk;
+                        lc = nc;
+                    }
+                    const uint32_t tag = e & ~kEntIndex;
+                    const int k = int(e & kEntIndex);
+                    const bool is_cell = act && tag == kEntCell, is_mono = act && tag == kEntMono;
+                    const bool is_range = act && tag == kEntRange;
+                    double4 pc = make_double4(0.0, 0.0, 0.0, 0.0);
+                    float4 cA = make_float4(0.f, 0.f, 0.f, 0.f), cB = cA;
+                    float2 cC = make_float2(0.f, 0.f);
+                    int rlo = 0, rhi = 0;
+                    if (is_cell || is_mono) {
+                        const float4* r = reinterpret_cast<const float4*>(a.mom + k);
+                        if (REL) {
+                            const NodeMomD& dm = a.momd[k];
+                            pc = make_double4(dm.cx, dm.cy, dm.cz, dm.m);
+                        } else {
+                            cA = r[0];
+                        }
+                        if (is_cell) {
+                            cB = r[1];
+                            const float4 r2 = r[2];
+                            cC = make_float2(r2.x, r2.y);
+                        }
+                    }
+                    if (is_range) {
+                        const int2 lh = *reinterpret_cast<const int2*>(&a.node[k].lo);
+                        rlo = lh.x;
+                        rhi = lh.y;
+                    }
+                    stage_cells(is_cell, [&]() {
+                        CellL c;
+                        c.A = REL ? make_float4(float(pc.x - ccx), float(pc.y - ccy), float(pc.z - ccz), float(pc.w))
+                                  : cA;
+                        c.B = cB;
+                        c.C = cC;
+                        return c;
+                    });
+                    stage_monos(is_mono, [&]() {
+                        return REL ? make_float4(float(pc.x - ccx), float(pc.y - ccy), float(pc.z - ccz), float(pc.w))
+                                   : cA;
+                    });
+                    stage_ranges(
+                        is_range, rlo, rhi - rlo,
+                        [&](int j) {
+                            if (REL) {
+                                const double4 d = a.posd[j];
+                                return make_float4(float(d.x - ccx), float(d.y - ccy), float(d.z - ccz), float(d.w));
+                            }
+                            return a.posm[j];
+                        },
+                        [](int j) { return j; });
+                }
+            } else if (!blocks) {
                 if (lane == 0) st[0] = 0;  // the root
                 int top = 1;
                 while (top > 0) {
@@ -389,25 +648,12 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                             qr = r2.w;
                         }
                     }
-                    // A node holding bodies of this (sub)group is always opened: the reference's
-                    // "contains" test for whole groups; any overlap for chunks of an over-full group.
-                    const bool keep_open = chunked ? (nd.lo < chi && clo < nd.hi) : (nd.lo <= clo && chi <= nd.hi);
-                    const float dx = fmaxf(fabsf(nd.cx - bx) - hx, 0.f);
-                    const float dy = fmaxf(fabsf(nd.cy - by) - hy, 0.f);
-                    const float dz = fmaxf(fabsf(nd.cz - bz) - hz, 0.f);
-                    // Rounded as the reference build evaluates it: three products, then two sums.
-                    const float d2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
-                    const bool acc = act && !keep_open && d2 >= nd.rcrit2;
-                    const bool leaf = act && info_leaf(nd.info);
-                    const bool small = leaf && (nd.hi - nd.lo) <= a.small_leaf;
-                    bool is_cell = acc && !small;
                     // Error-budgeted far field: an accepted cell whose quadrupole term is below
                     // quad_tau of its monopole at the group box's distance enters the body list as a
                     // point mass.
-                    const bool is_mono = is_cell && a.quad_tau > 0.f && qr < a.quad_tau * d2;
-                    is_cell = is_cell && !is_mono;
-                    const bool is_range = leaf && (!acc || small);
-                    const bool open_int = act && !acc && !leaf;
+                    const Pick pk = decide(act, nd, qr, bx, by, bz, hx, hy, hz, clo, chi, chunked, a.small_leaf,
+                                           a.quad_tau);
+                    const bool is_cell = pk.cell, is_mono = pk.mono, is_range = pk.range, open_int = pk.open;
 
                     // push the child blocks of opened internal nodes
                     const int nch = open_int ? info_nchild(nd.info) : 0;
@@ -616,6 +862,8 @@ __global__ void reset_walk(WalkCounters* __restrict__ c) {
     c->interactions = c->p2p = c->p2c = c->listed = c->p2m = 0;
     c->error = 0;
     c->next_group = 0;
+    c->next_list = 0;
+    c->list_chunks = 0;
 }
 
 
@@ -624,7 +872,7 @@ __global__ void reset_walk(WalkCounters* __restrict__ c) {
 int walk_blocks_per_cu() {
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks,
-                                                     reinterpret_cast<const void*>(walk_kernel<false, false, false, false>),
+                                                     reinterpret_cast<const void*>(walk_kernel<false, false, false, false, false>),
                                                      kWalkBS, 0) != hipSuccess ||
         blocks < 1) {
         (void)hipGetLastError();
@@ -633,14 +881,14 @@ int walk_blocks_per_cu() {
     return blocks;
 }
 
-template <bool REL, bool LET>
+template <bool REL, bool LET, bool LISTS>
 void walk_launch(hipStream_t s, const WalkArgs& a, bool potential, bool guard, int blocks) {
     if (potential) {
-        if (guard) walk_kernel<true, true, REL, LET><<<blocks, kWalkBS, 0, s>>>(a);
-        else walk_kernel<true, false, REL, LET><<<blocks, kWalkBS, 0, s>>>(a);
+        if (guard) walk_kernel<true, true, REL, LET, LISTS><<<blocks, kWalkBS, 0, s>>>(a);
+        else walk_kernel<true, false, REL, LET, LISTS><<<blocks, kWalkBS, 0, s>>>(a);
     } else {
-        if (guard) walk_kernel<false, true, REL, LET><<<blocks, kWalkBS, 0, s>>>(a);
-        else walk_kernel<false, false, REL, LET><<<blocks, kWalkBS, 0, s>>>(a);
+        if (guard) walk_kernel<false, true, REL, LET, LISTS><<<blocks, kWalkBS, 0, s>>>(a);
+        else walk_kernel<false, false, REL, LET, LISTS><<<blocks, kWalkBS, 0, s>>>(a);
     }
 }
 
@@ -651,11 +899,32 @@ void tree_walk(hipStream_t s, const WalkArgs& a, bool potential) {
     reset_walk<<<1, 1, 0, s>>>(a.counters);
     LHIP_LAUNCH_CHECK();
     const bool let = a.let_trees > 0;
-    if (a.posd) let ? walk_launch<true, true>(s, a, potential, guard, blocks)
-                    : walk_launch<true, false>(s, a, potential, guard, blocks);
-    else let ? walk_launch<false, true>(s, a, potential, guard, blocks)
-             : walk_launch<false, false>(s, a, potential, guard, blocks);
+    if (a.lpool && !let && !a.direct_blocks) {
+        const int lblocks = std::max(1, a.list_blocks);
+        if (a.posd) walk_lists_kernel<true><<<lblocks, kListBS, 0, s>>>(a);
+        else walk_lists_kernel<false><<<lblocks, kListBS, 0, s>>>(a);
+        LHIP_LAUNCH_CHECK();
+        if (a.posd) walk_launch<true, false, true>(s, a, potential, guard, blocks);
+        else walk_launch<false, false, true>(s, a, potential, guard, blocks);
+        LHIP_LAUNCH_CHECK();
+        return;
+    }
+    if (a.posd) let ? walk_launch<true, true, false>(s, a, potential, guard, blocks)
+                    : walk_launch<true, false, false>(s, a, potential, guard, blocks);
+    else let ? walk_launch<false, true, false>(s, a, potential, guard, blocks)
+             : walk_launch<false, false, false>(s, a, potential, guard, blocks);
     LHIP_LAUNCH_CHECK();
+}
+
+int walk_lists_blocks_per_cu() {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(walk_lists_kernel<true>),
+                                                     kListBS, 0) != hipSuccess ||
+        blocks < 1) {
+        (void)hipGetLastError();
+        blocks = 4;
+    }
+    return blocks;
 }
 
 }  // namespace lhip

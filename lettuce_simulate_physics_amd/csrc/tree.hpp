@@ -15,6 +15,8 @@ struct alignas(16) WalkCounters {
     int overfull;                     // bodies in finest-level cells with > 16 bodies
     unsigned int next_group;          // walk work counter
     int max_level;                    // build: deepest leaf level of the level pass (frames = 1)
+    unsigned int next_list;           // list traversal work counter
+    unsigned int list_chunks;         // list chunks the traversal asked for (> the pool's: overflow)
 };
 
 // The walk's node record, in level order (all cells of level 0, then level 1, ...; within a level
@@ -215,11 +217,27 @@ struct WalkArgs {
     int let_thr;
     int let_fan;
     int direct_blocks;
+    // Interaction lists (lpool != null, local tree only): walk_lists first traverses every group and
+    // writes its list to the pool, then the walk evaluates the lists instead of traversing. A
+    // group's list is one unit per kGroupMax-body chunk of it (one for every group of at most
+    // kGroupMax bodies): a count entry, then that many entries (kEnt* | node index). It fills pool
+    // chunks of kListChunk entries, chunk c continuing at chunk lnext[c], from chunk lhead[group].
+    uint32_t* lpool;
+    uint32_t* lnext;
+    uint32_t* lhead;
+    uint32_t list_cap;         // pool chunks; a traversal that needs more sets counters->list_chunks above it
+    int list_blocks;           // persistent grid of the list traversal (blocks of 4 waves)
 };
+// List entries: a quadrupole cell, a monopole cell or the bodies of a leaf, and the node index.
+constexpr int kListChunk = 256;
+constexpr uint32_t kEntCell = 0u << 30, kEntMono = 1u << 30, kEntRange = 2u << 30, kEntIndex = (1u << 30) - 1u;
+// Node indices must fit kEntIndex: lists are used only while the node capacity does.
+inline bool lists_fit(long long cap_nodes) { return cap_nodes <= (long long)kEntIndex; }
 void tree_walk(hipStream_t s, const WalkArgs& a, bool potential);
 // Entries of a walk wave's traversal stack (LDS).
 constexpr int kWalkStack = 896;
 int walk_blocks_per_cu();
+int walk_lists_blocks_per_cu();
 constexpr int kWalkBlockWaves = 4;  // waves per walk block
 
 // Vector-Jacobian product of the mixed-precision walk's forces with frozen topology

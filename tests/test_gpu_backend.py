@@ -455,6 +455,24 @@ def test_tree_key_restarts_bit_exact_with_oracle(hip, kind):
     assert rel_rms(a0, ad) < 1e-5 and t0["interactions"] == n * (n - 1)
 
 
+def test_keys_known_answers(hip):
+    # Bodies at exactly known quantised coordinates (origin 0, side 1 * (1 + 1e-6): 1 -> 0x1ffffd,
+    # 0.5 -> 0xffffe); the keys interleaved by hand (as tests/test_oracle.py states them).
+    b = O.Bodies.empty(4)
+    for k, (px, py, pz) in enumerate(((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (0.5, 0.5, 0.5), (1.0, 0.0, 0.5))):
+        b.x[k], b.y[k], b.z[k] = px, py, pz
+    b.m[:] = 0.25
+    b.id[:] = np.arange(4, dtype=np.uint32)
+    for mode in ("plain", "mixed"):
+        hip.set_precision(mode)
+        hip.upload(b)
+        hip.compute_forces(P("tree", 0.01), False)
+        assert [int(k) for k in hip.tree_keys()] == [0, 0x0FFFFFFFFFFFFFF8, 0x4B6DB6DB6DB6DB4C, 0x7FFFFFFFFFFFFFC7]
+        ids = hip.download_f64()[3] if mode == "mixed" else hip.download().id
+        assert list(ids) == [0, 2, 3, 1]
+    hip.set_precision("plain")
+
+
 def test_c0_direct_16384_ten_kdk_steps_match_reference(hip):
     """Config: 16384-body Plummer (seed 1, a = 1, cut 10), direct, eps 0.01, 10 KDK steps of dt 1e-3,
     against the reference CPU backend's outputs (tests/golden/c0_direct.npz, written from the
@@ -526,3 +544,34 @@ def test_c1_million_parity_with_reference(hip):
     print(f"c1 vs FP64 direct (4096 sample): median {np.median(e):.3g}, p99 {np.quantile(e, 0.99):.3g}, "
           f"rel rms {rel_rms(a[s], a64):.3g}")
     assert np.median(e) <= 1e-3
+
+
+@pytest.mark.parametrize("mode", ["plain", "mixed"])
+def test_walk_lists_equal_the_fused_walk(mode, monkeypatch):
+    # LETTUCE_HIP_WALK=lists splits the walk into a list traversal (its own kernel) and a list
+    # evaluation. The lists are the fused walk's (one decision function): the same interaction
+    # counts and per-body costs, forces equal up to FP32 summation order. The pool starts at 1
+    # chunk per 4 bodies; the first call of a larger set outgrows it and the walk runs again.
+    b = O.plummer(60000, 31)
+    p = P("tree", 0.01, 0.6)
+    out = {}
+    for walk in ("fused", "lists"):
+        monkeypatch.setenv("LETTUCE_HIP_WALK", walk)
+        be = L.HipBackend(0)
+        try:
+            be.set_precision(mode)
+            for bb in (O.plummer(1000, 3), b):  # a second, larger set grows the pool
+                a, _, t = got(be, bb, p, True)
+            be.upload(b)
+            be.compute_forces(p, True)
+            ids = be.download().id if mode == "plain" else be.download_f64()[3]
+            costs = np.zeros(len(b), np.float32)
+            costs[ids] = be.body_costs()
+            out[walk] = (a, t, costs)
+        finally:
+            be.close()
+    (af, tf, cf), (al, tl, cl) = out["fused"], out["lists"]
+    for k in ("interactions", "p2p", "p2c", "listed", "p2c_mono"):
+        assert tf[k] == tl[k], k
+    assert np.array_equal(cf, cl)
+    assert rel_rms(al, af) < 1e-6

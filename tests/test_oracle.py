@@ -46,14 +46,26 @@ def test_plummer_closed_form_moments():
 
 # ---------------------------------------------------------------- keys and sort
 
-def test_dilation_known_answers():
-    x = np.array([0x1FFFFF, 1, 2, 0], np.float32)  # not used as positions; check the dilation directly
-    v = np.array([0x1FFFFF], np.uint64)
-    r = np.zeros_like(v)
-    for bit in range(21):
-        r |= ((v >> np.uint64(bit)) & np.uint64(1)) << np.uint64(3 * bit)
-    assert int(r[0]) == 0x1249249249249249
-    assert x.size == 4
+def test_keys_known_answers():
+    # The oracle's keys for bodies at exactly known quantised coordinates: origin = the float minimum
+    # 0, side = 1 * (1 + 1e-6), so a coordinate v quantises to floor(v * 2^21 / 1.000001): 1 -> 2097149
+    # (0x1ffffd) and 0.5 -> 1048574 (0xffffe). Interleaving x << 2 | y << 1 | z of the 21-bit
+    # dilations by hand: (1, 1, 1) -> 7 * dilate(0x1ffffd) = 0x7fffffffffffffc7 (every bit but the
+    # three of level bit 1); (0.5, 0.5, 0.5) -> 0x0ffffffffffffff8; (1, 0, 0.5) ->
+    # dilate(0x1ffffd) << 2 | dilate(0xffffe) = 0x4b6db6db6db6db4c; the origin -> 0. The sorted order
+    # is then bodies 0, 2, 3, 1.
+    b = O.Bodies.empty(4)
+    for k, (px, py, pz) in enumerate(((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (0.5, 0.5, 0.5), (1.0, 0.0, 0.5))):
+        b.x[k], b.y[k], b.z[k] = px, py, pz
+    b.m[:] = 0.25
+    b.id[:] = np.arange(4, dtype=np.uint32)
+    o = O.Oracle()
+    o.upload(b)
+    o.compute_forces(1.0, 0.01, 0.6, 1)
+    keys, perm = o.keys()
+    assert [int(k) for k in keys] == [0, 0x0FFFFFFFFFFFFFF8, 0x4B6DB6DB6DB6DB4C, 0x7FFFFFFFFFFFFFC7]
+    assert list(perm) == [0, 2, 3, 1]
+    assert list(o.download().id) == [0, 2, 3, 1]
 
 
 def test_keys_match_closed_form_and_stable_sort():
