@@ -116,8 +116,8 @@ struct Context {
             if (v == "lists") lists_plain = lists_mixed = true;
             else if (v == "fused") lists_plain = lists_mixed = false;
         }
-        LHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_over), 2 * sizeof(int), hipHostMallocDefault));
-        host_over[0] = host_over[1] = 0;
+        LHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_over), 4 * sizeof(int), hipHostMallocDefault));
+        host_over[0] = host_over[1] = host_over[2] = host_over[3] = 0;
     }
     ~Context() {
         adj_destroy(adj);
@@ -290,7 +290,7 @@ struct Context {
         const std::size_t N = std::size_t(nn);
         key.reserve(N * 8); key2.reserve(N * 8);
         idx.reserve(N * 4); idx2.reserve(N * 4);
-        cap_nodes = node_capacity(nn);
+        cap_nodes = std::max(cap_nodes, node_reserve_initial(nn));
         tile_hist.reserve(radix_tile_hist_entries(int(std::max<long long>(nn, cap_nodes))) * 4);
         bl.reserve(N * 4);
         cnt.reserve((N + 1) * 4);
@@ -425,9 +425,7 @@ struct Context {
         const float inv_theta = p.theta > 0 ? float(1.0 / p.theta) : FLT_MAX;
         // The level pass flags over-full finest cells; the host reads the flag while the rest of the
         // build runs, and only then takes the key-restart path (a second build).
-        *reinterpret_cast<volatile int*>(host_over) = 0;
-        built_depth = tree_build(stream, bb, inv_theta, ev[9]);
-        LHIP_CHECK(hipEventSynchronize(ev[9]));
+        built_depth = build_waited(bb, inv_theta);
         if (*reinterpret_cast<volatile int*>(host_over)) {
             restart_keys(nullptr, nullptr, nullptr, true);
             rebuild(inv_theta, nullptr);
@@ -580,7 +578,7 @@ struct Context {
     // key frames. box (optional) replaces the positions the moments read (a replayed topology).
     void rebuild(float inv_theta, const double* const* box) {
         BuildBufs b = build_bufs();
-        if (frames > 1) grow_nodes(tree_node_count(stream, b) + 1);
+        grow_nodes(tree_node_count(stream, b) + 1);  // this build does not wait for the count
         b = build_bufs();
         b.host_over = nullptr;
         if (box) { b.xd = box[0]; b.yd = box[1]; b.zd = box[2]; }
@@ -588,6 +586,28 @@ struct Context {
     }
     // Deepest level of the last build's multipoles (tree_moments_keep needs no more).
     int built_depth = -1;
+
+    // tree_build with the host wait on the level pass (ev[9]); a tree with more nodes than the node
+    // arrays hold grows them to a quarter more than it has and is built again. bb's positions (the
+    // caller's) are kept; its node arrays follow the growth. Returns tree_build's depth, or -1 when
+    // host_over[0] asks for key restarts.
+    int build_waited(BuildBufs bb, float inv_theta) {
+        for (;;) {
+            *reinterpret_cast<volatile int*>(host_over) = 0;
+            const int d = tree_build(stream, bb, inv_theta, ev[9]);
+            LHIP_CHECK(hipEventSynchronize(ev[9]));
+            if (d != kBuildGrow) return d;
+            const long long M = (long long)uint32_t(*reinterpret_cast<volatile int*>(host_over + 2));
+            sync();
+            grow_nodes(std::min(node_capacity(n), M + M / 4 + 1024));
+            const BuildBufs g = build_bufs();
+            bb.cap_nodes = g.cap_nodes;
+            bb.pkey = g.pkey; bb.pkey2 = g.pkey2; bb.pval = g.pval; bb.pval2 = g.pval2; bb.inv = g.inv;
+            bb.tile_hist = g.tile_hist;
+            bb.node = g.node; bb.mom = g.mom; bb.prefix = g.prefix;
+            if (bb.momd) bb.momd = g.momd;
+        }
+    }
 
     BuildBufs build_bufs() {
         BuildBufs bb{};
@@ -1173,7 +1193,7 @@ inline void memory_need(uint64_t n, int method, bool mixed, uint64_t* dev, uint6
     if (mixed)
         for (int k = 0; k < 16; ++k) buf(N * 8);  // FP64 state, alternates, a, phi
     if (method == LHIP_METHOD_TREE) {
-        const uint64_t C = uint64_t(node_capacity((long long)N));
+        const uint64_t C = uint64_t(node_reserve_initial((long long)N));
         buf(N * 8); buf(N * 8); buf(N * 4); buf(N * 4);  // keys, sort index (double-buffered)
         buf(radix_tile_hist_entries(int(std::max<uint64_t>(N, C))) * 4);
         buf(N * 4);                                                        // levels word

@@ -287,9 +287,12 @@ void scan_exclusive(hipStream_t s, uint32_t* a, int n, uint32_t* tmp) {
 // Pre-order enumeration: every cell that starts at body i (levels c+1 .. min(L, 21)) is node
 // k = scan[i] + (level - c - 1). Writes the sort key (its level) and the value k, so a stable sort by
 // level yields the level order (within a level, pre-order = first-body order).
+// Nodes at or past ncap (the node arrays' capacity) are not written: the host sees the count and
+// builds again with larger arrays.
 __global__ void __launch_bounds__(kBS) enumerate_kernel(int n, const uint32_t* __restrict__ bl,
                                                         const uint32_t* __restrict__ scan, uint64_t* __restrict__ pkey,
-                                                        uint32_t* __restrict__ pval, const uint32_t* __restrict__ cap) {
+                                                        uint32_t* __restrict__ pval, const uint32_t* __restrict__ cap,
+                                                        uint32_t ncap) {
     const int i = blockIdx.x * kBS + threadIdx.x;
     if (i >= n) return;
     const uint32_t b = bl[i];
@@ -298,16 +301,22 @@ __global__ void __launch_bounds__(kBS) enumerate_kernel(int n, const uint32_t* _
     const uint32_t first = scan[i];
     for (int l = c + 1; l <= Lc; ++l) {
         const uint32_t k = first + uint32_t(l - c - 1);
+        if (k >= ncap) break;
         pkey[k] = uint64_t(l);
         pval[k] = k;
     }
 }
 
+// The node count (the pre-order scan's total) to the pinned host word the build waits on.
+__global__ void publish_count_kernel(const uint32_t* __restrict__ M_dev, int* __restrict__ host_word) {
+    __hip_atomic_store(host_word, int(*M_dev), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // inv[perm[p]] = p; level starts from the level totals (the radix digit counts).
 __global__ void __launch_bounds__(kBS) invert_kernel(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ M_dev,
                                                      uint32_t* __restrict__ inv, const uint32_t* __restrict__ level_tot,
-                                                     uint32_t* __restrict__ levstart) {
-    const int M = int(*M_dev);
+                                                     uint32_t* __restrict__ levstart, uint32_t ncap) {
+    const int M = int(min(*M_dev, ncap));
     const int p = blockIdx.x * kBS + threadIdx.x;
     if (p < M) inv[perm[p]] = uint32_t(p);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -333,7 +342,7 @@ __global__ void __launch_bounds__(kBS) nodes_kernel(const uint64_t* __restrict__
                                                     WalkNode* __restrict__ node, NodeMom* __restrict__ mom,
                                                     uint64_t* __restrict__ prefix, Frames F,
                                                     const uint32_t* __restrict__ cap, const uint32_t* __restrict__ fseg,
-                                                    const double4* __restrict__ fcube) {
+                                                    const double4* __restrict__ fcube, uint32_t ncap) {
     const int i = blockIdx.x * kBS + threadIdx.x;
     if (i >= n) return;
     const uint32_t b = bl[i];
@@ -359,7 +368,9 @@ __global__ void __launch_bounds__(kBS) nodes_kernel(const uint64_t* __restrict__
         }
         if (l > c) {
             const uint32_t k = first + uint32_t(l - c - 1);
-            const int p = int(inv[k]);
+            if (k >= ncap) return;  // a tree larger than the arrays: the host builds it again
+            const uint32_t p = inv[k];
+            if (p >= ncap) return;
             WalkNode nd;
             nd.cx = nd.cy = nd.cz = 0.f;
             nd.rcrit2 = 0.f;
@@ -367,7 +378,7 @@ __global__ void __launch_bounds__(kBS) nodes_kernel(const uint64_t* __restrict__
             nd.hi = -1;
             nd.info = l | (l == Lc ? kInfoLeaf : 0);
             // The first child is the next pre-order node (same first body, one level deeper).
-            nd.first = l == Lc ? -1 : int(inv[k + 1]);
+            nd.first = l == Lc || k + 1 >= ncap ? -1 : int(inv[k + 1]);
             node[p] = nd;
             prefix[p] = pre;
             NodeMom& md = mom[p];
@@ -927,24 +938,29 @@ int tree_build(hipStream_t s, const BuildBufs& b, float inv_theta, hipEvent_t le
         publish_depth_kernel<<<1, kPublish, 0, s>>>(b.gstart, grid_for(n, kLvTile), b.counters, b.host_over);
         LHIP_LAUNCH_CHECK();
     }
-    if (levels_done) LHIP_CHECK(hipEventRecord(levels_done, s));
     scan_exclusive(s, b.cnt, n, b.scan_tmp);
-    scan_exclusive(s, b.gflag, n, b.scan_tmp);
     const uint32_t* M_dev = b.cnt + n;  // node count
-    enumerate_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(n, b.bl, b.cnt, b.pkey, b.pval, ext ? b.cap : nullptr);
+    if (wait_levels) {
+        publish_count_kernel<<<1, 1, 0, s>>>(M_dev, b.host_over + 2);
+        LHIP_LAUNCH_CHECK();
+    }
+    if (levels_done) LHIP_CHECK(hipEventRecord(levels_done, s));
+    scan_exclusive(s, b.gflag, n, b.scan_tmp);
+    const uint32_t ncap = uint32_t(b.cap_nodes);
+    enumerate_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(n, b.bl, b.cnt, b.pkey, b.pval, ext ? b.cap : nullptr, ncap);
     LHIP_LAUNCH_CHECK();
     // Stable sort of the pre-order nodes by level (one 8-bit pass: levels <= kMaxDepth < 256).
     radix_pass(s, b.pkey, b.pval, b.pkey2, b.pval2, b.cap_nodes, M_dev, 0, b.tile_hist);
     const std::size_t ntiles = (static_cast<std::size_t>(b.cap_nodes) + kRadixTile - 1) / kRadixTile;
     const uint32_t* level_tot = b.tile_hist + ntiles * 256;
-    invert_kernel<<<grid_for(b.cap_nodes, kBS), kBS, 0, s>>>(b.pval2, M_dev, b.inv, level_tot, b.levstart);
+    invert_kernel<<<grid_for(b.cap_nodes, kBS), kBS, 0, s>>>(b.pval2, M_dev, b.inv, level_tot, b.levstart, ncap);
     LHIP_LAUNCH_CHECK();
     if (ext)
         nodes_kernel<true><<<grid_for(n, kBS), kBS, 0, s>>>(b.key, n, b.bl, b.cnt, b.inv, b.cube, b.node, b.mom,
-                                                            b.prefix, frames_of(b), b.cap, b.fseg, b.fcube);
+                                                            b.prefix, frames_of(b), b.cap, b.fseg, b.fcube, ncap);
     else
         nodes_kernel<false><<<grid_for(n, kBS), kBS, 0, s>>>(b.key, n, b.bl, b.cnt, b.inv, b.cube, b.node, b.mom,
-                                                             b.prefix, frames_of(b), nullptr, nullptr, nullptr);
+                                                             b.prefix, frames_of(b), nullptr, nullptr, nullptr, ncap);
     LHIP_LAUNCH_CHECK();
     groups_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(b.gflag, n, b.bl, b.gstart);
     LHIP_LAUNCH_CHECK();
@@ -964,6 +980,9 @@ int tree_build(hipStream_t s, const BuildBufs& b, float inv_theta, hipEvent_t le
         // The rest of the build above is queued; the multipoles wait for the level pass's report.
         LHIP_CHECK(hipEventSynchronize(levels_done));
         if (*reinterpret_cast<volatile int*>(b.host_over)) return -1;  // key restarts: the caller rebuilds
+        // More nodes than the arrays hold: nothing past them was written; the caller grows them and
+        // builds again.
+        if (*reinterpret_cast<volatile int*>(b.host_over + 2) > b.cap_nodes) return kBuildGrow;
         const int deepest = *reinterpret_cast<volatile int*>(b.host_over + 1);
         depth = std::min(depth, std::max(0, deepest));
     }

@@ -1,5 +1,7 @@
 // Device octree build and walk: buffers and launchers (kernels_build.hip, kernels_walk.hip).
 #pragma once
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace lhip {
@@ -103,7 +105,8 @@ struct BuildBufs {
     const uint32_t* fseg;
     const double4* fcube;
     int* host_over;            // pinned host words: [0] set by the level pass if a finest cell is
-                               // over-full; [1] the deepest leaf level, published after the pass
+                               // over-full; [1] the deepest leaf level, published after the pass;
+                               // [2] the node count
     int depth_hint;            // tree_moments_keep: deepest level of the tree (< 0: every level)
 };
 
@@ -111,6 +114,9 @@ std::size_t scan_tmp_entries(int n);
 // Bound on the node count for n bodies: n leaves at most, plus at most n/17 internal cells per
 // level (cells of one level are disjoint and hold >= 17 bodies each).
 inline long long node_capacity(long long n) { return n + (n / 17 + 1) * kLevels + 1; }
+// The node arrays' first size: a quarter of the bodies (trees of smooth distributions have 0.15-0.25
+// nodes per body at leaf size 16), grown to a quarter more than a build finds when one outgrows it.
+inline long long node_reserve_initial(long long n) { return std::min(node_capacity(n), n / 4 + 4096); }
 
 // Leaf levels, node enumeration, level-order layout, groups and multipoles. inv_theta as the
 // reference CPU computes it. levels_done (optional) is recorded after the level pass, which sets
@@ -119,6 +125,9 @@ inline long long node_capacity(long long n) { return n + (n / 17 + 1) * kLevels 
 // the multipoles: it launches only the levels the tree has, and none when host_over[0] is set (the
 // caller then restarts the keys and rebuilds). Returns the deepest level built (-1: no multipoles).
 int tree_build(hipStream_t s, const BuildBufs& b, float inv_theta, hipEvent_t levels_done = nullptr);
+// tree_build's return when the tree has more nodes than b.cap_nodes (host_over[2] holds the count;
+// only with levels_done and host_over): the node arrays must grow before the build runs again.
+constexpr int kBuildGrow = -2;
 // Deepest level of a tree with b.frames key frames.
 inline int tree_depth(int frames) { return kMaxLevel * frames; }
 

@@ -241,9 +241,7 @@ struct Run {
         BuildBufs bb = c.build_bufs();
         bb.xd = rx; bb.yd = rx + n; bb.zd = rx + 2 * std::size_t(n);
         const float inv_theta = fp.theta > 0 ? float(1.0 / fp.theta) : FLT_MAX;
-        *reinterpret_cast<volatile int*>(c.host_over) = 0;
-        c.built_depth = tree_build(s, bb, inv_theta, c.ev[9]);
-        LHIP_CHECK(hipEventSynchronize(c.ev[9]));
+        c.built_depth = c.build_waited(bb, inv_theta);
         if (*reinterpret_cast<volatile int*>(c.host_over)) {
             // key restarts of the recorded call: its order is already the restarted one
             const double* box[3] = {rx, rx + n, rx + 2 * std::size_t(n)};

@@ -432,6 +432,60 @@ def check_tree_against_oracle(hip, b, eps=0.01, theta=0.6):
     return no
 
 
+def test_tree_larger_than_the_first_node_reserve():
+    # The node arrays start at n / 4 + 4096 entries and grow when a build finds more nodes. 1000
+    # clusters of 17 bodies within ~1e-5 spread over a cube: each cluster's cell stays over-full down
+    # to level ~17, a chain of single-child cells (the reference keeps them), so the tree has 12,600
+    # nodes for 17,000 bodies against a first reserve of 8,346. The first build outgrows the reserve,
+    # writes nothing past it, and runs again with larger arrays; a second call needs no growth.
+    rng = np.random.default_rng(77)
+    ncl, k = 1000, 17
+    n = ncl * k
+    c = rng.uniform(-1, 1, (ncl, 3))
+    pos = (c[:, None, :] + 1e-5 * rng.standard_normal((ncl, k, 3))).reshape(-1, 3).astype(np.float32)
+    b = O.Bodies.empty(n)
+    b.x[:], b.y[:], b.z[:] = pos[:, 0], pos[:, 1], pos[:, 2]
+    b.m[:] = np.float32(1.0 / n)
+    b.id[:] = np.arange(n, dtype=np.uint32)
+    be = L.HipBackend(0)  # a fresh context: its node arrays start at the first reserve
+    try:
+        no = check_tree_against_oracle(be, b)
+        assert len(no["lo"]) > n // 4 + 4096
+        held = be.device_bytes_held()
+        check_tree_against_oracle(be, b)
+        assert be.device_bytes_held() == held
+    finally:
+        be.close()
+
+
+def test_one_hundred_million_body_force_call(hip):
+    # 1e8 Henon-unit Plummer bodies (the reference's largest benchmark row below 5e8,
+    # docs/running.md:80): one tree force call fits the memory estimate, and the state stays whole.
+    n = 100_000_000
+    need, _ = L.HipBackend.memory_need(n, "tree", "plain")
+    assert need / n < 250
+    be = L.HipBackend(0)
+    try:
+        b = L.plummer(n, 1, 3 * math.pi / 16, 20.0)
+        be.upload(b)
+        del b
+        p = P("tree", 0.01, 0.6)
+        be.compute_forces(p, False)
+        t = be.times()
+        held = be.device_bytes_held()
+        print(f"1e8 bodies: force {t['force_ms']:.1f} ms (sort {t['sort_ms']:.1f}, build {t['build_ms']:.1f}, "
+              f"walk {t['walk_ms']:.1f}), {t['interactions'] / n:.0f} int/body, held {held / n:.1f} B/body, "
+              f"memory_need {need / n:.1f} B/body")
+        assert held <= need
+        assert 2000 < t["interactions"] / n < 6000
+        ax, ay, az, _ = be.accelerations()
+        assert np.all(np.isfinite(ax)) and np.all(np.isfinite(ay)) and np.all(np.isfinite(az))
+        nodes, groups, _ = be.tree_counts()
+        assert 0 < nodes < n // 4 + 4096 and groups > 0
+    finally:
+        be.close()
+
+
 @pytest.mark.parametrize("n,seed", [(3000, 21), (50000, 22)])
 def test_tree_bit_exact_with_oracle(hip, n, seed):
     check_tree_against_oracle(hip, O.plummer(n, seed))
