@@ -159,8 +159,11 @@ int lhip_export_let(lhip_ctx* ctx, const float lo[3], const float hi[3], const l
 int lhip_export_let_copy(lhip_ctx* ctx, lhip_let_node* nodes, float* bx, float* by, float* bz, float* bm);
 /* Backend::memory_need (src/backend.hpp:137): device and host bytes a force call on n bodies
  * reserves in the plain mode, buffer by buffer as the reserves size them, plus 64 MiB for allocator
- * rounding; lhip_memory_need_mode (below) for either mode. Remote LETs and their exports (sized by
- * the other ranks) and the differentiable simulations are not included. */
+ * rounding; lhip_memory_need_mode (below) for either mode. Tree node arrays are counted for n/4 + 4096
+ * nodes, which smooth distributions stay below (a tree with more grows them). Not included: grown
+ * node arrays, the key-restart buffers (~76 B/body, reserved when a build first restarts keys in
+ * over-full cells), remote LETs and their exports (sized by the other ranks) and the differentiable
+ * simulations. */
 int lhip_memory_need(uint64_t n, int32_t method, uint64_t* device_bytes, uint64_t* host_bytes);
 /* Backend::device_memory (src/backend.hpp:138-143): free device bytes counting every buffer this
  * context's force path holds (lhip_device_bytes_held), and the total. */

@@ -1175,9 +1175,12 @@ struct Context {
 
 // Device bytes a force call on n bodies reserves (Backend::memory_need, src/backend.hpp:137): the
 // same sizes the reserve calls use, buffer by buffer, each at least 256 bytes, plus 64 MiB for
-// allocator rounding. Not included: remote LETs and their exports, whose sizes depend on the other
-// ranks, the differentiable simulations, and the interaction-list pool of LETTUCE_HIP_WALK=lists
-// (about 1 KiB per 4 bodies). Host: nothing per body beyond the caller's arrays.
+// allocator rounding. The node arrays are counted at their first size (n / 4 + 4096 nodes). Not
+// included: node arrays grown for a tree with more nodes; the key-restart buffers (about 76 B per body
+// with 4 key frames), reserved the first time a build restarts keys (clustered or coincident bodies)
+// and kept; remote LETs and their exports, whose sizes depend on the other ranks; the differentiable
+// simulations; and the interaction-list pool of LETTUCE_HIP_WALK=lists (about 1 KiB per 4 bodies).
+// Host: nothing per body beyond the caller's arrays.
 inline void memory_need(uint64_t n, int method, bool mixed, uint64_t* dev, uint64_t* host) {
     const uint64_t N = std::min<uint64_t>(n, INT32_MAX - 1);
     const int ni = int(N);
