@@ -299,7 +299,7 @@ struct Context {
         worder.reserve((N + 1) * 4);
         pflag.reserve((N + 1) * 4);
         pstart.reserve((N + 1) * 4);
-        scan_tmp.reserve(scan_tmp_entries(nn) * 4);
+        scan_tmp.reserve(2 * scan_tmp_entries(nn) * 4);  // two arrays scanned at once
         const std::size_t C = std::size_t(cap_nodes);
         pkey.reserve(C * 8); pkey2.reserve(C * 8);
         pval.reserve(C * 4); pval2.reserve(C * 4); inv.reserve(C * 4);
@@ -1201,7 +1201,7 @@ inline void memory_need(uint64_t n, int method, bool mixed, uint64_t* dev, uint6
         buf(radix_tile_hist_entries(int(std::max<uint64_t>(N, C))) * 4);
         buf(N * 4);                                                        // levels word
         for (int k = 0; k < 6; ++k) buf((N + 1) * 4);                      // counts, groups, packs, order
-        buf(scan_tmp_entries(ni) * 4);
+        buf(2 * scan_tmp_entries(ni) * 4);
         buf(C * 8); buf(C * 8); buf(C * 4); buf(C * 4); buf(C * 4);        // level sort, inverse
         buf((kDepthLevels + 2) * 4);
         buf(C * sizeof(WalkNode)); buf(C * sizeof(NodeMom)); buf(C * 8);  // records, key prefixes

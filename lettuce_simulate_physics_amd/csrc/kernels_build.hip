@@ -224,9 +224,15 @@ __device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t& tot
     return r;
 }
 
-__global__ void __launch_bounds__(kScanThreads) scan_reduce(const uint32_t* __restrict__ a, int n,
-                                                            uint32_t* __restrict__ bsum) {
+// One or two arrays scanned at once: blockIdx.y picks the array and its block sums.
+struct ScanArrays {
+    uint32_t* a[2];
+    uint32_t* bsum[2];
+};
+
+__global__ void __launch_bounds__(kScanThreads) scan_reduce(ScanArrays A, int n) {
     __shared__ uint32_t sh[kScanThreads];
+    const uint32_t* __restrict__ a = A.a[blockIdx.y];
     const long long b0 = (long long)blockIdx.x * kScanTile + threadIdx.x * kScanPT;
     uint32_t s = 0;
 #pragma unroll
@@ -234,12 +240,12 @@ __global__ void __launch_bounds__(kScanThreads) scan_reduce(const uint32_t* __re
         if (b0 + k < n) s += a[b0 + k];
     uint32_t total;
     block_exclusive_scan(s, sh, total);
-    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+    if (threadIdx.x == 0) A.bsum[blockIdx.y][blockIdx.x] = total;
 }
 
-__global__ void __launch_bounds__(kScanThreads) scan_top(uint32_t* __restrict__ bsum, int nb,
-                                                         uint32_t* __restrict__ out, int n) {
+__global__ void __launch_bounds__(kScanThreads) scan_top(ScanArrays A, int nb, int n) {
     __shared__ uint32_t sh[kScanThreads];
+    uint32_t* __restrict__ bsum = A.bsum[blockIdx.x];
     uint32_t carry = 0;
     for (int b = 0; b < nb; b += kScanThreads) {
         const int t = b + threadIdx.x;
@@ -249,12 +255,12 @@ __global__ void __launch_bounds__(kScanThreads) scan_top(uint32_t* __restrict__ 
         if (t < nb) bsum[t] = carry + e;
         carry += total;
     }
-    if (threadIdx.x == 0) out[n] = carry;
+    if (threadIdx.x == 0) A.a[blockIdx.x][n] = carry;
 }
 
-__global__ void __launch_bounds__(kScanThreads) scan_down(uint32_t* __restrict__ a, int n,
-                                                          const uint32_t* __restrict__ bsum) {
+__global__ void __launch_bounds__(kScanThreads) scan_down(ScanArrays A, int n) {
     __shared__ uint32_t sh[kScanThreads];
+    uint32_t* __restrict__ a = A.a[blockIdx.y];
     const long long b0 = (long long)blockIdx.x * kScanTile + threadIdx.x * kScanPT;
     uint32_t v[kScanPT];
     uint32_t s = 0;
@@ -264,7 +270,7 @@ __global__ void __launch_bounds__(kScanThreads) scan_down(uint32_t* __restrict__
         s += v[k];
     }
     uint32_t total;
-    uint32_t run = block_exclusive_scan(s, sh, total) + bsum[blockIdx.x];
+    uint32_t run = block_exclusive_scan(s, sh, total) + A.bsum[blockIdx.y][blockIdx.x];
 #pragma unroll
     for (int k = 0; k < kScanPT; ++k) {
         if (b0 + k < n) a[b0 + k] = run;
@@ -272,15 +278,20 @@ __global__ void __launch_bounds__(kScanThreads) scan_down(uint32_t* __restrict__
     }
 }
 
-void scan_exclusive(hipStream_t s, uint32_t* a, int n, uint32_t* tmp) {
+// In-place exclusive scans of `count` (1 or 2) arrays of n entries, a[n] = total, in one set of
+// three launches; tmp holds count * scan_tmp_entries(n) entries.
+void scan_exclusive_n(hipStream_t s, uint32_t* a0, uint32_t* a1, int count, int n, uint32_t* tmp) {
     const int nb = int((static_cast<long long>(n) + kScanTile - 1) / kScanTile);
-    scan_reduce<<<nb, kScanThreads, 0, s>>>(a, n, tmp);
+    ScanArrays A{{a0, a1}, {tmp, tmp + scan_tmp_entries(n)}};
+    scan_reduce<<<dim3(nb, count), kScanThreads, 0, s>>>(A, n);
     LHIP_LAUNCH_CHECK();
-    scan_top<<<1, kScanThreads, 0, s>>>(tmp, nb, a, n);
+    scan_top<<<count, kScanThreads, 0, s>>>(A, nb, n);
     LHIP_LAUNCH_CHECK();
-    scan_down<<<nb, kScanThreads, 0, s>>>(a, n, tmp);
+    scan_down<<<dim3(nb, count), kScanThreads, 0, s>>>(A, n);
     LHIP_LAUNCH_CHECK();
 }
+
+void scan_exclusive(hipStream_t s, uint32_t* a, int n, uint32_t* tmp) { scan_exclusive_n(s, a, a, 1, n, tmp); }
 
 // ---------------------------------------------------------------- nodes
 
@@ -439,21 +450,38 @@ __device__ __forceinline__ float rcrit2_of(float cx, float cy, float cz, float g
 // kMomInternal takes the internal nodes of `level` (their children are done by then). Each node's
 // sums are the same either way.
 constexpr int kMomLeaves = 1, kMomInternal = 2;
-template <bool MIXED, bool KEEP = false, bool EXT = false>
-__global__ void __launch_bounds__(kBS) moments_level(int part, int level, int n, const float4* __restrict__ posm,
-                                                     const double* __restrict__ xd, const double* __restrict__ yd,
-                                                     const double* __restrict__ zd, const float* __restrict__ bm,
-                                                     const uint32_t* __restrict__ bl,
-                                                     const uint32_t* __restrict__ levstart,
-                                                     const uint64_t* __restrict__ prefix, WalkNode* __restrict__ node,
-                                                     NodeMom* __restrict__ mom, NodeMomD* __restrict__ momd,
-                                                     float inv_theta, Frames F) {
-    const bool leaves = part == kMomLeaves;
-    const int p0 = leaves ? 0 : int(levstart[level]), p1 = int(levstart[level + 1]);
-    const int cend = int(levstart[level + 2]);  // end of the next level
-    for (int p = p0 + blockIdx.x * kBS + threadIdx.x; p < p1; p += gridDim.x * kBS) {
+struct MomArgs {
+    int n;
+    const float4* __restrict__ posm;
+    const double *__restrict__ xd, *__restrict__ yd, *__restrict__ zd;
+    const float* __restrict__ bm;
+    const uint32_t* __restrict__ bl;
+    const uint32_t* __restrict__ levstart;
+    const uint64_t* __restrict__ prefix;
+    WalkNode* __restrict__ node;
+    NodeMom* __restrict__ mom;
+    NodeMomD* __restrict__ momd;
+    float inv_theta;
+    Frames F;
+};
+
+// The multipoles of node p (a leaf from its bodies, an internal node of `level` from its children).
+template <bool MIXED, bool KEEP, bool EXT>
+__device__ __forceinline__ void moment_node(const MomArgs& A, int p, bool leaves, int level, int cend) {
+    const int n = A.n;
+    const float4* __restrict__ posm = A.posm;
+    const double *__restrict__ xd = A.xd, *__restrict__ yd = A.yd, *__restrict__ zd = A.zd;
+    const float* __restrict__ bm = A.bm;
+    const uint32_t* __restrict__ bl = A.bl;
+    const uint64_t* __restrict__ prefix = A.prefix;
+    WalkNode* __restrict__ node = A.node;
+    NodeMom* __restrict__ mom = A.mom;
+    NodeMomD* __restrict__ momd = A.momd;
+    const float inv_theta = A.inv_theta;
+    const Frames& F = A.F;
+    {
         WalkNode nd = node[p];
-        if (info_leaf(nd.info) != leaves) continue;
+        if (info_leaf(nd.info) != leaves) return;
         NodeMom md = mom[p];
         double M = 0, sx = 0, sy = 0, sz = 0;
         double q[6] = {0, 0, 0, 0, 0, 0};
@@ -599,7 +627,7 @@ __global__ void __launch_bounds__(kBS) moments_level(int part, int level, int n,
         md.qyy = float(q[3]); md.qyz = float(q[4]); md.qzz = float(q[5]);
         if (KEEP) {
             mom[p] = md;
-            continue;
+            return;
         }
         md.rcrit2 = rcrit2_of(md.cx, md.cy, md.cz, md.gx, md.gy, md.gz, md.half, inv_theta);
         {
@@ -613,6 +641,16 @@ __global__ void __launch_bounds__(kBS) moments_level(int part, int level, int n,
         node[p] = nd;
     }
 }
+
+template <bool MIXED, bool KEEP = false, bool EXT = false>
+__global__ void __launch_bounds__(kBS) moments_level(int part, int level, MomArgs A) {
+    const bool leaves = part == kMomLeaves;
+    const int p0 = leaves ? 0 : int(A.levstart[level]), p1 = int(A.levstart[level + 1]);
+    const int cend = int(A.levstart[level + 2]);  // end of the next level
+    for (int p = p0 + blockIdx.x * kBS + threadIdx.x; p < p1; p += gridDim.x * kBS)
+        moment_node<MIXED, KEEP, EXT>(A, p, leaves, level, cend);
+}
+
 
 // Walk packs: runs of consecutive sibling groups (same group level G, same level-(G-1) parent: the
 // first body of the later group shares exactly G - 1 octal digits with the body before it), each
@@ -901,24 +939,47 @@ void unit_order(hipStream_t s, const BuildBufs& b, const uint32_t* ustart, const
     radix_pass(s, b.pkey, b.pval, b.pkey2, order, b.n, ucount + b.n, 0, b.tile_hist);
 }
 
+// The multipole launches over a built level order: every leaf in one launch, then the internal
+// levels deepest first, one launch each. A level costs its chain of dependent loads (node record,
+// child prefixes, child moments), not its launch: levels 4..0 finished by one workgroup with
+// barriers between them took 51.6 us at 2^20 bodies against about 35 us as five launches.
+template <bool MIXED, bool KEEP, bool EXT>
+void launch_moments(hipStream_t s, const MomArgs& A, int depth) {
+    moments_level<MIXED, KEEP, EXT><<<leaf_grid(A.n), kBS, 0, s>>>(kMomLeaves, depth, A);
+    LHIP_LAUNCH_CHECK();
+    for (int l = depth - 1; l >= 0; --l) {
+        moments_level<MIXED, KEEP, EXT><<<level_grid(A.n), kBS, 0, s>>>(kMomInternal, l, A);
+        LHIP_LAUNCH_CHECK();
+    }
+}
+
+MomArgs mom_args(const BuildBufs& b, bool mixed, float inv_theta) {
+    MomArgs A{};
+    A.n = b.n;
+    A.posm = mixed ? nullptr : b.posm;
+    if (mixed) {
+        A.xd = b.xd; A.yd = b.yd; A.zd = b.zd; A.bm = b.m;
+    }
+    A.bl = b.bl;
+    A.levstart = b.levstart;
+    A.prefix = b.prefix;
+    A.node = b.node;
+    A.mom = b.mom;
+    A.momd = mixed ? b.momd : nullptr;
+    A.inv_theta = inv_theta;
+    A.F = frames_of(b);
+    return A;
+}
+
 void tree_moments_keep(hipStream_t s, const BuildBufs& b) {
     const bool ext = b.frames > 1;
     const int full = tree_depth(ext ? b.frames : 1);
     const int depth = b.depth_hint >= 0 ? std::min(b.depth_hint, full) : full;
-    auto launch = [&](int part, int l, unsigned g) {
-        if (ext)
-            moments_level<true, true, true><<<g, kBS, 0, s>>>(part, l, b.n, nullptr, b.xd, b.yd, b.zd, b.m, b.bl,
-                                                              b.levstart, b.prefix, b.node, b.mom, b.momd, 1.f,
-                                                              frames_of(b));
-        else
-            moments_level<true, true><<<g, kBS, 0, s>>>(part, l, b.n, nullptr, b.xd, b.yd, b.zd, b.m, b.bl,
-                                                        b.levstart, b.prefix, b.node, b.mom, b.momd, 1.f,
-                                                        frames_of(b));
-        LHIP_LAUNCH_CHECK();
-    };
-    launch(kMomLeaves, depth, leaf_grid(b.n));
-    for (int l = depth - 1; l >= 0; --l) launch(kMomInternal, l, level_grid(b.n));
+    const MomArgs A = mom_args(b, true, 1.f);
+    if (ext) launch_moments<true, true, true>(s, A, depth);
+    else launch_moments<true, true, false>(s, A, depth);
 }
+
 
 std::size_t scan_tmp_entries(int n) {
     return std::size_t((static_cast<long long>(n) + kScanTile) / kScanTile) + 1;
@@ -938,14 +999,13 @@ int tree_build(hipStream_t s, const BuildBufs& b, float inv_theta, hipEvent_t le
         publish_depth_kernel<<<1, kPublish, 0, s>>>(b.gstart, grid_for(n, kLvTile), b.counters, b.host_over);
         LHIP_LAUNCH_CHECK();
     }
-    scan_exclusive(s, b.cnt, n, b.scan_tmp);
+    scan_exclusive_n(s, b.cnt, b.gflag, 2, n, b.scan_tmp);  // node counts and group starts
     const uint32_t* M_dev = b.cnt + n;  // node count
     if (wait_levels) {
         publish_count_kernel<<<1, 1, 0, s>>>(M_dev, b.host_over + 2);
         LHIP_LAUNCH_CHECK();
     }
     if (levels_done) LHIP_CHECK(hipEventRecord(levels_done, s));
-    scan_exclusive(s, b.gflag, n, b.scan_tmp);
     const uint32_t ncap = uint32_t(b.cap_nodes);
     enumerate_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(n, b.bl, b.cnt, b.pkey, b.pval, ext ? b.cap : nullptr, ncap);
     LHIP_LAUNCH_CHECK();
@@ -986,30 +1046,10 @@ int tree_build(hipStream_t s, const BuildBufs& b, float inv_theta, hipEvent_t le
         const int deepest = *reinterpret_cast<volatile int*>(b.host_over + 1);
         depth = std::min(depth, std::max(0, deepest));
     }
-    const Frames F = frames_of(b);
     // The deepest level present holds only leaves, so the internal levels start one above it.
-    auto launch = [&](int part, int l, unsigned g) {
-        if (b.momd) {
-            if (ext)
-                moments_level<true, false, true><<<g, kBS, 0, s>>>(part, l, n, nullptr, b.xd, b.yd, b.zd, b.m, b.bl,
-                                                                   b.levstart, b.prefix, b.node, b.mom, b.momd,
-                                                                   inv_theta, F);
-            else
-                moments_level<true><<<g, kBS, 0, s>>>(part, l, n, nullptr, b.xd, b.yd, b.zd, b.m, b.bl, b.levstart,
-                                                      b.prefix, b.node, b.mom, b.momd, inv_theta, F);
-        } else {
-            if (ext)
-                moments_level<false, false, true><<<g, kBS, 0, s>>>(part, l, n, b.posm, nullptr, nullptr, nullptr,
-                                                                    nullptr, b.bl, b.levstart, b.prefix, b.node,
-                                                                    b.mom, nullptr, inv_theta, F);
-            else
-                moments_level<false><<<g, kBS, 0, s>>>(part, l, n, b.posm, nullptr, nullptr, nullptr, nullptr, b.bl,
-                                                       b.levstart, b.prefix, b.node, b.mom, nullptr, inv_theta, F);
-        }
-        LHIP_LAUNCH_CHECK();
-    };
-    launch(kMomLeaves, depth, leaf_grid(n));
-    for (int l = depth - 1; l >= 0; --l) launch(kMomInternal, l, level_grid(n));
+    const MomArgs A = mom_args(b, b.momd != nullptr, inv_theta);
+    if (b.momd) ext ? launch_moments<true, false, true>(s, A, depth) : launch_moments<true, false, false>(s, A, depth);
+    else ext ? launch_moments<false, false, true>(s, A, depth) : launch_moments<false, false, false>(s, A, depth);
     return depth;
 }
 

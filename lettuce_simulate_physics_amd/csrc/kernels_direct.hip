@@ -73,8 +73,7 @@ __global__ void __launch_bounds__(kDirectBS) direct_kernel(const float4* __restr
 
     auto plain = [&](int a, int b) {
         int j = a;
-        for (; j + 4 <= b; j += 4) {
-            const float4 q0 = posm[j], q1 = posm[j + 1], q2 = posm[j + 2], q3 = posm[j + 3];
+        auto four = [&](float4 q0, float4 q1, float4 q2, float4 q3) {
 #pragma unroll
             for (int k = 0; k < T; ++k) {
                 pair_plain<POT>(px[k], py[k], pz[k], q0, e2, sx[k], sy[k], sz[k], sp[k]);
@@ -82,6 +81,18 @@ __global__ void __launch_bounds__(kDirectBS) direct_kernel(const float4* __restr
                 pair_plain<POT>(px[k], py[k], pz[k], q2, e2, sx[k], sy[k], sz[k], sp[k]);
                 pair_plain<POT>(px[k], py[k], pz[k], q3, e2, sx[k], sy[k], sz[k], sp[k]);
             }
+        };
+        if (j + 4 <= b) {
+            // The next four sources load (scalar loads) while the current four are evaluated, so
+            // the scalar cache's latency hides behind the pair arithmetic.
+            float4 q0 = posm[j], q1 = posm[j + 1], q2 = posm[j + 2], q3 = posm[j + 3];
+            for (; j + 8 <= b; j += 4) {
+                const float4 n0 = posm[j + 4], n1 = posm[j + 5], n2 = posm[j + 6], n3 = posm[j + 7];
+                four(q0, q1, q2, q3);
+                q0 = n0; q1 = n1; q2 = n2; q3 = n3;
+            }
+            four(q0, q1, q2, q3);
+            j += 4;
         }
         for (; j < b; ++j) {
             const float4 q = posm[j];
