@@ -69,7 +69,6 @@ struct Slot {
     float x, y, z;
     int i;       // body index
     int bi;      // body within the chunk
-    int sp;      // list share (slots without a share read share 0 and are discarded)
     bool on;
     float sx, sy, sz, sp_pot;
     double dx, dy, dz, dp;  // FP64 carry of the mixed form
@@ -130,28 +129,29 @@ __device__ __forceinline__ void p2c(Slot& b, float4 A, float4 B, float2 C, float
     if (POT) b.sp_pot += __fmaf_rn(0.5f, t, A.w * inv);
 }
 
-// Every slot evaluates entries sp, sp + S, ... < steps * S (the list is padded to that length).
+// The lane's slots evaluate entries sp, sp + S, ... < steps * S of its list share sp (the list is
+// padded to that length): one LDS read of an entry feeds every slot of the lane.
 template <int NS, bool POT, bool GUARD, int NX>
-__device__ __forceinline__ void eval_cells(Slot (&sl)[NX], int S, int nc, const WaveLds& L, float e2) {
+__device__ __forceinline__ void eval_cells(Slot (&sl)[NX], int sp, int S, int nc, const WaveLds& L, float e2) {
     const int steps = (nc + S - 1) / S;
     for (int t = 0; t < steps; ++t) {
+        const CellL& c = L.c[t * S + sp];
+        const float4 A = c.A, B = c.B;
+        const float2 C = c.C;
 #pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            const int e = t * S + sl[s].sp;
-            p2c<POT, GUARD>(sl[s], L.c[e].A, L.c[e].B, L.c[e].C, e2);
-        }
+        for (int s = 0; s < NS; ++s) p2c<POT, GUARD>(sl[s], A, B, C, e2);
     }
 }
 
 template <int NS, bool POT, bool MASK, bool GUARD, int NX>
-__device__ __forceinline__ void eval_bodies(Slot (&sl)[NX], int S, int nb, const WaveLds& L, float e2) {
+__device__ __forceinline__ void eval_bodies(Slot (&sl)[NX], int sp, int S, int nb, const WaveLds& L, float e2) {
     const int steps = (nb + S - 1) / S;
     for (int t = 0; t < steps; ++t) {
+        const int e = t * S + sp;
+        const float4 q = L.bP[e];
+        const int j = MASK ? L.bI[e] : 0;
 #pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            const int e = t * S + sl[s].sp;
-            p2p<POT, MASK, GUARD>(sl[s], L.bP[e], MASK ? L.bI[e] : 0, e2);
-        }
+        for (int s = 0; s < NS; ++s) p2p<POT, MASK, GUARD>(sl[s], q, j, e2);
     }
 }
 
@@ -380,25 +380,31 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
         for (int clo = glo; clo < ghi; clo += kGroupMax) {
             const int chi = min(ghi, clo + kGroupMax);
             const int nbody = chi - clo;  // 1..128, wave-uniform
-            // Slots per lane and list shares: the pair (NS, S) that keeps the most lanes busy,
-            // S * nbody of the 64 * NS slots (NS = 2 wins when it more than doubles S).
-            // Slots per lane NS (up to NX) and list shares S = min(16, 64 NS / nbody): the pair
-            // that keeps the most slots busy (nbody S of 64 NS), the fewest slots on a tie.
+            // Slots per lane NS (up to NX) and list shares S. A share is taken by lps =
+            // ceil(nbody / NS) lanes, lane l of it holding bodies l, l + lps, ... in its NS slots,
+            // so all slots of a lane read the same entries: S = min(16, 64 / lps). The pair (NS, S)
+            // that keeps the most slots busy (nbody S of 64 NS) wins, the fewest slots on a tie.
             int NS = 1, S = min(kMaxShare, 64 / nbody);
 #pragma unroll
             for (int ns = 2; ns <= NX; ++ns) {
-                const int Sn = min(kMaxShare, 64 * ns / nbody);
+                const int Sn = min(kMaxShare, 64 / ((nbody + ns - 1) / ns));
                 if (Sn * NS > S * ns) { NS = ns; S = Sn; }
             }
+            const int lps = (nbody + NS - 1) / NS;
+            // The lane's share; lanes past the shares read share 0 and their slots are discarded.
+            int sp = lane / lps;
+            const bool lane_on = sp < S;
+            if (!lane_on) sp = 0;
             Slot sl[NX];
             float mnx = FLT_MAX, mny = FLT_MAX, mnz = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX, mxz = -FLT_MAX;
 #pragma unroll
             for (int s = 0; s < NX; ++s) {
-                const int v = s * 64 + lane;
-                sl[s].bi = v % nbody;
-                sl[s].sp = v / nbody;
-                sl[s].on = s < NS && sl[s].sp < S;
-                if (!sl[s].on) sl[s].sp = 0;
+                // Slot s of lane l: body (l mod lps) + s lps; off slots take body 0 (a valid load
+                // whose sums are dropped). The box below takes every body of the chunk: slot s of
+                // lane l also covers (s 64 + l) mod nbody.
+                const int b = lane % lps + s * lps;
+                sl[s].on = lane_on && s < NS && b < nbody;
+                sl[s].bi = sl[s].on ? b : (s * 64 + lane) % nbody;
                 sl[s].i = clo + sl[s].bi;
                 float4 p;
                 if (REL) {
@@ -450,10 +456,10 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                     L.c[nc + lane].C = make_float2(0.f, 0.f);
                 }
                 __builtin_amdgcn_wave_barrier();
-                if (NX >= 4 && NS == 4) eval_cells<4, POT, GUARD>(sl, S, nc, L, e2);
-                else if (NX >= 3 && NS == 3) eval_cells<3, POT, GUARD>(sl, S, nc, L, e2);
-                else if (NS == 2) eval_cells<2, POT, GUARD>(sl, S, nc, L, e2);
-                else eval_cells<1, POT, GUARD>(sl, S, nc, L, e2);
+                if (NX >= 4 && NS == 4) eval_cells<4, POT, GUARD>(sl, sp, S, nc, L, e2);
+                else if (NX >= 3 && NS == 3) eval_cells<3, POT, GUARD>(sl, sp, S, nc, L, e2);
+                else if (NS == 2) eval_cells<2, POT, GUARD>(sl, sp, S, nc, L, e2);
+                else eval_cells<1, POT, GUARD>(sl, sp, S, nc, L, e2);
                 fold();
                 __builtin_amdgcn_wave_barrier();
                 ncell_total += uint32_t(nc);
@@ -465,10 +471,10 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                     L.bI[nb + lane] = -1;
                 }
                 __builtin_amdgcn_wave_barrier();
-                if (NX >= 4 && NS == 4) eval_bodies<4, POT, MASK, GUARD>(sl, S, nb, L, e2);
-                else if (NX >= 3 && NS == 3) eval_bodies<3, POT, MASK, GUARD>(sl, S, nb, L, e2);
-                else if (NS == 2) eval_bodies<2, POT, MASK, GUARD>(sl, S, nb, L, e2);
-                else eval_bodies<1, POT, MASK, GUARD>(sl, S, nb, L, e2);
+                if (NX >= 4 && NS == 4) eval_bodies<4, POT, MASK, GUARD>(sl, sp, S, nb, L, e2);
+                else if (NX >= 3 && NS == 3) eval_bodies<3, POT, MASK, GUARD>(sl, sp, S, nb, L, e2);
+                else if (NS == 2) eval_bodies<2, POT, MASK, GUARD>(sl, sp, S, nb, L, e2);
+                else eval_bodies<1, POT, MASK, GUARD>(sl, sp, S, nb, L, e2);
                 fold();
                 __builtin_amdgcn_wave_barrier();
                 nbody_total += uint32_t(nb);
@@ -771,7 +777,7 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                 __builtin_amdgcn_wave_barrier();
 #pragma unroll
                 for (int s = 0; s < NX; ++s)
-                    if (sl[s].on) red[sl[s].sp * nbody + sl[s].bi] = D4{sl[s].dx, sl[s].dy, sl[s].dz, sl[s].dp};
+                    if (sl[s].on) red[sp * nbody + sl[s].bi] = D4{sl[s].dx, sl[s].dy, sl[s].dz, sl[s].dp};
                 __builtin_amdgcn_wave_barrier();
                 const double G = a.Gd;
 #pragma unroll
@@ -803,7 +809,7 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
 #pragma unroll
                 for (int s = 0; s < NX; ++s)
                     if (sl[s].on)
-                        red[sl[s].sp * nbody + sl[s].bi] = make_float4(sl[s].sx, sl[s].sy, sl[s].sz, sl[s].sp_pot);
+                        red[sp * nbody + sl[s].bi] = make_float4(sl[s].sx, sl[s].sy, sl[s].sz, sl[s].sp_pot);
                 __builtin_amdgcn_wave_barrier();
                 const float G = a.G;
 #pragma unroll
