@@ -50,7 +50,8 @@ struct Context {
     DevBuf x2, y2, z2, vx2, vy2, vz2, m2, id2, cost2;
     DevBuf ax, ay, az, pot, posm;
     // sort
-    DevBuf key, key2, idx, idx2, tile_hist;
+    DevBuf key, key2, idx, idx2, radix_buf;
+    RadixWork radix;
     // small
     DevBuf cube, bbox_part, red_part, red_out, direct_part;
     // tree
@@ -291,7 +292,7 @@ struct Context {
         key.reserve(N * 8); key2.reserve(N * 8);
         idx.reserve(N * 4); idx2.reserve(N * 4);
         cap_nodes = std::max(cap_nodes, node_reserve_initial(nn));
-        tile_hist.reserve(radix_tile_hist_entries(int(std::max<long long>(nn, cap_nodes))) * 4);
+        reserve_radix(std::max<long long>(nn, cap_nodes));
         bl.reserve(N * 4);
         cnt.reserve((N + 1) * 4);
         gflag.reserve((N + 1) * 4);
@@ -314,13 +315,19 @@ struct Context {
         if (M <= cap_nodes) return;
         cap_nodes = M;
         const std::size_t C = std::size_t(cap_nodes);
-        tile_hist.reserve(radix_tile_hist_entries(int(std::max<long long>(n, cap_nodes))) * 4);
+        reserve_radix(std::max<long long>(n, cap_nodes));
         pkey.reserve(C * 8); pkey2.reserve(C * 8);
         pval.reserve(C * 4); pval2.reserve(C * 4); inv.reserve(C * 4);
         node.reserve(C * sizeof(WalkNode));
         mom.reserve(C * sizeof(NodeMom));
         prefix.reserve(C * 8);
         if (mixed) momd.reserve(C * sizeof(NodeMomD));
+    }
+
+    // Radix sort work memory for `elems` elements.
+    void reserve_radix(long long elems) {
+        radix_buf.reserve(radix_work_words(int(elems)) * 4);
+        radix.words = radix_buf.as<uint32_t>();
     }
 
     float ms(int a, int b) const {
@@ -453,7 +460,7 @@ struct Context {
         tb.m2 = m2.as<float>(); tb.id2 = id2.as<uint32_t>(); tb.cost2 = cost2.as<float>();
         tb.key = key.as<uint64_t>(); tb.key2 = key2.as<uint64_t>();
         tb.idx = idx.as<uint32_t>(); tb.idx2 = idx2.as<uint32_t>();
-        tb.tile_hist = tile_hist.as<uint32_t>();
+        tb.radix = &radix;
         tb.cube = cube.as<Cube>();
         tb.bbox_part = bbox_part.as<void>();
         return tb;
@@ -534,22 +541,14 @@ struct Context {
                 // or the body itself outside restarted cells): cells stay in place, sorted inside.
                 uint64_t *ka = rs_ka.as<uint64_t>(), *kb = rs_kb.as<uint64_t>();
                 uint32_t *va = rs_va.as<uint32_t>(), *vb = rs_vb.as<uint32_t>();
-                iota_u32(stream, va, n);
                 LHIP_CHECK(hipMemcpyAsync(ka, fk, N * 8, hipMemcpyDeviceToDevice, stream));
-                for (int shift = 0; shift < 63; shift += 8) {
-                    radix_pass(stream, ka, va, kb, vb, n, nullptr, shift, tile_hist.as<uint32_t>());
-                    std::swap(ka, kb);
-                    std::swap(va, vb);
-                }
+                const RadixBufs rb{{ka, kb}, {va, vb}};
+                radix_sort(stream, radix, rb, true, n, nullptr, 0, kRadixDigits);  // even: back in ka, va
                 gather_u64(stream, va, rs_gkey.as<uint64_t>(), ka, n);
-                int gbits = 0;
-                while (gbits < 32 && (1ll << gbits) < (long long)n) gbits += 8;
-                for (int shift = 0; shift < gbits; shift += 8) {
-                    radix_pass(stream, ka, va, kb, vb, n, nullptr, shift, tile_hist.as<uint32_t>());
-                    std::swap(ka, kb);
-                    std::swap(va, vb);
-                }
-                const uint32_t* perm = va;
+                int gdig = 0;  // digits of a body index
+                while (gdig < 4 && (1ll << (8 * gdig)) < (long long)n) ++gdig;
+                radix_sort(stream, radix, rb, false, n, nullptr, 0, gdig);
+                const uint32_t* perm = rb.v[gdig & 1];
                 TreeBufs tb = tree_bufs();
                 tree_permute(stream, tb, perm, n, false);
                 swap_permuted(false);
@@ -603,7 +602,6 @@ struct Context {
             const BuildBufs g = build_bufs();
             bb.cap_nodes = g.cap_nodes;
             bb.pkey = g.pkey; bb.pkey2 = g.pkey2; bb.pval = g.pval; bb.pval2 = g.pval2; bb.inv = g.inv;
-            bb.tile_hist = g.tile_hist;
             bb.node = g.node; bb.mom = g.mom; bb.prefix = g.prefix;
             if (bb.momd) bb.momd = g.momd;
         }
@@ -630,7 +628,7 @@ struct Context {
         bb.pkey = pkey.as<uint64_t>(); bb.pkey2 = pkey2.as<uint64_t>();
         bb.pval = pval.as<uint32_t>(); bb.pval2 = pval2.as<uint32_t>();
         bb.inv = inv.as<uint32_t>();
-        bb.tile_hist = tile_hist.as<uint32_t>();
+        bb.radix = &radix;
         bb.levstart = levstart.as<uint32_t>();
         bb.node = node.as<WalkNode>();
         bb.mom = mom.as<NodeMom>();
@@ -1151,7 +1149,7 @@ struct Context {
         std::size_t held = 0;
         for (const DevBuf* b :
              {&x, &y, &z, &vx, &vy, &vz, &m, &id, &cost, &x2, &y2, &z2, &vx2, &vy2, &vz2, &m2, &id2, &cost2, &ax, &ay,
-              &az, &pot, &posm, &key, &key2, &idx, &idx2, &tile_hist, &cube, &bbox_part, &red_part, &red_out,
+              &az, &pot, &posm, &key, &key2, &idx, &idx2, &radix_buf, &cube, &bbox_part, &red_part, &red_out,
               &direct_part, &bl, &cnt, &gflag, &scan_tmp, &pkey, &pkey2, &pval, &pval2, &inv, &levstart, &node, &mom,
               &prefix, &gstart, &worder, &counters, &pflag, &pstart, &fkey, &fseg, &capb, &fcube, &rs_seg, &rs_bbox,
               &rs_cstart, &rs_gkey, &rs_ka, &rs_kb, &rs_va, &rs_vb, &xd, &yd, &zd, &vxd, &vyd, &vzd, &xd2, &yd2, &zd2,
@@ -1198,7 +1196,7 @@ inline void memory_need(uint64_t n, int method, bool mixed, uint64_t* dev, uint6
     if (method == LHIP_METHOD_TREE) {
         const uint64_t C = uint64_t(node_reserve_initial((long long)N));
         buf(N * 8); buf(N * 8); buf(N * 4); buf(N * 4);  // keys, sort index (double-buffered)
-        buf(radix_tile_hist_entries(int(std::max<uint64_t>(N, C))) * 4);
+        buf(radix_work_words(int(std::max<uint64_t>(N, C))) * 4);        // radix sort work
         buf(N * 4);                                                        // levels word
         for (int k = 0; k < 6; ++k) buf((N + 1) * 4);                      // counts, groups, packs, order
         buf(2 * scan_tmp_entries(ni) * 4);

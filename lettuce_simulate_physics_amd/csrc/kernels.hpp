@@ -55,6 +55,30 @@ void launch_direct_f64_targets(hipStream_t s, const void* x, const void* y, cons
                                double* ay, double* az, double* pot, double* part);
 
 // ---- tree (kernels_tree.hip) ----
+constexpr int kRadix = 256;      // 8-bit digits
+constexpr int kRadixDigits = 8;  // digits of a 64-bit key
+// Work memory of radix_sort, in device words: the tile counts of one digit, [256][tiles], then the
+// digit totals [256].
+struct RadixWork {
+    uint32_t* words = nullptr;
+};
+// Device words of radix_sort's work memory for up to n elements.
+std::size_t radix_work_words(int n);
+// Stable LSD radix sort of n (key, value) pairs by the 8-bit digits d0 <= d < d1 of the keys (bits
+// [8 d, 8 d + 8)): per digit a count, a scan and a scatter launch. The input is (k[0], v[0]), or
+// k[0] with the values 0, 1, ..., n - 1 when iota_values (v[0] is then not read, but later passes
+// still write it). Pass j reads buffer j & 1 and writes the other, so the result is in buffer
+// (d1 - d0) & 1. n_dev (optional) holds the element count on the device, n is then the capacity.
+// first_counted: the kernel that made the keys already wrote digit d0's tile counts (the body
+// sort's keys kernel). Returns the digit totals of the last pass, valid until the next sort on the
+// same work memory.
+struct RadixBufs {
+    uint64_t* k[2];
+    uint32_t* v[2];
+};
+const uint32_t* radix_sort(hipStream_t s, RadixWork& w, RadixBufs b, bool iota_values, int n,
+                           const uint32_t* n_dev, int d0, int d1, bool first_counted = false);
+
 struct TreeBufs {
     bool f64;  // positions and velocities are double (the mixed form), else float
     // bodies (sorted order after the permute)
@@ -71,7 +95,7 @@ struct TreeBufs {
     double4* posd; // FP64 (position, mass) in sorted order (mixed form), may be null
     uint64_t *key, *key2;
     uint32_t *idx, *idx2;
-    uint32_t* tile_hist;  // radix tile histograms
+    RadixWork* radix;
     Cube* cube;
     void* bbox_part;      // kReduceBlocks * 6 values of the position type
 };
@@ -85,12 +109,5 @@ void tree_permute(hipStream_t s, TreeBufs& b, const uint32_t* perm, int n, bool 
 // no sort (idx = identity).
 void tree_keys_sorted(hipStream_t s, const double* x, const double* y, const double* z, int n, void* bbox_part,
                       Cube* cube, uint64_t* key, uint32_t* idx);
-std::size_t radix_tile_hist_entries(int n);
-// One stable 8-bit LSD pass over bits [shift, shift + 8) of n (key, value) pairs; with n_dev the
-// count is read on the device and n is the capacity. The digit totals follow the tile histograms:
-// tile_hist[ntiles * 256 + d], ntiles = ceil(n / 2048).
-void radix_pass(hipStream_t s, const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, int n,
-                const uint32_t* n_dev, int shift, uint32_t* tile_hist);
-constexpr int kRadixTile = 2048;
 
 }  // namespace lhip

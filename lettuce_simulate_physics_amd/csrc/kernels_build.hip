@@ -936,7 +936,7 @@ void unit_order(hipStream_t s, const BuildBufs& b, const uint32_t* ustart, const
                 uint32_t* order) {
     unit_order_keys<<<grid_for(b.n, kBS), kBS, 0, s>>>(ustart, ucount, b.n, b.wcost, thr, b.pkey, b.pval);
     LHIP_LAUNCH_CHECK();
-    radix_pass(s, b.pkey, b.pval, b.pkey2, order, b.n, ucount + b.n, 0, b.tile_hist);
+    radix_sort(s, *b.radix, RadixBufs{{b.pkey, b.pkey2}, {b.pval, order}}, false, b.n, ucount + b.n, 0, 1);
 }
 
 // The multipole launches over a built level order: every leaf in one launch, then the internal
@@ -1010,9 +1010,8 @@ int tree_build(hipStream_t s, const BuildBufs& b, float inv_theta, hipEvent_t le
     enumerate_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(n, b.bl, b.cnt, b.pkey, b.pval, ext ? b.cap : nullptr, ncap);
     LHIP_LAUNCH_CHECK();
     // Stable sort of the pre-order nodes by level (one 8-bit pass: levels <= kMaxDepth < 256).
-    radix_pass(s, b.pkey, b.pval, b.pkey2, b.pval2, b.cap_nodes, M_dev, 0, b.tile_hist);
-    const std::size_t ntiles = (static_cast<std::size_t>(b.cap_nodes) + kRadixTile - 1) / kRadixTile;
-    const uint32_t* level_tot = b.tile_hist + ntiles * 256;
+    const uint32_t* level_tot =
+        radix_sort(s, *b.radix, RadixBufs{{b.pkey, b.pkey2}, {b.pval, b.pval2}}, false, b.cap_nodes, M_dev, 0, 1);
     invert_kernel<<<grid_for(b.cap_nodes, kBS), kBS, 0, s>>>(b.pval2, M_dev, b.inv, level_tot, b.levstart, ncap);
     LHIP_LAUNCH_CHECK();
     if (ext)

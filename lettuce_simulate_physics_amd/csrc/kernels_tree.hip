@@ -27,6 +27,7 @@
 #include "kernels.hpp"
 #include "morton.hpp"
 #include "tree.hpp"
+#include "wave.hpp"
 
 namespace lhip {
 namespace {
@@ -45,41 +46,118 @@ __global__ void __launch_bounds__(kBS) keys_kernel(const R* __restrict__ x, cons
 
 // ---------------------------------------------------------------- radix sort (stable LSD, 8-bit digits)
 //
-// Per pass: (1) tile histograms, (2) one block per digit scans that digit's row over the tiles,
-// (3) scatter: each tile ranks its keys stably in LDS (ballot matching per wave, running per-digit
-// counts per round), stores them tile-sorted in LDS, then writes runs of equal digits to
-// consecutive global addresses.
+// Per digit: (1) count: each tile of 2048 keys counts its digits; (2) one block per digit scans
+// that digit's counts over the tiles (exclusive, in place) and records the digit's total; (3)
+// scatter: each tile ranks its keys stably in LDS and writes runs of equal digits to consecutive
+// global addresses. The body sort's keys kernel counts digit 0 of its tiles as it makes the keys,
+// so the first pass has no count launch.
+//
+// Tile layout: wave w of a block holds keys [512 w, 512 w + 512) of the tile, lane l of round r
+// the key 512 w + 64 r + l, so each load is 64 consecutive keys and the tile order is (wave,
+// round, lane). A wave ranks its own keys round by round with a running count per digit in its own
+// LDS row (lanes of a round with the same digit are found by matching the digit's 8 bits with
+// ballots), so the rounds need no block barrier; two block scans over the 256 digits (wave scans,
+// then the wave totals) place each wave's run of every digit.
 
 constexpr int kSortThreads = 256;
+constexpr int kSortWaves = kSortThreads / 64;
 constexpr int kSortKPT = 8;                           // keys per thread per tile
 constexpr int kSortTile = kSortThreads * kSortKPT;    // 2048 keys per tile
-constexpr int kRadix = 256;
+__host__ __device__ inline int sort_tiles(int n) { return int((static_cast<long long>(n) + kSortTile - 1) / kSortTile); }
+
+__device__ __forceinline__ int tile_slot(int wave, int r, int lane) { return wave * (kSortKPT * 64) + r * 64 + lane; }
+
+// Lanes of the wave whose digit equals this lane's (8 ballots on its bits).
+__device__ __forceinline__ uint64_t digit_peers(int d) {
+    uint64_t peers = ~0ull;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const uint64_t m = __ballot((d >> b) & 1);
+        peers &= ((d >> b) & 1) ? m : ~m;
+    }
+    return peers;
+}
+
+// Stable ranks of a wave's keys by digit: pos[r] is the rank of the lane's round-r key among the
+// wave's keys of the same digit in wave order, and wc[d] ends as the wave's count of digit d. wc
+// is the wave's own LDS row, zeroed before; it is read and written by this wave only, in order.
+__device__ __forceinline__ void wave_rank(const int (&dg)[kSortKPT], uint32_t (&pos)[kSortKPT], uint32_t* wc,
+                                          int lane) {
+    const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int r = 0; r < kSortKPT; ++r) {
+        const uint64_t peers = digit_peers(dg[r]);
+        const uint32_t rank = __popcll(peers & lt);
+        const uint32_t before = wc[dg[r]];
+        pos[r] = before + rank;
+        if (rank == 0) wc[dg[r]] = before + uint32_t(__popcll(peers));
+    }
+}
+
+// The wave's digit counts of the keys whose ok bit is set (wc as in wave_rank).
+__device__ __forceinline__ void wave_count(const int (&dg)[kSortKPT], const bool (&ok)[kSortKPT], uint32_t* wc,
+                                           int lane) {
+    const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int r = 0; r < kSortKPT; ++r) {
+        const uint64_t peers = digit_peers(dg[r]) & __ballot(ok[r]);
+        if (ok[r] && (peers & lt) == 0) wc[dg[r]] += uint32_t(__popcll(peers));
+    }
+}
+
+// Exclusive scan of one value per thread over the block (256 threads: wave scans, then the wave
+// totals through wsum, one barrier); *total gets the block sum. wsum must not be rewritten before
+// every thread has passed the next barrier.
+__device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t* wsum, int lane, int wave, uint32_t* total) {
+    const uint32_t incl = uint32_t(prefix_sum_incl(int(v), lane));
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t off = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kSortWaves; ++w) {
+        const uint32_t t = wsum[w];
+        off += w < wave ? t : 0u;
+        all += t;
+    }
+    if (total) *total = all;
+    return off + incl - v;
+}
+
+// The tile's digit counts written to hist[d * ntiles + tile] (counts from the waves' rows wc).
+__device__ __forceinline__ void tile_counts_out(uint32_t (*wc)[kRadix], uint32_t* __restrict__ hist, int ntiles) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < kSortWaves; ++w) c += wc[w][threadIdx.x];
+    hist[std::size_t(threadIdx.x) * ntiles + blockIdx.x] = c;
+}
 
 // n_dev (optional) holds the element count on the device; n is then only the capacity the grid
 // was sized for.
 __global__ void __launch_bounds__(kSortThreads) radix_count(const uint64_t* __restrict__ key, int n,
                                                             const uint32_t* __restrict__ n_dev, int shift, int ntiles,
                                                             uint32_t* __restrict__ hist) {
-    __shared__ uint32_t h[kRadix];
+    __shared__ uint32_t wc[kSortWaves][kRadix];
+    const int tid = int(threadIdx.x), lane = tid & 63, wave = tid >> 6;
     if (n_dev) n = min(n, int(*n_dev));
     const int base = blockIdx.x * kSortTile;
     if (base >= n) return;  // a tile past the elements (n_dev below the capacity): never scanned
-    // All of the tile's loads are issued before the first use, so their latencies overlap.
-    uint64_t kk[kSortKPT];
+    uint64_t k[kSortKPT];
+    bool ok[kSortKPT];
 #pragma unroll
     for (int r = 0; r < kSortKPT; ++r) {
-        const int i = base + r * kSortThreads + threadIdx.x;
-        kk[r] = i < n ? key[i] : 0ull;
+        const int i = base + tile_slot(wave, r, lane);
+        ok[r] = i < n;
+        k[r] = ok[r] ? key[i] : 0ull;
     }
-    h[threadIdx.x] = 0;
-    __syncthreads();
 #pragma unroll
-    for (int r = 0; r < kSortKPT; ++r) {
-        const int i = base + r * kSortThreads + threadIdx.x;
-        if (i < n) atomicAdd(&h[(kk[r] >> shift) & 0xff], 1u);
-    }
+    for (int w = 0; w < kSortWaves; ++w) wc[w][tid] = 0;
     __syncthreads();
-    hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+    int dg[kSortKPT];
+#pragma unroll
+    for (int r = 0; r < kSortKPT; ++r) dg[r] = int((k[r] >> shift) & 0xff);
+    wave_count(dg, ok, wc[wave], lane);
+    __syncthreads();
+    tile_counts_out(wc, hist, ntiles);
 }
 
 // One block per digit: exclusive scan of hist[d][0..tiles) in place, over the tiles that hold
@@ -87,29 +165,36 @@ __global__ void __launch_bounds__(kSortThreads) radix_count(const uint64_t* __re
 __global__ void __launch_bounds__(kSortThreads) radix_row_scan(uint32_t* __restrict__ hist, int ntiles,
                                                                uint32_t* __restrict__ tot, int n,
                                                                const uint32_t* __restrict__ n_dev) {
-    __shared__ uint32_t sh[kSortThreads];
-    uint32_t* row = hist + blockIdx.x * ntiles;
-    if (n_dev) ntiles = min(ntiles, int((min(n, int(*n_dev)) + kSortTile - 1) / kSortTile));
+    __shared__ uint32_t wsum[2][kSortWaves];
+    const int tid = int(threadIdx.x), lane = tid & 63, wave = tid >> 6;
+    uint32_t* row = hist + std::size_t(blockIdx.x) * ntiles;
+    if (n_dev) ntiles = min(ntiles, sort_tiles(min(n, int(*n_dev))));
+    constexpr int K = 4;  // consecutive entries per thread per chunk
     uint32_t carry = 0;
-    for (int b = 0; b < ntiles; b += kSortThreads) {
-        const int t = b + threadIdx.x;
-        const uint32_t v = t < ntiles ? row[t] : 0u;
-        sh[threadIdx.x] = v;
-        __syncthreads();
-        // Hillis-Steele inclusive scan over 256 entries.
-        for (int o = 1; o < kSortThreads; o <<= 1) {
-            const uint32_t a = threadIdx.x >= o ? sh[threadIdx.x - o] : 0u;
-            __syncthreads();
-            sh[threadIdx.x] += a;
-            __syncthreads();
+    int parity = 0;
+    for (int b = 0; b < ntiles; b += kSortThreads * K) {
+        uint32_t v[K], sum = 0;
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const int t = b + tid * K + j;
+            v[j] = t < ntiles ? row[t] : 0u;
+            sum += v[j];
         }
-        if (t < ntiles) row[t] = carry + sh[threadIdx.x] - v;
-        carry += sh[kSortThreads - 1];
-        __syncthreads();
+        uint32_t chunk = 0;
+        uint32_t run = carry + block_scan_excl(sum, wsum[parity], lane, wave, &chunk);
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const int t = b + tid * K + j;
+            if (t < ntiles) row[t] = run;
+            run += v[j];
+        }
+        carry += chunk;
+        parity ^= 1;
     }
-    if (threadIdx.x == 0) tot[blockIdx.x] = carry;
+    if (tid == 0) tot[blockIdx.x] = carry;
 }
 
+// vin null: the values are the input positions.
 __global__ void __launch_bounds__(kSortThreads) radix_scatter(const uint64_t* __restrict__ kin,
                                                               const uint32_t* __restrict__ vin,
                                                               uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
@@ -118,93 +203,59 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter(const uint64_t* __
                                                               const uint32_t* __restrict__ tot) {
     __shared__ uint64_t sk[kSortTile];
     __shared__ uint32_t sv[kSortTile];
-    __shared__ uint32_t cnt[kSortThreads / 64][kRadix];
-    __shared__ uint32_t pre[kSortThreads / 64][kRadix];
-    __shared__ uint32_t run[kRadix];
-    __shared__ uint32_t dstart[kRadix];   // tile-local start of each digit
-    __shared__ uint32_t gbase[kRadix];    // global start of each digit for this tile
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int base = blockIdx.x * kSortTile;
+    __shared__ uint32_t wc[kSortWaves][kRadix];   // the waves' digit counts, then their tile offsets
+    __shared__ uint32_t dstart[kRadix];           // tile-local start of each digit
+    __shared__ uint32_t gbase[kRadix];            // global start of each digit for this tile
+    __shared__ uint32_t wsum[2][kSortWaves];
+    const int tid = int(threadIdx.x), lane = tid & 63, wave = tid >> 6;
     if (n_dev) n = min(n, int(*n_dev));
+    const int base = blockIdx.x * kSortTile;
     if (base >= n) return;  // nothing to place (the row scans skipped this tile)
-    // All global loads of the block are issued here, before the first barrier, so that their
-    // latencies overlap each other and the digit-base scan.
+    // All global loads of the block are issued before their first use.
     uint64_t k[kSortKPT];
     uint32_t v[kSortKPT], pos[kSortKPT];
     int dg[kSortKPT];
 #pragma unroll
     for (int r = 0; r < kSortKPT; ++r) {
-        const int i = base + r * kSortThreads + tid;
+        const int i = base + tile_slot(wave, r, lane);
         const bool ok = i < n;
         k[r] = ok ? kin[i] : ~0ull;
-        v[r] = ok ? vin[i] : 0u;
+        v[r] = ok ? (vin ? vin[i] : uint32_t(i)) : 0u;
     }
     const uint32_t my_tot = tot[tid];
-    const uint32_t my_hist = hist[tid * ntiles + blockIdx.x];
-    run[tid] = 0;
-    for (int w = 0; w < kSortThreads / 64; ++w) cnt[w][tid] = 0;
-    // Digit base = exclusive scan of the digit totals (block scan of 256 values).
-    {
-        dstart[tid] = my_tot;
-        __syncthreads();
-        for (int o = 1; o < kRadix; o <<= 1) {
-            const uint32_t a = tid >= o ? dstart[tid - o] : 0u;
-            __syncthreads();
-            dstart[tid] += a;
-            __syncthreads();
-        }
-        gbase[tid] = dstart[tid] - my_tot + my_hist;
-        __syncthreads();
+    const uint32_t my_row = hist[std::size_t(tid) * ntiles + blockIdx.x];
+#pragma unroll
+    for (int w = 0; w < kSortWaves; ++w) wc[w][tid] = 0;
+    // Out-of-range slots take digit 255; they are last in tile order, so they rank after every real
+    // key of the tile and land past the valid part of the tile-sorted LDS copy.
+#pragma unroll
+    for (int r = 0; r < kSortKPT; ++r)
+        dg[r] = base + tile_slot(wave, r, lane) < n ? int((k[r] >> shift) & 0xff) : 0xff;
+    __syncthreads();
+    wave_rank(dg, pos, wc[wave], lane);
+    // Global digit starts: exclusive scan of the digit totals (its barrier also completes the ranks).
+    const uint32_t dbase = block_scan_excl(my_tot, wsum[0], lane, wave, nullptr);
+    uint32_t c[kSortWaves], all = 0;
+#pragma unroll
+    for (int w = 0; w < kSortWaves; ++w) {
+        c[w] = wc[w][tid];
+        all += c[w];
     }
-    const uint64_t lt = (1ull << lane) - 1ull;
+    const uint32_t ds = block_scan_excl(all, wsum[1], lane, wave, nullptr);
+    {
+        uint32_t o = ds;  // this thread's column of wc only: no other thread reads it before the barrier
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w) {
+            wc[w][tid] = o;
+            o += c[w];
+        }
+    }
+    dstart[tid] = ds;
+    gbase[tid] = dbase + my_row;
+    __syncthreads();
 #pragma unroll
     for (int r = 0; r < kSortKPT; ++r) {
-        const bool ok = base + r * kSortThreads + tid < n;
-        // Out-of-range slots take digit 255 and sort after every real key of the tile (they are last
-        // in tile order), so real keys keep their ranks.
-        dg[r] = ok ? int((k[r] >> shift) & 0xff) : 0xff;
-        // Lanes of this wave with the same digit.
-        uint64_t peers = ~0ull;
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const uint64_t m = __ballot((dg[r] >> b) & 1);
-            peers &= ((dg[r] >> b) & 1) ? m : ~m;
-        }
-        const uint32_t rank = __popcll(peers & lt);
-        if (rank == 0) cnt[wave][dg[r]] = __popcll(peers);
-        __syncthreads();
-        {
-            uint32_t o = run[tid];
-#pragma unroll
-            for (int w = 0; w < kSortThreads / 64; ++w) {
-                pre[w][tid] = o;
-                o += cnt[w][tid];
-                cnt[w][tid] = 0;
-            }
-            run[tid] = o;
-        }
-        __syncthreads();
-        pos[r] = pre[wave][dg[r]] + rank;
-        __syncthreads();
-    }
-    // Tile-local digit starts: exclusive scan of run[].
-    {
-        const uint32_t mine = run[tid];
-        __syncthreads();
-        dstart[tid] = mine;
-        __syncthreads();
-        for (int o = 1; o < kRadix; o <<= 1) {
-            const uint32_t a = tid >= o ? dstart[tid - o] : 0u;
-            __syncthreads();
-            dstart[tid] += a;
-            __syncthreads();
-        }
-        dstart[tid] -= mine;
-        __syncthreads();
-    }
-#pragma unroll
-    for (int r = 0; r < kSortKPT; ++r) {
-        const uint32_t p = dstart[dg[r]] + pos[r];
+        const uint32_t p = wc[wave][dg[r]] + pos[r];
         sk[p] = k[r];
         sv[p] = v[r];
     }
@@ -221,6 +272,41 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter(const uint64_t* __
             vout[dst] = sv[p];
         }
     }
+}
+
+// Morton keys of the body sort, tile by tile, with digit 0 of each tile counted for its first pass.
+template <class R>
+__global__ void __launch_bounds__(kSortThreads) keys_count_kernel(const R* __restrict__ x, const R* __restrict__ y,
+                                                                  const R* __restrict__ z,
+                                                                  const Cube* __restrict__ cube,
+                                                                  uint64_t* __restrict__ key, int n, int ntiles,
+                                                                  uint32_t* __restrict__ hist) {
+    __shared__ uint32_t wc[kSortWaves][kRadix];
+    const int tid = int(threadIdx.x), lane = tid & 63, wave = tid >> 6;
+    const int base = blockIdx.x * kSortTile;
+    const double ox = cube->ox, oy = cube->oy, oz = cube->oz, scale = cube->scale;
+    R px[kSortKPT], py[kSortKPT], pz[kSortKPT];
+    bool ok[kSortKPT];
+#pragma unroll
+    for (int r = 0; r < kSortKPT; ++r) {
+        const int i = base + tile_slot(wave, r, lane);
+        ok[r] = i < n;
+        const int j = ok[r] ? i : n - 1;
+        px[r] = x[j]; py[r] = y[j]; pz[r] = z[j];
+    }
+#pragma unroll
+    for (int w = 0; w < kSortWaves; ++w) wc[w][tid] = 0;
+    int dg[kSortKPT];
+#pragma unroll
+    for (int r = 0; r < kSortKPT; ++r) {
+        const uint64_t kk = morton_key(px[r], py[r], pz[r], ox, oy, oz, scale);
+        if (ok[r]) key[base + tile_slot(wave, r, lane)] = kk;
+        dg[r] = int(kk & 0xff);
+    }
+    __syncthreads();
+    wave_count(dg, ok, wc[wave], lane);
+    __syncthreads();
+    tile_counts_out(wc, hist, ntiles);
 }
 
 // ---------------------------------------------------------------- permute
@@ -254,18 +340,13 @@ template <class R>
 void sort_impl(hipStream_t s, TreeBufs& b, int n, bool permute_cost) {
     const R *x = static_cast<const R*>(b.x), *y = static_cast<const R*>(b.y), *z = static_cast<const R*>(b.z);
     launch_bbox(s, x, y, z, n, b.bbox_part, b.cube);
-    keys_kernel<R><<<grid_for(n, kBS), kBS, 0, s>>>(x, y, z, b.cube, b.key, b.idx, n);
+    const int ntiles = sort_tiles(n);
+    keys_count_kernel<R><<<ntiles, kSortThreads, 0, s>>>(x, y, z, b.cube, b.key, n, ntiles, b.radix->words);
     LHIP_LAUNCH_CHECK();
-    uint64_t *ka = b.key, *kb = b.key2;
-    uint32_t *va = b.idx, *vb = b.idx2;
-    for (int shift = 0; shift < 63; shift += 8) {
-        radix_pass(s, ka, va, kb, vb, n, nullptr, shift, b.tile_hist);
-        std::swap(ka, kb);
-        std::swap(va, vb);
-    }
-    // Eight passes: the sorted keys and permutation are back in key/idx.
+    // Eight passes: the sorted keys and permutation end in key/idx.
+    radix_sort(s, *b.radix, RadixBufs{{b.key, b.key2}, {b.idx, b.idx2}}, true, n, nullptr, 0, kRadixDigits, true);
     auto P = [](void* p) { return static_cast<R*>(p); };
-    permute_kernel<R><<<grid_for(n, kBS), kBS, 0, s>>>(va, n, x, y, z, P(b.vx), P(b.vy), P(b.vz), b.m, b.id,
+    permute_kernel<R><<<grid_for(n, kBS), kBS, 0, s>>>(b.idx, n, x, y, z, P(b.vx), P(b.vy), P(b.vz), b.m, b.id,
                                                       permute_cost ? b.cost : nullptr, P(b.x2), P(b.y2), P(b.z2),
                                                       P(b.vx2), P(b.vy2), P(b.vz2), b.m2, b.id2,
                                                       permute_cost ? b.cost2 : nullptr, b.posm, b.posd);
@@ -274,21 +355,28 @@ void sort_impl(hipStream_t s, TreeBufs& b, int n, bool permute_cost) {
 
 }  // namespace
 
-void radix_pass(hipStream_t s, const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, int n,
-                const uint32_t* n_dev, int shift, uint32_t* tile_hist) {
-    const int ntiles = int((static_cast<long long>(n) + kSortTile - 1) / kSortTile);
-    uint32_t* tot = tile_hist + static_cast<std::size_t>(ntiles) * kRadix;
-    radix_count<<<ntiles, kSortThreads, 0, s>>>(kin, n, n_dev, shift, ntiles, tile_hist);
-    LHIP_LAUNCH_CHECK();
-    radix_row_scan<<<kRadix, kSortThreads, 0, s>>>(tile_hist, ntiles, tot, n, n_dev);
-    LHIP_LAUNCH_CHECK();
-    radix_scatter<<<ntiles, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, n_dev, shift, ntiles, tile_hist, tot);
-    LHIP_LAUNCH_CHECK();
-}
+std::size_t radix_work_words(int n) { return std::size_t(std::max(sort_tiles(n), 1)) * kRadix + kRadix; }
 
-std::size_t radix_tile_hist_entries(int n) {
-    const std::size_t ntiles = (static_cast<std::size_t>(n) + kSortTile - 1) / kSortTile;
-    return ntiles * kRadix + kRadix;  // + digit totals
+const uint32_t* radix_sort(hipStream_t s, RadixWork& w, RadixBufs b, bool iota_values, int n, const uint32_t* n_dev,
+                           int d0, int d1, bool first_counted) {
+    if (d0 < 0 || d1 > kRadixDigits || d0 > d1) throw Error(kInvalid, "radix_sort: digits out of range");
+    const int ntiles = sort_tiles(n);
+    uint32_t* hist = w.words;
+    uint32_t* tot = w.words + std::size_t(ntiles) * kRadix;
+    if (n <= 0) return tot;
+    for (int d = d0; d < d1; ++d) {
+        const int j = d - d0, src = j & 1, dst = src ^ 1;
+        if (j > 0 || !first_counted) {
+            radix_count<<<ntiles, kSortThreads, 0, s>>>(b.k[src], n, n_dev, 8 * d, ntiles, hist);
+            LHIP_LAUNCH_CHECK();
+        }
+        radix_row_scan<<<kRadix, kSortThreads, 0, s>>>(hist, ntiles, tot, n, n_dev);
+        LHIP_LAUNCH_CHECK();
+        radix_scatter<<<ntiles, kSortThreads, 0, s>>>(b.k[src], j == 0 && iota_values ? nullptr : b.v[src], b.k[dst],
+                                                      b.v[dst], n, n_dev, 8 * d, ntiles, hist, tot);
+        LHIP_LAUNCH_CHECK();
+    }
+    return tot;
 }
 
 void tree_keys_sorted(hipStream_t s, const double* x, const double* y, const double* z, int n, void* bbox_part,

@@ -79,7 +79,7 @@ struct BuildBufs {
     uint32_t* pval;            // cap: k, then the level-order permutation
     uint32_t* pval2;
     uint32_t* inv;             // cap: pre-order -> level order
-    uint32_t* tile_hist;       // radix histograms for cap_nodes elements
+    RadixWork* radix;          // radix sort work memory for cap_nodes elements
     uint32_t* levstart;        // kLevels + 2 level starts in level order
     // level-order node data
     WalkNode* node;            // cap
@@ -184,7 +184,7 @@ void tree_moments_keep(hipStream_t s, const BuildBufs& b);
 
 // The order in which a walk takes the units of a tree (ustart, count at ucount[n]): units estimated
 // at thr or more first, longest first, then the rest in their natural order (thr = 1: all sorted).
-// Estimates from b.wcost; scratch b.pkey, b.pval, b.pkey2, b.tile_hist.
+// Estimates from b.wcost; scratch b.pkey, b.pval, b.pkey2, b.radix.
 void unit_order(hipStream_t s, const BuildBufs& b, const uint32_t* ustart, const uint32_t* ucount, float thr,
                 uint32_t* order);
 
