@@ -362,7 +362,7 @@ __global__ void __launch_bounds__(kBS) nodes_kernel(const uint64_t* __restrict__
     const int Lc = min(L, top);
     if (Lc <= c) return;
     uint64_t kw = key[i];  // key word of the current frame
-    const uint32_t first = scan[i];
+    const uint32_t first = scan[i], M = scan[n];
     const Cube cb = *cube;
     double half = 0.5 * cb.side;
     double gx = cb.ox + half, gy = cb.oy + half, gz = cb.oz + half;
@@ -382,15 +382,22 @@ __global__ void __launch_bounds__(kBS) nodes_kernel(const uint64_t* __restrict__
             if (k >= ncap) return;  // a tree larger than the arrays: the host builds it again
             const uint32_t p = inv[k];
             if (p >= ncap) return;
-            WalkNode nd;
-            nd.cx = nd.cy = nd.cz = 0.f;
-            nd.rcrit2 = 0.f;
-            nd.lo = i;
-            nd.hi = -1;
-            nd.info = l | (l == Lc ? kInfoLeaf : 0);
+            // Every field but hi: a leaf's hi is written by the body that starts the next pre-order
+            // node (below), an internal node's by the multipole pass.
+            WalkNode* const np = node + p;
+            *reinterpret_cast<float4*>(np) = make_float4(0.f, 0.f, 0.f, 0.f);  // com, rcrit2
+            np->lo = i;
             // The first child is the next pre-order node (same first body, one level deeper).
-            nd.first = l == Lc || k + 1 >= ncap ? -1 : int(inv[k + 1]);
-            node[p] = nd;
+            *reinterpret_cast<int2*>(&np->info) =
+                make_int2(l | (l == Lc ? kInfoLeaf : 0), l == Lc || k + 1 >= ncap ? -1 : int(inv[k + 1]));
+            // The deepest cell body i starts is a leaf, and the pre-order node after a leaf starts at
+            // the body just past it: this body ends the leaf numbered first - 1, and the last leaf
+            // ends at n.
+            if (l == c + 1 && first > 0 && first - 1 < ncap) {
+                const uint32_t q = inv[first - 1];
+                if (q < ncap) node[q].hi = i;
+            }
+            if (l == Lc && k + 1 == M) node[p].hi = n;
             prefix[p] = pre;
             NodeMom& md = mom[p];
             md.gx = float(gx);
@@ -455,7 +462,6 @@ struct MomArgs {
     const float4* __restrict__ posm;
     const double *__restrict__ xd, *__restrict__ yd, *__restrict__ zd;
     const float* __restrict__ bm;
-    const uint32_t* __restrict__ bl;
     const uint32_t* __restrict__ levstart;
     const uint64_t* __restrict__ prefix;
     WalkNode* __restrict__ node;
@@ -468,11 +474,9 @@ struct MomArgs {
 // The multipoles of node p (a leaf from its bodies, an internal node of `level` from its children).
 template <bool MIXED, bool KEEP, bool EXT>
 __device__ __forceinline__ void moment_node(const MomArgs& A, int p, bool leaves, int level, int cend) {
-    const int n = A.n;
     const float4* __restrict__ posm = A.posm;
     const double *__restrict__ xd = A.xd, *__restrict__ yd = A.yd, *__restrict__ zd = A.zd;
     const float* __restrict__ bm = A.bm;
-    const uint32_t* __restrict__ bl = A.bl;
     const uint64_t* __restrict__ prefix = A.prefix;
     WalkNode* __restrict__ node = A.node;
     NodeMom* __restrict__ mom = A.mom;
@@ -495,28 +499,43 @@ __device__ __forceinline__ void moment_node(const MomArgs& A, int p, bool leaves
                 x = b.x; y = b.y; z = b.z; m = b.w;
             }
         };
-        // Loads are issued in batches (the leaf-end words, four bodies, the child prefixes, four
-        // children) before the sums that use them, which keep the one-at-a-time order.
+        // The loads that need only the node record are issued together (a leaf's bodies; a node's
+        // eight child candidates: prefixes, body ends and records), so a level costs two dependent
+        // round trips; the sums keep the one-at-a-time order.
         if (leaves) {
-            // The leaf ends at the first body past lo whose cells start above its level: within
-            // kLeafMax bodies, unless its bodies coincide (then the scan continues one by one).
-            const int level = info_level(nd.info);
-            int hi = nd.lo + 1;
-            {
-                uint32_t w[kLeafMax];
+            // The leaf's bodies [lo, hi) (hi from the node pass), the first KB of them loaded at once.
+            const int hi = nd.hi;
+            constexpr int KB = MIXED ? 8 : kLeafMax;  // bodies held in registers
+            float4 pb[MIXED ? 1 : KB];
+            double rx[MIXED ? KB : 1], ry[MIXED ? KB : 1], rz[MIXED ? KB : 1];
+            float rm[MIXED ? KB : 1];
 #pragma unroll
-                for (int t = 0; t < kLeafMax; ++t) w[t] = nd.lo + 1 + t < n ? bl[nd.lo + 1 + t] : 0u;
-                bool run = true;
-#pragma unroll
-                for (int t = 0; t < kLeafMax; ++t) {
-                    run = run && nd.lo + 1 + t < n && int((w[t] >> 16) & 0xff) - 1 >= level;
-                    hi += run ? 1 : 0;
+            for (int t = 0; t < KB; ++t) {
+                const int j = min(nd.lo + t, hi - 1);
+                if constexpr (MIXED) {
+                    rx[t] = xd[j]; ry[t] = yd[j]; rz[t] = zd[j]; rm[t] = bm[j];
+                } else {
+                    pb[t] = posm[j];
                 }
             }
-            if (hi == nd.lo + 1 + kLeafMax)
-                while (hi < n && int((bl[hi] >> 16) & 0xff) - 1 >= level) ++hi;
-            nd.hi = hi;
-            for (int j0 = nd.lo; j0 < hi; j0 += 4) {
+            auto held = [&](int t, double& x, double& y, double& z, double& m) {
+                if constexpr (MIXED) {
+                    x = rx[t]; y = ry[t]; z = rz[t]; m = rm[t];
+                } else {
+                    x = pb[t].x; y = pb[t].y; z = pb[t].z; m = pb[t].w;
+                }
+            };
+#pragma unroll
+            for (int t = 0; t < KB; ++t)
+                if (nd.lo + t < hi) {
+                    double x, y, z, m;
+                    held(t, x, y, z, m);
+                    M += m;
+                    sx = fma(m, x, sx);
+                    sy = fma(m, y, sy);
+                    sz = fma(m, z, sz);
+                }
+            for (int j0 = nd.lo + KB; j0 < hi; j0 += 4) {
                 double bx[4], by[4], bz[4], m[4];
 #pragma unroll
                 for (int t = 0; t < 4; ++t) body(min(j0 + t, hi - 1), bx[t], by[t], bz[t], m[t]);
@@ -532,81 +551,39 @@ __device__ __forceinline__ void moment_node(const MomArgs& A, int p, bool leaves
             cx = M > 0 ? sx / M : double(md.gx);
             cy = M > 0 ? sy / M : double(md.gy);
             cz = M > 0 ? sz / M : double(md.gz);
-            for (int j0 = nd.lo; j0 < hi; j0 += 4) {
+            auto quad = [&](double x, double y, double z, double m) {
+                const double dx = x - cx, dy = y - cy, dz = z - cz;
+                const double r2 = fma(dz, dz, fma(dx, dx, dy * dy));
+                q[0] = fma(m, fma(3 * dx, dx, -r2), q[0]);
+                q[3] = fma(m, fma(3 * dy, dy, -r2), q[3]);
+                q[5] += m * fma(3 * dz, dz, -r2);
+                const double m3 = m * 3;
+                q[1] = fma(m3 * dx, dy, q[1]);
+                q[2] = fma(m3 * dx, dz, q[2]);
+                q[4] += m3 * dy * dz;
+            };
+#pragma unroll
+            for (int t = 0; t < KB; ++t)
+                if (nd.lo + t < hi) {
+                    double x, y, z, m;
+                    held(t, x, y, z, m);
+                    quad(x, y, z, m);
+                }
+            for (int j0 = nd.lo + KB; j0 < hi; j0 += 4) {
                 double bx[4], by[4], bz[4], mm[4];
 #pragma unroll
                 for (int t = 0; t < 4; ++t) body(min(j0 + t, hi - 1), bx[t], by[t], bz[t], mm[t]);
 #pragma unroll
                 for (int t = 0; t < 4; ++t)
-                    if (j0 + t < hi) {
-                        const double m = mm[t];
-                        const double dx = bx[t] - cx, dy = by[t] - cy, dz = bz[t] - cz;
-                        const double r2 = fma(dz, dz, fma(dx, dx, dy * dy));
-                        q[0] = fma(m, fma(3 * dx, dx, -r2), q[0]);
-                        q[3] = fma(m, fma(3 * dy, dy, -r2), q[3]);
-                        q[5] += m * fma(3 * dz, dz, -r2);
-                        const double m3 = m * 3;
-                        q[1] = fma(m3 * dx, dy, q[1]);
-                        q[2] = fma(m3 * dx, dz, q[2]);
-                        q[4] += m3 * dy * dz;
-                    }
+                    if (j0 + t < hi) quad(bx[t], by[t], bz[t], mm[t]);
             }
         } else {
             const uint64_t pre = prefix[p];
             int ch = nd.first;
             // Children: the run of next-level nodes from `first` inside this cell (key prefix; over
             // key frames, the first bodies' common extended prefix). At most 8.
-            if (EXT) {
-                while (ch < cend && ext_common(F, nd.lo, node[ch].lo) >= level) ++ch;
-            } else {
-                uint64_t pr[8];
-#pragma unroll
-                for (int t = 0; t < 8; ++t) pr[t] = nd.first + t < cend ? prefix[nd.first + t] : 0ull;
-                bool run = true;
-#pragma unroll
-                for (int t = 0; t < 8; ++t) {
-                    run = run && nd.first + t < cend && (pr[t] >> 3) == pre;
-                    ch += run ? 1 : 0;
-                }
-            }
-            for (int k0 = nd.first; k0 < ch; k0 += 4) {
-                double m[4], x[4], y[4], z[4];
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const int k = min(k0 + t, ch - 1);
-                    if (MIXED) {
-                        const NodeMomD& c = momd[k];
-                        m[t] = c.m; x[t] = c.cx; y[t] = c.cy; z[t] = c.cz;
-                    } else {
-                        const NodeMom& c = mom[k];
-                        m[t] = c.m; x[t] = c.cx; y[t] = c.cy; z[t] = c.cz;
-                    }
-                }
-#pragma unroll
-                for (int t = 0; t < 4; ++t)
-                    if (k0 + t < ch) {
-                        M += m[t];
-                        sx = fma(m[t], x[t], sx);
-                        sy = fma(m[t], y[t], sy);
-                        sz = fma(m[t], z[t], sz);
-                    }
-            }
-            nchild = ch - nd.first;
-            nd.hi = node[ch - 1].hi;
-            cx = M > 0 ? sx / M : double(md.gx);
-            cy = M > 0 ? sy / M : double(md.gy);
-            cz = M > 0 ? sz / M : double(md.gz);
-            for (int k = nd.first; k < ch; ++k) {
-                double m, dx, dy, dz, cq[6];
-                if (MIXED) {
-                    const NodeMomD& c = momd[k];
-                    m = c.m; dx = c.cx - cx; dy = c.cy - cy; dz = c.cz - cz;
-                    for (int u = 0; u < 6; ++u) cq[u] = c.q[u];
-                } else {
-                    const NodeMom& c = mom[k];
-                    m = c.m; dx = c.cx - cx; dy = c.cy - cy; dz = c.cz - cz;
-                    cq[0] = c.qxx; cq[1] = c.qxy; cq[2] = c.qxz; cq[3] = c.qyy; cq[4] = c.qyz; cq[5] = c.qzz;
-                }
+            auto shifted = [&](double m, double ccx, double ccy, double ccz, const double (&cq)[6]) {
+                const double dx = ccx - cx, dy = ccy - cy, dz = ccz - cz;
                 const double r2 = fma(dz, dz, fma(dx, dx, dy * dy));
                 q[0] += cq[0] + m * (3 * dx * dx - r2);
                 q[3] += cq[3] + m * (3 * dy * dy - r2);
@@ -614,7 +591,76 @@ __device__ __forceinline__ void moment_node(const MomArgs& A, int p, bool leaves
                 q[1] += cq[1] + m * 3 * dx * dy;
                 q[2] += cq[2] + m * 3 * dx * dz;
                 q[4] += cq[4] + m * 3 * dy * dz;
+            };
+            auto child_at = [&](int k, double& m, double& ccx, double& ccy, double& ccz, double (&cq)[6]) {
+                if (MIXED) {
+                    const NodeMomD& c = momd[k];
+                    m = c.m; ccx = c.cx; ccy = c.cy; ccz = c.cz;
+                    for (int u = 0; u < 6; ++u) cq[u] = c.q[u];
+                } else {
+                    const NodeMom& c = mom[k];
+                    m = c.m; ccx = c.cx; ccy = c.cy; ccz = c.cz;
+                    cq[0] = c.qxx; cq[1] = c.qxy; cq[2] = c.qxz; cq[3] = c.qyy; cq[4] = c.qyz; cq[5] = c.qzz;
+                }
+            };
+            if (EXT) {
+                while (ch < cend && ext_common(F, nd.lo, node[ch].lo) >= level) ++ch;
+                for (int k = nd.first; k < ch; ++k) {
+                    double m, x, y, z, cq[6];
+                    child_at(k, m, x, y, z, cq);
+                    M += m;
+                    sx = fma(m, x, sx);
+                    sy = fma(m, y, sy);
+                    sz = fma(m, z, sz);
+                }
+                nd.hi = node[ch - 1].hi;
+                cx = M > 0 ? sx / M : double(md.gx);
+                cy = M > 0 ? sy / M : double(md.gy);
+                cz = M > 0 ? sz / M : double(md.gz);
+                for (int k = nd.first; k < ch; ++k) {
+                    double m, x, y, z, cq[6];
+                    child_at(k, m, x, y, z, cq);
+                    shifted(m, x, y, z, cq);
+                }
+            } else {
+                // The eight candidates' prefixes, body ends and whole records, loaded at once.
+                uint64_t pr[8];
+                int hi8[8];
+                double cm[8], ccx[8], ccy[8], ccz[8], cq[8][6];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const int k = min(nd.first + t, cend - 1);
+                    pr[t] = nd.first + t < cend ? prefix[k] : 0ull;
+                    hi8[t] = node[k].hi;
+                    child_at(k, cm[t], ccx[t], ccy[t], ccz[t], cq[t]);
+                }
+                bool run = true;
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    run = run && nd.first + t < cend && (pr[t] >> 3) == pre;
+                    ch += run ? 1 : 0;
+                }
+                const int nc = ch - nd.first;
+#pragma unroll
+                for (int t = 0; t < 8; ++t)
+                    if (t < nc) {
+                        M += cm[t];
+                        sx = fma(cm[t], ccx[t], sx);
+                        sy = fma(cm[t], ccy[t], sy);
+                        sz = fma(cm[t], ccz[t], sz);
+                    }
+                int h = hi8[0];
+#pragma unroll
+                for (int t = 1; t < 8; ++t) h = t < nc ? hi8[t] : h;
+                nd.hi = h;
+                cx = M > 0 ? sx / M : double(md.gx);
+                cy = M > 0 ? sy / M : double(md.gy);
+                cz = M > 0 ? sz / M : double(md.gz);
+#pragma unroll
+                for (int t = 0; t < 8; ++t)
+                    if (t < nc) shifted(cm[t], ccx[t], ccy[t], ccz[t], cq[t]);
             }
+            nchild = ch - nd.first;
         }
         if (MIXED) {
             NodeMomD d;
@@ -960,7 +1006,6 @@ MomArgs mom_args(const BuildBufs& b, bool mixed, float inv_theta) {
     if (mixed) {
         A.xd = b.xd; A.yd = b.yd; A.zd = b.zd; A.bm = b.m;
     }
-    A.bl = b.bl;
     A.levstart = b.levstart;
     A.prefix = b.prefix;
     A.node = b.node;
