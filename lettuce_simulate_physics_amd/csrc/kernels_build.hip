@@ -9,6 +9,7 @@
 #include "kernels.hpp"
 #include "morton.hpp"
 #include "tree.hpp"
+#include "wave.hpp"
 
 namespace lhip {
 namespace {
@@ -52,10 +53,14 @@ constexpr int kHalo = kGroupMax;  // 128
 // (-1 for i = 0).
 // cnt[i] = number of octree cells that start at body i = levels (c, min(L, 21)].
 // gflag[i] = 1 iff the group cell of body i starts at i.
+// Per tile of kLvTile bodies it also writes what the scan's top step reduces: the sums of cnt and
+// gflag over the tile (tsum[0][tile], tsum[1][tile]), its deepest leaf level (bdeep[tile]) and its
+// count of bodies in over-full finest cells (bdeep[tiles + tile]).
 __global__ void __launch_bounds__(kLvTile) levels_kernel(const uint64_t* __restrict__ key, int n, int gmax,
                                                          uint32_t* __restrict__ bl, uint32_t* __restrict__ cnt,
-                                                         uint32_t* __restrict__ gflag, WalkCounters* __restrict__ ctr,
-                                                         int* __restrict__ host_over, uint32_t* __restrict__ bdeep) {
+                                                         uint32_t* __restrict__ gflag, uint32_t* __restrict__ tsum0,
+                                                         uint32_t* __restrict__ tsum1, int* __restrict__ host_over,
+                                                         uint32_t* __restrict__ bdeep) {
     constexpr int kA = kHalo + kLvTile;  // window starts held per tile
     __shared__ uint64_t sk[kHalo + kLvTile + kHalo];
     // Window values as 1 + common octal prefix (0 where the window leaves the bodies), so that a
@@ -114,6 +119,7 @@ __global__ void __launch_bounds__(kLvTile) levels_kernel(const uint64_t* __restr
     const int i = base + tid;
     bool over = false;
     int deepest = 0;
+    uint32_t ci = 0, gi = 0;
     if (i < n) {
         // Leaf window starts [i - 16, i]: two overlapping 16-wide ranges.
         const int a = tid + kHalo - kLeafMax;
@@ -125,48 +131,40 @@ __global__ void __launch_bounds__(kLvTile) levels_kernel(const uint64_t* __restr
         const int Gc = min(Gw, Lc);
         const int c = i == 0 ? -1 : common_octs(sk[tid + kHalo - 1], sk[tid + kHalo]);
         bl[i] = uint32_t(L) | uint32_t(Gc) << 8 | uint32_t(c + 1) << 16;
-        cnt[i] = uint32_t(max(0, Lc - c));
-        gflag[i] = c < Gc ? 1u : 0u;
+        ci = uint32_t(max(0, Lc - c));
+        gi = c < Gc ? 1u : 0u;
+        cnt[i] = ci;
+        gflag[i] = gi;
         over = L > kMaxLevel;
         deepest = Lc;
     }
     const unsigned long long ov = __ballot(over);
-    if ((tid & 63) == 0 && ov) {
-        atomicAdd(&ctr->overfull, __popcll(ov));
-        if (host_over) __hip_atomic_store(host_over, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    if ((tid & 63) == 0 && ov && host_over) __hip_atomic_store(host_over, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    // Tile reductions: wave sums and maxima, then one plain store each per tile (no shared counter
+    // word that every wave would wait on); the scan's top step reduces the tiles.
+    __shared__ uint32_t wred[4][kLvTile / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        deepest = max(deepest, __shfl_xor(deepest, o, 64));
+        ci += __shfl_xor(ci, o, 64);
+        gi += __shfl_xor(gi, o, 64);
     }
-    // Deepest leaf level of the block: wave maxima, then one plain store per block (no shared
-    // counter word that every wave would wait on); the publish kernel reduces the blocks.
-    __shared__ int wdeep[kLvTile / 64];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) deepest = max(deepest, __shfl_xor(deepest, o, 64));
-    if ((tid & 63) == 0) wdeep[tid >> 6] = deepest;
-    __syncthreads();
-    if (tid == 0) {
-        int d = wdeep[0];
-#pragma unroll
-        for (int w = 1; w < kLvTile / 64; ++w) d = max(d, wdeep[w]);
-        bdeep[blockIdx.x] = uint32_t(d);
+    if ((tid & 63) == 0) {
+        const int w = tid >> 6;
+        wred[0][w] = uint32_t(deepest);
+        wred[1][w] = ci;
+        wred[2][w] = gi;
+        wred[3][w] = uint32_t(__popcll(ov));
     }
-}
-
-// Reduces the level pass's per-block deepest levels and publishes the result to the counters and
-// to the pinned host word the build waits on. One block of kPublish threads.
-constexpr int kPublish = 256;
-__global__ void __launch_bounds__(kPublish) publish_depth_kernel(const uint32_t* __restrict__ bdeep, int nblocks,
-                                                                  WalkCounters* __restrict__ ctr,
-                                                                  int* __restrict__ host_over) {
-    __shared__ int wdeep[kPublish / 64];
-    int d = 0;
-    for (int i = threadIdx.x; i < nblocks; i += kPublish) d = max(d, int(bdeep[i]));
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) d = max(d, __shfl_xor(d, o, 64));
-    if ((threadIdx.x & 63) == 0) wdeep[threadIdx.x >> 6] = d;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < kPublish / 64; ++w) d = max(d, wdeep[w]);
-        ctr->max_level = d;
-        __hip_atomic_store(host_over + 1, d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (tid < 4) {
+        uint32_t r = wred[tid][0];
+#pragma unroll
+        for (int w = 1; w < kLvTile / 64; ++w) r = tid == 0 ? max(r, wred[tid][w]) : r + wred[tid][w];
+        if (tid == 0) bdeep[blockIdx.x] = r;
+        if (tid == 1) tsum0[blockIdx.x] = r;
+        if (tid == 2) tsum1[blockIdx.x] = r;
+        if (tid == 3) bdeep[gridDim.x + blockIdx.x] = r;
     }
 }
 
@@ -208,19 +206,16 @@ constexpr int kScanThreads = 256;
 constexpr int kScanPT = 8;
 constexpr int kScanTile = kScanThreads * kScanPT;
 
-__device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t& total) {
-    const int tid = threadIdx.x;
-    sh[tid] = v;
-    __syncthreads();
-    for (int o = 1; o < kScanThreads; o <<= 1) {
-        const uint32_t a = tid >= o ? sh[tid - o] : 0u;
-        __syncthreads();
-        sh[tid] += a;
-        __syncthreads();
-    }
-    total = sh[kScanThreads - 1];
-    const uint32_t r = sh[tid] - v;
-    __syncthreads();
+constexpr int kScanWaves = kScanThreads / 64;
+
+// Block-wide exclusive scan (256 threads): wave scans, then the wave totals (wsum[2][4], used in
+// turn so that one barrier per scan suffices: a call's slot is rewritten two calls later, after
+// every thread has passed the barrier of the call in between).
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t (*wsum)[kScanWaves], int& turn,
+                                                         uint32_t& total) {
+    const int lane = int(threadIdx.x & 63), wave = int(threadIdx.x >> 6);
+    const uint32_t r = block_scan_excl<kScanWaves>(v, wsum[turn], lane, wave, &total);
+    turn ^= 1;
     return r;
 }
 
@@ -231,7 +226,8 @@ struct ScanArrays {
 };
 
 __global__ void __launch_bounds__(kScanThreads) scan_reduce(ScanArrays A, int n) {
-    __shared__ uint32_t sh[kScanThreads];
+    __shared__ uint32_t sh[2][kScanWaves];
+    int turn = 0;
     const uint32_t* __restrict__ a = A.a[blockIdx.y];
     const long long b0 = (long long)blockIdx.x * kScanTile + threadIdx.x * kScanPT;
     uint32_t s = 0;
@@ -239,27 +235,76 @@ __global__ void __launch_bounds__(kScanThreads) scan_reduce(ScanArrays A, int n)
     for (int k = 0; k < kScanPT; ++k)
         if (b0 + k < n) s += a[b0 + k];
     uint32_t total;
-    block_exclusive_scan(s, sh, total);
+    block_exclusive_scan(s, sh, turn, total);
     if (threadIdx.x == 0) A.bsum[blockIdx.y][blockIdx.x] = total;
 }
 
-__global__ void __launch_bounds__(kScanThreads) scan_top(ScanArrays A, int nb, int n) {
-    __shared__ uint32_t sh[kScanThreads];
+// What the level pass's scan also finishes (null: a plain scan): the tiles' deepest levels and
+// over-full counts (bdeep[0 .. tiles), bdeep[tiles .. 2 tiles)) to the counters and the pinned host
+// words (host[1] deepest level; host[2] the first array's total, the node count).
+struct ScanReport {
+    const uint32_t* bdeep;
+    int tiles;
+    WalkCounters* ctr;
+    int* host;
+};
+
+// One block per array: exclusive scan of the nb block sums in place (kTopPT consecutive sums per
+// thread per round), the array's total at a[n].
+constexpr int kTopPT = 16;
+__global__ void __launch_bounds__(kScanThreads) scan_top(ScanArrays A, int nb, int n, ScanReport R) {
+    __shared__ uint32_t sh[2][kScanWaves];
+    __shared__ uint32_t red[kScanWaves];
+    int turn = 0;
     uint32_t* __restrict__ bsum = A.bsum[blockIdx.x];
     uint32_t carry = 0;
-    for (int b = 0; b < nb; b += kScanThreads) {
-        const int t = b + threadIdx.x;
-        const uint32_t v = t < nb ? bsum[t] : 0u;
+    for (int b = 0; b < nb; b += kScanThreads * kTopPT) {
+        const int t0 = b + int(threadIdx.x) * kTopPT;
+        uint32_t v[kTopPT], sum = 0;
+#pragma unroll
+        for (int k = 0; k < kTopPT; ++k) {
+            v[k] = t0 + k < nb ? bsum[t0 + k] : 0u;
+            sum += v[k];
+        }
         uint32_t total;
-        const uint32_t e = block_exclusive_scan(v, sh, total);
-        if (t < nb) bsum[t] = carry + e;
+        uint32_t run = carry + block_exclusive_scan(sum, sh, turn, total);
+#pragma unroll
+        for (int k = 0; k < kTopPT; ++k) {
+            if (t0 + k < nb) bsum[t0 + k] = run;
+            run += v[k];
+        }
         carry += total;
     }
     if (threadIdx.x == 0) A.a[blockIdx.x][n] = carry;
+    if (!R.bdeep) return;
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0 && R.host) __hip_atomic_store(R.host + 2, int(carry), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        return;
+    }
+    uint32_t d = 0, over = 0;
+    for (int t = threadIdx.x; t < R.tiles; t += kScanThreads) {
+        d = max(d, R.bdeep[t]);
+        over += R.bdeep[R.tiles + t];
+    }
+    uint32_t total;
+    block_exclusive_scan(over, sh, turn, total);
+    over = total;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) d = max(d, __shfl_xor(d, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = d;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kScanWaves; ++w) d = max(d, red[w]);
+        R.ctr->max_level = int(d);
+        R.ctr->overfull = int(over);
+        if (R.host) __hip_atomic_store(R.host + 1, int(d), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
-__global__ void __launch_bounds__(kScanThreads) scan_down(ScanArrays A, int n) {
-    __shared__ uint32_t sh[kScanThreads];
+// stride: block sums per scan tile (1 after scan_reduce; kScanTile / kLvTile after the level pass).
+__global__ void __launch_bounds__(kScanThreads) scan_down(ScanArrays A, int n, int stride) {
+    __shared__ uint32_t sh[2][kScanWaves];
+    int turn = 0;
     uint32_t* __restrict__ a = A.a[blockIdx.y];
     const long long b0 = (long long)blockIdx.x * kScanTile + threadIdx.x * kScanPT;
     uint32_t v[kScanPT];
@@ -270,7 +315,7 @@ __global__ void __launch_bounds__(kScanThreads) scan_down(ScanArrays A, int n) {
         s += v[k];
     }
     uint32_t total;
-    uint32_t run = block_exclusive_scan(s, sh, total) + A.bsum[blockIdx.y][blockIdx.x];
+    uint32_t run = block_exclusive_scan(s, sh, turn, total) + A.bsum[blockIdx.y][std::size_t(blockIdx.x) * stride];
 #pragma unroll
     for (int k = 0; k < kScanPT; ++k) {
         if (b0 + k < n) a[b0 + k] = run;
@@ -285,9 +330,9 @@ void scan_exclusive_n(hipStream_t s, uint32_t* a0, uint32_t* a1, int count, int 
     ScanArrays A{{a0, a1}, {tmp, tmp + scan_tmp_entries(n)}};
     scan_reduce<<<dim3(nb, count), kScanThreads, 0, s>>>(A, n);
     LHIP_LAUNCH_CHECK();
-    scan_top<<<count, kScanThreads, 0, s>>>(A, nb, n);
+    scan_top<<<count, kScanThreads, 0, s>>>(A, nb, n, ScanReport{});
     LHIP_LAUNCH_CHECK();
-    scan_down<<<dim3(nb, count), kScanThreads, 0, s>>>(A, n);
+    scan_down<<<dim3(nb, count), kScanThreads, 0, s>>>(A, n, 1);
     LHIP_LAUNCH_CHECK();
 }
 
@@ -315,29 +360,6 @@ __global__ void __launch_bounds__(kBS) enumerate_kernel(int n, const uint32_t* _
         if (k >= ncap) break;
         pkey[k] = uint64_t(l);
         pval[k] = k;
-    }
-}
-
-// The node count (the pre-order scan's total) to the pinned host word the build waits on.
-__global__ void publish_count_kernel(const uint32_t* __restrict__ M_dev, int* __restrict__ host_word) {
-    __hip_atomic_store(host_word, int(*M_dev), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-}
-
-// inv[perm[p]] = p; level starts from the level totals (the radix digit counts).
-__global__ void __launch_bounds__(kBS) invert_kernel(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ M_dev,
-                                                     uint32_t* __restrict__ inv, const uint32_t* __restrict__ level_tot,
-                                                     uint32_t* __restrict__ levstart, uint32_t ncap) {
-    const int M = int(min(*M_dev, ncap));
-    const int p = blockIdx.x * kBS + threadIdx.x;
-    if (p < M) inv[perm[p]] = uint32_t(p);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        uint32_t s = 0;
-        for (int l = 0; l < kDepthLevels; ++l) {
-            levstart[l] = s;
-            s += level_tot[l];
-        }
-        levstart[kDepthLevels] = s;
-        levstart[kDepthLevels + 1] = s;
     }
 }
 
@@ -889,13 +911,8 @@ Frames frames_of(const BuildBufs& b) { return Frames{b.key, b.fkey, b.frames > 1
 }  // namespace
 
 void tree_levels(hipStream_t s, const BuildBufs& b) {
-    const int n = b.n;
-    if (b.frames > 1 || b.cap)
-        levels_ext_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(frames_of(b), b.cap, b.gmax, b.bl, b.cnt, b.gflag,
-                                                           b.counters, b.host_over);
-    else
-        levels_kernel<<<grid_for(n, kLvTile), kLvTile, 0, s>>>(b.key, n, b.gmax, b.bl, b.cnt, b.gflag, b.counters,
-                                                               b.host_over, b.gstart);
+    levels_ext_kernel<<<grid_for(b.n, kBS), kBS, 0, s>>>(frames_of(b), b.cap, b.gmax, b.bl, b.cnt, b.gflag, b.counters,
+                                                        b.host_over);
     LHIP_LAUNCH_CHECK();
 }
 
@@ -974,9 +991,13 @@ void fill_u32(hipStream_t s, uint32_t* a, uint32_t v, long long n) {
 }
 
 // Launch grids of the multipole pass: the leaves of every level at once, then one level of internal
-// nodes per launch (grid-stride loops, so any grid covers the nodes).
+// nodes per launch (grid-stride loops, so any grid covers the nodes). Level l holds at most 8^l
+// nodes, so the top levels take a block or a few.
 inline unsigned leaf_grid(int n) { return std::min<unsigned>(grid_for(n, kBS), 2048u); }
-inline unsigned level_grid(int n) { return std::min<unsigned>(grid_for(n, kBS), 512u); }
+inline unsigned level_grid(int n, int level) {
+    const long long most = level < 10 ? 1ll << (3 * level) : (long long)n;
+    return std::min<unsigned>(grid_for(std::min<long long>(n, most), kBS), 512u);
+}
 
 void unit_order(hipStream_t s, const BuildBufs& b, const uint32_t* ustart, const uint32_t* ucount, float thr,
                 uint32_t* order) {
@@ -986,15 +1007,20 @@ void unit_order(hipStream_t s, const BuildBufs& b, const uint32_t* ustart, const
 }
 
 // The multipole launches over a built level order: every leaf in one launch, then the internal
-// levels deepest first, one launch each. A level costs its chain of dependent loads (node record,
-// child prefixes, child moments), not its launch: levels 4..0 finished by one workgroup with
-// barriers between them took 51.6 us at 2^20 bodies against about 35 us as five launches.
+// levels deepest first, one launch each (at 2^20 bodies the ten internal levels hold 30,653 nodes,
+// at most 12,105 in one level, so each launch costs its two dependent round trips and its start).
+// Measured against the alternatives at 2^20 bodies: levels 4..0 finished by one workgroup with
+// barriers between them took 51.6 us against about 35 us as five launches; and a single climb
+// kernel (every leaf thread arrives at its parent through an acquire-release atomic at agent
+// scope, the last child computes the parent and climbs on) took 356 us against about 95 us for
+// these eleven launches: every arrival writes back and invalidates its XCD's L2 (buffer_wbl2 sc1,
+// buffer_inv sc1), and thousands of those serialise.
 template <bool MIXED, bool KEEP, bool EXT>
 void launch_moments(hipStream_t s, const MomArgs& A, int depth) {
     moments_level<MIXED, KEEP, EXT><<<leaf_grid(A.n), kBS, 0, s>>>(kMomLeaves, depth, A);
     LHIP_LAUNCH_CHECK();
     for (int l = depth - 1; l >= 0; --l) {
-        moments_level<MIXED, KEEP, EXT><<<level_grid(A.n), kBS, 0, s>>>(kMomInternal, l, A);
+        moments_level<MIXED, KEEP, EXT><<<level_grid(A.n, l), kBS, 0, s>>>(kMomInternal, l, A);
         LHIP_LAUNCH_CHECK();
     }
 }
@@ -1026,8 +1052,10 @@ void tree_moments_keep(hipStream_t s, const BuildBufs& b) {
 }
 
 
+// Block sums of a scan: one per kLvTile entries (the level pass's tiles; a plain scan uses one per
+// kScanTile), plus the total.
 std::size_t scan_tmp_entries(int n) {
-    return std::size_t((static_cast<long long>(n) + kScanTile) / kScanTile) + 1;
+    return std::size_t((static_cast<long long>(n) + kLvTile) / kLvTile) + 1;
 }
 
 void scan_u32(hipStream_t s, uint32_t* a, int n, uint32_t* tmp) { scan_exclusive(s, a, n, tmp); }
@@ -1035,30 +1063,37 @@ void scan_u32(hipStream_t s, uint32_t* a, int n, uint32_t* tmp) { scan_exclusive
 int tree_build(hipStream_t s, const BuildBufs& b, float inv_theta, hipEvent_t levels_done) {
     const int n = b.n;
     const bool ext = b.frames > 1;
-    reset_build<<<1, 64, 0, s>>>(b.counters);
-    LHIP_LAUNCH_CHECK();
     if (b.gmax < kLeafMax || b.gmax > kGroupMax) throw Error(kInvalid, "walk group size must be in [16, 128]");
-    tree_levels(s, b);
     const bool wait_levels = levels_done && b.host_over && !ext;
-    if (wait_levels) {
-        publish_depth_kernel<<<1, kPublish, 0, s>>>(b.gstart, grid_for(n, kLvTile), b.counters, b.host_over);
+    if (ext) {
+        reset_build<<<1, 64, 0, s>>>(b.counters);
+        LHIP_LAUNCH_CHECK();
+        tree_levels(s, b);
+        scan_exclusive_n(s, b.cnt, b.gflag, 2, n, b.scan_tmp);  // node counts and group starts
+    } else {
+        // The level pass with the first step of the scans of node counts and group starts (its tile
+        // sums), then the scans' top step, which also reports the depth and the node count.
+        const int tiles = int(grid_for(n, kLvTile));
+        const ScanArrays A{{b.cnt, b.gflag}, {b.scan_tmp, b.scan_tmp + scan_tmp_entries(n)}};
+        levels_kernel<<<tiles, kLvTile, 0, s>>>(b.key, n, b.gmax, b.bl, b.cnt, b.gflag, A.bsum[0], A.bsum[1],
+                                                b.host_over, b.gstart);
+        LHIP_LAUNCH_CHECK();
+        scan_top<<<2, kScanThreads, 0, s>>>(A, tiles, n,
+                                            ScanReport{b.gstart, tiles, b.counters, wait_levels ? b.host_over : nullptr});
+        LHIP_LAUNCH_CHECK();
+        scan_down<<<dim3(grid_for(n, kScanTile), 2), kScanThreads, 0, s>>>(A, n, kScanTile / kLvTile);
         LHIP_LAUNCH_CHECK();
     }
-    scan_exclusive_n(s, b.cnt, b.gflag, 2, n, b.scan_tmp);  // node counts and group starts
     const uint32_t* M_dev = b.cnt + n;  // node count
-    if (wait_levels) {
-        publish_count_kernel<<<1, 1, 0, s>>>(M_dev, b.host_over + 2);
-        LHIP_LAUNCH_CHECK();
-    }
     if (levels_done) LHIP_CHECK(hipEventRecord(levels_done, s));
     const uint32_t ncap = uint32_t(b.cap_nodes);
     enumerate_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(n, b.bl, b.cnt, b.pkey, b.pval, ext ? b.cap : nullptr, ncap);
     LHIP_LAUNCH_CHECK();
     // Stable sort of the pre-order nodes by level (one 8-bit pass: levels <= kMaxDepth < 256).
-    const uint32_t* level_tot =
-        radix_sort(s, *b.radix, RadixBufs{{b.pkey, b.pkey2}, {b.pval, b.pval2}}, false, b.cap_nodes, M_dev, 0, 1);
-    invert_kernel<<<grid_for(b.cap_nodes, kBS), kBS, 0, s>>>(b.pval2, M_dev, b.inv, level_tot, b.levstart, ncap);
-    LHIP_LAUNCH_CHECK();
+    // Its scatter also writes the inverse (inv[pre-order index] = level-order index) and the level
+    // starts (the digit starts: levels run 0 .. kMaxDepth, so starts kMaxDepth + 1 on are the count).
+    radix_sort(s, *b.radix, RadixBufs{{b.pkey, b.pkey2}, {b.pval, b.pval2}}, false, b.cap_nodes, M_dev, 0, 1, false,
+               b.inv, b.levstart, kDepthLevels + 2);
     if (ext)
         nodes_kernel<true><<<grid_for(n, kBS), kBS, 0, s>>>(b.key, n, b.bl, b.cnt, b.inv, b.cube, b.node, b.mom,
                                                             b.prefix, frames_of(b), b.cap, b.fseg, b.fcube, ncap);

@@ -105,24 +105,6 @@ __device__ __forceinline__ void wave_count(const int (&dg)[kSortKPT], const bool
     }
 }
 
-// Exclusive scan of one value per thread over the block (256 threads: wave scans, then the wave
-// totals through wsum, one barrier); *total gets the block sum. wsum must not be rewritten before
-// every thread has passed the next barrier.
-__device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t* wsum, int lane, int wave, uint32_t* total) {
-    const uint32_t incl = uint32_t(prefix_sum_incl(int(v), lane));
-    if (lane == 63) wsum[wave] = incl;
-    __syncthreads();
-    uint32_t off = 0, all = 0;
-#pragma unroll
-    for (int w = 0; w < kSortWaves; ++w) {
-        const uint32_t t = wsum[w];
-        off += w < wave ? t : 0u;
-        all += t;
-    }
-    if (total) *total = all;
-    return off + incl - v;
-}
-
 // The tile's digit counts written to hist[d * ntiles + tile] (counts from the waves' rows wc).
 __device__ __forceinline__ void tile_counts_out(uint32_t (*wc)[kRadix], uint32_t* __restrict__ hist, int ntiles) {
     uint32_t c = 0;
@@ -181,7 +163,7 @@ __global__ void __launch_bounds__(kSortThreads) radix_row_scan(uint32_t* __restr
             sum += v[j];
         }
         uint32_t chunk = 0;
-        uint32_t run = carry + block_scan_excl(sum, wsum[parity], lane, wave, &chunk);
+        uint32_t run = carry + block_scan_excl<kSortWaves>(sum, wsum[parity], lane, wave, &chunk);
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             const int t = b + tid * K + j;
@@ -194,13 +176,16 @@ __global__ void __launch_bounds__(kSortThreads) radix_row_scan(uint32_t* __restr
     if (tid == 0) tot[blockIdx.x] = carry;
 }
 
-// vin null: the values are the input positions.
+// vin null: the values are the input positions. inv (optional): inv[value] = its sorted position;
+// starts (optional): block 0 writes the digit starts 0 .. nstarts - 1.
 __global__ void __launch_bounds__(kSortThreads) radix_scatter(const uint64_t* __restrict__ kin,
                                                               const uint32_t* __restrict__ vin,
                                                               uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                               int n, const uint32_t* __restrict__ n_dev, int shift,
                                                               int ntiles, const uint32_t* __restrict__ hist,
-                                                              const uint32_t* __restrict__ tot) {
+                                                              const uint32_t* __restrict__ tot,
+                                                              uint32_t* __restrict__ inv, uint32_t* __restrict__ starts,
+                                                              int nstarts) {
     __shared__ uint64_t sk[kSortTile];
     __shared__ uint32_t sv[kSortTile];
     __shared__ uint32_t wc[kSortWaves][kRadix];   // the waves' digit counts, then their tile offsets
@@ -234,14 +219,15 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter(const uint64_t* __
     __syncthreads();
     wave_rank(dg, pos, wc[wave], lane);
     // Global digit starts: exclusive scan of the digit totals (its barrier also completes the ranks).
-    const uint32_t dbase = block_scan_excl(my_tot, wsum[0], lane, wave, nullptr);
+    const uint32_t dbase = block_scan_excl<kSortWaves>(my_tot, wsum[0], lane, wave, nullptr);
+    if (starts && blockIdx.x == 0 && tid < nstarts) starts[tid] = dbase;
     uint32_t c[kSortWaves], all = 0;
 #pragma unroll
     for (int w = 0; w < kSortWaves; ++w) {
         c[w] = wc[w][tid];
         all += c[w];
     }
-    const uint32_t ds = block_scan_excl(all, wsum[1], lane, wave, nullptr);
+    const uint32_t ds = block_scan_excl<kSortWaves>(all, wsum[1], lane, wave, nullptr);
     {
         uint32_t o = ds;  // this thread's column of wc only: no other thread reads it before the barrier
 #pragma unroll
@@ -270,6 +256,7 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter(const uint64_t* __
             const uint32_t dst = gbase[d] + (uint32_t(p) - dstart[d]);
             kout[dst] = kk;
             vout[dst] = sv[p];
+            if (inv) inv[sv[p]] = dst;
         }
     }
 }
@@ -358,8 +345,9 @@ void sort_impl(hipStream_t s, TreeBufs& b, int n, bool permute_cost) {
 std::size_t radix_work_words(int n) { return std::size_t(std::max(sort_tiles(n), 1)) * kRadix + kRadix; }
 
 const uint32_t* radix_sort(hipStream_t s, RadixWork& w, RadixBufs b, bool iota_values, int n, const uint32_t* n_dev,
-                           int d0, int d1, bool first_counted) {
-    if (d0 < 0 || d1 > kRadixDigits || d0 > d1) throw Error(kInvalid, "radix_sort: digits out of range");
+                           int d0, int d1, bool first_counted, uint32_t* inv, uint32_t* starts, int nstarts) {
+    if (d0 < 0 || d1 > kRadixDigits || d0 > d1 || nstarts < 0 || nstarts > kRadix)
+        throw Error(kInvalid, "radix_sort: digits or starts out of range");
     const int ntiles = sort_tiles(n);
     uint32_t* hist = w.words;
     uint32_t* tot = w.words + std::size_t(ntiles) * kRadix;
@@ -372,8 +360,10 @@ const uint32_t* radix_sort(hipStream_t s, RadixWork& w, RadixBufs b, bool iota_v
         }
         radix_row_scan<<<kRadix, kSortThreads, 0, s>>>(hist, ntiles, tot, n, n_dev);
         LHIP_LAUNCH_CHECK();
+        const bool last = d + 1 == d1;
         radix_scatter<<<ntiles, kSortThreads, 0, s>>>(b.k[src], j == 0 && iota_values ? nullptr : b.v[src], b.k[dst],
-                                                      b.v[dst], n, n_dev, 8 * d, ntiles, hist, tot);
+                                                      b.v[dst], n, n_dev, 8 * d, ntiles, hist, tot,
+                                                      last ? inv : nullptr, last ? starts : nullptr, nstarts);
         LHIP_LAUNCH_CHECK();
     }
     return tot;

@@ -21,6 +21,25 @@ __device__ __forceinline__ int prefix_sum_incl(int v, int lane) {
     return v + (row == 0 ? 0 : row == 1 ? r0 : row == 2 ? r0 + r1 : r0 + r1 + r2);
 }
 
+// Exclusive scan of one value per thread over a block of W waves (wave scans, then the wave totals
+// through wsum[W], one barrier); *total gets the block sum. wsum must not be rewritten before every
+// thread has passed the next barrier.
+template <int W>
+__device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t* wsum, int lane, int wave, uint32_t* total) {
+    const uint32_t incl = uint32_t(prefix_sum_incl(int(v), lane));
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t off = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        const uint32_t t = wsum[w];
+        off += w < wave ? t : 0u;
+        all += t;
+    }
+    if (total) *total = all;
+    return off + incl - v;
+}
+
 // Sum of a float over all 64 lanes (every lane active), returned wave-uniform: butterflies within
 // each row of 16 (quad swaps, half-row and row mirrors), then the four row sums.
 template <int CTRL>
