@@ -94,52 +94,32 @@ __device__ __forceinline__ void wave_rank(const int (&dg)[kSortKPT], uint32_t (&
     }
 }
 
-// The wave's digit counts of the keys whose ok bit is set (wc as in wave_rank).
-__device__ __forceinline__ void wave_count(const int (&dg)[kSortKPT], const bool (&ok)[kSortKPT], uint32_t* wc,
-                                           int lane) {
-    const uint64_t lt = (1ull << lane) - 1ull;
-#pragma unroll
-    for (int r = 0; r < kSortKPT; ++r) {
-        const uint64_t peers = digit_peers(dg[r]) & __ballot(ok[r]);
-        if (ok[r] && (peers & lt) == 0) wc[dg[r]] += uint32_t(__popcll(peers));
-    }
-}
-
-// The tile's digit counts written to hist[d * ntiles + tile] (counts from the waves' rows wc).
-__device__ __forceinline__ void tile_counts_out(uint32_t (*wc)[kRadix], uint32_t* __restrict__ hist, int ntiles) {
-    uint32_t c = 0;
-#pragma unroll
-    for (int w = 0; w < kSortWaves; ++w) c += wc[w][threadIdx.x];
-    hist[std::size_t(threadIdx.x) * ntiles + blockIdx.x] = c;
-}
-
 // n_dev (optional) holds the element count on the device; n is then only the capacity the grid
 // was sized for.
+// The digit being counted is effectively random in the order a pass reads (the keys are sorted by
+// the lower digits by then, or near their previous order for the first), so LDS atomics on one
+// histogram rarely collide within a wave.
 __global__ void __launch_bounds__(kSortThreads) radix_count(const uint64_t* __restrict__ key, int n,
                                                             const uint32_t* __restrict__ n_dev, int shift, int ntiles,
                                                             uint32_t* __restrict__ hist) {
-    __shared__ uint32_t wc[kSortWaves][kRadix];
-    const int tid = int(threadIdx.x), lane = tid & 63, wave = tid >> 6;
+    __shared__ uint32_t h[kRadix];
+    const int tid = int(threadIdx.x);
     if (n_dev) n = min(n, int(*n_dev));
     const int base = blockIdx.x * kSortTile;
     if (base >= n) return;  // a tile past the elements (n_dev below the capacity): never scanned
     uint64_t k[kSortKPT];
-    bool ok[kSortKPT];
 #pragma unroll
     for (int r = 0; r < kSortKPT; ++r) {
-        const int i = base + tile_slot(wave, r, lane);
-        ok[r] = i < n;
-        k[r] = ok[r] ? key[i] : 0ull;
+        const int i = base + r * kSortThreads + tid;
+        k[r] = i < n ? key[i] : 0ull;
     }
-#pragma unroll
-    for (int w = 0; w < kSortWaves; ++w) wc[w][tid] = 0;
+    h[tid] = 0;
     __syncthreads();
-    int dg[kSortKPT];
 #pragma unroll
-    for (int r = 0; r < kSortKPT; ++r) dg[r] = int((k[r] >> shift) & 0xff);
-    wave_count(dg, ok, wc[wave], lane);
+    for (int r = 0; r < kSortKPT; ++r)
+        if (base + r * kSortThreads + tid < n) atomicAdd(&h[(k[r] >> shift) & 0xff], 1u);
     __syncthreads();
-    tile_counts_out(wc, hist, ntiles);
+    hist[std::size_t(tid) * ntiles + blockIdx.x] = h[tid];
 }
 
 // One block per digit: exclusive scan of hist[d][0..tiles) in place, over the tiles that hold
@@ -261,39 +241,37 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter(const uint64_t* __
     }
 }
 
-// Morton keys of the body sort, tile by tile, with digit 0 of each tile counted for its first pass.
+// Morton keys of the body sort, tile by tile, with digit 0 of each tile counted for its first pass
+// (LDS atomics, as radix_count).
 template <class R>
 __global__ void __launch_bounds__(kSortThreads) keys_count_kernel(const R* __restrict__ x, const R* __restrict__ y,
                                                                   const R* __restrict__ z,
                                                                   const Cube* __restrict__ cube,
                                                                   uint64_t* __restrict__ key, int n, int ntiles,
                                                                   uint32_t* __restrict__ hist) {
-    __shared__ uint32_t wc[kSortWaves][kRadix];
-    const int tid = int(threadIdx.x), lane = tid & 63, wave = tid >> 6;
+    __shared__ uint32_t h[kRadix];
+    const int tid = int(threadIdx.x);
     const int base = blockIdx.x * kSortTile;
     const double ox = cube->ox, oy = cube->oy, oz = cube->oz, scale = cube->scale;
     R px[kSortKPT], py[kSortKPT], pz[kSortKPT];
-    bool ok[kSortKPT];
 #pragma unroll
     for (int r = 0; r < kSortKPT; ++r) {
-        const int i = base + tile_slot(wave, r, lane);
-        ok[r] = i < n;
-        const int j = ok[r] ? i : n - 1;
+        const int j = min(base + r * kSortThreads + tid, n - 1);
         px[r] = x[j]; py[r] = y[j]; pz[r] = z[j];
     }
-#pragma unroll
-    for (int w = 0; w < kSortWaves; ++w) wc[w][tid] = 0;
-    int dg[kSortKPT];
+    h[tid] = 0;
+    __syncthreads();
 #pragma unroll
     for (int r = 0; r < kSortKPT; ++r) {
-        const uint64_t kk = morton_key(px[r], py[r], pz[r], ox, oy, oz, scale);
-        if (ok[r]) key[base + tile_slot(wave, r, lane)] = kk;
-        dg[r] = int(kk & 0xff);
+        const int i = base + r * kSortThreads + tid;
+        if (i < n) {
+            const uint64_t kk = morton_key(px[r], py[r], pz[r], ox, oy, oz, scale);
+            key[i] = kk;
+            atomicAdd(&h[kk & 0xff], 1u);
+        }
     }
     __syncthreads();
-    wave_count(dg, ok, wc[wave], lane);
-    __syncthreads();
-    tile_counts_out(wc, hist, ntiles);
+    hist[std::size_t(tid) * ntiles + blockIdx.x] = h[tid];
 }
 
 // ---------------------------------------------------------------- permute
