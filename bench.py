@@ -28,22 +28,52 @@ N_BODIES = 1 << 20
 THETA, EPS, DT = 0.6, 0.01, 1e-3
 FLOP_P2P, FLOP_P2C = 20, 51
 PEAK_FP32 = 157.3e12  # MI355X published FP32 vector peak (packed issue)
-# Issue-slot ceiling of the pair loops: one wave64 VALU instruction per SIMD every 4 cycles, i.e.
-# 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz lane-instructions per second. The inner loops of the built
-# kernels take 12 VALU instructions per direct pair, about 13.3 per walk body pair and 32.3 per walk
-# cell term (gfx950 ISA of the 1-3 slot loops); v_rsq_f32 runs on the transcendental unit at half
-# rate beside the VALU stream (a 12 FMA + 1 rsq chain runs at the rate of its 12 FMAs alone), so it
-# takes no VALU slot.
-VALU_LANE_RATE = 256 * 4 * 16 * 2.4e9
+# Issue-slot ceiling of the pair loops. VALU instructions per interaction, read from the built
+# kernels' gfx950 ISA (the 1-3 slot loops): 12 per direct pair, about 13.3 per walk body pair and
+# 32.3 per walk cell term, each besides one v_rsq_f32. What an instruction slot and an rsq cost is
+# measured live by the issue probe (lettuce_simulate_physics_amd/csrc/probe/issue_probe.hip): wave64
+# streams of independent FMAs give the device's lane-instruction rate, and a stream of 12 FMAs plus
+# one v_rsq_f32 per step against the 12 FMAs alone gives the cost of an rsq in FMA slots (0 when it
+# issues beside the FMA stream).
 VALU_DIRECT, VALU_P2P, VALU_P2C = 12.0, 13.3, 32.3
+NOMINAL_LANE_RATE = 256 * 4 * 16 * 2.4e9  # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+
+_probe = None
+
+
+def issue_probe() -> dict:
+    """The issue probe's measurements (run once per process)."""
+    global _probe
+    if _probe is None:
+        import ctypes
+
+        lib = ctypes.CDLL(os.path.join(ROOT, "lettuce_simulate_physics_amd", "libissue_probe.so"))
+        lib.issue_probe_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        blocks, steps, chains = 256 * 8, 4096, 12
+        ms = (ctypes.c_double * 3)()
+        if lib.issue_probe_run(blocks, steps, ms) != 0:
+            raise RuntimeError("issue probe failed")
+        fma, rsq, mix = ms[0], ms[1], ms[2]
+        lanes = blocks * 256 * steps * chains
+        _probe = {"fma_ms": round(fma, 4), "rsq_ms": round(rsq, 4), "mix_ms": round(mix, 4),
+                  "lane_rate_tps": round(lanes / (fma * 1e-3) / 1e12, 3),
+                  "nominal_lane_rate_tps": round(NOMINAL_LANE_RATE / 1e12, 3),
+                  "rsq_throughput_vs_fma": round(fma / rsq, 3),
+                  "rsq_cost_fma_slots": round(max(0.0, chains * (mix - fma) / fma), 3),
+                  "blocks": blocks, "steps": steps, "chains": chains}
+    return _probe
 
 
 def issue_ceiling(n_pp: float, n_pc: float, valu_pp: float, seconds: float) -> dict:
-    """FP32 rate the instruction mix allows at full VALU issue, and the share of it achieved."""
-    t_min = (valu_pp * n_pp + VALU_P2C * n_pc) / VALU_LANE_RATE
+    """FP32 rate the instruction mix allows at the probe's issue rate, and the share of it achieved."""
+    P = issue_probe()
+    rsq = P["rsq_cost_fma_slots"]
+    slots = (valu_pp + rsq) * n_pp + (VALU_P2C + rsq) * n_pc
+    t_min = slots / (P["lane_rate_tps"] * 1e12)
     flops = FLOP_P2P * n_pp + FLOP_P2C * n_pc
-    return {"valu_per_p2p": valu_pp, "valu_per_p2c": VALU_P2C, "clock_ghz": 2.4,
-            "ceiling": round(flops / t_min / 1e12, 2), "unit": "TFLOP/s", "frac": round(t_min / seconds, 4)}
+    return {"valu_per_p2p": valu_pp, "valu_per_p2c": VALU_P2C, "rsq_cost_fma_slots": rsq,
+            "lane_rate_tps": P["lane_rate_tps"], "ceiling": round(flops / t_min / 1e12, 2), "unit": "TFLOP/s",
+            "frac": round(t_min / seconds, 4)}
 
 
 def plummer(n: int, seed: int = 1):
@@ -492,11 +522,16 @@ def main():
                     help="sizes of the like-for-like leg against the published numbers ('' skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy 3 times (HBM counter calibration)")
+    ap.add_argument("--issue-probe", action="store_true", help="only run the issue-rate probe and print it")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.issue_probe:
+        print(json.dumps(issue_probe()), flush=True)
+        return
 
     import torch
 
@@ -555,6 +590,7 @@ def main():
                      "traffic": traffic,
                      "issue_ceiling": issue_ceiling(r["p2p"] / k, r["p2c"] / k, VALU_P2P, walk_ms * 1e-3)},
         "hbm_phases": hbm_phases(r, k, args.n, be.tree_counts()[0]),
+        "issue_probe": issue_probe(),
         "cpu_baseline": None,
         "mixed": None,
         "disk": None,

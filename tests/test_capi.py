@@ -44,6 +44,14 @@ def test_library_is_built_for_gfx950():
     assert b"gfx950" in data
 
 
+def test_issue_probe_library_is_built_for_gfx950():
+    # The issue-rate probe bench.py runs beside the force path: its own library, entry issue_probe_run.
+    path = os.path.join(ROOT, "lettuce_simulate_physics_amd", "libissue_probe.so")
+    lib = ctypes.CDLL(path)
+    assert hasattr(lib, "issue_probe_run")
+    assert b"gfx950" in open(path, "rb").read()
+
+
 def test_memory_need_is_host_only_and_monotone():
     d1, h1 = L.HipBackend.memory_need(1 << 20, "tree")
     d2, _ = L.HipBackend.memory_need(1 << 22, "tree")
