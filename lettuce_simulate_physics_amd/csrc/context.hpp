@@ -71,6 +71,7 @@ struct Context {
     // stay in m and id; the float position arrays are then only download staging.
     bool mixed = false;
     lhip_walk_options wopt{kGroupMax, kSmallLeaf, 0.0};  // the reference's walk
+    int env_group = 0;  // LETTUCE_HIP_GROUP, 0 when unset
     DevBuf xd, yd, zd, vxd, vyd, vzd, xd2, yd2, zd2, vxd2, vyd2, vzd2, axd, ayd, azd, potd, momd, rec64, posd;
 
     // Frozen-topology replay of the differentiable tree simulation: group boxes from these recorded
@@ -93,6 +94,25 @@ struct Context {
     hipEvent_t ev[10] = {};  // 0 build start, 1 sorted, 2 built, 3 walk start, 4 walk end, 5-6 kick, 7-8 drift
 
     explicit Context(int device) : dev(device) {
+        // Environment knobs, as the reference's CUDA backend reads LETTUCE_CUDA_WALK and
+        // LETTUCE_CUDA_GROUP (src/cuda_backend.cu:1239-1254): LETTUCE_HIP_WALK=fused|lists picks the
+        // local walk (one fused kernel, or a list traversal and a list evaluation); LETTUCE_HIP_GROUP
+        // (16 .. 128, default 128) is the walk group size of both forms' default options. Other values
+        // are errors (read first, before any device resource is taken).
+        if (const char* w = std::getenv("LETTUCE_HIP_WALK")) {
+            const std::string v = w;
+            if (v != "fused" && v != "lists")
+                throw Error(kInvalid, "LETTUCE_HIP_WALK must be fused or lists, not '" + v + "'");
+            lists_plain = lists_mixed = v == "lists";
+        }
+        if (const char* g = std::getenv("LETTUCE_HIP_GROUP")) {
+            char* end = nullptr;
+            const long v = std::strtol(g, &end, 10);
+            if (!*g || *end || v < kLeafMax || v > kGroupMax)
+                throw Error(kInvalid, std::string("LETTUCE_HIP_GROUP must be a group size in [16, 128], not '") + g + "'");
+            env_group = int(v);
+            wopt.group_max = env_group;
+        }
         LHIP_CHECK(hipSetDevice(dev));
         hipDeviceProp_t prop{};
         LHIP_CHECK(hipGetDeviceProperties(&prop, dev));
@@ -112,11 +132,8 @@ struct Context {
         counters.reserve(sizeof(WalkCounters));
         walk_blocks = cus * walk_blocks_per_cu();
         list_blocks = cus * walk_lists_blocks_per_cu();
-        if (const char* w = std::getenv("LETTUCE_HIP_WALK")) {
-            const std::string v = w;
-            if (v == "lists") lists_plain = lists_mixed = true;
-            else if (v == "fused") lists_plain = lists_mixed = false;
-        }
+        name += lists_plain ? ", list walk" : ", fused walk";
+        name += ", group " + std::to_string(wopt.group_max);
         LHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_over), 4 * sizeof(int), hipHostMallocDefault));
         host_over[0] = host_over[1] = host_over[2] = host_over[3] = 0;
     }
@@ -150,8 +167,9 @@ struct Context {
     double* D(DevBuf& b) { return b.as<double>(); }
     float* F(DevBuf& b) { return b.as<float>(); }
 
-    static lhip_walk_options default_options(bool mix) {
-        return mix ? lhip_walk_options{kGroupMax, 1, 0.05} : lhip_walk_options{kGroupMax, kSmallLeaf, 0.0};
+    lhip_walk_options default_options(bool mix) const {
+        const int g = env_group ? env_group : kGroupMax;
+        return mix ? lhip_walk_options{g, 1, 0.05} : lhip_walk_options{g, kSmallLeaf, 0.0};
     }
 
     // Switches between the plain (FP32 state) and mixed-precision (FP64 state) forms, converting the

@@ -600,6 +600,39 @@ def test_c1_million_parity_with_reference(hip):
     assert np.median(e) <= 1e-3
 
 
+def test_environment_walk_knobs(monkeypatch):
+    # LETTUCE_HIP_GROUP sets the walk group size of the default options (as LETTUCE_CUDA_GROUP,
+    # src/cuda_backend.cu:1248-1252): a smaller group's box opens fewer cells, so its bodies take
+    # fewer interactions (more cells accepted farther from each body), and the forces stay within
+    # the walk's accuracy of the FP64 direct sum. Values outside [16, 128], and LETTUCE_HIP_WALK
+    # values other than fused or lists, make creation fail with the reason.
+    b = O.plummer(20000, 41)
+    p = P("tree", 0.01, 0.6)
+    idx = np.arange(0, len(b), 97)
+    a64, _ = O.direct_f64(b, 1.0, 0.01, idx)
+    counts = {}
+    for g in ("128", "64", "16"):
+        monkeypatch.setenv("LETTUCE_HIP_GROUP", g)
+        be = L.HipBackend(0)
+        try:
+            assert be.name().endswith(f"fused walk, group {g}"), be.name()
+            a, _, t = got(be, b, p, False)
+            counts[g] = t["interactions"]
+            e = per_body_rel(a[idx], a64)
+            print(f"group {g}: {t['interactions']} interactions, median rel error {np.median(e):.3g}")
+            assert np.median(e) < 1e-3
+        finally:
+            be.close()
+    assert counts["16"] < counts["64"] < counts["128"]
+    for key, bad in (("LETTUCE_HIP_GROUP", "12"), ("LETTUCE_HIP_GROUP", "129"), ("LETTUCE_HIP_GROUP", "64x"),
+                     ("LETTUCE_HIP_WALK", "body")):
+        monkeypatch.setenv("LETTUCE_HIP_GROUP", "128")
+        monkeypatch.setenv(key, bad)
+        be, why = L.make_hip_backend(0)
+        assert be is None and key in why, (key, bad, why)
+        monkeypatch.delenv(key)
+
+
 @pytest.mark.parametrize("mode", ["plain", "mixed"])
 def test_walk_lists_equal_the_fused_walk(mode, monkeypatch):
     # LETTUCE_HIP_WALK=lists splits the walk into a list traversal (its own kernel) and a list
