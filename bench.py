@@ -64,6 +64,28 @@ def issue_probe() -> dict:
     return _probe
 
 
+def issue_sweep() -> dict:
+    """Issue rate of single-form FP32 streams (the probe's second entry point) by instruction form,
+    independent chains per lane and waves per SIMD (1024, 2048 and 4096 blocks of 4 waves)."""
+    import ctypes
+
+    lib = ctypes.CDLL(os.path.join(ROOT, "lettuce_simulate_physics_amd", "libissue_probe.so"))
+    lib.issue_probe_sweep.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_double)]
+    forms = ["v_fmamk_f32", "v_fmac_f32", "v_fma_f32 (3 VGPR)", "v_mul_f32", "v_add_f32"]
+    steps, rows = 4096, []
+    for f, name in enumerate(forms):
+        for chains in (8, 12, 24):
+            for blocks in (1024, 2048, 4096):
+                ms = ctypes.c_double()
+                if lib.issue_probe_sweep(f, chains, blocks, steps, ctypes.byref(ms)) != 0:
+                    raise RuntimeError("issue sweep failed")
+                rate = blocks * 256 * steps * chains / (ms.value * 1e-3)
+                rows.append({"form": name, "chains": chains, "waves_per_simd": blocks * 4 // 1024,
+                             "ms": round(ms.value, 4), "lane_rate_tps": round(rate / 1e12, 2),
+                             "x_nominal": round(rate / NOMINAL_LANE_RATE, 3)})
+    return {"steps": steps, "nominal_lane_rate_tps": round(NOMINAL_LANE_RATE / 1e12, 3), "rows": rows}
+
+
 def issue_ceiling(n_pp: float, n_pc: float, valu_pp: float, seconds: float) -> dict:
     """FP32 rate the instruction mix allows at the probe's issue rate, and the share of it achieved."""
     P = issue_probe()
@@ -523,6 +545,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy 3 times (HBM counter calibration)")
     ap.add_argument("--issue-probe", action="store_true", help="only run the issue-rate probe and print it")
+    ap.add_argument("--issue-sweep", action="store_true",
+                    help="only run the issue-rate sweep (instruction form x chains x waves per SIMD) and print it")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -531,6 +555,9 @@ def main():
 
     if args.issue_probe:
         print(json.dumps(issue_probe()), flush=True)
+        return
+    if args.issue_sweep:
+        print(json.dumps(issue_sweep()), flush=True)
         return
 
     import torch

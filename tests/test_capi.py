@@ -48,8 +48,15 @@ def test_issue_probe_library_is_built_for_gfx950():
     # The issue-rate probe bench.py runs beside the force path: its own library, entry issue_probe_run.
     path = os.path.join(ROOT, "lettuce_simulate_physics_amd", "libissue_probe.so")
     lib = ctypes.CDLL(path)
-    assert hasattr(lib, "issue_probe_run")
+    assert hasattr(lib, "issue_probe_run") and hasattr(lib, "issue_probe_sweep")
     assert b"gfx950" in open(path, "rb").read()
+    # The sweep refuses forms and chain counts it has no kernel for before any device call.
+    lib.issue_probe_sweep.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_double)]
+    ms = ctypes.c_double(-1.0)
+    assert lib.issue_probe_sweep(5, 12, 1024, 16, ctypes.byref(ms)) == -1
+    assert lib.issue_probe_sweep(0, 7, 1024, 16, ctypes.byref(ms)) == -1
+    assert lib.issue_probe_sweep(0, 12, 0, 16, ctypes.byref(ms)) == -1
+    assert ms.value == -1.0
 
 
 def test_memory_need_is_host_only_and_monotone():
