@@ -333,8 +333,8 @@ int lhip_tree_groups(lhip_ctx* ctx, uint32_t* gstart);
  * current device without synchronising. posm: n float4 records (x, y, z, m), 16-byte aligned;
  * ax, ay, az (and pot when potential != 0): n floats each. a_i = G sum_j m_j d_ij / s^3 and
  * phi_i = -G sum_j m_j / s with s^2 = d^2 + eps^2, the target skipped by index. The partial sums of
- * the split source range live in a 32 MiB device array the library holds per device for the
- * process's lifetime (2,097,152 float4; every n below 2^31 - 1 fits). Nothing is allocated or freed.
+ * the split source range live in a 64 MiB device array the library holds per device for the
+ * process's lifetime (4,194,304 float4; every n below 2^31 - 1 fits). Nothing is allocated or freed.
  * Returns LHIP_INVALID for n >= 2^31 - 1, NULL or misaligned arrays; LHIP_RUNTIME for launch
  * failures. Calls that overlap on one device share that array and race: serialise them (one
  * stream, or events). */

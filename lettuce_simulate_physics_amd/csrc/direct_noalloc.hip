@@ -5,9 +5,9 @@
 // The kernels are the context's (launch_direct, kernels_direct.hip), with the same split count S
 // for every n, so the results are the context's bit for bit. Their one scratch buffer, the S raw
 // partial sums per target when the source range is split (S > 1), lives in `scratch`, a file-scope
-// device array of 2,097,152 float4 (32 MiB) that every device this library runs on holds until the
-// process ends. The split count direct_split(n) keeps S * n at or below 2,096,640 for every n below
-// 2^31 - 1 (largest at n = 1,048,320; S = 1, no scratch, from n = 2^22 on), so every size fits.
+// device array of 4,194,304 float4 (64 MiB) that every device this library runs on holds until the
+// process ends. The split count direct_split(n) keeps S * n at or below 4,193,792 for every n below
+// 2^31 - 1 (largest at n = 2,096,896; S = 1, no scratch, from n = 2^21 on), so every size fits.
 //
 // The array's name is a host-side stand-in in host code: its device address is taken once per
 // device with hipGetSymbolAddress (first call on that device) and only that address reaches the
@@ -21,7 +21,7 @@
 namespace lhip {
 namespace {
 
-constexpr std::size_t kScratchFloat4 = std::size_t(1) << 21;  // 2,097,152 partial sums, 32 MiB
+constexpr std::size_t kScratchFloat4 = std::size_t(1) << 22;  // 4,194,304 partial sums, 64 MiB
 constexpr int kMaxDevices = 64;
 
 static __device__ __align__(256) float4 scratch[kScratchFloat4];

@@ -71,9 +71,11 @@ def test_memory_need_per_body_in_each_mode():
     # levels, scans, groups, packs, walk order) and 132 B per node of the first node reserve n / 4
     # (a tree with more nodes grows it); the mixed form adds 128 B of FP64 state (tree: posd 32 B and
     # 80 B per node of FP64 moments; direct: 48-B records).
-    n1, n2 = 1 << 20, 1 << 22
+    # Direct from 2^21 on: below it the plain sum splits its source range and holds the partial sums.
+    n2 = 1 << 22
     slope = {}
     for m in ("tree", "direct"):
+        n1 = 1 << 20 if m == "tree" else 1 << 21
         for mode in ("plain", "mixed"):
             a, _ = L.HipBackend.memory_need(n1, m, mode)
             b, _ = L.HipBackend.memory_need(n2, m, mode)
@@ -83,7 +85,7 @@ def test_memory_need_per_body_in_each_mode():
     assert 103.5 < slope["direct", "plain"] < 104.5
     assert 311 < slope["direct", "mixed"] < 313
     with pytest.raises(ValueError):
-        L.HipBackend.memory_need(n1, "tree", "fp16")
+        L.HipBackend.memory_need(n2, "tree", "fp16")
 
 
 def test_factory_without_device_returns_reason(monkeypatch):
