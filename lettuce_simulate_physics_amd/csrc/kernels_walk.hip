@@ -18,13 +18,17 @@
 //
 // Evaluation. Accepted cells are staged in LDS with their moments, bodies of opened leaves with
 // their positions. Walk groups hold 1-128 bodies (39 on average for a Plummer sphere), so one body
-// per lane would idle most lanes. Instead the wave has V = 64 * NS virtual slots (NS = 1-3 per lane,
-// 1-2 with the potential or the self test, whichever keeps most slots busy); with
-// S = min(16, floor(V / nb)) list shares, slot v
-// evaluates body v mod nb against list entries e = v div nb (mod S). Partial sums of the S shares are added in
-// LDS at the end. Lists are evaluated whenever the next append would overflow them. Before an
-// evaluation the list is padded with zero-mass entries to a multiple of S, so the loops carry no
-// bounds test; a zero-mass entry adds exactly zero. With eps > 0 a body's own entry adds exactly
+// per lane would idle most lanes. Instead each lane holds up to 3 target slots (2 with the potential
+// or the self test), and the chunk's bodies are split into at most two parts, each on its own slots
+// of every lane: a part of n bodies on ns slots is taken by lps = ceil(n / ns) lanes per list share,
+// lane l of a share holding the part's bodies l, l + lps, ... (one LDS read of an entry feeds all of
+// them), and S = min(16, 64 / lps) shares divide the list entries between them (share s takes
+// entries s, s + S, ...). The split and the slots per part come from a table by chunk size
+// (walk_eval_plan: the fewest slot-steps per list entry, each loop step weighted a quarter slot-step
+// more); two parts fill the lanes where one would leave many slots idle. Partial sums of the shares
+// are added in LDS at the end. Lists are evaluated whenever the next append would overflow them.
+// Before an evaluation 15 zero-mass entries are appended to the list (S <= 16), so the loops carry
+// no bounds test; a zero-mass entry adds exactly zero. With eps > 0 a body's own entry adds exactly
 // zero too (d = 0), so only the potential and eps = 0 need the per-pair self test; every body of a
 // group meets itself exactly once (the leaves holding the group are always opened), which is how
 // the self pairs are taken out of the interaction counts.
@@ -50,7 +54,9 @@
 // consecutive bodies with the remote trees only and adds to the direct sum.
 //
 // Scheduling. A persistent grid; each wave takes groups from an atomic counter until none are left.
+#include <array>
 #include <cfloat>
+#include <cstring>
 
 #include "kernels.hpp"
 #include "tree.hpp"
@@ -129,29 +135,32 @@ __device__ __forceinline__ void p2c(Slot& b, float4 A, float4 B, float2 C, float
     if (POT) b.sp_pot += __fmaf_rn(0.5f, t, A.w * inv);
 }
 
-// The lane's slots evaluate entries sp, sp + S, ... < steps * S of its list share sp (the list is
-// padded to that length): one LDS read of an entry feeds every slot of the lane.
-template <int NS, bool POT, bool GUARD, int NX>
+// The lane's slots OFF .. OFF + NS - 1 (one part of the chunk's plan) evaluate entries sp, sp + S,
+// ... < steps * S of their list share sp (the list is padded to that length): one LDS read of an
+// entry feeds every slot of the part.
+template <int NS, int OFF, bool POT, bool GUARD, int NX>
 __device__ __forceinline__ void eval_cells(Slot (&sl)[NX], int sp, int S, int nc, const WaveLds& L, float e2) {
+    static_assert(OFF + NS <= NX, "slots of the part");
     const int steps = (nc + S - 1) / S;
     for (int t = 0; t < steps; ++t) {
         const CellL& c = L.c[t * S + sp];
         const float4 A = c.A, B = c.B;
         const float2 C = c.C;
 #pragma unroll
-        for (int s = 0; s < NS; ++s) p2c<POT, GUARD>(sl[s], A, B, C, e2);
+        for (int s = 0; s < NS; ++s) p2c<POT, GUARD>(sl[OFF + s], A, B, C, e2);
     }
 }
 
-template <int NS, bool POT, bool MASK, bool GUARD, int NX>
+template <int NS, int OFF, bool POT, bool MASK, bool GUARD, int NX>
 __device__ __forceinline__ void eval_bodies(Slot (&sl)[NX], int sp, int S, int nb, const WaveLds& L, float e2) {
+    static_assert(OFF + NS <= NX, "slots of the part");
     const int steps = (nb + S - 1) / S;
     for (int t = 0; t < steps; ++t) {
         const int e = t * S + sp;
         const float4 q = L.bP[e];
         const int j = MASK ? L.bI[e] : 0;
 #pragma unroll
-        for (int s = 0; s < NS; ++s) p2p<POT, MASK, GUARD>(sl[s], q, j, e2);
+        for (int s = 0; s < NS; ++s) p2p<POT, MASK, GUARD>(sl[OFF + s], q, j, e2);
     }
 }
 
@@ -380,30 +389,33 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
         for (int clo = glo; clo < ghi; clo += kGroupMax) {
             const int chi = min(ghi, clo + kGroupMax);
             const int nbody = chi - clo;  // 1..128, wave-uniform
-            // Slots per lane NS (up to NX) and list shares S. A share is taken by lps =
-            // ceil(nbody / NS) lanes, lane l of it holding bodies l, l + lps, ... in its NS slots,
-            // so all slots of a lane read the same entries: S = min(16, 64 / lps). The pair (NS, S)
-            // that keeps the most slots busy (nbody S of 64 NS) wins, the fewest slots on a tie.
-            int NS = 1, S = min(kMaxShare, 64 / nbody);
-#pragma unroll
-            for (int ns = 2; ns <= NX; ++ns) {
-                const int Sn = min(kMaxShare, 64 / ((nbody + ns - 1) / ns));
-                if (Sn * NS > S * ns) { NS = ns; S = Sn; }
-            }
-            const int lps = (nbody + NS - 1) / NS;
-            // The lane's share; lanes past the shares read share 0 and their slots are discarded.
-            int sp = lane / lps;
-            const bool lane_on = sp < S;
-            if (!lane_on) sp = 0;
+            // The chunk's evaluation plan (walk_eval_plan): part A, bodies [0, na), on slots
+            // 0 .. nsA - 1 with SA list shares; part B, bodies [na, nbody), on slots nsA .. nsA + nsB - 1
+            // with SB shares (nsB = 0: no part B). A share of part P is taken by lps_P =
+            // ceil(n_P / ns_P) lanes, lane l of it holding the part's bodies l, l + lps_P, ... in its
+            // slots of the part, so those slots read the same entries.
+            const uint32_t pw = a.plan[nbody - 1];
+            const int na = int(pw & 0xffu), nsA = int((pw >> 8) & 0xfu), SA = int((pw >> 12) & 0x3fu);
+            const int nsB = int((pw >> 18) & 0xfu), SB = int((pw >> 22) & 0x3fu);
+            const int nbB = nbody - na;
+            const int lpsA = (na + nsA - 1) / nsA, lpsB = nsB ? (nbB + nsB - 1) / nsB : 64;
+            // The lane's share in each part; lanes past a part's shares read share 0 and their
+            // slots of that part are discarded.
+            int spA = lane / lpsA, spB = lane / lpsB;
+            const bool onA = spA < SA, onB = nsB > 0 && spB < SB;
+            if (!onA) spA = 0;
+            if (!onB) spB = 0;
             Slot sl[NX];
             float mnx = FLT_MAX, mny = FLT_MAX, mnz = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX, mxz = -FLT_MAX;
 #pragma unroll
             for (int s = 0; s < NX; ++s) {
-                // Slot s of lane l: body (l mod lps) + s lps; off slots take body 0 (a valid load
-                // whose sums are dropped). The box below takes every body of the chunk: slot s of
-                // lane l also covers (s 64 + l) mod nbody.
-                const int b = lane % lps + s * lps;
-                sl[s].on = lane_on && s < NS && b < nbody;
+                // Slot s < nsA of lane l: body (l mod lpsA) + s lpsA; slot nsA + t: body na +
+                // (l mod lpsB) + t lpsB. Off slots take body (s 64 + l) mod nbody (a valid load whose
+                // sums are dropped). Share 0 of each part holds every body of the part, so the box
+                // below takes every body of the chunk.
+                const bool inA = s < nsA;
+                const int b = inA ? lane % lpsA + s * lpsA : na + lane % lpsB + (s - nsA) * lpsB;
+                sl[s].on = inA ? onA && b < na : onB && s < nsA + nsB && b < nbody;
                 sl[s].bi = sl[s].on ? b : (s * 64 + lane) % nbody;
                 sl[s].i = clo + sl[s].bi;
                 float4 p;
@@ -456,10 +468,18 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                     L.c[nc + lane].C = make_float2(0.f, 0.f);
                 }
                 __builtin_amdgcn_wave_barrier();
-                if (NX >= 4 && NS == 4) eval_cells<4, POT, GUARD>(sl, sp, S, nc, L, e2);
-                else if (NX >= 3 && NS == 3) eval_cells<3, POT, GUARD>(sl, sp, S, nc, L, e2);
-                else if (NS == 2) eval_cells<2, POT, GUARD>(sl, sp, S, nc, L, e2);
-                else eval_cells<1, POT, GUARD>(sl, sp, S, nc, L, e2);
+                if constexpr (NX >= 3) {
+                    if (nsA == 3) eval_cells<3, 0, POT, GUARD>(sl, spA, SA, nc, L, e2);
+                    else if (nsA == 2) eval_cells<2, 0, POT, GUARD>(sl, spA, SA, nc, L, e2);
+                    else eval_cells<1, 0, POT, GUARD>(sl, spA, SA, nc, L, e2);
+                    if (nsB == 2) eval_cells<2, 1, POT, GUARD>(sl, spB, SB, nc, L, e2);
+                    else if (nsB == 1 && nsA == 2) eval_cells<1, 2, POT, GUARD>(sl, spB, SB, nc, L, e2);
+                    else if (nsB == 1) eval_cells<1, 1, POT, GUARD>(sl, spB, SB, nc, L, e2);
+                } else {
+                    if (nsA == 2) eval_cells<2, 0, POT, GUARD>(sl, spA, SA, nc, L, e2);
+                    else eval_cells<1, 0, POT, GUARD>(sl, spA, SA, nc, L, e2);
+                    if (nsB == 1) eval_cells<1, 1, POT, GUARD>(sl, spB, SB, nc, L, e2);
+                }
                 fold();
                 __builtin_amdgcn_wave_barrier();
                 ncell_total += uint32_t(nc);
@@ -471,10 +491,18 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                     L.bI[nb + lane] = -1;
                 }
                 __builtin_amdgcn_wave_barrier();
-                if (NX >= 4 && NS == 4) eval_bodies<4, POT, MASK, GUARD>(sl, sp, S, nb, L, e2);
-                else if (NX >= 3 && NS == 3) eval_bodies<3, POT, MASK, GUARD>(sl, sp, S, nb, L, e2);
-                else if (NS == 2) eval_bodies<2, POT, MASK, GUARD>(sl, sp, S, nb, L, e2);
-                else eval_bodies<1, POT, MASK, GUARD>(sl, sp, S, nb, L, e2);
+                if constexpr (NX >= 3) {
+                    if (nsA == 3) eval_bodies<3, 0, POT, MASK, GUARD>(sl, spA, SA, nb, L, e2);
+                    else if (nsA == 2) eval_bodies<2, 0, POT, MASK, GUARD>(sl, spA, SA, nb, L, e2);
+                    else eval_bodies<1, 0, POT, MASK, GUARD>(sl, spA, SA, nb, L, e2);
+                    if (nsB == 2) eval_bodies<2, 1, POT, MASK, GUARD>(sl, spB, SB, nb, L, e2);
+                    else if (nsB == 1 && nsA == 2) eval_bodies<1, 2, POT, MASK, GUARD>(sl, spB, SB, nb, L, e2);
+                    else if (nsB == 1) eval_bodies<1, 1, POT, MASK, GUARD>(sl, spB, SB, nb, L, e2);
+                } else {
+                    if (nsA == 2) eval_bodies<2, 0, POT, MASK, GUARD>(sl, spA, SA, nb, L, e2);
+                    else eval_bodies<1, 0, POT, MASK, GUARD>(sl, spA, SA, nb, L, e2);
+                    if (nsB == 1) eval_bodies<1, 1, POT, MASK, GUARD>(sl, spB, SB, nb, L, e2);
+                }
                 fold();
                 __builtin_amdgcn_wave_barrier();
                 nbody_total += uint32_t(nb);
@@ -764,7 +792,10 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
             flush_cells();
             flush_bodies();
 
-            // Add the list shares of each body (slot v = share * nbody + body) through LDS.
+            // Add the list shares of each body through LDS: part A's share sp of body v at
+            // sp na + v, part B's at SA na + sp nbB + (v - na).
+            auto red_slot = [&](int s, int v) { return s < nsA ? spA * na + v : SA * na + spB * nbB + (v - na); };
+            auto red_base = [&](int v) { return v < na ? v : SA * na + (v - na); };
             const uint32_t listed = ncell_total + nbody_total;  // nbody_total includes the monopoles
             // Per-body cost, self excluded: the list less the body's own entry; remote blocks after
             // the direct sum add their list to n - 1.
@@ -777,16 +808,17 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                 __builtin_amdgcn_wave_barrier();
 #pragma unroll
                 for (int s = 0; s < NX; ++s)
-                    if (sl[s].on) red[sp * nbody + sl[s].bi] = D4{sl[s].dx, sl[s].dy, sl[s].dz, sl[s].dp};
+                    if (sl[s].on) red[red_slot(s, sl[s].bi)] = D4{sl[s].dx, sl[s].dy, sl[s].dz, sl[s].dp};
                 __builtin_amdgcn_wave_barrier();
                 const double G = a.Gd;
 #pragma unroll
                 for (int s = 0; s < 2; ++s) {
                     const int v = s * 64 + lane;
                     if (v >= nbody) continue;
-                    D4 acc4 = red[v];
-                    for (int sp = 1; sp < S; ++sp) {
-                        const D4 q = red[sp * nbody + v];
+                    const int r0 = red_base(v), rs = v < na ? na : nbB, nsh = v < na ? SA : SB;
+                    D4 acc4 = red[r0];
+                    for (int sp = 1; sp < nsh; ++sp) {
+                        const D4 q = red[r0 + sp * rs];
                         acc4.x += q.x; acc4.y += q.y; acc4.z += q.z; acc4.w += q.w;
                     }
                     const int i = clo + v;
@@ -809,16 +841,17 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
 #pragma unroll
                 for (int s = 0; s < NX; ++s)
                     if (sl[s].on)
-                        red[sp * nbody + sl[s].bi] = make_float4(sl[s].sx, sl[s].sy, sl[s].sz, sl[s].sp_pot);
+                        red[red_slot(s, sl[s].bi)] = make_float4(sl[s].sx, sl[s].sy, sl[s].sz, sl[s].sp_pot);
                 __builtin_amdgcn_wave_barrier();
                 const float G = a.G;
 #pragma unroll
                 for (int s = 0; s < 2; ++s) {
                     const int v = s * 64 + lane;
                     if (v >= nbody) continue;
-                    float4 acc4 = red[v];
-                    for (int sp = 1; sp < S; ++sp) {
-                        const float4 q = red[sp * nbody + v];
+                    const int r0 = red_base(v), rs = v < na ? na : nbB, nsh = v < na ? SA : SB;
+                    float4 acc4 = red[r0];
+                    for (int sp = 1; sp < nsh; ++sp) {
+                        const float4 q = red[r0 + sp * rs];
                         acc4.x += q.x; acc4.y += q.y; acc4.z += q.z; acc4.w += q.w;
                     }
                     const int i = clo + v;
@@ -875,6 +908,56 @@ __global__ void reset_walk(WalkCounters* __restrict__ c) {
 
 }  // namespace
 
+uint32_t walk_eval_plan(int nb, int nx) {
+    if (nb < 1 || nb > kGroupMax || nx < 1 || nx > 3) throw Error(kInvalid, "walk_eval_plan: chunk or slots out of range");
+    // Shares of a part of n bodies on ns slots per lane (0: the part does not fit the wave).
+    auto shares = [](int n, int ns) {
+        const int lps = (n + ns - 1) / ns;
+        return lps > 64 ? 0 : std::min(kMaxShare, 64 / lps);
+    };
+    long long bnum = 0, bden = 0;  // best cost bnum / bden
+    int bparts = 0, bslots = 0;
+    uint32_t word = 0;
+    for (int na = 1; na <= nb; ++na) {
+        const int nbb = nb - na;
+        for (int nsa = 1; nsa <= nx; ++nsa) {
+            const int sa = shares(na, nsa);
+            if (!sa) continue;
+            const int lo = nbb ? 1 : 0, hi = nbb ? nx - nsa : 0;
+            for (int nsb = lo; nsb <= hi; ++nsb) {
+                const int sb = nsb ? shares(nbb, nsb) : 1;
+                if (!sb) continue;
+                // Cost per list entry, in quarter slot-steps: (4 ns_A + 1) / S_A + (4 ns_B + 1) / S_B.
+                const long long ca = 4ll * nsa + 1, cb = nsb ? 4ll * nsb + 1 : 0;
+                const long long num = ca * sb + cb * sa, den = (long long)sa * sb;
+                const int parts = nsb ? 2 : 1, slots = nsa + nsb;
+                const long long l = num * bden, r = bnum * den;
+                const bool better = bden == 0 || l < r ||
+                                    (l == r && (parts < bparts || (parts == bparts && slots < bslots)));
+                if (better) {
+                    bnum = num; bden = den; bparts = parts; bslots = slots;
+                    word = uint32_t(na) | uint32_t(nsa) << 8 | uint32_t(sa) << 12 | uint32_t(nsb) << 18 |
+                           uint32_t(nsb ? sb : 0) << 22;
+                }
+            }
+        }
+    }
+    return word;
+}
+
+namespace {
+// The plans of every chunk size for 2 and 3 slots per lane, made once.
+const uint32_t* eval_plans(int nx) {
+    static const auto tables = [] {
+        std::array<std::array<uint32_t, kGroupMax>, 2> t{};
+        for (int k = 0; k < 2; ++k)
+            for (int nb = 1; nb <= kGroupMax; ++nb) t[k][nb - 1] = walk_eval_plan(nb, k + 2);
+        return t;
+    }();
+    return tables[nx - 2].data();
+}
+}  // namespace
+
 int walk_blocks_per_cu() {
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks,
@@ -898,9 +981,12 @@ void walk_launch(hipStream_t s, const WalkArgs& a, bool potential, bool guard, i
     }
 }
 
-void tree_walk(hipStream_t s, const WalkArgs& a, bool potential) {
-    if (a.n <= 0) return;
-    const bool guard = !(a.eps2 > 0.f);
+void tree_walk(hipStream_t s, const WalkArgs& args, bool potential) {
+    if (args.n <= 0) return;
+    const bool guard = !(args.eps2 > 0.f);
+    // The slots per lane of the launch's kernel: 2 with the potential or the self test, else 3.
+    WalkArgs a = args;
+    std::memcpy(a.plan, eval_plans(potential || guard ? 2 : 3), sizeof a.plan);
     const int blocks = std::max(1, a.blocks);
     reset_walk<<<1, 1, 0, s>>>(a.counters);
     LHIP_LAUNCH_CHECK();

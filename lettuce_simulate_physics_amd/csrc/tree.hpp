@@ -236,7 +236,19 @@ struct WalkArgs {
     uint32_t* lhead;
     uint32_t list_cap;         // pool chunks; a traversal that needs more sets counters->list_chunks above it
     int list_blocks;           // persistent grid of the list traversal (blocks of 4 waves)
+    // Evaluation plan per chunk size (entry nb - 1 for a chunk of nb bodies; tree_walk fills it for
+    // the launch's slots per lane): walk_eval_plan's packed parts.
+    uint32_t plan[kGroupMax];
 };
+// The evaluation plan of a walk chunk of nb bodies (1 .. kGroupMax) with at most nx target slots
+// per lane: its bodies split into part A = [0, na) and part B = [na, nb) (empty when na = nb);
+// part P takes ns_P slots per lane and S_P list shares, lps_P = ceil(n_P / ns_P) lanes per share
+// and S_P = min(16, 64 / lps_P), so a list of L entries costs it L / S_P steps of ns_P slots. The
+// plan minimises (ns_A + 1/4) / S_A + (ns_B + 1/4) / S_B per list entry: the slot-steps plus a
+// quarter of one per loop step, the step's LDS read and loop overhead (the weight that timed best of
+// 0, 1/8, 1/4, 3/8, 1/2 and 1 on the 1M Plummer sphere). Ties: one part, then fewer slots. Packed as
+// na | ns_A << 8 | S_A << 12 | ns_B << 18 | S_B << 22 (ns_B = 0: one part).
+uint32_t walk_eval_plan(int nb, int nx);
 // List entries: a quadrupole cell, a monopole cell or the bodies of a leaf, and the node index.
 constexpr int kListChunk = 256;
 constexpr uint32_t kEntCell = 0u << 30, kEntMono = 1u << 30, kEntRange = 2u << 30, kEntIndex = (1u << 30) - 1u;
