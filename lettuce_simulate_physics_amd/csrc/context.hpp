@@ -711,7 +711,9 @@ struct Context {
         a.list_blocks = list_blocks;
         a.small_leaf = wopt.small_leaf;
         a.quad_tau = float(wopt.quad_tau);
-        a.depth = tree_depth(frames);
+        // The deepest level the last build reported (its stack reserve is 7 per level), or the bound
+        // for the key frames when the build did not wait for the report.
+        a.depth = built_depth >= 0 ? std::min(built_depth, tree_depth(frames)) : tree_depth(frames);
         return a;
     }
 

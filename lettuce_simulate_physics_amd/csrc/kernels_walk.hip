@@ -167,8 +167,9 @@ __device__ __forceinline__ void eval_bodies(Slot (&sl)[NX], int sp, int S, int n
 constexpr int kStackCap = kWalkStack;
 // Stack budget: a step popping P nodes pushes at most 8P, so P is held to (thr - top) / 7 with thr
 // = capacity - 7 (depth + 1) - 8; at or above thr one node per step adds at most 7 per tree level.
+// depth is the deepest level of the tree walked (the build reports it; 21 x key frames bounds it).
 __device__ __forceinline__ int stack_threshold(int depth, int cap = kStackCap) {
-    return cap - 7 * (max(depth, kMaxLevel) + 1) - 8;
+    return cap - 7 * (depth + 1) - 8;
 }
 
 // What a local node contributes to the list of the bodies [clo, chi) with box centre b and half
@@ -651,8 +652,9 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                 if (lane == 0) st[0] = 0;  // the root
                 int top = 1;
                 while (top > 0) {
-                    const int room = (stack_thr - top) / 7;
-                    const int P = min(64, min(top, max(1, room)));
+                    // Wide steps pop up to 64 nodes while the stack is at least 7 below thr; their
+                    // pushes are held to thr (below). Above that, one node per step.
+                    const int P = top <= stack_thr - 7 ? min(64, top) : 1;
                     __builtin_amdgcn_wave_barrier();
                     const bool act = lane < P;
                     const int k = act ? st[top - P + lane] : 0;
@@ -689,16 +691,35 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                                            a.quad_tau);
                     const bool is_cell = pk.cell, is_mono = pk.mono, is_range = pk.range, open_int = pk.open;
 
-                    // push the child blocks of opened internal nodes
+                    // Push the child blocks of opened internal nodes. A wide step keeps the stack at
+                    // or below thr: the opened nodes (in lane order) whose children would pass it go
+                    // back on the stack themselves and are popped and opened again later (the same
+                    // decision: the lists stay the same sets). Popped at 7 or more below thr, the
+                    // first opened node's children always fit, so every step makes progress.
                     const int nch = open_int ? info_nchild(nd.info) : 0;
                     const int cend = prefix_sum_incl(nch, lane);
                     const int pushed = __builtin_amdgcn_readlane(cend, 63);
-                    if (top + pushed > kStackCap) {
-                        overflow = true;
-                        break;
+                    if (P > 1 && top + pushed > stack_thr) {
+                        const unsigned long long om = __ballot(open_int);
+                        const int nopen = __popcll(om);
+                        const int after = nopen - int(lanes_below(om)) - 1;  // opened lanes above this one
+                        const bool fit = open_int && top + cend + after <= stack_thr;
+                        const unsigned long long fm = __ballot(fit);
+                        const int nfit = __popcll(fm);
+                        const int cfit = nfit ? __builtin_amdgcn_readlane(cend, 63 - __clzll(fm)) : 0;
+                        if (fit)
+                            for (int c = 0; c < nch; ++c) st[top + cend - nch + c] = nd.first + c;
+                        else if (open_int)
+                            st[top + cfit + int(lanes_below(om)) - nfit] = k;
+                        top += cfit + nopen - nfit;
+                    } else {
+                        if (top + pushed > kStackCap) {
+                            overflow = true;
+                            break;
+                        }
+                        for (int c = 0; c < nch; ++c) st[top + cend - nch + c] = nd.first + c;
+                        top += pushed;
                     }
-                    for (int c = 0; c < nch; ++c) st[top + cend - nch + c] = nd.first + c;
-                    top += pushed;
 
                     stage_cells(is_cell, [&]() {
                         CellL c;
