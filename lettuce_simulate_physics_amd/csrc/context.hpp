@@ -603,6 +603,12 @@ struct Context {
     }
     // Deepest level of the last build's multipoles (tree_moments_keep needs no more).
     int built_depth = -1;
+    // The deepest level a walk of the current tree can reach (the traversal stacks reserve 7 entries
+    // per level): the level the last build reported, or the bound for the key frames when the build
+    // did not wait for the report.
+    int walk_depth() const {
+        return built_depth >= 0 ? std::min(built_depth, tree_depth(frames)) : tree_depth(frames);
+    }
 
     // tree_build with the host wait on the level pass (ev[9]); a tree with more nodes than the node
     // arrays hold grows them to a quarter more than it has and is built again. bb's positions (the
@@ -711,9 +717,7 @@ struct Context {
         a.list_blocks = list_blocks;
         a.small_leaf = wopt.small_leaf;
         a.quad_tau = float(wopt.quad_tau);
-        // The deepest level the last build reported (its stack reserve is 7 per level), or the bound
-        // for the key frames when the build did not wait for the report.
-        a.depth = built_depth >= 0 ? std::min(built_depth, tree_depth(frames)) : tree_depth(frames);
+        a.depth = walk_depth();
         return a;
     }
 
@@ -1054,7 +1058,7 @@ struct Context {
             e.mom = mom.as<NodeMom>();
             e.levstart = levstart.as<uint32_t>();
             e.count = int(count);
-            e.depth = tree_depth(frames);
+            e.depth = walk_depth();
             e.f64 = mixed;
             if (mixed) { e.x = D(xd); e.y = D(yd); e.z = D(zd); }
             else { e.x = F(x); e.y = F(y); e.z = F(z); }

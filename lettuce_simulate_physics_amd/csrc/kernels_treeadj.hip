@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(kBS4, 4) tree_vjp_kernel(TreeVjpArgs a) {
     const float e2 = a.eps2;
     bool overflow = false;
     // the walk's stack budget for this tree depth (kernels_walk.hip)
-    const int thr = kStack - 7 * (max(a.depth, kMaxLevel) + 1) - 8;
+    const int thr = kStack - 7 * (a.depth + 1) - 8;
 
     double* const priv = a.gpriv + std::size_t(blockIdx.x) * kHot * 12;
     auto cell_out = [&](int k) -> double* {  // quadrupole cell k: gC (3), G (6)
@@ -298,8 +298,7 @@ __global__ void __launch_bounds__(kBS4, 4) tree_vjp_kernel(TreeVjpArgs a) {
             if (lane == 0) st[0] = 0;
             int top = 1;
             while (top > 0) {
-                const int room = (thr - top) / 7;
-                const int P = min(64, min(top, max(1, room)));
+                const int P = top <= thr - 7 ? min(64, top) : 1;
                 __builtin_amdgcn_wave_barrier();
                 const bool act = lane < P;
                 const int k = act ? st[top - P + lane] : 0;
@@ -329,15 +328,32 @@ __global__ void __launch_bounds__(kBS4, 4) tree_vjp_kernel(TreeVjpArgs a) {
                 const bool is_range = leaf && (!acc || small);
                 const bool open_int = act && !acc && !leaf;
 
+                // Pushes as the walk's: a wide step stays at or below thr, pushing back the opened
+                // nodes whose children would pass it.
                 const int nch = open_int ? info_nchild(nd.info) : 0;
                 const int cend = prefix_sum_incl(nch, lane);
                 const int pushed = __builtin_amdgcn_readlane(cend, 63);
-                if (top + pushed > kStack) {
-                    overflow = true;
-                    break;
+                if (P > 1 && top + pushed > thr) {
+                    const unsigned long long om = __ballot(open_int);
+                    const int nopen = __popcll(om);
+                    const int after = nopen - int(lanes_below(om)) - 1;
+                    const bool fit = open_int && top + cend + after <= thr;
+                    const unsigned long long fm = __ballot(fit);
+                    const int nfit = __popcll(fm);
+                    const int cfit = nfit ? __builtin_amdgcn_readlane(cend, 63 - __clzll(fm)) : 0;
+                    if (fit)
+                        for (int c = 0; c < nch; ++c) st[top + cend - nch + c] = nd.first + c;
+                    else if (open_int)
+                        st[top + cfit + int(lanes_below(om)) - nfit] = k;
+                    top += cfit + nopen - nfit;
+                } else {
+                    if (top + pushed > kStack) {
+                        overflow = true;
+                        break;
+                    }
+                    for (int c = 0; c < nch; ++c) st[top + cend - nch + c] = nd.first + c;
+                    top += pushed;
                 }
-                for (int c = 0; c < nch; ++c) st[top + cend - nch + c] = nd.first + c;
-                top += pushed;
 
                 const unsigned long long mc = __ballot(is_cell);
                 const int ncnew = __popcll(mc);
@@ -522,7 +538,7 @@ void tree_vjp(hipStream_t s, const TreeVjpArgs& a, const uint32_t* levstart, con
     down_root<<<1, 64, 0, s>>>(a.gcell, a.gmono, a.momd, cum);
     LHIP_LAUNCH_CHECK();
     const unsigned g = std::min<unsigned>(grid_for(a.n, kBS), 512u);
-    for (int l = 0; l < std::max(a.depth, kMaxLevel); ++l) {
+    for (int l = 0; l < a.depth; ++l) {  // the deepest level holds leaves only
         down_level<<<g, kBS, 0, s>>>(l, levstart, a.node, a.gcell, a.gmono, a.momd, cum);
         LHIP_LAUNCH_CHECK();
     }

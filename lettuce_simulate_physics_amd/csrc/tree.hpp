@@ -213,7 +213,7 @@ struct WalkArgs {
     int blocks;                // persistent grid size (blocks of 4 waves)
     int small_leaf;            // accepted leaves with at most this many bodies are summed directly
     float quad_tau;            // error-budgeted far field threshold (0: every quadrupole kept)
-    int depth;                 // deepest tree level (21 x key frames)
+    int depth;                 // deepest level of the tree walked (at most 21 x key frames)
     // Remote trees (let_trees > 0): node records, bodies (x, y, z, m) and the root of each
     // non-empty tree; the stack threshold for their height and fan-out. Every group walks them
     // after the local tree, into the same lists. direct_blocks: the groups are blocks of kGroupMax
@@ -286,7 +286,7 @@ struct TreeVjpArgs {
     double* gpriv;  // blocks x kHot x 12 doubles (per-block gcell, gmono of the first kHot nodes), zeroed
     WalkCounters* counters;
     int blocks;
-    int depth;      // deepest tree level (21 x key frames)
+    int depth;      // deepest level of the tree walked (at most 21 x key frames)
 };
 // Nodes at the top of the tree (levels 0-3 and part of 4 in level order) whose VJP cotangents each
 // block accumulates privately.

@@ -133,6 +133,7 @@ struct TreeSim {
     long long tr_cap = 0;
     int tr_slots = 0;
     std::vector<int> tr_frames;
+    std::vector<int> tr_depth;  // deepest level of each stored tree
 
     TreeSim() {
         for (auto& e : ev) LHIP_CHECK(hipEventCreate(&e));
@@ -441,6 +442,7 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
         S.tr_order.reserve(sl * (N + 1) * 4);
         S.tr_flag.reserve(sl * 4);
         S.tr_frames.assign(sl, 1);
+        S.tr_depth.assign(sl, tree_depth(1));
     };
     auto slot_node = [&](int j) { return S.tr_node.as<WalkNode>() + std::size_t(j) * S.tr_cap; };
     auto slot_mom = [&](int j) { return S.tr_mom.as<NodeMom>() + std::size_t(j) * S.tr_cap; };
@@ -467,6 +469,7 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
         unit_order(s, c.build_bufs(), c.walk_starts(), c.walk_count(), 1.f,
                    S.tr_order.as<uint32_t>() + std::size_t(j) * (N + 1));
         S.tr_frames[j] = c.frames;
+        S.tr_depth[j] = c.walk_depth();
     };
     std::vector<uint32_t> tr_ok;  // per slot of the current segment: 1 if its tree was stored whole
     auto vjp_at = [&](int k, double w) {
@@ -522,7 +525,7 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
         a.gpriv = R.D(S.gpriv);
         a.counters = c.counters.as<WalkCounters>();
         a.blocks = blocks;
-        a.depth = tree_depth(stored ? S.tr_frames[j] : c.frames);
+        a.depth = stored ? S.tr_depth[j] : c.walk_depth();
         tree_vjp(s, a, stored ? S.tr_lev.as<uint32_t>() + std::size_t(j) * kLev : c.levstart.as<uint32_t>(),
                  stored ? S.tr_cnt.as<uint32_t>() + j : c.cnt.as<uint32_t>() + n, nodes, R.D(S.cum));
         scatter_add_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(c.id.as<uint32_t>(), gt, gt + N, gt + 2 * N, gs, gs + N,
