@@ -98,6 +98,46 @@ def issue_ceiling(n_pp: float, n_pc: float, valu_pp: float, seconds: float) -> d
             "frac": round(t_min / seconds, 4)}
 
 
+def accum_probe(cases=((262144, 0.01), (65536, 0.05))) -> dict:
+    """The accumulation probe (libaccum_probe.so): the mixed form's FP32 pair arithmetic summed in
+    FP32, with Kahan compensation in FP32, in FP32 runs of 64 carried into FP64 (the library's
+    choice) and term by term in FP64, against FP64 pair arithmetic, for every body of a Plummer
+    sphere (a = 1, cut 10, seed 1) at each (n, eps); relative errors of |a| per body and the
+    kernel time of each mode."""
+    import ctypes
+
+    import lettuce_simulate_physics_amd as L
+
+    lib = ctypes.CDLL(os.path.join(ROOT, "lettuce_simulate_physics_amd", "libaccum_probe.so"))
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.accum_probe_run.argtypes = [dp, dp, dp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, dp]
+    names = ["fp64_reference", "fp32_sum", "fp32_kahan", "fp32_runs_of_64_fp64_carry", "fp64_sum_of_fp32_terms"]
+    out = {}
+    for n, eps in cases:
+        x, _, m = L.plummer_f64(n, 1, 1.0, 10.0)
+        cols = [np.ascontiguousarray(x[:, k]) for k in range(3)] + [np.ascontiguousarray(m)]
+        res, ms = {}, {}
+        for mode in range(5):
+            a = np.zeros(3 * n)
+            t = ctypes.c_double(0.0)
+            rc = lib.accum_probe_run(*[c.ctypes.data_as(dp) for c in cols], n, eps, mode, a.ctypes.data_as(dp),
+                                     ctypes.byref(t))
+            if rc != 0:
+                raise RuntimeError(f"accumulation probe failed (mode {mode}): {rc}")
+            res[mode], ms[mode] = a.reshape(n, 3), t.value
+        ref = res[0]
+        mag = np.linalg.norm(ref, axis=1)
+        case = {"n": n, "eps": eps, "pairs": n * n}
+        for mode in range(1, 5):
+            err = np.linalg.norm(res[mode] - ref, axis=1) / mag
+            case[names[mode]] = {"median": float(np.median(err)), "p99": float(np.quantile(err, 0.99)),
+                                 "max": float(err.max()), "ms": round(ms[mode], 4),
+                                 "gpairs_per_s": round(n * n / (ms[mode] * 1e-3) / 1e9, 1)}
+        case[names[0]] = {"ms": round(ms[0], 4)}
+        out[str(n)] = case
+    return out
+
+
 def plummer(n: int, seed: int = 1):
     """The workload's initial conditions: the library's generator, bit-exact with the reference
     scenario 'plummer' (plummer.a = 1, plummer.cut = 10)."""
@@ -545,6 +585,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy 3 times (HBM counter calibration)")
     ap.add_argument("--issue-probe", action="store_true", help="only run the issue-rate probe and print it")
+    ap.add_argument("--accum-probe", action="store_true",
+                    help="only run the accumulation probe (FP32, Kahan, runs with an FP64 carry) and print it")
     ap.add_argument("--issue-sweep", action="store_true",
                     help="only run the issue-rate sweep (instruction form x chains x waves per SIMD) and print it")
     args = ap.parse_args()
@@ -555,6 +597,9 @@ def main():
 
     if args.issue_probe:
         print(json.dumps(issue_probe()), flush=True)
+        return
+    if args.accum_probe:
+        print(json.dumps(accum_probe()), flush=True)
         return
     if args.issue_sweep:
         print(json.dumps(issue_sweep()), flush=True)

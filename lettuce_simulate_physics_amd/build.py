@@ -1,5 +1,5 @@
-"""Build recipe for liblettuce_hip.so (gfx950) from csrc/*.hip with hipcc, and for the issue-rate
-probe bench.py runs (libissue_probe.so, csrc/probe).
+"""Build recipe for liblettuce_hip.so (gfx950) from csrc/*.hip with hipcc, and for the probes
+bench.py runs (libissue_probe.so and libaccum_probe.so, csrc/probe).
 
 Every HIP translation unit is compiled with packed-FP32 instruction selection disabled
 (``-Xclang -target-feature -Xclang -packed-fp32-ops``) and IEEE arithmetic (no fast-math): the
@@ -15,9 +15,10 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "liblettuce_hip.so")
-# The issue-rate probe bench.py runs beside the force path (its own library, not linked into it).
-PROBE_SRC = os.path.join(CSRC, "probe", "issue_probe.hip")
-PROBE_OUT = os.path.join(HERE, "libissue_probe.so")
+# The probes bench.py runs beside the force path (their own libraries, not linked into it): the
+# issue-rate probe and the accumulation probe.
+PROBES = [(os.path.join(CSRC, "probe", "issue_probe.hip"), os.path.join(HERE, "libissue_probe.so")),
+          (os.path.join(CSRC, "probe", "accum_probe.hip"), os.path.join(HERE, "libaccum_probe.so"))]
 BUILD = os.path.join(HERE, "_build")
 SOURCES = ["kernels_state.hip", "kernels_direct.hip", "kernels_tree.hip", "kernels_build.hip",
            "kernels_walk.hip", "kernels_let.hip", "kernels_adjoint.hip", "kernels_treeadj.hip", "adjoint.hip",
@@ -50,13 +51,14 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, OUT)
-    tmp = PROBE_OUT + ".tmp"
-    r = subprocess.run([HIPCC, *FLAGS, "-shared", "-o", tmp, PROBE_SRC], capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed for the issue probe:\n{r.stderr}")
-    os.replace(tmp, PROBE_OUT)
+    for src, out in PROBES:
+        tmp = out + ".tmp"
+        r = subprocess.run([HIPCC, *FLAGS, "-shared", "-o", tmp, src], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed for {os.path.basename(src)}:\n{r.stderr}")
+        os.replace(tmp, out)
     if verbose:
-        print(f"built {OUT} and {PROBE_OUT}")
+        print(f"built {OUT} and the probes {', '.join(os.path.basename(o) for _, o in PROBES)}")
     return OUT
 
 
