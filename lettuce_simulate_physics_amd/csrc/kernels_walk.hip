@@ -57,6 +57,7 @@
 #include <array>
 #include <cfloat>
 #include <cstring>
+#include <type_traits>
 
 #include "kernels.hpp"
 #include "tree.hpp"
@@ -460,6 +461,15 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                 }
             };
 
+            // A local body's list entry: its position (relative to c in the mixed form) and mass.
+            auto body_at = [&](int j) {
+                if (REL) {
+                    const double4 d = a.posd[j];
+                    return make_float4(float(d.x - ccx), float(d.y - ccy), float(d.z - ccz), float(d.w));
+                }
+                return a.posm[j];
+            };
+
             int nc = 0, nb = 0;
             uint32_t ncell_total = 0, nbody_total = 0, nmono_total = 0;  // body list = bodies + monopoles
             auto flush_cells = [&]() {
@@ -486,7 +496,19 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                 ncell_total += uint32_t(nc);
                 nc = 0;
             };
+            // A local body enters the body list as its index only (bI); its position is loaded here,
+            // for the whole list in one round trip, just before the list is evaluated. Monopoles and
+            // remote bodies (bI = -1) were written with their positions.
             auto flush_bodies = [&]() {
+                {
+                    const int j0 = lane < nb ? L.bI[lane] : -1;
+                    const int j1 = lane + 64 < nb ? L.bI[lane + 64] : -1;
+                    float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0;
+                    if (j0 >= 0) q0 = body_at(j0);
+                    if (j1 >= 0) q1 = body_at(j1);
+                    if (j0 >= 0) L.bP[lane] = q0;
+                    if (j1 >= 0) L.bP[lane + 64] = q1;
+                }
                 if (lane < kPad) {
                     L.bP[nb + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
                     L.bI[nb + lane] = -1;
@@ -541,8 +563,10 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                 }
             };
             // Body ranges [lo, lo + len) of the staging lanes, in chunks of at most the free buffer
-            // space; index(j) is what the self test compares (-1: never the target).
-            auto stage_ranges = [&](bool is_range, int lo, int len_in, auto&& load, auto&& index) {
+            // space. Local bodies (load null) enter as their indices, loaded when the list is
+            // evaluated; remote ones (index -1: never the target) are loaded here.
+            auto stage_ranges = [&](bool is_range, int lo, int len_in, auto&& load) {
+                constexpr bool local = std::is_same_v<std::decay_t<decltype(load)>, std::nullptr_t>;
                 const unsigned long long mr = __ballot(is_range);
                 if (mr) {
                     const int len = is_range ? len_in : 0;
@@ -554,18 +578,22 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                         if (nb == kBodyCap) flush_bodies();
                         const int take = min(kBodyCap - nb, total - done);
                         const int u0 = max(off, done), u1 = min(off + len, done + take);
-                        // Four loads in flight per lane before their LDS stores (a load past u1 reads
-                        // the range's last body again and is not stored).
-                        for (int u = u0; u < u1; u += 4) {
-                            float4 q[4];
+                        if constexpr (local) {
+                            for (int u = u0; u < u1; ++u) L.bI[nb + (u - done)] = lo + (u - off);
+                        } else {
+                            // Four loads in flight per lane before their LDS stores (a load past u1
+                            // reads the range's last body again and is not stored).
+                            for (int u = u0; u < u1; u += 4) {
+                                float4 q[4];
 #pragma unroll
-                            for (int t = 0; t < 4; ++t) q[t] = load(lo + (min(u + t, u1 - 1) - off));
+                                for (int t = 0; t < 4; ++t) q[t] = load(lo + (min(u + t, u1 - 1) - off));
 #pragma unroll
-                            for (int t = 0; t < 4; ++t)
-                                if (u + t < u1) {
-                                    L.bP[nb + (u + t - done)] = q[t];
-                                    if (MASK) L.bI[nb + (u + t - done)] = index(lo + (u + t - off));
-                                }
+                                for (int t = 0; t < 4; ++t)
+                                    if (u + t < u1) {
+                                        L.bP[nb + (u + t - done)] = q[t];
+                                        L.bI[nb + (u + t - done)] = -1;
+                                    }
+                            }
                         }
                         nb += take;
                         done += take;
@@ -637,16 +665,7 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                         return REL ? make_float4(float(pc.x - ccx), float(pc.y - ccy), float(pc.z - ccz), float(pc.w))
                                    : cA;
                     });
-                    stage_ranges(
-                        is_range, rlo, rhi - rlo,
-                        [&](int j) {
-                            if (REL) {
-                                const double4 d = a.posd[j];
-                                return make_float4(float(d.x - ccx), float(d.y - ccy), float(d.z - ccz), float(d.w));
-                            }
-                            return a.posm[j];
-                        },
-                        [](int j) { return j; });
+                    stage_ranges(is_range, rlo, rhi - rlo, nullptr);
                 }
             } else if (!blocks) {
                 if (lane == 0) st[0] = 0;  // the root
@@ -739,16 +758,7 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                         return REL ? make_float4(float(pc.x - ccx), float(pc.y - ccy), float(pc.z - ccz), float(pc.w))
                                    : make_float4(nd.cx, nd.cy, nd.cz, pm);
                     });
-                    stage_ranges(
-                        is_range, nd.lo, nd.hi - nd.lo,
-                        [&](int j) {
-                            if (REL) {
-                                const double4 d = a.posd[j];
-                                return make_float4(float(d.x - ccx), float(d.y - ccy), float(d.z - ccz), float(d.w));
-                            }
-                            return a.posm[j];
-                        },
-                        [](int j) { return j; });
+                    stage_ranges(is_range, nd.lo, nd.hi - nd.lo, nullptr);
                 }
             }
             if constexpr (LET) {
@@ -805,8 +815,7 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                             return c;
                         });
                         stage_monos(is_mono, [&]() { return rel(r.A); });
-                        stage_ranges(
-                            is_range, r.D.z, r.D.w, [&](int j) { return rel(a.let_body[j]); }, [](int) { return -1; });
+                        stage_ranges(is_range, r.D.z, r.D.w, [&](int j) { return rel(a.let_body[j]); });
                     }
                 }
             }
