@@ -218,9 +218,17 @@ __global__ void __launch_bounds__(kBS4, 4) tree_vjp_kernel(TreeVjpArgs a) {
                 for (int h = 0; h < 2; ++h) {
                     const int e = h * 64 + lane;
                     const bool valid = e < nb;
-                    P[h] = valid ? L.bP[e] : make_float4(0.f, 0.f, 0.f, 0.f);
                     J[h] = valid ? L.bI[e] : -1;
+                    P[h] = valid && J[h] < -1 ? L.bP[e] : make_float4(0.f, 0.f, 0.f, 0.f);
                 }
+                // Bodies entered the batch as their indices (as in the walk): positions load here,
+                // the batch's in one round trip.
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+                    if (J[h] >= 0) {
+                        const double4 d = a.posd[J[h]];
+                        P[h] = make_float4(float(d.x - ccx), float(d.y - ccy), float(d.z - ccz), float(d.w));
+                    }
                 float s[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
                 float* out = reinterpret_cast<float*>(L.bP);  // 512 floats
                 __builtin_amdgcn_wave_barrier();
@@ -392,22 +400,7 @@ __global__ void __launch_bounds__(kBS4, 4) tree_vjp_kernel(TreeVjpArgs a) {
                         if (nb == kBodies) flush_bodies();
                         const int take = min(kBodies - nb, total - done);
                         const int u0 = max(off, done), u1 = min(off + len, done + take);
-                        // Four loads in flight per lane before their LDS stores (a load past u1 reads
-                        // the range's last body again and is not stored).
-                        for (int u = u0; u < u1; u += 4) {
-                            float4 q[4];
-#pragma unroll
-                            for (int t = 0; t < 4; ++t) {
-                                const double4 d = a.posd[nd.lo + (min(u + t, u1 - 1) - off)];
-                                q[t] = make_float4(float(d.x - ccx), float(d.y - ccy), float(d.z - ccz), float(d.w));
-                            }
-#pragma unroll
-                            for (int t = 0; t < 4; ++t)
-                                if (u + t < u1) {
-                                    L.bP[nb + (u + t - done)] = q[t];
-                                    L.bI[nb + (u + t - done)] = nd.lo + (u + t - off);
-                                }
-                        }
+                        for (int u = u0; u < u1; ++u) L.bI[nb + (u - done)] = nd.lo + (u - off);
                         nb += take;
                         done += take;
                     }
