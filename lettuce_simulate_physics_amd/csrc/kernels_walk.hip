@@ -469,10 +469,31 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                 }
                 return a.posm[j];
             };
+            // Mixed form: a node's FP64 com and mass relative to c, as a monopole or a cell enters.
+            auto com_at = [&](int k) {
+                const NodeMomD& dm = a.momd[k];
+                return make_float4(float(dm.cx - ccx), float(dm.cy - ccy), float(dm.cz - ccz), float(dm.m));
+            };
 
             int nc = 0, nb = 0;
             uint32_t ncell_total = 0, nbody_total = 0, nmono_total = 0;  // body list = bodies + monopoles
+            // Mixed form: a local cell enters the cell list as its node index (pad.x); its FP64 com,
+            // mass and quadrupole are loaded here, one cell per lane. Remote cells (pad.x = -1) were
+            // written whole.
             auto flush_cells = [&]() {
+                if constexpr (REL) {
+                    const int k = lane < nc ? __float_as_int(L.c[lane].pad.x) : -1;
+                    if (k >= 0) {
+                        const float4 A = com_at(k);
+                        const NodeMom& md = a.mom[k];
+                        const float4 B = make_float4(md.qxx, md.qxy, md.qxz, md.qyy);
+                        const float2 C = make_float2(md.qyz, md.qzz);
+                        CellL& d = L.c[lane];
+                        d.A = A;
+                        d.B = B;
+                        d.C = C;
+                    }
+                }
                 if (lane < kPad) {
                     L.c[nc + lane].A = make_float4(0.f, 0.f, 0.f, 0.f);
                     L.c[nc + lane].B = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -497,17 +518,20 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                 nc = 0;
             };
             // A local body enters the body list as its index only (bI); its position is loaded here,
-            // for the whole list in one round trip, just before the list is evaluated. Monopoles and
-            // remote bodies (bI = -1) were written with their positions.
+            // for the whole list in one round trip, just before the list is evaluated. So does a local
+            // monopole of the mixed form (bI = -(node + 2)). Other monopoles and remote bodies
+            // (bI = -1) were written with their positions.
             auto flush_bodies = [&]() {
                 {
                     const int j0 = lane < nb ? L.bI[lane] : -1;
                     const int j1 = lane + 64 < nb ? L.bI[lane + 64] : -1;
                     float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0;
                     if (j0 >= 0) q0 = body_at(j0);
+                    else if (REL && j0 < -1) q0 = com_at(-j0 - 2);
                     if (j1 >= 0) q1 = body_at(j1);
-                    if (j0 >= 0) L.bP[lane] = q0;
-                    if (j1 >= 0) L.bP[lane + 64] = q1;
+                    else if (REL && j1 < -1) q1 = com_at(-j1 - 2);
+                    if (j0 >= 0 || (REL && j0 < -1)) L.bP[lane] = q0;
+                    if (j1 >= 0 || (REL && j1 < -1)) L.bP[lane + 64] = q1;
                 }
                 if (lane < kPad) {
                     L.bP[nb + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -544,8 +568,29 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                         d.A = c.A;
                         d.B = c.B;
                         d.C = c.C;
+                        if (REL) d.pad.x = __int_as_float(-1);
                     }
                     nc += ncnew;
+                }
+            };
+            // Mixed form: local cells and monopoles enter as node indices (loaded by the flushes).
+            auto stage_cells_at = [&](bool is_cell, int k) {
+                const unsigned long long mc = __ballot(is_cell);
+                const int ncnew = __popcll(mc);
+                if (ncnew) {
+                    if (nc + ncnew > kCellCap) flush_cells();
+                    if (is_cell) L.c[nc + int(lanes_below(mc))].pad.x = __int_as_float(k);
+                    nc += ncnew;
+                }
+            };
+            auto stage_monos_at = [&](bool is_mono, int k) {
+                const unsigned long long mm = __ballot(is_mono);
+                const int nmnew = __popcll(mm);
+                if (nmnew) {
+                    if (nb + nmnew > kBodyCap) flush_bodies();
+                    if (is_mono) L.bI[nb + int(lanes_below(mm))] = -(k + 2);
+                    nb += nmnew;
+                    nmono_total += uint32_t(nmnew);
                 }
             };
             auto stage_monos = [&](bool is_mono, auto&& get) {
@@ -630,18 +675,12 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                     const int k = int(e & kEntIndex);
                     const bool is_cell = act && tag == kEntCell, is_mono = act && tag == kEntMono;
                     const bool is_range = act && tag == kEntRange;
-                    double4 pc = make_double4(0.0, 0.0, 0.0, 0.0);
                     float4 cA = make_float4(0.f, 0.f, 0.f, 0.f), cB = cA;
                     float2 cC = make_float2(0.f, 0.f);
                     int rlo = 0, rhi = 0;
-                    if (is_cell || is_mono) {
+                    if (!REL && (is_cell || is_mono)) {
                         const float4* r = reinterpret_cast<const float4*>(a.mom + k);
-                        if (REL) {
-                            const NodeMomD& dm = a.momd[k];
-                            pc = make_double4(dm.cx, dm.cy, dm.cz, dm.m);
-                        } else {
-                            cA = r[0];
-                        }
+                        cA = r[0];
                         if (is_cell) {
                             cB = r[1];
                             const float4 r2 = r[2];
@@ -653,18 +692,19 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                         rlo = lh.x;
                         rhi = lh.y;
                     }
-                    stage_cells(is_cell, [&]() {
-                        CellL c;
-                        c.A = REL ? make_float4(float(pc.x - ccx), float(pc.y - ccy), float(pc.z - ccz), float(pc.w))
-                                  : cA;
-                        c.B = cB;
-                        c.C = cC;
-                        return c;
-                    });
-                    stage_monos(is_mono, [&]() {
-                        return REL ? make_float4(float(pc.x - ccx), float(pc.y - ccy), float(pc.z - ccz), float(pc.w))
-                                   : cA;
-                    });
+                    if constexpr (REL) {
+                        stage_cells_at(is_cell, k);
+                        stage_monos_at(is_mono, k);
+                    } else {
+                        stage_cells(is_cell, [&]() {
+                            CellL c;
+                            c.A = cA;
+                            c.B = cB;
+                            c.C = cC;
+                            return c;
+                        });
+                        stage_monos(is_mono, [&]() { return cA; });
+                    }
                     stage_ranges(is_range, rlo, rhi - rlo, nullptr);
                 }
             } else if (!blocks) {
@@ -680,19 +720,17 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                     __builtin_amdgcn_wave_barrier();  // the popped entries are read before pushes overwrite them
                     top -= P;
                     WalkNode nd{0.f, 0.f, 0.f, 0.f, 0, 0, 0, -1};
-                    // Mixed form: the FP64 com and mass (what a monopole or a cell stages) and qrel
-                    // (the far-field decision) load together with the node, so that deciding and
-                    // staging wait for one round trip.
-                    // Plain form: the mass, quadrupole and qrel (the com is the node record's) likewise.
-                    double4 pc = make_double4(0.0, 0.0, 0.0, 0.0);
+                    // Plain form: the mass, quadrupole and qrel (the com is the node record's) load
+                    // together with the node, so that deciding and staging wait for one round trip.
+                    // Mixed form: qrel (the far-field decision) only; accepted cells and monopoles
+                    // enter the lists as node indices and their FP64 moments load when a list is
+                    // evaluated.
                     float qr = 0.f, pm = 0.f;
                     float4 pq = make_float4(0.f, 0.f, 0.f, 0.f);
                     float2 pq2 = make_float2(0.f, 0.f);
                     if (act) {
                         nd = a.node[k];
                         if (REL) {
-                            const NodeMomD& dm = a.momd[k];
-                            pc = make_double4(dm.cx, dm.cy, dm.cz, dm.m);
                             if (a.quad_tau > 0.f) qr = a.mom[k].qrel;
                         } else {
                             const float4* r = reinterpret_cast<const float4*>(a.mom + k);
@@ -740,24 +778,19 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
                         top += pushed;
                     }
 
-                    stage_cells(is_cell, [&]() {
-                        CellL c;
-                        if (REL) {
-                            const NodeMom& md = a.mom[k];
-                            c.A = make_float4(float(pc.x - ccx), float(pc.y - ccy), float(pc.z - ccz), float(pc.w));
-                            c.B = make_float4(md.qxx, md.qxy, md.qxz, md.qyy);
-                            c.C = make_float2(md.qyz, md.qzz);
-                        } else {
+                    if constexpr (REL) {
+                        stage_cells_at(is_cell, k);
+                        stage_monos_at(is_mono, k);
+                    } else {
+                        stage_cells(is_cell, [&]() {
+                            CellL c;
                             c.A = make_float4(nd.cx, nd.cy, nd.cz, pm);
                             c.B = pq;
                             c.C = pq2;
-                        }
-                        return c;
-                    });
-                    stage_monos(is_mono, [&]() {
-                        return REL ? make_float4(float(pc.x - ccx), float(pc.y - ccy), float(pc.z - ccz), float(pc.w))
-                                   : make_float4(nd.cx, nd.cy, nd.cz, pm);
-                    });
+                            return c;
+                        });
+                        stage_monos(is_mono, [&]() { return make_float4(nd.cx, nd.cy, nd.cz, pm); });
+                    }
                     stage_ranges(is_range, nd.lo, nd.hi - nd.lo, nullptr);
                 }
             }
