@@ -138,10 +138,19 @@ __global__ void __launch_bounds__(kBS4, 4) tree_vjp_kernel(TreeVjpArgs a) {
             int nc = 0, nb = 0;
             auto flush_cells = [&]() {
                 __builtin_amdgcn_wave_barrier();
+                // Cells entered the batch as node indices: their FP64 com and mass (relative to the
+                // group centre) and quadrupole load here, one cell per lane.
                 const bool valid = lane < nc;
                 float4 A = make_float4(0.f, 0.f, 0.f, 0.f), B = A;
                 float2 C = make_float2(0.f, 0.f);
-                if (valid) { A = L.cA[lane]; B = L.cB[lane]; C = L.cC[lane]; }
+                if (valid) {
+                    const int k = L.cI[lane];
+                    const NodeMomD& dm = a.momd[k];
+                    const NodeMom& md = a.mom[k];
+                    A = make_float4(float(dm.cx - ccx), float(dm.cy - ccy), float(dm.cz - ccz), float(dm.m));
+                    B = make_float4(md.qxx, md.qxy, md.qxz, md.qyy);
+                    C = make_float2(md.qyz, md.qzz);
+                }
                 const float qxx = B.x, qxy = B.y, qxz = B.z, qyy = B.w, qyz = C.x, qzz = C.y;
                 float sCx = 0.f, sCy = 0.f, sCz = 0.f;
                 float Gxx = 0.f, Gxy = 0.f, Gxz = 0.f, Gyy = 0.f, Gyz = 0.f, Gzz = 0.f;
@@ -219,16 +228,20 @@ __global__ void __launch_bounds__(kBS4, 4) tree_vjp_kernel(TreeVjpArgs a) {
                     const int e = h * 64 + lane;
                     const bool valid = e < nb;
                     J[h] = valid ? L.bI[e] : -1;
-                    P[h] = valid && J[h] < -1 ? L.bP[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    P[h] = make_float4(0.f, 0.f, 0.f, 0.f);
                 }
-                // Bodies entered the batch as their indices (as in the walk): positions load here,
-                // the batch's in one round trip.
+                // Bodies and monopole cells entered the batch as indices (as in the walk): positions,
+                // FP64 coms and masses load here, the batch's in one round trip.
 #pragma unroll
-                for (int h = 0; h < 2; ++h)
+                for (int h = 0; h < 2; ++h) {
                     if (J[h] >= 0) {
                         const double4 d = a.posd[J[h]];
                         P[h] = make_float4(float(d.x - ccx), float(d.y - ccy), float(d.z - ccz), float(d.w));
+                    } else if (J[h] < -1) {
+                        const NodeMomD& dm = a.momd[-J[h] - 2];
+                        P[h] = make_float4(float(dm.cx - ccx), float(dm.cy - ccy), float(dm.cz - ccz), float(dm.m));
                     }
+                }
                 float s[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
                 float* out = reinterpret_cast<float*>(L.bP);  // 512 floats
                 __builtin_amdgcn_wave_barrier();
@@ -313,13 +326,11 @@ __global__ void __launch_bounds__(kBS4, 4) tree_vjp_kernel(TreeVjpArgs a) {
                 __builtin_amdgcn_wave_barrier();
                 top -= P;
                 WalkNode nd{0.f, 0.f, 0.f, 0.f, 0, 0, 0, -1};
-                // The FP64 com and mass and qrel load with the node (one round trip), as in the walk.
-                double4 pc = make_double4(0.0, 0.0, 0.0, 0.0);
+                // qrel loads with the node (one round trip), as in the walk; cells and monopoles enter
+                // the batches as node indices.
                 float qr = 0.f;
                 if (act) {
                     nd = a.node[k];
-                    const NodeMomD& dm0 = a.momd[k];
-                    pc = make_double4(dm0.cx, dm0.cy, dm0.cz, dm0.m);
                     if (a.quad_tau > 0.f) qr = a.mom[k].qrel;
                 }
                 const bool keep_open = chunked ? (nd.lo < chi && clo < nd.hi) : (nd.lo <= clo && chi <= nd.hi);
@@ -367,26 +378,14 @@ __global__ void __launch_bounds__(kBS4, 4) tree_vjp_kernel(TreeVjpArgs a) {
                 const int ncnew = __popcll(mc);
                 if (ncnew) {
                     if (nc + ncnew > kCells) flush_cells();
-                    if (is_cell) {
-                        const int slot = nc + int(lanes_below(mc));
-                        const NodeMom& md = a.mom[k];
-                        L.cA[slot] = make_float4(float(pc.x - ccx), float(pc.y - ccy), float(pc.z - ccz), float(pc.w));
-                        L.cB[slot] = make_float4(md.qxx, md.qxy, md.qxz, md.qyy);
-                        L.cC[slot] = make_float2(md.qyz, md.qzz);
-                        L.cI[slot] = k;
-                    }
+                    if (is_cell) L.cI[nc + int(lanes_below(mc))] = k;
                     nc += ncnew;
                 }
                 const unsigned long long mm = __ballot(is_mono);
                 const int nmnew = __popcll(mm);
                 if (nmnew) {
                     if (nb + nmnew > kBodies) flush_bodies();
-                    if (is_mono) {
-                        const int slot = nb + int(lanes_below(mm));
-                        L.bP[slot] = make_float4(float(pc.x - ccx), float(pc.y - ccy), float(pc.z - ccz),
-                                                 float(pc.w));
-                        L.bI[slot] = -(k + 2);
-                    }
+                    if (is_mono) L.bI[nb + int(lanes_below(mm))] = -(k + 2);
                     nb += nmnew;
                 }
                 const unsigned long long mr = __ballot(is_range);
