@@ -700,6 +700,7 @@ struct Context {
         a.gstart = walk_starts();
         a.gcount = walk_count();
         a.order = p.method == LHIP_METHOD_TREE ? worder.as<uint32_t>() : nullptr;
+        a.ranges = a.order ? 1 : 0;  // the build wrote the order's ranges
         a.n = n;
         a.eps2 = float(p.eps * p.eps);
         a.G = float(p.G);

@@ -71,7 +71,8 @@ std::size_t radix_work_words(int n);
 // (d1 - d0) & 1. n_dev (optional) holds the element count on the device, n is then the capacity.
 // first_counted: the kernel that made the keys already wrote digit d0's tile counts (the body
 // sort's keys kernel). The last pass can also write inv[value] = sorted position for every element
-// and its digit starts (exclusive sums of the digit totals) to starts[0 .. nstarts), nstarts <= 256.
+// and its digit starts (exclusive sums of the digit totals) to starts[0 .. nstarts), nstarts <= 256;
+// with starts_stride s > 1, only the starts of digits 0, s, 2s, ... < nstarts, to starts[d / s].
 // Returns the digit totals of the last pass, valid until the next sort on the same work memory.
 struct RadixBufs {
     uint64_t* k[2];
@@ -79,7 +80,8 @@ struct RadixBufs {
 };
 const uint32_t* radix_sort(hipStream_t s, RadixWork& w, RadixBufs b, bool iota_values, int n,
                            const uint32_t* n_dev, int d0, int d1, bool first_counted = false,
-                           uint32_t* inv = nullptr, uint32_t* starts = nullptr, int nstarts = 0);
+                           uint32_t* inv = nullptr, uint32_t* starts = nullptr, int nstarts = 0,
+                           int starts_stride = 1);
 
 struct TreeBufs {
     bool f64;  // positions and velocities are double (the mixed form), else float

@@ -759,17 +759,27 @@ __global__ void __launch_bounds__(kBS) packs_kernel(const uint32_t* __restrict__
 // 255 - 8 log2(estimate) (8 buckets per octave), the others 255, so one stable 8-bit radix pass
 // puts the costly units first, longest first, and leaves the rest in their natural (Morton) order,
 // which keeps neighbouring groups, and the tree nodes they read, together in time.
+// ranges: the key is (range << 5) | (31 - bucket) with range = the unit's eighth of the Morton order
+// and a coarser bucket (units at or above thr: 1 + 2.5 log2(est / thr), at most 31), so that each
+// range holds its costly units first, longest first, then the others in Morton order.
 __global__ void __launch_bounds__(kBS) unit_order_keys(const uint32_t* __restrict__ ustart,
                                                        const uint32_t* __restrict__ ucount, int n,
-                                                       const float* __restrict__ cost, float thr,
+                                                       const float* __restrict__ cost, float thr, int ranges,
                                                        uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
     const int g = blockIdx.x * kBS + threadIdx.x;
-    if (g >= int(ucount[n])) return;
+    const int units = int(ucount[n]);
+    if (g >= units) return;
     const int lo = int(ustart[g]), hi = int(ustart[g + 1]);
     float est = (fmaxf(cost[lo], 0.f) + 1.f) * float(hi - lo + 8);
     est = est >= 1.f ? fminf(est, 1e30f) : 1.f;  // NaN and garbage included
-    const int bucket = est >= thr ? min(255, int(8.f * log2f(est))) : 0;
-    key[g] = uint64_t(255 - bucket);
+    if (ranges) {
+        const int range = int((long long)g * kWalkRanges / units);
+        const int bucket = est >= thr ? min(31, 1 + int(2.5f * log2f(est / thr))) : 0;
+        key[g] = uint64_t((range << 5) | (31 - bucket));
+    } else {
+        const int bucket = est >= thr ? min(255, int(8.f * log2f(est))) : 0;
+        key[g] = uint64_t(255 - bucket);
+    }
     val[g] = uint32_t(g);
 }
 
@@ -999,11 +1009,15 @@ inline unsigned level_grid(int n, int level) {
     return std::min<unsigned>(grid_for(std::min<long long>(n, most), kBS), 512u);
 }
 
+// ranges: the order is kWalkRanges Morton ranges (costly units first in each), their starts written
+// to counters->range_start (the digit starts of digits 0, 32, ..., 224).
 void unit_order(hipStream_t s, const BuildBufs& b, const uint32_t* ustart, const uint32_t* ucount, float thr,
-                uint32_t* order) {
-    unit_order_keys<<<grid_for(b.n, kBS), kBS, 0, s>>>(ustart, ucount, b.n, b.wcost, thr, b.pkey, b.pval);
+                bool ranges, uint32_t* order) {
+    unit_order_keys<<<grid_for(b.n, kBS), kBS, 0, s>>>(ustart, ucount, b.n, b.wcost, thr, ranges ? 1 : 0, b.pkey,
+                                                       b.pval);
     LHIP_LAUNCH_CHECK();
-    radix_sort(s, *b.radix, RadixBufs{{b.pkey, b.pkey2}, {b.pval, order}}, false, b.n, ucount + b.n, 0, 1);
+    radix_sort(s, *b.radix, RadixBufs{{b.pkey, b.pkey2}, {b.pval, order}}, false, b.n, ucount + b.n, 0, 1, false,
+               nullptr, ranges ? range_starts_of(b.counters) : nullptr, ranges ? kRadix : 0, kRadix / kWalkRanges);
 }
 
 // The multipole launches over a built level order: every leaf in one launch, then the internal
@@ -1112,7 +1126,8 @@ int tree_build(hipStream_t s, const BuildBufs& b, float inv_theta, hipEvent_t le
     }
     if (b.worder && b.wcost) {
         // The walk order of this tree's units (queued before the host waits on the level pass).
-        unit_order(s, b, b.pmax > 0 ? b.pstart : b.gstart, b.pmax > 0 ? b.pflag : b.gflag, b.order_thr, b.worder);
+        unit_order(s, b, b.pmax > 0 ? b.pstart : b.gstart, b.pmax > 0 ? b.pflag : b.gflag, b.order_thr, true,
+                   b.worder);
     }
     int depth = tree_depth(ext ? b.frames : 1);
     if (wait_levels) {

@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(kSortThreads) radix_row_scan(uint32_t* __restr
 }
 
 // vin null: the values are the input positions. inv (optional): inv[value] = its sorted position;
-// starts (optional): block 0 writes the digit starts 0 .. nstarts - 1.
+// starts (optional): block 0 writes the starts of digits 0, ss, 2 ss, ... < nstarts to starts[d / ss].
 __global__ void __launch_bounds__(kSortThreads) radix_scatter(const uint64_t* __restrict__ kin,
                                                               const uint32_t* __restrict__ vin,
                                                               uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
@@ -165,7 +165,7 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter(const uint64_t* __
                                                               int ntiles, const uint32_t* __restrict__ hist,
                                                               const uint32_t* __restrict__ tot,
                                                               uint32_t* __restrict__ inv, uint32_t* __restrict__ starts,
-                                                              int nstarts) {
+                                                              int nstarts, int ss) {
     __shared__ uint64_t sk[kSortTile];
     __shared__ uint32_t sv[kSortTile];
     __shared__ uint32_t wc[kSortWaves][kRadix];   // the waves' digit counts, then their tile offsets
@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter(const uint64_t* __
     wave_rank(dg, pos, wc[wave], lane);
     // Global digit starts: exclusive scan of the digit totals (its barrier also completes the ranks).
     const uint32_t dbase = block_scan_excl<kSortWaves>(my_tot, wsum[0], lane, wave, nullptr);
-    if (starts && blockIdx.x == 0 && tid < nstarts) starts[tid] = dbase;
+    if (starts && blockIdx.x == 0 && tid < nstarts && tid % ss == 0) starts[tid / ss] = dbase;
     uint32_t c[kSortWaves], all = 0;
 #pragma unroll
     for (int w = 0; w < kSortWaves; ++w) {
@@ -323,8 +323,9 @@ void sort_impl(hipStream_t s, TreeBufs& b, int n, bool permute_cost) {
 std::size_t radix_work_words(int n) { return std::size_t(std::max(sort_tiles(n), 1)) * kRadix + kRadix; }
 
 const uint32_t* radix_sort(hipStream_t s, RadixWork& w, RadixBufs b, bool iota_values, int n, const uint32_t* n_dev,
-                           int d0, int d1, bool first_counted, uint32_t* inv, uint32_t* starts, int nstarts) {
-    if (d0 < 0 || d1 > kRadixDigits || d0 > d1 || nstarts < 0 || nstarts > kRadix)
+                           int d0, int d1, bool first_counted, uint32_t* inv, uint32_t* starts, int nstarts,
+                           int starts_stride) {
+    if (d0 < 0 || d1 > kRadixDigits || d0 > d1 || nstarts < 0 || nstarts > kRadix || starts_stride < 1)
         throw Error(kInvalid, "radix_sort: digits or starts out of range");
     const int ntiles = sort_tiles(n);
     uint32_t* hist = w.words;
@@ -341,7 +342,8 @@ const uint32_t* radix_sort(hipStream_t s, RadixWork& w, RadixBufs b, bool iota_v
         const bool last = d + 1 == d1;
         radix_scatter<<<ntiles, kSortThreads, 0, s>>>(b.k[src], j == 0 && iota_values ? nullptr : b.v[src], b.k[dst],
                                                       b.v[dst], n, n_dev, 8 * d, ntiles, hist, tot,
-                                                      last ? inv : nullptr, last ? starts : nullptr, nstarts);
+                                                      last ? inv : nullptr, last ? starts : nullptr, nstarts,
+                                                      starts_stride);
         LHIP_LAUNCH_CHECK();
     }
     return tot;

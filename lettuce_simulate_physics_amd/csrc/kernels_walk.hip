@@ -376,11 +376,34 @@ __global__ void __launch_bounds__(kWalkBS, 4) walk_kernel(WalkArgs a) {
     unsigned long long t_inter = 0, t_p2p = 0, t_p2c = 0, t_listed = 0, t_p2m = 0;
     bool overflow = false;
 
+    // With ranges, the waves of XCD x (blocks are placed on the XCDs round robin) take the positions
+    // of range x first (its costly units, then the others in Morton order), so that the groups one
+    // XCD walks at a time are neighbours and share its L2; then the other ranges. Lane r holds the
+    // start of range r (loaded once: the counters' line is busy with the atomics).
+    const bool ranged = a.ranges && !blocks && a.order;
+    const int rstart = ranged && lane < kWalkRanges ? min(ngroups, int(a.counters->range_start[lane])) : ngroups;
+    const int xcd = int(blockIdx.x) & (kWalkRanges - 1);
+    int tried = 0;  // ranges found empty (wave-uniform)
     while (true) {
         int g = 0;
-        if (lane == 0) g = int(atomicAdd(&a.counters->next_group, 1u));
-        g = __builtin_amdgcn_readfirstlane(__shfl(g, 0, 64));
-        if (g >= ngroups) break;
+        if (ranged) {
+            bool got = false;
+            while (!got && tried < kWalkRanges) {
+                const int r = (xcd + tried) & (kWalkRanges - 1);
+                const int lo = __builtin_amdgcn_readlane(rstart, r), hi = __builtin_amdgcn_readlane(rstart, r + 1);
+                int t = 0;
+                if (lane == 0) t = int(atomicAdd(&a.counters->next_range[r][0], 1u));
+                t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
+                got = lo + t < hi;
+                if (got) g = lo + t;
+                else ++tried;
+            }
+            if (!got) break;
+        } else {
+            if (lane == 0) g = int(atomicAdd(&a.counters->next_group, 1u));
+            g = __builtin_amdgcn_readfirstlane(__shfl(g, 0, 64));
+            if (g >= ngroups) break;
+        }
         const int gu = !blocks && a.order ? int(a.order[g]) : g;  // the g-th group to take
         const int glo = blocks ? g * kGroupMax : int(a.gstart[gu]);
         const int ghi = blocks ? min(a.n, glo + kGroupMax) : int(a.gstart[gu + 1]);
@@ -966,6 +989,7 @@ __global__ void reset_walk(WalkCounters* __restrict__ c) {
     c->next_group = 0;
     c->next_list = 0;
     c->list_chunks = 0;
+    for (int r = 0; r < kWalkRanges; ++r) c->next_range[r][0] = 0;
 }
 
 

@@ -1,13 +1,14 @@
 // Device octree build and walk: buffers and launchers (kernels_build.hip, kernels_walk.hip).
 #pragma once
 #include <algorithm>
+#include <cstddef>
 
 #include "common.hpp"
 
 namespace lhip {
 
 // Counters written by the device during one force call, read back once at its end.
-struct alignas(16) WalkCounters {
+struct alignas(128) WalkCounters {
     unsigned long long interactions;  // body-body + body-node, self excluded
     unsigned long long p2p;           // body-body (self excluded)
     unsigned long long p2c;           // body-node
@@ -19,7 +20,19 @@ struct alignas(16) WalkCounters {
     int max_level;                    // build: deepest leaf level of the level pass (frames = 1)
     unsigned int next_list;           // list traversal work counter
     unsigned int list_chunks;         // list chunks the traversal asked for (> the pool's: overflow)
+    // The walk order's kWalkRanges ranges (eighths of the Morton order, each with its costly units
+    // first): range r holds the positions [range_start[r], range_start[r + 1]), the last ending at the
+    // unit count; written by the build's order pass. next_range[r][0] counts the positions taken
+    // from range r (each on its own 128-byte line: the waves of one XCD take from it), reset by
+    // each walk.
+    unsigned int range_start[8];
+    alignas(128) unsigned int next_range[8][32];
 };
+constexpr int kWalkRanges = 8;
+// Device address of counters->range_start (no host access to the device struct).
+inline uint32_t* range_starts_of(WalkCounters* c) {
+    return reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(c) + offsetof(WalkCounters, range_start));
+}
 
 // The walk's node record, in level order (all cells of level 0, then level 1, ...; within a level
 // by first body), so that the children of a node are contiguous: [first, first + nchild).
@@ -185,7 +198,7 @@ void tree_moments_keep(hipStream_t s, const BuildBufs& b);
 // The order in which a walk takes the units of a tree (ustart, count at ucount[n]): units estimated
 // at thr or more first, longest first, then the rest in their natural order (thr = 1: all sorted).
 // Estimates from b.wcost; scratch b.pkey, b.pval, b.pkey2, b.radix.
-void unit_order(hipStream_t s, const BuildBufs& b, const uint32_t* ustart, const uint32_t* ucount, float thr,
+void unit_order(hipStream_t s, const BuildBufs& b, const uint32_t* ustart, const uint32_t* ucount, float thr, bool ranges,
                 uint32_t* order);
 
 struct WalkArgs {
@@ -201,6 +214,8 @@ struct WalkArgs {
     const uint32_t* gstart;
     const uint32_t* gcount;    // group count = gcount[n]
     const uint32_t* order;     // optional: the order in which the waves take the groups (a permutation)
+    int ranges;                // 1: order is the build's, in kWalkRanges ranges (counters->range_start),
+                               // each taken first by the waves of one XCD
     int n;
     float eps2;
     float G;

@@ -466,7 +466,7 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
                                   hipMemcpyDeviceToDevice, s));
         // The VJP takes every group longest first (its waves gain more from balance than from the
         // forward walk's Morton locality).
-        unit_order(s, c.build_bufs(), c.walk_starts(), c.walk_count(), 1.f,
+        unit_order(s, c.build_bufs(), c.walk_starts(), c.walk_count(), 1.f, false,
                    S.tr_order.as<uint32_t>() + std::size_t(j) * (N + 1));
         S.tr_frames[j] = c.frames;
         S.tr_depth[j] = c.walk_depth();
@@ -511,7 +511,7 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
         a.gcount = stored ? S.tr_count.as<uint32_t>() + std::size_t(j) * (N + 1) : c.walk_count();
         if (!stored) {
             S.vjp_order.reserve((N + 1) * 4);
-            unit_order(s, c.build_bufs(), c.walk_starts(), c.walk_count(), 1.f, S.vjp_order.as<uint32_t>());
+            unit_order(s, c.build_bufs(), c.walk_starts(), c.walk_count(), 1.f, false, S.vjp_order.as<uint32_t>());
         }
         a.order = stored ? S.tr_order.as<uint32_t>() + std::size_t(j) * (N + 1) : S.vjp_order.as<uint32_t>();
         a.n = n;
