@@ -1009,15 +1009,15 @@ inline unsigned level_grid(int n, int level) {
     return std::min<unsigned>(grid_for(std::min<long long>(n, most), kBS), 512u);
 }
 
-// ranges: the order is kWalkRanges Morton ranges (costly units first in each), their starts written
-// to counters->range_start (the digit starts of digits 0, 32, ..., 224).
+// ranges (optional): the order is kWalkRanges Morton ranges (costly units first in each), their
+// starts written to ranges[0 .. kWalkRanges) (the digit starts of digits 0, 32, ..., 224).
 void unit_order(hipStream_t s, const BuildBufs& b, const uint32_t* ustart, const uint32_t* ucount, float thr,
-                bool ranges, uint32_t* order) {
+                uint32_t* ranges, uint32_t* order) {
     unit_order_keys<<<grid_for(b.n, kBS), kBS, 0, s>>>(ustart, ucount, b.n, b.wcost, thr, ranges ? 1 : 0, b.pkey,
                                                        b.pval);
     LHIP_LAUNCH_CHECK();
     radix_sort(s, *b.radix, RadixBufs{{b.pkey, b.pkey2}, {b.pval, order}}, false, b.n, ucount + b.n, 0, 1, false,
-               nullptr, ranges ? range_starts_of(b.counters) : nullptr, ranges ? kRadix : 0, kRadix / kWalkRanges);
+               nullptr, ranges, ranges ? kRadix : 0, kRadix / kWalkRanges);
 }
 
 // The multipole launches over a built level order: every leaf in one launch, then the internal
@@ -1126,8 +1126,8 @@ int tree_build(hipStream_t s, const BuildBufs& b, float inv_theta, hipEvent_t le
     }
     if (b.worder && b.wcost) {
         // The walk order of this tree's units (queued before the host waits on the level pass).
-        unit_order(s, b, b.pmax > 0 ? b.pstart : b.gstart, b.pmax > 0 ? b.pflag : b.gflag, b.order_thr, true,
-                   b.worder);
+        unit_order(s, b, b.pmax > 0 ? b.pstart : b.gstart, b.pmax > 0 ? b.pflag : b.gflag, b.order_thr,
+                   range_starts_of(b.counters), b.worder);
     }
     int depth = tree_depth(ext ? b.frames : 1);
     if (wait_levels) {

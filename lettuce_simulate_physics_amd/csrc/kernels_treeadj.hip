@@ -20,7 +20,7 @@
 // group's list, so its cotangents go to a private copy per block (gpriv, blocks x kHot x 12) that
 // hot_reduce adds up afterwards; deeper cells and bodies take global atomics directly.
 //
-// Evaluation is entry-major: each lane owns a list entry (one cell, or two bodies) and loops over
+// Evaluation is entry-major: each lane owns a list entry (one cell, or three bodies) and loops over
 // the group's targets four at a time, whose positions and cotangents come from the slot registers
 // by readlane. The lane keeps its entry's source-side sums; the target sides of four targets are
 // reduced over the wave together (wave_sum4: 32- and 16-lane swaps, then DPP butterflies) and
@@ -40,7 +40,8 @@ namespace {
 constexpr int kWaves = 4;
 constexpr int kBS4 = 64 * kWaves;
 constexpr int kCells = 64;     // cells per batch (one per lane)
-constexpr int kBodies = 128;   // bodies per batch (two per lane)
+constexpr int kPer = 3;        // body entries per lane in a batch
+constexpr int kBodies = 64 * kPer;  // bodies (and monopole cells) per batch
 constexpr int kStack = 896;
 
 struct VjpLds {
@@ -48,8 +49,8 @@ struct VjpLds {
     float4 cB[kCells];  // qxx, qxy, qxz, qyy
     float2 cC[kCells];  // qyz, qzz
     int cI[kCells];     // node index
-    float4 bP[kBodies]; // position relative to the group centre, mass
-    int bI[kBodies];    // body index; -(k + 2) for a monopole cell k
+    float bP[3 * kBodies];  // scratch of a batch: the target sums (3 x 128), the monopole cotangents
+    int bI[kBodies];        // body index; -(k + 2) for a monopole cell k
 };
 
 struct Target {
@@ -78,11 +79,31 @@ __global__ void __launch_bounds__(kBS4, 4) tree_vjp_kernel(TreeVjpArgs a) {
         return k < kHot ? priv + 9 * kHot + std::size_t(k) * 3 : a.gmono + std::size_t(k) * 3;
     };
 
+    // The walk's XCD-local scheduling over the order's ranges (kernels_walk.hip), or one counter.
+    const bool ranged = a.ranges && a.order;
+    const int rstart = ranged && lane < kWalkRanges ? min(ngroups, int(a.ranges[lane])) : ngroups;
+    const int xcd = int(blockIdx.x) & (kWalkRanges - 1);
+    int tried = 0;
     while (true) {
         int g = 0;
-        if (lane == 0) g = int(atomicAdd(&a.counters->next_group, 1u));
-        g = __builtin_amdgcn_readfirstlane(__shfl(g, 0, 64));
-        if (g >= ngroups) break;
+        if (ranged) {
+            bool got = false;
+            while (!got && tried < kWalkRanges) {
+                const int r = (xcd + tried) & (kWalkRanges - 1);
+                const int lo = __builtin_amdgcn_readlane(rstart, r), hi = __builtin_amdgcn_readlane(rstart, r + 1);
+                int t = 0;
+                if (lane == 0) t = int(atomicAdd(&a.counters->next_range[r][0], 1u));
+                t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
+                got = lo + t < hi;
+                if (got) g = lo + t;
+                else ++tried;
+            }
+            if (!got) break;
+        } else {
+            if (lane == 0) g = int(atomicAdd(&a.counters->next_group, 1u));
+            g = __builtin_amdgcn_readfirstlane(__shfl(g, 0, 64));
+            if (g >= ngroups) break;
+        }
         const int gu = a.order ? int(a.order[g]) : g;  // the g-th group to take
         const int glo = int(a.gstart[gu]), ghi = int(a.gstart[gu + 1]);
         const bool chunked = ghi - glo > kGroupMax;
@@ -221,10 +242,10 @@ __global__ void __launch_bounds__(kBS4, 4) tree_vjp_kernel(TreeVjpArgs a) {
             };
             auto flush_bodies = [&]() {
                 __builtin_amdgcn_wave_barrier();
-                float4 P[2];
-                int J[2];
+                float4 P[kPer];
+                int J[kPer];
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
+                for (int h = 0; h < kPer; ++h) {
                     const int e = h * 64 + lane;
                     const bool valid = e < nb;
                     J[h] = valid ? L.bI[e] : -1;
@@ -233,7 +254,7 @@ __global__ void __launch_bounds__(kBS4, 4) tree_vjp_kernel(TreeVjpArgs a) {
                 // Bodies and monopole cells entered the batch as indices (as in the walk): positions,
                 // FP64 coms and masses load here, the batch's in one round trip.
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
+                for (int h = 0; h < kPer; ++h) {
                     if (J[h] >= 0) {
                         const double4 d = a.posd[J[h]];
                         P[h] = make_float4(float(d.x - ccx), float(d.y - ccy), float(d.z - ccz), float(d.w));
@@ -242,8 +263,10 @@ __global__ void __launch_bounds__(kBS4, 4) tree_vjp_kernel(TreeVjpArgs a) {
                         P[h] = make_float4(float(dm.cx - ccx), float(dm.cy - ccy), float(dm.cz - ccz), float(dm.m));
                     }
                 }
-                float s[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
-                float* out = reinterpret_cast<float*>(L.bP);  // 512 floats
+                float s[kPer][3];
+#pragma unroll
+                for (int h = 0; h < kPer; ++h) s[h][0] = s[h][1] = s[h][2] = 0.f;
+                float* out = L.bP;
                 __builtin_amdgcn_wave_barrier();
                 // MASK: the batch holds bodies of the group itself, whose own entry adds nothing.
                 auto run = [&](auto mask) {
@@ -263,15 +286,17 @@ __global__ void __launch_bounds__(kBS4, 4) tree_vjp_kernel(TreeVjpArgs a) {
                                 const float uy = readlane_f(T.uy, t0 + k), uz = readlane_f(T.uz, t0 + k);
                                 const int it = clo + sl * 64 + t0 + k;
 #pragma unroll
-                                for (int h = 0; h < 2; ++h) {
+                                for (int h = 0; h < kPer; ++h) {
                                     const float dx = P[h].x - x, dy = P[h].y - y, dz = P[h].z - z;
                                     const float r2 = __fmaf_rn(dz, dz, __fmaf_rn(dy, dy, __fmaf_rn(dx, dx, e2)));
                                     const float inv = __builtin_amdgcn_rsqf(r2);
-                                    const float i3 = inv * inv * inv;
+                                    const float i2 = inv * inv, i3 = i2 * inv;
                                     const float m = (MASK && J[h] == it) ? 0.f : P[h].w;
                                     const float mi3 = m * i3;
-                                    const float c = -3.f * mi3 * inv * inv * (dx * ux + dy * uy + dz * uz);
-                                    const float wx = mi3 * ux + c * dx, wy = mi3 * uy + c * dy, wz = mi3 * uz + c * dz;
+                                    // w = m r^-3 (u - 3 r^-2 (d.u) d)
+                                    const float kd = (-3.f * i2) * __fmaf_rn(dz, uz, __fmaf_rn(dy, uy, dx * ux));
+                                    const float wx = mi3 * __fmaf_rn(kd, dx, ux), wy = mi3 * __fmaf_rn(kd, dy, uy),
+                                                wz = mi3 * __fmaf_rn(kd, dz, uz);
                                     s[h][0] += wx; s[h][1] += wy; s[h][2] += wz;
                                     tx[k] -= wx; ty[k] -= wy; tz[k] -= wz;
                                 }
@@ -281,27 +306,35 @@ __global__ void __launch_bounds__(kBS4, 4) tree_vjp_kernel(TreeVjpArgs a) {
                         }
                     }
                 };
-                const bool own = (J[0] >= clo && J[0] < chi) || (J[1] >= clo && J[1] < chi);
+                bool own = false;
+#pragma unroll
+                for (int h = 0; h < kPer; ++h) own = own || (J[h] >= clo && J[h] < chi);
                 if (__ballot(own)) run(std::true_type{});
                 else run(std::false_type{});
                 take_sums(out);
                 // Bodies: consecutive entries are consecutive bodies of a leaf, so the adds coalesce.
 #pragma unroll
-                for (int h = 0; h < 2; ++h)
+                for (int h = 0; h < kPer; ++h)
                     if (J[h] >= 0) {
                         unsafeAtomicAdd(a.gsx + J[h], double(s[h][0]));
                         unsafeAtomicAdd(a.gsy + J[h], double(s[h][1]));
                         unsafeAtomicAdd(a.gsz + J[h], double(s[h][2]));
                     }
                 // Monopole cells: compacted through LDS (bP, bI) and added component-major.
-                const unsigned long long m0 = __ballot(J[0] <= -2), m1 = __ballot(J[1] <= -2);
-                const int n0 = __popcll(m0), nm = n0 + __popcll(m1);
+                unsigned long long mm[kPer];
+                int mbase[kPer], nm = 0;
+#pragma unroll
+                for (int h = 0; h < kPer; ++h) {
+                    mm[h] = __ballot(J[h] <= -2);
+                    mbase[h] = nm;
+                    nm += __popcll(mm[h]);
+                }
                 if (nm) {
                     float* mb = out;
 #pragma unroll
-                    for (int h = 0; h < 2; ++h)
+                    for (int h = 0; h < kPer; ++h)
                         if (J[h] <= -2) {
-                            const int slot = (h ? n0 : 0) + int(lanes_below(h ? m1 : m0));
+                            const int slot = mbase[h] + int(lanes_below(mm[h]));
                             L.bI[slot] = -J[h] - 2;
                             mb[3 * slot] = s[h][0]; mb[3 * slot + 1] = s[h][1]; mb[3 * slot + 2] = s[h][2];
                         }
@@ -445,6 +478,7 @@ __global__ void reset_vjp(WalkCounters* __restrict__ c) {
     c->interactions = c->p2p = c->p2c = c->listed = c->p2m = 0;
     c->error = 0;
     c->next_group = 0;
+    for (int r = 0; r < kWalkRanges; ++r) c->next_range[r][0] = 0;
 }
 
 // Per-node A (3) and B (6: xx, xy, xz, yy, yz, zz) of the downward pass: the node's own terms
@@ -496,11 +530,13 @@ __global__ void __launch_bounds__(kBS) down_level(int level, const uint32_t* __r
 }
 
 // Bodies of each leaf: g_j += m_j (A + B x_j) with the leaf's accumulated A, B.
+// Each leaf's bodies j: the cell-moment term m_j (A + B x_j) of the leaf's cum, added to the
+// body's target and source sides, and the whole scattered into lambda_x by id (the last step of
+// the VJP; every body is in exactly one leaf).
 __global__ void __launch_bounds__(kBS) down_leaves(const uint32_t* __restrict__ node_count,
                                                    const WalkNode* __restrict__ node,
                                                    const double* __restrict__ cum, const double4* __restrict__ posd,
-                                                   double* __restrict__ gx, double* __restrict__ gy,
-                                                   double* __restrict__ gz) {
+                                                   TreeVjpArgs a) {
     const int p = blockIdx.x * kBS + threadIdx.x;
     if (p >= int(*node_count)) return;
     const WalkNode nd = node[p];
@@ -508,9 +544,10 @@ __global__ void __launch_bounds__(kBS) down_leaves(const uint32_t* __restrict__ 
     const double* c = cum + std::size_t(p) * 9;
     for (int j = nd.lo; j < nd.hi; ++j) {
         const double4 b = posd[j];
-        gx[j] += b.w * (c[0] + c[3] * b.x + c[4] * b.y + c[5] * b.z);
-        gy[j] += b.w * (c[1] + c[4] * b.x + c[6] * b.y + c[7] * b.z);
-        gz[j] += b.w * (c[2] + c[5] * b.x + c[7] * b.y + c[8] * b.z);
+        const uint32_t i = a.id[j];
+        a.lx[i] += a.gtx[j] + (a.gsx[j] + b.w * (c[0] + c[3] * b.x + c[4] * b.y + c[5] * b.z));
+        a.ly[i] += a.gty[j] + (a.gsy[j] + b.w * (c[1] + c[4] * b.x + c[6] * b.y + c[7] * b.z));
+        a.lz[i] += a.gtz[j] + (a.gsz[j] + b.w * (c[2] + c[5] * b.x + c[7] * b.y + c[8] * b.z));
     }
 }
 
@@ -534,7 +571,7 @@ void tree_vjp(hipStream_t s, const TreeVjpArgs& a, const uint32_t* levstart, con
         down_level<<<g, kBS, 0, s>>>(l, levstart, a.node, a.gcell, a.gmono, a.momd, cum);
         LHIP_LAUNCH_CHECK();
     }
-    down_leaves<<<grid_for(nodes, kBS), kBS, 0, s>>>(node_count, a.node, cum, a.posd, a.gsx, a.gsy, a.gsz);
+    down_leaves<<<grid_for(nodes, kBS), kBS, 0, s>>>(node_count, a.node, cum, a.posd, a);
     LHIP_LAUNCH_CHECK();
 }
 

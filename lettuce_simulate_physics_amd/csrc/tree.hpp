@@ -198,8 +198,8 @@ void tree_moments_keep(hipStream_t s, const BuildBufs& b);
 // The order in which a walk takes the units of a tree (ustart, count at ucount[n]): units estimated
 // at thr or more first, longest first, then the rest in their natural order (thr = 1: all sorted).
 // Estimates from b.wcost; scratch b.pkey, b.pval, b.pkey2, b.radix.
-void unit_order(hipStream_t s, const BuildBufs& b, const uint32_t* ustart, const uint32_t* ucount, float thr, bool ranges,
-                uint32_t* order);
+void unit_order(hipStream_t s, const BuildBufs& b, const uint32_t* ustart, const uint32_t* ucount, float thr,
+                uint32_t* ranges, uint32_t* order);
 
 struct WalkArgs {
     const float4* posm;        // plain form: sorted (position, mass)
@@ -290,6 +290,7 @@ struct TreeVjpArgs {
     const uint32_t* gstart;
     const uint32_t* gcount;
     const uint32_t* order;     // optional: the order in which the waves take the groups (a permutation)
+    const uint32_t* ranges;    // optional: the order's kWalkRanges range starts (as the walk's)
     int n;
     float eps2;
     int small_leaf;
@@ -302,6 +303,9 @@ struct TreeVjpArgs {
     WalkCounters* counters;
     int blocks;
     int depth;      // deepest level of the tree walked (at most 21 x key frames)
+    // The result: lx[id[k]] += gt_k + gs_k for every body k (sorted order) of the tree.
+    const uint32_t* id;
+    double *lx, *ly, *lz;
 };
 // Nodes at the top of the tree (levels 0-3 and part of 4 in level order) whose VJP cotangents each
 // block accumulates privately.

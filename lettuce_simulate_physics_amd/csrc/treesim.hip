@@ -39,22 +39,6 @@ __global__ void __launch_bounds__(kBS) gather_u_kernel(const uint32_t* __restric
     uz[k] = c * lz[j];
 }
 
-// lambda_x[id_k] += gt_k + gs_k.
-__global__ void __launch_bounds__(kBS) scatter_add_kernel(const uint32_t* __restrict__ id, const double* __restrict__ gtx,
-                                                          const double* __restrict__ gty,
-                                                          const double* __restrict__ gtz,
-                                                          const double* __restrict__ gsx,
-                                                          const double* __restrict__ gsy,
-                                                          const double* __restrict__ gsz, double* __restrict__ lx,
-                                                          double* __restrict__ ly, double* __restrict__ lz, int n) {
-    const int k = blockIdx.x * kBS + threadIdx.x;
-    if (k >= n) return;
-    const uint32_t j = id[k];
-    lx[j] += gtx[k] + gsx[k];
-    ly[j] += gty[k] + gsy[k];
-    lz[j] += gtz[k] + gsz[k];
-}
-
 __global__ void __launch_bounds__(kBS) inverse_kernel(const uint32_t* __restrict__ id, uint32_t* __restrict__ inv, int n) {
     const int k = blockIdx.x * kBS + threadIdx.x;
     if (k < n) inv[id[k]] = uint32_t(k);
@@ -130,6 +114,7 @@ struct TreeSim {
     // units and their count; tr_flag[k] = 1 where a tree did not fit.
     DevBuf tr_node, tr_mom, tr_momd, tr_lev, tr_cnt, tr_starts, tr_count, tr_order, tr_flag;
     DevBuf vjp_order;  // the VJP's group order on a tree built for it
+    DevBuf tr_ranges, vjp_ranges;  // the range starts of those orders (kWalkRanges per order)
     long long tr_cap = 0;
     int tr_slots = 0;
     std::vector<int> tr_frames;
@@ -440,6 +425,7 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
         S.tr_starts.reserve(sl * (N + 1) * 4);
         S.tr_count.reserve(sl * (N + 1) * 4);
         S.tr_order.reserve(sl * (N + 1) * 4);
+        S.tr_ranges.reserve(sl * kWalkRanges * 4);
         S.tr_flag.reserve(sl * 4);
         S.tr_frames.assign(sl, 1);
         S.tr_depth.assign(sl, tree_depth(1));
@@ -448,6 +434,9 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
     auto slot_mom = [&](int j) { return S.tr_mom.as<NodeMom>() + std::size_t(j) * S.tr_cap; };
     auto slot_momd = [&](int j) { return S.tr_momd.as<NodeMomD>() + std::size_t(j) * S.tr_cap; };
     // the tree the recompute just built at segment step k (1 <= k <= K - 1) into slot k - 1
+    // The VJP's order: eight Morton ranges (one per XCD first, as the walk's), each with its units
+    // above 1/64 of a wave's share of the last walk's work first, longest first.
+    auto vjp_order_threshold = [&]() { return c.walk_order_threshold() * 0.125f; };
     auto store_tree = [&](int k) {
         if (!tr_sized) size_store();
         if (k < 1 || k > S.tr_slots) return;
@@ -464,9 +453,8 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
                                   hipMemcpyDeviceToDevice, s));
         LHIP_CHECK(hipMemcpyAsync(S.tr_count.as<uint32_t>() + std::size_t(j) * (N + 1), c.walk_count(), (N + 1) * 4,
                                   hipMemcpyDeviceToDevice, s));
-        // The VJP takes every group longest first (its waves gain more from balance than from the
-        // forward walk's Morton locality).
-        unit_order(s, c.build_bufs(), c.walk_starts(), c.walk_count(), 1.f, false,
+        unit_order(s, c.build_bufs(), c.walk_starts(), c.walk_count(), vjp_order_threshold(),
+                   S.tr_ranges.as<uint32_t>() + std::size_t(j) * kWalkRanges,
                    S.tr_order.as<uint32_t>() + std::size_t(j) * (N + 1));
         S.tr_frames[j] = c.frames;
         S.tr_depth[j] = c.walk_depth();
@@ -511,9 +499,13 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
         a.gcount = stored ? S.tr_count.as<uint32_t>() + std::size_t(j) * (N + 1) : c.walk_count();
         if (!stored) {
             S.vjp_order.reserve((N + 1) * 4);
-            unit_order(s, c.build_bufs(), c.walk_starts(), c.walk_count(), 1.f, false, S.vjp_order.as<uint32_t>());
+            S.vjp_ranges.reserve(kWalkRanges * 4);
+            unit_order(s, c.build_bufs(), c.walk_starts(), c.walk_count(), vjp_order_threshold(),
+                       S.vjp_ranges.as<uint32_t>(),
+                       S.vjp_order.as<uint32_t>());
         }
         a.order = stored ? S.tr_order.as<uint32_t>() + std::size_t(j) * (N + 1) : S.vjp_order.as<uint32_t>();
+        a.ranges = stored ? S.tr_ranges.as<uint32_t>() + std::size_t(j) * kWalkRanges : S.vjp_ranges.as<uint32_t>();
         a.n = n;
         a.eps2 = float(S.p.eps * S.p.eps);
         a.small_leaf = c.wopt.small_leaf;
@@ -526,11 +518,10 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
         a.counters = c.counters.as<WalkCounters>();
         a.blocks = blocks;
         a.depth = stored ? S.tr_depth[j] : c.walk_depth();
+        a.id = c.id.as<uint32_t>();
+        a.lx = lx; a.ly = lx + N; a.lz = lx + 2 * N;
         tree_vjp(s, a, stored ? S.tr_lev.as<uint32_t>() + std::size_t(j) * kLev : c.levstart.as<uint32_t>(),
                  stored ? S.tr_cnt.as<uint32_t>() + j : c.cnt.as<uint32_t>() + n, nodes, R.D(S.cum));
-        scatter_add_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(c.id.as<uint32_t>(), gt, gt + N, gt + 2 * N, gs, gs + N,
-                                                            gs + 2 * N, lx, lx + N, lx + 2 * N, n);
-        LHIP_LAUNCH_CHECK();
     };
     auto weight = [&](int q) { return (q == T || q == 0) ? 0.5 * dt : dt; };
     LHIP_CHECK(hipEventRecord(S.ev[2], s));
