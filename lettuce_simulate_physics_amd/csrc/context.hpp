@@ -64,6 +64,16 @@ struct Context {
     // cells and their cubes; the pinned flag the level pass sets; restart scratch.
     int frames = 1;
     int* host_over = nullptr;
+    // Pinned mirror of the walk counters' host-read fields (the force call's one readback).
+    WalkCounters* host_counters = nullptr;
+    // Copies the counters the host reads (those before the walk order's range fields) into the
+    // pinned mirror, waits for the stream, and returns them in hc.
+    void read_counters(WalkCounters& hc) {
+        constexpr std::size_t kRead = offsetof(WalkCounters, range_start);
+        LHIP_CHECK(hipMemcpyAsync(host_counters, counters.as<void>(), kRead, hipMemcpyDeviceToHost, stream));
+        sync();
+        std::memcpy(static_cast<void*>(&hc), host_counters, kRead);
+    }
     DevBuf fkey, fseg, capb, fcube, rs_seg, rs_bbox, rs_cstart, rs_gkey, rs_ka, rs_kb, rs_va, rs_vb;
 
     // Mixed-precision form: FP64 positions and velocities (backend order) with permute alternates,
@@ -135,6 +145,7 @@ struct Context {
         name += lists_plain ? ", list walk" : ", fused walk";
         name += ", group " + std::to_string(wopt.group_max);
         LHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_over), 4 * sizeof(int), hipHostMallocDefault));
+        LHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_counters), sizeof(WalkCounters), hipHostMallocDefault));
         host_over[0] = host_over[1] = host_over[2] = host_over[3] = 0;
     }
     ~Context() {
@@ -147,6 +158,7 @@ struct Context {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         if (host_over) (void)hipHostFree(host_over);
+        if (host_counters) (void)hipHostFree(host_counters);
     }
 
     void reserve_bodies(int nn) {
@@ -871,10 +883,10 @@ struct Context {
                 if (mixed) a.posd_box = a.posd;
                 remote_args(a, trees, true);
                 tree_walk(stream, a, potential);
-                LHIP_CHECK(hipMemcpyAsync(&hc, counters.as<void>(), sizeof hc, hipMemcpyDeviceToHost, stream));
             }
             LHIP_CHECK(hipEventRecord(ev[4], stream));
-            sync();
+            if (trees > 0) read_counters(hc);
+            else sync();
             const uint64_t pairs = uint64_t(n) * uint64_t(n - 1);
             hc.interactions += pairs;
             hc.p2p += pairs;
@@ -887,8 +899,7 @@ struct Context {
             if (trees > 0) remote_args(a, trees, false);  // one pass: local, then remote trees
             tree_walk(stream, a, potential);
             LHIP_CHECK(hipEventRecord(ev[4], stream));
-            LHIP_CHECK(hipMemcpyAsync(&hc, counters.as<void>(), sizeof hc, hipMemcpyDeviceToHost, stream));
-            sync();
+            read_counters(hc);
             if (lists && !hc.error && hc.list_chunks > a.list_cap) {
                 // The pool was too small (the first call, or lists that grew): the evaluation left
                 // everything alone; run the walk again with a pool of the size the traversal asked for.
@@ -896,8 +907,7 @@ struct Context {
                 list_args(a, list_reserve());
                 tree_walk(stream, a, potential);
                 LHIP_CHECK(hipEventRecord(ev[4], stream));
-                LHIP_CHECK(hipMemcpyAsync(&hc, counters.as<void>(), sizeof hc, hipMemcpyDeviceToHost, stream));
-                sync();
+                read_counters(hc);
                 if (!hc.error && hc.list_chunks > a.list_cap)
                     throw Error(kRuntime, "hip tree walk: interaction lists outgrew their pool");
             }
