@@ -82,6 +82,21 @@ struct Context {
     bool mixed = false;
     lhip_walk_options wopt{kGroupMax, kSmallLeaf, 0.0};  // the reference's walk
     int env_group = 0;  // LETTUCE_HIP_GROUP, 0 when unset
+    int env_sort = 0;   // LETTUCE_HIP_SORT: 0 auto, else the body sort's top digits (3, 5, 7 or 8)
+    // The body sort: the deepest level of the last tree built from these bodies (-1: none since the
+    // last upload, or key restarts). With a deepest level D, every cell of a level >= D holds at most
+    // a leaf's bodies, so the radix passes need only the digits above a level >= D + 2; the runs of
+    // equal top digits left are short and a fix-up pass orders them.
+    int sort_depth = -1;
+    int sort_top_digits() const {
+        if (env_sort) return env_sort;
+        if (sort_depth < 0) return kRadixDigits;
+        // top digits t cover the 8 t - 1 top key bits: levels 1 .. (8 t - 1) / 3
+        if (sort_depth <= 5) return 3;
+        if (sort_depth <= 11) return 5;
+        if (sort_depth <= 16) return 7;
+        return kRadixDigits;
+    }
     DevBuf xd, yd, zd, vxd, vyd, vzd, xd2, yd2, zd2, vxd2, vyd2, vzd2, axd, ayd, azd, potd, momd, rec64, posd;
 
     // Frozen-topology replay of the differentiable tree simulation: group boxes from these recorded
@@ -114,6 +129,15 @@ struct Context {
             if (v != "fused" && v != "lists")
                 throw Error(kInvalid, "LETTUCE_HIP_WALK must be fused or lists, not '" + v + "'");
             lists_plain = lists_mixed = v == "lists";
+        }
+        if (const char* q = std::getenv("LETTUCE_HIP_SORT")) {
+            // The body sort's radix passes: auto (from the last tree's depth), full (all eight), or 3,
+            // 5 or 7 top digits and the run fix-up (a diagnostic and test knob).
+            const std::string v = q;
+            if (v == "auto") env_sort = 0;
+            else if (v == "full") env_sort = kRadixDigits;
+            else if (v == "3" || v == "5" || v == "7") env_sort = v[0] - '0';
+            else throw Error(kInvalid, "LETTUCE_HIP_SORT must be auto, full, 3, 5 or 7, not '" + v + "'");
         }
         if (const char* g = std::getenv("LETTUCE_HIP_GROUP")) {
             char* end = nullptr;
@@ -187,6 +211,7 @@ struct Context {
     // Switches between the plain (FP32 state) and mixed-precision (FP64 state) forms, converting the
     // resident bodies, accelerations and potential; the walk options apply to later trees.
     void set_precision(int mode, const lhip_walk_options* opt) {
+        sort_depth = -1;  // new bodies: the next sort takes every digit
         LHIP_CHECK(hipSetDevice(dev));
         if (mode != LHIP_MODE_PLAIN && mode != LHIP_MODE_MIXED) throw Error(kInvalid, "unknown precision mode");
         const bool mix = mode == LHIP_MODE_MIXED;
@@ -219,6 +244,7 @@ struct Context {
     // FP64 state in and out (mixed form only).
     void upload_f64(uint64_t nn, const double* hx, const double* hy, const double* hz, const double* hvx,
                     const double* hvy, const double* hvz, const float* hm, const uint32_t* hid) {
+        sort_depth = -1;  // new bodies: the next sort takes every digit
         if (!mixed) throw Error(kLogic, "hip upload_f64: FP64 state needs the mixed-precision mode");
         if (nn >= uint64_t(INT32_MAX)) throw Error(kInvalid, "hip backend: n = " + std::to_string(nn) +
                                                                 " exceeds the 2^31-1 body limit");
@@ -372,6 +398,7 @@ struct Context {
 
     void upload(uint64_t nn, const float* hx, const float* hy, const float* hz, const float* hvx, const float* hvy,
                 const float* hvz, const float* hm, const uint32_t* hid) {
+        sort_depth = -1;  // new bodies: the next sort takes every digit
         if (nn >= uint64_t(INT32_MAX)) throw Error(kInvalid, "hip backend: n = " + std::to_string(nn) +
                                                                 " exceeds the 2^31-1 body limit");
         LHIP_CHECK(hipSetDevice(dev));
@@ -453,7 +480,7 @@ struct Context {
         acc_ok = false;
         LHIP_CHECK(hipEventRecord(ev[0], stream));
         TreeBufs tb = tree_bufs();
-        tree_sort(stream, tb, n, keep_cost);
+        tree_sort(stream, tb, n, keep_cost, sort_top_digits());
         swap_permuted(keep_cost);
         LHIP_CHECK(hipEventRecord(ev[1], stream));
         frames = 1;
@@ -467,6 +494,7 @@ struct Context {
             restart_keys(nullptr, nullptr, nullptr, true);
             rebuild(inv_theta, nullptr);
         }
+        sort_depth = frames == 1 ? built_depth : -1;
         LHIP_CHECK(hipEventRecord(ev[2], stream));
         tree_ok = true;
     }
@@ -1140,6 +1168,7 @@ struct Context {
     }
 
     void snapshot_restore(const char* path, lhip_snapshot_header* out) {
+        sort_depth = -1;  // new bodies: the next sort takes every digit
         lhip_snapshot_header h{};
         int rc = lhip_snapshot_read_header(path, &h);
         if (rc != LHIP_OK) throw Error(rc, lhip_snapshot_error());

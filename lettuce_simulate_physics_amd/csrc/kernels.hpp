@@ -105,8 +105,10 @@ struct TreeBufs {
 };
 
 // Morton keys, stable radix sort, permute of every body array into key order (swaps the caller's
-// pointers: after the call b.x etc. hold the sorted arrays). Keys stay in b.key.
-void tree_sort(hipStream_t s, TreeBufs& b, int n, bool permute_cost);
+// pointers: after the call b.x etc. hold the sorted arrays). Keys stay in b.key. top_digits < 8
+// (odd): radix passes over the top digits only, then each run of equal top digits ordered by its low
+// bits (the same stable order; fast when the runs are short).
+void tree_sort(hipStream_t s, TreeBufs& b, int n, bool permute_cost, int top_digits = kRadixDigits);
 // Every body array into the order perm gives (new i <- old perm[i]); swaps like tree_sort.
 void tree_permute(hipStream_t s, TreeBufs& b, const uint32_t* perm, int n, bool permute_cost);
 // Keys of FP64 positions already in key order (a recorded sorted state): bounding cube and keys,

@@ -241,14 +241,14 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter(const uint64_t* __
     }
 }
 
-// Morton keys of the body sort, tile by tile, with digit 0 of each tile counted for its first pass
-// (LDS atomics, as radix_count).
+// Morton keys of the body sort, tile by tile, with the digit at `shift` of each tile counted for its
+// first pass (LDS atomics, as radix_count).
 template <class R>
 __global__ void __launch_bounds__(kSortThreads) keys_count_kernel(const R* __restrict__ x, const R* __restrict__ y,
                                                                   const R* __restrict__ z,
                                                                   const Cube* __restrict__ cube,
                                                                   uint64_t* __restrict__ key, int n, int ntiles,
-                                                                  uint32_t* __restrict__ hist) {
+                                                                  uint32_t* __restrict__ hist, int shift) {
     __shared__ uint32_t h[kRadix];
     const int tid = int(threadIdx.x);
     const int base = blockIdx.x * kSortTile;
@@ -267,11 +267,146 @@ __global__ void __launch_bounds__(kSortThreads) keys_count_kernel(const R* __res
         if (i < n) {
             const uint64_t kk = morton_key(px[r], py[r], pz[r], ox, oy, oz, scale);
             key[i] = kk;
-            atomicAdd(&h[kk & 0xff], 1u);
+            atomicAdd(&h[(kk >> shift) & 0xff], 1u);
         }
     }
     __syncthreads();
     hist[std::size_t(tid) * ntiles + blockIdx.x] = h[tid];
+}
+
+// ---------------------------------------------------------------- runs of equal top digits
+//
+// A sort by the top digits only (the passes d0 .. 7) leaves the keys ordered by key >> 8 d0, each
+// run of equal top digits in the input order (the passes are stable). Ordering every run by the low
+// bits, ties by position, then gives the full stable sort. run_fixup does it out of place, one
+// element per thread: a singleton is copied; an element of a run of at most kShortRun finds the
+// run's bounds and its rank in it by scanning it; a longer run's first element records the run's
+// start for run_sort_long and the run's elements are left to it.
+
+constexpr int kShortRun = 64;
+constexpr int kLongRunBlocks = 64;  // blocks that take the long runs in turn
+
+__global__ void __launch_bounds__(kBS) run_fixup(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                 uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, int n,
+                                                 int shift, uint32_t* __restrict__ long_runs, int long_cap) {
+    const int e = blockIdx.x * kBS + threadIdx.x;
+    if (e >= n) return;
+    const uint64_t k = kin[e];
+    const uint64_t hi = k >> shift;
+    const bool prev_same = e > 0 && (kin[e - 1] >> shift) == hi;
+    const bool next_same = e + 1 < n && (kin[e + 1] >> shift) == hi;
+    if (!prev_same && !next_same) {
+        kout[e] = k;
+        vout[e] = vin[e];
+        return;
+    }
+    // The run's start, at most kShortRun back; further back it is a long run.
+    int s = e;
+    while (s > 0 && e - s < kShortRun && (kin[s - 1] >> shift) == hi) --s;
+    if (s > 0 && (kin[s - 1] >> shift) == hi) return;  // a long run (its start is further back)
+    int t = e + 1;
+    while (t < n && t - s <= kShortRun && (kin[t] >> shift) == hi) ++t;
+    if (t - s > kShortRun) {
+        // A long run: its first element records it (long_runs[0] counts, starts from [1]).
+        if (e == s) {
+            const uint32_t slot = atomicAdd(&long_runs[0], 1u);
+            if (int(slot) < long_cap) long_runs[1 + slot] = uint32_t(s);
+        }
+        return;
+    }
+    const uint64_t lmask = (1ull << shift) - 1ull;
+    const uint64_t lo = k & lmask;
+    int rank = 0;
+    for (int f = s; f < t; ++f) {
+        const uint64_t lf = kin[f] & lmask;
+        rank += (lf < lo || (lf == lo && f < e)) ? 1 : 0;
+    }
+    kout[s + rank] = k;
+    vout[s + rank] = vin[e];
+}
+
+// Long runs (more than kShortRun equal top digits; none unless bodies crowd a cell far beyond the
+// previous tree's leaves): each block takes recorded runs in turn and sorts its run stably by the
+// low digits 0 .. d0 - 1 with a block-wide LSD counting sort, passes alternating between the
+// input and the output arrays over the run's positions (d0 odd: the last pass writes the output).
+__global__ void __launch_bounds__(kSortThreads) run_sort_long(uint64_t* __restrict__ kin, uint32_t* __restrict__ vin,
+                                                              uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                              int n, int d0, const uint32_t* __restrict__ long_runs,
+                                                              int long_cap) {
+    __shared__ uint32_t hist[kRadix];
+    __shared__ uint32_t base[kRadix];
+    __shared__ uint32_t wc[kSortWaves][kRadix];
+    __shared__ uint32_t wsum[2][kSortWaves];
+    __shared__ int run_end;
+    const int tid = int(threadIdx.x), lane = tid & 63, wave = tid >> 6;
+    const int runs = min(int(long_runs[0]), long_cap);
+    const int shift = 8 * d0;
+    for (int r = blockIdx.x; r < runs; r += gridDim.x) {
+        const int s = int(long_runs[1 + r]);
+        const uint64_t hi = kin[s] >> shift;
+        // The run's end: the first position past s with other top digits.
+        if (tid == 0) run_end = n;
+        __syncthreads();
+        for (int c = s; c < n; c += kSortThreads) {
+            const int i = c + tid;
+            const bool other = i < n && (kin[i] >> shift) != hi;
+            const unsigned long long m = __ballot(other);
+            if (m && lane == int(__ffsll((long long)m) - 1)) atomicMin(&run_end, i);
+            __syncthreads();
+            const bool done = run_end < n;
+            __syncthreads();
+            if (done) break;
+        }
+        const int t = run_end;
+        for (int p = 0; p < d0; ++p) {
+            uint64_t* ks = (p & 1) ? kout : kin;
+            uint32_t* vs = (p & 1) ? vout : vin;
+            uint64_t* kd = (p & 1) ? kin : kout;
+            uint32_t* vd = (p & 1) ? vin : vout;
+            const int sh = 8 * p;
+            hist[tid] = 0;
+            __syncthreads();
+            for (int i = s + tid; i < t; i += kSortThreads) atomicAdd(&hist[(ks[i] >> sh) & 0xff], 1u);
+            __syncthreads();
+            const uint32_t b = block_scan_excl<kSortWaves>(hist[tid], wsum[0], lane, wave, nullptr);
+            base[tid] = b;  // running start of each digit (advanced chunk by chunk)
+            __syncthreads();
+            for (int c = s; c < t; c += kSortThreads) {
+                const int i = c + tid;
+                const bool ok = i < t;
+                const uint64_t kk = ok ? ks[i] : 0ull;
+                const uint32_t vv = ok ? vs[i] : 0u;
+                const int d = ok ? int((kk >> sh) & 0xff) : -1;
+#pragma unroll
+                for (int w = 0; w < kSortWaves; ++w) wc[w][tid] = 0;
+                __syncthreads();
+                // Rank among the wave's lanes of the same digit; the wave's count of each digit.
+                uint64_t peers = ok ? ~0ull : 0ull;
+#pragma unroll
+                for (int bit = 0; bit < 8; ++bit) {
+                    const uint64_t m = __ballot(ok && ((d >> bit) & 1));
+                    peers &= ((d >> bit) & 1) ? m : ~m;
+                }
+                peers &= __ballot(ok);
+                const uint32_t rank = ok ? uint32_t(__popcll(peers & ((1ull << lane) - 1ull))) : 0u;
+                if (ok && rank == 0) wc[wave][d] = uint32_t(__popcll(peers));
+                __syncthreads();
+                if (ok) {
+                    uint32_t off = base[d] + rank;
+                    for (int w = 0; w < wave; ++w) off += wc[w][d];
+                    kd[s + int(off)] = kk;
+                    vd[s + int(off)] = vv;
+                }
+                __syncthreads();
+                uint32_t add = 0;
+#pragma unroll
+                for (int w = 0; w < kSortWaves; ++w) add += wc[w][tid];
+                base[tid] += add;
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------- permute
@@ -302,14 +437,27 @@ __global__ void __launch_bounds__(kBS) permute_kernel(const uint32_t* __restrict
 }
 
 template <class R>
-void sort_impl(hipStream_t s, TreeBufs& b, int n, bool permute_cost) {
+void sort_impl(hipStream_t s, TreeBufs& b, int n, bool permute_cost, int top) {
     const R *x = static_cast<const R*>(b.x), *y = static_cast<const R*>(b.y), *z = static_cast<const R*>(b.z);
     launch_bbox(s, x, y, z, n, b.bbox_part, b.cube);
     const int ntiles = sort_tiles(n);
-    keys_count_kernel<R><<<ntiles, kSortThreads, 0, s>>>(x, y, z, b.cube, b.key, n, ntiles, b.radix->words);
+    const int d0 = kRadixDigits - top;
+    keys_count_kernel<R><<<ntiles, kSortThreads, 0, s>>>(x, y, z, b.cube, b.key, n, ntiles, b.radix->words, 8 * d0);
     LHIP_LAUNCH_CHECK();
-    // Eight passes: the sorted keys and permutation end in key/idx.
-    radix_sort(s, *b.radix, RadixBufs{{b.key, b.key2}, {b.idx, b.idx2}}, true, n, nullptr, 0, kRadixDigits, true);
+    // Passes d0 .. 7. All eight: the sorted keys and permutation end in key/idx. The top (odd) ones:
+    // they end in key2/idx2, and the run fix-up writes the full order back to key/idx.
+    radix_sort(s, *b.radix, RadixBufs{{b.key, b.key2}, {b.idx, b.idx2}}, true, n, nullptr, d0, kRadixDigits, true);
+    if (d0 > 0) {
+        // The radix work memory is free until the next sort: it holds the long runs' starts.
+        uint32_t* long_runs = b.radix->words;
+        const int long_cap = int(std::min<std::size_t>(radix_work_words(n) - 1, std::size_t(n / kShortRun + 1)));
+        launch_fill_u32(s, long_runs, 0u, 1);
+        run_fixup<<<grid_for(n, kBS), kBS, 0, s>>>(b.key2, b.idx2, b.key, b.idx, n, 8 * d0, long_runs, long_cap);
+        LHIP_LAUNCH_CHECK();
+        run_sort_long<<<kLongRunBlocks, kSortThreads, 0, s>>>(b.key2, b.idx2, b.key, b.idx, n, d0, long_runs,
+                                                             long_cap);
+        LHIP_LAUNCH_CHECK();
+    }
     auto P = [](void* p) { return static_cast<R*>(p); };
     permute_kernel<R><<<grid_for(n, kBS), kBS, 0, s>>>(b.idx, n, x, y, z, P(b.vx), P(b.vy), P(b.vz), b.m, b.id,
                                                       permute_cost ? b.cost : nullptr, P(b.x2), P(b.y2), P(b.z2),
@@ -376,10 +524,12 @@ void tree_permute(hipStream_t s, TreeBufs& b, const uint32_t* perm, int n, bool 
     if (permute_cost) std::swap(b.cost, b.cost2);
 }
 
-void tree_sort(hipStream_t s, TreeBufs& b, int n, bool permute_cost) {
+void tree_sort(hipStream_t s, TreeBufs& b, int n, bool permute_cost, int top_digits) {
     if (n <= 0) return;
-    if (b.f64) sort_impl<double>(s, b, n, permute_cost);
-    else sort_impl<float>(s, b, n, permute_cost);
+    if (top_digits != kRadixDigits && (top_digits < 1 || top_digits > kRadixDigits || top_digits % 2 == 0))
+        throw Error(kInvalid, "tree_sort: top digits must be 8 or odd");
+    if (b.f64) sort_impl<double>(s, b, n, permute_cost, top_digits);
+    else sort_impl<float>(s, b, n, permute_cost, top_digits);
     std::swap(b.x, b.x2); std::swap(b.y, b.y2); std::swap(b.z, b.z2);
     std::swap(b.vx, b.vx2); std::swap(b.vy, b.vy2); std::swap(b.vz, b.vz2);
     std::swap(b.m, b.m2); std::swap(b.id, b.id2);

@@ -458,6 +458,61 @@ def test_tree_larger_than_the_first_node_reserve():
         be.close()
 
 
+def _resort_against_oracle(hip, dt, eps=0.01, theta=0.6):
+    """Drift, then a force call on hip's own sorted bodies (no upload: the sort may use the last
+    tree's depth) against the oracle's full stable sort of the same bodies in the same order."""
+    hip.drift(dt)
+    b = hip.download()
+    o = O.Oracle()
+    o.upload(b)
+    o.compute_forces(1.0, eps, theta, 1, True)
+    hip.compute_forces(P("tree", eps, theta), True)
+    keys_o, _ = o.keys()
+    assert np.array_equal(hip.download().id, o.download().id)
+    assert np.array_equal(hip.tree_keys(), keys_o)
+    assert np.array_equal(canon_from_hip(hip.tree_nodes()), canon_from_oracle(o.nodes()))
+
+
+@pytest.mark.parametrize("mode", ["3", "5", "7", "full", "auto"])
+def test_sort_top_digits_and_run_fixup(monkeypatch, mode):
+    # The body sort may take only the top radix digits (LETTUCE_HIP_SORT; by default as many as the
+    # last tree's depth needs) and then order each run of equal top digits by its low bits: runs of
+    # up to 64 one body per thread, longer ones by a block-wide counting sort. Permutation, keys and
+    # octree equal the oracle's full stable sort: a Plummer sphere (at 3 top digits, 23 key bits,
+    # its core cells hold runs of hundreds), 20 clusters of 300 bodies within 1e-4 (runs of 300),
+    # and 300 coincident bodies (one run, equal low bits: the input order), each also after a drift
+    # with the sort informed by the previous tree.
+    monkeypatch.setenv("LETTUCE_HIP_SORT", mode)
+    rng = np.random.default_rng(5)
+    sets = [O.plummer(50000, 3)]
+    c = rng.uniform(-1, 1, (20, 3))
+    pos = (c[:, None, :] + 1e-4 * rng.standard_normal((20, 300, 3))).reshape(-1, 3).astype(np.float32)
+    b = O.Bodies.empty(len(pos))
+    b.x[:], b.y[:], b.z[:] = pos[:, 0], pos[:, 1], pos[:, 2]
+    b.vx[:] = rng.standard_normal(len(pos)).astype(np.float32)
+    b.m[:] = np.float32(1.0 / len(pos))
+    b.id[:] = np.arange(len(pos), dtype=np.uint32)
+    sets.append(b)
+    b = O.Bodies.empty(4000)
+    b.x[:] = rng.uniform(-1, 1, 4000).astype(np.float32)
+    b.y[:] = rng.uniform(-1, 1, 4000).astype(np.float32)
+    b.z[:] = rng.uniform(-1, 1, 4000).astype(np.float32)
+    b.x[1000:1300], b.y[1000:1300], b.z[1000:1300] = np.float32(0.25), np.float32(-0.5), np.float32(0.125)
+    b.m[:] = np.float32(1.0 / 4000)
+    b.id[:] = np.arange(4000, dtype=np.uint32)
+    sets.append(b)
+    hip = L.HipBackend(0)
+    try:
+        for bs in sets:
+            check_tree_against_oracle(hip, bs)
+            _resort_against_oracle(hip, 1e-3)
+    finally:
+        hip.close()
+    monkeypatch.setenv("LETTUCE_HIP_SORT", "4")
+    be, why = L.make_hip_backend(0)
+    assert be is None and "LETTUCE_HIP_SORT" in why, why
+
+
 def test_one_hundred_million_body_force_call(hip):
     # 1e8 Henon-unit Plummer bodies (the reference's largest benchmark row below 5e8,
     # docs/running.md:80): one tree force call fits the memory estimate, and the state stays whole.
