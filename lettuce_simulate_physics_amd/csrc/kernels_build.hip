@@ -282,9 +282,15 @@ __global__ void __launch_bounds__(kScanThreads) scan_top(ScanArrays A, int nb, i
         return;
     }
     uint32_t d = 0, over = 0;
-    for (int t = threadIdx.x; t < R.tiles; t += kScanThreads) {
-        d = max(d, R.bdeep[t]);
-        over += R.bdeep[R.tiles + t];
+    for (int b = 0; b < R.tiles; b += kScanThreads * kTopPT) {  // kTopPT independent loads per thread
+        const int t0 = b + int(threadIdx.x) * kTopPT;
+#pragma unroll
+        for (int k = 0; k < kTopPT; ++k) {
+            if (t0 + k < R.tiles) {
+                d = max(d, R.bdeep[t0 + k]);
+                over += R.bdeep[R.tiles + t0 + k];
+            }
+        }
     }
     uint32_t total;
     block_exclusive_scan(over, sh, turn, total);

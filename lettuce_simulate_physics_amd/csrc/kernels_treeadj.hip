@@ -508,46 +508,74 @@ __global__ void __launch_bounds__(kBS) down_root(const double* __restrict__ gcel
     }
 }
 
-// Children of the nodes of one level: cum(child) = cum(parent) + own(child).
+// Children of the nodes of one level: cum(child) = cum(parent) + own(child). Eight lanes per parent
+// (thread t: parent p0 + t / 8, child slots t mod 8, t mod 8 + 8, ...), so the children of a parent
+// are computed side by side and their records read and written as contiguous runs.
 __global__ void __launch_bounds__(kBS) down_level(int level, const uint32_t* __restrict__ levstart,
                                                   const WalkNode* __restrict__ node,
                                                   const double* __restrict__ gcell,
                                                   const double* __restrict__ gmono,
                                                   const NodeMomD* __restrict__ momd, double* __restrict__ cum) {
     const int p0 = int(levstart[level]), p1 = int(levstart[level + 1]);
-    for (int p = p0 + blockIdx.x * kBS + threadIdx.x; p < p1; p += gridDim.x * kBS) {
+    const long long slots = 8ll * (p1 - p0);
+    for (long long t = (long long)blockIdx.x * kBS + threadIdx.x; t < slots; t += (long long)gridDim.x * kBS) {
+        const int p = p0 + int(t >> 3);
         const WalkNode nd = node[p];
         const int nch = info_nchild(nd.info);
-        if (info_leaf(nd.info) || nch == 0) continue;
-        double cp[9];
-        for (int c = 0; c < 9; ++c) cp[c] = cum[std::size_t(p) * 9 + c];
-        for (int ch = nd.first; ch < nd.first + nch; ++ch) {
+        if (info_leaf(nd.info)) continue;
+        for (int ch = nd.first + int(t & 7); ch < nd.first + nch; ch += 8) {
             double o[9];
             own_terms(gcell + std::size_t(ch) * 9, gmono + std::size_t(ch) * 3, momd[ch], o);
-            for (int c = 0; c < 9; ++c) cum[std::size_t(ch) * 9 + c] = cp[c] + o[c];
+            for (int c = 0; c < 9; ++c) cum[std::size_t(ch) * 9 + c] = cum[std::size_t(p) * 9 + c] + o[c];
         }
     }
 }
 
-// Bodies of each leaf: g_j += m_j (A + B x_j) with the leaf's accumulated A, B.
 // Each leaf's bodies j: the cell-moment term m_j (A + B x_j) of the leaf's cum, added to the
 // body's target and source sides, and the whole scattered into lambda_x by id (the last step of
-// the VJP; every body is in exactly one leaf).
+// the VJP; every body is in exactly one leaf). A wave takes 64 consecutive nodes and lists its
+// leaves in LDS (node, first body, exclusive prefix of the body counts); its lanes then take the
+// listed leaves' bodies one each, lane f finding its leaf by a binary search over the prefixes, so
+// every lane holds a body whatever the leaf sizes.
 __global__ void __launch_bounds__(kBS) down_leaves(const uint32_t* __restrict__ node_count,
                                                    const WalkNode* __restrict__ node,
                                                    const double* __restrict__ cum, const double4* __restrict__ posd,
                                                    TreeVjpArgs a) {
-    const int p = blockIdx.x * kBS + threadIdx.x;
-    if (p >= int(*node_count)) return;
-    const WalkNode nd = node[p];
-    if (!info_leaf(nd.info)) return;
-    const double* c = cum + std::size_t(p) * 9;
-    for (int j = nd.lo; j < nd.hi; ++j) {
+    __shared__ int lq[kBS], llo[kBS], lex[kBS];
+    const int lane = int(threadIdx.x & 63), base = int(threadIdx.x >> 6) * 64;
+    const int p = int(blockIdx.x) * kBS + int(threadIdx.x);
+    WalkNode nd{};
+    if (p < int(*node_count)) nd = node[p];
+    const bool leaf = p < int(*node_count) && info_leaf(nd.info);
+    const unsigned long long m = __ballot(leaf);
+    const int cnt = leaf ? nd.hi - nd.lo : 0;
+    const int incl = prefix_sum_incl(cnt, lane);  // over lanes, so in leaf rank order
+    if (leaf) {
+        const int k = base + int(lanes_below(m));
+        lq[k] = p;
+        llo[k] = nd.lo;
+        lex[k] = incl - cnt;
+    }
+    const int nleaf = __popcll(m), total = __builtin_amdgcn_readlane(incl, 63);
+    __syncthreads();
+    const uint32_t* __restrict__ id = a.id;
+    const double *__restrict__ gtx = a.gtx, *__restrict__ gty = a.gty, *__restrict__ gtz = a.gtz;
+    const double *__restrict__ gsx = a.gsx, *__restrict__ gsy = a.gsy, *__restrict__ gsz = a.gsz;
+    double *__restrict__ lx = a.lx, *__restrict__ ly = a.ly, *__restrict__ lz = a.lz;
+    for (int f = lane; f < total; f += 64) {
+        int lo = 0, hi = nleaf;  // the last listed leaf whose prefix is <= f (it holds body f)
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (lex[base + mid] <= f) lo = mid;
+            else hi = mid;
+        }
+        const int j = llo[base + lo] + (f - lex[base + lo]);
+        const double* c = cum + std::size_t(lq[base + lo]) * 9;
         const double4 b = posd[j];
-        const uint32_t i = a.id[j];
-        a.lx[i] += a.gtx[j] + (a.gsx[j] + b.w * (c[0] + c[3] * b.x + c[4] * b.y + c[5] * b.z));
-        a.ly[i] += a.gty[j] + (a.gsy[j] + b.w * (c[1] + c[4] * b.x + c[6] * b.y + c[7] * b.z));
-        a.lz[i] += a.gtz[j] + (a.gsz[j] + b.w * (c[2] + c[5] * b.x + c[7] * b.y + c[8] * b.z));
+        const uint32_t i = id[j];
+        lx[i] += gtx[j] + (gsx[j] + b.w * (c[0] + c[3] * b.x + c[4] * b.y + c[5] * b.z));
+        ly[i] += gty[j] + (gsy[j] + b.w * (c[1] + c[4] * b.x + c[6] * b.y + c[7] * b.z));
+        lz[i] += gtz[j] + (gsz[j] + b.w * (c[2] + c[5] * b.x + c[7] * b.y + c[8] * b.z));
     }
 }
 
@@ -566,7 +594,7 @@ void tree_vjp(hipStream_t s, const TreeVjpArgs& a, const uint32_t* levstart, con
     LHIP_LAUNCH_CHECK();
     down_root<<<1, 64, 0, s>>>(a.gcell, a.gmono, a.momd, cum);
     LHIP_LAUNCH_CHECK();
-    const unsigned g = std::min<unsigned>(grid_for(a.n, kBS), 512u);
+    const unsigned g = std::min<unsigned>(grid_for(8ll * a.n, kBS), 2048u);
     for (int l = 0; l < a.depth; ++l) {  // the deepest level holds leaves only
         down_level<<<g, kBS, 0, s>>>(l, levstart, a.node, a.gcell, a.gmono, a.momd, cum);
         LHIP_LAUNCH_CHECK();

@@ -39,7 +39,7 @@ inline uint32_t* range_starts_of(WalkCounters* c) {
 struct alignas(32) WalkNode {
     float cx, cy, cz, rcrit2;  // centre of mass; open the node if dist^2(group box, com) < rcrit2
     int32_t lo, hi;            // body range [lo, hi) in sorted order
-    int32_t info;              // level | leaf << 5 | children << 8
+    int32_t info;              // level | leaf << 7 | children << 8
     int32_t first;             // first child (level order), -1 for a leaf
 };
 static_assert(sizeof(WalkNode) == 32, "WalkNode is one 32-byte record");
