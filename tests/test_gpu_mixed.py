@@ -250,7 +250,7 @@ def test_mixed_defaults_at_full_size_against_fp64_direct(budget, workload):
     # small_leaf 1, quad_tau 0.05}: the 2^20 Plummer sphere (theta 0.6, eps 0.01) and the 4,194,304-star
     # sech^2 disk around a core of 10 (theta 0.5, eps 0.02, 10 KDK steps of 5e-4). Force error per body
     # against the device FP64 direct sum (pinned to the oracle's CPU FP64 sum to 1e-13) on 8192 seeded
-    # targets, at the initial state and after the steps: median <= 1e-3 (the north star's budget);
+    # targets, at the initial state and after the steps: median <= 1e-3 (the mixed form's error budget);
     # p99 measured 1.4e-3 (sphere) and 2.1e-3 (disk), bounded here by 4e-3.
     if workload.startswith("plummer"):
         b, p, steps, dt = L.plummer(1 << 20, 1, 1.0, 10.0), L.ForceParams(G=1.0, eps=0.01, theta=0.6, method="tree"), 0, 0
