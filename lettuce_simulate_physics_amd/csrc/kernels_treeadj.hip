@@ -533,7 +533,7 @@ __global__ void __launch_bounds__(kBS) down_level(int level, const uint32_t* __r
 
 // Each leaf's bodies j: the cell-moment term m_j (A + B x_j) of the leaf's cum, added to the
 // body's target and source sides, and the whole scattered into lambda_x by id (the last step of
-// the VJP; every body is in exactly one leaf). A wave takes 64 consecutive nodes and lists its
+// the VJP; every body is in exactly one leaf; lambda holds (x, y, z) per body). A wave takes 64 consecutive nodes and lists its
 // leaves in LDS (node, first body, exclusive prefix of the body counts); its lanes then take the
 // listed leaves' bodies one each, lane f finding its leaf by a binary search over the prefixes, so
 // every lane holds a body whatever the leaf sizes.
@@ -561,7 +561,7 @@ __global__ void __launch_bounds__(kBS) down_leaves(const uint32_t* __restrict__ 
     const uint32_t* __restrict__ id = a.id;
     const double *__restrict__ gtx = a.gtx, *__restrict__ gty = a.gty, *__restrict__ gtz = a.gtz;
     const double *__restrict__ gsx = a.gsx, *__restrict__ gsy = a.gsy, *__restrict__ gsz = a.gsz;
-    double *__restrict__ lx = a.lx, *__restrict__ ly = a.ly, *__restrict__ lz = a.lz;
+    double* __restrict__ lam = a.lam;
     for (int f = lane; f < total; f += 64) {
         int lo = 0, hi = nleaf;  // the last listed leaf whose prefix is <= f (it holds body f)
         while (hi - lo > 1) {
@@ -572,10 +572,10 @@ __global__ void __launch_bounds__(kBS) down_leaves(const uint32_t* __restrict__ 
         const int j = llo[base + lo] + (f - lex[base + lo]);
         const double* c = cum + std::size_t(lq[base + lo]) * 9;
         const double4 b = posd[j];
-        const uint32_t i = id[j];
-        lx[i] += gtx[j] + (gsx[j] + b.w * (c[0] + c[3] * b.x + c[4] * b.y + c[5] * b.z));
-        ly[i] += gty[j] + (gsy[j] + b.w * (c[1] + c[4] * b.x + c[6] * b.y + c[7] * b.z));
-        lz[i] += gtz[j] + (gsz[j] + b.w * (c[2] + c[5] * b.x + c[7] * b.y + c[8] * b.z));
+        double* l = lam + 3 * std::size_t(id[j]);
+        l[0] += gtx[j] + (gsx[j] + b.w * (c[0] + c[3] * b.x + c[4] * b.y + c[5] * b.z));
+        l[1] += gty[j] + (gsy[j] + b.w * (c[1] + c[4] * b.x + c[6] * b.y + c[7] * b.z));
+        l[2] += gtz[j] + (gsz[j] + b.w * (c[2] + c[5] * b.x + c[7] * b.y + c[8] * b.z));
     }
 }
 

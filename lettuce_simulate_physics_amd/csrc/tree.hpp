@@ -303,9 +303,10 @@ struct TreeVjpArgs {
     WalkCounters* counters;
     int blocks;
     int depth;      // deepest level of the tree walked (at most 21 x key frames)
-    // The result: lx[id[k]] += gt_k + gs_k for every body k (sorted order) of the tree.
+    // The result: lam[id[k]] += gt_k + gs_k for every body k (sorted order) of the tree; lam holds
+    // (x, y, z) per body in id order.
     const uint32_t* id;
-    double *lx, *ly, *lz;
+    double* lam;
 };
 // Nodes at the top of the tree (levels 0-3 and part of 4 in level order) whose VJP cotangents each
 // block accumulates privately.

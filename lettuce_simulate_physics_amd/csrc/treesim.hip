@@ -27,16 +27,16 @@
 namespace lhip {
 namespace {
 
-__global__ void __launch_bounds__(kBS) gather_u_kernel(const uint32_t* __restrict__ id, const double* __restrict__ lx,
-                                                       const double* __restrict__ ly, const double* __restrict__ lz,
+// u (SoA, tree order) = c lambda (x, y, z per body, id order) of the tree's bodies.
+__global__ void __launch_bounds__(kBS) gather_u_kernel(const uint32_t* __restrict__ id, const double* __restrict__ lam,
                                                        double c, double* __restrict__ ux, double* __restrict__ uy,
                                                        double* __restrict__ uz, int n) {
     const int k = blockIdx.x * kBS + threadIdx.x;
     if (k >= n) return;
-    const uint32_t j = id[k];
-    ux[k] = c * lx[j];
-    uy[k] = c * ly[j];
-    uz[k] = c * lz[j];
+    const double* l = lam + 3 * std::size_t(id[k]);
+    ux[k] = c * l[0];
+    uy[k] = c * l[1];
+    uz[k] = c * l[2];
 }
 
 __global__ void __launch_bounds__(kBS) inverse_kernel(const uint32_t* __restrict__ id, uint32_t* __restrict__ inv, int n) {
@@ -390,12 +390,10 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
     double* lx = R.D(S.lx);
     double* lv = R.D(S.lvx);
     {
-        std::vector<double> h;
-        host_soa(gxT, N, h);
-        LHIP_CHECK(hipMemcpyAsync(lx, h.data(), 3 * N * 8, hipMemcpyHostToDevice, s));
-        c.sync();
-        host_soa(gvT, N, h);
-        LHIP_CHECK(hipMemcpyAsync(lv, h.data(), 3 * N * 8, hipMemcpyHostToDevice, s));
+        // lambda_x and lambda_v stay as the caller's (x, y, z) per body in id order, so that a
+        // gather or scatter by id touches one record
+        LHIP_CHECK(hipMemcpyAsync(lx, gxT, 3 * N * 8, hipMemcpyHostToDevice, s));
+        LHIP_CHECK(hipMemcpyAsync(lv, gvT, 3 * N * 8, hipMemcpyHostToDevice, s));
         c.sync();
     }
     const int blocks = c.cus * tree_vjp_blocks_per_cu();
@@ -479,8 +477,7 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
         S.gmono.reserve(std::size_t(nodes) * 3 * 8);
         S.cum.reserve(std::size_t(nodes) * 9 * 8);
         double* u = R.D(S.u);
-        gather_u_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(c.id.as<uint32_t>(), lv, lv + N, lv + 2 * N, S.p.G * w, u,
-                                                         u + N, u + 2 * N, n);
+        gather_u_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(c.id.as<uint32_t>(), lv, S.p.G * w, u, u + N, u + 2 * N, n);
         LHIP_LAUNCH_CHECK();
         double* gs = R.D(S.gs);
         double* gt = R.D(S.gt);
@@ -519,7 +516,7 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
         a.blocks = blocks;
         a.depth = stored ? S.tr_depth[j] : c.walk_depth();
         a.id = c.id.as<uint32_t>();
-        a.lx = lx; a.ly = lx + N; a.lz = lx + 2 * N;
+        a.lam = lx;
         tree_vjp(s, a, stored ? S.tr_lev.as<uint32_t>() + std::size_t(j) * kLev : c.levstart.as<uint32_t>(),
                  stored ? S.tr_cnt.as<uint32_t>() + j : c.cnt.as<uint32_t>() + n, nodes, R.D(S.cum));
     };
@@ -558,15 +555,8 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
         }
     }
     LHIP_CHECK(hipEventRecord(S.ev[3], s));
-    std::vector<double> h(3 * N);
-    auto out = [&](const double* d, double* dst) {
-        LHIP_CHECK(hipMemcpyAsync(h.data(), d, 3 * N * 8, hipMemcpyDeviceToHost, s));
-        c.sync();
-        for (std::size_t i = 0; i < N; ++i)
-            for (int k = 0; k < 3; ++k) dst[3 * i + k] = h[k * N + i];
-    };
-    if (gx0) out(lx, gx0);
-    if (gv0) out(lv, gv0);
+    if (gx0) LHIP_CHECK(hipMemcpyAsync(gx0, lx, 3 * N * 8, hipMemcpyDeviceToHost, s));
+    if (gv0) LHIP_CHECK(hipMemcpyAsync(gv0, lv, 3 * N * 8, hipMemcpyDeviceToHost, s));
     c.sync();
     float ms = 0;
     LHIP_CHECK(hipEventElapsedTime(&ms, S.ev[2], S.ev[3]));
