@@ -44,15 +44,73 @@ __global__ void __launch_bounds__(kBS) inverse_kernel(const uint32_t* __restrict
     if (k < n) inv[id[k]] = uint32_t(k);
 }
 
+// Up to kCopySegs device-to-device copies (whole 4-byte words) in one launch: blockIdx.y picks
+// the copy; 16-byte words where both ends allow them.
+constexpr int kCopySegs = 12;
+struct CopySet {
+    const void* src[kCopySegs];
+    void* dst[kCopySegs];
+    long long words[kCopySegs];
+};
+
+__global__ void __launch_bounds__(kBS) copy_set_kernel(CopySet cs) {
+    const int g = int(blockIdx.y);
+    const long long words = cs.words[g];
+    const long long t0 = (long long)blockIdx.x * kBS + threadIdx.x, step = (long long)gridDim.x * kBS;
+    const uint32_t* __restrict__ src = static_cast<const uint32_t*>(cs.src[g]);
+    uint32_t* __restrict__ dst = static_cast<uint32_t*>(cs.dst[g]);
+    long long done = 0;
+    if (((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15) == 0) {
+        const long long quads = words >> 2;
+        const uint4* __restrict__ s4 = reinterpret_cast<const uint4*>(src);
+        uint4* __restrict__ d4 = reinterpret_cast<uint4*>(dst);
+        for (long long q = t0; q < quads; q += step) d4[q] = s4[q];
+        done = quads << 2;
+    }
+    for (long long w = done + t0; w < words; w += step) dst[w] = src[w];
+}
+
+// Collects copies and issues them as one copy_set_kernel launch per kCopySegs copies (the last
+// launch at run()).
+struct Copies {
+    hipStream_t s;
+    CopySet cs{};
+    int count = 0;
+    long long most = 0;
+    void add(void* dst, const void* src, std::size_t bytes) {
+        if (bytes == 0) return;
+        if (count == kCopySegs) run();
+        cs.src[count] = src;
+        cs.dst[count] = dst;
+        cs.words[count] = static_cast<long long>(bytes / 4);
+        most = std::max(most, cs.words[count]);
+        ++count;
+    }
+    void run() {
+        if (count == 0) return;
+        const unsigned gx = std::min<unsigned>(std::max(1u, grid_for((most + 3) / 4, kBS)), 1024u);
+        const int segs = count;
+        count = 0;
+        most = 0;
+        copy_set_kernel<<<dim3(gx, segs), kBS, 0, s>>>(cs);
+        LHIP_LAUNCH_CHECK();
+    }
+};
+
+// The six FP64 state arrays (x, y, z, vx, vy, vz) as one argument.
+struct State6 {
+    const double* a[6];
+};
+
 // Moves the state into a recorded order: new position k holds the body with id rec[k], currently
-// at inv[rec[k]].
+// at inv[rec[k]]; dst holds the six arrays at stride n.
 __global__ void __launch_bounds__(kBS) reorder_kernel(const uint32_t* __restrict__ rec, const uint32_t* __restrict__ inv,
-                                                      const double* __restrict__ src, double* __restrict__ dst,
-                                                      int arrays, int n) {
+                                                      State6 src, double* __restrict__ dst, int n) {
     const int k = blockIdx.x * kBS + threadIdx.x;
     if (k >= n) return;
     const uint32_t j = inv[rec[k]];
-    for (int c = 0; c < arrays; ++c) dst[std::size_t(c) * n + k] = src[std::size_t(c) * n + j];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) dst[std::size_t(c) * n + k] = src.a[c][j];
 }
 
 __global__ void __launch_bounds__(kBS) reorder_small_kernel(const uint32_t* __restrict__ rec,
@@ -153,21 +211,23 @@ struct Run {
     void save_checkpoint(int k) {
         const std::size_t db = std::size_t(n) * 8, fb = std::size_t(n) * 4;
         double* base = D(S.ck_d) + std::size_t(k) * 9 * n;
-        for (int a = 0; a < 9; ++a)
-            LHIP_CHECK(hipMemcpyAsync(base + std::size_t(a) * n, state(a)->as<void>(), db, hipMemcpyDeviceToDevice, s));
-        LHIP_CHECK(hipMemcpyAsync(S.ck_id.as<uint32_t>() + std::size_t(k) * n, c.id.as<void>(), fb,
-                                  hipMemcpyDeviceToDevice, s));
-        LHIP_CHECK(hipMemcpyAsync(S.ck_m.as<float>() + std::size_t(k) * n, c.m.as<void>(), fb, hipMemcpyDeviceToDevice, s));
+        Copies cp{s};
+        for (int a = 0; a < 9; ++a) cp.add(base + std::size_t(a) * n, state(a)->as<void>(), db);
+        cp.add(S.ck_id.as<uint32_t>() + std::size_t(k) * n, c.id.as<void>(), fb);
+        cp.add(S.ck_m.as<float>() + std::size_t(k) * n, c.m.as<void>(), fb);
+        cp.run();
     }
 
     void load_checkpoint(int k) {
         const std::size_t db = std::size_t(n) * 8, fb = std::size_t(n) * 4;
         const double* base = D(S.ck_d) + std::size_t(k) * 9 * n;
-        for (int a = 0; a < 9; ++a)
-            LHIP_CHECK(hipMemcpyAsync(state(a)->as<void>(), base + std::size_t(a) * n, db, hipMemcpyDeviceToDevice, s));
-        LHIP_CHECK(hipMemcpyAsync(c.id.as<void>(), S.ck_id.as<uint32_t>() + std::size_t(k) * n, fb,
-                                  hipMemcpyDeviceToDevice, s));
-        LHIP_CHECK(hipMemcpyAsync(c.m.as<void>(), S.ck_m.as<float>() + std::size_t(k) * n, fb, hipMemcpyDeviceToDevice, s));
+        {
+            Copies cp{s};
+            for (int a = 0; a < 9; ++a) cp.add(state(a)->as<void>(), base + std::size_t(a) * n, db);
+            cp.add(c.id.as<void>(), S.ck_id.as<uint32_t>() + std::size_t(k) * n, fb);
+            cp.add(c.m.as<void>(), S.ck_m.as<float>() + std::size_t(k) * n, fb);
+            cp.run();
+        }
         c.tree_ok = false;
         c.acc_ok = true;
     }
@@ -176,19 +236,23 @@ struct Run {
     void save_positions(DevBuf& dd, DevBuf& did, DevBuf& dm, int k) {
         const std::size_t db = std::size_t(n) * 8, fb = std::size_t(n) * 4;
         double* base = D(dd) + std::size_t(k) * 3 * n;
-        for (int a = 0; a < 3; ++a)
-            LHIP_CHECK(hipMemcpyAsync(base + std::size_t(a) * n, state(a)->as<void>(), db, hipMemcpyDeviceToDevice, s));
-        LHIP_CHECK(hipMemcpyAsync(did.as<uint32_t>() + std::size_t(k) * n, c.id.as<void>(), fb, hipMemcpyDeviceToDevice, s));
-        LHIP_CHECK(hipMemcpyAsync(dm.as<float>() + std::size_t(k) * n, c.m.as<void>(), fb, hipMemcpyDeviceToDevice, s));
+        Copies cp{s};
+        for (int a = 0; a < 3; ++a) cp.add(base + std::size_t(a) * n, state(a)->as<void>(), db);
+        cp.add(did.as<uint32_t>() + std::size_t(k) * n, c.id.as<void>(), fb);
+        cp.add(dm.as<float>() + std::size_t(k) * n, c.m.as<void>(), fb);
+        cp.run();
     }
 
     void load_positions(DevBuf& dd, DevBuf& did, DevBuf& dm, int k) {
         const std::size_t db = std::size_t(n) * 8, fb = std::size_t(n) * 4;
         const double* base = D(dd) + std::size_t(k) * 3 * n;
-        for (int a = 0; a < 3; ++a)
-            LHIP_CHECK(hipMemcpyAsync(state(a)->as<void>(), base + std::size_t(a) * n, db, hipMemcpyDeviceToDevice, s));
-        LHIP_CHECK(hipMemcpyAsync(c.id.as<void>(), did.as<uint32_t>() + std::size_t(k) * n, fb, hipMemcpyDeviceToDevice, s));
-        LHIP_CHECK(hipMemcpyAsync(c.m.as<void>(), dm.as<float>() + std::size_t(k) * n, fb, hipMemcpyDeviceToDevice, s));
+        {
+            Copies cp{s};
+            for (int a = 0; a < 3; ++a) cp.add(state(a)->as<void>(), base + std::size_t(a) * n, db);
+            cp.add(c.id.as<void>(), did.as<uint32_t>() + std::size_t(k) * n, fb);
+            cp.add(c.m.as<void>(), dm.as<float>() + std::size_t(k) * n, fb);
+            cp.run();
+        }
         c.tree_ok = false;
     }
 
@@ -199,21 +263,22 @@ struct Run {
         const double* rx = D(S.rec_x) + std::size_t(r) * 3 * n;
         inverse_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(c.id.as<uint32_t>(), S.inv.as<uint32_t>(), n);
         LHIP_LAUNCH_CHECK();
-        // 6 state arrays (x, v) through tmp; masses and ids through tmp_m / tmp_id
+        // 6 state arrays (x, v) into tmp in the recorded order, masses into tmp_m (ids from the
+        // record), then back
         double* t = D(S.tmp);
-        for (int a = 0; a < 6; ++a)
-            LHIP_CHECK(hipMemcpyAsync(t + std::size_t(a) * n, state(a)->as<void>(), std::size_t(n) * 8,
-                                      hipMemcpyDeviceToDevice, s));
-        double* t2 = t + 6 * std::size_t(n);
-        reorder_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(rid, S.inv.as<uint32_t>(), t, t2, 6, n);
+        State6 st{};
+        for (int a = 0; a < 6; ++a) st.a[a] = D(*state(a));
+        reorder_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(rid, S.inv.as<uint32_t>(), st, t, n);
         LHIP_LAUNCH_CHECK();
-        for (int a = 0; a < 6; ++a)
-            LHIP_CHECK(hipMemcpyAsync(state(a)->as<void>(), t2 + std::size_t(a) * n, std::size_t(n) * 8,
-                                      hipMemcpyDeviceToDevice, s));
         reorder_small_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(rid, S.inv.as<uint32_t>(), c.m.as<float>(),
                                                               S.tmp_m.as<float>(), c.id.as<uint32_t>(), n);
         LHIP_LAUNCH_CHECK();
-        LHIP_CHECK(hipMemcpyAsync(c.m.as<void>(), S.tmp_m.as<void>(), std::size_t(n) * 4, hipMemcpyDeviceToDevice, s));
+        {
+            Copies cp{s};
+            for (int a = 0; a < 6; ++a) cp.add(state(a)->as<void>(), t + std::size_t(a) * n, std::size_t(n) * 8);
+            cp.add(c.m.as<void>(), S.tmp_m.as<void>(), std::size_t(n) * 4);
+            cp.run();
+        }
         // tree from the recorded positions (already in key order)
         c.reserve_tree(n);
         c.momd.reserve(std::size_t(c.cap_nodes) * sizeof(NodeMomD));
@@ -254,10 +319,10 @@ struct Run {
     void record(int r) {
         const std::size_t db = std::size_t(n) * 8, fb = std::size_t(n) * 4;
         double* base = D(S.rec_x) + std::size_t(r) * 3 * n;
-        for (int a = 0; a < 3; ++a)
-            LHIP_CHECK(hipMemcpyAsync(base + std::size_t(a) * n, state(a)->as<void>(), db, hipMemcpyDeviceToDevice, s));
-        LHIP_CHECK(hipMemcpyAsync(S.rec_id.as<uint32_t>() + std::size_t(r) * n, c.id.as<void>(), fb,
-                                  hipMemcpyDeviceToDevice, s));
+        Copies cp{s};
+        for (int a = 0; a < 3; ++a) cp.add(base + std::size_t(a) * n, state(a)->as<void>(), db);
+        cp.add(S.rec_id.as<uint32_t>() + std::size_t(r) * n, c.id.as<void>(), fb);
+        cp.run();
     }
 
     // One force evaluation of the forward: r = index of the force call.
@@ -328,7 +393,7 @@ void treesim_forward(Context& c, const lhip_sim_params& p, int n, const double* 
         S.rec_id.reserve(std::size_t(T + 1) * N * 4);
     }
     S.inv.reserve(N * 4);
-    S.tmp.reserve(12 * N * 8);
+    S.tmp.reserve(6 * N * 8);
     S.tmp_m.reserve(N * 4);
     // initial state: FP64 upload with ids 0..n-1 and float masses
     std::vector<double> sx, sv;
@@ -444,13 +509,14 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
                                              c.momd.as<NodeMomD>(), S.tr_cap, slot_node(j), slot_mom(j),
                                              slot_momd(j), S.tr_flag.as<uint32_t>() + j);
         LHIP_LAUNCH_CHECK();
-        LHIP_CHECK(hipMemcpyAsync(S.tr_lev.as<uint32_t>() + std::size_t(j) * kLev, c.levstart.as<void>(), kLev * 4,
-                                  hipMemcpyDeviceToDevice, s));
-        LHIP_CHECK(hipMemcpyAsync(S.tr_cnt.as<uint32_t>() + j, c.cnt.as<uint32_t>() + n, 4, hipMemcpyDeviceToDevice, s));
-        LHIP_CHECK(hipMemcpyAsync(S.tr_starts.as<uint32_t>() + std::size_t(j) * (N + 1), c.walk_starts(), (N + 1) * 4,
-                                  hipMemcpyDeviceToDevice, s));
-        LHIP_CHECK(hipMemcpyAsync(S.tr_count.as<uint32_t>() + std::size_t(j) * (N + 1), c.walk_count(), (N + 1) * 4,
-                                  hipMemcpyDeviceToDevice, s));
+        {
+            Copies cp{s};
+            cp.add(S.tr_lev.as<uint32_t>() + std::size_t(j) * kLev, c.levstart.as<void>(), kLev * 4);
+            cp.add(S.tr_cnt.as<uint32_t>() + j, c.cnt.as<uint32_t>() + n, 4);
+            cp.add(S.tr_starts.as<uint32_t>() + std::size_t(j) * (N + 1), c.walk_starts(), (N + 1) * 4);
+            cp.add(S.tr_count.as<uint32_t>() + std::size_t(j) * (N + 1), c.walk_count(), (N + 1) * 4);
+            cp.run();
+        }
         unit_order(s, c.build_bufs(), c.walk_starts(), c.walk_count(), vjp_order_threshold(),
                    S.tr_ranges.as<uint32_t>() + std::size_t(j) * kWalkRanges,
                    S.tr_order.as<uint32_t>() + std::size_t(j) * (N + 1));
