@@ -89,6 +89,41 @@ def test_direct_vs_double_brute_force_4096(hip):
     assert t1["interactions"] == 4096 * 4095
 
 
+@pytest.mark.parametrize("mode", ["plain", "mixed"])
+def test_empty_body_set(hip, mode):
+    # The reference CUDA backend's n == 0 paths (src/cuda_backend.cu:1301-1400, 1465, 1559-1624): an
+    # upload of no bodies after a non-empty state leaves a backend whose every call succeeds and
+    # reports nothing: zero times and counts, a zero Diag (has_pe as asked), zero bounds, no costs,
+    # zero exact potential energy, an empty download and an empty LET.
+    be = L.HipBackend()
+    try:
+        be.set_precision(mode)
+        be.upload(cube(1000, 5))
+        be.compute_forces(P("tree", 0.01), True)
+        assert be.times()["interactions"] > 0
+        be.upload(O.Bodies.empty(0))
+        assert be.size() == 0
+        for method in ("direct", "tree"):
+            for pot in (False, True):
+                be.compute_forces(P(method, 0.01), pot)
+                t = be.times()
+                assert t["interactions"] == 0 and t["walk_ms"] == 0 and t["force_ms"] == 0
+                d = be.diagnostics()
+                assert d["has_pe"] == pot
+                assert all(d[k] == 0 for k in ("mass", "ke", "pe", "px", "py", "pz", "lx", "ly", "lz"))
+        be.build_tree(P("tree", 0.01))
+        be.compute_forces_with(P("tree", 0.01), False)
+        assert be.body_costs() is None
+        assert be.exact_potential_energy(1.0, 0.01) == 0.0
+        assert be.local_bounds() == ([0.0, 0.0, 0.0], [0.0, 0.0, 0.0])
+        be.kick(0.1)
+        be.drift(0.1)
+        assert len(be.download()) == 0
+        assert len(be.export_let(([0, 0, 0], [1, 1, 1]), P("tree", 0.01)).nodes) == 0
+    finally:
+        be.close()
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 31, 32, 33, 64, 255, 257, 1000, 1023])
 def test_small_odd_n_direct_and_tree_theta0(hip, n):
     b = cube(n, 100 + n)
