@@ -166,7 +166,7 @@ struct TreeSim {
     // records per force call: sorted ids and sorted positions
     DevBuf rec_id, rec_x;
     // cotangents (id order) and VJP scratch (sorted order)
-    DevBuf lx, lvx, u, gt, gs, gcell, gmono, gpriv, cum, inv, tmp, tmp_m, tmp_id, posd_rec;
+    DevBuf lx, lvx, u, gt, gs, gcell, gmono, gpriv, cum, inv, tmp, tmp_m, posd_rec;
     // Trees of a segment's recomputed steps (slot k = step start + k), reused by their VJPs instead
     // of a second build: node records (at most tr_cap per slot), level starts, node count, walk
     // units and their count; tr_flag[k] = 1 where a tree did not fit.
