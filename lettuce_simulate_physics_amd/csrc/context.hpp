@@ -57,8 +57,6 @@ struct Context {
     // tree
     DevBuf bl, cnt, gflag, scan_tmp, pkey, pkey2, pval, pval2, inv, levstart, node, mom, prefix, gstart, counters;
     DevBuf worder;  // the walk's unit order, written by every build
-    DevBuf pflag, pstart;  // walk packs of sibling groups (pack_max > 0)
-    int pack_max = 0;
     long long cap_nodes = 0;
     // Key restarts: key frames of the current tree (1: none), frame keys, depth caps, restarted
     // cells and their cubes; the pinned flag the level pass sets; restart scratch.
@@ -354,8 +352,6 @@ struct Context {
         gflag.reserve((N + 1) * 4);
         gstart.reserve((N + 1) * 4);
         worder.reserve((N + 1) * 4);
-        pflag.reserve((N + 1) * 4);
-        pstart.reserve((N + 1) * 4);
         scan_tmp.reserve(2 * scan_tmp_entries(nn) * 4);  // two arrays scanned at once
         const std::size_t C = std::size_t(cap_nodes);
         pkey.reserve(C * 8); pkey2.reserve(C * 8);
@@ -701,9 +697,6 @@ struct Context {
         bb.wcost = cost.as<float>();
         bb.worder = worder.as<uint32_t>();
         bb.order_thr = walk_order_threshold();
-        bb.pmax = pack_max;
-        bb.pflag = pflag.as<uint32_t>();
-        bb.pstart = pstart.as<uint32_t>();
         bb.counters = counters.as<WalkCounters>();
         bb.frames = frames;
         if (frames > 1) {
@@ -727,9 +720,9 @@ struct Context {
         return float(std::max(1.0, kOrderShare * wave));
     }
 
-    // The walk's units: the groups, or their packs. The count is at [n].
-    const uint32_t* walk_starts() const { return pack_max > 0 ? pstart.as<uint32_t>() : gstart.as<uint32_t>(); }
-    const uint32_t* walk_count() const { return pack_max > 0 ? pflag.as<uint32_t>() : gflag.as<uint32_t>(); }
+    // The walk's units: the groups. The count is at [n].
+    const uint32_t* walk_starts() const { return gstart.as<uint32_t>(); }
+    const uint32_t* walk_count() const { return gflag.as<uint32_t>(); }
 
     // The walk's arguments for the current bodies: the local tree (tree method) and the outputs.
     WalkArgs walk_args(const lhip_force_params& p) {
@@ -1215,7 +1208,7 @@ struct Context {
              {&x, &y, &z, &vx, &vy, &vz, &m, &id, &cost, &x2, &y2, &z2, &vx2, &vy2, &vz2, &m2, &id2, &cost2, &ax, &ay,
               &az, &pot, &posm, &key, &key2, &idx, &idx2, &radix_buf, &cube, &bbox_part, &red_part, &red_out,
               &direct_part, &bl, &cnt, &gflag, &scan_tmp, &pkey, &pkey2, &pval, &pval2, &inv, &levstart, &node, &mom,
-              &prefix, &gstart, &worder, &counters, &pflag, &pstart, &fkey, &fseg, &capb, &fcube, &rs_seg, &rs_bbox,
+              &prefix, &gstart, &worder, &counters, &fkey, &fseg, &capb, &fcube, &rs_seg, &rs_bbox,
               &rs_cstart, &rs_gkey, &rs_ka, &rs_kb, &rs_va, &rs_vb, &xd, &yd, &zd, &vxd, &vyd, &vzd, &xd2, &yd2, &zd2,
               &vxd2, &vyd2, &vzd2, &axd, &ayd, &azd, &potd, &momd, &rec64, &posd, &chk_tgt, &chk_out, &chk_part,
               &let_rec, &let_body, &let_root, &let_state, &let_ex, &let_bcnt, &let_tmp, &let_out, &let_bods, &lpool, &lnext,
@@ -1262,7 +1255,7 @@ inline void memory_need(uint64_t n, int method, bool mixed, uint64_t* dev, uint6
         buf(N * 8); buf(N * 8); buf(N * 4); buf(N * 4);  // keys, sort index (double-buffered)
         buf(radix_work_words(int(std::max<uint64_t>(N, C))) * 4);        // radix sort work
         buf(N * 4);                                                        // levels word
-        for (int k = 0; k < 6; ++k) buf((N + 1) * 4);                      // counts, groups, packs, order
+        for (int k = 0; k < 4; ++k) buf((N + 1) * 4);                      // counts, groups (flags, starts), order
         buf(2 * scan_tmp_entries(ni) * 4);
         buf(C * 8); buf(C * 8); buf(C * 4); buf(C * 4); buf(C * 4);        // level sort, inverse
         buf((kDepthLevels + 2) * 4);

@@ -726,38 +726,6 @@ __global__ void __launch_bounds__(kBS) moments_level(int part, int level, MomArg
 }
 
 
-// Walk packs: runs of consecutive sibling groups (same group level G, same level-(G-1) parent: the
-// first body of the later group shares exactly G - 1 octal digits with the body before it), each
-// run cut greedily into packs of at most pmax bodies. A pack's box lies inside the parent cell.
-// pflag[g] = 1 iff group g starts a pack (g < group count); 0 for g in [group count, n). The first
-// group of each run writes the run's flags.
-__global__ void __launch_bounds__(kBS) packs_kernel(const uint32_t* __restrict__ gstart,
-                                                    const uint32_t* __restrict__ gscan, const uint32_t* __restrict__ bl,
-                                                    int n, int pmax, uint32_t* __restrict__ pflag) {
-    const int g = blockIdx.x * kBS + threadIdx.x;
-    if (g >= n) return;
-    const int ng = int(gscan[n]);
-    if (g >= ng) {
-        pflag[g] = 0u;
-        return;
-    }
-    auto level = [&](int h) { return int((bl[gstart[h]] >> 8) & 0xff); };
-    auto continues = [&](int h) {  // group h is a sibling of group h - 1
-        if (h == 0) return false;
-        const uint32_t b = bl[gstart[h]];
-        const int G = int((b >> 8) & 0xff), c = int((b >> 16) & 0xff) - 1;
-        return G > 0 && c == G - 1 && level(h - 1) == G;
-    };
-    if (continues(g)) return;
-    int acc = 0;
-    for (int h = g; h < ng && (h == g || continues(h)); ++h) {
-        const int size = int(gstart[h + 1] - gstart[h]);
-        const bool start = h == g || acc + size > pmax;
-        pflag[h] = start ? 1u : 0u;
-        acc = start ? size : acc + size;
-    }
-}
-
 // Keys that take the costly walk units first. A unit's bodies share one interaction list, so its
 // work is estimated as (list length + 1) x (bodies + 8), the list length taken from the previous
 // walk's cost of its first body (costs travel with the bodies through the sort; before any walk
@@ -787,17 +755,6 @@ __global__ void __launch_bounds__(kBS) unit_order_keys(const uint32_t* __restric
         key[g] = uint64_t(255 - bucket);
     }
     val[g] = uint32_t(g);
-}
-
-// Pack starts in body order from the scanned flags; pstart[count] = n.
-__global__ void __launch_bounds__(kBS) pstarts_kernel(const uint32_t* __restrict__ gstart,
-                                                      const uint32_t* __restrict__ gscan,
-                                                      const uint32_t* __restrict__ pscan, int n,
-                                                      uint32_t* __restrict__ pstart) {
-    const int g = blockIdx.x * kBS + threadIdx.x;
-    if (g >= n) return;
-    if (g < int(gscan[n]) && pscan[g + 1] != pscan[g]) pstart[pscan[g]] = gstart[g];
-    if (g == n - 1) pstart[pscan[n]] = uint32_t(n);
 }
 
 // ---------------------------------------------------------------- key restarts (the rare path)
@@ -1123,16 +1080,9 @@ int tree_build(hipStream_t s, const BuildBufs& b, float inv_theta, hipEvent_t le
     LHIP_LAUNCH_CHECK();
     groups_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(b.gflag, n, b.bl, b.gstart);
     LHIP_LAUNCH_CHECK();
-    if (b.pmax > 0) {
-        packs_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(b.gstart, b.gflag, b.bl, n, b.pmax, b.pflag);
-        LHIP_LAUNCH_CHECK();
-        scan_exclusive(s, b.pflag, n, b.scan_tmp);
-        pstarts_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(b.gstart, b.gflag, b.pflag, n, b.pstart);
-        LHIP_LAUNCH_CHECK();
-    }
     if (b.worder && b.wcost) {
         // The walk order of this tree's units (queued before the host waits on the level pass).
-        unit_order(s, b, b.pmax > 0 ? b.pstart : b.gstart, b.pmax > 0 ? b.pflag : b.gflag, b.order_thr,
+        unit_order(s, b, b.gstart, b.gflag, b.order_thr,
                    range_starts_of(b.counters), b.worder);
     }
     int depth = tree_depth(ext ? b.frames : 1);

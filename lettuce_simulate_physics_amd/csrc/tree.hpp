@@ -104,10 +104,6 @@ struct BuildBufs {
     const float* wcost;
     uint32_t* worder;          // n + 1
     float order_thr;           // units estimated below this keep their natural (Morton) order
-    // walk packs (pmax > 0): sibling groups packed up to pmax bodies
-    int pmax;
-    uint32_t* pflag;           // n + 1: pack starts per group, then their exclusive scan (count at [n])
-    uint32_t* pstart;          // n + 1
     WalkCounters* counters;
     // Key restarts (frames > 1). Frame f >= 1 keys of body i at fkey[(f - 1) n + i] (0 outside
     // restarted cells); cap[i] = 21 x the frames body i has (its deepest level); fseg[(f - 1) n + i]
