@@ -27,16 +27,33 @@
 namespace lhip {
 namespace {
 
-// u (SoA, tree order) = c lambda (x, y, z per body, id order) of the tree's bodies.
+// u (SoA, tree order) = c lambda (x, y, z per body, id order) of the tree's bodies; the source
+// sides (SoA) start at zero.
 __global__ void __launch_bounds__(kBS) gather_u_kernel(const uint32_t* __restrict__ id, const double* __restrict__ lam,
                                                        double c, double* __restrict__ ux, double* __restrict__ uy,
-                                                       double* __restrict__ uz, int n) {
+                                                       double* __restrict__ uz, double* __restrict__ gsx,
+                                                       double* __restrict__ gsy, double* __restrict__ gsz, int n) {
     const int k = blockIdx.x * kBS + threadIdx.x;
     if (k >= n) return;
     const double* l = lam + 3 * std::size_t(id[k]);
     ux[k] = c * l[0];
     uy[k] = c * l[1];
     uz[k] = c * l[2];
+    gsx[k] = 0.0;
+    gsy[k] = 0.0;
+    gsz[k] = 0.0;
+}
+
+// Zeroes up to three double arrays in one launch (blockIdx.y picks the array).
+struct ZeroSet {
+    double* p[3];
+    long long n[3];
+};
+
+__global__ void __launch_bounds__(kBS) zero_set_kernel(ZeroSet z) {
+    double* __restrict__ p = z.p[blockIdx.y];
+    const long long n = z.n[blockIdx.y];
+    for (long long i = (long long)blockIdx.x * kBS + threadIdx.x; i < n; i += (long long)gridDim.x * kBS) p[i] = 0.0;
 }
 
 __global__ void __launch_bounds__(kBS) inverse_kernel(const uint32_t* __restrict__ id, uint32_t* __restrict__ inv, int n) {
@@ -543,15 +560,19 @@ void treesim_backward(Context& c, const double* gxT, const double* gvT, double* 
         S.gmono.reserve(std::size_t(nodes) * 3 * 8);
         S.cum.reserve(std::size_t(nodes) * 9 * 8);
         double* u = R.D(S.u);
-        gather_u_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(c.id.as<uint32_t>(), lv, S.p.G * w, u, u + N, u + 2 * N, n);
-        LHIP_LAUNCH_CHECK();
         double* gs = R.D(S.gs);
         double* gt = R.D(S.gt);
-        launch_fill64(s, gs, 0.0, 3 * n);
-        launch_fill64(s, R.D(S.gcell), 0.0, nodes * 9);
-        launch_fill64(s, R.D(S.gmono), 0.0, nodes * 3);
+        gather_u_kernel<<<grid_for(n, kBS), kBS, 0, s>>>(c.id.as<uint32_t>(), lv, S.p.G * w, u, u + N, u + 2 * N, gs,
+                                                         gs + N, gs + 2 * N, n);
+        LHIP_LAUNCH_CHECK();
         S.gpriv.reserve(std::size_t(blocks) * kHot * 12 * 8);
-        launch_fill64(s, R.D(S.gpriv), 0.0, blocks * kHot * 12);
+        {
+            const ZeroSet z{{R.D(S.gcell), R.D(S.gmono), R.D(S.gpriv)},
+                            {9ll * nodes, 3ll * nodes, (long long)blocks * kHot * 12}};
+            const long long most = std::max(z.n[0], std::max(z.n[1], z.n[2]));
+            zero_set_kernel<<<dim3(std::min<unsigned>(std::max(1u, grid_for(most, kBS)), 1024u), 3), kBS, 0, s>>>(z);
+            LHIP_LAUNCH_CHECK();
+        }
         TreeVjpArgs a{};
         a.node = stored ? slot_node(j) : c.node.as<WalkNode>();
         a.mom = stored ? slot_mom(j) : c.mom.as<NodeMom>();
